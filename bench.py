@@ -1,0 +1,143 @@
+"""Throughput of the online-reconstruction chain on one MI355X.
+
+One step = one pass of the whole chain (reduction, XZ, YZ, back-partial
+cleaning, station-1 extension, output records) over one batch of synthetic
+events already resident in HBM.  Default workload: 1,000,000 events, hits per
+event ~ Poisson(300) capped at 1000, 1-2 muon tracks, basic_geometry_e906.
+
+Prints one JSON line with the throughput, the HBM roofline of the chain
+kernel (algorithmic bytes per event: 12 per input hit, 1 bit per hit of keep
+mask, 96 per output track) and the CPU oracle timed on a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "e1039-gpu_amd"))
+
+METRIC = "events/s/GPU, learned-pruned vs plain GPU chain, at ≥99.5% plain-track recall"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--events", type=int, default=1_000_000)
+    ap.add_argument("--mean-hits", type=float, default=300.0)
+    ap.add_argument("--cap-hits", type=int, default=1000)
+    ap.add_argument("--seed", type=int, default=0xE1039)
+    ap.add_argument("--cpu-sample", type=int, default=100_000, help="events in the CPU baseline sample (0 = skip)")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); events are independent, so N GPUs would "
+              "each run this same per-GPU workload on their own event range", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import orchain as oc
+
+    if not torch.cuda.is_available():
+        log("no GPU visible")
+        sys.exit(1)
+    geo = oc.Geometry.load()
+    t0 = time.time()
+    ev = oc.generate(geo, args.events, seed=args.seed, mult_mode=1, mult_mean=args.mean_hits, mult_cap=args.cap_hits)
+    log(f"generated {ev.n_events} events, {ev.n_hits} hits ({ev.n_hits / max(1, ev.n_events):.1f}/event) "
+        f"in {time.time() - t0:.1f}s")
+    params = oc.default_params(max_hits_per_event=1024)
+    sess = oc.Session(geo, params, device=0)
+    dev_in = sess.upload(ev)
+    dev_out = sess.alloc_outputs(ev.n_events, ev.n_hits, tracklets=True)
+    torch.cuda.synchronize()
+
+    for i in range(args.warmup):
+        sess.run_device(dev_in, dev_out)
+        torch.cuda.synchronize()
+        log(f"warmup step {i} done: kernel {sess.last_kernel_ms():.1f} ms")
+    kms = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        sess.run_device(dev_in, dev_out)
+        kms.append(sess.last_kernel_ms())
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / max(1, args.steps)
+    kernel_ms = float(np.mean(kms)) if kms else float("nan")
+    log(f"timed {args.steps} steps: {ms_per_step:.2f} ms/step, chain kernel {kernel_ms:.2f} ms")
+
+    res = oc.Session.fetch(dev_out, ev.n_events, params.max_tracks_out, params.max_tracklets_per_event)
+    n_out = int(res.n_tracks.sum())
+    rejected = int(((res.status & oc.EV_REJECT_MASK) != 0).sum())
+    counts = np.diff(ev.offsets)
+    b_alg = float(12 * counts.sum() + np.ceil(counts / 8.0).sum() + 96 * n_out)
+    achieved = b_alg / (kernel_ms * 1e-3) / 1e9
+    log(f"tracks out {n_out}, tracklets {int(res.n_tracklets.sum())}, rejected events {rejected}; "
+        f"algorithmic bytes {b_alg / 1e9:.3f} GB per launch")
+
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "chain_kernel_1M_poisson300_traffic.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+
+    cpu = None
+    if args.cpu_sample > 0:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as orc
+
+        ns = min(args.cpu_sample, ev.n_events)
+        sub = ev.subset(0, ns)
+        og = orc.OracleGeometry(oc.DEFAULT_GEOMETRY)
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        orc.reconstruct(og, params, sub.subset(0, min(ns, 2000)), threads=threads)  # warm-up
+        t0 = time.perf_counter()
+        orc.reconstruct(og, params, sub, threads=threads)
+        dt = time.perf_counter() - t0
+        cpu = {"value": ns / dt, "unit": "events/s", "cores": threads, "kind": "port",
+               "sample": f"first {ns} events of the same workload, oracle (C++ restatement, -O2, OpenMP over events) "
+                         f"in {dt:.1f}s"}
+        log(f"cpu baseline: {ns / dt:.0f} events/s on {threads} threads")
+
+    line = {
+        "metric": METRIC,
+        "value": ev.n_events / (ms_per_step * 1e-3),
+        "unit": "events/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"plain chain (no learned pruning), {ev.n_events} events, hits/event Poisson({args.mean_hits:g}) "
+                        f"capped at {args.cap_hits}, 1-2 muon tracks, basic_geometry_e906, reference caps",
+            "chain": "plain",
+            "events": ev.n_events,
+            "hits": int(ev.n_hits),
+            "kernel_ms": kernel_ms,
+        },
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

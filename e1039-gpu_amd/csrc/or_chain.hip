@@ -1,0 +1,1401 @@
+// The reconstruction chain on CDNA4: one 64-lane wave per event, persistent
+// grid, per-event hits staged in LDS, per-wave tracklet scratch in HBM.
+//
+// Stages per event (reference kernels in brackets):
+//   A  load + stable grouping by detector           [host staging, online_reconstruction.cu:506-640]
+//   B  per-plane reduction, one lane per plane      [gkernel_eR, reconstruction_kernels.cuh:8-172]
+//   C  binned X pair lists (st2, D3p, D3m)          [make_hitpairs_in_station_bins, reconstruction_helper.cuh:368-460]
+//   D  XZ combinations, one lane per combination    [gKernel_XZ_tracking, reconstruction_kernels.cuh:182-855]
+//   E  YZ: one lane per (tracklet, u3.v3 pair)      [gKernel_YZ_tracking, reconstruction_kernels.cuh:864-1832]
+//   F  back-partial cleaning                        [reconstruction_helper.cuh:1789-1909]
+//   G  station-1 extension, one lane per track      [gKernel_Global_tracking, reconstruction_kernels.cuh:1841-2534]
+//   H  outputs: status, keep bits, track records
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <new>
+
+#include "or_device.h"
+
+namespace ordev {
+__constant__ float c_geo[F_COUNT][ND];
+}
+
+using namespace ordev;
+
+namespace {
+
+constexpr int WAVE = 64;
+constexpr int MAXTKL = 512;   // tracklets per event (datasizes::TrackSizeMax)
+constexpr int BIN_CAP = 20;   // pairs per XZ bin (globalconsts::MaxHitsProj_X)
+constexpr int WIN_CAP = 100;  // pairs per window list (hitpairs_*[100])
+constexpr int PLANE_CAP = 400;
+constexpr int YZ_CHUNK = 16;  // tracklets whose window lists are built together
+constexpr uint16_t NOHIT = 0xFFFF;
+
+// Tracklet as kept in scratch between stages.
+struct __align__(16) Tk {
+    float x0, tx, ex0, etx;
+    float y0, ty, ey0, ety;
+    float chi2, invp, einvp, pad;
+    float xpos[4];          // x at y = 0 of the XZ hits (record 'pos' of those hits)
+    float resid[18];
+    uint16_t hraw[18];      // LDS raw index of each hit
+    uint8_t det[18];
+    int8_t sign[18];
+    uint32_t canon;         // position in the canonical XZ list
+    int8_t stid;            // 0 after XZ, 3 rejected/demoted, 5 YZ, 6 global
+    uint8_t nx, nuv, n1;
+    uint8_t t, st3, slot, pos_in_slot;
+    int8_t charge;
+    uint8_t pad2[3];
+};
+
+template <int MAXH>
+struct Lds {
+    int16_t elem[MAXH];
+    float pos[MAXH];
+    float tdc[MAXH];
+    float drift[MAXH];
+    uint16_t orig[MAXH];
+    uint16_t red[MAXH];       // reduced hits: raw indices, planes contiguous
+    int8_t keep[MAXH];
+    uint8_t intime[MAXH];
+    uint8_t korig[MAXH];      // kept flag by input order
+    int pcnt[56], pstart[56], cursor[56], rcnt[56], rstart[56];
+    uint16_t xpairs[3][28][BIN_CAP];   // (a | b << 8), 0xFF = none
+    int xcnt[3][28];
+    uint16_t slot_order[MAXTKL];
+    int nl[YZ_CHUNK][4];
+    int uoff[YZ_CHUNK + 1];
+    unsigned long long best[YZ_CHUNK];
+    int sstart[33];
+    uint32_t status;
+};
+
+struct Args {
+    const int16_t* det;
+    const int16_t* elem;
+    const float* tdc;
+    const float* drift;
+    const uint8_t* flag;
+    const int64_t* off;
+    int64_t n_events;
+    const float* wire;          // 55 x OR_MAX_ELEM
+    or_params p;
+    uint32_t* status;
+    uint32_t* keep_mask;
+    int32_t* n_tracks;
+    or_track* tracks;
+    int32_t* n_tracklets;
+    float* records;
+    char* scratch;
+    int64_t scratch_stride;
+};
+
+// scratch layout per block
+constexpr int64_t SCR_TK = 0;
+constexpr int64_t SCR_CELL = SCR_TK + (int64_t)sizeof(Tk) * MAXTKL;                 // uint32[1569]
+constexpr int64_t SCR_LIST = SCR_CELL + ((1569 * 4 + 255) / 256) * 256;             // uint16 lists
+constexpr int64_t SCR_LIST_BYTES = (int64_t)WAVE * 4 * WIN_CAP * 2;
+constexpr int64_t SCR_STRIDE = ((SCR_LIST + SCR_LIST_BYTES + 4095) / 4096) * 4096;
+
+__device__ __forceinline__ unsigned long long ballot(bool p) { return __ballot(p); }
+__device__ __forceinline__ int lane_id() { return threadIdx.x; }
+__device__ __forceinline__ int rank_in(unsigned long long m) {
+    return __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+}
+// inclusive wave prefix sum
+__device__ __forceinline__ int wave_scan(int v) {
+    const int l = lane_id();
+#pragma unroll
+    for (int o = 1; o < WAVE; o <<= 1) {
+        int t = __shfl_up(v, o, WAVE);
+        if (l >= o) v += t;
+    }
+    return v;
+}
+__device__ __forceinline__ int wave_sum(int v) {
+#pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// Hit access helpers on the staged event
+template <int MAXH>
+struct Ev {
+    Lds<MAXH>& L;
+    __device__ int nred(int d) const { return L.rcnt[d]; }
+    __device__ int raw(int d, int k) const { return L.red[L.rstart[d] + k]; }
+};
+
+// XZ bin of an element (WCHitsBins_X, reconstruction_constants.h:155-162):
+// 4-wire bins from 1, the last D3 bin spans 109-116.
+__device__ __forceinline__ int xbin(int s, int c) {
+    if (c < 1) return -1;
+    int b = (c - 1) >> 2;
+    if (b > 27) b = 27;
+    const int lo = 1 + 4 * b;
+    const int hi = (b == 27 && s > 0) ? 116 : 4 + 4 * b;
+    return (lo <= c && c <= hi) ? b : -1;
+}
+
+// hodoscope X match for an XZ candidate (reconstruction_helper.cuh:1090-1144)
+template <int MAXH>
+__device__ bool hodo_xz(const Ev<MAXH>& E, int st, int d, float x0, float tx, float ex0, float etx) {
+    const int n = E.nred(d);
+    if (n == 0) return false;
+    if (d > 40) st = 3;
+    const float xh = G(F_Z, d) * tx + x0;
+    const float f = hodo_fudge(st), cw = hodo_xcell(st);
+    float ex = 3.f * (fabsf(G(F_Z, d) * etx) + ex0);
+    for (int i = 0; i < n; ++i) {
+        const float p = E.L.pos[E.raw(d, i)];
+        float xmin = p - cw * 0.5f, xmax = p + cw * 0.5f;
+        xmin -= (xmax - xmin) * f;
+        xmax += (xmax - xmin) * f;
+        ex += (xmax - xmin) * 0.15f;
+        xmin -= ex;
+        xmax += ex;
+        if (xmin <= xh && xh <= xmax) return true;
+    }
+    return false;
+}
+
+// hodoscope match for a tracklet (reconstruction_helper.cuh:1002-1087)
+template <int MAXH>
+__device__ bool hodo_tkl(const Ev<MAXH>& E, int st, int d, float x0, float y0, float tx, float ty, float ex0, float ey0,
+                         float etx, float ety) {
+    const int n = E.nred(d);
+    if (n == 0) return false;
+    if (d > 40) st = 3;
+    const float f = hodo_fudge(st), cw = G(F_CELLWIDTH, d);
+    const float xh = G(F_Z, d) * tx + x0, yh = G(F_Z, d) * ty + y0;
+    float ex = 3.f * (fabsf(G(F_Z, d) * etx) + ex0);
+    const float ey = 3.f * (fabsf(G(F_Z, d) * ety) + ey0);
+    for (int i = 0; i < n; ++i) {
+        const float p = E.L.pos[E.raw(d, i)];
+        float xmin = p - cw * 0.5f, xmax = p + cw * 0.5f;
+        float ymin = G(F_Y1, d), ymax = G(F_Y2, d);
+        xmin -= (xmax - xmin) * f;
+        xmax += (xmax - xmin) * f;
+        ymin -= (ymax - ymin) * f;
+        ymax += (ymax - ymin) * f;
+        ex += (xmax - xmin) * 0.15f;
+        xmin -= ex;
+        xmax += ex;
+        ymin -= ey;
+        ymax += ey;
+        if (xmin <= xh && xh <= xmax && ymin <= yh && yh <= ymax) return true;
+    }
+    return false;
+}
+
+// first prop-tube hit within win of the expected coordinate, plane a then b
+template <int MAXH>
+__device__ __forceinline__ int prop_hit(const Ev<MAXH>& E, int da, int db, float slope, float icpt, float win) {
+    const float za = G(F_Z, da);
+    for (int m = 0; m < E.nred(da); ++m)
+        if (fabsf(E.L.pos[E.raw(da, m)] - (slope * za + icpt)) < win) return 1;
+    const float zb = G(F_Z, db);
+    for (int m = 0; m < E.nred(db); ++m)
+        if (fabsf(E.L.pos[E.raw(db, m)] - (slope * zb + icpt)) < win) return 1;
+    return 0;
+}
+
+// ---- window pair list (make_hitpairs_in_station, reconstruction_helper.cuh:464-549)
+// Returns the number of pairs, or -1 when a cap is exceeded.
+template <int MAXH>
+__device__ int pairs_window(const Ev<MAXH>& E, int st, int proj, float xmin, float xmax, uint16_t* out, int cap_plane,
+                            int cap_pairs) {
+    const int sid = super_id(st, proj);
+    const int d1 = sid * 2, d2 = sid * 2 - 1;
+    const int n1 = E.nred(d1), n2 = E.nred(d2);
+    const float dx1 = G(F_DELTAPX, d1), dx2 = G(F_DELTAPX, d2);
+    // in-window hits, as bit lists over the plane (n <= PLANE_CAP)
+    uint8_t w1[PLANE_CAP / 8 + 1], w2[PLANE_CAP / 8 + 1];
+    int c1 = 0, c2 = 0;
+    for (int i = 0; i < n1; ++i) {
+        const int e = E.L.elem[E.raw(d1, i)];
+        const float b = x_bep(d1, e), t = x_bep(d1, e) + dx1;
+        const float a = dx1 > 0 ? b : t, c = dx1 > 0 ? t : b;
+        const bool in = a <= xmax && c >= xmin;
+        if ((i & 7) == 0) w1[i >> 3] = 0;
+        if (in) { w1[i >> 3] |= (uint8_t)(1u << (i & 7)); ++c1; }
+    }
+    for (int i = 0; i < n2; ++i) {
+        const int e = E.L.elem[E.raw(d2, i)];
+        const float b = x_bep(d2, e), t = x_bep(d2, e) + dx2;
+        const float a = dx2 > 0 ? b : t, c = dx2 > 0 ? t : b;
+        const bool in = a <= xmax && c >= xmin;
+        if ((i & 7) == 0) w2[i >> 3] = 0;
+        if (in) { w2[i >> 3] |= (uint8_t)(1u << (i & 7)); ++c2; }
+    }
+    if (c1 > cap_plane || c2 > cap_plane) return -1;
+    const float sp = spacing_plane(sid);
+    uint8_t f1[PLANE_CAP / 8 + 1], f2[PLANE_CAP / 8 + 1];
+    for (int i = 0; i < (n1 + 7) / 8; ++i) f1[i] = 0;
+    for (int i = 0; i < (n2 + 7) / 8; ++i) f2[i] = 0;
+    int np = 0;
+    for (int i = 0; i < n1; ++i) {
+        if (!((w1[i >> 3] >> (i & 7)) & 1)) continue;
+        const float pi = E.L.pos[E.raw(d1, i)];
+        for (int j = 0; j < n2; ++j) {
+            if (!((w2[j >> 3] >> (j & 7)) & 1)) continue;
+            if (fabsf(pi - E.L.pos[E.raw(d2, j)]) > sp) continue;
+            if (np >= cap_pairs) return -1;
+            out[np++] = (uint16_t)(i | (j << 8));
+            f1[i >> 3] |= (uint8_t)(1u << (i & 7));
+            f2[j >> 3] |= (uint8_t)(1u << (j & 7));
+        }
+    }
+    for (int i = 0; i < n1; ++i)
+        if (((w1[i >> 3] >> (i & 7)) & 1) && !((f1[i >> 3] >> (i & 7)) & 1)) {
+            if (np >= cap_pairs) return -1;
+            out[np++] = (uint16_t)(i | (0xFF << 8));
+        }
+    for (int j = 0; j < n2; ++j)
+        if (((w2[j >> 3] >> (j & 7)) & 1) && !((f2[j >> 3] >> (j & 7)) & 1)) {
+            if (np >= cap_pairs) return -1;
+            out[np++] = (uint16_t)(0xFF | (j << 8));
+        }
+    return np;
+}
+
+__device__ __forceinline__ int pa(uint16_t q) { int a = q & 0xFF; return a == 0xFF ? -1 : a; }
+__device__ __forceinline__ int pb(uint16_t q) { int b = q >> 8; return b == 0xFF ? -1 : b; }
+
+// ---------------------------------------------------------------------------
+// XZ candidate (reconstruction_kernels.cuh:465-672)
+template <int MAXH>
+__device__ bool xz_candidate(const Ev<MAXH>& E, int s3, uint16_t q2, uint16_t q3, Tk& T) {
+    const int d3x = s3 == 1 ? 22 : 28, d3xp = d3x - 1;
+    short det[4], sign[4];
+    float X[4], Z[4], ex[4], drift[4];
+    uint16_t hr[4];
+    int n = 0;
+    auto add = [&](int d, int k) {
+        const int r = E.raw(d, k);
+        det[n] = (short)d;
+        ex[n] = G(F_RESOLUTION, d);
+        drift[n] = E.L.drift[r];
+        sign[n] = 0;
+        hr[n] = (uint16_t)r;
+        xz_fy(d, E.L.elem[r], drift[n], 0, 0.f, 0.f, X[n], Z[n]);
+        ++n;
+    };
+    if (pa(q2) >= 0) add(16, pa(q2));
+    if (pb(q2) >= 0) add(15, pb(q2));
+    const int n2 = n;
+    if (n2 == 0) return false;
+    if (pa(q3) >= 0) add(d3x, pa(q3));
+    if (pb(q3) >= 0) add(d3xp, pb(q3));
+    if (n == n2) return false;
+    Fit f = fit2d<4>(n, X, Z, ex);
+    if (fabsf(f.p0) > X0_MAX + 2.f * f.e0 || fabsf(f.p1) > TX_MAX + 2.f * f.e1) return false;
+    lr_xhits<4>(f.p0, f.p1, f.e0, f.e1, n, det, X, drift, sign, 150.f);
+    lr_single_xhits(f.p0, f.p1, n, det, X, sign);
+    float xc[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) xc[i] = i < n ? X[i] + drift[i] * (float)sign[i] : 0.f;
+    f = fit2d<4>(n, xc, Z, ex);
+    const float x0 = f.p0, tx = f.p1, ex0 = f.e0, etx = f.e1;
+    if (fabsf(x0) > X0_MAX || fabsf(tx) > TX_MAX) return false;
+    // prop X: P1X (50 then 49) within 5.08, P2X (52 then 51) within 7.62
+    const int nprop = prop_hit(E, 50, 49, tx, x0, 5.08f) + prop_hit(E, 52, 51, tx, x0, 7.62f);
+    if (nprop < 2) return false;
+    if (!hodo_xz(E, 1, 37, x0, tx, ex0, etx) && !hodo_xz(E, 1, 38, x0, tx, ex0, etx)) return false;
+    if (!hodo_xz(E, 2, 39, x0, tx, ex0, etx) && !hodo_xz(E, 2, 40, x0, tx, ex0, etx)) return false;
+    if (!hodo_xz(E, 3, 45, x0, tx, ex0, etx) && !hodo_xz(E, 3, 46, x0, tx, ex0, etx)) return false;
+    T.x0 = x0; T.tx = tx; T.ex0 = ex0; T.etx = etx;
+    T.y0 = T.ty = T.ey0 = T.ety = 0.f;
+    T.chi2 = T.invp = T.einvp = 0.f;
+    T.nx = (uint8_t)n; T.nuv = 0; T.n1 = 0;
+    T.stid = 0; T.charge = 0;
+    for (int h = 0; h < 18; ++h) {
+        T.resid[h] = 0.f;
+        if (h < 4) T.xpos[h] = h < n ? X[h] : 0.f;
+        T.hraw[h] = h < n ? hr[h] : NOHIT;
+        T.det[h] = h < n ? (uint8_t)det[h] : 0;
+        T.sign[h] = h < n ? (int8_t)sign[h] : 0;
+    }
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// YZ unit: one tracklet, one (u3, v3) pair, inner loop over (u2, v2) pairs in
+// reference order (reconstruction_kernels.cuh:1320-1800).  Station-3 signs set
+// by the left/right step persist across the inner loop as in the reference.
+// search: returns best (chi2, i_uv2) of the unit.  replay (stop >= 0): fills T
+// with the candidate at i_uv2 == stop.
+struct YzLists { const uint16_t* l[4]; int n[4]; };
+
+template <int MAXH>
+__device__ bool yz_unit(const Ev<MAXH>& E, Tk& T, const YzLists& W, int iuv3, int stop, float& best_chi2,
+                        int& best_i) {
+    const float x0 = T.x0, tx = T.tx, ex0 = T.ex0, etx = T.etx;
+    const int nx = T.nx;
+    const int st3 = T.st3;
+    const int du[4][2] = {{super_id(2, 1) * 2, super_id(2, 1) * 2 - 1},
+                          {super_id(2, 2) * 2, super_id(2, 2) * 2 - 1},
+                          {super_id(st3, 1) * 2, super_id(st3, 1) * 2 - 1},
+                          {super_id(st3, 2) * 2, super_id(st3, 2) * 2 - 1}};
+    const int nu2 = W.n[0], nv2 = W.n[1], nu3 = W.n[2];
+    short det[8], sign[8];
+    float Y[8], eY[8], Zz[8], pos[8], drift[8];
+    uint16_t hr[8];
+    int nuv = 0;
+    auto add = [&](int d, int k) {
+        const int r = E.raw(d, k);
+        const int e = E.L.elem[r];
+        det[nuv] = (short)d;
+        Zz[nuv] = G(F_Z, d);
+        sign[nuv] = 0;
+        drift[nuv] = E.L.drift[r];
+        pos[nuv] = E.L.pos[r];
+        hr[nuv] = (uint16_t)r;
+        float y, ey;
+        if (y_uvhit(d, e, drift[nuv], 0, x0, tx, y, ey)) {
+            Y[nuv] = y;
+            eY[nuv] = ey;
+            nuv++;
+        }
+    };
+    const int iu3 = iuv3 % nu3, iv3 = iuv3 / nu3;
+    const uint16_t qu3 = W.l[2][iu3], qv3 = W.l[3][iv3];
+    if (pa(qu3) >= 0) add(du[2][0], pa(qu3));
+    if (pb(qu3) >= 0) add(du[2][1], pb(qu3));
+    const int nu3h = nuv;
+    if (nu3h == 0) return false;
+    if (pa(qv3) >= 0) add(du[3][0], pa(qv3));
+    if (pb(qv3) >= 0) add(du[3][1], pb(qv3));
+    const int nv3h = nuv - nu3h;
+    if (nv3h == 0) return false;
+    const int n3 = nuv;
+    const int nin = nu2 * nv2;
+    const int last = stop >= 0 ? stop : nin - 1;
+    bool found = false;
+    for (int iuv2 = 0; iuv2 <= last; ++iuv2) {
+        const int iu2 = iuv2 % nu2, iv2 = iuv2 / nu2;
+        nuv = n3;
+        const uint16_t qu2 = W.l[0][iu2], qv2 = W.l[1][iv2];
+        if (pa(qu2) >= 0) add(du[0][0], pa(qu2));
+        if (pb(qu2) >= 0) add(du[0][1], pb(qu2));
+        const int nu2h = nuv - n3;
+        if (nu2h == 0) continue;
+        if (pa(qv2) >= 0) add(du[1][0], pa(qv2));
+        if (pb(qv2) >= 0) add(du[1][1], pb(qv2));
+        if (nuv - n3 - nu2h == 0) continue;
+        Fit f = fit2d<8>(nuv, Y, Zz, eY);
+        if (fabsf(f.p0) > Y0_MAX + 2.f * f.e0 || fabsf(f.p1) > TY_MAX + 2.f * f.e1) continue;
+        lr_newhits(x0, tx, f.p0, f.p1, ex0, etx, f.e0, f.e1, nuv, det, pos, drift, sign, 150.f);
+        lr_single_newhits(x0, tx, f.p0, f.p1, nuv, det, pos, sign);
+        f = fit2d<8>(nuv, Y, Zz, eY);  // identical refit (reconstruction_kernels.cuh:1628)
+        const float y0 = f.p0, ty = f.p1, ey0 = f.e0, ety = f.e1;
+        if (fabsf(y0) > Y0_MAX || fabsf(ty) > TY_MAX) continue;
+        // prop Y: P1Y (48 then 47) within 5.08, P2Y (54 then 53) within 10.16
+        const int nprop = prop_hit(E, 48, 47, ty, y0, 5.08f) + prop_hit(E, 54, 53, ty, y0, 10.16f);
+        if (nprop < 2) continue;
+        if (!hodo_tkl(E, 1, 37, x0, y0, tx, ty, ex0, ey0, etx, ety) &&
+            !hodo_tkl(E, 1, 38, x0, y0, tx, ty, ex0, ey0, etx, ety))
+            continue;
+        if (!hodo_tkl(E, 2, 39, x0, y0, tx, ty, ex0, ey0, etx, ety) &&
+            !hodo_tkl(E, 2, 40, x0, y0, tx, ty, ex0, ey0, etx, ety))
+            continue;
+        if (!hodo_tkl(E, 3, 45, x0, y0, tx, ty, ex0, ey0, etx, ety) &&
+            !hodo_tkl(E, 3, 46, x0, y0, tx, ty, ex0, ey0, etx, ety))
+            continue;
+        // chi2 over x hits (stored signs) and UV hits (reconstruction_helper.cuh:1182-1204)
+        float chi2 = 0.f;
+        float rs[12];
+        for (int k = 0; k < nx + nuv; ++k) {
+            int d, e;
+            float dr;
+            short s;
+            if (k < nx) { d = T.det[k]; e = E.L.elem[T.hraw[k]]; dr = E.L.drift[T.hraw[k]]; s = T.sign[k]; }
+            else { d = det[k - nx]; e = E.L.elem[hr[k - nx]]; dr = drift[k - nx]; s = sign[k - nx]; }
+            const float res = G(F_RESOLUTION, d);
+            const float dca = dca_wire(d, e, x0, y0, tx, ty);
+            rs[k] = dr * (float)s - dca;
+            chi2 += ((rs[k] * rs[k]) / res) / res;
+        }
+        if (stop >= 0) {
+            if (iuv2 == stop) {
+                T.stid = 5;
+                T.nuv = (uint8_t)nuv;
+                T.chi2 = chi2; T.y0 = y0; T.ty = ty; T.ey0 = ey0; T.ety = ety;
+                for (int k = 0; k < nx + nuv; ++k) T.resid[k] = rs[k];
+                for (int k = 0; k < nuv; ++k) {
+                    T.hraw[nx + k] = hr[k];
+                    T.det[nx + k] = (uint8_t)det[k];
+                    T.sign[nx + k] = (int8_t)sign[k];
+                }
+                return true;
+            }
+            continue;
+        }
+        if (chi2 < best_chi2) { best_chi2 = chi2; best_i = iuv2; found = true; }
+    }
+    return found;
+}
+
+// ---------------------------------------------------------------------------
+// Station-1 extension of one track (reconstruction_kernels.cuh:2053-2531).
+// Sequential over x pairs: y0/ty left by the last fit carry into the next x
+// pair's x-at-y computation, as in the reference.
+template <int MAXH>
+__device__ int global_track(const Ev<MAXH>& E, Tk& T, uint16_t* lists, const or_params& P) {
+    const int n23 = T.nx + T.nuv;
+    const float x0 = T.x0, tx = T.tx, ex0 = T.ex0, etx = T.etx;
+    float y0 = T.y0, ty = T.ty, ey0 = T.ey0, ety = T.ety;
+    // last hit in z (reconstruction_classes.cuh:407-422)
+    int dlast = 1;
+    float zmax = -10000.f;
+    for (int k = 0; k < n23; ++k) {
+        const int d = T.det[k];
+        if (d > 0 && G(F_Z, d) > zmax) { dlast = d; zmax = G(F_Z, d); }
+    }
+    // SagittaRatioInStation1 (reconstruction_helper.cuh:556-613)
+    float pexp[3], win[3];
+    {
+        const float z3 = G(F_Z, dlast), x3 = x0 + tx * z3, y3 = y0 + ty * z3;
+        for (int i = 0; i < 3; ++i) {
+            const int d = 2 * i + 2;
+            const int idx = i == 0 ? 1 : (i == 1 ? 0 : 2);  // planetype[d/2-1]
+            const int d2 = super_id(2, idx) * 2;
+            const float ct = G(F_COSTHETA, d2), st = G(F_SINTHETA, d2);
+            const float ps3 = x3 * ct + y3 * st;
+            const float z1 = G(F_Z, d), z2 = G(F_Z, d2);
+            const float x2 = x0 + tx * z2, y2 = y0 + ty * z2;
+            const float ps2 = x2 * ct + y2 * st;
+            const float s2t = ps2 - (ps3 * (z2 - Z_TARGET)) / (z3 - Z_TARGET);
+            const float s2d = ps2 - (ps3 * (z2 - Z_DUMP)) / (z3 - Z_DUMP);
+            const float pet = SAG_T_C * s2t + (ps3 * (z1 - Z_TARGET)) / (z3 - Z_TARGET);
+            const float ped = SAG_D_C * s2d + (ps3 * (z1 - Z_DUMP)) / (z3 - Z_DUMP);
+            const float wt = fabsf(s2t * SAG_T_W), wd = fabsf(s2d * SAG_D_W);
+            const float pmn = fminf(pet - wt, ped - wd), pmx = fmaxf(pet + wt, ped + wd);
+            pexp[idx] = (float)(0.5 * (double)(pmx + pmn));
+            win[idx] = (float)(0.5 * (double)(pmx - pmn));
+        }
+    }
+    uint16_t* lx = lists;
+    uint16_t* lu = lists + WIN_CAP;
+    uint16_t* lv = lists + 2 * WIN_CAP;
+    const int capw = P.max_pairs_window < WIN_CAP ? P.max_pairs_window : WIN_CAP;
+    const int nx1 = pairs_window(E, 0, 0, pexp[0] - win[0] * 1.0f, pexp[0] + win[0] * 1.0f, lx, P.max_pair_plane_hits, capw);
+    const int nu1 = pairs_window(E, 0, 1, pexp[1] - win[1] * 1.0f, pexp[1] + win[1] * 1.0f, lu, P.max_pair_plane_hits, capw);
+    const int nv1 = pairs_window(E, 0, 2, pexp[2] - win[2] * 1.0f, pexp[2] + win[2] * 1.0f, lv, P.max_pair_plane_hits, capw);
+    if (nx1 < 0 || nu1 < 0 || nv1 < 0) return -1;
+    if (nx1 == 0 || nu1 == 0 || nv1 == 0) return 0;
+    // y of the station-2/3 UV hits with their stored signs
+    float Y[18], eY[18], Zy[18];
+    int ny = 0;
+    for (int k = 0; k < n23; ++k) {
+        const int d = T.det[k];
+        if ((double)G(F_COSTHETA, d) > 0.99) continue;
+        float y, ey;
+        if (y_uvhit(d, E.L.elem[T.hraw[k]], E.L.drift[T.hraw[k]], T.sign[k], x0, tx, y, ey)) {
+            Y[ny] = y; eY[ny] = ey; Zy[ny] = G(F_Z, d); ny++;
+        }
+    }
+    short det[6], sign[6];
+    float pos[6], drift[6], X[3], eX[3], Z[3];
+    uint16_t hr[6];
+    float chi2min = 10000.1f;
+    bool found = false;
+    float b_chi2 = 0, b_y0 = 0, b_ty = 0, b_ey0 = 0, b_ety = 0, b_invp = 0, b_einvp = 0;
+    int b_q = 0, b_n1 = 0;
+    float b_r23[12], b_r1[6];
+    uint16_t b_hr[6];
+    short b_det[6], b_sign[6];
+    for (int ix = 0; ix < nx1; ++ix) {
+        int nx = 0;
+        auto addx = [&](int d, int k) {
+            const int r = E.raw(d, k);
+            det[nx] = (short)d;
+            pos[nx] = E.L.pos[r];
+            eX[nx] = G(F_RESOLUTION, d);
+            sign[nx] = 0;
+            drift[nx] = E.L.drift[r];
+            hr[nx] = (uint16_t)r;
+            xz_fy(d, E.L.elem[r], drift[nx], 0, y0, ty, X[nx], Z[nx]);
+            ++nx;
+        };
+        if (pa(lx[ix]) >= 0) addx(4, pa(lx[ix]));
+        if (pb(lx[ix]) >= 0) addx(3, pb(lx[ix]));
+        if (nx == 0) continue;
+        X[nx] = x0 + tx * Z_KMAG_BEND;
+        eX[nx] = ex0 + etx * Z_KMAG_BEND;
+        Z[nx] = Z_KMAG_BEND;
+        Fit f = fit2d<3>(nx + 1, X, Z, eX);
+        float x01 = f.p0, tx1 = f.p1;
+        lr_xhits<2>(x01, tx1, x01, tx1, nx, det, X, drift, sign, 150.f);
+        lr_single_xhits(x01, tx1, nx, det, X, sign);
+        for (int l = 0; l < nx; ++l) X[l] += (float)sign[l] * drift[l];
+        f = fit2d<3>(nx + 1, X, Z, eX);
+        x01 = f.p0; tx1 = f.p1;
+        const float ex01 = f.p0, etx1 = f.p1;  // parameters used as errors (reconstruction_kernels.cuh:2194-2195)
+        // calculate_charge (E1039), reconstruction_helper.cuh:684-691
+        const int q = (double)(-0.0033f) * copysign(1.0, (double)FMAGSTR) * (double)x0 < (double)tx ? 1 : -1;
+        const float invp = ((tx1 - tx) * (float)q) / PT_KICK_KMAG;
+        const float einvp = (etx - etx) / PT_KICK_KMAG;
+        if (invp > INVP_MAX + einvp || invp < INVP_MIN - einvp) continue;
+        for (int iuv = 0; iuv < nu1 * nv1; ++iuv) {
+            const int iu = iuv % nu1, iv = iuv / nu1;
+            int nuv = 0;
+            auto adduv = [&](int d, int k) {
+                const int r = E.raw(d, k);
+                const int kk = nx + nuv;
+                det[kk] = (short)d;
+                sign[kk] = 0;
+                drift[kk] = E.L.drift[r];
+                pos[kk] = E.L.pos[r];
+                hr[kk] = (uint16_t)r;
+                float y, ey;
+                if (y_uvhit(d, E.L.elem[r], drift[kk], 0, x01, tx1, y, ey)) {
+                    Y[ny + nuv] = y; eY[ny + nuv] = ey; Zy[ny + nuv] = G(F_Z, d);
+                    nuv++;
+                }
+            };
+            if (pa(lu[iu]) >= 0) adduv(2, pa(lu[iu]));
+            if (pb(lu[iu]) >= 0) adduv(1, pb(lu[iu]));
+            const int nuh = nuv;
+            if (nuh == 0) continue;
+            if (pa(lv[iv]) >= 0) adduv(6, pa(lv[iv]));
+            if (pb(lv[iv]) >= 0) adduv(5, pb(lv[iv]));
+            if (nuv - nuh == 0) continue;
+            Fit g = fit2d<12>(ny + nuv, Y, Zy, eY);
+            y0 = g.p0; ty = g.p1; ey0 = g.e0; ety = g.e1;
+            if (fabsf(y0) > Y0_MAX + 2.f * ey0 || fabsf(ty) > TY_MAX + 2.f * ety) continue;
+            lr_newhits(x01, tx1, y0, ty, ex01, etx1, ey0, ety, nx + nuv, det, pos, drift, sign, 150.f);
+            lr_single_newhits(x01, tx1, y0, ty, nx + nuv, det, pos, sign);
+            g = fit2d<12>(ny + nuv, Y, Zy, eY);
+            y0 = g.p0; ty = g.p1; ey0 = g.e0; ety = g.e1;
+            if (fabsf(y0) > Y0_MAX || fabsf(ty) > TY_MAX) continue;
+            // H1 match is computed but not enforced (reconstruction_kernels.cuh:2424-2440)
+            float chi2 = 0.f;
+            float r23[12];
+            for (int m = 0; m < n23; ++m) {
+                const int d = T.det[m];
+                const int e = E.L.elem[T.hraw[m]];
+                const float res = G(F_RESOLUTION, d);
+                const float dca = dca_wire(d, e, x0, y0, tx, ty);
+                r23[m] = E.L.drift[T.hraw[m]] * (float)T.sign[m] - dca;
+                chi2 += (r23[m] * r23[m]) / (res * res);
+            }
+            float c1 = 0.f;
+            float r1[6];
+            for (int k = 0; k < nx + nuv; ++k) {
+                const int d = det[k];
+                const float res = G(F_RESOLUTION, d);
+                const float dca = dca_wire(d, E.L.elem[hr[k]], x01, y0, tx1, ty);
+                r1[k] = drift[k] * (float)sign[k] - dca;
+                c1 += ((r1[k] * r1[k]) / res) / res;
+            }
+            chi2 += c1;
+            if (chi2 > 10000.f) continue;
+            if (chi2 < chi2min) {
+                chi2min = chi2;
+                found = true;
+                b_chi2 = chi2; b_y0 = y0; b_ty = ty; b_ey0 = ey0; b_ety = ety; b_invp = invp; b_einvp = einvp;
+                b_q = q; b_n1 = nx + nuv;
+                for (int m = 0; m < n23; ++m) b_r23[m] = r23[m];
+                for (int k = 0; k < nx + nuv; ++k) {
+                    b_r1[k] = r1[k]; b_hr[k] = hr[k]; b_det[k] = det[k]; b_sign[k] = sign[k];
+                }
+            }
+        }
+    }
+    if (found) {
+        T.stid = 6;
+        T.n1 = (uint8_t)b_n1;
+        T.chi2 = b_chi2; T.y0 = b_y0; T.ty = b_ty; T.ey0 = b_ey0; T.ety = b_ety;
+        T.invp = b_invp; T.einvp = b_einvp; T.charge = (int8_t)b_q;
+        for (int m = 0; m < n23; ++m) T.resid[m] = b_r23[m];
+        for (int k = 0; k < b_n1; ++k) {
+            T.resid[n23 + k] = b_r1[k];
+            T.hraw[n23 + k] = b_hr[k];
+            T.det[n23 + k] = (uint8_t)b_det[k];
+            T.sign[n23 + k] = (int8_t)b_sign[k];
+        }
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+template <int MAXH>
+__global__ __launch_bounds__(WAVE) void chain_kernel(Args a) {
+    __shared__ Lds<MAXH> L;
+    const int lane = lane_id();
+    Ev<MAXH> E{L};
+    char* scr = a.scratch + (int64_t)blockIdx.x * a.scratch_stride;
+    Tk* tks = reinterpret_cast<Tk*>(scr + SCR_TK);
+    uint32_t* celloff = reinterpret_cast<uint32_t*>(scr + SCR_CELL);
+    uint16_t* wl = reinterpret_cast<uint16_t*>(scr + SCR_LIST);
+    const or_params& P = a.p;
+
+    for (int64_t e = blockIdx.x; e < a.n_events; e += gridDim.x) {
+        const int64_t h0 = a.off[e];
+        const int n = (int)(a.off[e + 1] - h0);
+        if (lane == 0) L.status = 0;
+        if (lane < 56) L.pcnt[lane] = 0;
+        __syncthreads();
+        int ntkl = 0;
+        bool staged = false;
+        if (n > MAXH || n > P.max_hits_per_event) {
+            if (lane == 0) L.status = OR_EV_EVENT_TOO_LARGE;
+        } else {
+            // ---- A: count per detector, validate ------------------------------
+            for (int h = lane; h < n; h += WAVE) {
+                const int d = a.det[h0 + h], el = a.elem[h0 + h];
+                if (d > 54) continue;
+                if (d < 1 || el < 1 || el > (int)G(F_NELEM, d) || el >= OR_MAX_ELEM) atomicOr(&L.status, (uint32_t)OR_EV_BAD_HIT);
+                else atomicAdd(&L.pcnt[d], 1);
+            }
+            __syncthreads();
+            const bool badhit = L.status != 0;
+            {
+                const int d = lane;
+                int c = (d >= 1 && d <= 54) ? L.pcnt[d] : 0;
+                if (!badhit && d >= 1 && d <= 54) {
+                    const int cap = d <= 30 ? P.max_hits_chamber_plane : (d <= 46 ? P.max_hits_hodo_plane : P.max_hits_prop_plane);
+                    if (c > cap || c > PLANE_CAP) atomicOr(&L.status, (uint32_t)OR_EV_PLANE_OVERFLOW);
+                }
+                const int inc = wave_scan(c);
+                if (lane < 56) { L.pstart[lane] = inc - c; L.cursor[lane] = inc - c; }
+            }
+            __syncthreads();
+            if (L.status == 0) {
+                // stable scatter by detector
+                for (int base = 0; base < n; base += WAVE) {
+                    const int h = base + lane;
+                    int d = 0, el = 0;
+                    bool v = h < n;
+                    if (v) { d = a.det[h0 + h]; el = a.elem[h0 + h]; v = d >= 1 && d <= 54; }
+                    unsigned long long todo = ballot(v);
+                    int dest = 0;
+                    while (todo) {
+                        const int leader = __ffsll((long long)todo) - 1;
+                        const int dl = __shfl(d, leader, WAVE);
+                        const unsigned long long m = ballot(v && d == dl);
+                        const int cur = L.cursor[dl];
+                        if (v && d == dl) dest = cur + rank_in(m);
+                        __syncthreads();
+                        if (lane == leader) L.cursor[dl] = cur + __popcll(m);
+                        __syncthreads();
+                        todo &= ~m;
+                    }
+                    if (v) {
+                        L.elem[dest] = (int16_t)el;
+                        L.pos[dest] = a.wire[d * OR_MAX_ELEM + el];
+                        L.tdc[dest] = a.tdc[h0 + h];
+                        L.drift[dest] = fabsf(a.drift[h0 + h]);
+                        L.orig[dest] = (uint16_t)h;
+                        const int fl = a.flag ? (int)a.flag[h0 + h] : 2;
+                        L.intime[dest] = (fl & 2) ? 1 : 0;
+                    }
+                }
+                for (int h = lane; h < n; h += WAVE) L.korig[h] = 0;
+                __syncthreads();
+                // ---- B: reduction, one lane per plane (reconstruction_helper.cuh:168-361)
+                int nred = 0;
+                bool clover = false;
+                const int d = lane;
+                if (d >= 1 && d <= 54) {
+                    const int s = L.pstart[d], np = L.pcnt[d];
+                    const bool reduce = d <= 6 || d >= 13;
+                    if (!reduce) {
+                        for (int i = 0; i < np; ++i) L.keep[s + i] = 1;
+                        nred = np;
+                    } else {
+                        int ncl = 0, first = 0, lastm = 0;
+                        for (int i = 0; i < np; ++i) {
+                            const int r = s + i;
+                            if (!L.intime[r]) { L.keep[r] = -1; continue; }
+                            L.keep[r] = 1;
+                            if (d > 30) continue;
+                            if (ncl == 0) { first = i; lastm = i; ncl = 1; continue; }
+                            const bool adj = fabsf((float)L.elem[r] - (float)L.elem[s + lastm]) <= 1.f;
+                            if (adj && i != np - 1) { if (ncl >= 192) clover = true; ++ncl; lastm = i; continue; }
+                            if (i == np - 1 && adj) { if (ncl >= 192) clover = true; ++ncl; lastm = i; }
+                            if (ncl == 2) {
+                                const int ra = s + first, rb = s + lastm;
+                                const float wmax = (float)(0.9 * 0.5 * (double)(L.pos[rb] - L.pos[ra]));
+                                const float wmin = (float)(4.0 / 9.0 * (double)wmax);
+                                const float da = L.drift[ra], db = L.drift[rb];
+                                if ((da > wmax && db > wmin) || (da > wmin && db > wmax)) {
+                                    if (da > db) L.keep[ra] = -1;
+                                    else L.keep[rb] = -1;
+                                } else {
+                                    const float dt = L.tdc[ra] - L.tdc[rb];
+                                    if (((dt >= 0.0f && dt < 8.0f) || (dt <= 0.0f && dt > -8.0f)) && d >= 19 && d <= 24) {
+                                        L.keep[ra] = -1;
+                                        L.keep[rb] = -1;
+                                    }
+                                }
+                            }
+                            if (ncl >= 3) {
+                                float dtm = 0.0f;
+                                int prev = -1;
+                                for (int c = first; c <= lastm; ++c) {
+                                    if (!L.intime[s + c]) continue;
+                                    if (prev >= 0) {
+                                        const float dd = L.tdc[s + c] - L.tdc[s + prev];
+                                        dtm += dd > 0.0f ? dd : (L.tdc[s + prev] - L.tdc[s + c]);
+                                    }
+                                    prev = c;
+                                }
+                                dtm = dtm / (float)(ncl - 1);
+                                const bool all = dtm < 10.0f;
+                                for (int c = first; c <= lastm; ++c) {
+                                    if (!L.intime[s + c]) continue;
+                                    if (all || c != first) L.keep[s + c] = -1;
+                                }
+                            }
+                            ncl = 0;
+                        }
+                        // first good hit per channel wins (reconstruction_helper.cuh:331-349)
+                        for (int i = 0; i < np; ++i) {
+                            const int r = s + i;
+                            if (L.keep[r] <= 0) continue;
+                            bool dup = false;
+                            for (int j = 0; j < i; ++j)
+                                if (L.keep[s + j] > 0 && L.elem[s + j] == L.elem[r]) { dup = true; break; }
+                            if (dup) L.keep[r] = -1;
+                            else ++nred;
+                        }
+                    }
+                    if (reduce && nred > P.max_reduced_per_plane) atomicOr(&L.status, (uint32_t)OR_EV_PLANE_OVERFLOW);
+                    if (clover) atomicOr(&L.status, (uint32_t)OR_EV_CLUSTER_OVERFLOW);
+                    L.rcnt[d] = nred;
+                } else if (lane < 56) {
+                    L.rcnt[lane] = 0;
+                }
+                {
+                    const int c = (d >= 1 && d <= 54) ? nred : 0;
+                    const int inc = wave_scan(c);
+                    if (lane < 56) L.rstart[lane] = inc - c;
+                    // station multiplicities (reconstruction_kernels.cuh:127-139), 8 written rows summed
+                    const int m0 = wave_sum((d >= 1 && d <= 6) ? nred : 0);
+                    const int m2 = wave_sum((d >= 13 && d <= 18) ? nred : 0);
+                    const int m3 = wave_sum((d >= 19 && d <= 24) ? nred : 0);
+                    const int m4 = wave_sum((d >= 25 && d <= 30) ? nred : 0);
+                    __syncthreads();
+                    if (lane == 0 && L.status == 0) {
+                        uint32_t st = 0;
+                        if (m0 > P.max_d0_mult) st |= OR_EV_D0_MULT;
+                        if (m2 > P.max_d2_mult) st |= OR_EV_D2_MULT;
+                        if (m3 > P.max_d3_mult) st |= OR_EV_D3P_MULT;
+                        if (m4 > P.max_d3_mult) st |= OR_EV_D3M_MULT;
+                        L.status = st;
+                        staged = true;
+                    }
+                }
+                __syncthreads();
+                // reduced lists, kept flags by input order
+                if (d >= 1 && d <= 54) {
+                    const int s = L.pstart[d], np = L.pcnt[d];
+                    int k = L.rstart[d];
+                    for (int i = 0; i < np; ++i)
+                        if (L.keep[s + i] > 0) { L.red[k++] = (uint16_t)(s + i); L.korig[L.orig[s + i]] = 1; }
+                }
+                __syncthreads();
+            }
+        }
+        staged = __shfl(staged ? 1 : 0, 0, WAVE) != 0;
+        // keep bits: written when staging and reduction passed their caps
+        if (staged) {
+            const int64_t w0 = h0 >> 5, w1 = (h0 + n - 1) >> 5;
+            for (int64_t w = w0 + lane; n > 0 && w <= w1; w += WAVE) {
+                uint32_t bits = 0;
+                for (int b = 0; b < 32; ++b) {
+                    const int64_t h = w * 32 + b - h0;
+                    if (h >= 0 && h < n && L.korig[h]) bits |= 1u << b;
+                }
+                if (bits) atomicOr(&a.keep_mask[w], bits);
+            }
+        }
+        __syncthreads();
+
+        // ---- C: binned X pairs -------------------------------------------------
+        if (L.status == 0) {
+            // hitflag1/2[100] bound (reconstruction_kernels.cuh:225-226) on all six planes first
+            const int pc = (lane < 6) ? L.rcnt[lane < 2 ? 16 - lane : (lane < 4 ? 24 - lane : 32 - lane)] : 0;
+            const bool over = ballot(lane < 6 && pc > P.max_pair_plane_hits) != 0ull;
+            if (over) {
+                if (lane == 0) L.status |= OR_EV_PAIR_OVERFLOW;
+            } else if (lane < 3) {
+                const int s = lane;
+                const int d1 = s == 0 ? 16 : (s == 1 ? 22 : 28), d2 = d1 - 1;
+                const int n1 = L.rcnt[d1], n2 = L.rcnt[d2];
+                for (int b = 0; b < 28; ++b) L.xcnt[s][b] = 0;
+                {
+                    const float sp = spacing_plane(super_id(2 + s, 0));
+                    const int cap = P.max_pairs_per_bin < BIN_CAP ? P.max_pairs_per_bin : BIN_CAP;
+                    uint8_t f1[PLANE_CAP / 8 + 1], f2[PLANE_CAP / 8 + 1];
+                    for (int i = 0; i < PLANE_CAP / 8 + 1; ++i) { f1[i] = 0; f2[i] = 0; }
+                    auto put = [&](int c, int ia, int ib) {
+                        const int b = xbin(s, c);
+                        if (b < 0) return;
+                        const int k = L.xcnt[s][b];
+                        if (k < cap) L.xpairs[s][b][k] = (uint16_t)((ia & 0xFF) | ((ib & 0xFF) << 8));
+                        L.xcnt[s][b] = k + 1;
+                    };
+                    for (int i = 0; i < n1; ++i) {
+                        const int ri = E.raw(d1, i);
+                        for (int j = 0; j < n2; ++j) {
+                            const int rj = E.raw(d2, j);
+                            if (fabsf(L.pos[ri] - L.pos[rj]) > sp) continue;
+                            put(L.elem[ri], i, j);
+                            f1[i >> 3] |= (uint8_t)(1u << (i & 7));
+                            f2[j >> 3] |= (uint8_t)(1u << (j & 7));
+                        }
+                    }
+                    for (int i = 0; i < n1; ++i)
+                        if (!((f1[i >> 3] >> (i & 7)) & 1)) put(L.elem[E.raw(d1, i)], i, 0xFF);
+                    for (int j = 0; j < n2; ++j)
+                        if (!((f2[j >> 3] >> (j & 7)) & 1)) put(L.elem[E.raw(d2, j)], 0xFF, j);
+                    for (int b = 0; b < 28; ++b)
+                        if (L.xcnt[s][b] > P.max_pairs_per_bin) atomicOr(&L.status, (uint32_t)OR_EV_XZ_BIN_OVERFLOW);
+                }
+            }
+            __syncthreads();
+        }
+
+        // ---- D: XZ combinations in canonical order --------------------------
+        if (L.status == 0) {
+            uint32_t run = 0;
+            for (int c0 = 0; c0 < 1568; c0 += WAVE) {
+                const int c = c0 + lane;
+                int cnt = 0;
+                if (c < 1568) {
+                    const int t = c / 49, i = c % 49;
+                    const int s3 = 1 + (t & 1);
+                    const int b2 = (t / 8) * 7 + i % 7, b3 = ((t % 8) / 2) * 7 + i / 7;
+                    cnt = L.xcnt[0][b2] * L.xcnt[s3][b3];
+                }
+                const int inc = wave_scan(cnt);
+                if (c < 1568) celloff[c + 1] = run + (uint32_t)inc;
+                if (c == 0) celloff[0] = 0;
+                run += (uint32_t)__shfl(inc, WAVE - 1, WAVE);
+            }
+            __syncthreads();
+            const uint32_t total = run;
+            int np = 0, nm = 0;
+            for (uint32_t g0 = 0; g0 < total; g0 += WAVE) {
+                const uint32_t g = g0 + lane;
+                bool acc = false;
+                Tk T;
+                int par = 0;
+                if (g < total) {
+                    int lo = 0, hi = 1568;  // largest c with celloff[c] <= g
+                    while (hi - lo > 1) {
+                        const int mid = (lo + hi) >> 1;
+                        if (celloff[mid] <= g) lo = mid;
+                        else hi = mid;
+                    }
+                    const int c = lo;
+                    const int t = c / 49, i = c % 49;
+                    const int s3 = 1 + (t & 1);
+                    const int b2 = (t / 8) * 7 + i % 7, b3 = ((t % 8) / 2) * 7 + i / 7;
+                    const int nx2 = L.xcnt[0][b2];
+                    const int ix = (int)(g - celloff[c]);
+                    const int ix2 = ix % nx2, ix3 = ix / nx2;
+                    par = t & 1;
+                    acc = xz_candidate(E, s3, L.xpairs[0][b2][ix2], L.xpairs[s3][b3][ix3], T);
+                    T.t = (uint8_t)t;
+                    T.st3 = (uint8_t)(3 + par);
+                }
+                const unsigned long long m = ballot(acc);
+                const unsigned long long mp = ballot(acc && par == 0);
+                const unsigned long long mm = ballot(acc && par == 1);
+                if (acc) {
+                    const int k = ntkl + rank_in(m);
+                    const int kp = par == 0 ? np + rank_in(mp) : nm + rank_in(mm);
+                    T.canon = (uint32_t)k;
+                    T.slot = (uint8_t)(2 * (kp % 16) + par);
+                    T.pos_in_slot = (uint8_t)(kp / 16 < 255 ? kp / 16 : 255);
+                    if (k < MAXTKL) tks[k] = T;
+                }
+                ntkl += __popcll(m);
+                np += __popcll(mp);
+                nm += __popcll(mm);
+            }
+            if (lane == 0) {
+                if (np > 16 * P.max_tracklets_per_slot || nm > 16 * P.max_tracklets_per_slot ||
+                    ntkl >= P.max_tracklets_per_event || ntkl > MAXTKL)
+                    atomicOr(&L.status, (uint32_t)OR_EV_TRACKLET_OVERFLOW);
+            }
+            // slot order: slot s = 2q + p holds items kp = q, q+16, ...
+            {
+                const int p = lane & 1, q = lane >> 1;
+                const int npar = p ? nm : np;
+                const int cs = (lane < 32 && npar > q) ? (npar - q + 15) / 16 : 0;
+                const int inc = wave_scan(cs);
+                if (lane < 32) L.sstart[lane] = inc - cs;
+                if (lane == 32) L.sstart[32] = ntkl < MAXTKL ? ntkl : MAXTKL;
+            }
+            __syncthreads();
+            if (L.status == 0)
+                for (int k = lane; k < ntkl; k += WAVE) L.slot_order[L.sstart[tks[k].slot] + tks[k].pos_in_slot] = (uint16_t)k;
+            __syncthreads();
+        }
+
+        // ---- E: YZ ------------------------------------------------------------
+        if (L.status == 0) {
+            const int capw = P.max_pairs_window < WIN_CAP ? P.max_pairs_window : WIN_CAP;
+            for (int kb = 0; kb < ntkl; kb += YZ_CHUNK) {
+                // window lists: lane = (tracklet j, view v)
+                {
+                    const int j = lane >> 2, v = lane & 3;
+                    const int k = kb + j;
+                    if (k < ntkl) {
+                        const Tk& T = tks[k];
+                        const int st = v < 2 ? 2 : T.st3, pj = (v & 1) ? 2 : 1;
+                        const int sid = super_id(st, pj);
+                        const int da = sid * 2, db = sid * 2 - 1;
+                        const float za = G(F_Z, da), zb = G(F_Z, db);
+                        const float xmin = fminf((T.tx - T.etx) * za, (T.tx - T.etx) * zb) + T.x0 - T.ex0 - G(F_SPACING, da);
+                        const float xmax = fmaxf((T.tx + T.etx) * za, (T.tx + T.etx) * zb) + T.x0 + T.ex0 + G(F_SPACING, da);
+                        const int nl = pairs_window(E, st, pj, xmin, xmax, wl + (j * 4 + v) * WIN_CAP, P.max_pair_plane_hits, capw);
+                        if (nl < 0) atomicOr(&L.status, (uint32_t)OR_EV_PAIR_OVERFLOW);
+                        L.nl[j][v] = nl < 0 ? 0 : nl;
+                    }
+                }
+                __syncthreads();
+                if (L.status) break;
+                {
+                    const int j = lane, k = kb + j;
+                    int u = 0;
+                    if (j < YZ_CHUNK && k < ntkl) {
+                        const int* nl = L.nl[j];
+                        if (nl[0] > 0 && nl[1] > 0 && nl[2] > 0 && nl[3] > 0) u = nl[2] * nl[3];
+                    }
+                    const int inc = wave_scan(u);
+                    if (j < YZ_CHUNK) {
+                        L.best[j] = ~0ull;
+                        L.uoff[j + 1] = inc;
+                    }
+                    if (j == 0) L.uoff[0] = 0;
+                }
+                __syncthreads();
+                const int U = L.uoff[YZ_CHUNK];
+                for (int u0 = 0; u0 < U; u0 += WAVE) {
+                    const int u = u0 + lane;
+                    if (u < U) {
+                        int j = 0;
+                        while (L.uoff[j + 1] <= u) ++j;
+                        const int iuv3 = u - L.uoff[j];
+                        Tk T = tks[kb + j];
+                        YzLists W;
+                        for (int v = 0; v < 4; ++v) { W.l[v] = wl + (j * 4 + v) * WIN_CAP; W.n[v] = L.nl[j][v]; }
+                        float bc = 10000.1f;
+                        int bi = -1;
+                        if (yz_unit(E, T, W, iuv3, -1, bc, bi)) {
+                            const unsigned long long key =
+                                ((unsigned long long)__float_as_uint(bc) << 32) |
+                                (unsigned long long)(uint32_t)(iuv3 * (W.n[0] * W.n[1]) + bi);
+                            atomicMin(&L.best[j], key);
+                        }
+                    }
+                }
+                __syncthreads();
+                if (lane < YZ_CHUNK && kb + lane < ntkl) {
+                    const int j = lane;
+                    Tk T = tks[kb + j];
+                    const unsigned long long key = L.best[j];
+                    if (key != ~0ull) {
+                        YzLists W;
+                        for (int v = 0; v < 4; ++v) { W.l[v] = wl + (j * 4 + v) * WIN_CAP; W.n[v] = L.nl[j][v]; }
+                        const int flat = (int)(uint32_t)key, nin = W.n[0] * W.n[1];
+                        float bc = 0.f;
+                        int bi = 0;
+                        yz_unit(E, T, W, flat / nin, flat % nin, bc, bi);
+                    } else {
+                        T.stid = 3;
+                    }
+                    tks[kb + j] = T;
+                }
+                __syncthreads();
+            }
+        }
+
+        // ---- F: back-partial cleaning (reconstruction_helper.cuh:1789-1909) ----
+        if (L.status == 0) {
+            if (lane == 0 && ntkl > 0) {
+                auto shared_hits = [&](const Tk& A, int na, const Tk& B, int nb) {
+                    int c = 0;
+                    for (int i = 0; i < na; ++i)
+                        for (int j = 0; j < nb; ++j)
+                            if (B.det[j] == A.det[i] && B.hraw[j] == A.hraw[i]) { c++; break; }
+                    return c;
+                };
+                auto pass = [&](int ib, int ie, int jb_self, int jb, int je) {
+                    for (int ii = ib; ii < ie; ++ii) {
+                        Tk& A = tks[L.slot_order[ii]];
+                        if (A.stid < 5) continue;
+                        const int ni = A.nx + A.nuv;
+                        int common = 0;
+                        for (int jj = (jb_self ? ii + 1 : jb); jj < je; ++jj) {
+                            Tk& B = tks[L.slot_order[jj]];
+                            if (B.stid < 5) continue;
+                            const int nj = B.nx + B.nuv;
+                            common += shared_hits(A, ni, B, nj);
+                            if (A.chi2 / (float)(ni - 4) < B.chi2 / (float)(nj - 4)) {
+                                if ((float)common > (float)ni * 0.3333f) B.stid = 3;
+                            } else {
+                                if ((float)common > (float)nj * 0.3333f) A.stid = 3;
+                            }
+                        }
+                    }
+                };
+                // within each slot, then slot against slot in slot order
+                for (int s = 0; s < 32; ++s) pass(L.sstart[s], L.sstart[s + 1], 1, 0, L.sstart[s + 1]);
+                for (int s = 0; s < 32; ++s)
+                    for (int o = 0; o < 32; ++o)
+                        if (o != s) pass(L.sstart[s], L.sstart[s + 1], 0, L.sstart[o], L.sstart[o + 1]);
+            }
+            __syncthreads();
+        }
+
+        // ---- G: station-1 extension ------------------------------------------
+        if (L.status == 0) {
+            for (int kb = 0; kb < ntkl; kb += WAVE) {
+                const int k = kb + lane;
+                if (k < ntkl) {
+                    Tk T = tks[k];
+                    if (T.stid >= 5) {
+                        const int r = global_track(E, T, wl + lane * 3 * WIN_CAP, P);
+                        if (r < 0) atomicOr(&L.status, (uint32_t)OR_EV_PAIR_OVERFLOW);
+                        tks[k] = T;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+
+        // ---- H: outputs --------------------------------------------------------
+        const uint32_t st = L.status;
+        const bool rejected = (st & OR_EV_REJECT_MASK) != 0;
+        int nout = 0;
+        if (!rejected) {
+            for (int j0 = 0; j0 < ntkl; j0 += WAVE) {
+                const int j = j0 + lane;
+                const Tk* Tp = j < ntkl ? &tks[L.slot_order[j]] : nullptr;
+                const bool is6 = Tp && Tp->stid == 6;
+                const unsigned long long m = ballot(is6);
+                if (is6) {
+                    const int o = nout + rank_in(m);
+                    if (o < P.max_tracks_out) {
+                        const Tk& T = *Tp;
+                        or_track R;
+                        R.x0 = T.x0; R.tx = T.tx; R.y0 = T.y0; R.ty = T.ty; R.invp = T.invp;
+                        R.err_x0 = T.ex0; R.err_tx = T.etx; R.err_y0 = T.ey0; R.err_ty = T.ety; R.err_invp = T.einvp;
+                        R.chi2 = T.chi2;
+                        const int nh = T.nx + T.nuv + T.n1;
+                        uint32_t sp = 0, sm = 0;
+                        for (int h = 0; h < OR_MAX_HITS_PER_TRACK; ++h) {
+                            R.hit_index[h] = h < nh ? L.orig[T.hraw[h]] : 0;
+                            if (h < nh && T.sign[h] > 0) sp |= 1u << h;
+                            if (h < nh && T.sign[h] < 0) sm |= 1u << h;
+                        }
+                        R.sign_plus = sp; R.sign_minus = sm;
+                        R.station_id = T.stid; R.nhits = (uint8_t)nh; R.charge = T.charge; R.slot = T.slot;
+                        R.tracklet_id = T.canon;
+                        a.tracks[e * P.max_tracks_out + o] = R;
+                    }
+                }
+                nout += __popcll(m);
+            }
+        }
+        if (lane == 0) {
+            uint32_t so = st;
+            if (!rejected && nout > P.max_tracks_out) so |= OR_EV_OUTPUT_OVERFLOW;
+            a.status[e] = so;
+            a.n_tracks[e] = rejected ? 0 : (nout < P.max_tracks_out ? nout : P.max_tracks_out);
+            if (a.n_tracklets) a.n_tracklets[e] = rejected ? 0 : ntkl;
+        }
+        if (a.records) {
+            float* base = a.records + (int64_t)e * P.max_tracklets_per_event * OR_TRACK_PARAMS;
+            const int nrec = rejected ? 0 : (ntkl < P.max_tracklets_per_event ? ntkl : P.max_tracklets_per_event);
+            for (int64_t w = lane; w < (int64_t)P.max_tracklets_per_event * OR_TRACK_PARAMS; w += WAVE) {
+                const int j = (int)(w / OR_TRACK_PARAMS), f = (int)(w % OR_TRACK_PARAMS);
+                float v = 0.f;
+                if (j < nrec) {
+                    const Tk& T = tks[L.slot_order[j]];
+                    const int nh = T.nx + T.nuv + T.n1;
+                    switch (f) {
+                        case 0: v = (float)(T.stid == 0 ? 3 : T.stid); break;
+                        case 1: v = (float)T.t; break;
+                        case 2: v = (float)nh; break;
+                        case 3: v = T.chi2; break;
+                        case 5: v = T.tx; break;
+                        case 6: v = T.ty; break;
+                        case 7: v = T.x0; break;
+                        case 8: v = T.y0; break;
+                        case 9: v = T.invp; break;
+                        case 10: v = T.etx; break;
+                        case 11: v = T.ety; break;
+                        case 12: v = T.ex0; break;
+                        case 13: v = T.ey0; break;
+                        case 14: v = T.einvp; break;
+                        case 15: v = (float)T.charge; break;
+                        default:
+                            if (f >= 22 && f < 130 + 18) {
+                                const int blk = (f - 22) / 18, h = (f - 22) % 18;
+                                if (h < nh) {
+                                    const int r = T.hraw[h];
+                                    switch (blk) {
+                                        case 0: v = (float)T.det[h]; break;
+                                        case 1: v = (float)L.elem[r]; break;
+                                        case 2: v = h < T.nx ? T.xpos[h] : L.pos[r]; break;
+                                        case 3: v = L.drift[r]; break;
+                                        case 4: v = (float)T.sign[h]; break;
+                                        case 5: v = L.tdc[r]; break;
+                                        case 6: v = T.resid[h]; break;
+                                    }
+                                }
+                            }
+                    }
+                }
+                base[w] = v;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// keep bits are OR-ed in; clear them first (a fill kernel, not a memset)
+__global__ void zero_u32(uint32_t* p, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = 0;
+}
+
+}  // namespace
+
+// ============================================================================
+// Host side: session and C ABI
+// ============================================================================
+struct or_session {
+    int device = 0;
+    or_params p;
+    float* d_wire = nullptr;
+    char* d_scratch = nullptr;
+    int64_t scratch_blocks = 0;
+    int grid = 0;
+    int maxh = 1024;
+    hipStream_t own = nullptr;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    float last_ms = 0.f;
+    // host staging (host-pointer calls)
+    void* d_in = nullptr;
+    size_t d_in_bytes = 0;
+    void* d_out = nullptr;
+    size_t d_out_bytes = 0;
+};
+
+#define OR_HIP(x)                                                          \
+    do {                                                                   \
+        hipError_t _e = (x);                                               \
+        if (_e != hipSuccess) {                                            \
+            std::fprintf(stderr, "libe1039or: %s failed: %s\n", #x, hipGetErrorString(_e)); \
+            return OR_ERR_DEVICE;                                          \
+        }                                                                  \
+    } while (0)
+
+namespace {
+
+template <int MAXH>
+int occupancy_blocks() {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, chain_kernel<MAXH>, WAVE, 0) != hipSuccess) nb = 1;
+    return nb < 1 ? 1 : nb;
+}
+
+template <int MAXH>
+hipError_t launch_chain(const Args& a, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(chain_kernel<MAXH>, dim3(grid), dim3(WAVE), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch(int maxh, const Args& a, int grid, hipStream_t s) {
+    switch (maxh) {
+        case 512: return launch_chain<512>(a, grid, s);
+        case 1024: return launch_chain<1024>(a, grid, s);
+        default: return launch_chain<2048>(a, grid, s);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+or_status or_session_create(const or_geometry* g, const or_params* p, int device, or_session** out) {
+    if (!g || !out) return OR_ERR_INVALID_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return OR_ERR_NO_DEVICE;
+    if (device < 0 || device >= ndev) return OR_ERR_INVALID_ARG;
+    or_params prm;
+    if (p) prm = *p;
+    else or_params_default(&prm);
+    if (prm.max_pairs_per_bin > BIN_CAP || prm.max_pairs_window > WIN_CAP ||
+        prm.max_tracklets_per_event > MAXTKL || prm.max_hits_per_event > 2048 ||
+        prm.max_hits_chamber_plane > PLANE_CAP || prm.max_hits_prop_plane > PLANE_CAP ||
+        prm.max_hits_hodo_plane > PLANE_CAP || prm.max_pair_plane_hits > 254 || prm.max_reduced_per_plane > 254 ||
+        prm.max_tracks_out < 0)
+        return OR_ERR_CAPACITY;
+    auto* s = new (std::nothrow) or_session;
+    if (!s) return OR_ERR_NOMEM;
+    s->device = device;
+    s->p = prm;
+    s->maxh = prm.max_hits_per_event <= 512 ? 512 : (prm.max_hits_per_event <= 1024 ? 1024 : 2048);
+    OR_HIP(hipSetDevice(device));
+    OR_HIP(hipStreamCreateWithFlags(&s->own, hipStreamNonBlocking));
+    s->stream = s->own;
+    OR_HIP(hipEventCreate(&s->ev0));
+    OR_HIP(hipEventCreate(&s->ev1));
+    OR_HIP(hipMemcpyToSymbol(HIP_SYMBOL(ordev::c_geo), g->f, sizeof g->f));
+    OR_HIP(hipMalloc(&s->d_wire, sizeof g->wire));
+    OR_HIP(hipMemcpy(s->d_wire, g->wire, sizeof g->wire, hipMemcpyHostToDevice));
+    hipDeviceProp_t prop;
+    OR_HIP(hipGetDeviceProperties(&prop, device));
+    int occ = s->maxh == 512 ? occupancy_blocks<512>() : (s->maxh == 1024 ? occupancy_blocks<1024>() : occupancy_blocks<2048>());
+    s->grid = prop.multiProcessorCount * occ;
+    s->scratch_blocks = s->grid;
+    OR_HIP(hipMalloc(&s->d_scratch, (size_t)SCR_STRIDE * s->scratch_blocks));
+    *out = s;
+    return OR_OK;
+}
+
+or_status or_session_set_stream(or_session* s, void* stream) {
+    if (!s) return OR_ERR_INVALID_ARG;
+    s->stream = stream ? (hipStream_t)stream : s->own;
+    return OR_OK;
+}
+
+or_status or_session_sync(or_session* s) {
+    if (!s) return OR_ERR_INVALID_ARG;
+    OR_HIP(hipStreamSynchronize(s->stream));
+    return OR_OK;
+}
+
+float or_session_last_kernel_ms(const or_session* s) { return s ? s->last_ms : 0.f; }
+
+static or_status run_device(or_session* s, const or_hits* in, const or_result* out, bool timed) {
+    Args a;
+    a.det = in->det; a.elem = in->elem; a.tdc = in->tdc; a.drift = in->drift; a.flag = in->flag;
+    a.off = in->event_offsets; a.n_events = in->n_events;
+    a.wire = s->d_wire;
+    a.p = s->p;
+    a.status = out->status; a.keep_mask = out->keep_mask; a.n_tracks = out->n_tracks; a.tracks = out->tracks;
+    a.n_tracklets = out->n_tracklets; a.records = out->records;
+    a.scratch = s->d_scratch;
+    a.scratch_stride = SCR_STRIDE;
+    const int64_t nw = (in->n_hits + 31) / 32;
+    if (nw > 0) {
+        hipLaunchKernelGGL(zero_u32, dim3(1024), dim3(256), 0, s->stream, out->keep_mask, nw);
+        OR_HIP(hipGetLastError());
+    }
+    if (in->n_events == 0) return OR_OK;
+    const int grid = (int)std::min<int64_t>(s->grid, in->n_events);
+    if (timed) OR_HIP(hipEventRecord(s->ev0, s->stream));
+    OR_HIP(launch(s->maxh, a, grid, s->stream));
+    if (timed) OR_HIP(hipEventRecord(s->ev1, s->stream));
+    return OR_OK;
+}
+
+or_status or_reconstruct(or_session* s, const or_hits* in, or_result* out) {
+    if (!s || !in || !out || in->n_events < 0 || in->n_hits < 0) return OR_ERR_INVALID_ARG;
+    if (!in->det || !in->elem || !in->tdc || !in->drift || !in->event_offsets) return OR_ERR_INVALID_ARG;
+    if (!out->status || !out->keep_mask || !out->n_tracks || (!out->tracks && s->p.max_tracks_out > 0))
+        return OR_ERR_INVALID_ARG;
+    OR_HIP(hipSetDevice(s->device));
+    if (in->on_device && out->on_device) {
+        or_status st = run_device(s, in, out, true);
+        if (st != OR_OK) return st;
+        // timing is read back lazily by or_session_last_kernel_ms after a sync
+        OR_HIP(hipEventSynchronize(s->ev1));
+        float ms = 0.f;
+        OR_HIP(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+        s->last_ms = ms;
+        return OR_OK;
+    }
+    if (in->on_device != out->on_device) return OR_ERR_INVALID_ARG;
+    // host buffers: stage through device memory
+    const int64_t ne = in->n_events, nh = in->n_hits;
+    const int64_t nw = (nh + 31) / 32;
+    const size_t in_bytes = (size_t)nh * (2 + 2 + 4 + 4 + 1) + (size_t)(ne + 1) * 8 + 256 * 8;
+    const size_t tk_bytes = (size_t)ne * s->p.max_tracks_out * sizeof(or_track);
+    const size_t rec_bytes = out->records ? (size_t)ne * s->p.max_tracklets_per_event * OR_TRACK_PARAMS * 4 : 0;
+    const size_t out_bytes = (size_t)ne * 4 * 3 + (size_t)nw * 4 + tk_bytes + rec_bytes + 256 * 8;
+    if (s->d_in_bytes < in_bytes) {
+        if (s->d_in) (void)hipFree(s->d_in);
+        s->d_in = nullptr;
+        OR_HIP(hipMalloc(&s->d_in, in_bytes));
+        s->d_in_bytes = in_bytes;
+    }
+    if (s->d_out_bytes < out_bytes) {
+        if (s->d_out) (void)hipFree(s->d_out);
+        s->d_out = nullptr;
+        OR_HIP(hipMalloc(&s->d_out, out_bytes));
+        s->d_out_bytes = out_bytes;
+    }
+    auto carve = [](char*& p, size_t bytes) { char* r = p; p += (bytes + 255) / 256 * 256; return r; };
+    char* pi = (char*)s->d_in;
+    or_hits din = *in;
+    din.det = (int16_t*)carve(pi, nh * 2);
+    din.elem = (int16_t*)carve(pi, nh * 2);
+    din.tdc = (float*)carve(pi, nh * 4);
+    din.drift = (float*)carve(pi, nh * 4);
+    din.flag = in->flag ? (uint8_t*)carve(pi, nh) : nullptr;
+    din.event_offsets = (int64_t*)carve(pi, (ne + 1) * 8);
+    din.on_device = 1;
+    hipStream_t st = s->stream;
+    OR_HIP(hipMemcpyAsync((void*)din.det, in->det, nh * 2, hipMemcpyHostToDevice, st));
+    OR_HIP(hipMemcpyAsync((void*)din.elem, in->elem, nh * 2, hipMemcpyHostToDevice, st));
+    OR_HIP(hipMemcpyAsync((void*)din.tdc, in->tdc, nh * 4, hipMemcpyHostToDevice, st));
+    OR_HIP(hipMemcpyAsync((void*)din.drift, in->drift, nh * 4, hipMemcpyHostToDevice, st));
+    if (in->flag) OR_HIP(hipMemcpyAsync((void*)din.flag, in->flag, nh, hipMemcpyHostToDevice, st));
+    OR_HIP(hipMemcpyAsync((void*)din.event_offsets, in->event_offsets, (ne + 1) * 8, hipMemcpyHostToDevice, st));
+    char* po = (char*)s->d_out;
+    or_result dout;
+    dout.status = (uint32_t*)carve(po, ne * 4);
+    dout.n_tracks = (int32_t*)carve(po, ne * 4);
+    dout.n_tracklets = out->n_tracklets ? (int32_t*)carve(po, ne * 4) : nullptr;
+    dout.keep_mask = (uint32_t*)carve(po, nw * 4);
+    dout.tracks = (or_track*)carve(po, tk_bytes);
+    dout.records = out->records ? (float*)carve(po, rec_bytes) : nullptr;
+    dout.on_device = 1;
+    or_status rs = run_device(s, &din, &dout, true);
+    if (rs != OR_OK) return rs;
+    OR_HIP(hipMemcpyAsync(out->status, dout.status, ne * 4, hipMemcpyDeviceToHost, st));
+    OR_HIP(hipMemcpyAsync(out->n_tracks, dout.n_tracks, ne * 4, hipMemcpyDeviceToHost, st));
+    if (out->n_tracklets) OR_HIP(hipMemcpyAsync(out->n_tracklets, dout.n_tracklets, ne * 4, hipMemcpyDeviceToHost, st));
+    if (nw) OR_HIP(hipMemcpyAsync(out->keep_mask, dout.keep_mask, nw * 4, hipMemcpyDeviceToHost, st));
+    if (tk_bytes) OR_HIP(hipMemcpyAsync(out->tracks, dout.tracks, tk_bytes, hipMemcpyDeviceToHost, st));
+    if (rec_bytes) OR_HIP(hipMemcpyAsync(out->records, dout.records, rec_bytes, hipMemcpyDeviceToHost, st));
+    OR_HIP(hipStreamSynchronize(st));
+    if (ne > 0) {
+        float ms = 0.f;
+        OR_HIP(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+        s->last_ms = ms;
+    }
+    return OR_OK;
+}
+
+void or_session_destroy(or_session* s) {
+    if (!s) return;
+    (void)hipSetDevice(s->device);
+    if (s->d_wire) (void)hipFree(s->d_wire);
+    if (s->d_scratch) (void)hipFree(s->d_scratch);
+    if (s->d_in) (void)hipFree(s->d_in);
+    if (s->d_out) (void)hipFree(s->d_out);
+    if (s->ev0) (void)hipEventDestroy(s->ev0);
+    if (s->ev1) (void)hipEventDestroy(s->ev1);
+    if (s->own) (void)hipStreamDestroy(s->own);
+    delete s;
+}
+}

@@ -1,0 +1,357 @@
+"""Python host side of the MI355X online-reconstruction chain.
+
+Thin ctypes layer over libe1039or.so (C ABI in include/e1039_or.h and
+include/e1039_or_gen.h).  It mirrors the reference's staging/launch sequence
+(OnlineReconstruction/online_reconstruction.cu:776-894) as a library call:
+
+    geo = Geometry.load(DEFAULT_GEOMETRY)
+    sess = Session(geo)                      # device 0, reference caps
+    ev = generate(geo, 10_000, seed=0xE1039)  # seeded synthetic events
+    res = sess.reconstruct(ev)               # status, keep bits, tracks
+
+Device buffers come from torch (plumbing only); all arithmetic runs in the
+HIP kernels of the library.  There is no CPU fallback: a missing library or
+device raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libe1039or.so")
+DEFAULT_GEOMETRY = os.path.join(HERE, "data", "basic_geometry_e906.txt")
+
+N_DETECTORS = 55
+MAX_ELEM = 404
+TRACK_PARAMS = 148
+MAX_HITS_PER_TRACK = 18
+
+# per-event status bits (include/e1039_or.h)
+EV_D0_MULT = 1 << 0
+EV_D2_MULT = 1 << 1
+EV_D3P_MULT = 1 << 2
+EV_D3M_MULT = 1 << 3
+EV_XZ_BIN_OVERFLOW = 1 << 4
+EV_TRACKLET_OVERFLOW = 1 << 5
+EV_PLANE_OVERFLOW = 1 << 6
+EV_PAIR_OVERFLOW = 1 << 7
+EV_CLUSTER_OVERFLOW = 1 << 8
+EV_EVENT_TOO_LARGE = 1 << 9
+EV_BAD_HIT = 1 << 10
+EV_OUTPUT_OVERFLOW = 1 << 11
+EV_REJECT_MASK = 0x7FF
+
+PARAM_FIELDS = [
+    "max_hits_chamber_plane", "max_hits_hodo_plane", "max_hits_prop_plane", "max_reduced_per_plane",
+    "max_pair_plane_hits", "max_pairs_per_bin", "max_pairs_window", "max_tracklets_per_slot",
+    "max_tracklets_per_event", "max_d0_mult", "max_d2_mult", "max_d3_mult", "max_tracks_out",
+    "max_hits_per_event",
+]
+
+
+class OrParams(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in PARAM_FIELDS]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n in PARAM_FIELDS}
+
+
+class OrHits(C.Structure):
+    _fields_ = [("n_events", C.c_int64), ("n_hits", C.c_int64), ("det", C.c_void_p), ("elem", C.c_void_p),
+                ("tdc", C.c_void_p), ("drift", C.c_void_p), ("flag", C.c_void_p), ("event_offsets", C.c_void_p),
+                ("on_device", C.c_int32)]
+
+
+class OrResult(C.Structure):
+    _fields_ = [("status", C.c_void_p), ("keep_mask", C.c_void_p), ("n_tracks", C.c_void_p), ("tracks", C.c_void_p),
+                ("n_tracklets", C.c_void_p), ("records", C.c_void_p), ("on_device", C.c_int32)]
+
+
+class OrGenParams(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("n_tracks_min", C.c_int32), ("n_tracks_max", C.c_int32),
+                ("tx_max", C.c_float), ("ty_max", C.c_float), ("invp_min", C.c_float), ("invp_max", C.c_float),
+                ("drift_sigma", C.c_float), ("noise_occupancy", C.c_float), ("afterpulse_frac", C.c_float),
+                ("mult_mode", C.c_int32), ("mult_mean", C.c_float), ("mult_mean2", C.c_float),
+                ("mult_frac2", C.c_float), ("mult_cap", C.c_int32)]
+
+
+class OrGenEvents(C.Structure):
+    _fields_ = [("n_events", C.c_int64), ("n_hits", C.c_int64), ("det", C.c_void_p), ("elem", C.c_void_p),
+                ("tdc", C.c_void_p), ("drift", C.c_void_p), ("flag", C.c_void_p), ("event_offsets", C.c_void_p),
+                ("n_true", C.c_void_p), ("truth", C.c_void_p)]
+
+
+TRACK_DTYPE = np.dtype([
+    ("x0", "<f4"), ("tx", "<f4"), ("y0", "<f4"), ("ty", "<f4"), ("invp", "<f4"),
+    ("err_x0", "<f4"), ("err_tx", "<f4"), ("err_y0", "<f4"), ("err_ty", "<f4"), ("err_invp", "<f4"),
+    ("chi2", "<f4"), ("hit_index", "<u2", (MAX_HITS_PER_TRACK,)), ("sign_plus", "<u4"), ("sign_minus", "<u4"),
+    ("station_id", "i1"), ("nhits", "u1"), ("charge", "i1"), ("slot", "u1"), ("tracklet_id", "<u4"),
+])
+assert TRACK_DTYPE.itemsize == 96
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Loads libe1039or.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it with __graft_entry__.build() (make -C e1039-gpu_amd/csrc)")
+    lib = C.CDLL(path)
+    vp, i32, i64 = C.c_void_p, C.c_int32, C.c_int64
+    lib.or_params_default.argtypes = [C.POINTER(OrParams)]
+    lib.or_geometry_load.argtypes = [C.c_char_p, C.POINTER(vp)]
+    lib.or_geometry_load.restype = i32
+    lib.or_geometry_free.argtypes = [vp]
+    lib.or_geometry_export.argtypes = [vp, vp, vp]
+    lib.or_geometry_export.restype = i32
+    lib.or_session_create.argtypes = [vp, C.POINTER(OrParams), C.c_int, C.POINTER(vp)]
+    lib.or_session_create.restype = i32
+    lib.or_reconstruct.argtypes = [vp, C.POINTER(OrHits), C.POINTER(OrResult)]
+    lib.or_reconstruct.restype = i32
+    lib.or_session_sync.argtypes = [vp]
+    lib.or_session_sync.restype = i32
+    lib.or_session_set_stream.argtypes = [vp, vp]
+    lib.or_session_set_stream.restype = i32
+    lib.or_session_last_kernel_ms.argtypes = [vp]
+    lib.or_session_last_kernel_ms.restype = C.c_float
+    lib.or_session_destroy.argtypes = [vp]
+    lib.or_status_string.argtypes = [i32]
+    lib.or_status_string.restype = C.c_char_p
+    lib.or_gen_params_default.argtypes = [C.POINTER(OrGenParams)]
+    lib.or_generate.argtypes = [vp, C.POINTER(OrGenParams), i64, C.c_int, C.POINTER(OrGenEvents)]
+    lib.or_generate.restype = i32
+    lib.or_gen_free.argtypes = [C.POINTER(OrGenEvents)]
+    _lib = lib
+    return lib
+
+
+class OrError(RuntimeError):
+    pass
+
+
+def _check(st: int):
+    if st != 0:
+        raise OrError(f"libe1039or: {load_library().or_status_string(st).decode()} (status {st})")
+
+
+def default_params(**overrides) -> OrParams:
+    p = OrParams()
+    load_library().or_params_default(C.byref(p))
+    for k, v in overrides.items():
+        setattr(p, k, int(v))
+    return p
+
+
+class Geometry:
+    """Parsed geometry + wire table (or_geometry_load)."""
+
+    def __init__(self, handle):
+        self.handle = handle
+
+    @classmethod
+    def load(cls, path: str = DEFAULT_GEOMETRY) -> "Geometry":
+        h = C.c_void_p()
+        _check(load_library().or_geometry_load(path.encode(), C.byref(h)))
+        return cls(h)
+
+    def export(self):
+        planes = np.zeros((27, N_DETECTORS), np.float32)
+        wire = np.zeros((N_DETECTORS, MAX_ELEM), np.float32)
+        _check(load_library().or_geometry_export(self.handle, planes.ctypes.data, wire.ctypes.data))
+        return planes, wire
+
+    def __del__(self):
+        if getattr(self, "handle", None) and _lib is not None:
+            _lib.or_geometry_free(self.handle)
+            self.handle = None
+
+
+@dataclass
+class Events:
+    """Hits in the boundary layout: SoA, events contiguous (or_hits)."""
+    det: np.ndarray        # int16
+    elem: np.ndarray       # int16
+    tdc: np.ndarray        # float32
+    drift: np.ndarray      # float32
+    flag: np.ndarray | None  # uint8 or None (all in time)
+    offsets: np.ndarray    # int64, n_events + 1
+    n_true: np.ndarray | None = None
+    truth: np.ndarray | None = None
+
+    @property
+    def n_events(self) -> int:
+        return len(self.offsets) - 1
+
+    @property
+    def n_hits(self) -> int:
+        return int(self.offsets[-1])
+
+    def subset(self, i0: int, i1: int) -> "Events":
+        h0, h1 = int(self.offsets[i0]), int(self.offsets[i1])
+        return Events(self.det[h0:h1].copy(), self.elem[h0:h1].copy(), self.tdc[h0:h1].copy(),
+                      self.drift[h0:h1].copy(), None if self.flag is None else self.flag[h0:h1].copy(),
+                      (self.offsets[i0:i1 + 1] - h0).astype(np.int64),
+                      None if self.n_true is None else self.n_true[i0:i1].copy(),
+                      None if self.truth is None else self.truth[i0:i1].copy())
+
+
+def gen_params(**kw) -> OrGenParams:
+    p = OrGenParams()
+    load_library().or_gen_params_default(C.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def generate(geo: Geometry, n_events: int, threads: int = 0, **kw) -> Events:
+    """Seeded synthetic events (include/e1039_or_gen.h)."""
+    lib = load_library()
+    p = gen_params(**kw)
+    out = OrGenEvents()
+    _check(lib.or_generate(geo.handle, C.byref(p), int(n_events), int(threads), C.byref(out)))
+    try:
+        nh, ne = out.n_hits, out.n_events
+
+        def arr(ptr, dtype, n):
+            if n == 0:
+                return np.zeros(0, dtype)
+            return np.ctypeslib.as_array(C.cast(ptr, C.POINTER(np.ctypeslib.as_ctypes_type(dtype))), (n,)).copy()
+
+        ev = Events(arr(out.det, np.int16, nh), arr(out.elem, np.int16, nh), arr(out.tdc, np.float32, nh),
+                    arr(out.drift, np.float32, nh), arr(out.flag, np.uint8, nh),
+                    arr(out.event_offsets, np.int64, ne + 1), arr(out.n_true, np.int8, ne),
+                    arr(out.truth, np.float32, ne * 4 * 6).reshape(ne, 4, 6))
+    finally:
+        lib.or_gen_free(C.byref(out))
+    return ev
+
+
+@dataclass
+class Result:
+    status: np.ndarray       # uint32 per event
+    keep_mask: np.ndarray    # uint32 words, bit h = hit h kept
+    n_tracks: np.ndarray     # int32 per event
+    tracks: np.ndarray       # TRACK_DTYPE, (n_events, max_tracks_out)
+    n_tracklets: np.ndarray | None = None
+    records: np.ndarray | None = None  # float32 (n_events, max_tracklets_per_event, 148)
+
+    def keep_bits(self, n_hits: int) -> np.ndarray:
+        b = np.unpackbits(self.keep_mask.view(np.uint8), bitorder="little")
+        return b[:n_hits].astype(bool)
+
+
+class Session:
+    """A device session (or_session_create) bound to torch's current stream."""
+
+    def __init__(self, geo: Geometry, params: OrParams | None = None, device: int = 0):
+        import torch
+        if not torch.cuda.is_available():
+            raise OrError("no GPU visible: the chain runs only on the device")
+        self.lib = load_library()
+        self.params = params if params is not None else default_params()
+        self.device = device
+        h = C.c_void_p()
+        _check(self.lib.or_session_create(geo.handle, C.byref(self.params), device, C.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.or_session_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        self.close()
+
+    def last_kernel_ms(self) -> float:
+        return float(self.lib.or_session_last_kernel_ms(self.handle))
+
+    # --- device-resident path -------------------------------------------------
+    def upload(self, ev: Events):
+        import torch
+        dev = torch.device("cuda", self.device)
+        t = {
+            "det": torch.from_numpy(ev.det).to(dev),
+            "elem": torch.from_numpy(ev.elem).to(dev),
+            "tdc": torch.from_numpy(ev.tdc).to(dev),
+            "drift": torch.from_numpy(ev.drift).to(dev),
+            "flag": None if ev.flag is None else torch.from_numpy(ev.flag).to(dev),
+            "offsets": torch.from_numpy(ev.offsets).to(dev),
+            "n_events": ev.n_events, "n_hits": ev.n_hits,
+        }
+        return t
+
+    def alloc_outputs(self, n_events: int, n_hits: int, tracklets: bool = False, records: bool = False):
+        import torch
+        dev = torch.device("cuda", self.device)
+        p = self.params
+        o = {
+            "status": torch.empty(n_events, dtype=torch.int32, device=dev),
+            "keep_mask": torch.empty(max(1, (n_hits + 31) // 32), dtype=torch.int32, device=dev),
+            "n_tracks": torch.empty(n_events, dtype=torch.int32, device=dev),
+            "tracks": torch.empty(max(1, n_events * p.max_tracks_out * 96), dtype=torch.uint8, device=dev),
+            "n_tracklets": torch.empty(n_events, dtype=torch.int32, device=dev) if tracklets else None,
+            "records": (torch.empty(n_events * p.max_tracklets_per_event * TRACK_PARAMS, dtype=torch.float32,
+                                    device=dev) if records else None),
+        }
+        return o
+
+    def run_device(self, t, o):
+        """Asynchronous-free device call: inputs and outputs already in HBM."""
+        import torch
+        stream = torch.cuda.current_stream(self.device)
+        _check(self.lib.or_session_set_stream(self.handle, C.c_void_p(stream.cuda_stream)))
+        hits = OrHits(t["n_events"], t["n_hits"], t["det"].data_ptr(), t["elem"].data_ptr(), t["tdc"].data_ptr(),
+                      t["drift"].data_ptr(), None if t["flag"] is None else t["flag"].data_ptr(),
+                      t["offsets"].data_ptr(), 1)
+        res = OrResult(o["status"].data_ptr(), o["keep_mask"].data_ptr(), o["n_tracks"].data_ptr(),
+                       o["tracks"].data_ptr(), None if o["n_tracklets"] is None else o["n_tracklets"].data_ptr(),
+                       None if o["records"] is None else o["records"].data_ptr(), 1)
+        _check(self.lib.or_reconstruct(self.handle, C.byref(hits), C.byref(res)))
+
+    @staticmethod
+    def fetch(o, n_events: int, max_tracks_out: int, max_tkl: int) -> Result:
+        tr = o["tracks"].cpu().numpy().view(TRACK_DTYPE)[: n_events * max_tracks_out].reshape(n_events, max_tracks_out)
+        return Result(
+            status=o["status"].cpu().numpy().view(np.uint32),
+            keep_mask=o["keep_mask"].cpu().numpy().view(np.uint32),
+            n_tracks=o["n_tracks"].cpu().numpy(),
+            tracks=tr,
+            n_tracklets=None if o["n_tracklets"] is None else o["n_tracklets"].cpu().numpy(),
+            records=(None if o["records"] is None else
+                     o["records"].cpu().numpy().reshape(n_events, max_tkl, TRACK_PARAMS)),
+        )
+
+    def reconstruct(self, ev: Events, tracklets: bool = True, records: bool = False) -> Result:
+        """Uploads, runs on the device, downloads."""
+        import torch
+        t = self.upload(ev)
+        o = self.alloc_outputs(ev.n_events, ev.n_hits, tracklets, records)
+        self.run_device(t, o)
+        torch.cuda.synchronize(self.device)
+        return self.fetch(o, ev.n_events, self.params.max_tracks_out, self.params.max_tracklets_per_event)
+
+    def reconstruct_host(self, ev: Events, records: bool = False) -> Result:
+        """Host buffers in and out; the library stages through device memory."""
+        p = self.params
+        ne, nh = ev.n_events, ev.n_hits
+        status = np.zeros(ne, np.uint32)
+        keep = np.zeros(max(1, (nh + 31) // 32), np.uint32)
+        ntr = np.zeros(ne, np.int32)
+        tracks = np.zeros((ne, p.max_tracks_out), TRACK_DTYPE)
+        ntkl = np.zeros(ne, np.int32)
+        rec = np.zeros((ne, p.max_tracklets_per_event, TRACK_PARAMS), np.float32) if records else None
+        cont = [np.ascontiguousarray(a) for a in (ev.det, ev.elem, ev.tdc, ev.drift, ev.offsets)]
+        flag = None if ev.flag is None else np.ascontiguousarray(ev.flag)
+        hits = OrHits(ne, nh, cont[0].ctypes.data, cont[1].ctypes.data, cont[2].ctypes.data, cont[3].ctypes.data,
+                      None if flag is None else flag.ctypes.data, cont[4].ctypes.data, 0)
+        res = OrResult(status.ctypes.data, keep.ctypes.data, ntr.ctypes.data, tracks.ctypes.data, ntkl.ctypes.data,
+                       None if rec is None else rec.ctypes.data, 0)
+        _check(self.lib.or_reconstruct(self.handle, C.byref(hits), C.byref(res)))
+        return Result(status, keep, ntr, tracks, ntkl, rec)

@@ -1,0 +1,178 @@
+/*
+ * e1039_or.h -- C ABI of the MI355X online-reconstruction chain.
+ *
+ * The chain: event reduction -> station-2/3 XZ tracklets -> YZ extension ->
+ * back-partial cleaning -> station-1 extension (global tracks).
+ *
+ * The reference (OnlineReconstruction/online_reconstruction.cu) has no library
+ * API: main() stages hits into device structs and launches kernels in sequence
+ * (online_reconstruction.cu:776-894).  The entry points below replace that
+ * sequence; each cites the reference code it stands in for.  All functions
+ * have C linkage, take plain pointers and sizes, and never throw.
+ *
+ * Threading: one or_session per host thread (or per stream).  Sessions are
+ * independent.  Outputs are in a canonical order independent of scheduling.
+ */
+#ifndef E1039_OR_H
+#define E1039_OR_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------- */
+typedef enum {
+    OR_OK = 0,
+    OR_ERR_INVALID_ARG = 1,
+    OR_ERR_IO = 2,            /* geometry file cannot be opened            */
+    OR_ERR_PARSE = 3,         /* geometry line with a detector id out of range */
+    OR_ERR_DEVICE = 4,        /* HIP runtime error                          */
+    OR_ERR_NOMEM = 5,
+    OR_ERR_NO_DEVICE = 6,     /* no GPU visible                             */
+    OR_ERR_CAPACITY = 7       /* a parameter exceeds a compiled bound       */
+} or_status;
+
+/* ---- per-event status bits (the reference's HasTooManyHits, split by cause)
+ * Any bit set means the event is rejected and produces no tracks, as
+ * HasTooManyHits does in the reference (online_reconstruction.cu:1067-1072). */
+enum {
+    OR_EV_D0_MULT        = 1u << 0,  /* station-1 multiplicity > cap (reconstruction_kernels.cuh:135) */
+    OR_EV_D2_MULT        = 1u << 1,  /* (reconstruction_kernels.cuh:136) */
+    OR_EV_D3P_MULT       = 1u << 2,  /* (reconstruction_kernels.cuh:137) */
+    OR_EV_D3M_MULT       = 1u << 3,  /* (reconstruction_kernels.cuh:138) */
+    OR_EV_XZ_BIN_OVERFLOW= 1u << 4,  /* > 20 X pairs in one bin (reconstruction_kernels.cuh:359-375) */
+    OR_EV_TRACKLET_OVERFLOW = 1u << 5, /* tracklet slots exhausted (reconstruction_kernels.cuh:719,824,835) */
+    OR_EV_PLANE_OVERFLOW = 1u << 6,  /* more hits in a plane than the staging arrays hold (reconstruction_constants.h:173-175) */
+    OR_EV_PAIR_OVERFLOW  = 1u << 7,  /* > 100 hits in a pair-builder plane or > 100 pairs in a window list (reconstruction_kernels.cuh:225,896) */
+    OR_EV_CLUSTER_OVERFLOW = 1u << 8,/* chamber cluster longer than 192 hits (reconstruction_helper.cuh:237) */
+    OR_EV_EVENT_TOO_LARGE= 1u << 9,  /* event has more hits than the session's per-event bound */
+    OR_EV_BAD_HIT        = 1u << 10, /* detectorID < 1 or elementID outside [1, nelem] */
+    OR_EV_OUTPUT_OVERFLOW= 1u << 11  /* more global tracks than max_tracks_out (tracks beyond it dropped, event kept) */
+};
+#define OR_EV_REJECT_MASK 0x7FFu
+
+/* ---- constants of the reference data model ----------------------------- */
+#define OR_N_DETECTORS 55          /* ids 0..54, 0 unused (reconstruction_constants.h:13) */
+#define OR_MAX_ELEM 404            /* wire table width (online_reconstruction.cu:191) */
+#define OR_TRACK_PARAMS 148        /* float record size (reconstruction_constants.h:171) */
+#define OR_MAX_HITS_PER_TRACK 18   /* (reconstruction_constants.h:182) */
+
+typedef struct or_geometry or_geometry;   /* opaque */
+typedef struct or_session or_session;     /* opaque */
+
+/* Runtime parameters: caps and cuts of the reference as values, so that
+ * high-occupancy workloads can lift them.  or_params_default() fills the
+ * reference values. */
+typedef struct {
+    int32_t max_hits_chamber_plane;   /* 200  datasizes::NMaxHitsChambers (reconstruction_constants.h:173) */
+    int32_t max_hits_hodo_plane;      /* 60   (reconstruction_constants.h:174) */
+    int32_t max_hits_prop_plane;      /* 120  (reconstruction_constants.h:175) */
+    int32_t max_reduced_per_plane;    /* 120  hitarraycopy[600]/5 (reconstruction_kernels.cuh:16) */
+    int32_t max_pair_plane_hits;      /* 100  hitflag[100] (reconstruction_kernels.cuh:225) */
+    int32_t max_pairs_per_bin;        /* 20   MaxHitsProj_X (reconstruction_constants.h:154) */
+    int32_t max_pairs_window;         /* 100  hitpairs_*[100] (reconstruction_kernels.cuh:896) */
+    int32_t max_tracklets_per_slot;   /* 16   TrackSizeMax/THREADS_PER_BLOCK */
+    int32_t max_tracklets_per_event;  /* 512  TrackSizeMax (reconstruction_constants.h:181) */
+    int32_t max_d0_mult;              /* 210  (reconstruction_constants.h:188) */
+    int32_t max_d2_mult;              /* 120  (reconstruction_constants.h:189) */
+    int32_t max_d3_mult;              /* 90   (reconstruction_constants.h:190) */
+    int32_t max_tracks_out;           /* output records per event (no reference counterpart) */
+    int32_t max_hits_per_event;       /* per-event bound the device tiles are sized for */
+} or_params;
+
+/* Input hits, structure of arrays, events contiguous.  Field meaning follows
+ * the reference's Hit class (LoadInput.h:10-30): detectorID and elementID are
+ * int16, tdcTime and driftDistance fp32.  driftDistance is used as |d|
+ * (online_reconstruction.cu:600).  flag: bit 1 set = in time
+ * (online_reconstruction.cu:599 stores 1<<is_in_time); NULL = all in time.
+ * Within an event the hit order is the reference's staging order: hits are
+ * grouped by detector keeping their relative order (online_reconstruction.cu:536-640). */
+typedef struct {
+    int64_t n_events;
+    int64_t n_hits;
+    const int16_t* det;
+    const int16_t* elem;
+    const float* tdc;
+    const float* drift;
+    const uint8_t* flag;            /* may be NULL */
+    const int64_t* event_offsets;   /* n_events+1 */
+    int32_t on_device;              /* 1: the pointers above are device pointers */
+} or_hits;
+
+/* Compact output track, 96 bytes.  Fields map to the reference record
+ * (reconstruction_classes.cuh:245-440) and OROutput branches (OROutput.cxx:34-60). */
+typedef struct {
+    float x0, tx, y0, ty, invp;
+    float err_x0, err_tx, err_y0, err_ty, err_invp;
+    float chi2;
+    uint16_t hit_index[OR_MAX_HITS_PER_TRACK]; /* event-local input index of each hit */
+    uint32_t sign_plus;      /* bit k: hit k has sign +1 */
+    uint32_t sign_minus;     /* bit k: hit k has sign -1 */
+    int8_t station_id;       /* 6 = global track */
+    uint8_t nhits;
+    int8_t charge;
+    uint8_t slot;            /* storage slot (reference thread array) */
+    uint32_t tracklet_id;    /* position in the canonical XZ tracklet list */
+} or_track;
+
+/* Output buffers, owned by the caller.  Optional ones may be NULL. */
+typedef struct {
+    uint32_t* status;        /* n_events: OR_EV_* bits */
+    uint32_t* keep_mask;     /* ceil(n_hits/32) words: bit h = input hit h survives the reducer */
+    int32_t* n_tracks;       /* n_events: global tracks written */
+    or_track* tracks;        /* n_events * max_tracks_out, slot order */
+    int32_t* n_tracklets;    /* optional, n_events: XZ tracklets found */
+    float* records;          /* optional, n_events * max_tracklets_per_event * 148 floats:
+                                every tracklet in slot order, reference record layout */
+    int32_t on_device;       /* 1: device pointers */
+} or_result;
+
+/* ---- entry points ---------------------------------------------------------- */
+
+/* Fills p with the reference caps and cuts. */
+void or_params_default(or_params* p);
+
+/* Parses a basic_geometry_*.txt file and builds the wire table.
+ * Replaces online_reconstruction.cu:115-218 (geometry parse into gPlane,
+ * slope_max/inter_max, wire_position[55][400]). */
+or_status or_geometry_load(const char* path, or_geometry** out);
+void or_geometry_free(or_geometry* g);
+
+/* Copies the 27 per-plane geometry arrays (reconstruction_classes.cuh:947-977,
+ * 27 x 55 floats, nelem as float) and the wire table (55 x OR_MAX_ELEM floats)
+ * into caller buffers; either pointer may be NULL. */
+or_status or_geometry_export(const or_geometry* g, float* planes_27x55, float* wire_table);
+
+/* Creates a device session: uploads geometry, sizes the per-wave scratch.
+ * Replaces the cudaMalloc/cudaMemcpy block (online_reconstruction.cu:652-757). */
+or_status or_session_create(const or_geometry* g, const or_params* p, int device, or_session** out);
+
+/* Runs the chain on a batch of events.  Replaces the kernel sequence
+ * gkernel_eR -> gKernel_XZ_tracking -> gKernel_YZ_tracking ->
+ * gKernel_BackTrackCleaning(_crossthreads) -> gKernel_Global_tracking
+ * (online_reconstruction.cu:776-894) and the unpack of reduced hits and
+ * tracks (online_reconstruction.cu:1063-1166).  With on_device inputs and
+ * outputs the call is asynchronous on the session stream; otherwise it stages
+ * through device buffers and returns when results are on the host. */
+or_status or_reconstruct(or_session* s, const or_hits* in, or_result* out);
+
+/* Waits for the session stream. */
+or_status or_session_sync(or_session* s);
+
+/* Sets the HIP stream (hipStream_t) work is issued on; NULL = session stream. */
+or_status or_session_set_stream(or_session* s, void* stream);
+
+/* Mean device time in ms of the last or_reconstruct's main kernel, measured
+ * with HIP events on the stream it ran on. */
+float or_session_last_kernel_ms(const or_session* s);
+
+void or_session_destroy(or_session* s);
+const char* or_status_string(or_status st);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,99 @@
+"""TEST INFRASTRUCTURE ONLY: ctypes access to the CPU oracle (liboracle.so).
+
+The oracle restates the reference chain on the host (or_oracle.cpp).  Only the
+tests, the smoke check and bench.py's cpu_baseline leg load it, as the checker
+or the timed CPU baseline; the product never does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        build()
+    lib = C.CDLL(LIB_PATH)
+    vp, i32, f32 = C.c_void_p, C.c_int32, C.c_float
+    lib.orc_geometry_load.argtypes = [C.c_char_p, C.POINTER(vp)]
+    lib.orc_geometry_load.restype = i32
+    lib.orc_geometry_free.argtypes = [vp]
+    lib.orc_geometry_export.argtypes = [vp, vp, vp, vp]
+    lib.orc_reconstruct.argtypes = [vp, vp, vp, vp, C.c_int]
+    lib.orc_reconstruct.restype = i32
+    lib.orc_fit2d.argtypes = [C.c_int, vp, vp, vp, vp]
+    lib.orc_fit2d_drift.argtypes = [C.c_int, vp, vp, vp, vp, vp, vp]
+    lib.orc_matinv2.argtypes = [vp, vp]
+    lib.orc_reduce_plane.argtypes = [C.c_int, C.c_int, vp, vp, vp, vp, vp, vp]
+    lib.orc_reduce_plane.restype = C.c_int
+    lib.orc_dca.argtypes = [f32] * 10
+    lib.orc_dca.restype = f32
+    lib.orc_y_uvhit.argtypes = [vp, C.c_int, C.c_int, f32, C.c_int, f32, f32, C.POINTER(f32), C.POINTER(f32)]
+    lib.orc_y_uvhit.restype = C.c_int
+    lib.orc_xz_fy.argtypes = [vp, C.c_int, C.c_int, f32, C.c_int, f32, f32, C.POINTER(f32), C.POINTER(f32)]
+    lib.orc_hodo_xz.argtypes = [vp, C.c_int, C.c_int, C.c_int, vp, f32, f32, f32, f32]
+    lib.orc_hodo_xz.restype = C.c_int
+    lib.orc_lr_xhits.argtypes = [vp, f32, f32, f32, f32, C.c_int, vp, vp, vp, vp]
+    lib.orc_pairs_binned.argtypes = [C.c_int, C.c_int, vp, vp, C.c_int, vp, vp, C.c_int, vp, vp]
+    _lib = lib
+    return lib
+
+
+class OracleGeometry:
+    def __init__(self, path: str):
+        self.lib = load()
+        h = C.c_void_p()
+        st = self.lib.orc_geometry_load(path.encode(), C.byref(h))
+        if st != 0:
+            raise RuntimeError(f"oracle geometry load failed ({st})")
+        self.handle = h
+
+    def export(self):
+        planes = np.zeros((27, 55), np.float32)
+        wire = np.zeros((55, 404), np.float32)
+        dw = np.zeros((55, 9), np.float32)
+        self.lib.orc_geometry_export(self.handle, planes.ctypes.data, wire.ctypes.data, dw.ctypes.data)
+        return planes, wire, dw
+
+    def __del__(self):
+        if getattr(self, "handle", None) and _lib is not None:
+            _lib.orc_geometry_free(self.handle)
+            self.handle = None
+
+
+def reconstruct(geo: OracleGeometry, params, ev, records: bool = False, threads: int = 0):
+    """Runs the oracle chain; returns the same Result layout as the product."""
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(HERE), "e1039-gpu_amd"))
+    import orchain as oc
+
+    ne, nh = ev.n_events, ev.n_hits
+    status = np.zeros(ne, np.uint32)
+    keep = np.zeros(max(1, (nh + 31) // 32), np.uint32)
+    ntr = np.zeros(ne, np.int32)
+    tracks = np.zeros((ne, params.max_tracks_out), oc.TRACK_DTYPE)
+    ntkl = np.zeros(ne, np.int32)
+    rec = np.zeros((ne, params.max_tracklets_per_event, oc.TRACK_PARAMS), np.float32) if records else None
+    arrs = [np.ascontiguousarray(a) for a in (ev.det, ev.elem, ev.tdc, ev.drift, ev.offsets)]
+    flag = None if ev.flag is None else np.ascontiguousarray(ev.flag)
+    hits = oc.OrHits(ne, nh, arrs[0].ctypes.data, arrs[1].ctypes.data, arrs[2].ctypes.data, arrs[3].ctypes.data,
+                     None if flag is None else flag.ctypes.data, arrs[4].ctypes.data, 0)
+    res = oc.OrResult(status.ctypes.data, keep.ctypes.data, ntr.ctypes.data, tracks.ctypes.data, ntkl.ctypes.data,
+                      None if rec is None else rec.ctypes.data, 0)
+    load().orc_reconstruct(geo.handle, C.byref(params), C.byref(hits), C.byref(res), int(threads))
+    return oc.Result(status, keep, ntr, tracks, ntkl, rec)

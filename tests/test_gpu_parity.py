@@ -1,0 +1,124 @@
+"""HIP chain against the CPU oracle, bit for bit, through the C ABI.
+
+Integer/index outputs (status, keep bits, tracklet counts, hit indices, signs)
+and every float of the tracklet records are compared as bit patterns: the
+device build and the oracle both round every float operation once
+(contraction off, IEEE division and square root)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def assert_same(got, ref, ev, what=""):
+    bad = np.nonzero(got.status != ref.status)[0]
+    assert bad.size == 0, f"{what} status differs at events {bad[:10]}: {got.status[bad[:5]]} vs {ref.status[bad[:5]]}"
+    assert np.array_equal(got.keep_mask, ref.keep_mask), f"{what} keep bits differ"
+    bad = np.nonzero(got.n_tracklets != ref.n_tracklets)[0]
+    assert bad.size == 0, f"{what} n_tracklets differ at {bad[:10]}"
+    if ref.records is not None:
+        g, r = got.records.view(np.uint32), ref.records.view(np.uint32)
+        diff = np.nonzero((g != r).any(axis=2))
+        assert diff[0].size == 0, (f"{what} records differ at (event, tracklet) {list(zip(diff[0][:5], diff[1][:5]))}; "
+                                   f"fields {np.nonzero(g[diff[0][0], diff[1][0]] != r[diff[0][0], diff[1][0]])[0] if diff[0].size else ''}")
+    bad = np.nonzero(got.n_tracks != ref.n_tracks)[0]
+    assert bad.size == 0, f"{what} n_tracks differ at {bad[:10]}"
+    assert got.tracks.tobytes() == ref.tracks.tobytes(), f"{what} track records differ"
+
+
+def run_both(oc, orc, session, ogeo, ev, records=True):
+    got = session.reconstruct(ev, records=records)
+    ref = orc.reconstruct(ogeo, session.params, ev, records=records)
+    return got, ref
+
+
+def test_c0_low_occupancy(oc, orc, geo, ogeo, session):
+    ev = oc.generate(geo, 3000, seed=0xE1039)
+    got, ref = run_both(oc, orc, session, ogeo, ev)
+    assert ref.n_tracks.sum() > 500
+    assert_same(got, ref, ev, "c0")
+
+
+def test_c1_poisson300(oc, orc, geo, ogeo, session):
+    ev = oc.generate(geo, 2000, seed=11, mult_mode=1, mult_mean=300.0, mult_cap=1000)
+    got, ref = run_both(oc, orc, session, ogeo, ev)
+    assert_same(got, ref, ev, "c1")
+
+
+def test_high_occupancy_reference_caps(oc, orc, geo, ogeo, session):
+    # most events breach the station multiplicity cuts; status bits must agree
+    ev = oc.generate(geo, 300, seed=12, mult_mode=1, mult_mean=1200.0, mult_cap=2000, noise_occupancy=0.08)
+    got, ref = run_both(oc, orc, session, ogeo, ev)
+    assert (ref.status != 0).mean() > 0.5
+    assert_same(got, ref, ev, "high occupancy")
+
+
+def test_lifted_multiplicity_caps(oc, orc, geo, ogeo):
+    params = oc.default_params(max_hits_per_event=2048, max_d0_mult=2000, max_d2_mult=2000, max_d3_mult=2000)
+    sess = oc.Session(geo, params, device=0)
+    ev = oc.generate(geo, 200, seed=13, mult_mode=1, mult_mean=700.0, mult_cap=2000)
+    got, ref = run_both(oc, orc, sess, ogeo, ev)
+    assert_same(got, ref, ev, "lifted caps")
+
+
+def test_edge_cases(oc, orc, geo, ogeo, session):
+    rng = np.random.default_rng(5)
+    base = oc.generate(geo, 40, seed=14)
+    evs = []
+    for e in range(base.n_events):
+        sub = base.subset(e, e + 1)
+        k = e % 8
+        det, elem, tdc, drift, flag = sub.det.copy(), sub.elem.copy(), sub.tdc.copy(), sub.drift.copy(), sub.flag.copy()
+        if k == 1:     # empty event
+            det, elem, tdc, drift, flag = det[:0], elem[:0], tdc[:0], drift[:0], flag[:0]
+        elif k == 2:   # dark-photon planes beyond 54 are skipped
+            det = np.concatenate([det, np.array([55, 60], np.int16)]); elem = np.concatenate([elem, np.array([1, 2], np.int16)])
+            tdc = np.concatenate([tdc, np.zeros(2, np.float32)]); drift = np.concatenate([drift, np.zeros(2, np.float32)])
+            flag = np.concatenate([flag, np.full(2, 2, np.uint8)])
+        elif k == 3:   # element outside the plane
+            elem[len(elem) // 2] = 999
+        elif k == 4:   # detector 0
+            det[0] = 0
+        elif k == 5:   # out-of-time hits
+            flag[rng.random(len(flag)) < 0.3] = 1
+        elif k == 6:   # negative drift values are used as |drift|
+            drift = -drift
+        elif k == 7:   # unsorted hits within the event
+            perm = rng.permutation(len(det)); det, elem, tdc, drift, flag = det[perm], elem[perm], tdc[perm], drift[perm], flag[perm]
+        evs.append((det, elem, tdc, drift, flag))
+    cat = lambda i: np.concatenate([x[i] for x in evs])
+    offs = np.concatenate([[0], np.cumsum([len(x[0]) for x in evs])]).astype(np.int64)
+    ev = oc.Events(cat(0), cat(1), cat(2), cat(3), cat(4), offs)
+    got, ref = run_both(oc, orc, session, ogeo, ev)
+    assert (ref.status & oc.EV_BAD_HIT).sum() >= 8
+    assert_same(got, ref, ev, "edge cases")
+
+
+def test_event_too_large(oc, orc, geo, ogeo):
+    params = oc.default_params(max_hits_per_event=512)
+    sess = oc.Session(geo, params, device=0)
+    ev = oc.generate(geo, 200, seed=15, mult_mode=2, mult_mean=300.0, mult_mean2=1500.0, mult_frac2=0.2, mult_cap=2000)
+    got, ref = run_both(oc, orc, sess, ogeo, ev)
+    assert (ref.status & oc.EV_EVENT_TOO_LARGE).sum() > 10
+    assert_same(got, ref, ev, "too large")
+
+
+def test_deterministic_and_batch_independent(oc, geo, session):
+    ev = oc.generate(geo, 4000, seed=16, mult_mode=1, mult_mean=300.0, mult_cap=1000)
+    a = session.reconstruct(ev)
+    b = session.reconstruct(ev)
+    assert a.tracks.tobytes() == b.tracks.tobytes() and np.array_equal(a.status, b.status)
+    # a sub-batch gives the same per-event results as the full batch
+    i0, i1 = 1234, 2345
+    s = session.reconstruct(ev.subset(i0, i1))
+    assert np.array_equal(s.status, a.status[i0:i1])
+    assert s.tracks.tobytes() == a.tracks[i0:i1].tobytes()
+    h0, h1 = int(ev.offsets[i0]), int(ev.offsets[i1])
+    assert np.array_equal(s.keep_bits(h1 - h0), a.keep_bits(ev.n_hits)[h0:h1])
+
+
+def test_host_buffer_path(oc, orc, geo, ogeo, session):
+    ev = oc.generate(geo, 500, seed=17)
+    got = session.reconstruct_host(ev, records=True)
+    ref = orc.reconstruct(ogeo, session.params, ev, records=True)
+    assert_same(got, ref, ev, "host path")
