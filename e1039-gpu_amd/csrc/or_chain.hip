@@ -57,17 +57,13 @@ template <int MAXH>
 struct Lds {
     int16_t elem[MAXH];
     float pos[MAXH];
-    float tdc[MAXH];
     float drift[MAXH];
-    uint16_t orig[MAXH];
+    uint16_t orig[MAXH];      // event-local input index
     uint16_t red[MAXH];       // reduced hits: raw indices, planes contiguous
-    int8_t keep[MAXH];
-    uint8_t intime[MAXH];
+    int8_t keep[MAXH];        // staged: 1 in time, -1 out of time; after reduction: 1 kept, -1 dropped
     uint8_t korig[MAXH];      // kept flag by input order
     int pcnt[56], pstart[56], cursor[56], rcnt[56], rstart[56];
-    uint16_t xpairs[3][28][BIN_CAP];   // (a | b << 8), 0xFF = none
     int xcnt[3][28];
-    uint16_t slot_order[MAXTKL];
     int nl[YZ_CHUNK][4];
     int uoff[YZ_CHUNK + 1];
     unsigned long long best[YZ_CHUNK];
@@ -83,6 +79,8 @@ struct Args {
     const uint8_t* flag;
     const int64_t* off;
     int64_t n_events;
+    const int32_t* ev_list;     // events of this launch's size class (NULL: all events in order)
+    const int32_t* ev_count;    // device count of ev_list
     const float* wire;          // 55 x OR_MAX_ELEM
     or_params p;
     uint32_t* status;
@@ -100,7 +98,13 @@ constexpr int64_t SCR_TK = 0;
 constexpr int64_t SCR_CELL = SCR_TK + (int64_t)sizeof(Tk) * MAXTKL;                 // uint32[1569]
 constexpr int64_t SCR_LIST = SCR_CELL + ((1569 * 4 + 255) / 256) * 256;             // uint16 lists
 constexpr int64_t SCR_LIST_BYTES = (int64_t)WAVE * 4 * WIN_CAP * 2;
-constexpr int64_t SCR_STRIDE = ((SCR_LIST + SCR_LIST_BYTES + 4095) / 4096) * 4096;
+constexpr int64_t SCR_YV = ((SCR_LIST + SCR_LIST_BYTES + 255) / 256) * 256;         // float4 per YZ list entry
+constexpr int64_t SCR_YV_BYTES = (int64_t)YZ_CHUNK * 4 * WIN_CAP * 16;
+constexpr int64_t SCR_YF = SCR_YV + SCR_YV_BYTES;                                   // uint8 per YZ list entry
+constexpr int64_t SCR_YF_BYTES = (int64_t)YZ_CHUNK * 4 * WIN_CAP;
+constexpr int64_t SCR_XP = ((SCR_YF + SCR_YF_BYTES + 255) / 256) * 256;           // uint16 [3][28][BIN_CAP] X pairs
+constexpr int64_t SCR_SO = SCR_XP + ((3 * 28 * BIN_CAP * 2 + 255) / 256) * 256;      // uint16 [MAXTKL] slot order
+constexpr int64_t SCR_STRIDE = ((SCR_SO + MAXTKL * 2 + 4095) / 4096) * 4096;
 
 __device__ __forceinline__ unsigned long long ballot(bool p) { return __ballot(p); }
 __device__ __forceinline__ int lane_id() { return threadIdx.x; }
@@ -125,11 +129,19 @@ __device__ __forceinline__ int wave_sum(int v) {
 
 // ---------------------------------------------------------------------------
 // Hit access helpers on the staged event
+// The event tile lives in a namespace-scope LDS variable so that non-inlined
+// stage functions keep LDS addressing (ds_* instructions) without a pointer.
+template <int MAXH>
+__shared__ Lds<MAXH> g_lds;
+
 template <int MAXH>
 struct Ev {
-    Lds<MAXH>& L;
-    __device__ int nred(int d) const { return L.rcnt[d]; }
-    __device__ int raw(int d, int k) const { return L.red[L.rstart[d] + k]; }
+    struct Ref {
+        __device__ __forceinline__ Lds<MAXH>* operator->() const { return &g_lds<MAXH>; }
+    };
+    static constexpr Ref Lp{};
+    __device__ __forceinline__ int nred(int d) const { return g_lds<MAXH>.rcnt[d]; }
+    __device__ __forceinline__ int raw(int d, int k) const { return g_lds<MAXH>.red[g_lds<MAXH>.rstart[d] + k]; }
 };
 
 // XZ bin of an element (WCHitsBins_X, reconstruction_constants.h:155-162):
@@ -153,7 +165,7 @@ __device__ bool hodo_xz(const Ev<MAXH>& E, int st, int d, float x0, float tx, fl
     const float f = hodo_fudge(st), cw = hodo_xcell(st);
     float ex = 3.f * (fabsf(G(F_Z, d) * etx) + ex0);
     for (int i = 0; i < n; ++i) {
-        const float p = E.L.pos[E.raw(d, i)];
+        const float p = E.Lp->pos[E.raw(d, i)];
         float xmin = p - cw * 0.5f, xmax = p + cw * 0.5f;
         xmin -= (xmax - xmin) * f;
         xmax += (xmax - xmin) * f;
@@ -177,7 +189,7 @@ __device__ bool hodo_tkl(const Ev<MAXH>& E, int st, int d, float x0, float y0, f
     float ex = 3.f * (fabsf(G(F_Z, d) * etx) + ex0);
     const float ey = 3.f * (fabsf(G(F_Z, d) * ety) + ey0);
     for (int i = 0; i < n; ++i) {
-        const float p = E.L.pos[E.raw(d, i)];
+        const float p = E.Lp->pos[E.raw(d, i)];
         float xmin = p - cw * 0.5f, xmax = p + cw * 0.5f;
         float ymin = G(F_Y1, d), ymax = G(F_Y2, d);
         xmin -= (xmax - xmin) * f;
@@ -199,17 +211,17 @@ template <int MAXH>
 __device__ __forceinline__ int prop_hit(const Ev<MAXH>& E, int da, int db, float slope, float icpt, float win) {
     const float za = G(F_Z, da);
     for (int m = 0; m < E.nred(da); ++m)
-        if (fabsf(E.L.pos[E.raw(da, m)] - (slope * za + icpt)) < win) return 1;
+        if (fabsf(E.Lp->pos[E.raw(da, m)] - (slope * za + icpt)) < win) return 1;
     const float zb = G(F_Z, db);
     for (int m = 0; m < E.nred(db); ++m)
-        if (fabsf(E.L.pos[E.raw(db, m)] - (slope * zb + icpt)) < win) return 1;
+        if (fabsf(E.Lp->pos[E.raw(db, m)] - (slope * zb + icpt)) < win) return 1;
     return 0;
 }
 
 // ---- window pair list (make_hitpairs_in_station, reconstruction_helper.cuh:464-549)
 // Returns the number of pairs, or -1 when a cap is exceeded.
 template <int MAXH>
-__device__ int pairs_window(const Ev<MAXH>& E, int st, int proj, float xmin, float xmax, uint16_t* out, int cap_plane,
+__device__ __attribute__((noinline)) int pairs_window(const Ev<MAXH>& E, int st, int proj, float xmin, float xmax, uint16_t* out, int cap_plane,
                             int cap_pairs) {
     const int sid = super_id(st, proj);
     const int d1 = sid * 2, d2 = sid * 2 - 1;
@@ -219,7 +231,7 @@ __device__ int pairs_window(const Ev<MAXH>& E, int st, int proj, float xmin, flo
     uint8_t w1[PLANE_CAP / 8 + 1], w2[PLANE_CAP / 8 + 1];
     int c1 = 0, c2 = 0;
     for (int i = 0; i < n1; ++i) {
-        const int e = E.L.elem[E.raw(d1, i)];
+        const int e = E.Lp->elem[E.raw(d1, i)];
         const float b = x_bep(d1, e), t = x_bep(d1, e) + dx1;
         const float a = dx1 > 0 ? b : t, c = dx1 > 0 ? t : b;
         const bool in = a <= xmax && c >= xmin;
@@ -227,7 +239,7 @@ __device__ int pairs_window(const Ev<MAXH>& E, int st, int proj, float xmin, flo
         if (in) { w1[i >> 3] |= (uint8_t)(1u << (i & 7)); ++c1; }
     }
     for (int i = 0; i < n2; ++i) {
-        const int e = E.L.elem[E.raw(d2, i)];
+        const int e = E.Lp->elem[E.raw(d2, i)];
         const float b = x_bep(d2, e), t = x_bep(d2, e) + dx2;
         const float a = dx2 > 0 ? b : t, c = dx2 > 0 ? t : b;
         const bool in = a <= xmax && c >= xmin;
@@ -242,10 +254,10 @@ __device__ int pairs_window(const Ev<MAXH>& E, int st, int proj, float xmin, flo
     int np = 0;
     for (int i = 0; i < n1; ++i) {
         if (!((w1[i >> 3] >> (i & 7)) & 1)) continue;
-        const float pi = E.L.pos[E.raw(d1, i)];
+        const float pi = E.Lp->pos[E.raw(d1, i)];
         for (int j = 0; j < n2; ++j) {
             if (!((w2[j >> 3] >> (j & 7)) & 1)) continue;
-            if (fabsf(pi - E.L.pos[E.raw(d2, j)]) > sp) continue;
+            if (fabsf(pi - E.Lp->pos[E.raw(d2, j)]) > sp) continue;
             if (np >= cap_pairs) return -1;
             out[np++] = (uint16_t)(i | (j << 8));
             f1[i >> 3] |= (uint8_t)(1u << (i & 7));
@@ -271,7 +283,7 @@ __device__ __forceinline__ int pb(uint16_t q) { int b = q >> 8; return b == 0xFF
 // ---------------------------------------------------------------------------
 // XZ candidate (reconstruction_kernels.cuh:465-672)
 template <int MAXH>
-__device__ bool xz_candidate(const Ev<MAXH>& E, int s3, uint16_t q2, uint16_t q3, Tk& T) {
+__device__ __attribute__((noinline)) bool xz_candidate(const Ev<MAXH>& E, int s3, uint16_t q2, uint16_t q3, Tk& T) {
     const int d3x = s3 == 1 ? 22 : 28, d3xp = d3x - 1;
     short det[4], sign[4];
     float X[4], Z[4], ex[4], drift[4];
@@ -281,10 +293,10 @@ __device__ bool xz_candidate(const Ev<MAXH>& E, int s3, uint16_t q2, uint16_t q3
         const int r = E.raw(d, k);
         det[n] = (short)d;
         ex[n] = G(F_RESOLUTION, d);
-        drift[n] = E.L.drift[r];
+        drift[n] = E.Lp->drift[r];
         sign[n] = 0;
         hr[n] = (uint16_t)r;
-        xz_fy(d, E.L.elem[r], drift[n], 0, 0.f, 0.f, X[n], Z[n]);
+        xz_fy(d, E.Lp->elem[r], drift[n], 0, 0.f, 0.f, X[n], Z[n]);
         ++n;
     };
     if (pa(q2) >= 0) add(16, pa(q2));
@@ -327,117 +339,170 @@ __device__ bool xz_candidate(const Ev<MAXH>& E, int s3, uint16_t q2, uint16_t q3
 
 // ---------------------------------------------------------------------------
 // YZ unit: one tracklet, one (u3, v3) pair, inner loop over (u2, v2) pairs in
-// reference order (reconstruction_kernels.cuh:1320-1800).  Station-3 signs set
-// by the left/right step persist across the inner loop as in the reference.
-// search: returns best (chi2, i_uv2) of the unit.  replay (stop >= 0): fills T
-// with the candidate at i_uv2 == stop.
-struct YzLists { const uint16_t* l[4]; int n[4]; };
+// reference order (reconstruction_kernels.cuh:1320-1800).
+//  * The y of every UV hit depends only on the hit and the tracklet's (x0, tx)
+//    (calculate_y_uvhit with sign 0), so it is computed once per window-list
+//    entry (stage E) and read here.
+//  * The fit sums run over hits in the order u3, u3', v3, v3', u2, u2', v2, v2';
+//    the station-3 part is summed once per unit and continued per inner pair,
+//    which is the same sequence of float additions.
+//  * The refit after left/right (reconstruction_kernels.cuh:1628) has the same
+//    inputs as the first fit and is not repeated.
+//  * Station-3 signs set by the left/right step persist across the inner loop,
+//    as in the reference.
+// search: returns the unit's best (chi2, i_uv2).  replay (stop >= 0): writes
+// the candidate at i_uv2 == stop into the tracklet.
+struct YzLists {
+    const uint16_t* l[4];   // pair entries (a | b << 8)
+    const float4* y[4];     // (y_a, ey_a, y_b, ey_b)
+    const uint8_t* f[4];    // bit 0: hit a present and accepted, bit 1: hit b
+    int n[4];
+};
+
+// Hits of one candidate in the reference order u3, u3', v3, v3', u2, u2', v2, v2'.
+// Kept in the lane's private memory; the station-3 part and its signs persist
+// across the inner loop of a unit.
+struct YzHits {
+    short det[8], sign[8];
+    float pos[8], drift[8];
+    uint16_t hr[8];
+};
 
 template <int MAXH>
-__device__ bool yz_unit(const Ev<MAXH>& E, Tk& T, const YzLists& W, int iuv3, int stop, float& best_chi2,
-                        int& best_i) {
-    const float x0 = T.x0, tx = T.tx, ex0 = T.ex0, etx = T.etx;
-    const int nx = T.nx;
-    const int st3 = T.st3;
-    const int du[4][2] = {{super_id(2, 1) * 2, super_id(2, 1) * 2 - 1},
-                          {super_id(2, 2) * 2, super_id(2, 2) * 2 - 1},
-                          {super_id(st3, 1) * 2, super_id(st3, 1) * 2 - 1},
-                          {super_id(st3, 2) * 2, super_id(st3, 2) * 2 - 1}};
-    const int nu2 = W.n[0], nv2 = W.n[1], nu3 = W.n[2];
-    short det[8], sign[8];
-    float Y[8], eY[8], Zz[8], pos[8], drift[8];
-    uint16_t hr[8];
-    int nuv = 0;
-    auto add = [&](int d, int k) {
-        const int r = E.raw(d, k);
-        const int e = E.L.elem[r];
-        det[nuv] = (short)d;
-        Zz[nuv] = G(F_Z, d);
-        sign[nuv] = 0;
-        drift[nuv] = E.L.drift[r];
-        pos[nuv] = E.L.pos[r];
-        hr[nuv] = (uint16_t)r;
-        float y, ey;
-        if (y_uvhit(d, e, drift[nuv], 0, x0, tx, y, ey)) {
-            Y[nuv] = y;
-            eY[nuv] = ey;
-            nuv++;
+__device__ __forceinline__ void yz_put(const Ev<MAXH>& E, YzHits& H, int& n, const YzLists& W, int v, int k, int which,
+                                       int d) {
+    const uint16_t q = W.l[v][k];
+    const int idx = which ? pb(q) : pa(q);
+    if (idx < 0 || !((W.f[v][k] >> which) & 1)) return;
+    const int r = E.raw(d, idx);
+    H.det[n] = (short)d;
+    H.pos[n] = E.Lp->pos[r];
+    H.drift[n] = E.Lp->drift[r];
+    H.hr[n] = (uint16_t)r;
+    H.sign[n] = 0;
+    ++n;
+}
+
+// fit sums of one list entry's accepted hits, in order a then b
+__device__ __forceinline__ int yz_sum(const YzLists& W, int v, int k, float za, float zb, float& A0, float& A1,
+                                      float& A3, float& B0, float& B1) {
+    const uint8_t fl = W.f[v][k];
+    if (!fl) return 0;
+    const float4 yv = W.y[v][k];
+    int c = 0;
+    if (fl & 1) {
+        B0 += yv.y * yv.x; B1 += (yv.y * yv.x) * za; A0 += yv.y; A1 += yv.y * za; A3 += (yv.y * za) * za; ++c;
+    }
+    if (fl & 2) {
+        B0 += yv.w * yv.z; B1 += (yv.w * yv.z) * zb; A0 += yv.w; A1 += yv.w * zb; A3 += (yv.w * zb) * zb; ++c;
+    }
+    return c;
+}
+
+// The part of a YZ candidate after the first box cut: left/right on all UV hits
+// (station-3 signs in H persist), second box cut, prop-tube Y and hodoscope
+// matches, chi2.  Returns false if the candidate is rejected.  With Tw set the
+// candidate is written into the tracklet.
+template <int MAXH>
+__device__ __attribute__((noinline)) bool yz_tail(const Ev<MAXH>& E, const Tk* Tp, const YzLists& W, YzHits& H,
+                                                 int n3, int iu2, int iv2, int su2, int sv2, float y0, float ty,
+                                                 float ey0, float ety, float& chi2_out, Tk* Tw) {
+    const float x0 = Tp->x0, tx = Tp->tx, ex0 = Tp->ex0, etx = Tp->etx;
+    const int nx = Tp->nx;
+    int n = n3;
+    yz_put(E, H, n, W, 0, iu2, 0, su2);
+    yz_put(E, H, n, W, 0, iu2, 1, su2 - 1);
+    yz_put(E, H, n, W, 1, iv2, 0, sv2);
+    yz_put(E, H, n, W, 1, iv2, 1, sv2 - 1);
+    lr_newhits(x0, tx, y0, ty, ex0, etx, ey0, ety, n, H.det, H.pos, H.drift, H.sign, 150.f);
+    lr_single_newhits(x0, tx, y0, ty, n, H.det, H.pos, H.sign);
+    if (fabsf(y0) > Y0_MAX || fabsf(ty) > TY_MAX) return false;
+    // prop Y: P1Y (48 then 47) within 5.08, P2Y (54 then 53) within 10.16
+    const int nprop = prop_hit(E, 48, 47, ty, y0, 5.08f) + prop_hit(E, 54, 53, ty, y0, 10.16f);
+    if (nprop < 2) return false;
+    if (!hodo_tkl(E, 1, 37, x0, y0, tx, ty, ex0, ey0, etx, ety) &&
+        !hodo_tkl(E, 1, 38, x0, y0, tx, ty, ex0, ey0, etx, ety))
+        return false;
+    if (!hodo_tkl(E, 2, 39, x0, y0, tx, ty, ex0, ey0, etx, ety) &&
+        !hodo_tkl(E, 2, 40, x0, y0, tx, ty, ex0, ey0, etx, ety))
+        return false;
+    if (!hodo_tkl(E, 3, 45, x0, y0, tx, ty, ex0, ey0, etx, ety) &&
+        !hodo_tkl(E, 3, 46, x0, y0, tx, ty, ex0, ey0, etx, ety))
+        return false;
+    // chi2 over x hits (stored signs) and UV hits (reconstruction_helper.cuh:1182-1204)
+    float chi2 = 0.f;
+    for (int k = 0; k < nx + n; ++k) {
+        int d, e;
+        float dr;
+        short sg;
+        if (k < nx) { d = Tp->det[k]; e = E.Lp->elem[Tp->hraw[k]]; dr = E.Lp->drift[Tp->hraw[k]]; sg = Tp->sign[k]; }
+        else { d = H.det[k - nx]; e = E.Lp->elem[H.hr[k - nx]]; dr = H.drift[k - nx]; sg = H.sign[k - nx]; }
+        const float res = G(F_RESOLUTION, d);
+        const float rs = dr * (float)sg - dca_wire(d, e, x0, y0, tx, ty);
+        chi2 += ((rs * rs) / res) / res;
+        if (Tw) Tw->resid[k] = rs;
+    }
+    if (Tw) {
+        Tw->stid = 5;
+        Tw->nuv = (uint8_t)n;
+        Tw->chi2 = chi2; Tw->y0 = y0; Tw->ty = ty; Tw->ey0 = ey0; Tw->ety = ety;
+        for (int k = 0; k < n; ++k) {
+            Tw->hraw[nx + k] = H.hr[k];
+            Tw->det[nx + k] = (uint8_t)H.det[k];
+            Tw->sign[nx + k] = (int8_t)H.sign[k];
         }
-    };
+    }
+    chi2_out = chi2;
+    return true;
+}
+
+// YZ unit: one tracklet, one (u3, v3) pair, inner loop over (u2, v2) pairs in
+// reference order (reconstruction_kernels.cuh:1320-1800).
+//  * The y of every UV hit depends only on the hit and the tracklet's (x0, tx)
+//    (calculate_y_uvhit with sign 0): computed once per window-list entry.
+//  * Fit sums run over hits in the order u3, u3', v3, v3', u2, u2', v2, v2'; the
+//    station-3 part is summed once and continued per inner pair (the same
+//    sequence of float additions).
+//  * The refit after left/right (reconstruction_kernels.cuh:1628) has the same
+//    inputs as the first fit and is not repeated.
+//  * Station-3 signs set by the left/right step persist across the inner loop.
+// search (stop < 0): returns the unit's best (chi2, i_uv2).  replay: writes the
+// candidate at i_uv2 == stop into the tracklet.
+template <int MAXH>
+__device__ __attribute__((noinline)) bool yz_unit(const Ev<MAXH>& E, Tk* Tp, const YzLists& W, int iuv3, int stop, float& best_chi2,
+                        int& best_i) {
+    const int st3 = Tp->st3;
+    const int su2 = super_id(2, 1) * 2, sv2 = super_id(2, 2) * 2;
+    const int su3 = super_id(st3, 1) * 2, sv3 = super_id(st3, 2) * 2;
+    const float zu2 = G(F_Z, su2), zu2p = G(F_Z, su2 - 1), zv2 = G(F_Z, sv2), zv2p = G(F_Z, sv2 - 1);
+    const int nu2 = W.n[0], nv2 = W.n[1], nu3 = W.n[2];
     const int iu3 = iuv3 % nu3, iv3 = iuv3 / nu3;
-    const uint16_t qu3 = W.l[2][iu3], qv3 = W.l[3][iv3];
-    if (pa(qu3) >= 0) add(du[2][0], pa(qu3));
-    if (pb(qu3) >= 0) add(du[2][1], pb(qu3));
-    const int nu3h = nuv;
+    float sA0 = 0.f, sA1 = 0.f, sA3 = 0.f, sB0 = 0.f, sB1 = 0.f;
+    const int nu3h = yz_sum(W, 2, iu3, G(F_Z, su3), G(F_Z, su3 - 1), sA0, sA1, sA3, sB0, sB1);
     if (nu3h == 0) return false;
-    if (pa(qv3) >= 0) add(du[3][0], pa(qv3));
-    if (pb(qv3) >= 0) add(du[3][1], pb(qv3));
-    const int nv3h = nuv - nu3h;
+    const int nv3h = yz_sum(W, 3, iv3, G(F_Z, sv3), G(F_Z, sv3 - 1), sA0, sA1, sA3, sB0, sB1);
     if (nv3h == 0) return false;
-    const int n3 = nuv;
-    const int nin = nu2 * nv2;
-    const int last = stop >= 0 ? stop : nin - 1;
+    YzHits H;
+    int n3 = 0;
+    yz_put(E, H, n3, W, 2, iu3, 0, su3);
+    yz_put(E, H, n3, W, 2, iu3, 1, su3 - 1);
+    yz_put(E, H, n3, W, 3, iv3, 0, sv3);
+    yz_put(E, H, n3, W, 3, iv3, 1, sv3 - 1);
+    const int last = stop >= 0 ? stop : nu2 * nv2 - 1;
     bool found = false;
     for (int iuv2 = 0; iuv2 <= last; ++iuv2) {
         const int iu2 = iuv2 % nu2, iv2 = iuv2 / nu2;
-        nuv = n3;
-        const uint16_t qu2 = W.l[0][iu2], qv2 = W.l[1][iv2];
-        if (pa(qu2) >= 0) add(du[0][0], pa(qu2));
-        if (pb(qu2) >= 0) add(du[0][1], pb(qu2));
-        const int nu2h = nuv - n3;
-        if (nu2h == 0) continue;
-        if (pa(qv2) >= 0) add(du[1][0], pa(qv2));
-        if (pb(qv2) >= 0) add(du[1][1], pb(qv2));
-        if (nuv - n3 - nu2h == 0) continue;
-        Fit f = fit2d<8>(nuv, Y, Zz, eY);
+        float A0 = sA0, A1 = sA1, A3 = sA3, B0 = sB0, B1 = sB1;
+        if (yz_sum(W, 0, iu2, zu2, zu2p, A0, A1, A3, B0, B1) == 0) continue;
+        if (yz_sum(W, 1, iv2, zv2, zv2p, A0, A1, A3, B0, B1) == 0) continue;
+        const Fit f = fit_finish(A0, A1, A3, B0, B1);
         if (fabsf(f.p0) > Y0_MAX + 2.f * f.e0 || fabsf(f.p1) > TY_MAX + 2.f * f.e1) continue;
-        lr_newhits(x0, tx, f.p0, f.p1, ex0, etx, f.e0, f.e1, nuv, det, pos, drift, sign, 150.f);
-        lr_single_newhits(x0, tx, f.p0, f.p1, nuv, det, pos, sign);
-        f = fit2d<8>(nuv, Y, Zz, eY);  // identical refit (reconstruction_kernels.cuh:1628)
-        const float y0 = f.p0, ty = f.p1, ey0 = f.e0, ety = f.e1;
-        if (fabsf(y0) > Y0_MAX || fabsf(ty) > TY_MAX) continue;
-        // prop Y: P1Y (48 then 47) within 5.08, P2Y (54 then 53) within 10.16
-        const int nprop = prop_hit(E, 48, 47, ty, y0, 5.08f) + prop_hit(E, 54, 53, ty, y0, 10.16f);
-        if (nprop < 2) continue;
-        if (!hodo_tkl(E, 1, 37, x0, y0, tx, ty, ex0, ey0, etx, ety) &&
-            !hodo_tkl(E, 1, 38, x0, y0, tx, ty, ex0, ey0, etx, ety))
+        float chi2;
+        const bool write = stop >= 0 && iuv2 == stop;
+        if (!yz_tail(E, Tp, W, H, n3, iu2, iv2, su2, sv2, f.p0, f.p1, f.e0, f.e1, chi2, write ? Tp : nullptr))
             continue;
-        if (!hodo_tkl(E, 2, 39, x0, y0, tx, ty, ex0, ey0, etx, ety) &&
-            !hodo_tkl(E, 2, 40, x0, y0, tx, ty, ex0, ey0, etx, ety))
-            continue;
-        if (!hodo_tkl(E, 3, 45, x0, y0, tx, ty, ex0, ey0, etx, ety) &&
-            !hodo_tkl(E, 3, 46, x0, y0, tx, ty, ex0, ey0, etx, ety))
-            continue;
-        // chi2 over x hits (stored signs) and UV hits (reconstruction_helper.cuh:1182-1204)
-        float chi2 = 0.f;
-        float rs[12];
-        for (int k = 0; k < nx + nuv; ++k) {
-            int d, e;
-            float dr;
-            short s;
-            if (k < nx) { d = T.det[k]; e = E.L.elem[T.hraw[k]]; dr = E.L.drift[T.hraw[k]]; s = T.sign[k]; }
-            else { d = det[k - nx]; e = E.L.elem[hr[k - nx]]; dr = drift[k - nx]; s = sign[k - nx]; }
-            const float res = G(F_RESOLUTION, d);
-            const float dca = dca_wire(d, e, x0, y0, tx, ty);
-            rs[k] = dr * (float)s - dca;
-            chi2 += ((rs[k] * rs[k]) / res) / res;
-        }
-        if (stop >= 0) {
-            if (iuv2 == stop) {
-                T.stid = 5;
-                T.nuv = (uint8_t)nuv;
-                T.chi2 = chi2; T.y0 = y0; T.ty = ty; T.ey0 = ey0; T.ety = ety;
-                for (int k = 0; k < nx + nuv; ++k) T.resid[k] = rs[k];
-                for (int k = 0; k < nuv; ++k) {
-                    T.hraw[nx + k] = hr[k];
-                    T.det[nx + k] = (uint8_t)det[k];
-                    T.sign[nx + k] = (int8_t)sign[k];
-                }
-                return true;
-            }
-            continue;
-        }
-        if (chi2 < best_chi2) { best_chi2 = chi2; best_i = iuv2; found = true; }
+        if (write) return true;
+        if (stop < 0 && chi2 < best_chi2) { best_chi2 = chi2; best_i = iuv2; found = true; }
     }
     return found;
 }
@@ -447,7 +512,7 @@ __device__ bool yz_unit(const Ev<MAXH>& E, Tk& T, const YzLists& W, int iuv3, in
 // Sequential over x pairs: y0/ty left by the last fit carry into the next x
 // pair's x-at-y computation, as in the reference.
 template <int MAXH>
-__device__ int global_track(const Ev<MAXH>& E, Tk& T, uint16_t* lists, const or_params& P) {
+__device__ __attribute__((noinline)) int global_track(const Ev<MAXH>& E, Tk& T, uint16_t* lists, const or_params& P) {
     const int n23 = T.nx + T.nuv;
     const float x0 = T.x0, tx = T.tx, ex0 = T.ex0, etx = T.etx;
     float y0 = T.y0, ty = T.ty, ey0 = T.ey0, ety = T.ety;
@@ -497,7 +562,7 @@ __device__ int global_track(const Ev<MAXH>& E, Tk& T, uint16_t* lists, const or_
         const int d = T.det[k];
         if ((double)G(F_COSTHETA, d) > 0.99) continue;
         float y, ey;
-        if (y_uvhit(d, E.L.elem[T.hraw[k]], E.L.drift[T.hraw[k]], T.sign[k], x0, tx, y, ey)) {
+        if (y_uvhit(d, E.Lp->elem[T.hraw[k]], E.Lp->drift[T.hraw[k]], T.sign[k], x0, tx, y, ey)) {
             Y[ny] = y; eY[ny] = ey; Zy[ny] = G(F_Z, d); ny++;
         }
     }
@@ -516,12 +581,12 @@ __device__ int global_track(const Ev<MAXH>& E, Tk& T, uint16_t* lists, const or_
         auto addx = [&](int d, int k) {
             const int r = E.raw(d, k);
             det[nx] = (short)d;
-            pos[nx] = E.L.pos[r];
+            pos[nx] = E.Lp->pos[r];
             eX[nx] = G(F_RESOLUTION, d);
             sign[nx] = 0;
-            drift[nx] = E.L.drift[r];
+            drift[nx] = E.Lp->drift[r];
             hr[nx] = (uint16_t)r;
-            xz_fy(d, E.L.elem[r], drift[nx], 0, y0, ty, X[nx], Z[nx]);
+            xz_fy(d, E.Lp->elem[r], drift[nx], 0, y0, ty, X[nx], Z[nx]);
             ++nx;
         };
         if (pa(lx[ix]) >= 0) addx(4, pa(lx[ix]));
@@ -551,11 +616,11 @@ __device__ int global_track(const Ev<MAXH>& E, Tk& T, uint16_t* lists, const or_
                 const int kk = nx + nuv;
                 det[kk] = (short)d;
                 sign[kk] = 0;
-                drift[kk] = E.L.drift[r];
-                pos[kk] = E.L.pos[r];
+                drift[kk] = E.Lp->drift[r];
+                pos[kk] = E.Lp->pos[r];
                 hr[kk] = (uint16_t)r;
                 float y, ey;
-                if (y_uvhit(d, E.L.elem[r], drift[kk], 0, x01, tx1, y, ey)) {
+                if (y_uvhit(d, E.Lp->elem[r], drift[kk], 0, x01, tx1, y, ey)) {
                     Y[ny + nuv] = y; eY[ny + nuv] = ey; Zy[ny + nuv] = G(F_Z, d);
                     nuv++;
                 }
@@ -580,10 +645,10 @@ __device__ int global_track(const Ev<MAXH>& E, Tk& T, uint16_t* lists, const or_
             float r23[12];
             for (int m = 0; m < n23; ++m) {
                 const int d = T.det[m];
-                const int e = E.L.elem[T.hraw[m]];
+                const int e = E.Lp->elem[T.hraw[m]];
                 const float res = G(F_RESOLUTION, d);
                 const float dca = dca_wire(d, e, x0, y0, tx, ty);
-                r23[m] = E.L.drift[T.hraw[m]] * (float)T.sign[m] - dca;
+                r23[m] = E.Lp->drift[T.hraw[m]] * (float)T.sign[m] - dca;
                 chi2 += (r23[m] * r23[m]) / (res * res);
             }
             float c1 = 0.f;
@@ -591,7 +656,7 @@ __device__ int global_track(const Ev<MAXH>& E, Tk& T, uint16_t* lists, const or_
             for (int k = 0; k < nx + nuv; ++k) {
                 const int d = det[k];
                 const float res = G(F_RESOLUTION, d);
-                const float dca = dca_wire(d, E.L.elem[hr[k]], x01, y0, tx1, ty);
+                const float dca = dca_wire(d, E.Lp->elem[hr[k]], x01, y0, tx1, ty);
                 r1[k] = drift[k] * (float)sign[k] - dca;
                 c1 += ((r1[k] * r1[k]) / res) / res;
             }
@@ -626,545 +691,658 @@ __device__ int global_track(const Ev<MAXH>& E, Tk& T, uint16_t* lists, const or_
 }
 
 // ---------------------------------------------------------------------------
-template <int MAXH>
-__global__ __launch_bounds__(WAVE) void chain_kernel(Args a) {
-    __shared__ Lds<MAXH> L;
-    const int lane = lane_id();
-    Ev<MAXH> E{L};
-    char* scr = a.scratch + (int64_t)blockIdx.x * a.scratch_stride;
-    Tk* tks = reinterpret_cast<Tk*>(scr + SCR_TK);
-    uint32_t* celloff = reinterpret_cast<uint32_t*>(scr + SCR_CELL);
-    uint16_t* wl = reinterpret_cast<uint16_t*>(scr + SCR_LIST);
-    const or_params& P = a.p;
+// Stages.  Each is a separate (non-inlined) function so that no single
+// function carries the live state of every stage; all lanes of the wave call
+// each stage, and stage-level branches depend only on wave-uniform values.
+#define STAGE __device__ __attribute__((noinline))
 
-    for (int64_t e = blockIdx.x; e < a.n_events; e += gridDim.x) {
+struct Scr {                  // per-wave scratch in HBM
+    Tk* tks;
+    uint32_t* celloff;
+    uint16_t* wl;
+    float4* wy;
+    uint8_t* wf;
+    uint16_t* xpairs;         // [3][28][BIN_CAP], (a | b << 8), 0xFF = none
+    uint16_t* slot_order;     // tracklet index by slot order
+};
+
+// A + B: load, group by detector, reduce.  Returns true when the keep bits are
+// valid (staging and reduction caps passed); station cuts set status bits.
+template <int MAXH>
+STAGE bool stage_load_reduce(const Args& a, int64_t h0, int n) {
+    Lds<MAXH>& L = g_lds<MAXH>;
+    const int lane = lane_id();
+    const or_params& P = a.p;
+    bool staged = false;
+    if (n > MAXH || n > P.max_hits_per_event) {
+        if (lane == 0) L.status = OR_EV_EVENT_TOO_LARGE;
+    } else {
+        // ---- A: count per detector, validate ------------------------------
+        for (int h = lane; h < n; h += WAVE) {
+            const int d = a.det[h0 + h], el = a.elem[h0 + h];
+            if (d > 54) continue;
+            if (d < 1 || el < 1 || el > (int)G(F_NELEM, d) || el >= OR_MAX_ELEM) atomicOr(&L.status, (uint32_t)OR_EV_BAD_HIT);
+            else atomicAdd(&L.pcnt[d], 1);
+        }
+        __syncthreads();
+        const bool badhit = L.status != 0;
+        {
+            const int d = lane;
+            int c = (d >= 1 && d <= 54) ? L.pcnt[d] : 0;
+            if (!badhit && d >= 1 && d <= 54) {
+                const int cap = d <= 30 ? P.max_hits_chamber_plane : (d <= 46 ? P.max_hits_hodo_plane : P.max_hits_prop_plane);
+                if (c > cap || c > PLANE_CAP) atomicOr(&L.status, (uint32_t)OR_EV_PLANE_OVERFLOW);
+            }
+            const int inc = wave_scan(c);
+            if (lane < 56) { L.pstart[lane] = inc - c; L.cursor[lane] = inc - c; }
+        }
+        __syncthreads();
+        if (L.status == 0) {
+            // stable scatter by detector
+            for (int base = 0; base < n; base += WAVE) {
+                const int h = base + lane;
+                int d = 0, el = 0;
+                bool v = h < n;
+                if (v) { d = a.det[h0 + h]; el = a.elem[h0 + h]; v = d >= 1 && d <= 54; }
+                unsigned long long todo = ballot(v);
+                int dest = 0;
+                while (todo) {
+                    const int leader = __ffsll((long long)todo) - 1;
+                    const int dl = __shfl(d, leader, WAVE);
+                    const unsigned long long m = ballot(v && d == dl);
+                    const int cur = L.cursor[dl];
+                    if (v && d == dl) dest = cur + rank_in(m);
+                    __syncthreads();
+                    if (lane == leader) L.cursor[dl] = cur + __popcll(m);
+                    __syncthreads();
+                    todo &= ~m;
+                }
+                if (v) {
+                    L.elem[dest] = (int16_t)el;
+                    L.pos[dest] = a.wire[d * OR_MAX_ELEM + el];
+                    L.drift[dest] = fabsf(a.drift[h0 + h]);
+                    L.orig[dest] = (uint16_t)h;
+                    const int fl = a.flag ? (int)a.flag[h0 + h] : 2;
+                    L.keep[dest] = (fl & 2) ? 1 : -1;
+                }
+            }
+            for (int h = lane; h < n; h += WAVE) L.korig[h] = 0;
+            __syncthreads();
+            // ---- B: reduction, one lane per plane (reconstruction_helper.cuh:168-361)
+            int nred = 0;
+            bool clover = false;
+            const int d = lane;
+            if (d >= 1 && d <= 54) {
+                const int s = L.pstart[d], np = L.pcnt[d];
+                const bool reduce = d <= 6 || d >= 13;
+                if (!reduce) {
+                    for (int i = 0; i < np; ++i) L.keep[s + i] = 1;
+                    nred = np;
+                } else {
+                    int ncl = 0, first = 0, lastm = 0;
+                    for (int i = 0; i < np; ++i) {
+                        const int r = s + i;
+                        if (L.keep[r] < 0) continue;  // out of time
+                        if (d > 30) continue;
+                        if (ncl == 0) { first = i; lastm = i; ncl = 1; continue; }
+                        const bool adj = fabsf((float)L.elem[r] - (float)L.elem[s + lastm]) <= 1.f;
+                        if (adj && i != np - 1) { if (ncl >= 192) clover = true; ++ncl; lastm = i; continue; }
+                        if (i == np - 1 && adj) { if (ncl >= 192) clover = true; ++ncl; lastm = i; }
+                        if (ncl == 2) {
+                            const int ra = s + first, rb = s + lastm;
+                            const float wmax = (float)(0.9 * 0.5 * (double)(L.pos[rb] - L.pos[ra]));
+                            const float wmin = (float)(4.0 / 9.0 * (double)wmax);
+                            const float da = L.drift[ra], db = L.drift[rb];
+                            if ((da > wmax && db > wmin) || (da > wmin && db > wmax)) {
+                                if (da > db) L.keep[ra] = -1;
+                                else L.keep[rb] = -1;
+                            } else {
+                                const float dt = a.tdc[h0 + L.orig[ra]] - a.tdc[h0 + L.orig[rb]];
+                                if (((dt >= 0.0f && dt < 8.0f) || (dt <= 0.0f && dt > -8.0f)) && d >= 19 && d <= 24) {
+                                    L.keep[ra] = -1;
+                                    L.keep[rb] = -1;
+                                }
+                            }
+                        }
+                        if (ncl >= 3) {
+                            float dtm = 0.0f;
+                            int prev = -1;
+                            for (int c = first; c <= lastm; ++c) {
+                                if (L.keep[s + c] < 0) continue;  // not a member (out of time)
+                                if (prev >= 0) {
+                                    const float tc = a.tdc[h0 + L.orig[s + c]], tp = a.tdc[h0 + L.orig[s + prev]];
+                                    const float dd = tc - tp;
+                                    dtm += dd > 0.0f ? dd : (tp - tc);
+                                }
+                                prev = c;
+                            }
+                            dtm = dtm / (float)(ncl - 1);
+                            const bool all = dtm < 10.0f;
+                            for (int c = first; c <= lastm; ++c) {
+                                if (L.keep[s + c] < 0) continue;
+                                if (all || c != first) L.keep[s + c] = -1;
+                            }
+                        }
+                        ncl = 0;
+                    }
+                    // first good hit per channel wins (reconstruction_helper.cuh:331-349)
+                    for (int i = 0; i < np; ++i) {
+                        const int r = s + i;
+                        if (L.keep[r] <= 0) continue;
+                        bool dup = false;
+                        for (int j = 0; j < i; ++j)
+                            if (L.keep[s + j] > 0 && L.elem[s + j] == L.elem[r]) { dup = true; break; }
+                        if (dup) L.keep[r] = -1;
+                        else ++nred;
+                    }
+                }
+                if (reduce && nred > P.max_reduced_per_plane) atomicOr(&L.status, (uint32_t)OR_EV_PLANE_OVERFLOW);
+                if (clover) atomicOr(&L.status, (uint32_t)OR_EV_CLUSTER_OVERFLOW);
+                L.rcnt[d] = nred;
+            } else if (lane < 56) {
+                L.rcnt[lane] = 0;
+            }
+            {
+                const int c = (d >= 1 && d <= 54) ? nred : 0;
+                const int inc = wave_scan(c);
+                if (lane < 56) L.rstart[lane] = inc - c;
+                // station multiplicities (reconstruction_kernels.cuh:127-139), 8 written rows summed
+                const int m0 = wave_sum((d >= 1 && d <= 6) ? nred : 0);
+                const int m2 = wave_sum((d >= 13 && d <= 18) ? nred : 0);
+                const int m3 = wave_sum((d >= 19 && d <= 24) ? nred : 0);
+                const int m4 = wave_sum((d >= 25 && d <= 30) ? nred : 0);
+                __syncthreads();
+                if (lane == 0 && L.status == 0) {
+                    uint32_t st = 0;
+                    if (m0 > P.max_d0_mult) st |= OR_EV_D0_MULT;
+                    if (m2 > P.max_d2_mult) st |= OR_EV_D2_MULT;
+                    if (m3 > P.max_d3_mult) st |= OR_EV_D3P_MULT;
+                    if (m4 > P.max_d3_mult) st |= OR_EV_D3M_MULT;
+                    L.status = st;
+                    staged = true;
+                }
+            }
+            __syncthreads();
+            // reduced lists, kept flags by input order
+            if (d >= 1 && d <= 54) {
+                const int s = L.pstart[d], np = L.pcnt[d];
+                int k = L.rstart[d];
+                for (int i = 0; i < np; ++i)
+                    if (L.keep[s + i] > 0) { L.red[k++] = (uint16_t)(s + i); L.korig[L.orig[s + i]] = 1; }
+            }
+            __syncthreads();
+        }
+    }
+    return __shfl(staged ? 1 : 0, 0, WAVE) != 0;
+}
+
+template <int MAXH>
+STAGE void stage_keep_bits(const Args& a, int64_t h0, int n) {
+    Lds<MAXH>& L = g_lds<MAXH>;
+    const int lane = lane_id();
+    const int64_t w0 = h0 >> 5, w1 = (h0 + n - 1) >> 5;
+        for (int64_t w = w0 + lane; n > 0 && w <= w1; w += WAVE) {
+            uint32_t bits = 0;
+            for (int b = 0; b < 32; ++b) {
+                const int64_t h = w * 32 + b - h0;
+                if (h >= 0 && h < n && L.korig[h]) bits |= 1u << b;
+            }
+            if (bits) atomicOr(&a.keep_mask[w], bits);
+        }
+    }
+
+// C: binned X pairs for station 2, D3p, D3m
+template <int MAXH>
+STAGE void stage_xpairs(const or_params& P, uint16_t* xpairs) {
+    Lds<MAXH>& L = g_lds<MAXH>;
+    Ev<MAXH> E;
+    const int lane = lane_id();
+    // hitflag1/2[100] bound (reconstruction_kernels.cuh:225-226) on all six planes first
+        const int pc = (lane < 6) ? L.rcnt[lane < 2 ? 16 - lane : (lane < 4 ? 24 - lane : 32 - lane)] : 0;
+        const bool over = ballot(lane < 6 && pc > P.max_pair_plane_hits) != 0ull;
+        if (over) {
+            if (lane == 0) L.status |= OR_EV_PAIR_OVERFLOW;
+        } else if (lane < 3) {
+            const int s = lane;
+            const int d1 = s == 0 ? 16 : (s == 1 ? 22 : 28), d2 = d1 - 1;
+            const int n1 = L.rcnt[d1], n2 = L.rcnt[d2];
+            for (int b = 0; b < 28; ++b) L.xcnt[s][b] = 0;
+            {
+                const float sp = spacing_plane(super_id(2 + s, 0));
+                const int cap = P.max_pairs_per_bin < BIN_CAP ? P.max_pairs_per_bin : BIN_CAP;
+                uint8_t f1[PLANE_CAP / 8 + 1], f2[PLANE_CAP / 8 + 1];
+                for (int i = 0; i < PLANE_CAP / 8 + 1; ++i) { f1[i] = 0; f2[i] = 0; }
+                auto put = [&](int c, int ia, int ib) {
+                    const int b = xbin(s, c);
+                    if (b < 0) return;
+                    const int k = L.xcnt[s][b];
+                    if (k < cap) xpairs[(s * 28 + b) * BIN_CAP + k] = (uint16_t)((ia & 0xFF) | ((ib & 0xFF) << 8));
+                    L.xcnt[s][b] = k + 1;
+                };
+                for (int i = 0; i < n1; ++i) {
+                    const int ri = E.raw(d1, i);
+                    for (int j = 0; j < n2; ++j) {
+                        const int rj = E.raw(d2, j);
+                        if (fabsf(L.pos[ri] - L.pos[rj]) > sp) continue;
+                        put(L.elem[ri], i, j);
+                        f1[i >> 3] |= (uint8_t)(1u << (i & 7));
+                        f2[j >> 3] |= (uint8_t)(1u << (j & 7));
+                    }
+                }
+                for (int i = 0; i < n1; ++i)
+                    if (!((f1[i >> 3] >> (i & 7)) & 1)) put(L.elem[E.raw(d1, i)], i, 0xFF);
+                for (int j = 0; j < n2; ++j)
+                    if (!((f2[j >> 3] >> (j & 7)) & 1)) put(L.elem[E.raw(d2, j)], 0xFF, j);
+                for (int b = 0; b < 28; ++b)
+                    if (L.xcnt[s][b] > P.max_pairs_per_bin) atomicOr(&L.status, (uint32_t)OR_EV_XZ_BIN_OVERFLOW);
+            }
+        }
+        __syncthreads();
+    }
+
+// D: XZ combinations in canonical order, slot order.  Returns the tracklet count.
+template <int MAXH>
+STAGE int stage_xz(const or_params& P, const Scr& S) {
+    Lds<MAXH>& L = g_lds<MAXH>;
+    Ev<MAXH> E;
+    const int lane = lane_id();
+    Tk* tks = S.tks;
+    uint32_t* celloff = S.celloff;
+    int ntkl = 0;
+    uint32_t run = 0;
+        for (int c0 = 0; c0 < 1568; c0 += WAVE) {
+            const int c = c0 + lane;
+            int cnt = 0;
+            if (c < 1568) {
+                const int t = c / 49, i = c % 49;
+                const int s3 = 1 + (t & 1);
+                const int b2 = (t / 8) * 7 + i % 7, b3 = ((t % 8) / 2) * 7 + i / 7;
+                cnt = L.xcnt[0][b2] * L.xcnt[s3][b3];
+            }
+            const int inc = wave_scan(cnt);
+            if (c < 1568) celloff[c + 1] = run + (uint32_t)inc;
+            if (c == 0) celloff[0] = 0;
+            run += (uint32_t)__shfl(inc, WAVE - 1, WAVE);
+        }
+        __syncthreads();
+        const uint32_t total = run;
+        int np = 0, nm = 0;
+        for (uint32_t g0 = 0; g0 < total; g0 += WAVE) {
+            const uint32_t g = g0 + lane;
+            bool acc = false;
+            Tk T;
+            int par = 0;
+            if (g < total) {
+                int lo = 0, hi = 1568;  // largest c with celloff[c] <= g
+                while (hi - lo > 1) {
+                    const int mid = (lo + hi) >> 1;
+                    if (celloff[mid] <= g) lo = mid;
+                    else hi = mid;
+                }
+                const int c = lo;
+                const int t = c / 49, i = c % 49;
+                const int s3 = 1 + (t & 1);
+                const int b2 = (t / 8) * 7 + i % 7, b3 = ((t % 8) / 2) * 7 + i / 7;
+                const int nx2 = L.xcnt[0][b2];
+                const int ix = (int)(g - celloff[c]);
+                const int ix2 = ix % nx2, ix3 = ix / nx2;
+                par = t & 1;
+                acc = xz_candidate(E, s3, S.xpairs[b2 * BIN_CAP + ix2], S.xpairs[(s3 * 28 + b3) * BIN_CAP + ix3], T);
+                T.t = (uint8_t)t;
+                T.st3 = (uint8_t)(3 + par);
+            }
+            const unsigned long long m = ballot(acc);
+            const unsigned long long mp = ballot(acc && par == 0);
+            const unsigned long long mm = ballot(acc && par == 1);
+            if (acc) {
+                const int k = ntkl + rank_in(m);
+                const int kp = par == 0 ? np + rank_in(mp) : nm + rank_in(mm);
+                T.canon = (uint32_t)k;
+                T.slot = (uint8_t)(2 * (kp % 16) + par);
+                T.pos_in_slot = (uint8_t)(kp / 16 < 255 ? kp / 16 : 255);
+                if (k < MAXTKL) tks[k] = T;
+            }
+            ntkl += __popcll(m);
+            np += __popcll(mp);
+            nm += __popcll(mm);
+        }
+        if (lane == 0) {
+            if (np > 16 * P.max_tracklets_per_slot || nm > 16 * P.max_tracklets_per_slot ||
+                ntkl >= P.max_tracklets_per_event || ntkl > MAXTKL)
+                atomicOr(&L.status, (uint32_t)OR_EV_TRACKLET_OVERFLOW);
+        }
+        // slot order: slot s = 2q + p holds items kp = q, q+16, ...
+        {
+            const int p = lane & 1, q = lane >> 1;
+            const int npar = p ? nm : np;
+            const int cs = (lane < 32 && npar > q) ? (npar - q + 15) / 16 : 0;
+            const int inc = wave_scan(cs);
+            if (lane < 32) L.sstart[lane] = inc - cs;
+            if (lane == 32) L.sstart[32] = ntkl < MAXTKL ? ntkl : MAXTKL;
+        }
+        __syncthreads();
+        if (L.status == 0)
+            for (int k = lane; k < ntkl; k += WAVE) S.slot_order[L.sstart[tks[k].slot] + tks[k].pos_in_slot] = (uint16_t)k;
+        __syncthreads();
+        return ntkl;
+}
+
+// E: YZ extension of every tracklet
+template <int MAXH>
+STAGE void stage_yz(const or_params& P, const Scr& S, int ntkl) {
+    Lds<MAXH>& L = g_lds<MAXH>;
+    Ev<MAXH> E;
+    const int lane = lane_id();
+    Tk* tks = S.tks;
+    uint16_t* wl = S.wl;
+    float4* wy = S.wy;
+    uint8_t* wf = S.wf;
+    const int capw = P.max_pairs_window < WIN_CAP ? P.max_pairs_window : WIN_CAP;
+        for (int kb = 0; kb < ntkl; kb += YZ_CHUNK) {
+            // window lists: lane = (tracklet j, view v)
+            {
+                const int j = lane >> 2, v = lane & 3;
+                const int k = kb + j;
+                if (k < ntkl) {
+                    const Tk& T = tks[k];
+                    const int st = v < 2 ? 2 : T.st3, pj = (v & 1) ? 2 : 1;
+                    const int sid = super_id(st, pj);
+                    const int da = sid * 2, db = sid * 2 - 1;
+                    const float za = G(F_Z, da), zb = G(F_Z, db);
+                    const float xmin = fminf((T.tx - T.etx) * za, (T.tx - T.etx) * zb) + T.x0 - T.ex0 - G(F_SPACING, da);
+                    const float xmax = fmaxf((T.tx + T.etx) * za, (T.tx + T.etx) * zb) + T.x0 + T.ex0 + G(F_SPACING, da);
+                    uint16_t* lst = wl + (j * 4 + v) * WIN_CAP;
+                    const int nl = pairs_window(E, st, pj, xmin, xmax, lst, P.max_pair_plane_hits, capw);
+                    if (nl < 0) atomicOr(&L.status, (uint32_t)OR_EV_PAIR_OVERFLOW);
+                    L.nl[j][v] = nl < 0 ? 0 : nl;
+                    // y of each listed hit from the tracklet's x line (calculate_y_uvhit, sign 0)
+                    for (int q = 0; q < nl; ++q) {
+                        float4 yv = make_float4(0.f, 0.f, 0.f, 0.f);
+                        uint8_t fl = 0;
+                        const int ia = pa(lst[q]), ib = pb(lst[q]);
+                        if (ia >= 0) {
+                            const int r = E.raw(da, ia);
+                            if (y_uvhit(da, L.elem[r], L.drift[r], 0, T.x0, T.tx, yv.x, yv.y)) fl |= 1;
+                        }
+                        if (ib >= 0) {
+                            const int r = E.raw(db, ib);
+                            if (y_uvhit(db, L.elem[r], L.drift[r], 0, T.x0, T.tx, yv.z, yv.w)) fl |= 2;
+                        }
+                        wy[(j * 4 + v) * WIN_CAP + q] = yv;
+                        wf[(j * 4 + v) * WIN_CAP + q] = fl;
+                    }
+                }
+            }
+            __syncthreads();
+            if (L.status) break;
+            {
+                const int j = lane, k = kb + j;
+                int u = 0;
+                if (j < YZ_CHUNK && k < ntkl) {
+                    const int* nl = L.nl[j];
+                    if (nl[0] > 0 && nl[1] > 0 && nl[2] > 0 && nl[3] > 0) u = nl[2] * nl[3];
+                }
+                const int inc = wave_scan(u);
+                if (j < YZ_CHUNK) {
+                    L.best[j] = ~0ull;
+                    L.uoff[j + 1] = inc;
+                }
+                if (j == 0) L.uoff[0] = 0;
+            }
+            __syncthreads();
+            const int U = L.uoff[YZ_CHUNK];
+            for (int u0 = 0; u0 < U; u0 += WAVE) {
+                const int u = u0 + lane;
+                if (u < U) {
+                    int j = 0;
+                    while (L.uoff[j + 1] <= u) ++j;
+                    const int iuv3 = u - L.uoff[j];
+                    YzLists W;
+                    for (int v = 0; v < 4; ++v) {
+                        W.l[v] = wl + (j * 4 + v) * WIN_CAP; W.y[v] = wy + (j * 4 + v) * WIN_CAP;
+                        W.f[v] = wf + (j * 4 + v) * WIN_CAP; W.n[v] = L.nl[j][v];
+                    }
+                    float bc = 10000.1f;
+                    int bi = -1;
+                    if (yz_unit(E, &tks[kb + j], W, iuv3, -1, bc, bi)) {
+                        const unsigned long long key =
+                            ((unsigned long long)__float_as_uint(bc) << 32) |
+                            (unsigned long long)(uint32_t)(iuv3 * (W.n[0] * W.n[1]) + bi);
+                        atomicMin(&L.best[j], key);
+                    }
+                }
+            }
+            __syncthreads();
+            if (lane < YZ_CHUNK && kb + lane < ntkl) {
+                const int j = lane;
+                Tk* Tp = &tks[kb + j];
+                const unsigned long long key = L.best[j];
+                if (key != ~0ull) {
+                    YzLists W;
+                    for (int v = 0; v < 4; ++v) {
+                        W.l[v] = wl + (j * 4 + v) * WIN_CAP; W.y[v] = wy + (j * 4 + v) * WIN_CAP;
+                        W.f[v] = wf + (j * 4 + v) * WIN_CAP; W.n[v] = L.nl[j][v];
+                    }
+                    const int flat = (int)(uint32_t)key, nin = W.n[0] * W.n[1];
+                    float bc = 0.f;
+                    int bi = 0;
+                    yz_unit(E, Tp, W, flat / nin, flat % nin, bc, bi);
+                } else {
+                    Tp->stid = 3;
+                }
+            }
+            __syncthreads();
+        }
+    }
+
+// F: back-partial cleaning (reconstruction_helper.cuh:1789-1909), slot order
+template <int MAXH>
+STAGE void stage_clean(Tk* tks, const uint16_t* slot_order, int ntkl) {
+    Lds<MAXH>& L = g_lds<MAXH>;
+    const int lane = lane_id();
+    if (lane == 0 && ntkl > 0) {
+            auto shared_hits = [&](const Tk& A, int na, const Tk& B, int nb) {
+                int c = 0;
+                for (int i = 0; i < na; ++i)
+                    for (int j = 0; j < nb; ++j)
+                        if (B.det[j] == A.det[i] && B.hraw[j] == A.hraw[i]) { c++; break; }
+                return c;
+            };
+            auto pass = [&](int ib, int ie, int jb_self, int jb, int je) {
+                for (int ii = ib; ii < ie; ++ii) {
+                    Tk& A = tks[slot_order[ii]];
+                    if (A.stid < 5) continue;
+                    const int ni = A.nx + A.nuv;
+                    int common = 0;
+                    for (int jj = (jb_self ? ii + 1 : jb); jj < je; ++jj) {
+                        Tk& B = tks[slot_order[jj]];
+                        if (B.stid < 5) continue;
+                        const int nj = B.nx + B.nuv;
+                        common += shared_hits(A, ni, B, nj);
+                        if (A.chi2 / (float)(ni - 4) < B.chi2 / (float)(nj - 4)) {
+                            if ((float)common > (float)ni * 0.3333f) B.stid = 3;
+                        } else {
+                            if ((float)common > (float)nj * 0.3333f) A.stid = 3;
+                        }
+                    }
+                }
+            };
+            // within each slot, then slot against slot in slot order
+            for (int s = 0; s < 32; ++s) pass(L.sstart[s], L.sstart[s + 1], 1, 0, L.sstart[s + 1]);
+            for (int s = 0; s < 32; ++s)
+                for (int o = 0; o < 32; ++o)
+                    if (o != s) pass(L.sstart[s], L.sstart[s + 1], 0, L.sstart[o], L.sstart[o + 1]);
+        }
+        __syncthreads();
+    }
+
+// G: station-1 extension, one lane per track
+template <int MAXH>
+STAGE void stage_global(const or_params& P, const Scr& S, int ntkl) {
+    Lds<MAXH>& L = g_lds<MAXH>;
+    Ev<MAXH> E;
+    const int lane = lane_id();
+    Tk* tks = S.tks;
+    uint16_t* wl = S.wl;
+    for (int kb = 0; kb < ntkl; kb += WAVE) {
+            const int k = kb + lane;
+            if (k < ntkl) {
+                Tk T = tks[k];
+                if (T.stid >= 5) {
+                    const int r = global_track(E, T, wl + lane * 3 * WIN_CAP, P);
+                    if (r < 0) atomicOr(&L.status, (uint32_t)OR_EV_PAIR_OVERFLOW);
+                    tks[k] = T;
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+// H: outputs
+template <int MAXH>
+STAGE void stage_out(const Args& a, const Scr& S, int64_t e, int ntkl) {
+    Lds<MAXH>& L = g_lds<MAXH>;
+    const int lane = lane_id();
+    const or_params& P = a.p;
+    Tk* tks = S.tks;
+    const uint32_t st = L.status;
+    const bool rejected = (st & OR_EV_REJECT_MASK) != 0;
+    int nout = 0;
+    if (!rejected) {
+        for (int j0 = 0; j0 < ntkl; j0 += WAVE) {
+            const int j = j0 + lane;
+            const Tk* Tp = j < ntkl ? &tks[S.slot_order[j]] : nullptr;
+            const bool is6 = Tp && Tp->stid == 6;
+            const unsigned long long m = ballot(is6);
+            if (is6) {
+                const int o = nout + rank_in(m);
+                if (o < P.max_tracks_out) {
+                    const Tk& T = *Tp;
+                    or_track R;
+                    R.x0 = T.x0; R.tx = T.tx; R.y0 = T.y0; R.ty = T.ty; R.invp = T.invp;
+                    R.err_x0 = T.ex0; R.err_tx = T.etx; R.err_y0 = T.ey0; R.err_ty = T.ety; R.err_invp = T.einvp;
+                    R.chi2 = T.chi2;
+                    const int nh = T.nx + T.nuv + T.n1;
+                    uint32_t sp = 0, sm = 0;
+                    for (int h = 0; h < OR_MAX_HITS_PER_TRACK; ++h) {
+                        R.hit_index[h] = h < nh ? L.orig[T.hraw[h]] : 0;
+                        if (h < nh && T.sign[h] > 0) sp |= 1u << h;
+                        if (h < nh && T.sign[h] < 0) sm |= 1u << h;
+                    }
+                    R.sign_plus = sp; R.sign_minus = sm;
+                    R.station_id = T.stid; R.nhits = (uint8_t)nh; R.charge = T.charge; R.slot = T.slot;
+                    R.tracklet_id = T.canon;
+                    a.tracks[e * P.max_tracks_out + o] = R;
+                }
+            }
+            nout += __popcll(m);
+        }
+    }
+    if (lane == 0) {
+        uint32_t so = st;
+        if (!rejected && nout > P.max_tracks_out) so |= OR_EV_OUTPUT_OVERFLOW;
+        a.status[e] = so;
+        a.n_tracks[e] = rejected ? 0 : (nout < P.max_tracks_out ? nout : P.max_tracks_out);
+        if (a.n_tracklets) a.n_tracklets[e] = rejected ? 0 : ntkl;
+    }
+    if (a.records) {
+        float* base = a.records + (int64_t)e * P.max_tracklets_per_event * OR_TRACK_PARAMS;
+        const int nrec = rejected ? 0 : (ntkl < P.max_tracklets_per_event ? ntkl : P.max_tracklets_per_event);
+        for (int64_t w = lane; w < (int64_t)P.max_tracklets_per_event * OR_TRACK_PARAMS; w += WAVE) {
+            const int j = (int)(w / OR_TRACK_PARAMS), f = (int)(w % OR_TRACK_PARAMS);
+            float v = 0.f;
+            if (j < nrec) {
+                const Tk& T = tks[S.slot_order[j]];
+                const int nh = T.nx + T.nuv + T.n1;
+                switch (f) {
+                    case 0: v = (float)(T.stid == 0 ? 3 : T.stid); break;
+                    case 1: v = (float)T.t; break;
+                    case 2: v = (float)nh; break;
+                    case 3: v = T.chi2; break;
+                    case 5: v = T.tx; break;
+                    case 6: v = T.ty; break;
+                    case 7: v = T.x0; break;
+                    case 8: v = T.y0; break;
+                    case 9: v = T.invp; break;
+                    case 10: v = T.etx; break;
+                    case 11: v = T.ety; break;
+                    case 12: v = T.ex0; break;
+                    case 13: v = T.ey0; break;
+                    case 14: v = T.einvp; break;
+                    case 15: v = (float)T.charge; break;
+                    default:
+                        if (f >= 22 && f < 130 + 18) {
+                            const int blk = (f - 22) / 18, h = (f - 22) % 18;
+                            if (h < nh) {
+                                const int r = T.hraw[h];
+                                switch (blk) {
+                                    case 0: v = (float)T.det[h]; break;
+                                    case 1: v = (float)L.elem[r]; break;
+                                    case 2: v = h < T.nx ? T.xpos[h] : L.pos[r]; break;
+                                    case 3: v = L.drift[r]; break;
+                                    case 4: v = (float)T.sign[h]; break;
+                                    case 5: v = a.tdc[a.off[e] + L.orig[r]]; break;
+                                    case 6: v = T.resid[h]; break;
+                                }
+                            }
+                        }
+                }
+            }
+            base[w] = v;
+        }
+    }
+}
+
+template <int MAXH>
+__global__ __launch_bounds__(WAVE, MAXH <= 512 ? 4 : (MAXH <= 1024 ? 2 : 1)) void chain_kernel(Args a) {
+    Lds<MAXH>& L = g_lds<MAXH>;
+    const int lane = lane_id();
+    char* scr = a.scratch + (int64_t)blockIdx.x * a.scratch_stride;
+    Scr S;
+    S.tks = reinterpret_cast<Tk*>(scr + SCR_TK);
+    S.celloff = reinterpret_cast<uint32_t*>(scr + SCR_CELL);
+    S.wl = reinterpret_cast<uint16_t*>(scr + SCR_LIST);
+    S.wy = reinterpret_cast<float4*>(scr + SCR_YV);
+    S.wf = reinterpret_cast<uint8_t*>(scr + SCR_YF);
+    S.xpairs = reinterpret_cast<uint16_t*>(scr + SCR_XP);
+    S.slot_order = reinterpret_cast<uint16_t*>(scr + SCR_SO);
+    const int64_t ne = a.ev_list ? (int64_t)*a.ev_count : a.n_events;
+    for (int64_t i = blockIdx.x; i < ne; i += gridDim.x) {
+        const int64_t e = a.ev_list ? (int64_t)a.ev_list[i] : i;
         const int64_t h0 = a.off[e];
         const int n = (int)(a.off[e + 1] - h0);
         if (lane == 0) L.status = 0;
         if (lane < 56) L.pcnt[lane] = 0;
         __syncthreads();
+        const bool staged = stage_load_reduce<MAXH>(a, h0, n);
+        if (staged) stage_keep_bits<MAXH>(a, h0, n);
+        __syncthreads();
         int ntkl = 0;
-        bool staged = false;
-        if (n > MAXH || n > P.max_hits_per_event) {
-            if (lane == 0) L.status = OR_EV_EVENT_TOO_LARGE;
-        } else {
-            // ---- A: count per detector, validate ------------------------------
-            for (int h = lane; h < n; h += WAVE) {
-                const int d = a.det[h0 + h], el = a.elem[h0 + h];
-                if (d > 54) continue;
-                if (d < 1 || el < 1 || el > (int)G(F_NELEM, d) || el >= OR_MAX_ELEM) atomicOr(&L.status, (uint32_t)OR_EV_BAD_HIT);
-                else atomicAdd(&L.pcnt[d], 1);
-            }
-            __syncthreads();
-            const bool badhit = L.status != 0;
-            {
-                const int d = lane;
-                int c = (d >= 1 && d <= 54) ? L.pcnt[d] : 0;
-                if (!badhit && d >= 1 && d <= 54) {
-                    const int cap = d <= 30 ? P.max_hits_chamber_plane : (d <= 46 ? P.max_hits_hodo_plane : P.max_hits_prop_plane);
-                    if (c > cap || c > PLANE_CAP) atomicOr(&L.status, (uint32_t)OR_EV_PLANE_OVERFLOW);
-                }
-                const int inc = wave_scan(c);
-                if (lane < 56) { L.pstart[lane] = inc - c; L.cursor[lane] = inc - c; }
-            }
-            __syncthreads();
-            if (L.status == 0) {
-                // stable scatter by detector
-                for (int base = 0; base < n; base += WAVE) {
-                    const int h = base + lane;
-                    int d = 0, el = 0;
-                    bool v = h < n;
-                    if (v) { d = a.det[h0 + h]; el = a.elem[h0 + h]; v = d >= 1 && d <= 54; }
-                    unsigned long long todo = ballot(v);
-                    int dest = 0;
-                    while (todo) {
-                        const int leader = __ffsll((long long)todo) - 1;
-                        const int dl = __shfl(d, leader, WAVE);
-                        const unsigned long long m = ballot(v && d == dl);
-                        const int cur = L.cursor[dl];
-                        if (v && d == dl) dest = cur + rank_in(m);
-                        __syncthreads();
-                        if (lane == leader) L.cursor[dl] = cur + __popcll(m);
-                        __syncthreads();
-                        todo &= ~m;
-                    }
-                    if (v) {
-                        L.elem[dest] = (int16_t)el;
-                        L.pos[dest] = a.wire[d * OR_MAX_ELEM + el];
-                        L.tdc[dest] = a.tdc[h0 + h];
-                        L.drift[dest] = fabsf(a.drift[h0 + h]);
-                        L.orig[dest] = (uint16_t)h;
-                        const int fl = a.flag ? (int)a.flag[h0 + h] : 2;
-                        L.intime[dest] = (fl & 2) ? 1 : 0;
-                    }
-                }
-                for (int h = lane; h < n; h += WAVE) L.korig[h] = 0;
-                __syncthreads();
-                // ---- B: reduction, one lane per plane (reconstruction_helper.cuh:168-361)
-                int nred = 0;
-                bool clover = false;
-                const int d = lane;
-                if (d >= 1 && d <= 54) {
-                    const int s = L.pstart[d], np = L.pcnt[d];
-                    const bool reduce = d <= 6 || d >= 13;
-                    if (!reduce) {
-                        for (int i = 0; i < np; ++i) L.keep[s + i] = 1;
-                        nred = np;
-                    } else {
-                        int ncl = 0, first = 0, lastm = 0;
-                        for (int i = 0; i < np; ++i) {
-                            const int r = s + i;
-                            if (!L.intime[r]) { L.keep[r] = -1; continue; }
-                            L.keep[r] = 1;
-                            if (d > 30) continue;
-                            if (ncl == 0) { first = i; lastm = i; ncl = 1; continue; }
-                            const bool adj = fabsf((float)L.elem[r] - (float)L.elem[s + lastm]) <= 1.f;
-                            if (adj && i != np - 1) { if (ncl >= 192) clover = true; ++ncl; lastm = i; continue; }
-                            if (i == np - 1 && adj) { if (ncl >= 192) clover = true; ++ncl; lastm = i; }
-                            if (ncl == 2) {
-                                const int ra = s + first, rb = s + lastm;
-                                const float wmax = (float)(0.9 * 0.5 * (double)(L.pos[rb] - L.pos[ra]));
-                                const float wmin = (float)(4.0 / 9.0 * (double)wmax);
-                                const float da = L.drift[ra], db = L.drift[rb];
-                                if ((da > wmax && db > wmin) || (da > wmin && db > wmax)) {
-                                    if (da > db) L.keep[ra] = -1;
-                                    else L.keep[rb] = -1;
-                                } else {
-                                    const float dt = L.tdc[ra] - L.tdc[rb];
-                                    if (((dt >= 0.0f && dt < 8.0f) || (dt <= 0.0f && dt > -8.0f)) && d >= 19 && d <= 24) {
-                                        L.keep[ra] = -1;
-                                        L.keep[rb] = -1;
-                                    }
-                                }
-                            }
-                            if (ncl >= 3) {
-                                float dtm = 0.0f;
-                                int prev = -1;
-                                for (int c = first; c <= lastm; ++c) {
-                                    if (!L.intime[s + c]) continue;
-                                    if (prev >= 0) {
-                                        const float dd = L.tdc[s + c] - L.tdc[s + prev];
-                                        dtm += dd > 0.0f ? dd : (L.tdc[s + prev] - L.tdc[s + c]);
-                                    }
-                                    prev = c;
-                                }
-                                dtm = dtm / (float)(ncl - 1);
-                                const bool all = dtm < 10.0f;
-                                for (int c = first; c <= lastm; ++c) {
-                                    if (!L.intime[s + c]) continue;
-                                    if (all || c != first) L.keep[s + c] = -1;
-                                }
-                            }
-                            ncl = 0;
-                        }
-                        // first good hit per channel wins (reconstruction_helper.cuh:331-349)
-                        for (int i = 0; i < np; ++i) {
-                            const int r = s + i;
-                            if (L.keep[r] <= 0) continue;
-                            bool dup = false;
-                            for (int j = 0; j < i; ++j)
-                                if (L.keep[s + j] > 0 && L.elem[s + j] == L.elem[r]) { dup = true; break; }
-                            if (dup) L.keep[r] = -1;
-                            else ++nred;
-                        }
-                    }
-                    if (reduce && nred > P.max_reduced_per_plane) atomicOr(&L.status, (uint32_t)OR_EV_PLANE_OVERFLOW);
-                    if (clover) atomicOr(&L.status, (uint32_t)OR_EV_CLUSTER_OVERFLOW);
-                    L.rcnt[d] = nred;
-                } else if (lane < 56) {
-                    L.rcnt[lane] = 0;
-                }
-                {
-                    const int c = (d >= 1 && d <= 54) ? nred : 0;
-                    const int inc = wave_scan(c);
-                    if (lane < 56) L.rstart[lane] = inc - c;
-                    // station multiplicities (reconstruction_kernels.cuh:127-139), 8 written rows summed
-                    const int m0 = wave_sum((d >= 1 && d <= 6) ? nred : 0);
-                    const int m2 = wave_sum((d >= 13 && d <= 18) ? nred : 0);
-                    const int m3 = wave_sum((d >= 19 && d <= 24) ? nred : 0);
-                    const int m4 = wave_sum((d >= 25 && d <= 30) ? nred : 0);
-                    __syncthreads();
-                    if (lane == 0 && L.status == 0) {
-                        uint32_t st = 0;
-                        if (m0 > P.max_d0_mult) st |= OR_EV_D0_MULT;
-                        if (m2 > P.max_d2_mult) st |= OR_EV_D2_MULT;
-                        if (m3 > P.max_d3_mult) st |= OR_EV_D3P_MULT;
-                        if (m4 > P.max_d3_mult) st |= OR_EV_D3M_MULT;
-                        L.status = st;
-                        staged = true;
-                    }
-                }
-                __syncthreads();
-                // reduced lists, kept flags by input order
-                if (d >= 1 && d <= 54) {
-                    const int s = L.pstart[d], np = L.pcnt[d];
-                    int k = L.rstart[d];
-                    for (int i = 0; i < np; ++i)
-                        if (L.keep[s + i] > 0) { L.red[k++] = (uint16_t)(s + i); L.korig[L.orig[s + i]] = 1; }
-                }
-                __syncthreads();
-            }
-        }
-        staged = __shfl(staged ? 1 : 0, 0, WAVE) != 0;
-        // keep bits: written when staging and reduction passed their caps
-        if (staged) {
-            const int64_t w0 = h0 >> 5, w1 = (h0 + n - 1) >> 5;
-            for (int64_t w = w0 + lane; n > 0 && w <= w1; w += WAVE) {
-                uint32_t bits = 0;
-                for (int b = 0; b < 32; ++b) {
-                    const int64_t h = w * 32 + b - h0;
-                    if (h >= 0 && h < n && L.korig[h]) bits |= 1u << b;
-                }
-                if (bits) atomicOr(&a.keep_mask[w], bits);
-            }
-        }
+        if (L.status == 0) stage_xpairs<MAXH>(a.p, S.xpairs);
+        if (L.status == 0) ntkl = stage_xz<MAXH>(a.p, S);
+        if (L.status == 0) stage_yz<MAXH>(a.p, S, ntkl);
+        if (L.status == 0) stage_clean<MAXH>(S.tks, S.slot_order, ntkl);
+        if (L.status == 0) stage_global<MAXH>(a.p, S, ntkl);
+        stage_out<MAXH>(a, S, e, ntkl);
         __syncthreads();
+    }
+}
 
-        // ---- C: binned X pairs -------------------------------------------------
-        if (L.status == 0) {
-            // hitflag1/2[100] bound (reconstruction_kernels.cuh:225-226) on all six planes first
-            const int pc = (lane < 6) ? L.rcnt[lane < 2 ? 16 - lane : (lane < 4 ? 24 - lane : 32 - lane)] : 0;
-            const bool over = ballot(lane < 6 && pc > P.max_pair_plane_hits) != 0ull;
-            if (over) {
-                if (lane == 0) L.status |= OR_EV_PAIR_OVERFLOW;
-            } else if (lane < 3) {
-                const int s = lane;
-                const int d1 = s == 0 ? 16 : (s == 1 ? 22 : 28), d2 = d1 - 1;
-                const int n1 = L.rcnt[d1], n2 = L.rcnt[d2];
-                for (int b = 0; b < 28; ++b) L.xcnt[s][b] = 0;
-                {
-                    const float sp = spacing_plane(super_id(2 + s, 0));
-                    const int cap = P.max_pairs_per_bin < BIN_CAP ? P.max_pairs_per_bin : BIN_CAP;
-                    uint8_t f1[PLANE_CAP / 8 + 1], f2[PLANE_CAP / 8 + 1];
-                    for (int i = 0; i < PLANE_CAP / 8 + 1; ++i) { f1[i] = 0; f2[i] = 0; }
-                    auto put = [&](int c, int ia, int ib) {
-                        const int b = xbin(s, c);
-                        if (b < 0) return;
-                        const int k = L.xcnt[s][b];
-                        if (k < cap) L.xpairs[s][b][k] = (uint16_t)((ia & 0xFF) | ((ib & 0xFF) << 8));
-                        L.xcnt[s][b] = k + 1;
-                    };
-                    for (int i = 0; i < n1; ++i) {
-                        const int ri = E.raw(d1, i);
-                        for (int j = 0; j < n2; ++j) {
-                            const int rj = E.raw(d2, j);
-                            if (fabsf(L.pos[ri] - L.pos[rj]) > sp) continue;
-                            put(L.elem[ri], i, j);
-                            f1[i >> 3] |= (uint8_t)(1u << (i & 7));
-                            f2[j >> 3] |= (uint8_t)(1u << (j & 7));
-                        }
-                    }
-                    for (int i = 0; i < n1; ++i)
-                        if (!((f1[i >> 3] >> (i & 7)) & 1)) put(L.elem[E.raw(d1, i)], i, 0xFF);
-                    for (int j = 0; j < n2; ++j)
-                        if (!((f2[j >> 3] >> (j & 7)) & 1)) put(L.elem[E.raw(d2, j)], 0xFF, j);
-                    for (int b = 0; b < 28; ++b)
-                        if (L.xcnt[s][b] > P.max_pairs_per_bin) atomicOr(&L.status, (uint32_t)OR_EV_XZ_BIN_OVERFLOW);
-                }
-            }
-            __syncthreads();
-        }
+// Size classes: an event goes to the smallest LDS tile that holds its hits
+// (the largest class takes the rest and flags them as too large).  The order
+// inside a class list depends on scheduling; outputs are indexed by event, so
+// results do not.
+constexpr int NCLASS = 3;
+__host__ __device__ constexpr int class_maxh(int c) { return c == 0 ? 512 : (c == 1 ? 1024 : 2048); }
 
-        // ---- D: XZ combinations in canonical order --------------------------
-        if (L.status == 0) {
-            uint32_t run = 0;
-            for (int c0 = 0; c0 < 1568; c0 += WAVE) {
-                const int c = c0 + lane;
-                int cnt = 0;
-                if (c < 1568) {
-                    const int t = c / 49, i = c % 49;
-                    const int s3 = 1 + (t & 1);
-                    const int b2 = (t / 8) * 7 + i % 7, b3 = ((t % 8) / 2) * 7 + i / 7;
-                    cnt = L.xcnt[0][b2] * L.xcnt[s3][b3];
-                }
-                const int inc = wave_scan(cnt);
-                if (c < 1568) celloff[c + 1] = run + (uint32_t)inc;
-                if (c == 0) celloff[0] = 0;
-                run += (uint32_t)__shfl(inc, WAVE - 1, WAVE);
-            }
-            __syncthreads();
-            const uint32_t total = run;
-            int np = 0, nm = 0;
-            for (uint32_t g0 = 0; g0 < total; g0 += WAVE) {
-                const uint32_t g = g0 + lane;
-                bool acc = false;
-                Tk T;
-                int par = 0;
-                if (g < total) {
-                    int lo = 0, hi = 1568;  // largest c with celloff[c] <= g
-                    while (hi - lo > 1) {
-                        const int mid = (lo + hi) >> 1;
-                        if (celloff[mid] <= g) lo = mid;
-                        else hi = mid;
-                    }
-                    const int c = lo;
-                    const int t = c / 49, i = c % 49;
-                    const int s3 = 1 + (t & 1);
-                    const int b2 = (t / 8) * 7 + i % 7, b3 = ((t % 8) / 2) * 7 + i / 7;
-                    const int nx2 = L.xcnt[0][b2];
-                    const int ix = (int)(g - celloff[c]);
-                    const int ix2 = ix % nx2, ix3 = ix / nx2;
-                    par = t & 1;
-                    acc = xz_candidate(E, s3, L.xpairs[0][b2][ix2], L.xpairs[s3][b3][ix3], T);
-                    T.t = (uint8_t)t;
-                    T.st3 = (uint8_t)(3 + par);
-                }
-                const unsigned long long m = ballot(acc);
-                const unsigned long long mp = ballot(acc && par == 0);
-                const unsigned long long mm = ballot(acc && par == 1);
-                if (acc) {
-                    const int k = ntkl + rank_in(m);
-                    const int kp = par == 0 ? np + rank_in(mp) : nm + rank_in(mm);
-                    T.canon = (uint32_t)k;
-                    T.slot = (uint8_t)(2 * (kp % 16) + par);
-                    T.pos_in_slot = (uint8_t)(kp / 16 < 255 ? kp / 16 : 255);
-                    if (k < MAXTKL) tks[k] = T;
-                }
-                ntkl += __popcll(m);
-                np += __popcll(mp);
-                nm += __popcll(mm);
-            }
-            if (lane == 0) {
-                if (np > 16 * P.max_tracklets_per_slot || nm > 16 * P.max_tracklets_per_slot ||
-                    ntkl >= P.max_tracklets_per_event || ntkl > MAXTKL)
-                    atomicOr(&L.status, (uint32_t)OR_EV_TRACKLET_OVERFLOW);
-            }
-            // slot order: slot s = 2q + p holds items kp = q, q+16, ...
-            {
-                const int p = lane & 1, q = lane >> 1;
-                const int npar = p ? nm : np;
-                const int cs = (lane < 32 && npar > q) ? (npar - q + 15) / 16 : 0;
-                const int inc = wave_scan(cs);
-                if (lane < 32) L.sstart[lane] = inc - cs;
-                if (lane == 32) L.sstart[32] = ntkl < MAXTKL ? ntkl : MAXTKL;
-            }
-            __syncthreads();
-            if (L.status == 0)
-                for (int k = lane; k < ntkl; k += WAVE) L.slot_order[L.sstart[tks[k].slot] + tks[k].pos_in_slot] = (uint16_t)k;
-            __syncthreads();
-        }
-
-        // ---- E: YZ ------------------------------------------------------------
-        if (L.status == 0) {
-            const int capw = P.max_pairs_window < WIN_CAP ? P.max_pairs_window : WIN_CAP;
-            for (int kb = 0; kb < ntkl; kb += YZ_CHUNK) {
-                // window lists: lane = (tracklet j, view v)
-                {
-                    const int j = lane >> 2, v = lane & 3;
-                    const int k = kb + j;
-                    if (k < ntkl) {
-                        const Tk& T = tks[k];
-                        const int st = v < 2 ? 2 : T.st3, pj = (v & 1) ? 2 : 1;
-                        const int sid = super_id(st, pj);
-                        const int da = sid * 2, db = sid * 2 - 1;
-                        const float za = G(F_Z, da), zb = G(F_Z, db);
-                        const float xmin = fminf((T.tx - T.etx) * za, (T.tx - T.etx) * zb) + T.x0 - T.ex0 - G(F_SPACING, da);
-                        const float xmax = fmaxf((T.tx + T.etx) * za, (T.tx + T.etx) * zb) + T.x0 + T.ex0 + G(F_SPACING, da);
-                        const int nl = pairs_window(E, st, pj, xmin, xmax, wl + (j * 4 + v) * WIN_CAP, P.max_pair_plane_hits, capw);
-                        if (nl < 0) atomicOr(&L.status, (uint32_t)OR_EV_PAIR_OVERFLOW);
-                        L.nl[j][v] = nl < 0 ? 0 : nl;
-                    }
-                }
-                __syncthreads();
-                if (L.status) break;
-                {
-                    const int j = lane, k = kb + j;
-                    int u = 0;
-                    if (j < YZ_CHUNK && k < ntkl) {
-                        const int* nl = L.nl[j];
-                        if (nl[0] > 0 && nl[1] > 0 && nl[2] > 0 && nl[3] > 0) u = nl[2] * nl[3];
-                    }
-                    const int inc = wave_scan(u);
-                    if (j < YZ_CHUNK) {
-                        L.best[j] = ~0ull;
-                        L.uoff[j + 1] = inc;
-                    }
-                    if (j == 0) L.uoff[0] = 0;
-                }
-                __syncthreads();
-                const int U = L.uoff[YZ_CHUNK];
-                for (int u0 = 0; u0 < U; u0 += WAVE) {
-                    const int u = u0 + lane;
-                    if (u < U) {
-                        int j = 0;
-                        while (L.uoff[j + 1] <= u) ++j;
-                        const int iuv3 = u - L.uoff[j];
-                        Tk T = tks[kb + j];
-                        YzLists W;
-                        for (int v = 0; v < 4; ++v) { W.l[v] = wl + (j * 4 + v) * WIN_CAP; W.n[v] = L.nl[j][v]; }
-                        float bc = 10000.1f;
-                        int bi = -1;
-                        if (yz_unit(E, T, W, iuv3, -1, bc, bi)) {
-                            const unsigned long long key =
-                                ((unsigned long long)__float_as_uint(bc) << 32) |
-                                (unsigned long long)(uint32_t)(iuv3 * (W.n[0] * W.n[1]) + bi);
-                            atomicMin(&L.best[j], key);
-                        }
-                    }
-                }
-                __syncthreads();
-                if (lane < YZ_CHUNK && kb + lane < ntkl) {
-                    const int j = lane;
-                    Tk T = tks[kb + j];
-                    const unsigned long long key = L.best[j];
-                    if (key != ~0ull) {
-                        YzLists W;
-                        for (int v = 0; v < 4; ++v) { W.l[v] = wl + (j * 4 + v) * WIN_CAP; W.n[v] = L.nl[j][v]; }
-                        const int flat = (int)(uint32_t)key, nin = W.n[0] * W.n[1];
-                        float bc = 0.f;
-                        int bi = 0;
-                        yz_unit(E, T, W, flat / nin, flat % nin, bc, bi);
-                    } else {
-                        T.stid = 3;
-                    }
-                    tks[kb + j] = T;
-                }
-                __syncthreads();
-            }
-        }
-
-        // ---- F: back-partial cleaning (reconstruction_helper.cuh:1789-1909) ----
-        if (L.status == 0) {
-            if (lane == 0 && ntkl > 0) {
-                auto shared_hits = [&](const Tk& A, int na, const Tk& B, int nb) {
-                    int c = 0;
-                    for (int i = 0; i < na; ++i)
-                        for (int j = 0; j < nb; ++j)
-                            if (B.det[j] == A.det[i] && B.hraw[j] == A.hraw[i]) { c++; break; }
-                    return c;
-                };
-                auto pass = [&](int ib, int ie, int jb_self, int jb, int je) {
-                    for (int ii = ib; ii < ie; ++ii) {
-                        Tk& A = tks[L.slot_order[ii]];
-                        if (A.stid < 5) continue;
-                        const int ni = A.nx + A.nuv;
-                        int common = 0;
-                        for (int jj = (jb_self ? ii + 1 : jb); jj < je; ++jj) {
-                            Tk& B = tks[L.slot_order[jj]];
-                            if (B.stid < 5) continue;
-                            const int nj = B.nx + B.nuv;
-                            common += shared_hits(A, ni, B, nj);
-                            if (A.chi2 / (float)(ni - 4) < B.chi2 / (float)(nj - 4)) {
-                                if ((float)common > (float)ni * 0.3333f) B.stid = 3;
-                            } else {
-                                if ((float)common > (float)nj * 0.3333f) A.stid = 3;
-                            }
-                        }
-                    }
-                };
-                // within each slot, then slot against slot in slot order
-                for (int s = 0; s < 32; ++s) pass(L.sstart[s], L.sstart[s + 1], 1, 0, L.sstart[s + 1]);
-                for (int s = 0; s < 32; ++s)
-                    for (int o = 0; o < 32; ++o)
-                        if (o != s) pass(L.sstart[s], L.sstart[s + 1], 0, L.sstart[o], L.sstart[o + 1]);
-            }
-            __syncthreads();
-        }
-
-        // ---- G: station-1 extension ------------------------------------------
-        if (L.status == 0) {
-            for (int kb = 0; kb < ntkl; kb += WAVE) {
-                const int k = kb + lane;
-                if (k < ntkl) {
-                    Tk T = tks[k];
-                    if (T.stid >= 5) {
-                        const int r = global_track(E, T, wl + lane * 3 * WIN_CAP, P);
-                        if (r < 0) atomicOr(&L.status, (uint32_t)OR_EV_PAIR_OVERFLOW);
-                        tks[k] = T;
-                    }
-                }
-            }
-            __syncthreads();
-        }
-
-        // ---- H: outputs --------------------------------------------------------
-        const uint32_t st = L.status;
-        const bool rejected = (st & OR_EV_REJECT_MASK) != 0;
-        int nout = 0;
-        if (!rejected) {
-            for (int j0 = 0; j0 < ntkl; j0 += WAVE) {
-                const int j = j0 + lane;
-                const Tk* Tp = j < ntkl ? &tks[L.slot_order[j]] : nullptr;
-                const bool is6 = Tp && Tp->stid == 6;
-                const unsigned long long m = ballot(is6);
-                if (is6) {
-                    const int o = nout + rank_in(m);
-                    if (o < P.max_tracks_out) {
-                        const Tk& T = *Tp;
-                        or_track R;
-                        R.x0 = T.x0; R.tx = T.tx; R.y0 = T.y0; R.ty = T.ty; R.invp = T.invp;
-                        R.err_x0 = T.ex0; R.err_tx = T.etx; R.err_y0 = T.ey0; R.err_ty = T.ety; R.err_invp = T.einvp;
-                        R.chi2 = T.chi2;
-                        const int nh = T.nx + T.nuv + T.n1;
-                        uint32_t sp = 0, sm = 0;
-                        for (int h = 0; h < OR_MAX_HITS_PER_TRACK; ++h) {
-                            R.hit_index[h] = h < nh ? L.orig[T.hraw[h]] : 0;
-                            if (h < nh && T.sign[h] > 0) sp |= 1u << h;
-                            if (h < nh && T.sign[h] < 0) sm |= 1u << h;
-                        }
-                        R.sign_plus = sp; R.sign_minus = sm;
-                        R.station_id = T.stid; R.nhits = (uint8_t)nh; R.charge = T.charge; R.slot = T.slot;
-                        R.tracklet_id = T.canon;
-                        a.tracks[e * P.max_tracks_out + o] = R;
-                    }
-                }
-                nout += __popcll(m);
-            }
-        }
-        if (lane == 0) {
-            uint32_t so = st;
-            if (!rejected && nout > P.max_tracks_out) so |= OR_EV_OUTPUT_OVERFLOW;
-            a.status[e] = so;
-            a.n_tracks[e] = rejected ? 0 : (nout < P.max_tracks_out ? nout : P.max_tracks_out);
-            if (a.n_tracklets) a.n_tracklets[e] = rejected ? 0 : ntkl;
-        }
-        if (a.records) {
-            float* base = a.records + (int64_t)e * P.max_tracklets_per_event * OR_TRACK_PARAMS;
-            const int nrec = rejected ? 0 : (ntkl < P.max_tracklets_per_event ? ntkl : P.max_tracklets_per_event);
-            for (int64_t w = lane; w < (int64_t)P.max_tracklets_per_event * OR_TRACK_PARAMS; w += WAVE) {
-                const int j = (int)(w / OR_TRACK_PARAMS), f = (int)(w % OR_TRACK_PARAMS);
-                float v = 0.f;
-                if (j < nrec) {
-                    const Tk& T = tks[L.slot_order[j]];
-                    const int nh = T.nx + T.nuv + T.n1;
-                    switch (f) {
-                        case 0: v = (float)(T.stid == 0 ? 3 : T.stid); break;
-                        case 1: v = (float)T.t; break;
-                        case 2: v = (float)nh; break;
-                        case 3: v = T.chi2; break;
-                        case 5: v = T.tx; break;
-                        case 6: v = T.ty; break;
-                        case 7: v = T.x0; break;
-                        case 8: v = T.y0; break;
-                        case 9: v = T.invp; break;
-                        case 10: v = T.etx; break;
-                        case 11: v = T.ety; break;
-                        case 12: v = T.ex0; break;
-                        case 13: v = T.ey0; break;
-                        case 14: v = T.einvp; break;
-                        case 15: v = (float)T.charge; break;
-                        default:
-                            if (f >= 22 && f < 130 + 18) {
-                                const int blk = (f - 22) / 18, h = (f - 22) % 18;
-                                if (h < nh) {
-                                    const int r = T.hraw[h];
-                                    switch (blk) {
-                                        case 0: v = (float)T.det[h]; break;
-                                        case 1: v = (float)L.elem[r]; break;
-                                        case 2: v = h < T.nx ? T.xpos[h] : L.pos[r]; break;
-                                        case 3: v = L.drift[r]; break;
-                                        case 4: v = (float)T.sign[h]; break;
-                                        case 5: v = L.tdc[r]; break;
-                                        case 6: v = T.resid[h]; break;
-                                    }
-                                }
-                            }
-                    }
-                }
-                base[w] = v;
-            }
-        }
-        __syncthreads();
+__global__ void classify_events(const int64_t* off, int64_t n, int nclass, int32_t* lists, int64_t stride,
+                                int32_t* counts) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t h = off[e + 1] - off[e];
+        int c = 0;
+        while (c < nclass - 1 && h > class_maxh(c)) ++c;
+        const int k = atomicAdd(&counts[c], 1);
+        lists[c * stride + k] = (int32_t)e;
     }
 }
 
@@ -1186,6 +1364,11 @@ struct or_session {
     int64_t scratch_blocks = 0;
     int grid = 0;
     int maxh = 1024;
+    int nclass = 2;
+    int grid_c[3] = {0, 0, 0};
+    int32_t* d_lists = nullptr;   // NCLASS x list_cap event indices
+    int32_t* d_counts = nullptr;  // NCLASS counters
+    int64_t list_cap = 0;
     hipStream_t own = nullptr;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -1221,10 +1404,10 @@ hipError_t launch_chain(const Args& a, int grid, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch(int maxh, const Args& a, int grid, hipStream_t s) {
-    switch (maxh) {
-        case 512: return launch_chain<512>(a, grid, s);
-        case 1024: return launch_chain<1024>(a, grid, s);
+hipError_t launch_class(int c, const Args& a, int grid, hipStream_t s) {
+    switch (c) {
+        case 0: return launch_chain<512>(a, grid, s);
+        case 1: return launch_chain<1024>(a, grid, s);
         default: return launch_chain<2048>(a, grid, s);
     }
 }
@@ -1262,9 +1445,15 @@ or_status or_session_create(const or_geometry* g, const or_params* p, int device
     OR_HIP(hipMemcpy(s->d_wire, g->wire, sizeof g->wire, hipMemcpyHostToDevice));
     hipDeviceProp_t prop;
     OR_HIP(hipGetDeviceProperties(&prop, device));
-    int occ = s->maxh == 512 ? occupancy_blocks<512>() : (s->maxh == 1024 ? occupancy_blocks<1024>() : occupancy_blocks<2048>());
-    s->grid = prop.multiProcessorCount * occ;
+    s->nclass = s->maxh == 512 ? 1 : (s->maxh == 1024 ? 2 : 3);
+    const int occ[3] = {occupancy_blocks<512>(), occupancy_blocks<1024>(), occupancy_blocks<2048>()};
+    s->grid = 0;
+    for (int c = 0; c < s->nclass; ++c) {
+        s->grid_c[c] = prop.multiProcessorCount * occ[c];
+        s->grid = std::max(s->grid, s->grid_c[c]);
+    }
     s->scratch_blocks = s->grid;
+    OR_HIP(hipMalloc(&s->d_counts, sizeof(int32_t) * NCLASS));
     OR_HIP(hipMalloc(&s->d_scratch, (size_t)SCR_STRIDE * s->scratch_blocks));
     *out = s;
     return OR_OK;
@@ -1288,6 +1477,7 @@ static or_status run_device(or_session* s, const or_hits* in, const or_result* o
     Args a;
     a.det = in->det; a.elem = in->elem; a.tdc = in->tdc; a.drift = in->drift; a.flag = in->flag;
     a.off = in->event_offsets; a.n_events = in->n_events;
+    a.ev_list = nullptr; a.ev_count = nullptr;
     a.wire = s->d_wire;
     a.p = s->p;
     a.status = out->status; a.keep_mask = out->keep_mask; a.n_tracks = out->n_tracks; a.tracks = out->tracks;
@@ -1300,9 +1490,24 @@ static or_status run_device(or_session* s, const or_hits* in, const or_result* o
         OR_HIP(hipGetLastError());
     }
     if (in->n_events == 0) return OR_OK;
-    const int grid = (int)std::min<int64_t>(s->grid, in->n_events);
+    if (s->list_cap < in->n_events) {
+        if (s->d_lists) (void)hipFree(s->d_lists);
+        s->d_lists = nullptr;
+        OR_HIP(hipMalloc(&s->d_lists, sizeof(int32_t) * NCLASS * (size_t)in->n_events));
+        s->list_cap = in->n_events;
+    }
+    hipLaunchKernelGGL(zero_u32, dim3(1), dim3(64), 0, s->stream, reinterpret_cast<uint32_t*>(s->d_counts), (int64_t)NCLASS);
+    OR_HIP(hipGetLastError());
+    hipLaunchKernelGGL(classify_events, dim3(std::min<int64_t>(1024, (in->n_events + 255) / 256)), dim3(256), 0,
+                       s->stream, in->event_offsets, in->n_events, s->nclass, s->d_lists, s->list_cap, s->d_counts);
+    OR_HIP(hipGetLastError());
     if (timed) OR_HIP(hipEventRecord(s->ev0, s->stream));
-    OR_HIP(launch(s->maxh, a, grid, s->stream));
+    for (int c = 0; c < s->nclass; ++c) {
+        a.ev_list = s->d_lists + c * s->list_cap;
+        a.ev_count = s->d_counts + c;
+        const int grid = (int)std::min<int64_t>(s->grid_c[c], in->n_events);
+        OR_HIP(launch_class(c, a, grid, s->stream));
+    }
     if (timed) OR_HIP(hipEventRecord(s->ev1, s->stream));
     return OR_OK;
 }
@@ -1391,6 +1596,8 @@ void or_session_destroy(or_session* s) {
     (void)hipSetDevice(s->device);
     if (s->d_wire) (void)hipFree(s->d_wire);
     if (s->d_scratch) (void)hipFree(s->d_scratch);
+    if (s->d_lists) (void)hipFree(s->d_lists);
+    if (s->d_counts) (void)hipFree(s->d_counts);
     if (s->d_in) (void)hipFree(s->d_in);
     if (s->d_out) (void)hipFree(s->d_out);
     if (s->ev0) (void)hipEventDestroy(s->ev0);

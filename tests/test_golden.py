@@ -1,0 +1,40 @@
+"""Golden fixture (tests/golden/c0_small.npz, written by make_golden.py): the
+oracle must keep reproducing it, and the HIP chain must match it bit for bit."""
+import os
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+@pytest.fixture(scope="module")
+def golden(oc):
+    z = np.load(os.path.join(HERE, "golden", "c0_small.npz"))
+    ev = oc.Events(z["det"], z["elem"], z["tdc"], z["drift"], z["flag"], z["offsets"])
+    return ev, z
+
+
+def check(oc, res, ev, z):
+    assert np.array_equal(res.status, z["status"])
+    assert np.array_equal(res.keep_mask, z["keep_mask"])
+    assert np.array_equal(res.n_tracklets, z["n_tracklets"])
+    ntk = z["n_tracklets"]
+    rec = np.concatenate([res.records[e, :ntk[e]] for e in range(ev.n_events)])
+    assert np.array_equal(rec.view(np.uint32), z["records"].view(np.uint32))
+    assert np.array_equal(res.n_tracks, z["n_tracks"])
+    assert np.array_equal(res.tracks.view(np.uint8).reshape(ev.n_events, -1), z["tracks"])
+
+
+def test_oracle_reproduces_golden(oc, orc, ogeo, golden):
+    ev, z = golden
+    res = orc.reconstruct(ogeo, oc.default_params(), ev, records=True)
+    check(oc, res, ev, z)
+
+
+@pytest.mark.gpu
+def test_hip_chain_matches_golden(oc, geo, golden):
+    ev, z = golden
+    sess = oc.Session(geo, oc.default_params(), device=0)
+    res = sess.reconstruct(ev, records=True)
+    check(oc, res, ev, z)
