@@ -508,189 +508,6 @@ __device__ __attribute__((noinline)) bool yz_unit(const Ev<MAXH>& E, Tk* Tp, con
 }
 
 // ---------------------------------------------------------------------------
-// Station-1 extension of one track (reconstruction_kernels.cuh:2053-2531).
-// Sequential over x pairs: y0/ty left by the last fit carry into the next x
-// pair's x-at-y computation, as in the reference.
-template <int MAXH>
-__device__ __attribute__((noinline)) int global_track(const Ev<MAXH>& E, Tk& T, uint16_t* lists, const or_params& P) {
-    const int n23 = T.nx + T.nuv;
-    const float x0 = T.x0, tx = T.tx, ex0 = T.ex0, etx = T.etx;
-    float y0 = T.y0, ty = T.ty, ey0 = T.ey0, ety = T.ety;
-    // last hit in z (reconstruction_classes.cuh:407-422)
-    int dlast = 1;
-    float zmax = -10000.f;
-    for (int k = 0; k < n23; ++k) {
-        const int d = T.det[k];
-        if (d > 0 && G(F_Z, d) > zmax) { dlast = d; zmax = G(F_Z, d); }
-    }
-    // SagittaRatioInStation1 (reconstruction_helper.cuh:556-613)
-    float pexp[3], win[3];
-    {
-        const float z3 = G(F_Z, dlast), x3 = x0 + tx * z3, y3 = y0 + ty * z3;
-        for (int i = 0; i < 3; ++i) {
-            const int d = 2 * i + 2;
-            const int idx = i == 0 ? 1 : (i == 1 ? 0 : 2);  // planetype[d/2-1]
-            const int d2 = super_id(2, idx) * 2;
-            const float ct = G(F_COSTHETA, d2), st = G(F_SINTHETA, d2);
-            const float ps3 = x3 * ct + y3 * st;
-            const float z1 = G(F_Z, d), z2 = G(F_Z, d2);
-            const float x2 = x0 + tx * z2, y2 = y0 + ty * z2;
-            const float ps2 = x2 * ct + y2 * st;
-            const float s2t = ps2 - (ps3 * (z2 - Z_TARGET)) / (z3 - Z_TARGET);
-            const float s2d = ps2 - (ps3 * (z2 - Z_DUMP)) / (z3 - Z_DUMP);
-            const float pet = SAG_T_C * s2t + (ps3 * (z1 - Z_TARGET)) / (z3 - Z_TARGET);
-            const float ped = SAG_D_C * s2d + (ps3 * (z1 - Z_DUMP)) / (z3 - Z_DUMP);
-            const float wt = fabsf(s2t * SAG_T_W), wd = fabsf(s2d * SAG_D_W);
-            const float pmn = fminf(pet - wt, ped - wd), pmx = fmaxf(pet + wt, ped + wd);
-            pexp[idx] = (float)(0.5 * (double)(pmx + pmn));
-            win[idx] = (float)(0.5 * (double)(pmx - pmn));
-        }
-    }
-    uint16_t* lx = lists;
-    uint16_t* lu = lists + WIN_CAP;
-    uint16_t* lv = lists + 2 * WIN_CAP;
-    const int capw = P.max_pairs_window < WIN_CAP ? P.max_pairs_window : WIN_CAP;
-    const int nx1 = pairs_window(E, 0, 0, pexp[0] - win[0] * 1.0f, pexp[0] + win[0] * 1.0f, lx, P.max_pair_plane_hits, capw);
-    const int nu1 = pairs_window(E, 0, 1, pexp[1] - win[1] * 1.0f, pexp[1] + win[1] * 1.0f, lu, P.max_pair_plane_hits, capw);
-    const int nv1 = pairs_window(E, 0, 2, pexp[2] - win[2] * 1.0f, pexp[2] + win[2] * 1.0f, lv, P.max_pair_plane_hits, capw);
-    if (nx1 < 0 || nu1 < 0 || nv1 < 0) return -1;
-    if (nx1 == 0 || nu1 == 0 || nv1 == 0) return 0;
-    // y of the station-2/3 UV hits with their stored signs
-    float Y[18], eY[18], Zy[18];
-    int ny = 0;
-    for (int k = 0; k < n23; ++k) {
-        const int d = T.det[k];
-        if ((double)G(F_COSTHETA, d) > 0.99) continue;
-        float y, ey;
-        if (y_uvhit(d, E.Lp->elem[T.hraw[k]], E.Lp->drift[T.hraw[k]], T.sign[k], x0, tx, y, ey)) {
-            Y[ny] = y; eY[ny] = ey; Zy[ny] = G(F_Z, d); ny++;
-        }
-    }
-    short det[6], sign[6];
-    float pos[6], drift[6], X[3], eX[3], Z[3];
-    uint16_t hr[6];
-    float chi2min = 10000.1f;
-    bool found = false;
-    float b_chi2 = 0, b_y0 = 0, b_ty = 0, b_ey0 = 0, b_ety = 0, b_invp = 0, b_einvp = 0;
-    int b_q = 0, b_n1 = 0;
-    float b_r23[12], b_r1[6];
-    uint16_t b_hr[6];
-    short b_det[6], b_sign[6];
-    for (int ix = 0; ix < nx1; ++ix) {
-        int nx = 0;
-        auto addx = [&](int d, int k) {
-            const int r = E.raw(d, k);
-            det[nx] = (short)d;
-            pos[nx] = E.Lp->pos[r];
-            eX[nx] = G(F_RESOLUTION, d);
-            sign[nx] = 0;
-            drift[nx] = E.Lp->drift[r];
-            hr[nx] = (uint16_t)r;
-            xz_fy(d, E.Lp->elem[r], drift[nx], 0, y0, ty, X[nx], Z[nx]);
-            ++nx;
-        };
-        if (pa(lx[ix]) >= 0) addx(4, pa(lx[ix]));
-        if (pb(lx[ix]) >= 0) addx(3, pb(lx[ix]));
-        if (nx == 0) continue;
-        X[nx] = x0 + tx * Z_KMAG_BEND;
-        eX[nx] = ex0 + etx * Z_KMAG_BEND;
-        Z[nx] = Z_KMAG_BEND;
-        Fit f = fit2d<3>(nx + 1, X, Z, eX);
-        float x01 = f.p0, tx1 = f.p1;
-        lr_xhits<2>(x01, tx1, x01, tx1, nx, det, X, drift, sign, 150.f);
-        lr_single_xhits(x01, tx1, nx, det, X, sign);
-        for (int l = 0; l < nx; ++l) X[l] += (float)sign[l] * drift[l];
-        f = fit2d<3>(nx + 1, X, Z, eX);
-        x01 = f.p0; tx1 = f.p1;
-        const float ex01 = f.p0, etx1 = f.p1;  // parameters used as errors (reconstruction_kernels.cuh:2194-2195)
-        // calculate_charge (E1039), reconstruction_helper.cuh:684-691
-        const int q = (double)(-0.0033f) * copysign(1.0, (double)FMAGSTR) * (double)x0 < (double)tx ? 1 : -1;
-        const float invp = ((tx1 - tx) * (float)q) / PT_KICK_KMAG;
-        const float einvp = (etx - etx) / PT_KICK_KMAG;
-        if (invp > INVP_MAX + einvp || invp < INVP_MIN - einvp) continue;
-        for (int iuv = 0; iuv < nu1 * nv1; ++iuv) {
-            const int iu = iuv % nu1, iv = iuv / nu1;
-            int nuv = 0;
-            auto adduv = [&](int d, int k) {
-                const int r = E.raw(d, k);
-                const int kk = nx + nuv;
-                det[kk] = (short)d;
-                sign[kk] = 0;
-                drift[kk] = E.Lp->drift[r];
-                pos[kk] = E.Lp->pos[r];
-                hr[kk] = (uint16_t)r;
-                float y, ey;
-                if (y_uvhit(d, E.Lp->elem[r], drift[kk], 0, x01, tx1, y, ey)) {
-                    Y[ny + nuv] = y; eY[ny + nuv] = ey; Zy[ny + nuv] = G(F_Z, d);
-                    nuv++;
-                }
-            };
-            if (pa(lu[iu]) >= 0) adduv(2, pa(lu[iu]));
-            if (pb(lu[iu]) >= 0) adduv(1, pb(lu[iu]));
-            const int nuh = nuv;
-            if (nuh == 0) continue;
-            if (pa(lv[iv]) >= 0) adduv(6, pa(lv[iv]));
-            if (pb(lv[iv]) >= 0) adduv(5, pb(lv[iv]));
-            if (nuv - nuh == 0) continue;
-            Fit g = fit2d<12>(ny + nuv, Y, Zy, eY);
-            y0 = g.p0; ty = g.p1; ey0 = g.e0; ety = g.e1;
-            if (fabsf(y0) > Y0_MAX + 2.f * ey0 || fabsf(ty) > TY_MAX + 2.f * ety) continue;
-            lr_newhits(x01, tx1, y0, ty, ex01, etx1, ey0, ety, nx + nuv, det, pos, drift, sign, 150.f);
-            lr_single_newhits(x01, tx1, y0, ty, nx + nuv, det, pos, sign);
-            g = fit2d<12>(ny + nuv, Y, Zy, eY);
-            y0 = g.p0; ty = g.p1; ey0 = g.e0; ety = g.e1;
-            if (fabsf(y0) > Y0_MAX || fabsf(ty) > TY_MAX) continue;
-            // H1 match is computed but not enforced (reconstruction_kernels.cuh:2424-2440)
-            float chi2 = 0.f;
-            float r23[12];
-            for (int m = 0; m < n23; ++m) {
-                const int d = T.det[m];
-                const int e = E.Lp->elem[T.hraw[m]];
-                const float res = G(F_RESOLUTION, d);
-                const float dca = dca_wire(d, e, x0, y0, tx, ty);
-                r23[m] = E.Lp->drift[T.hraw[m]] * (float)T.sign[m] - dca;
-                chi2 += (r23[m] * r23[m]) / (res * res);
-            }
-            float c1 = 0.f;
-            float r1[6];
-            for (int k = 0; k < nx + nuv; ++k) {
-                const int d = det[k];
-                const float res = G(F_RESOLUTION, d);
-                const float dca = dca_wire(d, E.Lp->elem[hr[k]], x01, y0, tx1, ty);
-                r1[k] = drift[k] * (float)sign[k] - dca;
-                c1 += ((r1[k] * r1[k]) / res) / res;
-            }
-            chi2 += c1;
-            if (chi2 > 10000.f) continue;
-            if (chi2 < chi2min) {
-                chi2min = chi2;
-                found = true;
-                b_chi2 = chi2; b_y0 = y0; b_ty = ty; b_ey0 = ey0; b_ety = ety; b_invp = invp; b_einvp = einvp;
-                b_q = q; b_n1 = nx + nuv;
-                for (int m = 0; m < n23; ++m) b_r23[m] = r23[m];
-                for (int k = 0; k < nx + nuv; ++k) {
-                    b_r1[k] = r1[k]; b_hr[k] = hr[k]; b_det[k] = det[k]; b_sign[k] = sign[k];
-                }
-            }
-        }
-    }
-    if (found) {
-        T.stid = 6;
-        T.n1 = (uint8_t)b_n1;
-        T.chi2 = b_chi2; T.y0 = b_y0; T.ty = b_ty; T.ey0 = b_ey0; T.ety = b_ety;
-        T.invp = b_invp; T.einvp = b_einvp; T.charge = (int8_t)b_q;
-        for (int m = 0; m < n23; ++m) T.resid[m] = b_r23[m];
-        for (int k = 0; k < b_n1; ++k) {
-            T.resid[n23 + k] = b_r1[k];
-            T.hraw[n23 + k] = b_hr[k];
-            T.det[n23 + k] = (uint8_t)b_det[k];
-            T.sign[n23 + k] = (int8_t)b_sign[k];
-        }
-    }
-    return 0;
-}
-
-// ---------------------------------------------------------------------------
 // Stages.  Each is a separate (non-inlined) function so that no single
 // function carries the live state of every stage; all lanes of the wave call
 // each stage, and stage-level branches depend only on wave-uniform values.
@@ -1176,27 +993,266 @@ STAGE void stage_clean(Tk* tks, const uint16_t* slot_order, int ntkl) {
         __syncthreads();
     }
 
-// G: station-1 extension, one lane per track
+// G: station-1 extension (reconstruction_kernels.cuh:1841-2534), one track at a
+// time, the wave's lanes spread over the track's (u, v) pairs.
+//  * Per x pair the x part (x at the current y0/ty, KMAG-point fit, left/right,
+//    refit, 1/p window) is computed by every lane alike.
+//  * (u, v) pairs of one x pair are independent except for one case: a lone X'
+//    hit (detector 3) next to a U hit (detector 2) forms a left/right pair
+//    whose sign writes carry to later (u, v) pairs.  That write does not
+//    depend on the current sign, so each lane records whether it writes
+//    (sentinel 2 in the sign slot) and a ballot scan hands each lane the sign
+//    left by the last writer before it.
+//  * y0/ty after an x pair are those of the last (u, v) pair that reached the
+//    y fit (the refit after left/right repeats the same fit); they feed the
+//    next x pair's x-at-y.
+//  * The best candidate is the minimum of (chi2, flat index ix*ncomb + iuv),
+//    which is the reference's strict-< first minimum.
+struct GBest {
+    float chi2, y0, ty, ey0, ety, invp, einvp;
+    int q, n1;
+    float r23[12], r1[6];
+    uint16_t hr[6];
+    uint8_t det[6];
+    int8_t sign[6];
+};
+
+__device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long v) {
+#pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1) {
+        const unsigned long long t = __shfl_xor(v, o, WAVE);
+        v = t < v ? t : v;
+    }
+    return v;
+}
+
 template <int MAXH>
 STAGE void stage_global(const or_params& P, const Scr& S, int ntkl) {
     Lds<MAXH>& L = g_lds<MAXH>;
     Ev<MAXH> E;
     const int lane = lane_id();
     Tk* tks = S.tks;
-    uint16_t* wl = S.wl;
-    for (int kb = 0; kb < ntkl; kb += WAVE) {
-            const int k = kb + lane;
-            if (k < ntkl) {
-                Tk T = tks[k];
-                if (T.stid >= 5) {
-                    const int r = global_track(E, T, wl + lane * 3 * WIN_CAP, P);
-                    if (r < 0) atomicOr(&L.status, (uint32_t)OR_EV_PAIR_OVERFLOW);
-                    tks[k] = T;
+    uint16_t* lx = S.wl;
+    uint16_t* lu = S.wl + WIN_CAP;
+    uint16_t* lv = S.wl + 2 * WIN_CAP;
+    GBest* best = reinterpret_cast<GBest*>(S.wy);
+    const int capw = P.max_pairs_window < WIN_CAP ? P.max_pairs_window : WIN_CAP;
+    for (int k = 0; k < ntkl; ++k) {
+        Tk* T = &tks[k];
+        if (T->stid < 5) continue;
+        const int n23 = T->nx + T->nuv;
+        const float x0 = T->x0, tx = T->tx, ex0 = T->ex0, etx = T->etx;
+        float y0 = T->y0, ty = T->ty;
+        // window lists from the sagitta ratio (reconstruction_helper.cuh:556-613), built by lane 0
+        if (lane == 0) {
+            int dlast = 1;
+            float zmax = -10000.f;
+            for (int m = 0; m < n23; ++m) {
+                const int d = T->det[m];
+                if (d > 0 && G(F_Z, d) > zmax) { dlast = d; zmax = G(F_Z, d); }
+            }
+            float pexp[3], win[3];
+            const float z3 = G(F_Z, dlast), x3 = x0 + tx * z3, y3 = y0 + ty * z3;
+            for (int i = 0; i < 3; ++i) {
+                const int d = 2 * i + 2;
+                const int idx = i == 0 ? 1 : (i == 1 ? 0 : 2);  // planetype[d/2-1]
+                const int d2 = super_id(2, idx) * 2;
+                const float ct = G(F_COSTHETA, d2), st = G(F_SINTHETA, d2);
+                const float ps3 = x3 * ct + y3 * st;
+                const float z1 = G(F_Z, d), z2 = G(F_Z, d2);
+                const float x2 = x0 + tx * z2, y2 = y0 + ty * z2;
+                const float ps2 = x2 * ct + y2 * st;
+                const float s2t = ps2 - (ps3 * (z2 - Z_TARGET)) / (z3 - Z_TARGET);
+                const float s2d = ps2 - (ps3 * (z2 - Z_DUMP)) / (z3 - Z_DUMP);
+                const float pet = SAG_T_C * s2t + (ps3 * (z1 - Z_TARGET)) / (z3 - Z_TARGET);
+                const float ped = SAG_D_C * s2d + (ps3 * (z1 - Z_DUMP)) / (z3 - Z_DUMP);
+                const float wt = fabsf(s2t * SAG_T_W), wd = fabsf(s2d * SAG_D_W);
+                const float pmn = fminf(pet - wt, ped - wd), pmx = fmaxf(pet + wt, ped + wd);
+                pexp[idx] = (float)(0.5 * (double)(pmx + pmn));
+                win[idx] = (float)(0.5 * (double)(pmx - pmn));
+            }
+            const int nx1 = pairs_window(E, 0, 0, pexp[0] - win[0] * 1.0f, pexp[0] + win[0] * 1.0f, lx, P.max_pair_plane_hits, capw);
+            const int nu1 = pairs_window(E, 0, 1, pexp[1] - win[1] * 1.0f, pexp[1] + win[1] * 1.0f, lu, P.max_pair_plane_hits, capw);
+            const int nv1 = pairs_window(E, 0, 2, pexp[2] - win[2] * 1.0f, pexp[2] + win[2] * 1.0f, lv, P.max_pair_plane_hits, capw);
+            if (nx1 < 0 || nu1 < 0 || nv1 < 0) atomicOr(&L.status, (uint32_t)OR_EV_PAIR_OVERFLOW);
+            L.nl[0][0] = nx1; L.nl[0][1] = nu1; L.nl[0][2] = nv1;
+        }
+        __syncthreads();
+        if (L.status) break;
+        const int nx1 = L.nl[0][0], nu1 = L.nl[0][1], nv1 = L.nl[0][2];
+        if (nx1 == 0 || nu1 == 0 || nv1 == 0) continue;
+        // y of the station-2/3 UV hits with their stored signs (reconstruction_kernels.cuh:2086-2102)
+        float Y[12], eY[12], Zy[12];
+        int ny = 0;
+        for (int m = 0; m < n23; ++m) {
+            const int d = T->det[m];
+            if ((double)G(F_COSTHETA, d) > 0.99) continue;
+            float y, ey;
+            if (y_uvhit(d, E.Lp->elem[T->hraw[m]], E.Lp->drift[T->hraw[m]], T->sign[m], x0, tx, y, ey)) {
+                Y[ny] = y; eY[ny] = ey; Zy[ny] = G(F_Z, d); ny++;
+            }
+        }
+        float A0s = 0.f, A1s = 0.f, A3s = 0.f, B0s = 0.f, B1s = 0.f;
+        for (int m = 0; m < ny; ++m) {
+            B0s += eY[m] * Y[m]; B1s += (eY[m] * Y[m]) * Zy[m];
+            A0s += eY[m]; A1s += eY[m] * Zy[m]; A3s += (eY[m] * Zy[m]) * Zy[m];
+        }
+        unsigned long long best_key = ~0ull;
+        const int ncomb = nu1 * nv1;
+        for (int ix = 0; ix < nx1; ++ix) {
+            // ---- x part, every lane alike
+            short xdet[2], xsign[2];
+            float xpos[2], xdr[2], X[3], eX[3], Z[3];
+            uint16_t xhr[2];
+            int nx = 0;
+            const uint16_t qx = lx[ix];
+            for (int w = 0; w < 2; ++w) {
+                const int idx = w ? pb(qx) : pa(qx);
+                if (idx < 0) continue;
+                const int d = w ? 3 : 4;
+                const int r = E.raw(d, idx);
+                xdet[nx] = (short)d; xpos[nx] = E.Lp->pos[r]; eX[nx] = G(F_RESOLUTION, d); xsign[nx] = 0;
+                xdr[nx] = E.Lp->drift[r]; xhr[nx] = (uint16_t)r;
+                xz_fy(d, E.Lp->elem[r], xdr[nx], 0, y0, ty, X[nx], Z[nx]);
+                ++nx;
+            }
+            if (nx == 0) continue;
+            X[nx] = x0 + tx * Z_KMAG_BEND;
+            eX[nx] = ex0 + etx * Z_KMAG_BEND;
+            Z[nx] = Z_KMAG_BEND;
+            Fit f = fit2d<3>(nx + 1, X, Z, eX);
+            lr_xhits<2>(f.p0, f.p1, f.p0, f.p1, nx, xdet, X, xdr, xsign, 150.f);
+            lr_single_xhits(f.p0, f.p1, nx, xdet, X, xsign);
+            for (int l = 0; l < nx; ++l) X[l] += (float)xsign[l] * xdr[l];
+            f = fit2d<3>(nx + 1, X, Z, eX);
+            const float x01 = f.p0, tx1 = f.p1;  // parameters used as errors (reconstruction_kernels.cuh:2194-2195)
+            const int q = (double)(-0.0033f) * copysign(1.0, (double)FMAGSTR) * (double)x0 < (double)tx ? 1 : -1;
+            const float invp = ((tx1 - tx) * (float)q) / PT_KICK_KMAG;
+            const float einvp = (etx - etx) / PT_KICK_KMAG;
+            if (invp > INVP_MAX + einvp || invp < INVP_MIN - einvp) continue;
+            const bool cross = nx == 1 && xdet[0] == 3;
+            short xs_state = xsign[0];
+            // ---- (u, v) pairs, one per lane
+            for (int base = 0; base < ncomb; base += WAVE) {
+                const int iuv = base + lane;
+                bool reached = false, cand = false, wrote = false;
+                float fy0 = 0.f, fty = 0.f, fey0 = 0.f, fety = 0.f, chi23 = 0.f;
+                short det[6], sign[6];
+                float pos[6], drift[6], r1[6], r23[12];
+                uint16_t hr[6];
+                int n = 0;
+                if (iuv < ncomb) {
+                    for (int l = 0; l < nx; ++l) {
+                        det[l] = xdet[l]; pos[l] = xpos[l]; drift[l] = xdr[l]; hr[l] = xhr[l];
+                        sign[l] = cross ? (short)2 : xsign[l];
+                    }
+                    n = nx;
+                    float A0 = A0s, A1 = A1s, A3 = A3s, B0 = B0s, B1 = B1s;
+                    const int iu = iuv % nu1, iv = iuv / nu1;
+                    auto adduv = [&](int d, int idx) {
+                        const int r = E.raw(d, idx);
+                        float y, ey;
+                        if (y_uvhit(d, E.Lp->elem[r], E.Lp->drift[r], 0, x01, tx1, y, ey)) {
+                            det[n] = (short)d; pos[n] = E.Lp->pos[r]; drift[n] = E.Lp->drift[r]; hr[n] = (uint16_t)r;
+                            sign[n] = 0;
+                            const float z = G(F_Z, d);
+                            B0 += ey * y; B1 += (ey * y) * z; A0 += ey; A1 += ey * z; A3 += (ey * z) * z;
+                            ++n;
+                        }
+                    };
+                    if (pa(lu[iu]) >= 0) adduv(2, pa(lu[iu]));
+                    if (pb(lu[iu]) >= 0) adduv(1, pb(lu[iu]));
+                    const int nuh = n - nx;
+                    if (nuh > 0) {
+                        if (pa(lv[iv]) >= 0) adduv(6, pa(lv[iv]));
+                        if (pb(lv[iv]) >= 0) adduv(5, pb(lv[iv]));
+                    }
+                    if (nuh > 0 && n - nx - nuh > 0) {
+                        reached = true;
+                        const Fit g = fit_finish(A0, A1, A3, B0, B1);
+                        fy0 = g.p0; fty = g.p1; fey0 = g.e0; fety = g.e1;
+                        if (!(fabsf(fy0) > Y0_MAX + 2.f * fey0 || fabsf(fty) > TY_MAX + 2.f * fety)) {
+                            lr_newhits(x01, tx1, fy0, fty, x01, tx1, fey0, fety, n, det, pos, drift, sign, 150.f);
+                            lr_single_newhits(x01, tx1, fy0, fty, n, det, pos, sign);
+                            if (cross) wrote = sign[0] != 2;
+                            if (!(fabsf(fy0) > Y0_MAX || fabsf(fty) > TY_MAX)) {
+                                // station-2/3 residuals with (x0, tx) of the tracklet and the new y line
+                                for (int m = 0; m < n23; ++m) {
+                                    const int d = T->det[m];
+                                    const float res = G(F_RESOLUTION, d);
+                                    const float dca = dca_wire(d, E.Lp->elem[T->hraw[m]], x0, fy0, tx, fty);
+                                    r23[m] = E.Lp->drift[T->hraw[m]] * (float)T->sign[m] - dca;
+                                    chi23 += (r23[m] * r23[m]) / (res * res);
+                                }
+                                cand = true;
+                            }
+                        }
+                    }
+                }
+                // sign of a lone X' hit: last writer at or before this lane, else the running state
+                if (cross) {
+                    const unsigned long long wm = ballot(wrote);
+                    const unsigned long long le = wm & (lane == 63 ? ~0ull : ((2ull << lane) - 1ull));
+                    const int src = le ? 63 - __clzll((long long)le) : 0;
+                    const int sv = __shfl((int)sign[0], src, WAVE);
+                    if (iuv < ncomb && n > 0) sign[0] = (short)(le ? sv : xs_state);
+                    if (wm) xs_state = (short)__shfl((int)sign[0], 63 - __clzll((long long)wm), WAVE);
+                }
+                unsigned long long key = ~0ull;
+                if (cand) {
+                    float c1 = 0.f;
+                    for (int l = 0; l < n; ++l) {
+                        const int d = det[l];
+                        const float res = G(F_RESOLUTION, d);
+                        const float dca = dca_wire(d, E.Lp->elem[hr[l]], x01, fy0, tx1, fty);
+                        r1[l] = drift[l] * (float)sign[l] - dca;
+                        c1 += ((r1[l] * r1[l]) / res) / res;
+                    }
+                    const float chi2 = chi23 + c1;
+                    if (!(chi2 > 10000.f) && chi2 < 10000.1f)
+                        key = ((unsigned long long)__float_as_uint(chi2) << 32) | (uint32_t)(ix * ncomb + iuv);
+                }
+                const unsigned long long cmin = wave_min_u64(key);
+                if (cmin < best_key) {
+                    if (key == cmin) {
+                        GBest& B = *best;
+                        B.chi2 = __uint_as_float((uint32_t)(key >> 32));
+                        B.y0 = fy0; B.ty = fty; B.ey0 = fey0; B.ety = fety; B.invp = invp; B.einvp = einvp; B.q = q;
+                        B.n1 = n;
+                        for (int m = 0; m < n23; ++m) B.r23[m] = r23[m];
+                        for (int l = 0; l < n; ++l) {
+                            B.r1[l] = r1[l]; B.hr[l] = hr[l]; B.det[l] = (uint8_t)det[l]; B.sign[l] = (int8_t)sign[l];
+                        }
+                    }
+                    best_key = cmin;
+                }
+                // y0/ty for the next x pair: the last (u, v) pair that reached the y fit
+                const unsigned long long rm = ballot(reached);
+                if (rm) {
+                    const int src = 63 - __clzll((long long)rm);
+                    y0 = __shfl(fy0, src, WAVE);
+                    ty = __shfl(fty, src, WAVE);
                 }
             }
         }
         __syncthreads();
+        if (best_key != ~0ull && lane == 0) {
+            const GBest& B = *best;
+            T->stid = 6;
+            T->n1 = (uint8_t)B.n1;
+            T->chi2 = B.chi2; T->y0 = B.y0; T->ty = B.ty; T->ey0 = B.ey0; T->ety = B.ety;
+            T->invp = B.invp; T->einvp = B.einvp; T->charge = (int8_t)B.q;
+            for (int m = 0; m < n23; ++m) T->resid[m] = B.r23[m];
+            for (int l = 0; l < B.n1; ++l) {
+                T->resid[n23 + l] = B.r1[l];
+                T->hraw[n23 + l] = B.hr[l];
+                T->det[n23 + l] = B.det[l];
+                T->sign[n23 + l] = B.sign[l];
+            }
+        }
+        __syncthreads();
     }
+}
 
 // H: outputs
 template <int MAXH>
