@@ -35,9 +35,21 @@ def main():
     ap.add_argument("--mean-hits", type=float, default=300.0)
     ap.add_argument("--cap-hits", type=int, default=1000)
     ap.add_argument("--seed", type=int, default=0xE1039)
-    ap.add_argument("--cpu-sample", type=int, default=100_000, help="events in the CPU baseline sample (0 = skip)")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0,
+                    help="target CPU time of the CPU baseline sample (0 = skip the baseline)")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
+                    help="only run a device-to-device copy of BYTES (reference for HBM byte counters) and exit")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        src = torch.ones(args.calibrate_copy, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copied {args.calibrate_copy} bytes")
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); events are independent, so N GPUs would "
               "each run this same per-GPU workload on their own event range", file=sys.stderr)
@@ -94,22 +106,26 @@ def main():
             traffic = json.load(f).get("hbm_bytes_per_launch")
 
     cpu = None
-    if args.cpu_sample > 0:
+    if args.cpu_seconds > 0:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as orc
 
-        ns = min(args.cpu_sample, ev.n_events)
-        sub = ev.subset(0, ns)
         og = orc.OracleGeometry(oc.DEFAULT_GEOMETRY)
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        orc.reconstruct(og, params, sub.subset(0, min(ns, 2000)), threads=threads)  # warm-up
+        # size the sample from a short probe so the timed run takes about cpu_seconds
+        probe = ev.subset(0, min(ev.n_events, 20_000))
+        t0 = time.perf_counter()
+        orc.reconstruct(og, params, probe, threads=threads)
+        rate = probe.n_events / max(1e-6, time.perf_counter() - t0)
+        ns = int(min(ev.n_events, max(probe.n_events, rate * args.cpu_seconds)))
+        sub = ev.subset(0, ns)
         t0 = time.perf_counter()
         orc.reconstruct(og, params, sub, threads=threads)
         dt = time.perf_counter() - t0
         cpu = {"value": ns / dt, "unit": "events/s", "cores": threads, "kind": "port",
-               "sample": f"first {ns} events of the same workload, oracle (C++ restatement, -O2, OpenMP over events) "
-                         f"in {dt:.1f}s"}
-        log(f"cpu baseline: {ns / dt:.0f} events/s on {threads} threads")
+               "sample": f"first {ns} events of the same workload; oracle (C++ restatement of the reference "
+                         f"per-event routines, g++ -O2, OpenMP over events) took {dt:.1f}s"}
+        log(f"cpu baseline: {ns / dt:.0f} events/s on {threads} threads ({ns} events)")
 
     line = {
         "metric": METRIC,

@@ -103,6 +103,14 @@ def load_library(path: str = LIB_PATH):
         return _lib
     if not os.path.exists(path):
         raise RuntimeError(f"{path} is missing: build it with __graft_entry__.build() (make -C e1039-gpu_amd/csrc)")
+    # torch-ROCm ships its own HIP and HSA runtimes.  Importing torch first makes
+    # the library's libamdhip64.so.7 dependency bind to torch's already-loaded
+    # copy (same soname); loading the library first would bring in a second
+    # runtime from /opt/rocm, and only one runtime per process can open the GPU.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(path)
     vp, i32, i64 = C.c_void_p, C.c_int32, C.c_int64
     lib.or_params_default.argtypes = [C.POINTER(OrParams)]
