@@ -26,6 +26,17 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -99,6 +110,15 @@ def main():
     log(f"tracks out {n_out}, tracklets {int(res.n_tracklets.sum())}, rejected events {rejected}; "
         f"algorithmic bytes {b_alg / 1e9:.3f} GB per launch")
 
+    # end to end through host buffers (PCIe both ways) on a slice, for reference only
+    n_e2e = min(ev.n_events, 200_000)
+    sub = ev.subset(0, n_e2e)
+    sess.reconstruct_host(sub)
+    t0 = time.perf_counter()
+    sess.reconstruct_host(sub)
+    e2e = n_e2e / (time.perf_counter() - t0)
+    log(f"host-buffer path (H2D + chain + D2H): {e2e:.0f} events/s on {n_e2e} events")
+
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "chain_kernel_1M_poisson300_traffic.json")
     if os.path.exists(tfile):
@@ -122,10 +142,16 @@ def main():
         t0 = time.perf_counter()
         orc.reconstruct(og, params, sub, threads=threads)
         dt = time.perf_counter() - t0
+        # single-thread rate on a short prefix (about a tenth of the time budget)
+        n1 = int(min(ns, max(500, rate / threads * args.cpu_seconds * 0.1)))
+        t0 = time.perf_counter()
+        orc.reconstruct(og, params, ev.subset(0, n1), threads=1)
+        r1 = n1 / (time.perf_counter() - t0)
         cpu = {"value": ns / dt, "unit": "events/s", "cores": threads, "kind": "port",
                "sample": f"first {ns} events of the same workload; oracle (C++ restatement of the reference "
-                         f"per-event routines, g++ -O2, OpenMP over events) took {dt:.1f}s"}
-        log(f"cpu baseline: {ns / dt:.0f} events/s on {threads} threads ({ns} events)")
+                         f"per-event routines, g++ -O2, OpenMP over events) took {dt:.1f}s",
+               "value_1_thread": r1, "host_cpu": cpu_model()}
+        log(f"cpu baseline: {ns / dt:.0f} events/s on {threads} threads ({ns} events), {r1:.0f} events/s on 1")
 
     line = {
         "metric": METRIC,
@@ -147,6 +173,7 @@ def main():
             "events": ev.n_events,
             "hits": int(ev.n_hits),
             "kernel_ms": kernel_ms,
+            "host_buffer_events_per_s": e2e,
         },
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},

@@ -33,6 +33,7 @@ constexpr int BIN_CAP = 20;   // pairs per XZ bin (globalconsts::MaxHitsProj_X)
 constexpr int WIN_CAP = 100;  // pairs per window list (hitpairs_*[100])
 constexpr int PLANE_CAP = 400;
 constexpr int YZ_CHUNK = 16;  // tracklets whose window lists are built together
+constexpr int YZ_LIST_CAP = 4096;  // YZ candidates passing the first box cut, per chunk
 constexpr uint16_t NOHIT = 0xFFFF;
 
 // Tracklet as kept in scratch between stages.
@@ -65,7 +66,7 @@ struct Lds {
     int pcnt[56], pstart[56], cursor[56], rcnt[56], rstart[56];
     int xcnt[3][28];
     int nl[YZ_CHUNK][4];
-    int uoff[YZ_CHUNK + 1];
+    int uoff[YZ_CHUNK + 1];   // prefix of (u3, v3) units or of candidates per tracklet of the chunk
     unsigned long long best[YZ_CHUNK];
     int sstart[33];
     uint32_t status;
@@ -91,6 +92,7 @@ struct Args {
     float* records;
     char* scratch;
     int64_t scratch_stride;
+    int yz_list_cap;            // <= YZ_LIST_CAP; chunks with more candidates take the sequential path
 };
 
 // scratch layout per block
@@ -104,7 +106,9 @@ constexpr int64_t SCR_YF = SCR_YV + SCR_YV_BYTES;                               
 constexpr int64_t SCR_YF_BYTES = (int64_t)YZ_CHUNK * 4 * WIN_CAP;
 constexpr int64_t SCR_XP = ((SCR_YF + SCR_YF_BYTES + 255) / 256) * 256;           // uint16 [3][28][BIN_CAP] X pairs
 constexpr int64_t SCR_SO = SCR_XP + ((3 * 28 * BIN_CAP * 2 + 255) / 256) * 256;      // uint16 [MAXTKL] slot order
-constexpr int64_t SCR_STRIDE = ((SCR_SO + MAXTKL * 2 + 4095) / 4096) * 4096;
+constexpr int64_t SCR_PL = ((SCR_SO + MAXTKL * 2 + 255) / 256) * 256;               // uint32 [YZ_LIST_CAP] YZ items
+constexpr int64_t SCR_PB = SCR_PL + (int64_t)YZ_LIST_CAP * 4;                       // uint8 x4 [YZ_LIST_CAP] item state
+constexpr int64_t SCR_STRIDE = ((SCR_PB + (int64_t)YZ_LIST_CAP * 4 + 4095) / 4096) * 4096;
 
 __device__ __forceinline__ unsigned long long ballot(bool p) { return __ballot(p); }
 __device__ __forceinline__ int lane_id() { return threadIdx.x; }
@@ -521,6 +525,10 @@ struct Scr {                  // per-wave scratch in HBM
     uint8_t* wf;
     uint16_t* xpairs;         // [3][28][BIN_CAP], (a | b << 8), 0xFF = none
     uint16_t* slot_order;     // tracklet index by slot order
+    uint32_t* plist;          // YZ candidates passing the first box cut, in reference order
+    uchar4* pst;              // per candidate: x first-of-unit, y station-3 signs after its left/right
+                              // (first of unit), z sign written to the last station-3 hit (0: none),
+                              // w station-3 signs it starts from
 };
 
 // A + B: load, group by detector, reduce.  Returns true when the keep bits are
@@ -844,9 +852,341 @@ STAGE int stage_xz(const or_params& P, const Scr& S) {
         return ntkl;
 }
 
-// E: YZ extension of every tracklet
+// E: YZ extension (reconstruction_kernels.cuh:864-1832), chunks of YZ_CHUNK
+// tracklets.
+//
+// Reference order per tracklet: (u3, v3) pairs outer with u3 fastest, (u2, v2)
+// pairs inner with u2 fastest; the minimum chi2 with strict < wins.  Within one
+// (u3, v3) unit the station-3 hit signs persist from candidate to candidate:
+// the first candidate that passes the first box cut sets all of them through
+// left/right; afterwards only the last station-3 hit can change, when the
+// pairing walk pairs it with the first station-2 hit (D3p' 19 with D2 U 18)
+// and that pair's left/right decision is taken.  Every other input of a
+// candidate (its fit, which uses y at sign 0, and the pair decisions) depends
+// on the candidate alone.  So the sign state each candidate starts from is a
+// last-writer scan over the unit's box-cut survivors, and all candidates are
+// evaluated in parallel (yz_chunk_par).  yz_chunk_seq walks units one lane
+// each, in reference order, for chunks with more survivors than the list holds.
+
 template <int MAXH>
-STAGE void stage_yz(const or_params& P, const Scr& S, int ntkl) {
+__device__ __forceinline__ void yz_lists_of(const Scr& S, int j, YzLists& W) {
+    for (int v = 0; v < 4; ++v) {
+        W.l[v] = S.wl + (j * 4 + v) * WIN_CAP;
+        W.y[v] = S.wy + (j * 4 + v) * WIN_CAP;
+        W.f[v] = S.wf + (j * 4 + v) * WIN_CAP;
+        W.n[v] = g_lds<MAXH>.nl[j][v];
+    }
+}
+
+struct YzItem { int j, local, iu3, iv3, iu2, iv2; };
+
+// candidate index within the chunk -> tracklet and list entries
+template <int MAXH>
+__device__ __forceinline__ YzItem yz_decode(int i) {
+    const Lds<MAXH>& L = g_lds<MAXH>;
+    YzItem it;
+    int j = 0;
+    while (L.uoff[j + 1] <= i) ++j;
+    it.j = j;
+    it.local = i - L.uoff[j];
+    const int n0 = L.nl[j][0], nin = n0 * L.nl[j][1];
+    const int iuv3 = it.local / nin, iuv2 = it.local - iuv3 * nin;
+    it.iu2 = iuv2 % n0; it.iv2 = iuv2 / n0;
+    it.iu3 = iuv3 % L.nl[j][2]; it.iv3 = iuv3 / L.nl[j][2];
+    return it;
+}
+
+// first fit of a candidate over u3, u3', v3, v3', u2, u2', v2, v2' (the
+// reference's float order); false when a view has no accepted hit
+__device__ __forceinline__ bool yz_fit(const YzLists& W, int st3, const YzItem& it, Fit& f) {
+    const int su2 = super_id(2, 1) * 2, sv2 = super_id(2, 2) * 2;
+    const int su3 = super_id(st3, 1) * 2, sv3 = super_id(st3, 2) * 2;
+    float A0 = 0.f, A1 = 0.f, A3 = 0.f, B0 = 0.f, B1 = 0.f;
+    if (!yz_sum(W, 2, it.iu3, G(F_Z, su3), G(F_Z, su3 - 1), A0, A1, A3, B0, B1)) return false;
+    if (!yz_sum(W, 3, it.iv3, G(F_Z, sv3), G(F_Z, sv3 - 1), A0, A1, A3, B0, B1)) return false;
+    if (!yz_sum(W, 0, it.iu2, G(F_Z, su2), G(F_Z, su2 - 1), A0, A1, A3, B0, B1)) return false;
+    if (!yz_sum(W, 1, it.iv2, G(F_Z, sv2), G(F_Z, sv2 - 1), A0, A1, A3, B0, B1)) return false;
+    f = fit_finish(A0, A1, A3, B0, B1);
+    return true;
+}
+
+__device__ __forceinline__ bool yz_box1(const Fit& f) {
+    return !(fabsf(f.p0) > Y0_MAX + 2.f * f.e0 || fabsf(f.p1) > TY_MAX + 2.f * f.e1);
+}
+
+// station-3 hits of a candidate, signs 0; returns their number
+template <int MAXH>
+__device__ __forceinline__ int yz_put3(const Ev<MAXH>& E, YzHits& H, const YzLists& W, int st3, const YzItem& it) {
+    const int su3 = super_id(st3, 1) * 2, sv3 = super_id(st3, 2) * 2;
+    int n = 0;
+    yz_put(E, H, n, W, 2, it.iu3, 0, su3);
+    yz_put(E, H, n, W, 2, it.iu3, 1, su3 - 1);
+    yz_put(E, H, n, W, 3, it.iv3, 0, sv3);
+    yz_put(E, H, n, W, 3, it.iv3, 1, sv3 - 1);
+    return n;
+}
+
+// station-3 signs packed two bits each (3 = -1)
+__device__ __forceinline__ uint8_t yz_pack3(const YzHits& H, int n3) {
+    unsigned c = 0;
+    for (int q = 0; q < n3; ++q) c |= (unsigned)(H.sign[q] & 3) << (2 * q);
+    return (uint8_t)c;
+}
+__device__ __forceinline__ void yz_unpack3(YzHits& H, int n3, unsigned c) {
+    for (int q = 0; q < n3; ++q) {
+        const int v = (c >> (2 * q)) & 3;
+        H.sign[q] = (short)(v == 3 ? -1 : v);
+    }
+}
+
+__device__ __forceinline__ int wave_max_scan(int v) {
+    const int l = lane_id();
+#pragma unroll
+    for (int o = 1; o < WAVE; o <<= 1) {
+        const int t = __shfl_up(v, o, WAVE);
+        if (l >= o) v = max(v, t);
+    }
+    return v;
+}
+
+// Two-pass form.  Returns false (nothing written) when the survivors of the
+// first box cut do not fit the list.
+template <int MAXH>
+__device__ __attribute__((noinline)) bool yz_chunk_par(const Scr& S, int kb, int ntkl, int cap) {
+    Lds<MAXH>& L = g_lds<MAXH>;
+    Ev<MAXH> E;
+    const int lane = lane_id();
+    Tk* tks = S.tks;
+    {
+        const int j = lane, k = kb + j;
+        int items = 0;
+        if (j < YZ_CHUNK && k < ntkl) {
+            const int* nl = L.nl[j];
+            if (nl[0] > 0 && nl[1] > 0 && nl[2] > 0 && nl[3] > 0) items = nl[0] * nl[1] * nl[2] * nl[3];
+        }
+        const int inc = wave_scan(items);   // at most 16 * 100^4 < 2^31
+        if (j < YZ_CHUNK) {
+            L.best[j] = ~0ull;
+            L.uoff[j + 1] = inc;
+        }
+        if (j == 0) L.uoff[0] = 0;
+    }
+    __syncthreads();
+    const int I = L.uoff[YZ_CHUNK];
+    // pass 1: first fit and box cut of every candidate; survivors in order
+    int np = 0;
+    for (int i0 = 0; i0 < I; i0 += WAVE) {
+        const int i = i0 + lane;
+        bool pass = false;
+        if (i < I) {
+            const YzItem it = yz_decode<MAXH>(i);
+            YzLists W;
+            yz_lists_of<MAXH>(S, it.j, W);
+            Fit f;
+            pass = yz_fit(W, tks[kb + it.j].st3, it, f) && yz_box1(f);
+        }
+        const unsigned long long m = ballot(pass);
+        const int c = __popcll(m);
+        if (np + c > cap) return false;
+        if (pass) S.plist[np + rank_in(m)] = (uint32_t)i;
+        np += c;
+    }
+    __syncthreads();
+    // pass 2: per survivor, first-of-unit flag; the first of a unit runs the
+    // full left/right from zero signs (its station-3 result seeds the unit),
+    // later ones only the station-3 / station-2 cross pair
+    for (int k0 = 0; k0 < np; k0 += WAVE) {
+        const int k = k0 + lane;
+        if (k < np) {
+            const int i = (int)S.plist[k];
+            const YzItem it = yz_decode<MAXH>(i);
+            const int nin = L.nl[it.j][0] * L.nl[it.j][1];
+            bool first = k == 0;
+            if (!first) {
+                const int ip = (int)S.plist[k - 1];
+                const YzItem pt = yz_decode<MAXH>(ip);
+                first = pt.j != it.j || (pt.local / nin) != (it.local / nin);
+            }
+            const Tk* Tp = &tks[kb + it.j];
+            YzLists W;
+            yz_lists_of<MAXH>(S, it.j, W);
+            Fit f;
+            yz_fit(W, Tp->st3, it, f);
+            YzHits H;
+            const int n3 = yz_put3(E, H, W, Tp->st3, it);
+            const int su2 = super_id(2, 1) * 2, sv2 = super_id(2, 2) * 2;
+            int n = n3;
+            yz_put(E, H, n, W, 0, it.iu2, 0, su2);
+            yz_put(E, H, n, W, 0, it.iu2, 1, su2 - 1);
+            yz_put(E, H, n, W, 1, it.iv2, 0, sv2);
+            yz_put(E, H, n, W, 1, it.iv2, 1, sv2 - 1);
+            uchar4 ps = make_uchar4(first ? 1 : 0, 0, 0, 0);
+            if (first) {
+                lr_newhits(Tp->x0, Tp->tx, f.p0, f.p1, Tp->ex0, Tp->etx, f.e0, f.e1, n, H.det, H.pos, H.drift,
+                           H.sign, 150.f);
+                lr_single_newhits(Tp->x0, Tp->tx, f.p0, f.p1, n, H.det, H.pos, H.sign);
+                ps.y = yz_pack3(H, n3);
+                ps.z = (uint8_t)(H.sign[n3 - 1] & 3);
+            } else {
+                const int ci = lr_cross_index(n, n3, H.det);
+                if (ci >= 0) {
+                    const int h = lr_pair_decide(Tp->x0, Tp->tx, f.p0, f.p1, Tp->ex0, Tp->etx, f.e0, f.e1, H.det[ci],
+                                                 H.det[ci + 1], H.pos[ci], H.pos[ci + 1], H.drift[ci],
+                                                 H.drift[ci + 1], 150.f);
+                    if (h >= 0) ps.z = (uint8_t)(lr_si(h) & 3);
+                }
+            }
+            S.pst[k] = ps;
+        }
+    }
+    __syncthreads();
+    // pass 3: starting signs by scan (unit's first survivor, last writer of
+    // the last station-3 sign before this one), then the rest of the candidate
+    int carry_f = -1, carry_w = -1;
+    for (int k0 = 0; k0 < np; k0 += WAVE) {
+        const int k = k0 + lane;
+        const bool valid = k < np;
+        uchar4 ps = make_uchar4(0, 0, 0, 0);
+        if (valid) ps = S.pst[k];
+        int fi = wave_max_scan(valid && ps.x ? k : -1);
+        fi = max(fi, carry_f);
+        const int wincl = wave_max_scan(valid && (ps.x || ps.z) ? k : -1);
+        int wexcl = __shfl_up(wincl, 1, WAVE);
+        if (lane == 0) wexcl = -1;
+        wexcl = max(wexcl, carry_w);
+        carry_f = __shfl(fi, WAVE - 1, WAVE);
+        carry_w = max(carry_w, __shfl(wincl, WAVE - 1, WAVE));
+        if (valid) {
+            const int i = (int)S.plist[k];
+            const YzItem it = yz_decode<MAXH>(i);
+            Tk* Tp = &tks[kb + it.j];
+            YzLists W;
+            yz_lists_of<MAXH>(S, it.j, W);
+            Fit f;
+            yz_fit(W, Tp->st3, it, f);
+            YzHits H;
+            const int n3 = yz_put3(E, H, W, Tp->st3, it);
+            uint8_t st = 0;
+            if (!ps.x) {
+                const unsigned last = S.pst[wexcl].z;   // written by the first of the unit at least
+                st = (uint8_t)((S.pst[fi].y & ~(3u << (2 * (n3 - 1)))) | (last << (2 * (n3 - 1))));
+                yz_unpack3(H, n3, st);
+            }
+            ps.w = st;
+            S.pst[k] = ps;
+            float chi2;
+            // the reference starts its minimum at 10000.1 (reconstruction_kernels.cuh:1240)
+            if (yz_tail(E, Tp, W, H, n3, it.iu2, it.iv2, super_id(2, 1) * 2, super_id(2, 2) * 2, f.p0, f.p1, f.e0,
+                        f.e1, chi2, nullptr) &&
+                chi2 < 10000.1f) {
+                const unsigned long long key =
+                    ((unsigned long long)__float_as_uint(chi2) << 32) | (unsigned long long)(uint32_t)it.local;
+                atomicMin(&L.best[it.j], key);
+            }
+        }
+    }
+    __syncthreads();
+    // winners: replay the candidate with its starting signs and write it
+    if (lane < YZ_CHUNK && kb + lane < ntkl) {
+        const int j = lane;
+        Tk* Tp = &tks[kb + j];
+        const unsigned long long key = L.best[j];
+        if (key != ~0ull) {
+            const int i = L.uoff[j] + (int)(uint32_t)key;
+            int lo = 0, hi = np - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if ((int)S.plist[mid] < i) lo = mid + 1;
+                else hi = mid;
+            }
+            const YzItem it = yz_decode<MAXH>(i);
+            YzLists W;
+            yz_lists_of<MAXH>(S, j, W);
+            Fit f;
+            yz_fit(W, Tp->st3, it, f);
+            YzHits H;
+            const int n3 = yz_put3(E, H, W, Tp->st3, it);
+            if (!S.pst[lo].x) yz_unpack3(H, n3, S.pst[lo].w);
+            float chi2;
+            yz_tail(E, Tp, W, H, n3, it.iu2, it.iv2, super_id(2, 1) * 2, super_id(2, 2) * 2, f.p0, f.p1, f.e0, f.e1,
+                    chi2, Tp);
+        } else {
+            Tp->stid = 3;
+        }
+    }
+    return true;
+}
+
+// Unit loop: one lane per (tracklet, u3.v3) unit, candidates in reference order.
+template <int MAXH>
+__device__ __attribute__((noinline)) void yz_chunk_seq(const Scr& S, int kb, int ntkl) {
+    Lds<MAXH>& L = g_lds<MAXH>;
+    Ev<MAXH> E;
+    const int lane = lane_id();
+    Tk* tks = S.tks;
+    uint16_t* wl = S.wl;
+    float4* wy = S.wy;
+    uint8_t* wf = S.wf;
+    {
+        const int j = lane, k = kb + j;
+        int u = 0;
+        if (j < YZ_CHUNK && k < ntkl) {
+            const int* nl = L.nl[j];
+            if (nl[0] > 0 && nl[1] > 0 && nl[2] > 0 && nl[3] > 0) u = nl[2] * nl[3];
+        }
+        const int inc = wave_scan(u);
+        if (j < YZ_CHUNK) {
+            L.best[j] = ~0ull;
+            L.uoff[j + 1] = inc;
+        }
+        if (j == 0) L.uoff[0] = 0;
+    }
+    __syncthreads();
+    const int U = L.uoff[YZ_CHUNK];
+    for (int u0 = 0; u0 < U; u0 += WAVE) {
+        const int u = u0 + lane;
+        if (u < U) {
+            int j = 0;
+            while (L.uoff[j + 1] <= u) ++j;
+            const int iuv3 = u - L.uoff[j];
+            YzLists W;
+            for (int v = 0; v < 4; ++v) {
+                W.l[v] = wl + (j * 4 + v) * WIN_CAP; W.y[v] = wy + (j * 4 + v) * WIN_CAP;
+                W.f[v] = wf + (j * 4 + v) * WIN_CAP; W.n[v] = L.nl[j][v];
+            }
+            float bc = 10000.1f;
+            int bi = -1;
+            if (yz_unit(E, &tks[kb + j], W, iuv3, -1, bc, bi)) {
+                const unsigned long long key =
+                    ((unsigned long long)__float_as_uint(bc) << 32) |
+                    (unsigned long long)(uint32_t)(iuv3 * (W.n[0] * W.n[1]) + bi);
+                atomicMin(&L.best[j], key);
+            }
+        }
+    }
+    __syncthreads();
+    if (lane < YZ_CHUNK && kb + lane < ntkl) {
+        const int j = lane;
+        Tk* Tp = &tks[kb + j];
+        const unsigned long long key = L.best[j];
+        if (key != ~0ull) {
+            YzLists W;
+            for (int v = 0; v < 4; ++v) {
+                W.l[v] = wl + (j * 4 + v) * WIN_CAP; W.y[v] = wy + (j * 4 + v) * WIN_CAP;
+                W.f[v] = wf + (j * 4 + v) * WIN_CAP; W.n[v] = L.nl[j][v];
+            }
+            const int flat = (int)(uint32_t)key, nin = W.n[0] * W.n[1];
+            float bc = 0.f;
+            int bi = 0;
+            yz_unit(E, Tp, W, flat / nin, flat % nin, bc, bi);
+        } else {
+            Tp->stid = 3;
+        }
+    }
+}
+
+template <int MAXH>
+STAGE void stage_yz(const or_params& P, const Scr& S, int ntkl, int list_cap) {
     Lds<MAXH>& L = g_lds<MAXH>;
     Ev<MAXH> E;
     const int lane = lane_id();
@@ -855,102 +1195,50 @@ STAGE void stage_yz(const or_params& P, const Scr& S, int ntkl) {
     float4* wy = S.wy;
     uint8_t* wf = S.wf;
     const int capw = P.max_pairs_window < WIN_CAP ? P.max_pairs_window : WIN_CAP;
-        for (int kb = 0; kb < ntkl; kb += YZ_CHUNK) {
-            // window lists: lane = (tracklet j, view v)
-            {
-                const int j = lane >> 2, v = lane & 3;
-                const int k = kb + j;
-                if (k < ntkl) {
-                    const Tk& T = tks[k];
-                    const int st = v < 2 ? 2 : T.st3, pj = (v & 1) ? 2 : 1;
-                    const int sid = super_id(st, pj);
-                    const int da = sid * 2, db = sid * 2 - 1;
-                    const float za = G(F_Z, da), zb = G(F_Z, db);
-                    const float xmin = fminf((T.tx - T.etx) * za, (T.tx - T.etx) * zb) + T.x0 - T.ex0 - G(F_SPACING, da);
-                    const float xmax = fmaxf((T.tx + T.etx) * za, (T.tx + T.etx) * zb) + T.x0 + T.ex0 + G(F_SPACING, da);
-                    uint16_t* lst = wl + (j * 4 + v) * WIN_CAP;
-                    const int nl = pairs_window(E, st, pj, xmin, xmax, lst, P.max_pair_plane_hits, capw);
-                    if (nl < 0) atomicOr(&L.status, (uint32_t)OR_EV_PAIR_OVERFLOW);
-                    L.nl[j][v] = nl < 0 ? 0 : nl;
-                    // y of each listed hit from the tracklet's x line (calculate_y_uvhit, sign 0)
-                    for (int q = 0; q < nl; ++q) {
-                        float4 yv = make_float4(0.f, 0.f, 0.f, 0.f);
-                        uint8_t fl = 0;
-                        const int ia = pa(lst[q]), ib = pb(lst[q]);
-                        if (ia >= 0) {
-                            const int r = E.raw(da, ia);
-                            if (y_uvhit(da, L.elem[r], L.drift[r], 0, T.x0, T.tx, yv.x, yv.y)) fl |= 1;
-                        }
-                        if (ib >= 0) {
-                            const int r = E.raw(db, ib);
-                            if (y_uvhit(db, L.elem[r], L.drift[r], 0, T.x0, T.tx, yv.z, yv.w)) fl |= 2;
-                        }
-                        wy[(j * 4 + v) * WIN_CAP + q] = yv;
-                        wf[(j * 4 + v) * WIN_CAP + q] = fl;
+    for (int kb = 0; kb < ntkl; kb += YZ_CHUNK) {
+        // window lists: lane = (tracklet j, view v)
+        {
+            const int j = lane >> 2, v = lane & 3;
+            const int k = kb + j;
+            if (k < ntkl) {
+                const Tk& T = tks[k];
+                const int st = v < 2 ? 2 : T.st3, pj = (v & 1) ? 2 : 1;
+                const int sid = super_id(st, pj);
+                const int da = sid * 2, db = sid * 2 - 1;
+                const float za = G(F_Z, da), zb = G(F_Z, db);
+                const float xmin = fminf((T.tx - T.etx) * za, (T.tx - T.etx) * zb) + T.x0 - T.ex0 - G(F_SPACING, da);
+                const float xmax = fmaxf((T.tx + T.etx) * za, (T.tx + T.etx) * zb) + T.x0 + T.ex0 + G(F_SPACING, da);
+                uint16_t* lst = wl + (j * 4 + v) * WIN_CAP;
+                const int nl = pairs_window(E, st, pj, xmin, xmax, lst, P.max_pair_plane_hits, capw);
+                if (nl < 0) atomicOr(&L.status, (uint32_t)OR_EV_PAIR_OVERFLOW);
+                L.nl[j][v] = nl < 0 ? 0 : nl;
+                // y of each listed hit from the tracklet's x line (calculate_y_uvhit, sign 0)
+                for (int q = 0; q < nl; ++q) {
+                    float4 yv = make_float4(0.f, 0.f, 0.f, 0.f);
+                    uint8_t fl = 0;
+                    const int ia = pa(lst[q]), ib = pb(lst[q]);
+                    if (ia >= 0) {
+                        const int r = E.raw(da, ia);
+                        if (y_uvhit(da, L.elem[r], L.drift[r], 0, T.x0, T.tx, yv.x, yv.y)) fl |= 1;
                     }
+                    if (ib >= 0) {
+                        const int r = E.raw(db, ib);
+                        if (y_uvhit(db, L.elem[r], L.drift[r], 0, T.x0, T.tx, yv.z, yv.w)) fl |= 2;
+                    }
+                    wy[(j * 4 + v) * WIN_CAP + q] = yv;
+                    wf[(j * 4 + v) * WIN_CAP + q] = fl;
                 }
             }
-            __syncthreads();
-            if (L.status) break;
-            {
-                const int j = lane, k = kb + j;
-                int u = 0;
-                if (j < YZ_CHUNK && k < ntkl) {
-                    const int* nl = L.nl[j];
-                    if (nl[0] > 0 && nl[1] > 0 && nl[2] > 0 && nl[3] > 0) u = nl[2] * nl[3];
-                }
-                const int inc = wave_scan(u);
-                if (j < YZ_CHUNK) {
-                    L.best[j] = ~0ull;
-                    L.uoff[j + 1] = inc;
-                }
-                if (j == 0) L.uoff[0] = 0;
-            }
-            __syncthreads();
-            const int U = L.uoff[YZ_CHUNK];
-            for (int u0 = 0; u0 < U; u0 += WAVE) {
-                const int u = u0 + lane;
-                if (u < U) {
-                    int j = 0;
-                    while (L.uoff[j + 1] <= u) ++j;
-                    const int iuv3 = u - L.uoff[j];
-                    YzLists W;
-                    for (int v = 0; v < 4; ++v) {
-                        W.l[v] = wl + (j * 4 + v) * WIN_CAP; W.y[v] = wy + (j * 4 + v) * WIN_CAP;
-                        W.f[v] = wf + (j * 4 + v) * WIN_CAP; W.n[v] = L.nl[j][v];
-                    }
-                    float bc = 10000.1f;
-                    int bi = -1;
-                    if (yz_unit(E, &tks[kb + j], W, iuv3, -1, bc, bi)) {
-                        const unsigned long long key =
-                            ((unsigned long long)__float_as_uint(bc) << 32) |
-                            (unsigned long long)(uint32_t)(iuv3 * (W.n[0] * W.n[1]) + bi);
-                        atomicMin(&L.best[j], key);
-                    }
-                }
-            }
-            __syncthreads();
-            if (lane < YZ_CHUNK && kb + lane < ntkl) {
-                const int j = lane;
-                Tk* Tp = &tks[kb + j];
-                const unsigned long long key = L.best[j];
-                if (key != ~0ull) {
-                    YzLists W;
-                    for (int v = 0; v < 4; ++v) {
-                        W.l[v] = wl + (j * 4 + v) * WIN_CAP; W.y[v] = wy + (j * 4 + v) * WIN_CAP;
-                        W.f[v] = wf + (j * 4 + v) * WIN_CAP; W.n[v] = L.nl[j][v];
-                    }
-                    const int flat = (int)(uint32_t)key, nin = W.n[0] * W.n[1];
-                    float bc = 0.f;
-                    int bi = 0;
-                    yz_unit(E, Tp, W, flat / nin, flat % nin, bc, bi);
-                } else {
-                    Tp->stid = 3;
-                }
-            }
-            __syncthreads();
         }
+        __syncthreads();
+        if (L.status) break;
+        if (!yz_chunk_par<MAXH>(S, kb, ntkl, list_cap)) {
+            __syncthreads();
+            yz_chunk_seq<MAXH>(S, kb, ntkl);
+        }
+        __syncthreads();
     }
+}
 
 // F: back-partial cleaning (reconstruction_helper.cuh:1789-1909), slot order
 template <int MAXH>
@@ -1362,6 +1650,8 @@ __global__ __launch_bounds__(WAVE, MAXH <= 512 ? 4 : (MAXH <= 1024 ? 2 : 1)) voi
     S.wf = reinterpret_cast<uint8_t*>(scr + SCR_YF);
     S.xpairs = reinterpret_cast<uint16_t*>(scr + SCR_XP);
     S.slot_order = reinterpret_cast<uint16_t*>(scr + SCR_SO);
+    S.plist = reinterpret_cast<uint32_t*>(scr + SCR_PL);
+    S.pst = reinterpret_cast<uchar4*>(scr + SCR_PB);
     const int64_t ne = a.ev_list ? (int64_t)*a.ev_count : a.n_events;
     for (int64_t i = blockIdx.x; i < ne; i += gridDim.x) {
         const int64_t e = a.ev_list ? (int64_t)a.ev_list[i] : i;
@@ -1376,7 +1666,7 @@ __global__ __launch_bounds__(WAVE, MAXH <= 512 ? 4 : (MAXH <= 1024 ? 2 : 1)) voi
         int ntkl = 0;
         if (L.status == 0) stage_xpairs<MAXH>(a.p, S.xpairs);
         if (L.status == 0) ntkl = stage_xz<MAXH>(a.p, S);
-        if (L.status == 0) stage_yz<MAXH>(a.p, S, ntkl);
+        if (L.status == 0) stage_yz<MAXH>(a.p, S, ntkl, a.yz_list_cap);
         if (L.status == 0) stage_clean<MAXH>(S.tks, S.slot_order, ntkl);
         if (L.status == 0) stage_global<MAXH>(a.p, S, ntkl);
         stage_out<MAXH>(a, S, e, ntkl);
@@ -1428,7 +1718,9 @@ struct or_session {
     hipStream_t own = nullptr;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    float last_ms = 0.f;
+    mutable float last_ms = 0.f;
+    mutable bool timing_pending = false;   // ev0/ev1 recorded, elapsed time not read yet
+    int yz_list_cap = YZ_LIST_CAP;
     // host staging (host-pointer calls)
     void* d_in = nullptr;
     size_t d_in_bytes = 0;
@@ -1527,7 +1819,27 @@ or_status or_session_sync(or_session* s) {
     return OR_OK;
 }
 
-float or_session_last_kernel_ms(const or_session* s) { return s ? s->last_ms : 0.f; }
+float or_session_last_kernel_ms(const or_session* s) {
+    if (!s) return 0.f;
+    if (s->timing_pending) {
+        float ms = 0.f;
+        if (hipEventSynchronize(s->ev1) == hipSuccess && hipEventElapsedTime(&ms, s->ev0, s->ev1) == hipSuccess)
+            s->last_ms = ms;
+        s->timing_pending = false;
+    }
+    return s->last_ms;
+}
+
+or_status or_session_set_tuning(or_session* s, or_tuning key, int64_t value) {
+    if (!s) return OR_ERR_INVALID_ARG;
+    switch (key) {
+        case OR_TUNE_YZ_LIST_CAP:
+            if (value < 1 || value > YZ_LIST_CAP) return OR_ERR_INVALID_ARG;
+            s->yz_list_cap = (int)value;
+            return OR_OK;
+    }
+    return OR_ERR_INVALID_ARG;
+}
 
 static or_status run_device(or_session* s, const or_hits* in, const or_result* out, bool timed) {
     Args a;
@@ -1540,12 +1852,17 @@ static or_status run_device(or_session* s, const or_hits* in, const or_result* o
     a.n_tracklets = out->n_tracklets; a.records = out->records;
     a.scratch = s->d_scratch;
     a.scratch_stride = SCR_STRIDE;
+    a.yz_list_cap = s->yz_list_cap;
     const int64_t nw = (in->n_hits + 31) / 32;
     if (nw > 0) {
         hipLaunchKernelGGL(zero_u32, dim3(1024), dim3(256), 0, s->stream, out->keep_mask, nw);
         OR_HIP(hipGetLastError());
     }
-    if (in->n_events == 0) return OR_OK;
+    if (in->n_events == 0) {
+        s->last_ms = 0.f;
+        s->timing_pending = false;
+        return OR_OK;
+    }
     if (s->list_cap < in->n_events) {
         if (s->d_lists) (void)hipFree(s->d_lists);
         s->d_lists = nullptr;
@@ -1564,7 +1881,10 @@ static or_status run_device(or_session* s, const or_hits* in, const or_result* o
         const int grid = (int)std::min<int64_t>(s->grid_c[c], in->n_events);
         OR_HIP(launch_class(c, a, grid, s->stream));
     }
-    if (timed) OR_HIP(hipEventRecord(s->ev1, s->stream));
+    if (timed) {
+        OR_HIP(hipEventRecord(s->ev1, s->stream));
+        s->timing_pending = true;
+    }
     return OR_OK;
 }
 
@@ -1575,14 +1895,8 @@ or_status or_reconstruct(or_session* s, const or_hits* in, or_result* out) {
         return OR_ERR_INVALID_ARG;
     OR_HIP(hipSetDevice(s->device));
     if (in->on_device && out->on_device) {
-        or_status st = run_device(s, in, out, true);
-        if (st != OR_OK) return st;
-        // timing is read back lazily by or_session_last_kernel_ms after a sync
-        OR_HIP(hipEventSynchronize(s->ev1));
-        float ms = 0.f;
-        OR_HIP(hipEventElapsedTime(&ms, s->ev0, s->ev1));
-        s->last_ms = ms;
-        return OR_OK;
+        // asynchronous; or_session_last_kernel_ms reads the timing events later
+        return run_device(s, in, out, true);
     }
     if (in->on_device != out->on_device) return OR_ERR_INVALID_ARG;
     // host buffers: stage through device memory
@@ -1639,11 +1953,6 @@ or_status or_reconstruct(or_session* s, const or_hits* in, or_result* out) {
     if (tk_bytes) OR_HIP(hipMemcpyAsync(out->tracks, dout.tracks, tk_bytes, hipMemcpyDeviceToHost, st));
     if (rec_bytes) OR_HIP(hipMemcpyAsync(out->records, dout.records, rec_bytes, hipMemcpyDeviceToHost, st));
     OR_HIP(hipStreamSynchronize(st));
-    if (ne > 0) {
-        float ms = 0.f;
-        OR_HIP(hipEventElapsedTime(&ms, s->ev0, s->ev1));
-        s->last_ms = ms;
-    }
     return OR_OK;
 }
 

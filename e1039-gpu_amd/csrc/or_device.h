@@ -184,6 +184,34 @@ __device__ __forceinline__ void lr_single_xhits(float x0, float tx, int n, const
 }
 
 // ---- left/right of new hits (reconstruction_helper.cuh:714-804) ------------
+// Left/right decision for one adjacent pair (i, j) of a candidate: the
+// hypothesis index h (signs lr_si(h), lr_sj(h)) or -1 when the pair keeps its
+// signs.  Depends only on the pair's hits and the track; lr_newhits applies it.
+__device__ __forceinline__ int lr_pair_decide(float x0, float tx, float y0, float ty, float ex0, float etx, float ey0,
+                                              float ety, int di, int dj, float posi, float posj, float dri, float drj,
+                                              float thr) {
+    const float ct = G(F_COSTHETA, di), st = G(F_SINTHETA, di);
+    const float sexp = ct * tx + st * ty;
+    const float esl = fabsf(ct * etx) + fabsf(st * ety);
+    const float iexp = ct * x0 + st * y0;
+    const float ein = fabsf(ct * ex0) + fabsf(st * ey0);
+    const float zi = G(F_Z, di), zj = G(F_Z, dj);
+    const float smax = G(F_SLOPE_MAX, di), imax = G(F_INTER_MAX, di);
+    int imin = -1;
+    float pmin = 1.e6f;
+    for (int h = 0; h < 4; ++h) {
+        const float pi = position(posi, dri, lr_si(h));
+        const float sl = (pi - position(posj, drj, lr_sj(h))) / (zi - zj);
+        const float it = pi - sl * zi;
+        if (fabsf(sl) > smax || fabsf(it) > imax) continue;
+        const float pull = sqrtf(((sexp - sl) * (sexp - sl)) / esl / esl + ((iexp - it) * (iexp - it)) / ein / ein);
+        if (pull < pmin) { imin = h; pmin = pull; }
+    }
+    return (imin > 0 && pmin < thr) ? imin : -1;
+}
+
+// Pairing walk of the new-hits left/right step (reconstruction_helper.cuh:714-782):
+// steps of two, back by one when the detectors are not adjacent.
 __device__ __forceinline__ void lr_newhits(float x0, float tx, float y0, float ty, float ex0, float etx, float ey0,
                                            float ety, int n, const short* det, const float* pos, const float* drift,
                                            short* sign, float thr) {
@@ -191,26 +219,22 @@ __device__ __forceinline__ void lr_newhits(float x0, float tx, float y0, float t
         const int i = k, j = k + 1;
         if (j >= n || abs(det[i] - det[j]) != 1) { k--; continue; }
         if (sign[i] * sign[j] != 0) continue;
-        const int di = det[i];
-        const float ct = G(F_COSTHETA, di), st = G(F_SINTHETA, di);
-        const float sexp = ct * tx + st * ty;
-        const float esl = fabsf(ct * etx) + fabsf(st * ety);
-        const float iexp = ct * x0 + st * y0;
-        const float ein = fabsf(ct * ex0) + fabsf(st * ey0);
-        const float zi = G(F_Z, di), zj = G(F_Z, det[j]);
-        const float smax = G(F_SLOPE_MAX, di), imax = G(F_INTER_MAX, di);
-        int imin = -1;
-        float pmin = 1.e6f;
-        for (int h = 0; h < 4; ++h) {
-            const float pi = position(pos[i], drift[i], lr_si(h));
-            const float sl = (pi - position(pos[j], drift[j], lr_sj(h))) / (zi - zj);
-            const float it = pi - sl * zi;
-            if (fabsf(sl) > smax || fabsf(it) > imax) continue;
-            const float pull = sqrtf(((sexp - sl) * (sexp - sl)) / esl / esl + ((iexp - it) * (iexp - it)) / ein / ein);
-            if (pull < pmin) { imin = h; pmin = pull; }
-        }
-        if (imin > 0 && pmin < thr) { sign[i] = lr_si(imin); sign[j] = lr_sj(imin); }
+        const int h = lr_pair_decide(x0, tx, y0, ty, ex0, etx, ey0, ety, det[i], det[j], pos[i], pos[j], drift[i],
+                                     drift[j], thr);
+        if (h >= 0) { sign[i] = lr_si(h); sign[j] = lr_sj(h); }
     }
+}
+
+// Index i < n3 that the walk above pairs with an index j >= n3 (the first
+// hit after the station-3 hits), or -1.
+__device__ __forceinline__ int lr_cross_index(int n, int n3, const short* det) {
+    for (int k = 0; k < n; k += 2) {
+        const int i = k, j = k + 1;
+        if (j >= n || abs(det[i] - det[j]) != 1) { k--; continue; }
+        if (i < n3 && j >= n3) return i;
+        if (i >= n3) return -1;
+    }
+    return -1;
 }
 
 __device__ __forceinline__ void lr_single_newhits(float x0, float tx, float y0, float ty, int n, const short* det,

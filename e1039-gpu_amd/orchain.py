@@ -44,6 +44,7 @@ EV_EVENT_TOO_LARGE = 1 << 9
 EV_BAD_HIT = 1 << 10
 EV_OUTPUT_OVERFLOW = 1 << 11
 EV_REJECT_MASK = 0x7FF
+TUNE_YZ_LIST_CAP = 1
 
 PARAM_FIELDS = [
     "max_hits_chamber_plane", "max_hits_hodo_plane", "max_hits_prop_plane", "max_reduced_per_plane",
@@ -129,6 +130,8 @@ def load_library(path: str = LIB_PATH):
     lib.or_session_set_stream.restype = i32
     lib.or_session_last_kernel_ms.argtypes = [vp]
     lib.or_session_last_kernel_ms.restype = C.c_float
+    lib.or_session_set_tuning.argtypes = [vp, C.c_int, i64]
+    lib.or_session_set_tuning.restype = i32
     lib.or_session_destroy.argtypes = [vp]
     lib.or_status_string.argtypes = [i32]
     lib.or_status_string.restype = C.c_char_p
@@ -279,6 +282,10 @@ class Session:
 
     def last_kernel_ms(self) -> float:
         return float(self.lib.or_session_last_kernel_ms(self.handle))
+
+    def set_tuning(self, key: int, value: int):
+        """or_session_set_tuning; key e.g. TUNE_YZ_LIST_CAP."""
+        _check(self.lib.or_session_set_tuning(self.handle, key, value))
 
     # --- device-resident path -------------------------------------------------
     def upload(self, ev: Events):

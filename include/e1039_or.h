@@ -165,9 +165,20 @@ or_status or_session_sync(or_session* s);
 /* Sets the HIP stream (hipStream_t) work is issued on; NULL = session stream. */
 or_status or_session_set_stream(or_session* s, void* stream);
 
-/* Mean device time in ms of the last or_reconstruct's main kernel, measured
- * with HIP events on the stream it ran on. */
+/* Device time in ms of the last or_reconstruct's chain kernels (all size
+ * classes), measured with HIP events on the stream they ran on.  Waits for
+ * that call's kernels to finish. */
 float or_session_last_kernel_ms(const or_session* s);
+
+/* Tuning knobs of a session; results do not depend on them.  No reference
+ * counterpart (the reference hard-codes its launch shapes). */
+typedef enum {
+    /* YZ candidates surviving the first box cut that one chunk of 16
+     * tracklets keeps for parallel evaluation, 1..4096 (default 4096); a chunk
+     * with more is evaluated unit by unit instead */
+    OR_TUNE_YZ_LIST_CAP = 1
+} or_tuning;
+or_status or_session_set_tuning(or_session* s, or_tuning key, int64_t value);
 
 void or_session_destroy(or_session* s);
 const char* or_status_string(or_status st);

@@ -30,6 +30,7 @@
 #include "../include/e1039_or.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <fstream>
@@ -602,6 +603,46 @@ static void run_xz(Ctx& C) {
 }
 
 /* ---- YZ: reconstruction_kernels.cuh:864-1832, one tracklet ---------------- */
+/* ---- check of the YZ sign-state model the device relies on ------------------
+ * Within one (u3, v3) unit the device derives each candidate's starting
+ * station-3 signs instead of carrying them: zero for the unit's first
+ * first-box survivor; afterwards the signs that survivor's left/right left,
+ * with the last station-3 sign replaced by the latest earlier survivor whose
+ * station-3/station-2 cross pair took a left/right decision.  When enabled,
+ * run_yz_one compares that prediction with the carried state before and after
+ * every left/right step.  Counters: checks, mismatches, cross-pair cases. */
+static std::atomic<int> g_yzcheck{0};
+static std::atomic<long long> g_yzcnt[3];
+
+static int yzcheck_pair(float x0, float tx, float y0, float ty, float ex0, float etx, float ey0, float ety, int i,
+                        int j, const short* det, const float* pos, const float* drift, const Geo& g, float thr) {
+    int di = det[i];
+    float sexp = g.costheta[di] * tx + g.sintheta[di] * ty;
+    float esl = std::fabs(g.costheta[di] * etx) + std::fabs(g.sintheta[di] * ety);
+    float iexp = g.costheta[di] * x0 + g.sintheta[di] * y0;
+    float ein = std::fabs(g.costheta[di] * ex0) + std::fabs(g.sintheta[di] * ey0);
+    int imin = -1;
+    float pmin = 1.e6f;
+    for (int h = 0; h < 4; ++h) {
+        float sl = (position(pos[i], drift[i], kLR[h][0]) - position(pos[j], drift[j], kLR[h][1])) /
+                   (g.z[di] - g.z[det[j]]);
+        float it = position(pos[i], drift[i], kLR[h][0]) - sl * g.z[di];
+        if (std::fabs(sl) > g.slope_max[di] || std::fabs(it) > g.inter_max[di]) continue;
+        float pull = std::sqrt(((sexp - sl) * (sexp - sl)) / esl / esl + ((iexp - it) * (iexp - it)) / ein / ein);
+        if (pull < pmin) { imin = h; pmin = pull; }
+    }
+    return (imin > 0 && pmin < thr) ? imin : -1;
+}
+
+static int yzcheck_cross(const short* det, int n, int n3) {  /* i < n3 paired with j >= n3 by the walk, or -1 */
+    for (int k = 0; k < n; k += 2) {
+        int i = k, j = k + 1;
+        if (j >= n || std::abs(det[i] - det[j]) != 1) { k--; continue; }
+        if (i < n3 && j >= n3) return i;
+    }
+    return -1;
+}
+
 static bool run_yz_one(Ctx& C, Tkl& T) {
     const Geo& g = *C.g;
     float* R = T.rec;
@@ -671,6 +712,9 @@ static bool run_yz_one(Ctx& C, Tkl& T) {
         if (lists[3][iv3].b >= 0) add(vdet[3][1], lists[3][iv3].b);
         int nv3h = nuv - nu3h;
         if (nv3h == 0) continue;
+        const int n3 = nu3h + nv3h;
+        int yz_seen = 0;
+        short yz_seed[4] = {0, 0, 0, 0}, yz_last = 0;
         for (int iuv2 = 0; iuv2 < nu2 * nv2; ++iuv2) {
             int iu2 = iuv2 % nu2, iv2 = iuv2 / nu2;
             nuv = nu3h + nv3h;
@@ -686,8 +730,35 @@ static bool run_yz_one(Ctx& C, Tkl& T) {
             fit2d(nuv, Y, Z, eY, f);
             float y0 = f[0], ty = f[1], ey0 = f[2], ety = f[3];
             if (std::fabs(y0) > Y0_MAX + 2 * ey0 || std::fabs(ty) > TY_MAX + 2 * ety) continue;
+            const bool yzc = g_yzcheck.load(std::memory_order_relaxed) != 0;
+            int yz_ov = -1;
+            if (yzc) {
+                bool ok = true;
+                for (int q = 0; q < n3; ++q)
+                    ok &= sign[q] == (yz_seen == 0 ? 0 : (q == n3 - 1 ? yz_last : yz_seed[q]));
+                const int ci = yzcheck_cross(det, nuv, n3);
+                if (ci >= 0) {
+                    yz_ov = yzcheck_pair(x0, tx, y0, ty, ex0, etx, ey0, ety, ci, ci + 1, det, pos, drift, g, 150.f);
+                    g_yzcnt[2]++;
+                }
+                g_yzcnt[0]++;
+                if (!ok) g_yzcnt[1]++;
+            }
             lr_newhits(x0, tx, y0, ty, ex0, etx, ey0, ety, nuv, det, pos, drift, sign, g, 150.f);
             lr_single_newhits(x0, tx, y0, ty, nuv, det, pos, sign, g);
+            if (yzc) {
+                if (yz_seen == 0) {
+                    for (int q = 0; q < n3; ++q) yz_seed[q] = sign[q];
+                    yz_last = sign[n3 - 1];
+                } else {
+                    if (yz_ov >= 0) yz_last = kLR[yz_ov][0];
+                    bool ok = true;
+                    for (int q = 0; q < n3; ++q) ok &= sign[q] == (q == n3 - 1 ? yz_last : yz_seed[q]);
+                    g_yzcnt[0]++;
+                    if (!ok) g_yzcnt[1]++;
+                }
+                yz_seen++;
+            }
             for (int l = 0; l < nuv; ++l) hs[nx + l] = sign[l];
             fit2d(nuv, Y, Z, eY, f); /* identical refit (Q13, reconstruction_kernels.cuh:1628) */
             y0 = f[0]; ty = f[1]; ey0 = f[2]; ety = f[3];
@@ -1105,6 +1176,14 @@ static void run_event(const Geo& g, const or_params& P, const or_hits& in, int64
 
 /* ============================ C ABI of the oracle ============================ */
 extern "C" {
+
+void orc_yz_state_check(int enable, long long* counts /* 3, may be NULL */) {
+    if (counts)
+        for (int i = 0; i < 3; ++i) counts[i] = orc::g_yzcnt[i].load();
+    orc::g_yzcheck.store(enable);
+    if (enable)
+        for (int i = 0; i < 3; ++i) orc::g_yzcnt[i].store(0);
+}
 
 struct orc_geometry { orc::Geo g; float deltaW[55 * 9]; };
 

@@ -50,6 +50,7 @@ def load():
     lib.orc_hodo_xz.restype = C.c_int
     lib.orc_lr_xhits.argtypes = [vp, f32, f32, f32, f32, C.c_int, vp, vp, vp, vp]
     lib.orc_pairs_binned.argtypes = [C.c_int, C.c_int, vp, vp, C.c_int, vp, vp, C.c_int, vp, vp]
+    lib.orc_yz_state_check.argtypes = [C.c_int, vp]
     _lib = lib
     return lib
 

@@ -92,3 +92,24 @@ def test_slot_layout_and_cleaning_outcome(oc, orc, geo, ogeo):
                 assert all(19 <= d <= 24 for d in d3)
             else:
                 assert all(25 <= d <= 30 for d in d3)
+
+
+def test_yz_sign_state_model(oc, orc, geo, ogeo):
+    # The device evaluates YZ candidates in parallel, deriving each one's
+    # starting station-3 signs (unit seed + last writer of the last sign) instead
+    # of carrying them; the oracle carries them and compares at every step.
+    import ctypes as C
+    lib = orc.load()
+    cnt = (C.c_longlong * 3)()
+    lib.orc_yz_state_check(1, None)
+    try:
+        for kw, n in ((dict(mult_mode=1, mult_mean=300.0, mult_cap=1000), 4000), ({}, 3000),
+                      (dict(mult_mode=1, mult_mean=500.0, mult_cap=2000), 600)):
+            ev = oc.generate(geo, n, seed=31, **kw)
+            p = oc.default_params(max_hits_per_event=2048, max_d0_mult=2000, max_d2_mult=2000, max_d3_mult=2000)
+            orc.reconstruct(ogeo, p, ev)
+    finally:
+        lib.orc_yz_state_check(0, C.cast(cnt, C.c_void_p))
+    checks, bad, cross = list(cnt)
+    assert checks > 1_000_000 and cross > 100_000, (checks, cross)
+    assert bad == 0, f"{bad} of {checks} sign states differ from the model"
