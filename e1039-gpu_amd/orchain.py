@@ -253,6 +253,12 @@ class Result:
     n_tracklets: np.ndarray | None = None
     records: np.ndarray | None = None  # float32 (n_events, max_tracklets_per_event, 148)
 
+    def valid_tracks(self) -> np.ndarray:
+        """Tracks of all events, slots [0, n_tracks[e]) of each event in event
+        order; the library leaves the other slots unwritten."""
+        k = np.arange(self.tracks.shape[1])[None, :] < self.n_tracks[:, None]
+        return np.ascontiguousarray(self.tracks[k])
+
     def keep_bits(self, n_hits: int) -> np.ndarray:
         b = np.unpackbits(self.keep_mask.view(np.uint8), bitorder="little")
         return b[:n_hits].astype(bool)

@@ -123,7 +123,7 @@ typedef struct {
     uint32_t* status;        /* n_events: OR_EV_* bits */
     uint32_t* keep_mask;     /* ceil(n_hits/32) words: bit h = input hit h survives the reducer */
     int32_t* n_tracks;       /* n_events: global tracks written */
-    or_track* tracks;        /* n_events * max_tracks_out, slot order */
+    or_track* tracks;        /* n_events * max_tracks_out, slot order; only slots [0, n_tracks[e]) of event e are written */
     int32_t* n_tracklets;    /* optional, n_events: XZ tracklets found */
     float* records;          /* optional, n_events * max_tracklets_per_event * 148 floats:
                                 every tracklet in slot order, reference record layout */

@@ -23,7 +23,14 @@ def assert_same(got, ref, ev, what=""):
                                    f"fields {np.nonzero(g[diff[0][0], diff[1][0]] != r[diff[0][0], diff[1][0]])[0] if diff[0].size else ''}")
     bad = np.nonzero(got.n_tracks != ref.n_tracks)[0]
     assert bad.size == 0, f"{what} n_tracks differ at {bad[:10]}"
-    assert got.tracks.tobytes() == ref.tracks.tobytes(), f"{what} track records differ"
+    same = same_tracks(got, ref)
+    assert same, f"{what} track records differ"
+
+
+def same_tracks(a, b):
+    # slots beyond n_tracks are not written by the library; compare the valid ones
+    return bool(np.array_equal(a.n_tracks, b.n_tracks) and
+                np.array_equal(a.valid_tracks().view(np.uint8), b.valid_tracks().view(np.uint8)))
 
 
 def run_both(oc, orc, session, ogeo, ev, records=True):
@@ -45,7 +52,6 @@ def test_c1_poisson300(oc, orc, geo, ogeo, session):
     assert_same(got, ref, ev, "c1")
 
 
-@pytest.mark.skip(reason="did not finish within 180 s on the GPU (1200-hit events, 2048-hit tile); cause not found yet")
 def test_high_occupancy_reference_caps(oc, orc, geo, ogeo, session):
     # most events breach the station multiplicity cuts; status bits must agree
     ev = oc.generate(geo, 300, seed=12, mult_mode=1, mult_mean=1200.0, mult_cap=2000, noise_occupancy=0.08)
@@ -121,12 +127,15 @@ def test_deterministic_and_batch_independent(oc, geo, session):
     ev = oc.generate(geo, 4000, seed=16, mult_mode=1, mult_mean=300.0, mult_cap=1000)
     a = session.reconstruct(ev)
     b = session.reconstruct(ev)
-    assert a.tracks.tobytes() == b.tracks.tobytes() and np.array_equal(a.status, b.status)
+    same = same_tracks(a, b) and np.array_equal(a.status, b.status)
+    assert same
     # a sub-batch gives the same per-event results as the full batch
     i0, i1 = 1234, 2345
     s = session.reconstruct(ev.subset(i0, i1))
     assert np.array_equal(s.status, a.status[i0:i1])
-    assert s.tracks.tobytes() == a.tracks[i0:i1].tobytes()
+    sub = oc.Result(a.status[i0:i1], a.keep_mask, a.n_tracks[i0:i1], a.tracks[i0:i1])
+    same = same_tracks(s, sub)
+    assert same
     h0, h1 = int(ev.offsets[i0]), int(ev.offsets[i1])
     assert np.array_equal(s.keep_bits(h1 - h0), a.keep_bits(ev.n_hits)[h0:h1])
 
