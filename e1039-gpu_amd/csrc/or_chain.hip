@@ -33,7 +33,8 @@ constexpr int BIN_CAP = 20;   // pairs per XZ bin (globalconsts::MaxHitsProj_X)
 constexpr int WIN_CAP = 100;  // pairs per window list (hitpairs_*[100])
 constexpr int PLANE_CAP = 400;
 constexpr int YZ_CHUNK = 16;  // tracklets whose window lists are built together
-constexpr int YZ_LIST_CAP = 4096;  // YZ candidates passing the first box cut, per chunk
+constexpr int YZ_LIST_CAP = 4096;
+constexpr int NCLASS_MAX = 5;    // event tile classes: 512, 1024, 2048, 4096, 8192 hits  // YZ candidates passing the first box cut, per chunk
 constexpr uint16_t NOHIT = 0xFFFF;
 
 // Tracklet as kept in scratch between stages.
@@ -82,6 +83,7 @@ struct Args {
     int64_t n_events;
     const int32_t* ev_list;     // events of this launch's size class (NULL: all events in order)
     const int32_t* ev_count;    // device count of ev_list
+    int32_t* ev_head;           // work queue head (events handed out beyond the first gridDim.x), zeroed per launch
     const float* wire;          // 55 x OR_MAX_ELEM
     or_params p;
     uint32_t* status;
@@ -1637,6 +1639,18 @@ STAGE void stage_out(const Args& a, const Scr& S, int64_t e, int ntkl) {
     }
 }
 
+// Per-stage cycle counters, compiled in only for the profiling build
+// (-DOR_STAGE_TIMING, libe1039or_timing.so); they add no work to the stages.
+#ifdef OR_STAGE_TIMING
+constexpr int N_STAGE_TIMERS = 7;
+__device__ unsigned long long g_stage_cycles[NCLASS_MAX][N_STAGE_TIMERS];
+#define OR_TIMER_START unsigned long long t_0 = clock64(), t_1;
+#define OR_TIMER(k) t_1 = clock64(); tacc[k] += t_1 - t_0; t_0 = t_1;
+#else
+#define OR_TIMER_START
+#define OR_TIMER(k)
+#endif
+
 template <int MAXH>
 __global__ __launch_bounds__(WAVE, MAXH <= 512 ? 4 : (MAXH <= 1024 ? 2 : 1)) void chain_kernel(Args a) {
     Lds<MAXH>& L = g_lds<MAXH>;
@@ -1653,42 +1667,82 @@ __global__ __launch_bounds__(WAVE, MAXH <= 512 ? 4 : (MAXH <= 1024 ? 2 : 1)) voi
     S.plist = reinterpret_cast<uint32_t*>(scr + SCR_PL);
     S.pst = reinterpret_cast<uchar4*>(scr + SCR_PB);
     const int64_t ne = a.ev_list ? (int64_t)*a.ev_count : a.n_events;
-    for (int64_t i = blockIdx.x; i < ne; i += gridDim.x) {
+#ifdef OR_STAGE_TIMING
+    unsigned long long tacc[N_STAGE_TIMERS] = {0, 0, 0, 0, 0, 0, 0};
+#endif
+    // persistent work queue: the first event of each wave is its block index,
+    // later ones come from an atomic head, so waves that drew cheap events take more
+    for (int64_t i = blockIdx.x; i < ne;) {
         const int64_t e = a.ev_list ? (int64_t)a.ev_list[i] : i;
         const int64_t h0 = a.off[e];
         const int n = (int)(a.off[e + 1] - h0);
+        OR_TIMER_START
         if (lane == 0) L.status = 0;
         if (lane < 56) L.pcnt[lane] = 0;
         __syncthreads();
         const bool staged = stage_load_reduce<MAXH>(a, h0, n);
         if (staged) stage_keep_bits<MAXH>(a, h0, n);
         __syncthreads();
+        OR_TIMER(0)
         int ntkl = 0;
         if (L.status == 0) stage_xpairs<MAXH>(a.p, S.xpairs);
+        OR_TIMER(1)
         if (L.status == 0) ntkl = stage_xz<MAXH>(a.p, S);
+        OR_TIMER(2)
         if (L.status == 0) stage_yz<MAXH>(a.p, S, ntkl, a.yz_list_cap);
+        OR_TIMER(3)
         if (L.status == 0) stage_clean<MAXH>(S.tks, S.slot_order, ntkl);
+        OR_TIMER(4)
         if (L.status == 0) stage_global<MAXH>(a.p, S, ntkl);
+        OR_TIMER(5)
         stage_out<MAXH>(a, S, e, ntkl);
         __syncthreads();
+        OR_TIMER(6)
+        if (a.ev_head) {
+            int next = 0;
+            if (lane == 0) next = atomicAdd(a.ev_head, 1);
+            i = (int64_t)gridDim.x + __shfl(next, 0, WAVE);
+        } else {
+            i += gridDim.x;
+        }
     }
+#ifdef OR_STAGE_TIMING
+    constexpr int cls = MAXH == 512 ? 0 : MAXH == 1024 ? 1 : MAXH == 2048 ? 2 : MAXH == 4096 ? 3 : 4;
+    if (lane == 0)
+        for (int k = 0; k < N_STAGE_TIMERS; ++k) atomicAdd(&g_stage_cycles[cls][k], tacc[k]);
+#endif
 }
 
 // Size classes: an event goes to the smallest LDS tile that holds its hits
 // (the largest class takes the rest and flags them as too large).  The order
 // inside a class list depends on scheduling; outputs are indexed by event, so
 // results do not.
-constexpr int NCLASS = 3;
-__host__ __device__ constexpr int class_maxh(int c) { return c == 0 ? 512 : (c == 1 ? 1024 : 2048); }
+constexpr int NCLASS = NCLASS_MAX;
+constexpr int MAX_EVENT_HITS = 8192;  // largest tile: 8192 hits x 17 B + tables, within the 160 KB of LDS
+__host__ __device__ constexpr int class_maxh(int c) { return 512 << c; }
 
 __global__ void classify_events(const int64_t* off, int64_t n, int nclass, int32_t* lists, int64_t stride,
                                 int32_t* counts) {
-    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t h = off[e + 1] - off[e];
-        int c = 0;
-        while (c < nclass - 1 && h > class_maxh(c)) ++c;
-        const int k = atomicAdd(&counts[c], 1);
-        lists[c * stride + k] = (int32_t)e;
+    // one atomic per wave and class (wave-aggregated), slots by lane rank
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int64_t step = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t base = blockIdx.x * (int64_t)blockDim.x + (threadIdx.x & ~(WAVE - 1)); base < n; base += step) {
+        const int64_t e = base + lane;
+        int c = -1;
+        if (e < n) {
+            const int64_t h = off[e + 1] - off[e];
+            c = 0;
+            while (c < nclass - 1 && h > class_maxh(c)) ++c;
+        }
+        for (int cc = 0; cc < nclass; ++cc) {
+            const unsigned long long m = ballot(c == cc);
+            if (!m) continue;
+            const int leader = __ffsll((long long)m) - 1;
+            int k0 = 0;
+            if (lane == leader) k0 = atomicAdd(&counts[cc], __popcll(m));
+            k0 = __shfl(k0, leader, WAVE);
+            if (c == cc) lists[cc * stride + k0 + rank_in(m)] = (int32_t)e;
+        }
     }
 }
 
@@ -1711,9 +1765,9 @@ struct or_session {
     int grid = 0;
     int maxh = 1024;
     int nclass = 2;
-    int grid_c[3] = {0, 0, 0};
+    int grid_c[NCLASS] = {0, 0, 0, 0, 0};
     int32_t* d_lists = nullptr;   // NCLASS x list_cap event indices
-    int32_t* d_counts = nullptr;  // NCLASS counters
+    int32_t* d_counts = nullptr;  // NCLASS class counts, then NCLASS work queue heads
     int64_t list_cap = 0;
     hipStream_t own = nullptr;
     hipStream_t stream = nullptr;
@@ -1756,7 +1810,9 @@ hipError_t launch_class(int c, const Args& a, int grid, hipStream_t s) {
     switch (c) {
         case 0: return launch_chain<512>(a, grid, s);
         case 1: return launch_chain<1024>(a, grid, s);
-        default: return launch_chain<2048>(a, grid, s);
+        case 2: return launch_chain<2048>(a, grid, s);
+        case 3: return launch_chain<4096>(a, grid, s);
+        default: return launch_chain<8192>(a, grid, s);
     }
 }
 
@@ -1773,7 +1829,7 @@ or_status or_session_create(const or_geometry* g, const or_params* p, int device
     if (p) prm = *p;
     else or_params_default(&prm);
     if (prm.max_pairs_per_bin > BIN_CAP || prm.max_pairs_window > WIN_CAP ||
-        prm.max_tracklets_per_event > MAXTKL || prm.max_hits_per_event > 2048 ||
+        prm.max_tracklets_per_event > MAXTKL || prm.max_hits_per_event > MAX_EVENT_HITS ||
         prm.max_hits_chamber_plane > PLANE_CAP || prm.max_hits_prop_plane > PLANE_CAP ||
         prm.max_hits_hodo_plane > PLANE_CAP || prm.max_pair_plane_hits > 254 || prm.max_reduced_per_plane > 254 ||
         prm.max_tracks_out < 0)
@@ -1782,7 +1838,9 @@ or_status or_session_create(const or_geometry* g, const or_params* p, int device
     if (!s) return OR_ERR_NOMEM;
     s->device = device;
     s->p = prm;
-    s->maxh = prm.max_hits_per_event <= 512 ? 512 : (prm.max_hits_per_event <= 1024 ? 1024 : 2048);
+    s->nclass = 1;
+    while (s->nclass < NCLASS && class_maxh(s->nclass - 1) < prm.max_hits_per_event) ++s->nclass;
+    s->maxh = class_maxh(s->nclass - 1);
     OR_HIP(hipSetDevice(device));
     OR_HIP(hipStreamCreateWithFlags(&s->own, hipStreamNonBlocking));
     s->stream = s->own;
@@ -1793,15 +1851,15 @@ or_status or_session_create(const or_geometry* g, const or_params* p, int device
     OR_HIP(hipMemcpy(s->d_wire, g->wire, sizeof g->wire, hipMemcpyHostToDevice));
     hipDeviceProp_t prop;
     OR_HIP(hipGetDeviceProperties(&prop, device));
-    s->nclass = s->maxh == 512 ? 1 : (s->maxh == 1024 ? 2 : 3);
-    const int occ[3] = {occupancy_blocks<512>(), occupancy_blocks<1024>(), occupancy_blocks<2048>()};
+    const int occ[NCLASS] = {occupancy_blocks<512>(), occupancy_blocks<1024>(), occupancy_blocks<2048>(),
+                             occupancy_blocks<4096>(), occupancy_blocks<8192>()};
     s->grid = 0;
     for (int c = 0; c < s->nclass; ++c) {
         s->grid_c[c] = prop.multiProcessorCount * occ[c];
         s->grid = std::max(s->grid, s->grid_c[c]);
     }
     s->scratch_blocks = s->grid;
-    OR_HIP(hipMalloc(&s->d_counts, sizeof(int32_t) * NCLASS));
+    OR_HIP(hipMalloc(&s->d_counts, sizeof(int32_t) * 2 * NCLASS));
     OR_HIP(hipMalloc(&s->d_scratch, (size_t)SCR_STRIDE * s->scratch_blocks));
     *out = s;
     return OR_OK;
@@ -1845,7 +1903,7 @@ static or_status run_device(or_session* s, const or_hits* in, const or_result* o
     Args a;
     a.det = in->det; a.elem = in->elem; a.tdc = in->tdc; a.drift = in->drift; a.flag = in->flag;
     a.off = in->event_offsets; a.n_events = in->n_events;
-    a.ev_list = nullptr; a.ev_count = nullptr;
+    a.ev_list = nullptr; a.ev_count = nullptr; a.ev_head = nullptr;
     a.wire = s->d_wire;
     a.p = s->p;
     a.status = out->status; a.keep_mask = out->keep_mask; a.n_tracks = out->n_tracks; a.tracks = out->tracks;
@@ -1869,7 +1927,8 @@ static or_status run_device(or_session* s, const or_hits* in, const or_result* o
         OR_HIP(hipMalloc(&s->d_lists, sizeof(int32_t) * NCLASS * (size_t)in->n_events));
         s->list_cap = in->n_events;
     }
-    hipLaunchKernelGGL(zero_u32, dim3(1), dim3(64), 0, s->stream, reinterpret_cast<uint32_t*>(s->d_counts), (int64_t)NCLASS);
+    hipLaunchKernelGGL(zero_u32, dim3(1), dim3(64), 0, s->stream, reinterpret_cast<uint32_t*>(s->d_counts),
+                       (int64_t)(2 * NCLASS));
     OR_HIP(hipGetLastError());
     hipLaunchKernelGGL(classify_events, dim3(std::min<int64_t>(1024, (in->n_events + 255) / 256)), dim3(256), 0,
                        s->stream, in->event_offsets, in->n_events, s->nclass, s->d_lists, s->list_cap, s->d_counts);
@@ -1878,6 +1937,7 @@ static or_status run_device(or_session* s, const or_hits* in, const or_result* o
     for (int c = 0; c < s->nclass; ++c) {
         a.ev_list = s->d_lists + c * s->list_cap;
         a.ev_count = s->d_counts + c;
+        a.ev_head = s->d_counts + NCLASS + c;
         const int grid = (int)std::min<int64_t>(s->grid_c[c], in->n_events);
         OR_HIP(launch_class(c, a, grid, s->stream));
     }
@@ -1970,4 +2030,24 @@ void or_session_destroy(or_session* s) {
     if (s->own) (void)hipStreamDestroy(s->own);
     delete s;
 }
+}
+
+/* Cycles per stage summed over all waves since the last reset, [class][stage]
+ * with stages load+reduce, X pairs, XZ, YZ, cleaning, global, outputs.  Only
+ * the profiling build (-DOR_STAGE_TIMING) counts; the product build returns
+ * OR_ERR_INVALID_ARG. */
+extern "C" or_status or_debug_stage_cycles(unsigned long long* out, int reset) {
+    if (!out) return OR_ERR_INVALID_ARG;
+#ifdef OR_STAGE_TIMING
+    if (hipMemcpyFromSymbol(out, HIP_SYMBOL(g_stage_cycles), sizeof(g_stage_cycles)) != hipSuccess) return OR_ERR_DEVICE;
+    if (reset) {
+        static const unsigned long long zero[NCLASS_MAX][N_STAGE_TIMERS] = {};
+        if (hipMemcpyToSymbol(HIP_SYMBOL(g_stage_cycles), zero, sizeof(zero)) != hipSuccess) return OR_ERR_DEVICE;
+    }
+    return OR_OK;
+#else
+    (void)reset;
+    for (int i = 0; i < 35; ++i) out[i] = 0;
+    return OR_ERR_INVALID_ARG;
+#endif
 }

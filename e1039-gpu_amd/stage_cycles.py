@@ -1,0 +1,55 @@
+"""Cycles per chain stage on the GPU, from the profiling build.
+
+    make -C e1039-gpu_amd/csrc timing
+    python e1039-gpu_amd/stage_cycles.py [n_events] [mean_hits]
+
+Runs one warm-up and one measured launch of the chain on seeded events
+(hits/event ~ Poisson(mean_hits) capped at 1000) through libe1039or_timing.so
+and prints each stage's share of the summed wave cycles.
+"""
+import ctypes as C
+import os
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+STAGES = ["load+reduce", "X pairs", "XZ", "YZ", "cleaning", "global", "outputs"]
+
+
+def main():
+    import numpy as np
+    import torch
+
+    import orchain as oc
+
+    n = int(sys.argv[1]) if len(sys.argv) > 1 else 200_000
+    mean = float(sys.argv[2]) if len(sys.argv) > 2 else 300.0
+    lib = oc.load_library(os.path.join(HERE, "libe1039or_timing.so"))
+    lib.or_debug_stage_cycles.argtypes = [C.c_void_p, C.c_int]
+    lib.or_debug_stage_cycles.restype = C.c_int32
+    geo = oc.Geometry.load()
+    ev = oc.generate(geo, n, seed=7, mult_mode=1, mult_mean=mean, mult_cap=1000)
+    sess = oc.Session(geo, oc.default_params(max_hits_per_event=1024), device=0)
+    din = sess.upload(ev)
+    dout = sess.alloc_outputs(ev.n_events, ev.n_hits, tracklets=True)
+    sess.run_device(din, dout)
+    torch.cuda.synchronize()
+    buf = (C.c_ulonglong * 35)()
+    assert lib.or_debug_stage_cycles(buf, 1) == 0, "not the profiling build"
+    sess.run_device(din, dout)
+    torch.cuda.synchronize()
+    print(f"{n} events, Poisson({mean:g}) hits/event: chain kernels {sess.last_kernel_ms():.1f} ms")
+    lib.or_debug_stage_cycles(buf, 0)
+    a = np.array(list(buf), dtype=np.float64).reshape(5, 7)
+    for c in range(5):
+        tot = a[c].sum()
+        if tot == 0:
+            continue
+        print(f"tile class {512 << c} hits: {tot:.3e} cycles")
+        for k, name in enumerate(STAGES):
+            print(f"  {name:12s} {a[c, k] / tot * 100:5.1f} %  {a[c, k] / n:12.0f} cycles/event")
+
+
+if __name__ == "__main__":
+    main()

@@ -80,7 +80,7 @@ typedef struct {
     int32_t max_d2_mult;              /* 120  (reconstruction_constants.h:189) */
     int32_t max_d3_mult;              /* 90   (reconstruction_constants.h:190) */
     int32_t max_tracks_out;           /* output records per event (no reference counterpart) */
-    int32_t max_hits_per_event;       /* per-event bound the device tiles are sized for */
+    int32_t max_hits_per_event;       /* per-event bound the device tiles are sized for, <= 8192; larger events get OR_EV_EVENT_TOO_LARGE */
 } or_params;
 
 /* Input hits, structure of arrays, events contiguous.  Field meaning follows
@@ -179,6 +179,13 @@ typedef enum {
     OR_TUNE_YZ_LIST_CAP = 1
 } or_tuning;
 or_status or_session_set_tuning(or_session* s, or_tuning key, int64_t value);
+
+/* Profiling: device cycles per chain stage summed over all waves since the last
+ * reset, out[class * 7 + stage] for the 5 tile classes (512 ... 8192 hits) and
+ * the stages load+reduce, X pairs, XZ, YZ, cleaning, global, outputs.  Counts
+ * only in the profiling build libe1039or_timing.so; libe1039or.so returns
+ * OR_ERR_INVALID_ARG.  No reference counterpart. */
+or_status or_debug_stage_cycles(unsigned long long* out /* 35 */, int reset);
 
 void or_session_destroy(or_session* s);
 const char* or_status_string(or_status st);

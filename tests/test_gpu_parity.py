@@ -114,6 +114,25 @@ def test_edge_cases(oc, orc, geo, ogeo, session):
     assert_same(got, ref, ev, "edge cases")
 
 
+def test_large_tiles(oc, orc, geo, ogeo):
+    # 2048-, 4096- and 8192-hit tiles and the too-large flag above 8192; with the
+    # reference caps these events stop at the station multiplicity cuts, after
+    # the reducer (whose keep bits are compared)
+    params = oc.default_params(max_hits_per_event=8192)
+    sess = oc.Session(geo, params, device=0)
+    parts = [oc.generate(geo, 20, seed=18 + k, mult_mode=1, mult_mean=m, mult_cap=9000)
+             for k, m in enumerate((1800.0, 3500.0, 8190.0))]
+    cat = lambda f: np.concatenate([getattr(x, f) for x in parts])
+    offs = np.concatenate([[0], np.cumsum([x.n_hits for x in parts])]).astype(np.int64)
+    ev = oc.Events(cat("det"), cat("elem"), cat("tdc"), cat("drift"), cat("flag"), offs)
+    counts = np.diff(ev.offsets)
+    for lo, hi in ((1024, 2048), (2048, 4096), (4096, 8192), (8192, 10**9)):
+        assert ((counts > lo) & (counts <= hi)).sum() > 3
+    got, ref = run_both(oc, orc, sess, ogeo, ev)
+    assert (ref.status & oc.EV_EVENT_TOO_LARGE).sum() == (counts > 8192).sum()
+    assert_same(got, ref, ev, "large tiles")
+
+
 def test_event_too_large(oc, orc, geo, ogeo):
     params = oc.default_params(max_hits_per_event=512)
     sess = oc.Session(geo, params, device=0)
