@@ -34,7 +34,25 @@ constexpr int WIN_CAP = 100;  // pairs per window list (hitpairs_*[100])
 constexpr int PLANE_CAP = 400;
 constexpr int YZ_CHUNK = 16;  // tracklets whose window lists are built together
 constexpr int YZ_LIST_CAP = 4096;
-constexpr int NCLASS_MAX = 5;    // event tile classes: 512, 1024, 2048, 4096, 8192 hits  // YZ candidates passing the first box cut, per chunk
+constexpr int NCLASS_MAX = 5;    // event tile classes: 512, 1024, 2048, 4096, 8192 hits
+
+// Sub-stage cycle counters of the profiling build (-DOR_STAGE_TIMING); no-ops
+// otherwise.  Indices: YZ window lists, YZ pass 1, 2, 3, YZ winners, YZ unit
+// path, global window lists, global x/uv loop, global write.
+constexpr int N_SUB_TIMERS = 16;
+#ifdef OR_STAGE_TIMING
+__device__ unsigned long long g_sub_cycles[N_SUB_TIMERS];
+#define OR_SUB_START unsigned long long s_0 = clock64(), s_1;
+#define OR_SUB(k)                                                              \
+    do {                                                                       \
+        s_1 = clock64();                                                       \
+        if (threadIdx.x == 0) atomicAdd(&g_sub_cycles[k], s_1 - s_0);          \
+        s_0 = s_1;                                                             \
+    } while (0)
+#else
+#define OR_SUB_START
+#define OR_SUB(k) do { } while (0)
+#endif  // YZ candidates passing the first box cut, per chunk
 constexpr uint16_t NOHIT = 0xFFFF;
 
 // Tracklet as kept in scratch between stages.
@@ -66,7 +84,10 @@ struct Lds {
     uint8_t korig[MAXH];      // kept flag by input order
     int pcnt[56], pstart[56], cursor[56], rcnt[56], rstart[56];
     int xcnt[3][28];
-    int nl[YZ_CHUNK][4];
+    union {
+        int nl[YZ_CHUNK][4];      // YZ / global: window list lengths
+        uint16_t live[WAVE];      // cleaning: tracklets at stationID >= 5, slot order
+    };
     int uoff[YZ_CHUNK + 1];   // prefix of (u3, v3) units or of candidates per tracklet of the chunk
     unsigned long long best[YZ_CHUNK];
     int sstart[33];
@@ -227,59 +248,59 @@ __device__ __forceinline__ int prop_hit(const Ev<MAXH>& E, int da, int db, float
 // ---- window pair list (make_hitpairs_in_station, reconstruction_helper.cuh:464-549)
 // Returns the number of pairs, or -1 when a cap is exceeded.
 template <int MAXH>
-__device__ __attribute__((noinline)) int pairs_window(const Ev<MAXH>& E, int st, int proj, float xmin, float xmax, uint16_t* out, int cap_plane,
-                            int cap_pairs) {
+__device__ __attribute__((noinline)) int pairs_window(const Ev<MAXH>& E, int st, int proj, float xmin, float xmax,
+                                                      uint16_t* out, int cap_plane, int cap_pairs) {
     const int sid = super_id(st, proj);
     const int d1 = sid * 2, d2 = sid * 2 - 1;
     const int n1 = E.nred(d1), n2 = E.nred(d2);
     const float dx1 = G(F_DELTAPX, d1), dx2 = G(F_DELTAPX, d2);
-    // in-window hits, as bit lists over the plane (n <= PLANE_CAP)
-    uint8_t w1[PLANE_CAP / 8 + 1], w2[PLANE_CAP / 8 + 1];
+    // wire span inside [xmin, xmax]; the first 64 hits of a plane keep the answer in a bit mask
+    auto inwin = [&](int d, float dx, int i) {
+        const int e = E.Lp->elem[E.raw(d, i)];
+        const float b = x_bep(d, e), t = x_bep(d, e) + dx;
+        const float lo = dx > 0 ? b : t, hi = dx > 0 ? t : b;
+        return lo <= xmax && hi >= xmin;
+    };
+    unsigned long long w1 = 0, w2 = 0;
     int c1 = 0, c2 = 0;
-    for (int i = 0; i < n1; ++i) {
-        const int e = E.Lp->elem[E.raw(d1, i)];
-        const float b = x_bep(d1, e), t = x_bep(d1, e) + dx1;
-        const float a = dx1 > 0 ? b : t, c = dx1 > 0 ? t : b;
-        const bool in = a <= xmax && c >= xmin;
-        if ((i & 7) == 0) w1[i >> 3] = 0;
-        if (in) { w1[i >> 3] |= (uint8_t)(1u << (i & 7)); ++c1; }
-    }
-    for (int i = 0; i < n2; ++i) {
-        const int e = E.Lp->elem[E.raw(d2, i)];
-        const float b = x_bep(d2, e), t = x_bep(d2, e) + dx2;
-        const float a = dx2 > 0 ? b : t, c = dx2 > 0 ? t : b;
-        const bool in = a <= xmax && c >= xmin;
-        if ((i & 7) == 0) w2[i >> 3] = 0;
-        if (in) { w2[i >> 3] |= (uint8_t)(1u << (i & 7)); ++c2; }
-    }
+    for (int i = 0; i < n1; ++i)
+        if (inwin(d1, dx1, i)) { ++c1; if (i < 64) w1 |= 1ull << i; }
+    for (int i = 0; i < n2; ++i)
+        if (inwin(d2, dx2, i)) { ++c2; if (i < 64) w2 |= 1ull << i; }
     if (c1 > cap_plane || c2 > cap_plane) return -1;
+    auto in1 = [&](int i) { return i < 64 ? ((w1 >> i) & 1ull) != 0 : inwin(d1, dx1, i); };
+    auto in2 = [&](int i) { return i < 64 ? ((w2 >> i) & 1ull) != 0 : inwin(d2, dx2, i); };
     const float sp = spacing_plane(sid);
-    uint8_t f1[PLANE_CAP / 8 + 1], f2[PLANE_CAP / 8 + 1];
-    for (int i = 0; i < (n1 + 7) / 8; ++i) f1[i] = 0;
-    for (int i = 0; i < (n2 + 7) / 8; ++i) f2[i] = 0;
     int np = 0;
     for (int i = 0; i < n1; ++i) {
-        if (!((w1[i >> 3] >> (i & 7)) & 1)) continue;
+        if (!in1(i)) continue;
         const float pi = E.Lp->pos[E.raw(d1, i)];
         for (int j = 0; j < n2; ++j) {
-            if (!((w2[j >> 3] >> (j & 7)) & 1)) continue;
+            if (!in2(j)) continue;
             if (fabsf(pi - E.Lp->pos[E.raw(d2, j)]) > sp) continue;
             if (np >= cap_pairs) return -1;
             out[np++] = (uint16_t)(i | (j << 8));
-            f1[i >> 3] |= (uint8_t)(1u << (i & 7));
-            f2[j >> 3] |= (uint8_t)(1u << (j & 7));
         }
     }
-    for (int i = 0; i < n1; ++i)
-        if (((w1[i >> 3] >> (i & 7)) & 1) && !((f1[i >> 3] >> (i & 7)) & 1)) {
-            if (np >= cap_pairs) return -1;
-            out[np++] = (uint16_t)(i | (0xFF << 8));
-        }
-    for (int j = 0; j < n2; ++j)
-        if (((w2[j >> 3] >> (j & 7)) & 1) && !((f2[j >> 3] >> (j & 7)) & 1)) {
-            if (np >= cap_pairs) return -1;
-            out[np++] = (uint16_t)(0xFF | (j << 8));
-        }
+    // hits in no pair: X (a) then X' (b)
+    for (int i = 0; i < n1; ++i) {
+        if (!in1(i)) continue;
+        const float pi = E.Lp->pos[E.raw(d1, i)];
+        bool paired = false;
+        for (int j = 0; j < n2 && !paired; ++j) paired = in2(j) && !(fabsf(pi - E.Lp->pos[E.raw(d2, j)]) > sp);
+        if (paired) continue;
+        if (np >= cap_pairs) return -1;
+        out[np++] = (uint16_t)(i | (0xFF << 8));
+    }
+    for (int j = 0; j < n2; ++j) {
+        if (!in2(j)) continue;
+        const float pj = E.Lp->pos[E.raw(d2, j)];
+        bool paired = false;
+        for (int i = 0; i < n1 && !paired; ++i) paired = in1(i) && !(fabsf(E.Lp->pos[E.raw(d1, i)] - pj) > sp);
+        if (paired) continue;
+        if (np >= cap_pairs) return -1;
+        out[np++] = (uint16_t)(0xFF | (j << 8));
+    }
     return np;
 }
 
@@ -289,37 +310,63 @@ __device__ __forceinline__ int pb(uint16_t q) { int b = q >> 8; return b == 0xFF
 // ---------------------------------------------------------------------------
 // XZ candidate (reconstruction_kernels.cuh:465-672)
 template <int MAXH>
-__device__ __attribute__((noinline)) bool xz_candidate(const Ev<MAXH>& E, int s3, uint16_t q2, uint16_t q3, Tk& T) {
-    const int d3x = s3 == 1 ? 22 : 28, d3xp = d3x - 1;
-    short det[4], sign[4];
+__device__ __forceinline__ bool xz_candidate(const Ev<MAXH>& E, int s3, uint16_t q2, uint16_t q3, Tk* out) {
+    // hits in fixed slots D2 X, X', D3 X, X' (the reference's list order); see lr_slots_new
+    const int d3x = s3 == 1 ? 22 : 28;
+    int det[4] = {16, 15, d3x, d3x - 1};
+    const int idx[4] = {pa(q2), pb(q2), pa(q3), pb(q3)};
+    const unsigned pres = (idx[0] >= 0 ? 1u : 0u) | (idx[1] >= 0 ? 2u : 0u) | (idx[2] >= 0 ? 4u : 0u) |
+                          (idx[3] >= 0 ? 8u : 0u);
+    if (!(pres & 3u) || !(pres & 12u)) return false;
+    int sign[4], raw[4];
     float X[4], Z[4], ex[4], drift[4];
-    uint16_t hr[4];
-    int n = 0;
-    auto add = [&](int d, int k) {
-        const int r = E.raw(d, k);
-        det[n] = (short)d;
-        ex[n] = G(F_RESOLUTION, d);
-        drift[n] = E.Lp->drift[r];
-        sign[n] = 0;
-        hr[n] = (uint16_t)r;
-        xz_fy(d, E.Lp->elem[r], drift[n], 0, 0.f, 0.f, X[n], Z[n]);
-        ++n;
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        sign[s] = 0;
+        raw[s] = 0;
+        X[s] = Z[s] = drift[s] = 0.f;
+        ex[s] = G(F_RESOLUTION, det[s]);
+        if ((pres >> s) & 1u) {
+            const int r = E.raw(det[s], idx[s]);
+            raw[s] = r;
+            drift[s] = E.Lp->drift[r];
+            xz_fy(det[s], E.Lp->elem[r], drift[s], 0, 0.f, 0.f, X[s], Z[s]);
+        }
+    }
+    auto fit = [&](const float (&x)[4]) {
+        float A0 = 0.f, A1 = 0.f, A3 = 0.f, B0 = 0.f, B1 = 0.f;
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            if (!((pres >> s) & 1u)) continue;
+            B0 += ex[s] * x[s];
+            B1 += (ex[s] * x[s]) * Z[s];
+            A0 += ex[s];
+            A1 += ex[s] * Z[s];
+            A3 += (ex[s] * Z[s]) * Z[s];
+        }
+        return fit_finish(A0, A1, A3, B0, B1);
     };
-    if (pa(q2) >= 0) add(16, pa(q2));
-    if (pb(q2) >= 0) add(15, pb(q2));
-    const int n2 = n;
-    if (n2 == 0) return false;
-    if (pa(q3) >= 0) add(d3x, pa(q3));
-    if (pb(q3) >= 0) add(d3xp, pb(q3));
-    if (n == n2) return false;
-    Fit f = fit2d<4>(n, X, Z, ex);
+    Fit f = fit(X);
     if (fabsf(f.p0) > X0_MAX + 2.f * f.e0 || fabsf(f.p1) > TX_MAX + 2.f * f.e1) return false;
-    lr_xhits<4>(f.p0, f.p1, f.e0, f.e1, n, det, X, drift, sign, 150.f);
-    lr_single_xhits(f.p0, f.p1, n, det, X, sign);
+    // left/right of x pairs (x-only variant), then single hits
+    bool used = false;
+#pragma unroll
+    for (int s = 0; s < 3; ++s) {
+        const bool start = !used && ((pres >> s) & 1u) && ((pres >> (s + 1)) & 1u) && abs(det[s] - det[s + 1]) == 1;
+        if (start) {
+            const int h = lr_xpair_decide(f.p0, f.p1, f.e0, f.e1, det[s], det[s + 1], X[s], X[s + 1], drift[s],
+                                          drift[s + 1], 150.f);
+            if (h >= 0) { sign[s] = lr_si(h); sign[s + 1] = lr_sj(h); }
+        }
+        used = start;
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+        if (((pres >> s) & 1u) && sign[s] == 0) sign[s] = G(F_Z, det[s]) * f.p1 + f.p0 > X[s] ? 1 : -1;
     float xc[4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) xc[i] = i < n ? X[i] + drift[i] * (float)sign[i] : 0.f;
-    f = fit2d<4>(n, xc, Z, ex);
+    for (int s = 0; s < 4; ++s) xc[s] = X[s] + drift[s] * (float)sign[s];
+    f = fit(xc);
     const float x0 = f.p0, tx = f.p1, ex0 = f.e0, etx = f.e1;
     if (fabsf(x0) > X0_MAX || fabsf(tx) > TX_MAX) return false;
     // prop X: P1X (50 then 49) within 5.08, P2X (52 then 51) within 7.62
@@ -328,66 +375,52 @@ __device__ __attribute__((noinline)) bool xz_candidate(const Ev<MAXH>& E, int s3
     if (!hodo_xz(E, 1, 37, x0, tx, ex0, etx) && !hodo_xz(E, 1, 38, x0, tx, ex0, etx)) return false;
     if (!hodo_xz(E, 2, 39, x0, tx, ex0, etx) && !hodo_xz(E, 2, 40, x0, tx, ex0, etx)) return false;
     if (!hodo_xz(E, 3, 45, x0, tx, ex0, etx) && !hodo_xz(E, 3, 46, x0, tx, ex0, etx)) return false;
-    T.x0 = x0; T.tx = tx; T.ex0 = ex0; T.etx = etx;
-    T.y0 = T.ty = T.ey0 = T.ety = 0.f;
-    T.chi2 = T.invp = T.einvp = 0.f;
-    T.nx = (uint8_t)n; T.nuv = 0; T.n1 = 0;
-    T.stid = 0; T.charge = 0;
-    for (int h = 0; h < 18; ++h) {
-        T.resid[h] = 0.f;
-        if (h < 4) T.xpos[h] = h < n ? X[h] : 0.f;
-        T.hraw[h] = h < n ? hr[h] : NOHIT;
-        T.det[h] = h < n ? (uint8_t)det[h] : 0;
-        T.sign[h] = h < n ? (int8_t)sign[h] : 0;
+    if (out) {
+        Tk& T = *out;
+        T.x0 = x0; T.tx = tx; T.ex0 = ex0; T.etx = etx;
+        T.y0 = T.ty = T.ey0 = T.ety = 0.f;
+        T.chi2 = T.invp = T.einvp = 0.f;
+        T.nuv = 0; T.n1 = 0;
+        T.stid = 0; T.charge = 0;
+        int n = 0;
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            if (!((pres >> s) & 1u)) continue;
+            T.xpos[n] = X[s];
+            T.hraw[n] = (uint16_t)raw[s];
+            T.det[n] = (uint8_t)det[s];
+            T.sign[n] = (int8_t)sign[s];
+            ++n;
+        }
+        T.nx = (uint8_t)n;
+        for (int h = 0; h < 18; ++h) {
+            T.resid[h] = 0.f;
+            if (h >= n) {
+                if (h < 4) T.xpos[h] = 0.f;
+                T.hraw[h] = NOHIT;
+                T.det[h] = 0;
+                T.sign[h] = 0;
+            }
+        }
     }
     return true;
 }
 
 // ---------------------------------------------------------------------------
-// YZ unit: one tracklet, one (u3, v3) pair, inner loop over (u2, v2) pairs in
-// reference order (reconstruction_kernels.cuh:1320-1800).
+// YZ candidates (reconstruction_kernels.cuh:1320-1800).
 //  * The y of every UV hit depends only on the hit and the tracklet's (x0, tx)
 //    (calculate_y_uvhit with sign 0), so it is computed once per window-list
-//    entry (stage E) and read here.
-//  * The fit sums run over hits in the order u3, u3', v3, v3', u2, u2', v2, v2';
-//    the station-3 part is summed once per unit and continued per inner pair,
-//    which is the same sequence of float additions.
+//    entry and read here.
+//  * The first fit runs over hits in the order u3, u3', v3, v3', u2, u2', v2,
+//    v2' (the same sequence of float additions as the reference).
 //  * The refit after left/right (reconstruction_kernels.cuh:1628) has the same
 //    inputs as the first fit and is not repeated.
-//  * Station-3 signs set by the left/right step persist across the inner loop,
-//    as in the reference.
-// search: returns the unit's best (chi2, i_uv2).  replay (stop >= 0): writes
-// the candidate at i_uv2 == stop into the tracklet.
 struct YzLists {
-    const uint16_t* l[4];   // pair entries (a | b << 8)
+    const uint16_t* l[4];   // pair entries (a | b << 8), views u2, v2, u3, v3
     const float4* y[4];     // (y_a, ey_a, y_b, ey_b)
     const uint8_t* f[4];    // bit 0: hit a present and accepted, bit 1: hit b
     int n[4];
 };
-
-// Hits of one candidate in the reference order u3, u3', v3, v3', u2, u2', v2, v2'.
-// Kept in the lane's private memory; the station-3 part and its signs persist
-// across the inner loop of a unit.
-struct YzHits {
-    short det[8], sign[8];
-    float pos[8], drift[8];
-    uint16_t hr[8];
-};
-
-template <int MAXH>
-__device__ __forceinline__ void yz_put(const Ev<MAXH>& E, YzHits& H, int& n, const YzLists& W, int v, int k, int which,
-                                       int d) {
-    const uint16_t q = W.l[v][k];
-    const int idx = which ? pb(q) : pa(q);
-    if (idx < 0 || !((W.f[v][k] >> which) & 1)) return;
-    const int r = E.raw(d, idx);
-    H.det[n] = (short)d;
-    H.pos[n] = E.Lp->pos[r];
-    H.drift[n] = E.Lp->drift[r];
-    H.hr[n] = (uint16_t)r;
-    H.sign[n] = 0;
-    ++n;
-}
 
 // fit sums of one list entry's accepted hits, in order a then b
 __device__ __forceinline__ int yz_sum(const YzLists& W, int v, int k, float za, float zb, float& A0, float& A1,
@@ -405,23 +438,68 @@ __device__ __forceinline__ int yz_sum(const YzLists& W, int v, int k, float za, 
     return c;
 }
 
-// The part of a YZ candidate after the first box cut: left/right on all UV hits
-// (station-3 signs in H persist), second box cut, prop-tube Y and hodoscope
-// matches, chi2.  Returns false if the candidate is rejected.  With Tw set the
-// candidate is written into the tracklet.
+// Hits of one candidate in fixed slots u3, u3', v3, v3', u2, u2', v2, v2' (the
+// reference's list order with absent hits left out); bit s of pres = slot s
+// holds a hit.  Only compile-time slot indices are used, so this stays in
+// registers.
+struct YzSlots {
+    unsigned pres;
+    int det[8], sign[8], raw[8];
+    float pos[8], drift[8];
+};
+
+__device__ __forceinline__ void yz_dets(YzSlots& H, int st3) {
+    const int su3 = super_id(st3, 1) * 2, sv3 = super_id(st3, 2) * 2;
+    const int su2 = super_id(2, 1) * 2, sv2 = super_id(2, 2) * 2;
+    H.det[0] = su3; H.det[1] = su3 - 1; H.det[2] = sv3; H.det[3] = sv3 - 1;
+    H.det[4] = su2; H.det[5] = su2 - 1; H.det[6] = sv2; H.det[7] = sv2 - 1;
+}
+
+// accepted hits of entry k of view v into slots S0 (hit a) and S0 + 1 (hit b), signs 0
+template <int S0, int MAXH>
+__device__ __forceinline__ void yz_fill(const Ev<MAXH>& E, const YzLists& W, int v, int k, YzSlots& H) {
+    const uint16_t q = W.l[v][k];
+    const unsigned fl = W.f[v][k];
+    H.pres &= ~(3u << S0);
+    H.sign[S0] = 0;
+    H.sign[S0 + 1] = 0;
+    if (fl & 1u) {
+        const int r = E.raw(H.det[S0], pa(q));
+        H.pres |= 1u << S0;
+        H.raw[S0] = r; H.pos[S0] = E.Lp->pos[r]; H.drift[S0] = E.Lp->drift[r];
+    }
+    if (fl & 2u) {
+        const int r = E.raw(H.det[S0 + 1], pb(q));
+        H.pres |= 2u << S0;
+        H.raw[S0 + 1] = r; H.pos[S0 + 1] = E.Lp->pos[r]; H.drift[S0 + 1] = E.Lp->drift[r];
+    }
+}
+
+// slots where the pairing walk starts a pair (see lr_slots_new)
+template <int NS>
+__device__ __forceinline__ unsigned lr_slot_starts(unsigned pres, const int (&det)[NS]) {
+    unsigned m = 0;
+    bool used = false;
+#pragma unroll
+    for (int s = 0; s + 1 < NS; ++s) {
+        const bool start = !used && ((pres >> s) & 1u) && ((pres >> (s + 1)) & 1u) && abs(det[s] - det[s + 1]) == 1;
+        if (start) m |= 1u << s;
+        used = start;
+    }
+    return m;
+}
+
+// The rest of a candidate after the first box cut: left/right on the UV hits
+// (station-3 signs in H as they stand), second box cut, prop-tube Y and
+// hodoscope matches, chi2 over the x hits (stored signs) and the UV hits
+// (reconstruction_helper.cuh:1182-1204).  With Tw set the candidate is written
+// into the tracklet.
 template <int MAXH>
-__device__ __attribute__((noinline)) bool yz_tail(const Ev<MAXH>& E, const Tk* Tp, const YzLists& W, YzHits& H,
-                                                 int n3, int iu2, int iv2, int su2, int sv2, float y0, float ty,
-                                                 float ey0, float ety, float& chi2_out, Tk* Tw) {
+__device__ __forceinline__ bool yz_tail(const Ev<MAXH>& E, const Tk* Tp, YzSlots& H, const Fit& f, float& chi2_out,
+                                        Tk* Tw) {
     const float x0 = Tp->x0, tx = Tp->tx, ex0 = Tp->ex0, etx = Tp->etx;
-    const int nx = Tp->nx;
-    int n = n3;
-    yz_put(E, H, n, W, 0, iu2, 0, su2);
-    yz_put(E, H, n, W, 0, iu2, 1, su2 - 1);
-    yz_put(E, H, n, W, 1, iv2, 0, sv2);
-    yz_put(E, H, n, W, 1, iv2, 1, sv2 - 1);
-    lr_newhits(x0, tx, y0, ty, ex0, etx, ey0, ety, n, H.det, H.pos, H.drift, H.sign, 150.f);
-    lr_single_newhits(x0, tx, y0, ty, n, H.det, H.pos, H.sign);
+    const float y0 = f.p0, ty = f.p1, ey0 = f.e0, ety = f.e1;
+    lr_slots_new<8>(x0, tx, y0, ty, ex0, etx, ey0, ety, H.pres, H.det, H.pos, H.drift, H.sign, 150.f);
     if (fabsf(y0) > Y0_MAX || fabsf(ty) > TY_MAX) return false;
     // prop Y: P1Y (48 then 47) within 5.08, P2Y (54 then 53) within 10.16
     const int nprop = prop_hit(E, 48, 47, ty, y0, 5.08f) + prop_hit(E, 54, 53, ty, y0, 10.16f);
@@ -435,48 +513,47 @@ __device__ __attribute__((noinline)) bool yz_tail(const Ev<MAXH>& E, const Tk* T
     if (!hodo_tkl(E, 3, 45, x0, y0, tx, ty, ex0, ey0, etx, ety) &&
         !hodo_tkl(E, 3, 46, x0, y0, tx, ty, ex0, ey0, etx, ety))
         return false;
-    // chi2 over x hits (stored signs) and UV hits (reconstruction_helper.cuh:1182-1204)
     float chi2 = 0.f;
-    for (int k = 0; k < nx + n; ++k) {
-        int d, e;
-        float dr;
-        short sg;
-        if (k < nx) { d = Tp->det[k]; e = E.Lp->elem[Tp->hraw[k]]; dr = E.Lp->drift[Tp->hraw[k]]; sg = Tp->sign[k]; }
-        else { d = H.det[k - nx]; e = E.Lp->elem[H.hr[k - nx]]; dr = H.drift[k - nx]; sg = H.sign[k - nx]; }
+    const int nx = Tp->nx;
+    for (int k = 0; k < nx; ++k) {
+        const int d = Tp->det[k], r = Tp->hraw[k];
         const float res = G(F_RESOLUTION, d);
-        const float rs = dr * (float)sg - dca_wire(d, e, x0, y0, tx, ty);
+        const float rs = E.Lp->drift[r] * (float)Tp->sign[k] - dca_wire(d, E.Lp->elem[r], x0, y0, tx, ty);
         chi2 += ((rs * rs) / res) / res;
         if (Tw) Tw->resid[k] = rs;
     }
+    int m = nx;
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+        if (!((H.pres >> s) & 1u)) continue;
+        const int d = H.det[s];
+        const float res = G(F_RESOLUTION, d);
+        const float rs = H.drift[s] * (float)H.sign[s] - dca_wire(d, E.Lp->elem[H.raw[s]], x0, y0, tx, ty);
+        chi2 += ((rs * rs) / res) / res;
+        if (Tw) {
+            Tw->resid[m] = rs;
+            Tw->hraw[m] = (uint16_t)H.raw[s];
+            Tw->det[m] = (uint8_t)d;
+            Tw->sign[m] = (int8_t)H.sign[s];
+        }
+        ++m;
+    }
     if (Tw) {
         Tw->stid = 5;
-        Tw->nuv = (uint8_t)n;
+        Tw->nuv = (uint8_t)(m - nx);
         Tw->chi2 = chi2; Tw->y0 = y0; Tw->ty = ty; Tw->ey0 = ey0; Tw->ety = ety;
-        for (int k = 0; k < n; ++k) {
-            Tw->hraw[nx + k] = H.hr[k];
-            Tw->det[nx + k] = (uint8_t)H.det[k];
-            Tw->sign[nx + k] = (int8_t)H.sign[k];
-        }
     }
     chi2_out = chi2;
     return true;
 }
 
-// YZ unit: one tracklet, one (u3, v3) pair, inner loop over (u2, v2) pairs in
-// reference order (reconstruction_kernels.cuh:1320-1800).
-//  * The y of every UV hit depends only on the hit and the tracklet's (x0, tx)
-//    (calculate_y_uvhit with sign 0): computed once per window-list entry.
-//  * Fit sums run over hits in the order u3, u3', v3, v3', u2, u2', v2, v2'; the
-//    station-3 part is summed once and continued per inner pair (the same
-//    sequence of float additions).
-//  * The refit after left/right (reconstruction_kernels.cuh:1628) has the same
-//    inputs as the first fit and is not repeated.
-//  * Station-3 signs set by the left/right step persist across the inner loop.
-// search (stop < 0): returns the unit's best (chi2, i_uv2).  replay: writes the
-// candidate at i_uv2 == stop into the tracklet.
+// YZ unit: one tracklet, one (u3, v3) pair, the (u2, v2) pairs in reference
+// order with the station-3 signs carried from candidate to candidate, as the
+// reference does.  search (stop < 0): the unit's best (chi2, i_uv2).  replay
+// (stop >= 0): writes the candidate at i_uv2 == stop into the tracklet.
 template <int MAXH>
-__device__ __attribute__((noinline)) bool yz_unit(const Ev<MAXH>& E, Tk* Tp, const YzLists& W, int iuv3, int stop, float& best_chi2,
-                        int& best_i) {
+__device__ __attribute__((noinline)) bool yz_unit(const Ev<MAXH>& E, Tk* Tp, const YzLists& W, int iuv3, int stop,
+                                                  float& best_chi2, int& best_i) {
     const int st3 = Tp->st3;
     const int su2 = super_id(2, 1) * 2, sv2 = super_id(2, 2) * 2;
     const int su3 = super_id(st3, 1) * 2, sv3 = super_id(st3, 2) * 2;
@@ -484,16 +561,13 @@ __device__ __attribute__((noinline)) bool yz_unit(const Ev<MAXH>& E, Tk* Tp, con
     const int nu2 = W.n[0], nv2 = W.n[1], nu3 = W.n[2];
     const int iu3 = iuv3 % nu3, iv3 = iuv3 / nu3;
     float sA0 = 0.f, sA1 = 0.f, sA3 = 0.f, sB0 = 0.f, sB1 = 0.f;
-    const int nu3h = yz_sum(W, 2, iu3, G(F_Z, su3), G(F_Z, su3 - 1), sA0, sA1, sA3, sB0, sB1);
-    if (nu3h == 0) return false;
-    const int nv3h = yz_sum(W, 3, iv3, G(F_Z, sv3), G(F_Z, sv3 - 1), sA0, sA1, sA3, sB0, sB1);
-    if (nv3h == 0) return false;
-    YzHits H;
-    int n3 = 0;
-    yz_put(E, H, n3, W, 2, iu3, 0, su3);
-    yz_put(E, H, n3, W, 2, iu3, 1, su3 - 1);
-    yz_put(E, H, n3, W, 3, iv3, 0, sv3);
-    yz_put(E, H, n3, W, 3, iv3, 1, sv3 - 1);
+    if (yz_sum(W, 2, iu3, G(F_Z, su3), G(F_Z, su3 - 1), sA0, sA1, sA3, sB0, sB1) == 0) return false;
+    if (yz_sum(W, 3, iv3, G(F_Z, sv3), G(F_Z, sv3 - 1), sA0, sA1, sA3, sB0, sB1) == 0) return false;
+    YzSlots H;
+    yz_dets(H, st3);
+    H.pres = 0;
+    yz_fill<0>(E, W, 2, iu3, H);
+    yz_fill<2>(E, W, 3, iv3, H);
     const int last = stop >= 0 ? stop : nu2 * nv2 - 1;
     bool found = false;
     for (int iuv2 = 0; iuv2 <= last; ++iuv2) {
@@ -503,12 +577,13 @@ __device__ __attribute__((noinline)) bool yz_unit(const Ev<MAXH>& E, Tk* Tp, con
         if (yz_sum(W, 1, iv2, zv2, zv2p, A0, A1, A3, B0, B1) == 0) continue;
         const Fit f = fit_finish(A0, A1, A3, B0, B1);
         if (fabsf(f.p0) > Y0_MAX + 2.f * f.e0 || fabsf(f.p1) > TY_MAX + 2.f * f.e1) continue;
+        yz_fill<4>(E, W, 0, iu2, H);
+        yz_fill<6>(E, W, 1, iv2, H);
         float chi2;
         const bool write = stop >= 0 && iuv2 == stop;
-        if (!yz_tail(E, Tp, W, H, n3, iu2, iv2, su2, sv2, f.p0, f.p1, f.e0, f.e1, chi2, write ? Tp : nullptr))
-            continue;
+        if (!yz_tail(E, Tp, H, f, chi2, write ? Tp : nullptr)) continue;
         if (write) return true;
-        if (stop < 0 && chi2 < best_chi2) { best_chi2 = chi2; best_i = iuv2; found = true; }
+        if (chi2 < best_chi2) { best_chi2 = chi2; best_i = iuv2; found = true; }
     }
     return found;
 }
@@ -737,8 +812,6 @@ STAGE void stage_xpairs(const or_params& P, uint16_t* xpairs) {
             {
                 const float sp = spacing_plane(super_id(2 + s, 0));
                 const int cap = P.max_pairs_per_bin < BIN_CAP ? P.max_pairs_per_bin : BIN_CAP;
-                uint8_t f1[PLANE_CAP / 8 + 1], f2[PLANE_CAP / 8 + 1];
-                for (int i = 0; i < PLANE_CAP / 8 + 1; ++i) { f1[i] = 0; f2[i] = 0; }
                 auto put = [&](int c, int ia, int ib) {
                     const int b = xbin(s, c);
                     if (b < 0) return;
@@ -746,20 +819,30 @@ STAGE void stage_xpairs(const or_params& P, uint16_t* xpairs) {
                     if (k < cap) xpairs[(s * 28 + b) * BIN_CAP + k] = (uint16_t)((ia & 0xFF) | ((ib & 0xFF) << 8));
                     L.xcnt[s][b] = k + 1;
                 };
+                auto close = [&](int i, int j) { return !(fabsf(L.pos[E.raw(d1, i)] - L.pos[E.raw(d2, j)]) > sp); };
+                // hits that form a pair: bit masks for the first 64 of a plane, recomputed beyond
+                unsigned long long f1 = 0, f2 = 0;
                 for (int i = 0; i < n1; ++i) {
                     const int ri = E.raw(d1, i);
                     for (int j = 0; j < n2; ++j) {
-                        const int rj = E.raw(d2, j);
-                        if (fabsf(L.pos[ri] - L.pos[rj]) > sp) continue;
+                        if (!close(i, j)) continue;
                         put(L.elem[ri], i, j);
-                        f1[i >> 3] |= (uint8_t)(1u << (i & 7));
-                        f2[j >> 3] |= (uint8_t)(1u << (j & 7));
+                        if (i < 64) f1 |= 1ull << i;
+                        if (j < 64) f2 |= 1ull << j;
                     }
                 }
-                for (int i = 0; i < n1; ++i)
-                    if (!((f1[i >> 3] >> (i & 7)) & 1)) put(L.elem[E.raw(d1, i)], i, 0xFF);
-                for (int j = 0; j < n2; ++j)
-                    if (!((f2[j >> 3] >> (j & 7)) & 1)) put(L.elem[E.raw(d2, j)], 0xFF, j);
+                for (int i = 0; i < n1; ++i) {
+                    bool paired = i < 64 ? ((f1 >> i) & 1ull) != 0 : false;
+                    if (i >= 64)
+                        for (int j = 0; j < n2 && !paired; ++j) paired = close(i, j);
+                    if (!paired) put(L.elem[E.raw(d1, i)], i, 0xFF);
+                }
+                for (int j = 0; j < n2; ++j) {
+                    bool paired = j < 64 ? ((f2 >> j) & 1ull) != 0 : false;
+                    if (j >= 64)
+                        for (int i = 0; i < n1 && !paired; ++i) paired = close(i, j);
+                    if (!paired) put(L.elem[E.raw(d2, j)], 0xFF, j);
+                }
                 for (int b = 0; b < 28; ++b)
                     if (L.xcnt[s][b] > P.max_pairs_per_bin) atomicOr(&L.status, (uint32_t)OR_EV_XZ_BIN_OVERFLOW);
             }
@@ -797,8 +880,8 @@ STAGE int stage_xz(const or_params& P, const Scr& S) {
         for (uint32_t g0 = 0; g0 < total; g0 += WAVE) {
             const uint32_t g = g0 + lane;
             bool acc = false;
-            Tk T;
-            int par = 0;
+            int par = 0, t = 0, s3 = 0;
+            uint16_t q2 = 0, q3 = 0;
             if (g < total) {
                 int lo = 0, hi = 1568;  // largest c with celloff[c] <= g
                 while (hi - lo > 1) {
@@ -807,27 +890,34 @@ STAGE int stage_xz(const or_params& P, const Scr& S) {
                     else hi = mid;
                 }
                 const int c = lo;
-                const int t = c / 49, i = c % 49;
-                const int s3 = 1 + (t & 1);
+                t = c / 49;
+                const int i = c % 49;
+                s3 = 1 + (t & 1);
                 const int b2 = (t / 8) * 7 + i % 7, b3 = ((t % 8) / 2) * 7 + i / 7;
                 const int nx2 = L.xcnt[0][b2];
                 const int ix = (int)(g - celloff[c]);
                 const int ix2 = ix % nx2, ix3 = ix / nx2;
                 par = t & 1;
-                acc = xz_candidate(E, s3, S.xpairs[b2 * BIN_CAP + ix2], S.xpairs[(s3 * 28 + b3) * BIN_CAP + ix3], T);
-                T.t = (uint8_t)t;
-                T.st3 = (uint8_t)(3 + par);
+                q2 = S.xpairs[b2 * BIN_CAP + ix2];
+                q3 = S.xpairs[(s3 * 28 + b3) * BIN_CAP + ix3];
+                acc = xz_candidate(E, s3, q2, q3, nullptr);
             }
             const unsigned long long m = ballot(acc);
             const unsigned long long mp = ballot(acc && par == 0);
             const unsigned long long mm = ballot(acc && par == 1);
             if (acc) {
+                // accepted: evaluate again writing straight into the tracklet slot
                 const int k = ntkl + rank_in(m);
                 const int kp = par == 0 ? np + rank_in(mp) : nm + rank_in(mm);
-                T.canon = (uint32_t)k;
-                T.slot = (uint8_t)(2 * (kp % 16) + par);
-                T.pos_in_slot = (uint8_t)(kp / 16 < 255 ? kp / 16 : 255);
-                if (k < MAXTKL) tks[k] = T;
+                if (k < MAXTKL) {
+                    Tk& T = tks[k];
+                    xz_candidate(E, s3, q2, q3, &T);
+                    T.t = (uint8_t)t;
+                    T.st3 = (uint8_t)(3 + par);
+                    T.canon = (uint32_t)k;
+                    T.slot = (uint8_t)(2 * (kp % 16) + par);
+                    T.pos_in_slot = (uint8_t)(kp / 16 < 255 ? kp / 16 : 255);
+                }
             }
             ntkl += __popcll(m);
             np += __popcll(mp);
@@ -916,28 +1006,18 @@ __device__ __forceinline__ bool yz_box1(const Fit& f) {
     return !(fabsf(f.p0) > Y0_MAX + 2.f * f.e0 || fabsf(f.p1) > TY_MAX + 2.f * f.e1);
 }
 
-// station-3 hits of a candidate, signs 0; returns their number
-template <int MAXH>
-__device__ __forceinline__ int yz_put3(const Ev<MAXH>& E, YzHits& H, const YzLists& W, int st3, const YzItem& it) {
-    const int su3 = super_id(st3, 1) * 2, sv3 = super_id(st3, 2) * 2;
-    int n = 0;
-    yz_put(E, H, n, W, 2, it.iu3, 0, su3);
-    yz_put(E, H, n, W, 2, it.iu3, 1, su3 - 1);
-    yz_put(E, H, n, W, 3, it.iv3, 0, sv3);
-    yz_put(E, H, n, W, 3, it.iv3, 1, sv3 - 1);
-    return n;
-}
-
-// station-3 signs packed two bits each (3 = -1)
-__device__ __forceinline__ uint8_t yz_pack3(const YzHits& H, int n3) {
+// station-3 signs (slots 0-3) packed two bits each, 3 = -1
+__device__ __forceinline__ uint8_t yz_pack3(const YzSlots& H) {
     unsigned c = 0;
-    for (int q = 0; q < n3; ++q) c |= (unsigned)(H.sign[q] & 3) << (2 * q);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) c |= (unsigned)(H.sign[q] & 3) << (2 * q);
     return (uint8_t)c;
 }
-__device__ __forceinline__ void yz_unpack3(YzHits& H, int n3, unsigned c) {
-    for (int q = 0; q < n3; ++q) {
+__device__ __forceinline__ void yz_unpack3(YzSlots& H, unsigned c) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
         const int v = (c >> (2 * q)) & 3;
-        H.sign[q] = (short)(v == 3 ? -1 : v);
+        H.sign[q] = v == 3 ? -1 : v;
     }
 }
 
@@ -949,6 +1029,21 @@ __device__ __forceinline__ int wave_max_scan(int v) {
         if (l >= o) v = max(v, t);
     }
     return v;
+}
+
+// candidate slots and first fit of candidate i of the chunk
+template <int MAXH>
+__device__ __forceinline__ void yz_candidate(const Ev<MAXH>& E, const Scr& S, const Tk* Tp, const YzItem& it,
+                                             YzSlots& H, Fit& f) {
+    YzLists W;
+    yz_lists_of<MAXH>(S, it.j, W);
+    yz_fit(W, Tp->st3, it, f);
+    yz_dets(H, Tp->st3);
+    H.pres = 0;
+    yz_fill<0>(E, W, 2, it.iu3, H);
+    yz_fill<2>(E, W, 3, it.iv3, H);
+    yz_fill<4>(E, W, 0, it.iu2, H);
+    yz_fill<6>(E, W, 1, it.iv2, H);
 }
 
 // Two-pass form.  Returns false (nothing written) when the survivors of the
@@ -975,6 +1070,7 @@ __device__ __attribute__((noinline)) bool yz_chunk_par(const Scr& S, int kb, int
     }
     __syncthreads();
     const int I = L.uoff[YZ_CHUNK];
+    OR_SUB_START
     // pass 1: first fit and box cut of every candidate; survivors in order
     int np = 0;
     for (int i0 = 0; i0 < I; i0 += WAVE) {
@@ -994,56 +1090,43 @@ __device__ __attribute__((noinline)) bool yz_chunk_par(const Scr& S, int kb, int
         np += c;
     }
     __syncthreads();
+    OR_SUB(1);
     // pass 2: per survivor, first-of-unit flag; the first of a unit runs the
     // full left/right from zero signs (its station-3 result seeds the unit),
-    // later ones only the station-3 / station-2 cross pair
+    // later ones only the decision of a v3' / u2 pair (the one pair that can
+    // change a station-3 sign once all are set)
     for (int k0 = 0; k0 < np; k0 += WAVE) {
         const int k = k0 + lane;
         if (k < np) {
-            const int i = (int)S.plist[k];
-            const YzItem it = yz_decode<MAXH>(i);
+            const YzItem it = yz_decode<MAXH>((int)S.plist[k]);
             const int nin = L.nl[it.j][0] * L.nl[it.j][1];
             bool first = k == 0;
             if (!first) {
-                const int ip = (int)S.plist[k - 1];
-                const YzItem pt = yz_decode<MAXH>(ip);
+                const YzItem pt = yz_decode<MAXH>((int)S.plist[k - 1]);
                 first = pt.j != it.j || (pt.local / nin) != (it.local / nin);
             }
             const Tk* Tp = &tks[kb + it.j];
-            YzLists W;
-            yz_lists_of<MAXH>(S, it.j, W);
+            YzSlots H;
             Fit f;
-            yz_fit(W, Tp->st3, it, f);
-            YzHits H;
-            const int n3 = yz_put3(E, H, W, Tp->st3, it);
-            const int su2 = super_id(2, 1) * 2, sv2 = super_id(2, 2) * 2;
-            int n = n3;
-            yz_put(E, H, n, W, 0, it.iu2, 0, su2);
-            yz_put(E, H, n, W, 0, it.iu2, 1, su2 - 1);
-            yz_put(E, H, n, W, 1, it.iv2, 0, sv2);
-            yz_put(E, H, n, W, 1, it.iv2, 1, sv2 - 1);
+            yz_candidate(E, S, Tp, it, H, f);
             uchar4 ps = make_uchar4(first ? 1 : 0, 0, 0, 0);
             if (first) {
-                lr_newhits(Tp->x0, Tp->tx, f.p0, f.p1, Tp->ex0, Tp->etx, f.e0, f.e1, n, H.det, H.pos, H.drift,
-                           H.sign, 150.f);
-                lr_single_newhits(Tp->x0, Tp->tx, f.p0, f.p1, n, H.det, H.pos, H.sign);
-                ps.y = yz_pack3(H, n3);
-                ps.z = (uint8_t)(H.sign[n3 - 1] & 3);
-            } else {
-                const int ci = lr_cross_index(n, n3, H.det);
-                if (ci >= 0) {
-                    const int h = lr_pair_decide(Tp->x0, Tp->tx, f.p0, f.p1, Tp->ex0, Tp->etx, f.e0, f.e1, H.det[ci],
-                                                 H.det[ci + 1], H.pos[ci], H.pos[ci + 1], H.drift[ci],
-                                                 H.drift[ci + 1], 150.f);
-                    if (h >= 0) ps.z = (uint8_t)(lr_si(h) & 3);
-                }
+                lr_slots_new<8>(Tp->x0, Tp->tx, f.p0, f.p1, Tp->ex0, Tp->etx, f.e0, f.e1, H.pres, H.det, H.pos,
+                                H.drift, H.sign, 150.f);
+                ps.y = yz_pack3(H);
+                ps.z = (uint8_t)(H.sign[3] & 3);
+            } else if ((lr_slot_starts<8>(H.pres, H.det) >> 3) & 1u) {
+                const int h = lr_pair_decide(Tp->x0, Tp->tx, f.p0, f.p1, Tp->ex0, Tp->etx, f.e0, f.e1, H.det[3],
+                                             H.det[4], H.pos[3], H.pos[4], H.drift[3], H.drift[4], 150.f);
+                if (h >= 0) ps.z = (uint8_t)(lr_si(h) & 3);
             }
             S.pst[k] = ps;
         }
     }
     __syncthreads();
-    // pass 3: starting signs by scan (unit's first survivor, last writer of
-    // the last station-3 sign before this one), then the rest of the candidate
+    OR_SUB(2);
+    // pass 3: starting signs by scan (the unit's seed, and the last writer of
+    // the v3' sign before this candidate), then the rest of the candidate
     int carry_f = -1, carry_w = -1;
     for (int k0 = 0; k0 < np; k0 += WAVE) {
         const int k = k0 + lane;
@@ -1059,28 +1142,23 @@ __device__ __attribute__((noinline)) bool yz_chunk_par(const Scr& S, int kb, int
         carry_f = __shfl(fi, WAVE - 1, WAVE);
         carry_w = max(carry_w, __shfl(wincl, WAVE - 1, WAVE));
         if (valid) {
-            const int i = (int)S.plist[k];
-            const YzItem it = yz_decode<MAXH>(i);
+            const YzItem it = yz_decode<MAXH>((int)S.plist[k]);
             Tk* Tp = &tks[kb + it.j];
-            YzLists W;
-            yz_lists_of<MAXH>(S, it.j, W);
+            YzSlots H;
             Fit f;
-            yz_fit(W, Tp->st3, it, f);
-            YzHits H;
-            const int n3 = yz_put3(E, H, W, Tp->st3, it);
+            yz_candidate(E, S, Tp, it, H, f);
             uint8_t st = 0;
             if (!ps.x) {
-                const unsigned last = S.pst[wexcl].z;   // written by the first of the unit at least
-                st = (uint8_t)((S.pst[fi].y & ~(3u << (2 * (n3 - 1)))) | (last << (2 * (n3 - 1))));
-                yz_unpack3(H, n3, st);
+                // seed of the unit with the v3' sign (slot 3) of the last writer; the
+                // first survivor of the unit is a writer, so wexcl >= fi
+                st = (uint8_t)((S.pst[fi].y & 0x3Fu) | ((unsigned)S.pst[wexcl].z << 6));
+                yz_unpack3(H, st);
             }
             ps.w = st;
             S.pst[k] = ps;
             float chi2;
             // the reference starts its minimum at 10000.1 (reconstruction_kernels.cuh:1240)
-            if (yz_tail(E, Tp, W, H, n3, it.iu2, it.iv2, super_id(2, 1) * 2, super_id(2, 2) * 2, f.p0, f.p1, f.e0,
-                        f.e1, chi2, nullptr) &&
-                chi2 < 10000.1f) {
+            if (yz_tail(E, Tp, H, f, chi2, nullptr) && chi2 < 10000.1f) {
                 const unsigned long long key =
                     ((unsigned long long)__float_as_uint(chi2) << 32) | (unsigned long long)(uint32_t)it.local;
                 atomicMin(&L.best[it.j], key);
@@ -1088,6 +1166,7 @@ __device__ __attribute__((noinline)) bool yz_chunk_par(const Scr& S, int kb, int
         }
     }
     __syncthreads();
+    OR_SUB(3);
     // winners: replay the candidate with its starting signs and write it
     if (lane < YZ_CHUNK && kb + lane < ntkl) {
         const int j = lane;
@@ -1102,20 +1181,17 @@ __device__ __attribute__((noinline)) bool yz_chunk_par(const Scr& S, int kb, int
                 else hi = mid;
             }
             const YzItem it = yz_decode<MAXH>(i);
-            YzLists W;
-            yz_lists_of<MAXH>(S, j, W);
+            YzSlots H;
             Fit f;
-            yz_fit(W, Tp->st3, it, f);
-            YzHits H;
-            const int n3 = yz_put3(E, H, W, Tp->st3, it);
-            if (!S.pst[lo].x) yz_unpack3(H, n3, S.pst[lo].w);
+            yz_candidate(E, S, Tp, it, H, f);
+            if (!S.pst[lo].x) yz_unpack3(H, S.pst[lo].w);
             float chi2;
-            yz_tail(E, Tp, W, H, n3, it.iu2, it.iv2, super_id(2, 1) * 2, super_id(2, 2) * 2, f.p0, f.p1, f.e0, f.e1,
-                    chi2, Tp);
+            yz_tail(E, Tp, H, f, chi2, Tp);
         } else {
             Tp->stid = 3;
         }
     }
+    OR_SUB(4);
     return true;
 }
 
@@ -1198,6 +1274,7 @@ STAGE void stage_yz(const or_params& P, const Scr& S, int ntkl, int list_cap) {
     uint8_t* wf = S.wf;
     const int capw = P.max_pairs_window < WIN_CAP ? P.max_pairs_window : WIN_CAP;
     for (int kb = 0; kb < ntkl; kb += YZ_CHUNK) {
+        OR_SUB_START
         // window lists: lane = (tracklet j, view v)
         {
             const int j = lane >> 2, v = lane & 3;
@@ -1233,78 +1310,181 @@ STAGE void stage_yz(const or_params& P, const Scr& S, int ntkl, int list_cap) {
             }
         }
         __syncthreads();
+        OR_SUB(0);
         if (L.status) break;
         if (!yz_chunk_par<MAXH>(S, kb, ntkl, list_cap)) {
             __syncthreads();
+            OR_SUB_START
             yz_chunk_seq<MAXH>(S, kb, ntkl);
+            OR_SUB(5);
         }
         __syncthreads();
     }
 }
 
-// F: back-partial cleaning (reconstruction_helper.cuh:1789-1909), slot order
+// F: back-partial cleaning (reconstruction_helper.cuh:1789-1909), slot order:
+// the within-slot pass for slots 0..31, then the cross pass slot against slot.
+//
+// Lane form: the tracklets at stationID >= 5 are gathered in slot order, one
+// per lane (up to 64), each holding its hits as one raw index per detector
+// 13-30 (a tracklet has at most one hit per plane).  The reference's loops run
+// in their order over i; its inner loop over j is done by all lanes at once:
+// j's alive at the start of the i-step stay alive until their own comparison,
+// so nhits_common at j is a prefix sum over them, and the step's decisions
+// (demote j, or demote i) are independent and applied together.  More than 64
+// such tracklets take the one-lane form.
+__device__ __forceinline__ int clean_shared(const int (&ha)[18], const int (&hb)[18]) {
+    int c = 0;
+#pragma unroll
+    for (int t = 0; t < 18; ++t) c += (ha[t] >= 0 && ha[t] == hb[t]) ? 1 : 0;
+    return c;
+}
+
 template <int MAXH>
 STAGE void stage_clean(Tk* tks, const uint16_t* slot_order, int ntkl) {
     Lds<MAXH>& L = g_lds<MAXH>;
     const int lane = lane_id();
-    if (lane == 0 && ntkl > 0) {
-            auto shared_hits = [&](const Tk& A, int na, const Tk& B, int nb) {
-                int c = 0;
-                for (int i = 0; i < na; ++i)
-                    for (int j = 0; j < nb; ++j)
-                        if (B.det[j] == A.det[i] && B.hraw[j] == A.hraw[i]) { c++; break; }
-                return c;
-            };
-            auto pass = [&](int ib, int ie, int jb_self, int jb, int je) {
-                for (int ii = ib; ii < ie; ++ii) {
-                    Tk& A = tks[slot_order[ii]];
-                    if (A.stid < 5) continue;
-                    const int ni = A.nx + A.nuv;
-                    int common = 0;
-                    for (int jj = (jb_self ? ii + 1 : jb); jj < je; ++jj) {
-                        Tk& B = tks[slot_order[jj]];
-                        if (B.stid < 5) continue;
-                        const int nj = B.nx + B.nuv;
-                        common += shared_hits(A, ni, B, nj);
-                        if (A.chi2 / (float)(ni - 4) < B.chi2 / (float)(nj - 4)) {
-                            if ((float)common > (float)ni * 0.3333f) B.stid = 3;
-                        } else {
-                            if ((float)common > (float)nj * 0.3333f) A.stid = 3;
-                        }
+    // gather in slot order
+    int nlive = 0;
+    for (int j0 = 0; j0 < ntkl; j0 += WAVE) {
+        const int j = j0 + lane;
+        const int k = j < ntkl ? slot_order[j] : 0;
+        const bool live = j < ntkl && tks[k].stid >= 5;
+        const unsigned long long m = ballot(live);
+        if (live && nlive + rank_in(m) < WAVE) L.live[nlive + rank_in(m)] = (uint16_t)k;
+        nlive += __popcll(m);
+    }
+    __syncthreads();
+    if (nlive == 0) return;
+    if (nlive <= WAVE) {
+        const bool mine = lane < nlive;
+        const int k = mine ? L.live[lane] : 0;
+        int slot = -1, n = 0;
+        float ratio = 0.f;
+        int hm[18];
+#pragma unroll
+        for (int t = 0; t < 18; ++t) hm[t] = -1;
+        if (mine) {
+            const Tk& T = tks[k];
+            slot = T.slot;
+            n = T.nx + T.nuv;
+            ratio = T.chi2 / (float)(n - 4);
+            for (int h = 0; h < 12; ++h) {
+                if (h >= n) break;
+                const int d = T.det[h], r = T.hraw[h];
+#pragma unroll
+                for (int t = 0; t < 18; ++t)
+                    if (d == 13 + t) hm[t] = r;
+            }
+        }
+        unsigned long long alive = nlive == WAVE ? ~0ull : ((1ull << nlive) - 1ull);
+        // one i-step: A at position pa against the lanes flagged in part
+        auto step = [&](int pa, bool part) {
+            int ha[18];
+#pragma unroll
+            for (int t = 0; t < 18; ++t) ha[t] = __shfl(hm[t], pa, WAVE);
+            const int na = __shfl(n, pa, WAVE);
+            const float ra = __shfl(ratio, pa, WAVE);
+            const int c = part ? clean_shared(ha, hm) : 0;
+            const int common = wave_scan(c);
+            bool dem_j = false, dem_i = false;
+            if (part) {
+                if (ra < ratio) dem_j = (float)common > (float)na * 0.3333f;
+                else dem_i = (float)common > (float)n * 0.3333f;
+            }
+            alive &= ~ballot(dem_j);
+            if (ballot(dem_i)) alive &= ~(1ull << pa);
+        };
+        // within each slot
+        for (int pa = 0; pa < nlive; ++pa) {
+            if (!((alive >> pa) & 1ull)) continue;
+            const int sa = __shfl(slot, pa, WAVE);
+            step(pa, mine && lane > pa && slot == sa && ((alive >> lane) & 1ull));
+        }
+        // slot against slot: for s, for o != s, for i in s, for j in o
+        unsigned slots = 0;
+        {
+            const unsigned long long m = ballot(mine);
+            (void)m;
+            unsigned v = mine ? (1u << slot) : 0u;
+#pragma unroll
+            for (int o = WAVE / 2; o > 0; o >>= 1) v |= __shfl_xor(v, o, WAVE);
+            slots = v;
+        }
+        for (unsigned ss = slots; ss; ss &= ss - 1) {
+            const int sidx = __ffs(ss) - 1;
+            const unsigned long long in_s = ballot(mine && slot == sidx);
+            for (unsigned oo = slots & ~(1u << sidx); oo; oo &= oo - 1) {
+                const int oidx = __ffs(oo) - 1;
+                for (unsigned long long pm = in_s; pm; pm &= pm - 1) {
+                    const int pa = __ffsll((long long)pm) - 1;
+                    if (!((alive >> pa) & 1ull)) continue;
+                    step(pa, mine && slot == oidx && ((alive >> lane) & 1ull));
+                }
+            }
+        }
+        if (mine && !((alive >> lane) & 1ull)) tks[k].stid = 3;
+        __syncthreads();
+        return;
+    }
+    if (lane == 0) {
+        auto shared_hits = [&](const Tk& A, int na, const Tk& B, int nb) {
+            int c = 0;
+            for (int i = 0; i < na; ++i)
+                for (int j = 0; j < nb; ++j)
+                    if (B.det[j] == A.det[i] && B.hraw[j] == A.hraw[i]) { c++; break; }
+            return c;
+        };
+        auto pass = [&](int ib, int ie, int jb_self, int jb, int je) {
+            for (int ii = ib; ii < ie; ++ii) {
+                Tk& A = tks[slot_order[ii]];
+                if (A.stid < 5) continue;
+                const int ni = A.nx + A.nuv;
+                int common = 0;
+                for (int jj = (jb_self ? ii + 1 : jb); jj < je; ++jj) {
+                    Tk& B = tks[slot_order[jj]];
+                    if (B.stid < 5) continue;
+                    const int nj = B.nx + B.nuv;
+                    common += shared_hits(A, ni, B, nj);
+                    if (A.chi2 / (float)(ni - 4) < B.chi2 / (float)(nj - 4)) {
+                        if ((float)common > (float)ni * 0.3333f) B.stid = 3;
+                    } else {
+                        if ((float)common > (float)nj * 0.3333f) A.stid = 3;
                     }
                 }
-            };
-            // within each slot, then slot against slot in slot order
-            for (int s = 0; s < 32; ++s) pass(L.sstart[s], L.sstart[s + 1], 1, 0, L.sstart[s + 1]);
-            for (int s = 0; s < 32; ++s)
-                for (int o = 0; o < 32; ++o)
-                    if (o != s) pass(L.sstart[s], L.sstart[s + 1], 0, L.sstart[o], L.sstart[o + 1]);
-        }
-        __syncthreads();
+            }
+        };
+        for (int s = 0; s < 32; ++s) pass(L.sstart[s], L.sstart[s + 1], 1, 0, L.sstart[s + 1]);
+        for (int s = 0; s < 32; ++s)
+            for (int o = 0; o < 32; ++o)
+                if (o != s) pass(L.sstart[s], L.sstart[s + 1], 0, L.sstart[o], L.sstart[o + 1]);
     }
+    __syncthreads();
+}
 
 // G: station-1 extension (reconstruction_kernels.cuh:1841-2534), one track at a
 // time, the wave's lanes spread over the track's (u, v) pairs.
+//  * The three station-1 window lists (X, U, V) are built by three lanes.
 //  * Per x pair the x part (x at the current y0/ty, KMAG-point fit, left/right,
 //    refit, 1/p window) is computed by every lane alike.
+//  * Hits sit in fixed slots X (4), X' (3), U (2), U' (1), V (6), V' (5), the
+//    reference's list order; see lr_slots_new.
 //  * (u, v) pairs of one x pair are independent except for one case: a lone X'
-//    hit (detector 3) next to a U hit (detector 2) forms a left/right pair
-//    whose sign writes carry to later (u, v) pairs.  That write does not
-//    depend on the current sign, so each lane records whether it writes
-//    (sentinel 2 in the sign slot) and a ballot scan hands each lane the sign
-//    left by the last writer before it.
+//    hit next to a U hit forms a left/right pair whose sign writes carry to
+//    later (u, v) pairs.  That write does not depend on the current sign, so
+//    each lane records whether it writes (sentinel 2 in the sign slot) and a
+//    ballot scan hands each lane the sign left by the last writer before it.
 //  * y0/ty after an x pair are those of the last (u, v) pair that reached the
 //    y fit (the refit after left/right repeats the same fit); they feed the
 //    next x pair's x-at-y.
 //  * The best candidate is the minimum of (chi2, flat index ix*ncomb + iuv),
-//    which is the reference's strict-< first minimum.
+//    which is the reference's strict-< first minimum; its residuals are
+//    recomputed from its parameters when the track is written.
 struct GBest {
-    float chi2, y0, ty, ey0, ety, invp, einvp;
-    int q, n1;
-    float r23[12], r1[6];
-    uint16_t hr[6];
-    uint8_t det[6];
-    int8_t sign[6];
+    float chi2, y0, ty, ey0, ety, invp, einvp, x01, tx1;
+    int q;
+    unsigned pres;
+    int raw[6], sign[6];
 };
 
 __device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long v) {
@@ -1327,152 +1507,162 @@ STAGE void stage_global(const or_params& P, const Scr& S, int ntkl) {
     uint16_t* lv = S.wl + 2 * WIN_CAP;
     GBest* best = reinterpret_cast<GBest*>(S.wy);
     const int capw = P.max_pairs_window < WIN_CAP ? P.max_pairs_window : WIN_CAP;
+    const int det6[6] = {4, 3, 2, 1, 6, 5};
     for (int k = 0; k < ntkl; ++k) {
         Tk* T = &tks[k];
         if (T->stid < 5) continue;
         const int n23 = T->nx + T->nuv;
         const float x0 = T->x0, tx = T->tx, ex0 = T->ex0, etx = T->etx;
         float y0 = T->y0, ty = T->ty;
-        // window lists from the sagitta ratio (reconstruction_helper.cuh:556-613), built by lane 0
-        if (lane == 0) {
+        OR_SUB_START
+        // window lists from the sagitta ratio (reconstruction_helper.cuh:556-613); lane v: X, U, V
+        if (lane < 3) {
             int dlast = 1;
             float zmax = -10000.f;
             for (int m = 0; m < n23; ++m) {
                 const int d = T->det[m];
                 if (d > 0 && G(F_Z, d) > zmax) { dlast = d; zmax = G(F_Z, d); }
             }
-            float pexp[3], win[3];
+            const int v = lane;
+            const int i = v == 0 ? 1 : (v == 1 ? 0 : 2);   // planetype[d/2-1] of d = 2i+2 is v
             const float z3 = G(F_Z, dlast), x3 = x0 + tx * z3, y3 = y0 + ty * z3;
-            for (int i = 0; i < 3; ++i) {
-                const int d = 2 * i + 2;
-                const int idx = i == 0 ? 1 : (i == 1 ? 0 : 2);  // planetype[d/2-1]
-                const int d2 = super_id(2, idx) * 2;
-                const float ct = G(F_COSTHETA, d2), st = G(F_SINTHETA, d2);
-                const float ps3 = x3 * ct + y3 * st;
-                const float z1 = G(F_Z, d), z2 = G(F_Z, d2);
-                const float x2 = x0 + tx * z2, y2 = y0 + ty * z2;
-                const float ps2 = x2 * ct + y2 * st;
-                const float s2t = ps2 - (ps3 * (z2 - Z_TARGET)) / (z3 - Z_TARGET);
-                const float s2d = ps2 - (ps3 * (z2 - Z_DUMP)) / (z3 - Z_DUMP);
-                const float pet = SAG_T_C * s2t + (ps3 * (z1 - Z_TARGET)) / (z3 - Z_TARGET);
-                const float ped = SAG_D_C * s2d + (ps3 * (z1 - Z_DUMP)) / (z3 - Z_DUMP);
-                const float wt = fabsf(s2t * SAG_T_W), wd = fabsf(s2d * SAG_D_W);
-                const float pmn = fminf(pet - wt, ped - wd), pmx = fmaxf(pet + wt, ped + wd);
-                pexp[idx] = (float)(0.5 * (double)(pmx + pmn));
-                win[idx] = (float)(0.5 * (double)(pmx - pmn));
-            }
-            const int nx1 = pairs_window(E, 0, 0, pexp[0] - win[0] * 1.0f, pexp[0] + win[0] * 1.0f, lx, P.max_pair_plane_hits, capw);
-            const int nu1 = pairs_window(E, 0, 1, pexp[1] - win[1] * 1.0f, pexp[1] + win[1] * 1.0f, lu, P.max_pair_plane_hits, capw);
-            const int nv1 = pairs_window(E, 0, 2, pexp[2] - win[2] * 1.0f, pexp[2] + win[2] * 1.0f, lv, P.max_pair_plane_hits, capw);
-            if (nx1 < 0 || nu1 < 0 || nv1 < 0) atomicOr(&L.status, (uint32_t)OR_EV_PAIR_OVERFLOW);
-            L.nl[0][0] = nx1; L.nl[0][1] = nu1; L.nl[0][2] = nv1;
+            const int d = 2 * i + 2;
+            const int d2 = super_id(2, v) * 2;
+            const float ct = G(F_COSTHETA, d2), st = G(F_SINTHETA, d2);
+            const float ps3 = x3 * ct + y3 * st;
+            const float z1 = G(F_Z, d), z2 = G(F_Z, d2);
+            const float x2 = x0 + tx * z2, y2 = y0 + ty * z2;
+            const float ps2 = x2 * ct + y2 * st;
+            const float s2t = ps2 - (ps3 * (z2 - Z_TARGET)) / (z3 - Z_TARGET);
+            const float s2d = ps2 - (ps3 * (z2 - Z_DUMP)) / (z3 - Z_DUMP);
+            const float pet = SAG_T_C * s2t + (ps3 * (z1 - Z_TARGET)) / (z3 - Z_TARGET);
+            const float ped = SAG_D_C * s2d + (ps3 * (z1 - Z_DUMP)) / (z3 - Z_DUMP);
+            const float wt = fabsf(s2t * SAG_T_W), wd = fabsf(s2d * SAG_D_W);
+            const float pmn = fminf(pet - wt, ped - wd), pmx = fmaxf(pet + wt, ped + wd);
+            const float pexp = (float)(0.5 * (double)(pmx + pmn));
+            const float win = (float)(0.5 * (double)(pmx - pmn));
+            uint16_t* out = v == 0 ? lx : (v == 1 ? lu : lv);
+            const int nl = pairs_window(E, 0, v, pexp - win * 1.0f, pexp + win * 1.0f, out, P.max_pair_plane_hits, capw);
+            if (nl < 0) atomicOr(&L.status, (uint32_t)OR_EV_PAIR_OVERFLOW);
+            L.nl[0][v] = nl;
         }
         __syncthreads();
+        OR_SUB(6);
         if (L.status) break;
         const int nx1 = L.nl[0][0], nu1 = L.nl[0][1], nv1 = L.nl[0][2];
         if (nx1 == 0 || nu1 == 0 || nv1 == 0) continue;
-        // y of the station-2/3 UV hits with their stored signs (reconstruction_kernels.cuh:2086-2102)
-        float Y[12], eY[12], Zy[12];
-        int ny = 0;
+        // y sums of the station-2/3 UV hits with their stored signs (reconstruction_kernels.cuh:2086-2102)
+        float A0s = 0.f, A1s = 0.f, A3s = 0.f, B0s = 0.f, B1s = 0.f;
         for (int m = 0; m < n23; ++m) {
             const int d = T->det[m];
             if ((double)G(F_COSTHETA, d) > 0.99) continue;
             float y, ey;
             if (y_uvhit(d, E.Lp->elem[T->hraw[m]], E.Lp->drift[T->hraw[m]], T->sign[m], x0, tx, y, ey)) {
-                Y[ny] = y; eY[ny] = ey; Zy[ny] = G(F_Z, d); ny++;
+                const float z = G(F_Z, d);
+                B0s += ey * y; B1s += (ey * y) * z; A0s += ey; A1s += ey * z; A3s += (ey * z) * z;
             }
-        }
-        float A0s = 0.f, A1s = 0.f, A3s = 0.f, B0s = 0.f, B1s = 0.f;
-        for (int m = 0; m < ny; ++m) {
-            B0s += eY[m] * Y[m]; B1s += (eY[m] * Y[m]) * Zy[m];
-            A0s += eY[m]; A1s += eY[m] * Zy[m]; A3s += (eY[m] * Zy[m]) * Zy[m];
         }
         unsigned long long best_key = ~0ull;
         const int ncomb = nu1 * nv1;
         for (int ix = 0; ix < nx1; ++ix) {
-            // ---- x part, every lane alike
-            short xdet[2], xsign[2];
-            float xpos[2], xdr[2], X[3], eX[3], Z[3];
-            uint16_t xhr[2];
-            int nx = 0;
+            // ---- x part, every lane alike: slots X (4), X' (3) and the KMAG point
             const uint16_t qx = lx[ix];
+            const int xi[2] = {pa(qx), pb(qx)};
+            unsigned xpres = 0;
+            int xraw[2] = {0, 0}, xsign[2] = {0, 0};
+            float X[2] = {0.f, 0.f}, Zx[2] = {0.f, 0.f}, xdr[2] = {0.f, 0.f}, xpos[2] = {0.f, 0.f};
+#pragma unroll
             for (int w = 0; w < 2; ++w) {
-                const int idx = w ? pb(qx) : pa(qx);
-                if (idx < 0) continue;
-                const int d = w ? 3 : 4;
-                const int r = E.raw(d, idx);
-                xdet[nx] = (short)d; xpos[nx] = E.Lp->pos[r]; eX[nx] = G(F_RESOLUTION, d); xsign[nx] = 0;
-                xdr[nx] = E.Lp->drift[r]; xhr[nx] = (uint16_t)r;
-                xz_fy(d, E.Lp->elem[r], xdr[nx], 0, y0, ty, X[nx], Z[nx]);
-                ++nx;
+                if (xi[w] < 0) continue;
+                const int d = det6[w];
+                const int r = E.raw(d, xi[w]);
+                xpres |= 1u << w;
+                xraw[w] = r; xdr[w] = E.Lp->drift[r]; xpos[w] = E.Lp->pos[r];
+                xz_fy(d, E.Lp->elem[r], xdr[w], 0, y0, ty, X[w], Zx[w]);
             }
-            if (nx == 0) continue;
-            X[nx] = x0 + tx * Z_KMAG_BEND;
-            eX[nx] = ex0 + etx * Z_KMAG_BEND;
-            Z[nx] = Z_KMAG_BEND;
-            Fit f = fit2d<3>(nx + 1, X, Z, eX);
-            lr_xhits<2>(f.p0, f.p1, f.p0, f.p1, nx, xdet, X, xdr, xsign, 150.f);
-            lr_single_xhits(f.p0, f.p1, nx, xdet, X, xsign);
-            for (int l = 0; l < nx; ++l) X[l] += (float)xsign[l] * xdr[l];
-            f = fit2d<3>(nx + 1, X, Z, eX);
+            if (xpres == 0) continue;
+            const float xk = x0 + tx * Z_KMAG_BEND, exk = ex0 + etx * Z_KMAG_BEND;
+            auto xfit = [&](const float (&xv)[2]) {
+                float A0 = 0.f, A1 = 0.f, A3 = 0.f, B0 = 0.f, B1 = 0.f;
+#pragma unroll
+                for (int w = 0; w < 2; ++w) {
+                    if (!((xpres >> w) & 1u)) continue;
+                    const float e = G(F_RESOLUTION, det6[w]);
+                    B0 += e * xv[w]; B1 += (e * xv[w]) * Zx[w]; A0 += e; A1 += e * Zx[w]; A3 += (e * Zx[w]) * Zx[w];
+                }
+                B0 += exk * xk; B1 += (exk * xk) * Z_KMAG_BEND; A0 += exk; A1 += exk * Z_KMAG_BEND;
+                A3 += (exk * Z_KMAG_BEND) * Z_KMAG_BEND;
+                return fit_finish(A0, A1, A3, B0, B1);
+            };
+            Fit f = xfit(X);
+            if (xpres == 3u) {   // X / X' pair, x-only decision with the fit's parameters as errors
+                const int h = lr_xpair_decide(f.p0, f.p1, f.p0, f.p1, 4, 3, X[0], X[1], xdr[0], xdr[1], 150.f);
+                if (h >= 0) { xsign[0] = lr_si(h); xsign[1] = lr_sj(h); }
+            }
+#pragma unroll
+            for (int w = 0; w < 2; ++w)
+                if (((xpres >> w) & 1u) && xsign[w] == 0) xsign[w] = G(F_Z, det6[w]) * f.p1 + f.p0 > X[w] ? 1 : -1;
+            float Xc[2];
+#pragma unroll
+            for (int w = 0; w < 2; ++w) Xc[w] = X[w] + (float)xsign[w] * xdr[w];
+            f = xfit(Xc);
             const float x01 = f.p0, tx1 = f.p1;  // parameters used as errors (reconstruction_kernels.cuh:2194-2195)
             const int q = (double)(-0.0033f) * copysign(1.0, (double)FMAGSTR) * (double)x0 < (double)tx ? 1 : -1;
             const float invp = ((tx1 - tx) * (float)q) / PT_KICK_KMAG;
             const float einvp = (etx - etx) / PT_KICK_KMAG;
             if (invp > INVP_MAX + einvp || invp < INVP_MIN - einvp) continue;
-            const bool cross = nx == 1 && xdet[0] == 3;
-            short xs_state = xsign[0];
+            const bool cross = xpres == 2u;   // lone X'
+            int xs_state = xsign[1];
             // ---- (u, v) pairs, one per lane
             for (int base = 0; base < ncomb; base += WAVE) {
                 const int iuv = base + lane;
                 bool reached = false, cand = false, wrote = false;
                 float fy0 = 0.f, fty = 0.f, fey0 = 0.f, fety = 0.f, chi23 = 0.f;
-                short det[6], sign[6];
-                float pos[6], drift[6], r1[6], r23[12];
-                uint16_t hr[6];
-                int n = 0;
+                unsigned pres = xpres;
+                int sign[6], raw[6];
+                float pos[6], drift[6];
+                sign[0] = xsign[0]; raw[0] = xraw[0]; pos[0] = xpos[0]; drift[0] = xdr[0];
+                sign[1] = cross ? 2 : xsign[1]; raw[1] = xraw[1]; pos[1] = xpos[1]; drift[1] = xdr[1];
+#pragma unroll
+                for (int s = 2; s < 6; ++s) { sign[s] = 0; raw[s] = 0; pos[s] = 0.f; drift[s] = 0.f; }
                 if (iuv < ncomb) {
-                    for (int l = 0; l < nx; ++l) {
-                        det[l] = xdet[l]; pos[l] = xpos[l]; drift[l] = xdr[l]; hr[l] = xhr[l];
-                        sign[l] = cross ? (short)2 : xsign[l];
-                    }
-                    n = nx;
                     float A0 = A0s, A1 = A1s, A3 = A3s, B0 = B0s, B1 = B1s;
                     const int iu = iuv % nu1, iv = iuv / nu1;
-                    auto adduv = [&](int d, int idx) {
+                    const int ui[4] = {pa(lu[iu]), pb(lu[iu]), pa(lv[iv]), pb(lv[iv])};
+                    int nuh = 0, nvh = 0;
+#pragma unroll
+                    for (int s = 2; s < 6; ++s) {
+                        const int idx = ui[s - 2];
+                        if (idx < 0 || (s >= 4 && nuh == 0)) continue;
+                        const int d = det6[s];
                         const int r = E.raw(d, idx);
                         float y, ey;
                         if (y_uvhit(d, E.Lp->elem[r], E.Lp->drift[r], 0, x01, tx1, y, ey)) {
-                            det[n] = (short)d; pos[n] = E.Lp->pos[r]; drift[n] = E.Lp->drift[r]; hr[n] = (uint16_t)r;
-                            sign[n] = 0;
+                            pres |= 1u << s;
+                            raw[s] = r; pos[s] = E.Lp->pos[r]; drift[s] = E.Lp->drift[r];
                             const float z = G(F_Z, d);
                             B0 += ey * y; B1 += (ey * y) * z; A0 += ey; A1 += ey * z; A3 += (ey * z) * z;
-                            ++n;
+                            if (s < 4) ++nuh;
+                            else ++nvh;
                         }
-                    };
-                    if (pa(lu[iu]) >= 0) adduv(2, pa(lu[iu]));
-                    if (pb(lu[iu]) >= 0) adduv(1, pb(lu[iu]));
-                    const int nuh = n - nx;
-                    if (nuh > 0) {
-                        if (pa(lv[iv]) >= 0) adduv(6, pa(lv[iv]));
-                        if (pb(lv[iv]) >= 0) adduv(5, pb(lv[iv]));
                     }
-                    if (nuh > 0 && n - nx - nuh > 0) {
+                    if (nuh > 0 && nvh > 0) {
                         reached = true;
                         const Fit g = fit_finish(A0, A1, A3, B0, B1);
                         fy0 = g.p0; fty = g.p1; fey0 = g.e0; fety = g.e1;
                         if (!(fabsf(fy0) > Y0_MAX + 2.f * fey0 || fabsf(fty) > TY_MAX + 2.f * fety)) {
-                            lr_newhits(x01, tx1, fy0, fty, x01, tx1, fey0, fety, n, det, pos, drift, sign, 150.f);
-                            lr_single_newhits(x01, tx1, fy0, fty, n, det, pos, sign);
-                            if (cross) wrote = sign[0] != 2;
+                            int dets[6] = {4, 3, 2, 1, 6, 5};
+                            lr_slots_new<6>(x01, tx1, fy0, fty, x01, tx1, fey0, fety, pres, dets, pos, drift, sign,
+                                            150.f);
+                            if (cross) wrote = sign[1] != 2;
                             if (!(fabsf(fy0) > Y0_MAX || fabsf(fty) > TY_MAX)) {
                                 // station-2/3 residuals with (x0, tx) of the tracklet and the new y line
                                 for (int m = 0; m < n23; ++m) {
                                     const int d = T->det[m];
                                     const float res = G(F_RESOLUTION, d);
                                     const float dca = dca_wire(d, E.Lp->elem[T->hraw[m]], x0, fy0, tx, fty);
-                                    r23[m] = E.Lp->drift[T->hraw[m]] * (float)T->sign[m] - dca;
-                                    chi23 += (r23[m] * r23[m]) / (res * res);
+                                    const float r23 = E.Lp->drift[T->hraw[m]] * (float)T->sign[m] - dca;
+                                    chi23 += (r23 * r23) / (res * res);
                                 }
                                 cand = true;
                             }
@@ -1484,19 +1674,20 @@ STAGE void stage_global(const or_params& P, const Scr& S, int ntkl) {
                     const unsigned long long wm = ballot(wrote);
                     const unsigned long long le = wm & (lane == 63 ? ~0ull : ((2ull << lane) - 1ull));
                     const int src = le ? 63 - __clzll((long long)le) : 0;
-                    const int sv = __shfl((int)sign[0], src, WAVE);
-                    if (iuv < ncomb && n > 0) sign[0] = (short)(le ? sv : xs_state);
-                    if (wm) xs_state = (short)__shfl((int)sign[0], 63 - __clzll((long long)wm), WAVE);
+                    const int sv = __shfl(sign[1], src, WAVE);
+                    if (iuv < ncomb) sign[1] = le ? sv : xs_state;
+                    if (wm) xs_state = __shfl(sign[1], 63 - __clzll((long long)wm), WAVE);
                 }
                 unsigned long long key = ~0ull;
                 if (cand) {
                     float c1 = 0.f;
-                    for (int l = 0; l < n; ++l) {
-                        const int d = det[l];
+#pragma unroll
+                    for (int s = 0; s < 6; ++s) {
+                        if (!((pres >> s) & 1u)) continue;
+                        const int d = det6[s];
                         const float res = G(F_RESOLUTION, d);
-                        const float dca = dca_wire(d, E.Lp->elem[hr[l]], x01, fy0, tx1, fty);
-                        r1[l] = drift[l] * (float)sign[l] - dca;
-                        c1 += ((r1[l] * r1[l]) / res) / res;
+                        const float r1 = drift[s] * (float)sign[s] - dca_wire(d, E.Lp->elem[raw[s]], x01, fy0, tx1, fty);
+                        c1 += ((r1 * r1) / res) / res;
                     }
                     const float chi2 = chi23 + c1;
                     if (!(chi2 > 10000.f) && chi2 < 10000.1f)
@@ -1507,12 +1698,10 @@ STAGE void stage_global(const or_params& P, const Scr& S, int ntkl) {
                     if (key == cmin) {
                         GBest& B = *best;
                         B.chi2 = __uint_as_float((uint32_t)(key >> 32));
-                        B.y0 = fy0; B.ty = fty; B.ey0 = fey0; B.ety = fety; B.invp = invp; B.einvp = einvp; B.q = q;
-                        B.n1 = n;
-                        for (int m = 0; m < n23; ++m) B.r23[m] = r23[m];
-                        for (int l = 0; l < n; ++l) {
-                            B.r1[l] = r1[l]; B.hr[l] = hr[l]; B.det[l] = (uint8_t)det[l]; B.sign[l] = (int8_t)sign[l];
-                        }
+                        B.y0 = fy0; B.ty = fty; B.ey0 = fey0; B.ety = fety; B.invp = invp; B.einvp = einvp;
+                        B.x01 = x01; B.tx1 = tx1; B.q = q; B.pres = pres;
+#pragma unroll
+                        for (int s = 0; s < 6; ++s) { B.raw[s] = raw[s]; B.sign[s] = sign[s]; }
                     }
                     best_key = cmin;
                 }
@@ -1526,21 +1715,31 @@ STAGE void stage_global(const or_params& P, const Scr& S, int ntkl) {
             }
         }
         __syncthreads();
+        OR_SUB(7);
         if (best_key != ~0ull && lane == 0) {
             const GBest& B = *best;
             T->stid = 6;
-            T->n1 = (uint8_t)B.n1;
             T->chi2 = B.chi2; T->y0 = B.y0; T->ty = B.ty; T->ey0 = B.ey0; T->ety = B.ety;
             T->invp = B.invp; T->einvp = B.einvp; T->charge = (int8_t)B.q;
-            for (int m = 0; m < n23; ++m) T->resid[m] = B.r23[m];
-            for (int l = 0; l < B.n1; ++l) {
-                T->resid[n23 + l] = B.r1[l];
-                T->hraw[n23 + l] = B.hr[l];
-                T->det[n23 + l] = B.det[l];
-                T->sign[n23 + l] = B.sign[l];
+            for (int m = 0; m < n23; ++m) {
+                const int d = T->det[m], r = T->hraw[m];
+                T->resid[m] = E.Lp->drift[r] * (float)T->sign[m] - dca_wire(d, E.Lp->elem[r], x0, B.y0, tx, B.ty);
             }
+            int n1 = 0;
+            for (int s = 0; s < 6; ++s) {
+                if (!((B.pres >> s) & 1u)) continue;
+                const int d = det6[s], r = B.raw[s];
+                T->resid[n23 + n1] = E.Lp->drift[r] * (float)B.sign[s] -
+                                     dca_wire(d, E.Lp->elem[r], B.x01, B.y0, B.tx1, B.ty);
+                T->hraw[n23 + n1] = (uint16_t)r;
+                T->det[n23 + n1] = (uint8_t)d;
+                T->sign[n23 + n1] = (int8_t)B.sign[s];
+                ++n1;
+            }
+            T->n1 = (uint8_t)n1;
         }
         __syncthreads();
+        OR_SUB(8);
     }
 }
 
@@ -2036,6 +2235,22 @@ void or_session_destroy(or_session* s) {
  * with stages load+reduce, X pairs, XZ, YZ, cleaning, global, outputs.  Only
  * the profiling build (-DOR_STAGE_TIMING) counts; the product build returns
  * OR_ERR_INVALID_ARG. */
+extern "C" or_status or_debug_sub_cycles(unsigned long long* out, int reset) {
+    if (!out) return OR_ERR_INVALID_ARG;
+#ifdef OR_STAGE_TIMING
+    if (hipMemcpyFromSymbol(out, HIP_SYMBOL(g_sub_cycles), sizeof(g_sub_cycles)) != hipSuccess) return OR_ERR_DEVICE;
+    if (reset) {
+        static const unsigned long long zero[N_SUB_TIMERS] = {};
+        if (hipMemcpyToSymbol(HIP_SYMBOL(g_sub_cycles), zero, sizeof(zero)) != hipSuccess) return OR_ERR_DEVICE;
+    }
+    return OR_OK;
+#else
+    (void)reset;
+    for (int i = 0; i < N_SUB_TIMERS; ++i) out[i] = 0;
+    return OR_ERR_INVALID_ARG;
+#endif
+}
+
 extern "C" or_status or_debug_stage_cycles(unsigned long long* out, int reset) {
     if (!out) return OR_ERR_INVALID_ARG;
 #ifdef OR_STAGE_TIMING

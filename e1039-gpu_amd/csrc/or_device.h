@@ -152,6 +152,22 @@ __device__ __forceinline__ float position(float pos, float drift, short s) { ret
 
 // ---- left/right of x hit pairs (reconstruction_helper.cuh:808-872) ---------
 // A partner index at or past n counts as not adjacent.
+// x-only decision for one adjacent pair: hypothesis index or -1
+__device__ __forceinline__ int lr_xpair_decide(float x0, float tx, float ex0, float etx, int di, int dj, float posi,
+                                               float posj, float dri, float drj, float thr) {
+    int imin = -1;
+    float pmin = 1.e6f;
+    const float zi = G(F_Z, di), zj = G(F_Z, dj);
+    for (int h = 0; h < 4; ++h) {
+        const float pi = position(posi, dri, lr_si(h));
+        const float sl = (pi - position(posj, drj, lr_sj(h))) / (zi - zj);
+        const float it = pi - sl * zi;
+        const float pull = sqrtf(((tx - sl) * (tx - sl)) / etx / etx + ((x0 - it) * (x0 - it)) / ex0 / ex0);
+        if (pull < pmin) { imin = h; pmin = pull; }
+    }
+    return (imin > 0 && pmin < thr) ? imin : -1;
+}
+
 template <int N>
 __device__ __forceinline__ void lr_xhits(float x0, float tx, float ex0, float etx, int n, const short* det,
                                          const float* pos, const float* drift, short* sign, float thr) {
@@ -159,17 +175,8 @@ __device__ __forceinline__ void lr_xhits(float x0, float tx, float ex0, float et
         const int i = k, j = k + 1;
         if (j >= n || abs(det[i] - det[j]) != 1) { k--; continue; }
         if (sign[i] * sign[j] != 0) continue;
-        int imin = -1;
-        float pmin = 1.e6f;
-        const float zi = G(F_Z, det[i]), zj = G(F_Z, det[j]);
-        for (int h = 0; h < 4; ++h) {
-            const float pi = position(pos[i], drift[i], lr_si(h));
-            const float sl = (pi - position(pos[j], drift[j], lr_sj(h))) / (zi - zj);
-            const float it = pi - sl * zi;
-            const float pull = sqrtf(((tx - sl) * (tx - sl)) / etx / etx + ((x0 - it) * (x0 - it)) / ex0 / ex0);
-            if (pull < pmin) { imin = h; pmin = pull; }
-        }
-        if (imin > 0 && pmin < thr) { sign[i] = lr_si(imin); sign[j] = lr_sj(imin); }
+        const int h = lr_xpair_decide(x0, tx, ex0, etx, det[i], det[j], pos[i], pos[j], drift[i], drift[j], thr);
+        if (h >= 0) { sign[i] = lr_si(h); sign[j] = lr_sj(h); }
     }
 }
 
@@ -245,6 +252,39 @@ __device__ __forceinline__ void lr_single_newhits(float x0, float tx, float y0, 
         if (d < 0) continue;
         const float pe = (G(F_Z, d) * tx + x0) * G(F_COSTHETA, d) + (G(F_Z, d) * ty + y0) * G(F_SINTHETA, d);
         sign[k] = pe > pos[k] ? 1 : -1;
+    }
+}
+
+// ---- left/right on hits kept in fixed slots --------------------------------
+// A candidate's hits sit in NS fixed slots in the reference's list order; bit s
+// of pres says slot s holds a hit.  In every slot layout used here, detectors
+// one apart only occur in neighbouring slots, so the pairing walk over the
+// present hits (steps of two, back by one at a non-adjacent neighbour) pairs
+// slot s with s+1 exactly when both are present, adjacent, and s was not taken
+// as the second hit of the pair before.  All indices below are compile-time
+// constants after unrolling, which keeps the slot arrays in registers.
+template <int NS>
+__device__ __forceinline__ void lr_slots_new(float x0, float tx, float y0, float ty, float ex0, float etx, float ey0,
+                                             float ety, unsigned pres, const int (&det)[NS], const float (&pos)[NS],
+                                             const float (&drift)[NS], int (&sign)[NS], float thr) {
+    bool used = false;
+#pragma unroll
+    for (int s = 0; s + 1 < NS; ++s) {
+        const bool start = !used && ((pres >> s) & 1u) && ((pres >> (s + 1)) & 1u) && abs(det[s] - det[s + 1]) == 1;
+        if (start && sign[s] * sign[s + 1] == 0) {
+            const int h = lr_pair_decide(x0, tx, y0, ty, ex0, etx, ey0, ety, det[s], det[s + 1], pos[s], pos[s + 1],
+                                         drift[s], drift[s + 1], thr);
+            if (h >= 0) { sign[s] = lr_si(h); sign[s + 1] = lr_sj(h); }
+        }
+        used = start;
+    }
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        if (((pres >> s) & 1u) && sign[s] == 0) {
+            const int d = det[s];
+            const float pe = (G(F_Z, d) * tx + x0) * G(F_COSTHETA, d) + (G(F_Z, d) * ty + y0) * G(F_SINTHETA, d);
+            sign[s] = pe > pos[s] ? 1 : -1;
+        }
     }
 }
 

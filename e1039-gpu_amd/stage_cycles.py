@@ -15,6 +15,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
 STAGES = ["load+reduce", "X pairs", "XZ", "YZ", "cleaning", "global", "outputs"]
+SUBS = ["YZ window lists", "YZ pass 1 (first fit)", "YZ pass 2 (seeds)", "YZ pass 3 (tails)", "YZ winners",
+        "YZ unit path", "st1 window lists", "st1 candidate loop", "st1 track write"]
 
 
 def main():
@@ -35,12 +37,17 @@ def main():
     dout = sess.alloc_outputs(ev.n_events, ev.n_hits, tracklets=True)
     sess.run_device(din, dout)
     torch.cuda.synchronize()
+    lib.or_debug_sub_cycles.argtypes = [C.c_void_p, C.c_int]
+    lib.or_debug_sub_cycles.restype = C.c_int32
     buf = (C.c_ulonglong * 35)()
+    sub = (C.c_ulonglong * 16)()
     assert lib.or_debug_stage_cycles(buf, 1) == 0, "not the profiling build"
+    lib.or_debug_sub_cycles(sub, 1)
     sess.run_device(din, dout)
     torch.cuda.synchronize()
     print(f"{n} events, Poisson({mean:g}) hits/event: chain kernels {sess.last_kernel_ms():.1f} ms")
     lib.or_debug_stage_cycles(buf, 0)
+    lib.or_debug_sub_cycles(sub, 0)
     a = np.array(list(buf), dtype=np.float64).reshape(5, 7)
     for c in range(5):
         tot = a[c].sum()
@@ -49,6 +56,9 @@ def main():
         print(f"tile class {512 << c} hits: {tot:.3e} cycles")
         for k, name in enumerate(STAGES):
             print(f"  {name:12s} {a[c, k] / tot * 100:5.1f} %  {a[c, k] / n:12.0f} cycles/event")
+    print("sub-stages (all classes), cycles/event:")
+    for k, name in enumerate(SUBS):
+        print(f"  {name:22s} {sub[k] / n:12.0f}")
 
 
 if __name__ == "__main__":

@@ -187,6 +187,12 @@ or_status or_session_set_tuning(or_session* s, or_tuning key, int64_t value);
  * OR_ERR_INVALID_ARG.  No reference counterpart. */
 or_status or_debug_stage_cycles(unsigned long long* out /* 35 */, int reset);
 
+/* Profiling: cycles of sub-stages summed over waves (wave-level timers, all
+ * tile classes): YZ window lists, YZ passes 1-3, YZ winners, YZ unit path,
+ * station-1 window lists, station-1 candidate loop, track write; 16 slots.
+ * Same build rule as or_debug_stage_cycles.  No reference counterpart. */
+or_status or_debug_sub_cycles(unsigned long long* out /* 16 */, int reset);
+
 void or_session_destroy(or_session* s);
 const char* or_status_string(or_status st);
 

@@ -123,7 +123,8 @@ def test_large_tiles(oc, orc, geo, ogeo):
     parts = [oc.generate(geo, 20, seed=18 + k, mult_mode=1, mult_mean=m, mult_cap=9000)
              for k, m in enumerate((1800.0, 3500.0, 8190.0))]
     cat = lambda f: np.concatenate([getattr(x, f) for x in parts])
-    offs = np.concatenate([[0], np.cumsum([x.n_hits for x in parts])]).astype(np.int64)
+    base = np.cumsum([0] + [x.n_hits for x in parts])
+    offs = np.concatenate([[0]] + [x.offsets[1:] + b for x, b in zip(parts, base)]).astype(np.int64)
     ev = oc.Events(cat("det"), cat("elem"), cat("tdc"), cat("drift"), cat("flag"), offs)
     counts = np.diff(ev.offsets)
     for lo, hi in ((1024, 2048), (2048, 4096), (4096, 8192), (8192, 10**9)):
