@@ -60,6 +60,7 @@ struct __align__(16) Tk {
     float x0, tx, ex0, etx;
     float y0, ty, ey0, ety;
     float chi2, invp, einvp, pad;
+    float vtx[6];           // vertexing: vx, vy, vz, px, py, pz (stationID 7)
     float xpos[4];          // x at y = 0 of the XZ hits (record 'pos' of those hits)
     float resid[18];
     uint16_t hraw[18];      // LDS raw index of each hit
@@ -73,19 +74,23 @@ struct __align__(16) Tk {
     uint8_t pad2[3];
 };
 
+// The event tile.  Hits are staged grouped by detector; after the reducer the
+// kept ones are compacted in place, so plane d's reduced hits are
+// [rstart[d], rstart[d] + rcnt[d]) of elem/pos/drift/orig.  Sized so that the
+// 512-hit tile stays within 8 KB (20 waves per CU).
 template <int MAXH>
 struct Lds {
     int16_t elem[MAXH];
     float pos[MAXH];
     float drift[MAXH];
     uint16_t orig[MAXH];      // event-local input index
-    uint16_t red[MAXH];       // reduced hits: raw indices, planes contiguous
     int8_t keep[MAXH];        // staged: 1 in time, -1 out of time; after reduction: 1 kept, -1 dropped
-    uint8_t korig[MAXH];      // kept flag by input order
-    int pcnt[56], pstart[56], cursor[56], rcnt[56], rstart[56];
-    int xcnt[3][28];
-    union {
-        int nl[YZ_CHUNK][4];      // YZ / global: window list lengths
+    uint32_t kbits[MAXH / 32];  // reducer keep bit by event-local input index
+    int pcnt[56];
+    int16_t pstart[56], cursor[56], rcnt[56], rstart[56];
+    uint16_t xcnt[3][28];
+    union alignas(16) {
+        int nl[YZ_CHUNK][4];      // YZ / global: window list lengths (read as int4)
         uint16_t live[WAVE];      // cleaning: tracklets at stationID >= 5, slot order
     };
     int uoff[YZ_CHUNK + 1];   // prefix of (u3, v3) units or of candidates per tracklet of the chunk
@@ -113,6 +118,7 @@ struct Args {
     or_track* tracks;
     int32_t* n_tracklets;
     float* records;
+    float* vertex;              // optional: swim global tracks to the beam line
     char* scratch;
     int64_t scratch_stride;
     int yz_list_cap;            // <= YZ_LIST_CAP; chunks with more candidates take the sequential path
@@ -168,7 +174,7 @@ struct Ev {
     };
     static constexpr Ref Lp{};
     __device__ __forceinline__ int nred(int d) const { return g_lds<MAXH>.rcnt[d]; }
-    __device__ __forceinline__ int raw(int d, int k) const { return g_lds<MAXH>.red[g_lds<MAXH>.rstart[d] + k]; }
+    __device__ __forceinline__ int raw(int d, int k) const { return g_lds<MAXH>.rstart[d] + k; }
 };
 
 // XZ bin of an element (WCHitsBins_X, reconstruction_constants.h:155-162):
@@ -636,7 +642,7 @@ STAGE bool stage_load_reduce(const Args& a, int64_t h0, int n) {
                 if (c > cap || c > PLANE_CAP) atomicOr(&L.status, (uint32_t)OR_EV_PLANE_OVERFLOW);
             }
             const int inc = wave_scan(c);
-            if (lane < 56) { L.pstart[lane] = inc - c; L.cursor[lane] = inc - c; }
+            if (lane < 56) { L.pstart[lane] = (int16_t)(inc - c); L.cursor[lane] = (int16_t)(inc - c); }
         }
         __syncthreads();
         if (L.status == 0) {
@@ -655,7 +661,7 @@ STAGE bool stage_load_reduce(const Args& a, int64_t h0, int n) {
                     const int cur = L.cursor[dl];
                     if (v && d == dl) dest = cur + rank_in(m);
                     __syncthreads();
-                    if (lane == leader) L.cursor[dl] = cur + __popcll(m);
+                    if (lane == leader) L.cursor[dl] = (int16_t)(cur + __popcll(m));
                     __syncthreads();
                     todo &= ~m;
                 }
@@ -668,7 +674,7 @@ STAGE bool stage_load_reduce(const Args& a, int64_t h0, int n) {
                     L.keep[dest] = (fl & 2) ? 1 : -1;
                 }
             }
-            for (int h = lane; h < n; h += WAVE) L.korig[h] = 0;
+            for (int w = lane; w < MAXH / 32; w += WAVE) L.kbits[w] = 0;
             __syncthreads();
             // ---- B: reduction, one lane per plane (reconstruction_helper.cuh:168-361)
             int nred = 0;
@@ -740,14 +746,14 @@ STAGE bool stage_load_reduce(const Args& a, int64_t h0, int n) {
                 }
                 if (reduce && nred > P.max_reduced_per_plane) atomicOr(&L.status, (uint32_t)OR_EV_PLANE_OVERFLOW);
                 if (clover) atomicOr(&L.status, (uint32_t)OR_EV_CLUSTER_OVERFLOW);
-                L.rcnt[d] = nred;
+                L.rcnt[d] = (int16_t)nred;
             } else if (lane < 56) {
                 L.rcnt[lane] = 0;
             }
             {
                 const int c = (d >= 1 && d <= 54) ? nred : 0;
                 const int inc = wave_scan(c);
-                if (lane < 56) L.rstart[lane] = inc - c;
+                if (lane < 56) L.rstart[lane] = (int16_t)(inc - c);
                 // station multiplicities (reconstruction_kernels.cuh:127-139), 8 written rows summed
                 const int m0 = wave_sum((d >= 1 && d <= 6) ? nred : 0);
                 const int m2 = wave_sum((d >= 13 && d <= 18) ? nred : 0);
@@ -765,14 +771,29 @@ STAGE bool stage_load_reduce(const Args& a, int64_t h0, int n) {
                 }
             }
             __syncthreads();
-            // reduced lists, kept flags by input order
-            if (d >= 1 && d <= 54) {
-                const int s = L.pstart[d], np = L.pcnt[d];
-                int k = L.rstart[d];
-                for (int i = 0; i < np; ++i)
-                    if (L.keep[s + i] > 0) { L.red[k++] = (uint16_t)(s + i); L.korig[L.orig[s + i]] = 1; }
+            // compact the kept hits in place (planes stay in detector order, so the
+            // k-th kept hit of plane d lands at rstart[d] + k); destinations never
+            // pass their sources, so chunks taken in order read before any write
+            // can reach them.  Keep bits by input index on the way.
+            const int ntot = L.pstart[55];
+            int base = 0;
+            for (int h0s = 0; h0s < ntot; h0s += WAVE) {
+                const int h = h0s + lane;
+                const bool kept = h < ntot && L.keep[h] > 0;
+                int16_t el = 0;
+                float ps = 0.f, dr = 0.f;
+                uint16_t og = 0;
+                if (kept) { el = L.elem[h]; ps = L.pos[h]; dr = L.drift[h]; og = L.orig[h]; }
+                const unsigned long long m = ballot(kept);
+                __syncthreads();
+                if (kept) {
+                    const int t = base + rank_in(m);
+                    L.elem[t] = el; L.pos[t] = ps; L.drift[t] = dr; L.orig[t] = og;
+                    atomicOr(&L.kbits[og >> 5], 1u << (og & 31));
+                }
+                base += __popcll(m);
+                __syncthreads();
             }
-            __syncthreads();
         }
     }
     return __shfl(staged ? 1 : 0, 0, WAVE) != 0;
@@ -783,19 +804,30 @@ STAGE void stage_keep_bits(const Args& a, int64_t h0, int n) {
     Lds<MAXH>& L = g_lds<MAXH>;
     const int lane = lane_id();
     const int64_t w0 = h0 >> 5, w1 = (h0 + n - 1) >> 5;
+        // global word w holds input hits [32w, 32w + 32); event-local bit i is input hit h0 + i
+        const int sh = (int)(h0 & 31);
         for (int64_t w = w0 + lane; n > 0 && w <= w1; w += WAVE) {
-            uint32_t bits = 0;
-            for (int b = 0; b < 32; ++b) {
-                const int64_t h = w * 32 + b - h0;
-                if (h >= 0 && h < n && L.korig[h]) bits |= 1u << b;
+            const int64_t lo = w * 32 - h0;        // event-local index of bit 0 of word w
+            const int k = (int)((lo + sh) >> 5);   // lo + sh = 32 (w - w0): bits b >= sh come from local word k
+            uint32_t bits;
+            if (sh == 0) {
+                bits = L.kbits[k];
+            } else {
+                const uint32_t hi = k < MAXH / 32 ? L.kbits[k] : 0u;
+                const uint32_t lw = k >= 1 ? L.kbits[k - 1] : 0u;
+                bits = (lw >> (32 - sh)) | (hi << sh);
             }
+            // only hits of this event: local indices [0, n)
+            const int64_t a0 = lo < 0 ? -lo : 0, a1 = (lo + 32 > n) ? n - lo : 32;
+            const uint32_t msk = (a1 >= 32 ? 0xFFFFFFFFu : ((1u << a1) - 1u)) & ~(a0 >= 32 ? 0xFFFFFFFFu : ((1u << a0) - 1u));
+            bits &= msk;
             if (bits) atomicOr(&a.keep_mask[w], bits);
         }
     }
 
 // C: binned X pairs for station 2, D3p, D3m
 template <int MAXH>
-STAGE void stage_xpairs(const or_params& P, uint16_t* xpairs) {
+STAGE void stage_xpairs(or_params P, uint16_t* xpairs) {
     Lds<MAXH>& L = g_lds<MAXH>;
     Ev<MAXH> E;
     const int lane = lane_id();
@@ -852,7 +884,7 @@ STAGE void stage_xpairs(const or_params& P, uint16_t* xpairs) {
 
 // D: XZ combinations in canonical order, slot order.  Returns the tracklet count.
 template <int MAXH>
-STAGE int stage_xz(const or_params& P, const Scr& S) {
+STAGE int stage_xz(or_params P, Scr S) {
     Lds<MAXH>& L = g_lds<MAXH>;
     Ev<MAXH> E;
     const int lane = lane_id();
@@ -1049,7 +1081,7 @@ __device__ __forceinline__ void yz_candidate(const Ev<MAXH>& E, const Scr& S, co
 // Two-pass form.  Returns false (nothing written) when the survivors of the
 // first box cut do not fit the list.
 template <int MAXH>
-__device__ __attribute__((noinline)) bool yz_chunk_par(const Scr& S, int kb, int ntkl, int cap) {
+__device__ __attribute__((noinline)) bool yz_chunk_par(Scr S, int kb, int ntkl, int cap) {
     Lds<MAXH>& L = g_lds<MAXH>;
     Ev<MAXH> E;
     const int lane = lane_id();
@@ -1071,17 +1103,65 @@ __device__ __attribute__((noinline)) bool yz_chunk_par(const Scr& S, int kb, int
     __syncthreads();
     const int I = L.uoff[YZ_CHUNK];
     OR_SUB_START
-    // pass 1: first fit and box cut of every candidate; survivors in order
+    // pass 1: first fit and box cut of every candidate; survivors in order.
+    // Chunk constants go to scalar registers (candidate offsets, which
+    // tracklets are D3m, the plane z values of both station-3 choices) and the
+    // candidate's four list entries are loaded together.
+    int off[YZ_CHUNK + 1];
+#pragma unroll
+    for (int t = 0; t <= YZ_CHUNK; ++t) off[t] = __builtin_amdgcn_readfirstlane(L.uoff[t]);
+    const unsigned d3m = (unsigned)ballot(lane < YZ_CHUNK && kb + lane < ntkl && tks[kb + lane].st3 == 4);
+    float zv[4][2][2];   // [view u2, v2, u3, v3][station 3 is D3m][hit a, b]
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+#pragma unroll
+        for (int m = 0; m < 2; ++m) {
+            const int sid = v < 2 ? super_id(2, v + 1) : super_id(3 + m, v - 1);
+            zv[v][m][0] = G(F_Z, sid * 2);
+            zv[v][m][1] = G(F_Z, sid * 2 - 1);
+        }
     int np = 0;
     for (int i0 = 0; i0 < I; i0 += WAVE) {
         const int i = i0 + lane;
         bool pass = false;
         if (i < I) {
-            const YzItem it = yz_decode<MAXH>(i);
-            YzLists W;
-            yz_lists_of<MAXH>(S, it.j, W);
-            Fit f;
-            pass = yz_fit(W, tks[kb + it.j].st3, it, f) && yz_box1(f);
+            int j = 0;
+#pragma unroll
+            for (int t = 1; t < YZ_CHUNK; ++t) j += i >= off[t] ? 1 : 0;
+            const int4 nl = *reinterpret_cast<const int4*>(L.nl[j]);
+            int ojs = off[0];
+#pragma unroll
+            for (int t = 1; t < YZ_CHUNK; ++t) ojs = j == t ? off[t] : ojs;
+            const int local = i - ojs, nin = nl.x * nl.y;
+            const int iuv3 = local / nin, iuv2 = local - iuv3 * nin;
+            const int e[4] = {iuv2 % nl.x, iuv2 / nl.x, iuv3 % nl.z, iuv3 / nl.z};  // u2, v2, u3, v3
+            const int m = (d3m >> j) & 1;
+            uint8_t fl[4];
+            float4 yv[4];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int q = (j * 4 + v) * WIN_CAP + e[v];
+                fl[v] = S.wf[q];
+                yv[v] = S.wy[q];
+            }
+            float A0 = 0.f, A1 = 0.f, A3 = 0.f, B0 = 0.f, B1 = 0.f;
+            bool ok = true;
+            // reference order u3, v3, u2, v2; hit a then b
+#pragma unroll
+            for (int o = 0; o < 4; ++o) {
+                const int v = o < 2 ? o + 2 : o - 2;
+                const float za = m ? zv[v][1][0] : zv[v][0][0], zb = m ? zv[v][1][1] : zv[v][0][1];
+                ok = ok && fl[v] != 0;
+                if (fl[v] & 1) {
+                    B0 += yv[v].y * yv[v].x; B1 += (yv[v].y * yv[v].x) * za; A0 += yv[v].y; A1 += yv[v].y * za;
+                    A3 += (yv[v].y * za) * za;
+                }
+                if (fl[v] & 2) {
+                    B0 += yv[v].w * yv[v].z; B1 += (yv[v].w * yv[v].z) * zb; A0 += yv[v].w; A1 += yv[v].w * zb;
+                    A3 += (yv[v].w * zb) * zb;
+                }
+            }
+            if (ok) pass = yz_box1(fit_finish(A0, A1, A3, B0, B1));
         }
         const unsigned long long m = ballot(pass);
         const int c = __popcll(m);
@@ -1197,7 +1277,7 @@ __device__ __attribute__((noinline)) bool yz_chunk_par(const Scr& S, int kb, int
 
 // Unit loop: one lane per (tracklet, u3.v3) unit, candidates in reference order.
 template <int MAXH>
-__device__ __attribute__((noinline)) void yz_chunk_seq(const Scr& S, int kb, int ntkl) {
+__device__ __attribute__((noinline)) void yz_chunk_seq(Scr S, int kb, int ntkl) {
     Lds<MAXH>& L = g_lds<MAXH>;
     Ev<MAXH> E;
     const int lane = lane_id();
@@ -1264,7 +1344,7 @@ __device__ __attribute__((noinline)) void yz_chunk_seq(const Scr& S, int kb, int
 }
 
 template <int MAXH>
-STAGE void stage_yz(const or_params& P, const Scr& S, int ntkl, int list_cap) {
+STAGE void stage_yz(or_params P, Scr S, int ntkl, int list_cap) {
     Lds<MAXH>& L = g_lds<MAXH>;
     Ev<MAXH> E;
     const int lane = lane_id();
@@ -1497,7 +1577,7 @@ __device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long v)
 }
 
 template <int MAXH>
-STAGE void stage_global(const or_params& P, const Scr& S, int ntkl) {
+STAGE void stage_global(or_params P, Scr S, int ntkl) {
     Lds<MAXH>& L = g_lds<MAXH>;
     Ev<MAXH> E;
     const int lane = lane_id();
@@ -1743,9 +1823,96 @@ STAGE void stage_global(const or_params& P, const Scr& S, int ntkl) {
     }
 }
 
+// G': vertexing (gKernel_Vertexing, reconstruction_kernels.cuh:2547-2810), one lane
+// per global track.  The track is swum upstream from its most upstream hit
+// through FMAG (100 steps: pT kick, energy loss outside the beam hole) and the
+// target region (100 straight steps) in double precision; the trajectory point
+// closest to the beam line in x-y (first minimum, points with the x momentum on
+// the wrong side skipped) is the vertex.  The minimum is taken on the fly in the
+// reference's point order, so no trajectory arrays are kept.  Float and double
+// are mixed exactly as in the reference (sqrtf of a double argument rounds it
+// to float first).  Constants: reconstruction_constants.h:75-121 (E1039).
+constexpr double V_FMAG_LENGTH = 502.92, V_FMAG_HOLE_LENGTH = 27.94, V_FMAG_HOLE_RADIUS = 1.27;
+constexpr int V_NSTEPS_FMAG = 100, V_NSTEPS_TARGET = 100;
+constexpr double V_STEP_FMAG = 2.514600, V_STEP_TARGET = 5.0, V_PTKICK_UNIT = -0.006096568;
+constexpr double V_DEDX0 = 0.014282073, V_DEDX1 = 7.1869681e-05, V_DEDX2 = -1.4279150e-06,
+                 V_DEDX3 = 1.5853655e-08, V_DEDX4 = -6.0741470e-11;
+constexpr float V_X_BEAM = 0.f, V_Y_BEAM = 0.f;
+
+__device__ __forceinline__ double dedx_poly(double p) {
+    return V_DEDX0 + V_DEDX1 * p + V_DEDX2 * p * p + V_DEDX3 * p * p * p + V_DEDX4 * p * p * p * p;
+}
+
+__device__ void vertex_track(Tk& T) {
+    const float x0 = T.x0, y0 = T.y0, tx = T.tx, ty = T.ty, invP = T.invp, charge = (float)T.charge;
+    const int nh = T.nx + T.nuv + T.n1;
+    // get_firsthitdetid (reconstruction_classes.cuh:517-531)
+    int det_first = 30;
+    float zmin = 10000.f;
+    for (int i = 0; i < nh; ++i) {
+        const int d = T.det[i];
+        if (d > 0 && G(F_Z, d) < zmin) { det_first = d; zmin = G(F_Z, d); }
+    }
+    // calculate_x0_tx_st1 (reconstruction_helper.cuh:622-626), charge as short
+    const short q = (short)charge;
+    const float tx_st1 = tx + PT_KICK_KMAG * invP * q;
+    const float x0_st1 = tx * Z_KMAG_BEND + x0 - tx_st1 * Z_KMAG_BEND;
+    const double z0 = G(F_Z, det_first);
+    const double s0 = charge * invP * sqrtf((1.f + tx_st1 * tx_st1) / (1.f + tx_st1 * tx_st1 + ty * ty));
+    const double s1 = tx_st1, s2 = ty;
+    const double s3 = (float)z0 * tx_st1 + x0_st1;   // x_trk / y_trk take float z
+    const double s4 = (float)z0 * ty + y0;
+    double px = s3 + s1 * (V_FMAG_LENGTH - z0), py = s4 + s2 * (V_FMAG_LENGTH - z0), pz = V_FMAG_LENGTH;
+    const double pz0 = 1.f / (fabs(s0) * sqrtf((float)(1.f + s1 * s1 + s2 * s2)));
+    double mx = pz0 * s1, my = pz0 * s2, mz = pz0;
+    double dmin = 1.e18;
+    bool found = false;
+    double v0 = 0, v1 = 0, v2 = 0, v3 = 0, v4 = 0, v5 = 0;
+    auto consider = [&]() {
+        if (FMAGSTR * charge * mx < 0.) return;
+        const double d2 = (px - V_X_BEAM) * (px - V_X_BEAM) + (py - V_Y_BEAM) * (py - V_Y_BEAM);
+        if (d2 < dmin) { dmin = d2; found = true; v0 = px; v1 = py; v2 = pz; v3 = mx; v4 = my; v5 = mz; }
+    };
+    consider();
+    for (int step = 1; step <= V_NSTEPS_FMAG; ++step) {
+        const double tx_i = mx / mz;
+        const double tx_f = tx_i + 2.f * charge * V_PTKICK_UNIT * V_STEP_FMAG / sqrt(mx * mx + mz * mz);
+        const double t1x = tx_i * V_STEP_FMAG, t1y = ty * V_STEP_FMAG, t1z = V_STEP_FMAG;
+        const double bx = px - t1x, by = py - t1y, bz = pz - t1z;
+        const double ptot_i = sqrt(mx * mx + my * my + mz * mz);
+        double ptot_b = ptot_i;
+        if (bz > V_FMAG_HOLE_LENGTH || bx * bx + by * by > V_FMAG_HOLE_RADIUS)
+            ptot_b += dedx_poly(ptot_i) * sqrtf((float)(t1x * t1x + t1y * t1y + t1z * t1z));
+        const double t2x = tx_f * V_STEP_FMAG, t2y = ty * V_STEP_FMAG, t2z = V_STEP_FMAG;
+        px = bx - t2x; py = by - t2y; pz = bz - t2z;
+        double ptot_f = ptot_b;
+        if (pz > V_FMAG_HOLE_LENGTH || px * px + py * py > V_FMAG_HOLE_RADIUS)
+            ptot_f += dedx_poly(ptot_b) * sqrtf((float)(t2x * t2x + t2y * t2y + t2z * t2z));
+        const double pz_f = ptot_f / sqrt(1.f + tx_f * tx_f + ty * ty);
+        mx = pz_f * tx_f; my = pz_f * ty; mz = pz_f;
+        consider();
+    }
+    for (int step = 1; step <= V_NSTEPS_TARGET; ++step) {
+        const double tx_i = mx / mz;
+        px -= tx_i * V_STEP_TARGET; py -= ty * V_STEP_TARGET; pz -= V_STEP_TARGET;
+        consider();
+    }
+    if (!found) { v0 = px; v1 = py; v2 = pz; v3 = mx; v4 = my; v5 = mz; }
+    T.stid = 7;
+    T.vtx[0] = (float)v0; T.vtx[1] = (float)v1; T.vtx[2] = (float)v2;
+    T.vtx[3] = (float)v3; T.vtx[4] = (float)v4; T.vtx[5] = (float)v5;
+}
+
+template <int MAXH>
+STAGE void stage_vertex(Tk* tks, int ntkl) {
+    for (int k = lane_id(); k < ntkl; k += WAVE)
+        if (tks[k].stid == 6) vertex_track(tks[k]);
+    __syncthreads();
+}
+
 // H: outputs
 template <int MAXH>
-STAGE void stage_out(const Args& a, const Scr& S, int64_t e, int ntkl) {
+STAGE void stage_out(const Args& a, Scr S, int64_t e, int ntkl) {
     Lds<MAXH>& L = g_lds<MAXH>;
     const int lane = lane_id();
     const or_params& P = a.p;
@@ -1757,7 +1924,7 @@ STAGE void stage_out(const Args& a, const Scr& S, int64_t e, int ntkl) {
         for (int j0 = 0; j0 < ntkl; j0 += WAVE) {
             const int j = j0 + lane;
             const Tk* Tp = j < ntkl ? &tks[S.slot_order[j]] : nullptr;
-            const bool is6 = Tp && Tp->stid == 6;
+            const bool is6 = Tp && Tp->stid >= 6;
             const unsigned long long m = ballot(is6);
             if (is6) {
                 const int o = nout + rank_in(m);
@@ -1778,6 +1945,8 @@ STAGE void stage_out(const Args& a, const Scr& S, int64_t e, int ntkl) {
                     R.station_id = T.stid; R.nhits = (uint8_t)nh; R.charge = T.charge; R.slot = T.slot;
                     R.tracklet_id = T.canon;
                     a.tracks[e * P.max_tracks_out + o] = R;
+                    if (a.vertex)
+                        for (int c = 0; c < 6; ++c) a.vertex[(e * P.max_tracks_out + o) * 6 + c] = T.vtx[c];
                 }
             }
             nout += __popcll(m);
@@ -1815,6 +1984,7 @@ STAGE void stage_out(const Args& a, const Scr& S, int64_t e, int ntkl) {
                     case 13: v = T.ey0; break;
                     case 14: v = T.einvp; break;
                     case 15: v = (float)T.charge; break;
+                    case 16: case 17: case 18: case 19: case 20: case 21: v = T.stid == 7 ? T.vtx[f - 16] : 0.f; break;
                     default:
                         if (f >= 22 && f < 130 + 18) {
                             const int blk = (f - 22) / 18, h = (f - 22) % 18;
@@ -1851,7 +2021,10 @@ __device__ unsigned long long g_stage_cycles[NCLASS_MAX][N_STAGE_TIMERS];
 #endif
 
 template <int MAXH>
-__global__ __launch_bounds__(WAVE, MAXH <= 512 ? 4 : (MAXH <= 1024 ? 2 : 1)) void chain_kernel(Args a) {
+#ifndef OR_WAVES_PER_SIMD_512
+#define OR_WAVES_PER_SIMD_512 4
+#endif
+__global__ __launch_bounds__(WAVE, MAXH <= 512 ? OR_WAVES_PER_SIMD_512 : (MAXH <= 1024 ? 2 : 1)) void chain_kernel(Args a) {
     Lds<MAXH>& L = g_lds<MAXH>;
     const int lane = lane_id();
     char* scr = a.scratch + (int64_t)blockIdx.x * a.scratch_stride;
@@ -1893,6 +2066,7 @@ __global__ __launch_bounds__(WAVE, MAXH <= 512 ? 4 : (MAXH <= 1024 ? 2 : 1)) voi
         if (L.status == 0) stage_clean<MAXH>(S.tks, S.slot_order, ntkl);
         OR_TIMER(4)
         if (L.status == 0) stage_global<MAXH>(a.p, S, ntkl);
+        if (L.status == 0 && a.vertex) stage_vertex<MAXH>(S.tks, ntkl);
         OR_TIMER(5)
         stage_out<MAXH>(a, S, e, ntkl);
         __syncthreads();
@@ -1974,6 +2148,16 @@ struct or_session {
     mutable float last_ms = 0.f;
     mutable bool timing_pending = false;   // ev0/ev1 recorded, elapsed time not read yet
     int yz_list_cap = YZ_LIST_CAP;
+    int grid_pct = 100;          // persistent grid, percent of the occupancy limit
+    bool use_graph = false;      // replay the launch sequence as a captured hipGraph
+    hipGraphExec_t graph_exec = nullptr;
+    struct GraphKey {
+        const void* p[14];
+        int64_t n_events, n_hits;
+        hipStream_t stream;
+        int yz_list_cap, grid_pct;
+        bool operator==(const GraphKey& o) const { return std::memcmp(this, &o, sizeof o) == 0; }
+    } graph_key;
     // host staging (host-pointer calls)
     void* d_in = nullptr;
     size_t d_in_bytes = 0;
@@ -2066,7 +2250,7 @@ or_status or_session_create(const or_geometry* g, const or_params* p, int device
 
 or_status or_session_set_stream(or_session* s, void* stream) {
     if (!s) return OR_ERR_INVALID_ARG;
-    s->stream = stream ? (hipStream_t)stream : s->own;
+    s->stream = (hipStream_t)stream;   // NULL: the null stream, ordered with the default-stream work of the caller
     return OR_OK;
 }
 
@@ -2094,11 +2278,22 @@ or_status or_session_set_tuning(or_session* s, or_tuning key, int64_t value) {
             if (value < 1 || value > YZ_LIST_CAP) return OR_ERR_INVALID_ARG;
             s->yz_list_cap = (int)value;
             return OR_OK;
+        case OR_TUNE_GRID_PERCENT:
+            if (value < 1 || value > 100) return OR_ERR_INVALID_ARG;
+            s->grid_pct = (int)value;
+            return OR_OK;
+        case OR_TUNE_GRAPH:
+            if (value != 0 && value != 1) return OR_ERR_INVALID_ARG;
+            s->use_graph = value != 0;
+            return OR_OK;
     }
     return OR_ERR_INVALID_ARG;
 }
 
-static or_status run_device(or_session* s, const or_hits* in, const or_result* out, bool timed) {
+// The launch sequence of one call: keep-mask fill, class counters fill,
+// classification, one persistent launch per tile class.  No memset nodes, so
+// it can be captured into a graph.
+static or_status issue_chain(or_session* s, const or_hits* in, const or_result* out) {
     Args a;
     a.det = in->det; a.elem = in->elem; a.tdc = in->tdc; a.drift = in->drift; a.flag = in->flag;
     a.off = in->event_offsets; a.n_events = in->n_events;
@@ -2106,7 +2301,7 @@ static or_status run_device(or_session* s, const or_hits* in, const or_result* o
     a.wire = s->d_wire;
     a.p = s->p;
     a.status = out->status; a.keep_mask = out->keep_mask; a.n_tracks = out->n_tracks; a.tracks = out->tracks;
-    a.n_tracklets = out->n_tracklets; a.records = out->records;
+    a.n_tracklets = out->n_tracklets; a.records = out->records; a.vertex = out->vertex;
     a.scratch = s->d_scratch;
     a.scratch_stride = SCR_STRIDE;
     a.yz_list_cap = s->yz_list_cap;
@@ -2115,30 +2310,75 @@ static or_status run_device(or_session* s, const or_hits* in, const or_result* o
         hipLaunchKernelGGL(zero_u32, dim3(1024), dim3(256), 0, s->stream, out->keep_mask, nw);
         OR_HIP(hipGetLastError());
     }
-    if (in->n_events == 0) {
-        s->last_ms = 0.f;
-        s->timing_pending = false;
-        return OR_OK;
-    }
-    if (s->list_cap < in->n_events) {
-        if (s->d_lists) (void)hipFree(s->d_lists);
-        s->d_lists = nullptr;
-        OR_HIP(hipMalloc(&s->d_lists, sizeof(int32_t) * NCLASS * (size_t)in->n_events));
-        s->list_cap = in->n_events;
-    }
     hipLaunchKernelGGL(zero_u32, dim3(1), dim3(64), 0, s->stream, reinterpret_cast<uint32_t*>(s->d_counts),
                        (int64_t)(2 * NCLASS));
     OR_HIP(hipGetLastError());
     hipLaunchKernelGGL(classify_events, dim3(std::min<int64_t>(1024, (in->n_events + 255) / 256)), dim3(256), 0,
                        s->stream, in->event_offsets, in->n_events, s->nclass, s->d_lists, s->list_cap, s->d_counts);
     OR_HIP(hipGetLastError());
-    if (timed) OR_HIP(hipEventRecord(s->ev0, s->stream));
     for (int c = 0; c < s->nclass; ++c) {
         a.ev_list = s->d_lists + c * s->list_cap;
         a.ev_count = s->d_counts + c;
         a.ev_head = s->d_counts + NCLASS + c;
-        const int grid = (int)std::min<int64_t>(s->grid_c[c], in->n_events);
+        const int64_t g = std::max<int64_t>(1, (int64_t)s->grid_c[c] * s->grid_pct / 100);
+        const int grid = (int)std::min<int64_t>(g, in->n_events);
         OR_HIP(launch_class(c, a, grid, s->stream));
+    }
+    return OR_OK;
+}
+
+static or_status run_device(or_session* s, const or_hits* in, const or_result* out, bool timed) {
+    if (in->n_events == 0) {
+        const int64_t nw = (in->n_hits + 31) / 32;
+        if (nw > 0) {
+            hipLaunchKernelGGL(zero_u32, dim3(1024), dim3(256), 0, s->stream, out->keep_mask, nw);
+            OR_HIP(hipGetLastError());
+        }
+        s->last_ms = 0.f;
+        s->timing_pending = false;
+        return OR_OK;
+    }
+    if (s->list_cap < in->n_events) {
+        if (s->graph_exec) { (void)hipGraphExecDestroy(s->graph_exec); s->graph_exec = nullptr; }
+        if (s->d_lists) (void)hipFree(s->d_lists);
+        s->d_lists = nullptr;
+        OR_HIP(hipMalloc(&s->d_lists, sizeof(int32_t) * NCLASS * (size_t)in->n_events));
+        s->list_cap = in->n_events;
+    }
+    if (!s->use_graph) {
+        if (timed) OR_HIP(hipEventRecord(s->ev0, s->stream));
+        const or_status st = issue_chain(s, in, out);
+        if (st != OR_OK) return st;
+    } else {
+        or_session::GraphKey key;
+        std::memset(&key, 0, sizeof key);
+        const void* ptrs[14] = {in->det, in->elem, in->tdc, in->drift, in->flag, in->event_offsets, out->status,
+                                out->keep_mask, out->n_tracks, out->tracks, out->n_tracklets, out->records,
+                                out->vertex, s->d_lists};
+        std::memcpy(key.p, ptrs, sizeof ptrs);
+        key.n_events = in->n_events; key.n_hits = in->n_hits; key.stream = s->stream;
+        key.yz_list_cap = s->yz_list_cap; key.grid_pct = s->grid_pct;
+        if (!s->graph_exec || !(key == s->graph_key)) {
+            if (s->graph_exec) { (void)hipGraphExecDestroy(s->graph_exec); s->graph_exec = nullptr; }
+            // captured on the session's own stream (the null stream cannot capture),
+            // launched on the caller's stream
+            hipGraph_t g = nullptr;
+            hipStream_t launch = s->stream;
+            const hipError_t be = hipStreamBeginCapture(s->own, hipStreamCaptureModeThreadLocal);
+            OR_HIP(be);
+            s->stream = s->own;
+            const or_status st = issue_chain(s, in, out);
+            const hipError_t ce = hipStreamEndCapture(s->own, &g);
+            s->stream = launch;
+            if (st != OR_OK) { if (g) (void)hipGraphDestroy(g); return st; }
+            OR_HIP(ce);
+            const hipError_t ie = hipGraphInstantiate(&s->graph_exec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            OR_HIP(ie);
+            std::memcpy(&s->graph_key, &key, sizeof key);
+        }
+        if (timed) OR_HIP(hipEventRecord(s->ev0, s->stream));
+        OR_HIP(hipGraphLaunch(s->graph_exec, s->stream));
     }
     if (timed) {
         OR_HIP(hipEventRecord(s->ev1, s->stream));
@@ -2164,7 +2404,8 @@ or_status or_reconstruct(or_session* s, const or_hits* in, or_result* out) {
     const size_t in_bytes = (size_t)nh * (2 + 2 + 4 + 4 + 1) + (size_t)(ne + 1) * 8 + 256 * 8;
     const size_t tk_bytes = (size_t)ne * s->p.max_tracks_out * sizeof(or_track);
     const size_t rec_bytes = out->records ? (size_t)ne * s->p.max_tracklets_per_event * OR_TRACK_PARAMS * 4 : 0;
-    const size_t out_bytes = (size_t)ne * 4 * 3 + (size_t)nw * 4 + tk_bytes + rec_bytes + 256 * 8;
+    const size_t vtx_bytes = out->vertex ? (size_t)ne * s->p.max_tracks_out * 6 * 4 : 0;
+    const size_t out_bytes = (size_t)ne * 4 * 3 + (size_t)nw * 4 + tk_bytes + rec_bytes + vtx_bytes + 256 * 8;
     if (s->d_in_bytes < in_bytes) {
         if (s->d_in) (void)hipFree(s->d_in);
         s->d_in = nullptr;
@@ -2202,6 +2443,7 @@ or_status or_reconstruct(or_session* s, const or_hits* in, or_result* out) {
     dout.keep_mask = (uint32_t*)carve(po, nw * 4);
     dout.tracks = (or_track*)carve(po, tk_bytes);
     dout.records = out->records ? (float*)carve(po, rec_bytes) : nullptr;
+    dout.vertex = out->vertex ? (float*)carve(po, vtx_bytes) : nullptr;
     dout.on_device = 1;
     or_status rs = run_device(s, &din, &dout, true);
     if (rs != OR_OK) return rs;
@@ -2211,7 +2453,31 @@ or_status or_reconstruct(or_session* s, const or_hits* in, or_result* out) {
     if (nw) OR_HIP(hipMemcpyAsync(out->keep_mask, dout.keep_mask, nw * 4, hipMemcpyDeviceToHost, st));
     if (tk_bytes) OR_HIP(hipMemcpyAsync(out->tracks, dout.tracks, tk_bytes, hipMemcpyDeviceToHost, st));
     if (rec_bytes) OR_HIP(hipMemcpyAsync(out->records, dout.records, rec_bytes, hipMemcpyDeviceToHost, st));
+    if (vtx_bytes) OR_HIP(hipMemcpyAsync(out->vertex, dout.vertex, vtx_bytes, hipMemcpyDeviceToHost, st));
     OR_HIP(hipStreamSynchronize(st));
+    return OR_OK;
+}
+
+or_status or_session_autotune(or_session* s, const or_hits* in, or_result* out, int reps, float* best_ms) {
+    if (!s || !in || !out || !in->on_device || !out->on_device || reps < 1) return OR_ERR_INVALID_ARG;
+    static const int grid_pcts[] = {100, 75, 50};
+    int best_pct = s->grid_pct;
+    float best = -1.f;
+    for (int pct : grid_pcts) {
+        s->grid_pct = pct;
+        or_status st = or_reconstruct(s, in, out);   // warm-up at this setting
+        if (st != OR_OK) return st;
+        float tot = 0.f;
+        for (int r = 0; r < reps; ++r) {
+            st = or_reconstruct(s, in, out);
+            if (st != OR_OK) return st;
+            tot += or_session_last_kernel_ms(s);
+        }
+        const float ms = tot / (float)reps;
+        if (best < 0.f || ms < best) { best = ms; best_pct = pct; }
+    }
+    s->grid_pct = best_pct;
+    if (best_ms) *best_ms = best;
     return OR_OK;
 }
 
@@ -2224,6 +2490,7 @@ void or_session_destroy(or_session* s) {
     if (s->d_counts) (void)hipFree(s->d_counts);
     if (s->d_in) (void)hipFree(s->d_in);
     if (s->d_out) (void)hipFree(s->d_out);
+    if (s->graph_exec) (void)hipGraphExecDestroy(s->graph_exec);
     if (s->ev0) (void)hipEventDestroy(s->ev0);
     if (s->ev1) (void)hipEventDestroy(s->ev1);
     if (s->own) (void)hipStreamDestroy(s->own);

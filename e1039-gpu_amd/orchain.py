@@ -45,6 +45,8 @@ EV_BAD_HIT = 1 << 10
 EV_OUTPUT_OVERFLOW = 1 << 11
 EV_REJECT_MASK = 0x7FF
 TUNE_YZ_LIST_CAP = 1
+TUNE_GRID_PERCENT = 2
+TUNE_GRAPH = 3
 
 PARAM_FIELDS = [
     "max_hits_chamber_plane", "max_hits_hodo_plane", "max_hits_prop_plane", "max_reduced_per_plane",
@@ -69,7 +71,7 @@ class OrHits(C.Structure):
 
 class OrResult(C.Structure):
     _fields_ = [("status", C.c_void_p), ("keep_mask", C.c_void_p), ("n_tracks", C.c_void_p), ("tracks", C.c_void_p),
-                ("n_tracklets", C.c_void_p), ("records", C.c_void_p), ("on_device", C.c_int32)]
+                ("n_tracklets", C.c_void_p), ("records", C.c_void_p), ("vertex", C.c_void_p), ("on_device", C.c_int32)]
 
 
 class OrGenParams(C.Structure):
@@ -132,6 +134,8 @@ def load_library(path: str = LIB_PATH):
     lib.or_session_last_kernel_ms.restype = C.c_float
     lib.or_session_set_tuning.argtypes = [vp, C.c_int, i64]
     lib.or_session_set_tuning.restype = i32
+    lib.or_session_autotune.argtypes = [vp, C.POINTER(OrHits), C.POINTER(OrResult), C.c_int, C.POINTER(C.c_float)]
+    lib.or_session_autotune.restype = i32
     lib.or_session_destroy.argtypes = [vp]
     lib.or_status_string.argtypes = [i32]
     lib.or_status_string.restype = C.c_char_p
@@ -252,6 +256,7 @@ class Result:
     tracks: np.ndarray       # TRACK_DTYPE, (n_events, max_tracks_out)
     n_tracklets: np.ndarray | None = None
     records: np.ndarray | None = None  # float32 (n_events, max_tracklets_per_event, 148)
+    vertex: np.ndarray | None = None   # float32 (n_events, max_tracks_out, 6): vx, vy, vz, px, py, pz
 
     def valid_tracks(self) -> np.ndarray:
         """Tracks of all events, slots [0, n_tracks[e]) of each event in event
@@ -308,7 +313,8 @@ class Session:
         }
         return t
 
-    def alloc_outputs(self, n_events: int, n_hits: int, tracklets: bool = False, records: bool = False):
+    def alloc_outputs(self, n_events: int, n_hits: int, tracklets: bool = False, records: bool = False,
+                      vertex: bool = False):
         import torch
         dev = torch.device("cuda", self.device)
         p = self.params
@@ -320,11 +326,12 @@ class Session:
             "n_tracklets": torch.empty(n_events, dtype=torch.int32, device=dev) if tracklets else None,
             "records": (torch.empty(n_events * p.max_tracklets_per_event * TRACK_PARAMS, dtype=torch.float32,
                                     device=dev) if records else None),
+            "vertex": (torch.empty(max(1, n_events * p.max_tracks_out * 6), dtype=torch.float32, device=dev)
+                       if vertex else None),
         }
         return o
 
-    def run_device(self, t, o):
-        """Asynchronous-free device call: inputs and outputs already in HBM."""
+    def _device_args(self, t, o):
         import torch
         stream = torch.cuda.current_stream(self.device)
         _check(self.lib.or_session_set_stream(self.handle, C.c_void_p(stream.cuda_stream)))
@@ -333,8 +340,21 @@ class Session:
                       t["offsets"].data_ptr(), 1)
         res = OrResult(o["status"].data_ptr(), o["keep_mask"].data_ptr(), o["n_tracks"].data_ptr(),
                        o["tracks"].data_ptr(), None if o["n_tracklets"] is None else o["n_tracklets"].data_ptr(),
-                       None if o["records"] is None else o["records"].data_ptr(), 1)
+                       None if o["records"] is None else o["records"].data_ptr(),
+                       None if o.get("vertex") is None else o["vertex"].data_ptr(), 1)
+        return hits, res
+
+    def run_device(self, t, o):
+        """Device call on torch's current stream: inputs and outputs already in HBM (asynchronous)."""
+        hits, res = self._device_args(t, o)
         _check(self.lib.or_reconstruct(self.handle, C.byref(hits), C.byref(res)))
+
+    def autotune(self, t, o, reps: int = 2) -> float:
+        """or_session_autotune on a device-resident batch; returns the best mean chain time in ms."""
+        hits, res = self._device_args(t, o)
+        best = C.c_float(0.0)
+        _check(self.lib.or_session_autotune(self.handle, C.byref(hits), C.byref(res), reps, C.byref(best)))
+        return float(best.value)
 
     @staticmethod
     def fetch(o, n_events: int, max_tracks_out: int, max_tkl: int) -> Result:
@@ -347,18 +367,20 @@ class Session:
             n_tracklets=None if o["n_tracklets"] is None else o["n_tracklets"].cpu().numpy(),
             records=(None if o["records"] is None else
                      o["records"].cpu().numpy().reshape(n_events, max_tkl, TRACK_PARAMS)),
+            vertex=(None if o.get("vertex") is None else
+                    o["vertex"].cpu().numpy()[: n_events * max_tracks_out * 6].reshape(n_events, max_tracks_out, 6)),
         )
 
-    def reconstruct(self, ev: Events, tracklets: bool = True, records: bool = False) -> Result:
+    def reconstruct(self, ev: Events, tracklets: bool = True, records: bool = False, vertex: bool = False) -> Result:
         """Uploads, runs on the device, downloads."""
         import torch
         t = self.upload(ev)
-        o = self.alloc_outputs(ev.n_events, ev.n_hits, tracklets, records)
+        o = self.alloc_outputs(ev.n_events, ev.n_hits, tracklets, records, vertex)
         self.run_device(t, o)
         torch.cuda.synchronize(self.device)
         return self.fetch(o, ev.n_events, self.params.max_tracks_out, self.params.max_tracklets_per_event)
 
-    def reconstruct_host(self, ev: Events, records: bool = False) -> Result:
+    def reconstruct_host(self, ev: Events, records: bool = False, vertex: bool = False) -> Result:
         """Host buffers in and out; the library stages through device memory."""
         p = self.params
         ne, nh = ev.n_events, ev.n_hits
@@ -368,11 +390,12 @@ class Session:
         tracks = np.zeros((ne, p.max_tracks_out), TRACK_DTYPE)
         ntkl = np.zeros(ne, np.int32)
         rec = np.zeros((ne, p.max_tracklets_per_event, TRACK_PARAMS), np.float32) if records else None
+        vtx = np.zeros((ne, p.max_tracks_out, 6), np.float32) if vertex else None
         cont = [np.ascontiguousarray(a) for a in (ev.det, ev.elem, ev.tdc, ev.drift, ev.offsets)]
         flag = None if ev.flag is None else np.ascontiguousarray(ev.flag)
         hits = OrHits(ne, nh, cont[0].ctypes.data, cont[1].ctypes.data, cont[2].ctypes.data, cont[3].ctypes.data,
                       None if flag is None else flag.ctypes.data, cont[4].ctypes.data, 0)
         res = OrResult(status.ctypes.data, keep.ctypes.data, ntr.ctypes.data, tracks.ctypes.data, ntkl.ctypes.data,
-                       None if rec is None else rec.ctypes.data, 0)
+                       None if rec is None else rec.ctypes.data, None if vtx is None else vtx.ctypes.data, 0)
         _check(self.lib.or_reconstruct(self.handle, C.byref(hits), C.byref(res)))
-        return Result(status, keep, ntr, tracks, ntkl, rec)
+        return Result(status, keep, ntr, tracks, ntkl, rec, vtx)

@@ -127,6 +127,10 @@ typedef struct {
     int32_t* n_tracklets;    /* optional, n_events: XZ tracklets found */
     float* records;          /* optional, n_events * max_tracklets_per_event * 148 floats:
                                 every tracklet in slot order, reference record layout */
+    float* vertex;           /* optional, n_events * max_tracks_out * 6 floats (vx, vy, vz, px, py, pz):
+                                when set, global tracks are swum to the beam line (stationID 7,
+                                gKernel_Vertexing, reconstruction_kernels.cuh:2547-2810) and records
+                                carry the same values in fields 16-21 */
     int32_t on_device;       /* 1: device pointers */
 } or_result;
 
@@ -162,7 +166,8 @@ or_status or_reconstruct(or_session* s, const or_hits* in, or_result* out);
 /* Waits for the session stream. */
 or_status or_session_sync(or_session* s);
 
-/* Sets the HIP stream (hipStream_t) work is issued on; NULL = session stream. */
+/* Sets the HIP stream (hipStream_t) work is issued on; NULL = the null
+ * (default) stream.  Until the first call the session uses a stream of its own. */
 or_status or_session_set_stream(or_session* s, void* stream);
 
 /* Device time in ms of the last or_reconstruct's chain kernels (all size
@@ -176,9 +181,21 @@ typedef enum {
     /* YZ candidates surviving the first box cut that one chunk of 16
      * tracklets keeps for parallel evaluation, 1..4096 (default 4096); a chunk
      * with more is evaluated unit by unit instead */
-    OR_TUNE_YZ_LIST_CAP = 1
+    OR_TUNE_YZ_LIST_CAP = 1,
+    /* persistent grid size in percent of the occupancy limit, 1..100 (default 100) */
+    OR_TUNE_GRID_PERCENT = 2,
+    /* 1: capture the launch sequence of a call into a hipGraph and replay it
+     * while the buffers, sizes and stream stay the same; 0 (default): launch */
+    OR_TUNE_GRAPH = 3
 } or_tuning;
 or_status or_session_set_tuning(or_session* s, or_tuning key, int64_t value);
+
+/* Times the device-resident batch (in, out) at each persistent grid size
+ * (100, 75, 50 % of the occupancy limit; reps timed calls after one warm-up
+ * each), keeps the fastest setting in the session and returns its mean chain
+ * time in *best_ms.  Outputs are those of the last call.  No reference
+ * counterpart. */
+or_status or_session_autotune(or_session* s, const or_hits* in, or_result* out, int reps, float* best_ms);
 
 /* Profiling: device cycles per chain stage summed over all waves since the last
  * reset, out[class * 7 + stage] for the 5 tile classes (512 ... 8192 hits) and

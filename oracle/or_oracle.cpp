@@ -1057,6 +1057,86 @@ static bool run_global_one(Ctx& C, Tkl& T) {
 }
 
 /* ---- one event: staging (online_reconstruction.cu:506-640) + eR + chain ---- */
+/* ---- vertexing: gKernel_Vertexing, reconstruction_kernels.cuh:2547-2810 -------
+ * Swims a global track from its most upstream hit through FMAG (100 steps with
+ * pT kicks and energy loss outside the beam hole) and the target region (100
+ * straight steps), then takes the trajectory point closest to the beam line in
+ * x-y (first minimum, points whose x momentum points the wrong way skipped).
+ * Constants: reconstruction_constants.h:75-121 (E1039 set).  The reference mixes
+ * float and double; every expression keeps its types (sqrtf on a double
+ * argument rounds the argument to float first).  The dump-plane point the
+ * reference also computes is not part of its output and is not computed. */
+constexpr double V_FMAG_LENGTH = 502.92, V_FMAG_HOLE_LENGTH = 27.94, V_FMAG_HOLE_RADIUS = 1.27;
+constexpr int V_NSTEPS_FMAG = 100, V_NSTEPS_TARGET = 100;
+constexpr double V_STEP_FMAG = 2.514600, V_STEP_TARGET = 5.0, V_PTKICK_UNIT = -0.006096568;
+constexpr double V_DEDX0 = 0.014282073, V_DEDX1 = 7.1869681e-05, V_DEDX2 = -1.4279150e-06, V_DEDX3 = 1.5853655e-08,
+                 V_DEDX4 = -6.0741470e-11;
+constexpr float V_X_BEAM = 0.f, V_Y_BEAM = 0.f;
+
+static double dedx_poly(double p) {
+    return V_DEDX0 + V_DEDX1 * p + V_DEDX2 * p * p + V_DEDX3 * p * p * p + V_DEDX4 * p * p * p * p;
+}
+
+/* R: the track's 148-float record; writes stationID 7 and fields 16-21 */
+static void vertex_track(const Geo& g, float* R) {
+    const float x0 = R[7], y0 = R[8], tx = R[5], ty = R[6], invP = R[9], charge = R[15];
+    const int nh = (int)R[2];
+    /* get_firsthitdetid, reconstruction_classes.cuh:517-531 */
+    short det_first = 30;
+    float zmin = 10000.f;
+    for (int i = 0; i < nh; ++i) {
+        const short d = (short)R[22 + i];
+        if (d > 0 && g.z[d] < zmin) { det_first = d; zmin = g.z[d]; }
+    }
+    /* calculate_x0_tx_st1, reconstruction_helper.cuh:622-626 (charge as short) */
+    const short q = (short)charge;
+    const float tx_st1 = tx + PT_KICK_KMAG * invP * q;
+    const float x0_st1 = tx * Z_KMAG_BEND + x0 - tx_st1 * Z_KMAG_BEND;
+    const double z0 = g.z[det_first];
+    const double s0 = charge * invP * sqrtf((1.f + tx_st1 * tx_st1) / (1.f + tx_st1 * tx_st1 + ty * ty));
+    const double s1 = tx_st1, s2 = ty;
+    const double s3 = (float)z0 * tx_st1 + x0_st1;   /* x_trk / y_trk take float z */
+    const double s4 = (float)z0 * ty + y0;
+    double px = s3 + s1 * (V_FMAG_LENGTH - z0), py = s4 + s2 * (V_FMAG_LENGTH - z0), pz = V_FMAG_LENGTH;
+    const double pz0 = 1.f / (std::fabs(s0) * sqrtf((float)(1.f + s1 * s1 + s2 * s2)));
+    double mx = pz0 * s1, my = pz0 * s2, mz = pz0;
+    double dmin = 1.e18;
+    bool found = false;
+    double v[6] = {0, 0, 0, 0, 0, 0};
+    auto consider = [&]() {
+        if (FMAGSTR * charge * mx < 0.) return;
+        const double d2 = (px - V_X_BEAM) * (px - V_X_BEAM) + (py - V_Y_BEAM) * (py - V_Y_BEAM);
+        if (d2 < dmin) { dmin = d2; found = true; v[0] = px; v[1] = py; v[2] = pz; v[3] = mx; v[4] = my; v[5] = mz; }
+    };
+    consider();
+    for (int step = 1; step <= V_NSTEPS_FMAG; ++step) {
+        const double tx_i = mx / mz;
+        const double tx_f = tx_i + 2.f * charge * V_PTKICK_UNIT * V_STEP_FMAG / std::sqrt(mx * mx + mz * mz);
+        const double t1x = tx_i * V_STEP_FMAG, t1y = ty * V_STEP_FMAG, t1z = V_STEP_FMAG;
+        const double bx = px - t1x, by = py - t1y, bz = pz - t1z;
+        const double ptot_i = std::sqrt(mx * mx + my * my + mz * mz);
+        double ptot_b = ptot_i;
+        if (bz > V_FMAG_HOLE_LENGTH || bx * bx + by * by > V_FMAG_HOLE_RADIUS)
+            ptot_b += dedx_poly(ptot_i) * sqrtf((float)(t1x * t1x + t1y * t1y + t1z * t1z));
+        const double t2x = tx_f * V_STEP_FMAG, t2y = ty * V_STEP_FMAG, t2z = V_STEP_FMAG;
+        px = bx - t2x; py = by - t2y; pz = bz - t2z;
+        double ptot_f = ptot_b;
+        if (pz > V_FMAG_HOLE_LENGTH || px * px + py * py > V_FMAG_HOLE_RADIUS)
+            ptot_f += dedx_poly(ptot_b) * sqrtf((float)(t2x * t2x + t2y * t2y + t2z * t2z));
+        const double pz_f = ptot_f / std::sqrt(1.f + tx_f * tx_f + ty * ty);
+        mx = pz_f * tx_f; my = pz_f * ty; mz = pz_f;
+        consider();
+    }
+    for (int step = 1; step <= V_NSTEPS_TARGET; ++step) {
+        const double tx_i = mx / mz;
+        px -= tx_i * V_STEP_TARGET; py -= ty * V_STEP_TARGET; pz -= V_STEP_TARGET;
+        consider();
+    }
+    if (!found) { v[0] = px; v[1] = py; v[2] = pz; v[3] = mx; v[4] = my; v[5] = mz; }
+    R[0] = 7;
+    for (int k = 0; k < 6; ++k) R[16 + k] = (float)v[k];
+}
+
 static void run_event(const Geo& g, const or_params& P, const or_hits& in, int64_t e, const or_result& out,
                       std::vector<uint8_t>& keep_bytes, Ctx& C) {
     const int64_t h0 = in.event_offsets[e], h1 = in.event_offsets[e + 1];
@@ -1136,6 +1216,9 @@ static void run_event(const Geo& g, const or_params& P, const or_hits& in, int64
     if (!C.status)
         for (auto& T : C.tk)
             if (!run_global_one(C, T)) break;
+    if (!C.status && out.vertex)
+        for (auto& T : C.tk)
+            if (T.rec[0] >= 6) vertex_track(g, T.rec);
     /* outputs */
     int nt = 0;
     if (!C.status) {
@@ -1156,6 +1239,8 @@ static void run_event(const Geo& g, const or_params& P, const or_hits& in, int64
             }
             o.station_id = (int8_t)R[0]; o.nhits = (uint8_t)nhit; o.charge = (int8_t)R[15];
             o.slot = (uint8_t)T.slot; o.tracklet_id = (uint32_t)T.canon;
+            if (out.vertex)
+                for (int k = 0; k < 6; ++k) out.vertex[(e * P.max_tracks_out + nt) * 6 + k] = R[16 + k];
             ++nt;
         }
     }

@@ -77,7 +77,7 @@ class OracleGeometry:
             self.handle = None
 
 
-def reconstruct(geo: OracleGeometry, params, ev, records: bool = False, threads: int = 0):
+def reconstruct(geo: OracleGeometry, params, ev, records: bool = False, threads: int = 0, vertex: bool = False):
     """Runs the oracle chain; returns the same Result layout as the product."""
     import sys
     sys.path.insert(0, os.path.join(os.path.dirname(HERE), "e1039-gpu_amd"))
@@ -90,11 +90,12 @@ def reconstruct(geo: OracleGeometry, params, ev, records: bool = False, threads:
     tracks = np.zeros((ne, params.max_tracks_out), oc.TRACK_DTYPE)
     ntkl = np.zeros(ne, np.int32)
     rec = np.zeros((ne, params.max_tracklets_per_event, oc.TRACK_PARAMS), np.float32) if records else None
+    vtx = np.zeros((ne, params.max_tracks_out, 6), np.float32) if vertex else None
     arrs = [np.ascontiguousarray(a) for a in (ev.det, ev.elem, ev.tdc, ev.drift, ev.offsets)]
     flag = None if ev.flag is None else np.ascontiguousarray(ev.flag)
     hits = oc.OrHits(ne, nh, arrs[0].ctypes.data, arrs[1].ctypes.data, arrs[2].ctypes.data, arrs[3].ctypes.data,
                      None if flag is None else flag.ctypes.data, arrs[4].ctypes.data, 0)
     res = oc.OrResult(status.ctypes.data, keep.ctypes.data, ntr.ctypes.data, tracks.ctypes.data, ntkl.ctypes.data,
-                      None if rec is None else rec.ctypes.data, 0)
+                      None if rec is None else rec.ctypes.data, None if vtx is None else vtx.ctypes.data, 0)
     load().orc_reconstruct(geo.handle, C.byref(params), C.byref(hits), C.byref(res), int(threads))
-    return oc.Result(status, keep, ntr, tracks, ntkl, rec)
+    return oc.Result(status, keep, ntr, tracks, ntkl, rec, vtx)

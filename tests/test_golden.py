@@ -43,3 +43,23 @@ def test_hip_chain_matches_golden(oc, geo, golden):
     sess = oc.Session(geo, oc.default_params(), device=0)
     res = sess.reconstruct(ev, records=True)
     check(oc, res, ev, z)
+
+
+def check_vertex(res, z):
+    k = np.arange(res.tracks.shape[1])[None, :] < res.n_tracks[:, None]
+    assert (res.tracks[k]["station_id"] == 7).all()
+    same = np.array_equal(res.vertex[k].view(np.uint32), z["vertex"].view(np.uint32))
+    assert same
+
+
+def test_oracle_reproduces_golden_vertex(oc, orc, ogeo, golden):
+    ev, z = golden
+    check_vertex(orc.reconstruct(ogeo, oc.default_params(), ev, vertex=True), z)
+
+
+@pytest.mark.gpu
+def test_hip_vertex_matches_golden(oc, geo, golden):
+    ev, z = golden
+    sess = oc.Session(geo, oc.default_params(), device=0)
+    check_vertex(sess.reconstruct(ev, vertex=True), z)
+
