@@ -1903,6 +1903,100 @@ __device__ void vertex_track(Tk& T) {
     T.vtx[3] = (float)v3; T.vtx[4] = (float)v4; T.vtx[5] = (float)v5;
 }
 
+// G'': prop-tube segment matching (reconstruction_helper.cuh:1916-2257) and global
+// track cleaning (:2265-2354), the post-global steps of the full chain.
+//  * Segment test: a hit of P1 (either plane) within win1 of the track and a hit
+//    of P2 (either plane) within win2 whose line passes the slope, intercept and
+//    z = 2028.19 cuts; any passing combination will do (the reference's loop
+//    nesting only orders the tries).  Every tracklet is tested, one per lane;
+//    one without an x or a y segment gets stationID 5.
+//  * Cleaning: slot by slot, then slot against slot in slot order (the
+//    reference runs the cross-slot pass concurrently), one lane; chi2/ndf =
+//    chisq / (nhits - 5), merge threshold 0.015 on the signed relative 1/p
+//    difference, same-sign charges; the within-slot else branch compares 1/p(i)
+//    with itself, as the reference does.
+template <int MAXH>
+__device__ bool prop_segment(int p1a, int p1b, int p2a, int p2b, float slope, float icpt, float win1, float win2,
+                             float smax, float imax) {
+    Ev<MAXH> E;
+    const float cut = 0.03f;
+    const int first[2] = {p1a, p1b}, second[2] = {p2a, p2b};
+    for (int f = 0; f < 2; ++f) {
+        const int da = first[f];
+        const float za = G(F_Z, da);
+        for (int n = 0; n < E.nred(da); ++n) {
+            const float pos0 = E.Lp->pos[E.raw(da, n)];
+            if (!(fabsf(pos0 - (slope * za + icpt)) < win1)) continue;
+            for (int s2 = 0; s2 < 2; ++s2) {
+                const int db = second[s2];
+                const float zb = G(F_Z, db);
+                for (int m = 0; m < E.nred(db); ++m) {
+                    const float pos1 = E.Lp->pos[E.raw(db, m)];
+                    if (fabsf(pos1 - (slope * zb + icpt)) > win2) continue;
+                    const float a = (pos1 - pos0) / (zb - za);
+                    if (fabsf(a) > smax) continue;
+                    if (fabsf(a - slope) > cut) continue;
+                    const float b = pos0 - a * za;
+                    if (fabsf(b) > imax) continue;
+                    if (fabsf((slope * 2028.19f + icpt) - (a * 2028.19f + b)) > 3.0f) continue;
+                    return true;
+                }
+            }
+        }
+    }
+    return false;
+}
+
+template <int MAXH>
+STAGE void stage_postglobal(Tk* tks, const uint16_t* slot_order, int ntkl) {
+    Lds<MAXH>& L = g_lds<MAXH>;
+    const int lane = lane_id();
+    for (int k = lane; k < ntkl; k += WAVE) {
+        Tk& T = tks[k];
+        if (!prop_segment<MAXH>(50, 49, 52, 51, T.tx, T.x0, 5.08f, 7.62f, TX_MAX, X0_MAX) ||
+            !prop_segment<MAXH>(48, 47, 54, 53, T.ty, T.y0, 5.08f, 10.16f, TY_MAX, Y0_MAX))
+            T.stid = 5;
+    }
+    __syncthreads();
+    if (lane == 0) {
+        const float thr = 0.015f;
+        auto ndf = [](const Tk& T) { return T.chi2 / ((float)(T.nx + T.nuv + T.n1) - 5.f); };
+        for (int s = 0; s < 32; ++s)
+            for (int ii = L.sstart[s]; ii < L.sstart[s + 1]; ++ii) {
+                Tk& A = tks[slot_order[ii]];
+                if (A.stid < 6) continue;
+                if (ndf(A) > 250.f) { A.stid = 5; continue; }
+                for (int jj = ii + 1; jj < L.sstart[s + 1]; ++jj) {
+                    Tk& B = tks[slot_order[jj]];
+                    if (B.stid < 6) continue;
+                    if (ndf(A) < ndf(B)) {
+                        if ((A.invp - B.invp) / A.invp < thr && (float)A.charge * (float)B.charge > 0) B.stid = 5;
+                    } else {
+                        if ((A.invp - A.invp) / B.invp < thr && (float)A.charge * (float)B.charge > 0) A.stid = 5;
+                    }
+                }
+            }
+        for (int s = 0; s < 32; ++s)
+            for (int o = 0; o < 32; ++o) {
+                if (o == s) continue;
+                for (int ii = L.sstart[s]; ii < L.sstart[s + 1]; ++ii) {
+                    Tk& A = tks[slot_order[ii]];
+                    if (A.stid < 6) continue;
+                    for (int jj = L.sstart[o]; jj < L.sstart[o + 1]; ++jj) {
+                        Tk& B = tks[slot_order[jj]];
+                        if (B.stid < 6) continue;
+                        if (ndf(A) < ndf(B)) {
+                            if ((A.invp - B.invp) / A.invp < thr && (float)A.charge * (float)B.charge > 0) B.stid = 5;
+                        } else {
+                            if ((A.invp - B.invp) / B.invp < thr && (float)A.charge * (float)B.charge > 0) A.stid = 5;
+                        }
+                    }
+                }
+            }
+    }
+    __syncthreads();
+}
+
 template <int MAXH>
 STAGE void stage_vertex(Tk* tks, int ntkl) {
     for (int k = lane_id(); k < ntkl; k += WAVE)
@@ -1924,7 +2018,7 @@ STAGE void stage_out(const Args& a, Scr S, int64_t e, int ntkl) {
         for (int j0 = 0; j0 < ntkl; j0 += WAVE) {
             const int j = j0 + lane;
             const Tk* Tp = j < ntkl ? &tks[S.slot_order[j]] : nullptr;
-            const bool is6 = Tp && Tp->stid >= 6;
+            const bool is6 = Tp && Tp->stid >= (P.full_chain ? 7 : 6);
             const unsigned long long m = ballot(is6);
             if (is6) {
                 const int o = nout + rank_in(m);
@@ -2066,7 +2160,8 @@ __global__ __launch_bounds__(WAVE, MAXH <= 512 ? OR_WAVES_PER_SIMD_512 : (MAXH <
         if (L.status == 0) stage_clean<MAXH>(S.tks, S.slot_order, ntkl);
         OR_TIMER(4)
         if (L.status == 0) stage_global<MAXH>(a.p, S, ntkl);
-        if (L.status == 0 && a.vertex) stage_vertex<MAXH>(S.tks, ntkl);
+        if (L.status == 0 && a.p.full_chain) stage_postglobal<MAXH>(S.tks, S.slot_order, ntkl);
+        if (L.status == 0 && (a.vertex || a.p.full_chain)) stage_vertex<MAXH>(S.tks, ntkl);
         OR_TIMER(5)
         stage_out<MAXH>(a, S, e, ntkl);
         __syncthreads();

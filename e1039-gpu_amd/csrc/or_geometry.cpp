@@ -135,6 +135,7 @@ void or_params_default(or_params* p) {
     p->max_d3_mult = 90;
     p->max_tracks_out = 16;
     p->max_hits_per_event = 1024;
+    p->full_chain = 0;
 }
 
 const char* or_status_string(or_status st) {

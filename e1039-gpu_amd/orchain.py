@@ -52,7 +52,7 @@ PARAM_FIELDS = [
     "max_hits_chamber_plane", "max_hits_hodo_plane", "max_hits_prop_plane", "max_reduced_per_plane",
     "max_pair_plane_hits", "max_pairs_per_bin", "max_pairs_window", "max_tracklets_per_slot",
     "max_tracklets_per_event", "max_d0_mult", "max_d2_mult", "max_d3_mult", "max_tracks_out",
-    "max_hits_per_event",
+    "max_hits_per_event", "full_chain",
 ]
 
 

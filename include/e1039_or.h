@@ -81,6 +81,11 @@ typedef struct {
     int32_t max_d3_mult;              /* 90   (reconstruction_constants.h:190) */
     int32_t max_tracks_out;           /* output records per event (no reference counterpart) */
     int32_t max_hits_per_event;       /* per-event bound the device tiles are sized for, <= 8192; larger events get OR_EV_EVENT_TOO_LARGE */
+    int32_t full_chain;               /* 0 (default): stop after global tracking, output stationID >= 6 tracks.
+                                         1: then prop-tube segment matching (reconstruction_helper.cuh:1916-2257),
+                                         global track cleaning (:2265-2354, slot by slot, then slot against slot in
+                                         slot order) and vertexing; output stationID >= 7 tracks, as the reference's
+                                         unpack (online_reconstruction.cu:1133-1136) */
 } or_params;
 
 /* Input hits, structure of arrays, events contiguous.  Field meaning follows

@@ -1137,6 +1137,111 @@ static void vertex_track(const Geo& g, float* R) {
     for (int k = 0; k < 6; ++k) R[16 + k] = (float)v[k];
 }
 
+/* ---- prop-tube segment matching: gKernel_PropSegmentMatching,
+ * reconstruction_helper.cuh:1916-2257.  Every tracklet of the event (any
+ * stationID) needs an x segment and a y segment in the prop tubes that agree
+ * with it; a tracklet without one gets stationID 5.  A segment is a hit of the
+ * first station (P1, either plane) within win1 of the track and a hit of the
+ * second (P2, either plane) within win2, whose line has |slope| <= smax,
+ * |slope - track slope| <= 0.03, |intercept| <= imax and lies within 3 cm of the
+ * track at z = 2028.19.  The reference's loop nesting only decides the order
+ * in which the combinations are tried; the outcome is whether any passes. */
+static bool prop_segment(Ctx& C, int p1a, int p1b, int p2a, int p2b, float slope, float icpt, float win1, float win2,
+                         float smax, float imax) {
+    const Geo& g = *C.g;
+    const float cut = 0.03f;
+    const int first[2] = {p1a, p1b}, second[2] = {p2a, p2b};
+    for (int f = 0; f < 2; ++f) {
+        const Plane& A = C.pl[first[f]];
+        const float za = g.z[first[f]];
+        for (int n = 0; n < A.n; ++n) {
+            const float pos0 = A.pos[n];
+            if (!(std::fabs(pos0 - (slope * za + icpt)) < win1)) continue;
+            for (int s2 = 0; s2 < 2; ++s2) {
+                const Plane& B = C.pl[second[s2]];
+                const float zb = g.z[second[s2]];
+                for (int m = 0; m < B.n; ++m) {
+                    const float pos1 = B.pos[m];
+                    if (std::fabs(pos1 - (slope * zb + icpt)) > win2) continue;
+                    const float a = (pos1 - pos0) / (zb - za);
+                    if (std::fabs(a) > smax) continue;
+                    if (std::fabs(a - slope) > cut) continue;
+                    const float b = pos0 - a * za;
+                    if (std::fabs(b) > imax) continue;
+                    if (std::fabs((slope * 2028.19f + icpt) - (a * 2028.19f + b)) > 3.0f) continue;
+                    return true;
+                }
+            }
+        }
+    }
+    return false;
+}
+
+static void run_prop_matching(Ctx& C) {
+    for (auto& T : C.tk) {
+        float* R = T.rec;
+        const float x0 = R[7], tx = R[5], y0 = R[8], ty = R[6];
+        if (!prop_segment(C, 50, 49, 52, 51, tx, x0, 5.08f, 7.62f, TX_MAX, X0_MAX) ||
+            !prop_segment(C, 48, 47, 54, 53, ty, y0, 5.08f, 10.16f, TY_MAX, Y0_MAX))
+            R[0] = 5;
+    }
+}
+
+/* ---- global track cleaning: gKernel_GlobalTrackCleaning (reconstruction_helper.cuh:2265-2301)
+ * per slot, then _crossthreads (:2303-2354) slot against slot in slot order.
+ * chi2/ndf = chisq / (nhits - 5); merge threshold 0.015 on the signed relative
+ * 1/p difference (no absolute value), same-sign charges only.  In the within-slot
+ * pass the else branch compares invP(i) with itself (difference 0, always below
+ * the threshold), as the reference does; chi2/ndf > 250 demotes i there too. */
+static void run_global_cleaning(Ctx& C) {
+    auto ndf = [](const float* R) { return R[3] / (R[2] - 5.f); };
+    const float thr = 0.015f;
+    /* slot ranges in the slot-sorted list */
+    int sstart[33];
+    std::vector<int> order(C.tk.size());
+    {
+        int k = 0;
+        for (int s = 0; s < 32; ++s) {
+            sstart[s] = k;
+            while (k < (int)C.tk.size() && C.tk[k].slot == s) ++k;
+        }
+        sstart[32] = (int)C.tk.size();
+        for (size_t i = 0; i < C.tk.size(); ++i) order[i] = (int)i;
+    }
+    for (int s = 0; s < 32; ++s)
+        for (int ii = sstart[s]; ii < sstart[s + 1]; ++ii) {
+            float* A = C.tk[order[ii]].rec;
+            if (A[0] < 6) continue;
+            if (ndf(A) > 250.f) { A[0] = 5; continue; }
+            for (int jj = ii + 1; jj < sstart[s + 1]; ++jj) {
+                float* B = C.tk[order[jj]].rec;
+                if (B[0] < 6) continue;
+                if (ndf(A) < ndf(B)) {
+                    if ((A[9] - B[9]) / A[9] < thr && A[15] * B[15] > 0) B[0] = 5;
+                } else {
+                    if ((A[9] - A[9]) / B[9] < thr && A[15] * B[15] > 0) A[0] = 5;
+                }
+            }
+        }
+    for (int s = 0; s < 32; ++s)
+        for (int o = 0; o < 32; ++o) {
+            if (o == s) continue;
+            for (int ii = sstart[s]; ii < sstart[s + 1]; ++ii) {
+                float* A = C.tk[order[ii]].rec;
+                if (A[0] < 6) continue;
+                for (int jj = sstart[o]; jj < sstart[o + 1]; ++jj) {
+                    float* B = C.tk[order[jj]].rec;
+                    if (B[0] < 6) continue;
+                    if (ndf(A) < ndf(B)) {
+                        if ((A[9] - B[9]) / A[9] < thr && A[15] * B[15] > 0) B[0] = 5;
+                    } else {
+                        if ((A[9] - B[9]) / B[9] < thr && A[15] * B[15] > 0) A[0] = 5;
+                    }
+                }
+            }
+        }
+}
+
 static void run_event(const Geo& g, const or_params& P, const or_hits& in, int64_t e, const or_result& out,
                       std::vector<uint8_t>& keep_bytes, Ctx& C) {
     const int64_t h0 = in.event_offsets[e], h1 = in.event_offsets[e + 1];
@@ -1216,14 +1321,19 @@ static void run_event(const Geo& g, const or_params& P, const or_hits& in, int64
     if (!C.status)
         for (auto& T : C.tk)
             if (!run_global_one(C, T)) break;
-    if (!C.status && out.vertex)
+    if (!C.status && P.full_chain) {
+        run_prop_matching(C);
+        run_global_cleaning(C);
+    }
+    if (!C.status && (out.vertex || P.full_chain))
         for (auto& T : C.tk)
             if (T.rec[0] >= 6) vertex_track(g, T.rec);
     /* outputs */
     int nt = 0;
     if (!C.status) {
+        const float min_stid = P.full_chain ? 7.f : 6.f;
         for (auto& T : C.tk) {
-            if (T.rec[0] < 6) continue;
+            if (T.rec[0] < min_stid) continue;
             if (nt >= P.max_tracks_out) { C.status |= OR_EV_OUTPUT_OVERFLOW; continue; }
             or_track& o = out.tracks[e * P.max_tracks_out + nt];
             std::memset(&o, 0, sizeof o);
@@ -1261,6 +1371,36 @@ static void run_event(const Geo& g, const or_params& P, const or_hits& in, int64
 
 /* ============================ C ABI of the oracle ============================ */
 extern "C" {
+
+/* prop-tube segment test on explicit planes (first station: planes a, b; second:
+ * c, d; positions and plane z), for unit tests of run_prop_matching */
+int orc_prop_segment(const float* pa, int na, float za, const float* pb, int nb, float zb, const float* pc, int nc,
+                     float zc, const float* pd, int nd, float zd, float slope, float icpt, float win1, float win2,
+                     float smax, float imax) {
+    orc::Geo g;
+    std::memset(&g, 0, sizeof g);
+    g.z[50] = za; g.z[49] = zb; g.z[52] = zc; g.z[51] = zd;
+    orc::Ctx C;
+    C.g = &g;
+    auto fill = [&](int d, const float* p, int n) { C.pl[d].n = n; for (int i = 0; i < n; ++i) C.pl[d].pos[i] = p[i]; };
+    fill(50, pa, na); fill(49, pb, nb); fill(52, pc, nc); fill(51, pd, nd);
+    return orc::prop_segment(C, 50, 49, 52, 51, slope, icpt, win1, win2, smax, imax) ? 1 : 0;
+}
+
+/* global track cleaning on explicit tracks (slot-sorted): per track stid, chi2,
+ * nhits, invP, charge and slot; stid is updated in place */
+void orc_global_cleaning(int n, float* stid, const float* chi2, const float* nhits, const float* invp,
+                         const float* charge, const int* slot) {
+    orc::Ctx C;
+    C.tk.resize(n);
+    for (int i = 0; i < n; ++i) {
+        std::memset(C.tk[i].rec, 0, sizeof C.tk[i].rec);
+        C.tk[i].rec[0] = stid[i]; C.tk[i].rec[3] = chi2[i]; C.tk[i].rec[2] = nhits[i];
+        C.tk[i].rec[9] = invp[i]; C.tk[i].rec[15] = charge[i]; C.tk[i].slot = slot[i];
+    }
+    orc::run_global_cleaning(C);
+    for (int i = 0; i < n; ++i) stid[i] = C.tk[i].rec[0];
+}
 
 void orc_yz_state_check(int enable, long long* counts /* 3, may be NULL */) {
     if (counts)

@@ -51,6 +51,11 @@ def load():
     lib.orc_lr_xhits.argtypes = [vp, f32, f32, f32, f32, C.c_int, vp, vp, vp, vp]
     lib.orc_pairs_binned.argtypes = [C.c_int, C.c_int, vp, vp, C.c_int, vp, vp, C.c_int, vp, vp]
     lib.orc_yz_state_check.argtypes = [C.c_int, vp]
+    f32 = C.c_float
+    lib.orc_prop_segment.argtypes = [vp, C.c_int, f32, vp, C.c_int, f32, vp, C.c_int, f32, vp, C.c_int, f32,
+                                     f32, f32, f32, f32, f32, f32]
+    lib.orc_prop_segment.restype = C.c_int
+    lib.orc_global_cleaning.argtypes = [C.c_int, vp, vp, vp, vp, vp, vp]
     _lib = lib
     return lib
 

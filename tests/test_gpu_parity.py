@@ -215,3 +215,18 @@ def test_vertexing(oc, orc, geo, ogeo, session):
     same = np.array_equal(host.vertex[k5].view(np.uint32), ref.vertex[:500][k5].view(np.uint32))
     assert same, "vertex differs on the host-buffer path"
 
+
+def test_full_chain(oc, orc, geo, ogeo):
+    # prop-tube segment matching, global cleaning and vertexing after global
+    # tracking; outputs are the stationID 7 tracks
+    sess = oc.Session(geo, oc.default_params(full_chain=1), device=0)
+    for kw, n in ((dict(mult_mode=1, mult_mean=300.0, mult_cap=1000), 2000), ({}, 2000)):
+        ev = oc.generate(geo, n, seed=22, **kw)
+        ref = orc.reconstruct(ogeo, sess.params, ev, records=True, vertex=True)
+        got = sess.reconstruct(ev, records=True, vertex=True)
+        assert ref.n_tracks.sum() > 300
+        assert_same(got, ref, ev, "full chain")
+        k = np.arange(ref.tracks.shape[1])[None, :] < ref.n_tracks[:, None]
+        same = np.array_equal(got.vertex[k].view(np.uint32), ref.vertex[k].view(np.uint32))
+        assert same, "vertex differs"
+
