@@ -274,10 +274,36 @@ __device__ __attribute__((noinline)) int pairs_window(const Ev<MAXH>& E, int st,
     for (int i = 0; i < n2; ++i)
         if (inwin(d2, dx2, i)) { ++c2; if (i < 64) w2 |= 1ull << i; }
     if (c1 > cap_plane || c2 > cap_plane) return -1;
-    auto in1 = [&](int i) { return i < 64 ? ((w1 >> i) & 1ull) != 0 : inwin(d1, dx1, i); };
-    auto in2 = [&](int i) { return i < 64 ? ((w2 >> i) & 1ull) != 0 : inwin(d2, dx2, i); };
     const float sp = spacing_plane(sid);
     int np = 0;
+    if (n1 <= 64 && n2 <= 64) {
+        // only the in-window hits, in index order; the paired hits are marked
+        // while the pairs are listed
+        unsigned long long p1 = 0, p2 = 0;
+        for (unsigned long long m1 = w1; m1; m1 &= m1 - 1) {
+            const int i = __ffsll((long long)m1) - 1;
+            const float pi = E.Lp->pos[E.raw(d1, i)];
+            for (unsigned long long m2 = w2; m2; m2 &= m2 - 1) {
+                const int j = __ffsll((long long)m2) - 1;
+                if (fabsf(pi - E.Lp->pos[E.raw(d2, j)]) > sp) continue;
+                p1 |= 1ull << i;
+                p2 |= 1ull << j;
+                if (np >= cap_pairs) return -1;
+                out[np++] = (uint16_t)(i | (j << 8));
+            }
+        }
+        for (unsigned long long m = w1 & ~p1; m; m &= m - 1) {
+            if (np >= cap_pairs) return -1;
+            out[np++] = (uint16_t)((__ffsll((long long)m) - 1) | (0xFF << 8));
+        }
+        for (unsigned long long m = w2 & ~p2; m; m &= m - 1) {
+            if (np >= cap_pairs) return -1;
+            out[np++] = (uint16_t)(0xFF | ((__ffsll((long long)m) - 1) << 8));
+        }
+        return np;
+    }
+    auto in1 = [&](int i) { return i < 64 ? ((w1 >> i) & 1ull) != 0 : inwin(d1, dx1, i); };
+    auto in2 = [&](int i) { return i < 64 ? ((w2 >> i) & 1ull) != 0 : inwin(d2, dx2, i); };
     for (int i = 0; i < n1; ++i) {
         if (!in1(i)) continue;
         const float pi = E.Lp->pos[E.raw(d1, i)];
@@ -1370,8 +1396,13 @@ STAGE void stage_yz(or_params P, Scr S, int ntkl, int list_cap) {
                 uint16_t* lst = wl + (j * 4 + v) * WIN_CAP;
                 const int nl = pairs_window(E, st, pj, xmin, xmax, lst, P.max_pair_plane_hits, capw);
                 if (nl < 0) atomicOr(&L.status, (uint32_t)OR_EV_PAIR_OVERFLOW);
-                L.nl[j][v] = nl < 0 ? 0 : nl;
-                // y of each listed hit from the tracklet's x line (calculate_y_uvhit, sign 0)
+                // y of each listed hit from the tracklet's x line (calculate_y_uvhit,
+                // sign 0).  An entry with no accepted hit ends every candidate that
+                // contains it before its fit (reconstruction_kernels.cuh:1390, 1457,
+                // 1533, 1600), so the list keeps only entries with an accepted hit,
+                // in order: the candidates enumerated over the kept lists are the
+                // reference's fitted candidates as an order-preserving subsequence.
+                int nacc = 0;
                 for (int q = 0; q < nl; ++q) {
                     float4 yv = make_float4(0.f, 0.f, 0.f, 0.f);
                     uint8_t fl = 0;
@@ -1384,9 +1415,14 @@ STAGE void stage_yz(or_params P, Scr S, int ntkl, int list_cap) {
                         const int r = E.raw(db, ib);
                         if (y_uvhit(db, L.elem[r], L.drift[r], 0, T.x0, T.tx, yv.z, yv.w)) fl |= 2;
                     }
-                    wy[(j * 4 + v) * WIN_CAP + q] = yv;
-                    wf[(j * 4 + v) * WIN_CAP + q] = fl;
+                    if (fl) {
+                        lst[nacc] = lst[q];
+                        wy[(j * 4 + v) * WIN_CAP + nacc] = yv;
+                        wf[(j * 4 + v) * WIN_CAP + nacc] = fl;
+                        ++nacc;
+                    }
                 }
+                L.nl[j][v] = nacc;
             }
         }
         __syncthreads();
@@ -1544,7 +1580,8 @@ STAGE void stage_clean(Tk* tks, const uint16_t* slot_order, int ntkl) {
 
 // G: station-1 extension (reconstruction_kernels.cuh:1841-2534), one track at a
 // time, the wave's lanes spread over the track's (u, v) pairs.
-//  * The three station-1 window lists (X, U, V) are built by three lanes.
+//  * The three station-1 window lists (X, U, V) of a batch of tracks are built
+//    together, one lane per (track, view).
 //  * Per x pair the x part (x at the current y0/ty, KMAG-point fit, left/right,
 //    refit, 1/p window) is computed by every lane alike.
 //  * Hits sit in fixed slots X (4), X' (3), U (2), U' (1), V (6), V' (5), the
@@ -1576,250 +1613,277 @@ __device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long v)
     return v;
 }
 
+// window list of view v (0 X, 1 U, 2 V) of station 1 from the sagitta ratio
+// (reconstruction_helper.cuh:556-613); -1 on overflow
+template <int MAXH>
+__device__ __forceinline__ int global_window(const Ev<MAXH>& E, const Tk* T, int v, uint16_t* out, int plane_cap,
+                                             int capw) {
+    const int n23 = T->nx + T->nuv;
+    const float x0 = T->x0, tx = T->tx, y0 = T->y0, ty = T->ty;
+    int dlast = 1;
+    float zmax = -10000.f;
+    for (int m = 0; m < n23; ++m) {
+        const int d = T->det[m];
+        if (d > 0 && G(F_Z, d) > zmax) { dlast = d; zmax = G(F_Z, d); }
+    }
+    const int i = v == 0 ? 1 : (v == 1 ? 0 : 2);   // planetype[d/2-1] of d = 2i+2 is v
+    const float z3 = G(F_Z, dlast), x3 = x0 + tx * z3, y3 = y0 + ty * z3;
+    const int d = 2 * i + 2;
+    const int d2 = super_id(2, v) * 2;
+    const float ct = G(F_COSTHETA, d2), st = G(F_SINTHETA, d2);
+    const float ps3 = x3 * ct + y3 * st;
+    const float z1 = G(F_Z, d), z2 = G(F_Z, d2);
+    const float x2 = x0 + tx * z2, y2 = y0 + ty * z2;
+    const float ps2 = x2 * ct + y2 * st;
+    const float s2t = ps2 - (ps3 * (z2 - Z_TARGET)) / (z3 - Z_TARGET);
+    const float s2d = ps2 - (ps3 * (z2 - Z_DUMP)) / (z3 - Z_DUMP);
+    const float pet = SAG_T_C * s2t + (ps3 * (z1 - Z_TARGET)) / (z3 - Z_TARGET);
+    const float ped = SAG_D_C * s2d + (ps3 * (z1 - Z_DUMP)) / (z3 - Z_DUMP);
+    const float wt = fabsf(s2t * SAG_T_W), wd = fabsf(s2d * SAG_D_W);
+    const float pmn = fminf(pet - wt, ped - wd), pmx = fmaxf(pet + wt, ped + wd);
+    const float pexp = (float)(0.5 * (double)(pmx + pmn));
+    const float win = (float)(0.5 * (double)(pmx - pmn));
+    return pairs_window(E, 0, v, pexp - win * 1.0f, pexp + win * 1.0f, out, plane_cap, capw);
+}
+
+// The window lists depend only on the tracklet, and one overflow rejects the
+// event, so the lists of up to GLOBAL_BATCH tracklets (three each) are built
+// together, lane (tracklet, view); the tracklets are then extended one after
+// the other, each independent of the others.
+constexpr int GLOBAL_BATCH = WAVE / 3;
+
 template <int MAXH>
 STAGE void stage_global(or_params P, Scr S, int ntkl) {
     Lds<MAXH>& L = g_lds<MAXH>;
     Ev<MAXH> E;
     const int lane = lane_id();
     Tk* tks = S.tks;
-    uint16_t* lx = S.wl;
-    uint16_t* lu = S.wl + WIN_CAP;
-    uint16_t* lv = S.wl + 2 * WIN_CAP;
+    int* nlf = &L.nl[0][0];
     GBest* best = reinterpret_cast<GBest*>(S.wy);
     const int capw = P.max_pairs_window < WIN_CAP ? P.max_pairs_window : WIN_CAP;
     const int det6[6] = {4, 3, 2, 1, 6, 5};
-    for (int k = 0; k < ntkl; ++k) {
-        Tk* T = &tks[k];
-        if (T->stid < 5) continue;
-        const int n23 = T->nx + T->nuv;
-        const float x0 = T->x0, tx = T->tx, ex0 = T->ex0, etx = T->etx;
-        float y0 = T->y0, ty = T->ty;
-        OR_SUB_START
-        // window lists from the sagitta ratio (reconstruction_helper.cuh:556-613); lane v: X, U, V
-        if (lane < 3) {
-            int dlast = 1;
-            float zmax = -10000.f;
-            for (int m = 0; m < n23; ++m) {
-                const int d = T->det[m];
-                if (d > 0 && G(F_Z, d) > zmax) { dlast = d; zmax = G(F_Z, d); }
-            }
-            const int v = lane;
-            const int i = v == 0 ? 1 : (v == 1 ? 0 : 2);   // planetype[d/2-1] of d = 2i+2 is v
-            const float z3 = G(F_Z, dlast), x3 = x0 + tx * z3, y3 = y0 + ty * z3;
-            const int d = 2 * i + 2;
-            const int d2 = super_id(2, v) * 2;
-            const float ct = G(F_COSTHETA, d2), st = G(F_SINTHETA, d2);
-            const float ps3 = x3 * ct + y3 * st;
-            const float z1 = G(F_Z, d), z2 = G(F_Z, d2);
-            const float x2 = x0 + tx * z2, y2 = y0 + ty * z2;
-            const float ps2 = x2 * ct + y2 * st;
-            const float s2t = ps2 - (ps3 * (z2 - Z_TARGET)) / (z3 - Z_TARGET);
-            const float s2d = ps2 - (ps3 * (z2 - Z_DUMP)) / (z3 - Z_DUMP);
-            const float pet = SAG_T_C * s2t + (ps3 * (z1 - Z_TARGET)) / (z3 - Z_TARGET);
-            const float ped = SAG_D_C * s2d + (ps3 * (z1 - Z_DUMP)) / (z3 - Z_DUMP);
-            const float wt = fabsf(s2t * SAG_T_W), wd = fabsf(s2d * SAG_D_W);
-            const float pmn = fminf(pet - wt, ped - wd), pmx = fmaxf(pet + wt, ped + wd);
-            const float pexp = (float)(0.5 * (double)(pmx + pmn));
-            const float win = (float)(0.5 * (double)(pmx - pmn));
-            uint16_t* out = v == 0 ? lx : (v == 1 ? lu : lv);
-            const int nl = pairs_window(E, 0, v, pexp - win * 1.0f, pexp + win * 1.0f, out, P.max_pair_plane_hits, capw);
-            if (nl < 0) atomicOr(&L.status, (uint32_t)OR_EV_PAIR_OVERFLOW);
-            L.nl[0][v] = nl;
-        }
-        __syncthreads();
-        OR_SUB(6);
-        if (L.status) break;
-        const int nx1 = L.nl[0][0], nu1 = L.nl[0][1], nv1 = L.nl[0][2];
-        if (nx1 == 0 || nu1 == 0 || nv1 == 0) continue;
-        // y sums of the station-2/3 UV hits with their stored signs (reconstruction_kernels.cuh:2086-2102)
-        float A0s = 0.f, A1s = 0.f, A3s = 0.f, B0s = 0.f, B1s = 0.f;
-        for (int m = 0; m < n23; ++m) {
-            const int d = T->det[m];
-            if ((double)G(F_COSTHETA, d) > 0.99) continue;
-            float y, ey;
-            if (y_uvhit(d, E.Lp->elem[T->hraw[m]], E.Lp->drift[T->hraw[m]], T->sign[m], x0, tx, y, ey)) {
-                const float z = G(F_Z, d);
-                B0s += ey * y; B1s += (ey * y) * z; A0s += ey; A1s += ey * z; A3s += (ey * z) * z;
-            }
-        }
-        unsigned long long best_key = ~0ull;
-        const int ncomb = nu1 * nv1;
-        for (int ix = 0; ix < nx1; ++ix) {
-            // ---- x part, every lane alike: slots X (4), X' (3) and the KMAG point
-            const uint16_t qx = lx[ix];
-            const int xi[2] = {pa(qx), pb(qx)};
-            unsigned xpres = 0;
-            int xraw[2] = {0, 0}, xsign[2] = {0, 0};
-            float X[2] = {0.f, 0.f}, Zx[2] = {0.f, 0.f}, xdr[2] = {0.f, 0.f}, xpos[2] = {0.f, 0.f};
-#pragma unroll
-            for (int w = 0; w < 2; ++w) {
-                if (xi[w] < 0) continue;
-                const int d = det6[w];
-                const int r = E.raw(d, xi[w]);
-                xpres |= 1u << w;
-                xraw[w] = r; xdr[w] = E.Lp->drift[r]; xpos[w] = E.Lp->pos[r];
-                xz_fy(d, E.Lp->elem[r], xdr[w], 0, y0, ty, X[w], Zx[w]);
-            }
-            if (xpres == 0) continue;
-            const float xk = x0 + tx * Z_KMAG_BEND, exk = ex0 + etx * Z_KMAG_BEND;
-            auto xfit = [&](const float (&xv)[2]) {
-                float A0 = 0.f, A1 = 0.f, A3 = 0.f, B0 = 0.f, B1 = 0.f;
-#pragma unroll
-                for (int w = 0; w < 2; ++w) {
-                    if (!((xpres >> w) & 1u)) continue;
-                    const float e = G(F_RESOLUTION, det6[w]);
-                    B0 += e * xv[w]; B1 += (e * xv[w]) * Zx[w]; A0 += e; A1 += e * Zx[w]; A3 += (e * Zx[w]) * Zx[w];
+    for (int k0 = 0; k0 < ntkl; k0 += WAVE) {
+        unsigned long long rest = ballot(k0 + lane < ntkl && tks[k0 + lane].stid >= 5);
+        while (rest) {
+            OR_SUB_START
+            {
+                const int b = lane / 3, v = lane - 3 * b;
+                unsigned long long mm = rest;
+                for (int t = 0; t < b && mm; ++t) mm &= mm - 1;
+                if (b < GLOBAL_BATCH && mm) {
+                    const int k = k0 + __ffsll((long long)mm) - 1;
+                    const int nl = global_window(E, &tks[k], v, S.wl + lane * WIN_CAP, P.max_pair_plane_hits, capw);
+                    if (nl < 0) atomicOr(&L.status, (uint32_t)OR_EV_PAIR_OVERFLOW);
+                    nlf[lane] = nl;
                 }
-                B0 += exk * xk; B1 += (exk * xk) * Z_KMAG_BEND; A0 += exk; A1 += exk * Z_KMAG_BEND;
-                A3 += (exk * Z_KMAG_BEND) * Z_KMAG_BEND;
-                return fit_finish(A0, A1, A3, B0, B1);
-            };
-            Fit f = xfit(X);
-            if (xpres == 3u) {   // X / X' pair, x-only decision with the fit's parameters as errors
-                const int h = lr_xpair_decide(f.p0, f.p1, f.p0, f.p1, 4, 3, X[0], X[1], xdr[0], xdr[1], 150.f);
-                if (h >= 0) { xsign[0] = lr_si(h); xsign[1] = lr_sj(h); }
             }
-#pragma unroll
-            for (int w = 0; w < 2; ++w)
-                if (((xpres >> w) & 1u) && xsign[w] == 0) xsign[w] = G(F_Z, det6[w]) * f.p1 + f.p0 > X[w] ? 1 : -1;
-            float Xc[2];
-#pragma unroll
-            for (int w = 0; w < 2; ++w) Xc[w] = X[w] + (float)xsign[w] * xdr[w];
-            f = xfit(Xc);
-            const float x01 = f.p0, tx1 = f.p1;  // parameters used as errors (reconstruction_kernels.cuh:2194-2195)
-            const int q = (double)(-0.0033f) * copysign(1.0, (double)FMAGSTR) * (double)x0 < (double)tx ? 1 : -1;
-            const float invp = ((tx1 - tx) * (float)q) / PT_KICK_KMAG;
-            const float einvp = (etx - etx) / PT_KICK_KMAG;
-            if (invp > INVP_MAX + einvp || invp < INVP_MIN - einvp) continue;
-            const bool cross = xpres == 2u;   // lone X'
-            int xs_state = xsign[1];
-            // ---- (u, v) pairs, one per lane
-            for (int base = 0; base < ncomb; base += WAVE) {
-                const int iuv = base + lane;
-                bool reached = false, cand = false, wrote = false;
-                float fy0 = 0.f, fty = 0.f, fey0 = 0.f, fety = 0.f, chi23 = 0.f;
-                unsigned pres = xpres;
-                int sign[6], raw[6];
-                float pos[6], drift[6];
-                sign[0] = xsign[0]; raw[0] = xraw[0]; pos[0] = xpos[0]; drift[0] = xdr[0];
-                sign[1] = cross ? 2 : xsign[1]; raw[1] = xraw[1]; pos[1] = xpos[1]; drift[1] = xdr[1];
-#pragma unroll
-                for (int s = 2; s < 6; ++s) { sign[s] = 0; raw[s] = 0; pos[s] = 0.f; drift[s] = 0.f; }
-                if (iuv < ncomb) {
-                    float A0 = A0s, A1 = A1s, A3 = A3s, B0 = B0s, B1 = B1s;
-                    const int iu = iuv % nu1, iv = iuv / nu1;
-                    const int ui[4] = {pa(lu[iu]), pb(lu[iu]), pa(lv[iv]), pb(lv[iv])};
-                    int nuh = 0, nvh = 0;
-#pragma unroll
-                    for (int s = 2; s < 6; ++s) {
-                        const int idx = ui[s - 2];
-                        if (idx < 0 || (s >= 4 && nuh == 0)) continue;
-                        const int d = det6[s];
-                        const int r = E.raw(d, idx);
-                        float y, ey;
-                        if (y_uvhit(d, E.Lp->elem[r], E.Lp->drift[r], 0, x01, tx1, y, ey)) {
-                            pres |= 1u << s;
-                            raw[s] = r; pos[s] = E.Lp->pos[r]; drift[s] = E.Lp->drift[r];
-                            const float z = G(F_Z, d);
-                            B0 += ey * y; B1 += (ey * y) * z; A0 += ey; A1 += ey * z; A3 += (ey * z) * z;
-                            if (s < 4) ++nuh;
-                            else ++nvh;
-                        }
+            __syncthreads();
+            OR_SUB(6);
+            if (L.status) return;
+            for (int b = 0; b < GLOBAL_BATCH && rest; ++b) {
+                const int k = k0 + __ffsll((long long)rest) - 1;
+                rest &= rest - 1;
+                Tk* T = &tks[k];
+                const int n23 = T->nx + T->nuv;
+                const float x0 = T->x0, tx = T->tx, ex0 = T->ex0, etx = T->etx;
+                float y0 = T->y0, ty = T->ty;
+                const uint16_t* lx = S.wl + (3 * b) * WIN_CAP;
+                const uint16_t* lu = S.wl + (3 * b + 1) * WIN_CAP;
+                const uint16_t* lv = S.wl + (3 * b + 2) * WIN_CAP;
+                OR_SUB_START
+                const int nx1 = nlf[3 * b], nu1 = nlf[3 * b + 1], nv1 = nlf[3 * b + 2];
+                if (nx1 == 0 || nu1 == 0 || nv1 == 0) continue;
+                // y sums of the station-2/3 UV hits with their stored signs (reconstruction_kernels.cuh:2086-2102)
+                float A0s = 0.f, A1s = 0.f, A3s = 0.f, B0s = 0.f, B1s = 0.f;
+                for (int m = 0; m < n23; ++m) {
+                    const int d = T->det[m];
+                    if ((double)G(F_COSTHETA, d) > 0.99) continue;
+                    float y, ey;
+                    if (y_uvhit(d, E.Lp->elem[T->hraw[m]], E.Lp->drift[T->hraw[m]], T->sign[m], x0, tx, y, ey)) {
+                        const float z = G(F_Z, d);
+                        B0s += ey * y; B1s += (ey * y) * z; A0s += ey; A1s += ey * z; A3s += (ey * z) * z;
                     }
-                    if (nuh > 0 && nvh > 0) {
-                        reached = true;
-                        const Fit g = fit_finish(A0, A1, A3, B0, B1);
-                        fy0 = g.p0; fty = g.p1; fey0 = g.e0; fety = g.e1;
-                        if (!(fabsf(fy0) > Y0_MAX + 2.f * fey0 || fabsf(fty) > TY_MAX + 2.f * fety)) {
-                            int dets[6] = {4, 3, 2, 1, 6, 5};
-                            lr_slots_new<6>(x01, tx1, fy0, fty, x01, tx1, fey0, fety, pres, dets, pos, drift, sign,
-                                            150.f);
-                            if (cross) wrote = sign[1] != 2;
-                            if (!(fabsf(fy0) > Y0_MAX || fabsf(fty) > TY_MAX)) {
-                                // station-2/3 residuals with (x0, tx) of the tracklet and the new y line
-                                for (int m = 0; m < n23; ++m) {
-                                    const int d = T->det[m];
-                                    const float res = G(F_RESOLUTION, d);
-                                    const float dca = dca_wire(d, E.Lp->elem[T->hraw[m]], x0, fy0, tx, fty);
-                                    const float r23 = E.Lp->drift[T->hraw[m]] * (float)T->sign[m] - dca;
-                                    chi23 += (r23 * r23) / (res * res);
+                }
+                unsigned long long best_key = ~0ull;
+                const int ncomb = nu1 * nv1;
+                for (int ix = 0; ix < nx1; ++ix) {
+                    // ---- x part, every lane alike: slots X (4), X' (3) and the KMAG point
+                    const uint16_t qx = lx[ix];
+                    const int xi[2] = {pa(qx), pb(qx)};
+                    unsigned xpres = 0;
+                    int xraw[2] = {0, 0}, xsign[2] = {0, 0};
+                    float X[2] = {0.f, 0.f}, Zx[2] = {0.f, 0.f}, xdr[2] = {0.f, 0.f}, xpos[2] = {0.f, 0.f};
+        #pragma unroll
+                    for (int w = 0; w < 2; ++w) {
+                        if (xi[w] < 0) continue;
+                        const int d = det6[w];
+                        const int r = E.raw(d, xi[w]);
+                        xpres |= 1u << w;
+                        xraw[w] = r; xdr[w] = E.Lp->drift[r]; xpos[w] = E.Lp->pos[r];
+                        xz_fy(d, E.Lp->elem[r], xdr[w], 0, y0, ty, X[w], Zx[w]);
+                    }
+                    if (xpres == 0) continue;
+                    const float xk = x0 + tx * Z_KMAG_BEND, exk = ex0 + etx * Z_KMAG_BEND;
+                    auto xfit = [&](const float (&xv)[2]) {
+                        float A0 = 0.f, A1 = 0.f, A3 = 0.f, B0 = 0.f, B1 = 0.f;
+        #pragma unroll
+                        for (int w = 0; w < 2; ++w) {
+                            if (!((xpres >> w) & 1u)) continue;
+                            const float e = G(F_RESOLUTION, det6[w]);
+                            B0 += e * xv[w]; B1 += (e * xv[w]) * Zx[w]; A0 += e; A1 += e * Zx[w]; A3 += (e * Zx[w]) * Zx[w];
+                        }
+                        B0 += exk * xk; B1 += (exk * xk) * Z_KMAG_BEND; A0 += exk; A1 += exk * Z_KMAG_BEND;
+                        A3 += (exk * Z_KMAG_BEND) * Z_KMAG_BEND;
+                        return fit_finish(A0, A1, A3, B0, B1);
+                    };
+                    Fit f = xfit(X);
+                    if (xpres == 3u) {   // X / X' pair, x-only decision with the fit's parameters as errors
+                        const int h = lr_xpair_decide(f.p0, f.p1, f.p0, f.p1, 4, 3, X[0], X[1], xdr[0], xdr[1], 150.f);
+                        if (h >= 0) { xsign[0] = lr_si(h); xsign[1] = lr_sj(h); }
+                    }
+        #pragma unroll
+                    for (int w = 0; w < 2; ++w)
+                        if (((xpres >> w) & 1u) && xsign[w] == 0) xsign[w] = G(F_Z, det6[w]) * f.p1 + f.p0 > X[w] ? 1 : -1;
+                    float Xc[2];
+        #pragma unroll
+                    for (int w = 0; w < 2; ++w) Xc[w] = X[w] + (float)xsign[w] * xdr[w];
+                    f = xfit(Xc);
+                    const float x01 = f.p0, tx1 = f.p1;  // parameters used as errors (reconstruction_kernels.cuh:2194-2195)
+                    const int q = (double)(-0.0033f) * copysign(1.0, (double)FMAGSTR) * (double)x0 < (double)tx ? 1 : -1;
+                    const float invp = ((tx1 - tx) * (float)q) / PT_KICK_KMAG;
+                    const float einvp = (etx - etx) / PT_KICK_KMAG;
+                    if (invp > INVP_MAX + einvp || invp < INVP_MIN - einvp) continue;
+                    const bool cross = xpres == 2u;   // lone X'
+                    int xs_state = xsign[1];
+                    // ---- (u, v) pairs, one per lane
+                    for (int base = 0; base < ncomb; base += WAVE) {
+                        const int iuv = base + lane;
+                        bool reached = false, cand = false, wrote = false;
+                        float fy0 = 0.f, fty = 0.f, fey0 = 0.f, fety = 0.f, chi23 = 0.f;
+                        unsigned pres = xpres;
+                        int sign[6], raw[6];
+                        float pos[6], drift[6];
+                        sign[0] = xsign[0]; raw[0] = xraw[0]; pos[0] = xpos[0]; drift[0] = xdr[0];
+                        sign[1] = cross ? 2 : xsign[1]; raw[1] = xraw[1]; pos[1] = xpos[1]; drift[1] = xdr[1];
+        #pragma unroll
+                        for (int s = 2; s < 6; ++s) { sign[s] = 0; raw[s] = 0; pos[s] = 0.f; drift[s] = 0.f; }
+                        if (iuv < ncomb) {
+                            float A0 = A0s, A1 = A1s, A3 = A3s, B0 = B0s, B1 = B1s;
+                            const int iu = iuv % nu1, iv = iuv / nu1;
+                            const int ui[4] = {pa(lu[iu]), pb(lu[iu]), pa(lv[iv]), pb(lv[iv])};
+                            int nuh = 0, nvh = 0;
+        #pragma unroll
+                            for (int s = 2; s < 6; ++s) {
+                                const int idx = ui[s - 2];
+                                if (idx < 0 || (s >= 4 && nuh == 0)) continue;
+                                const int d = det6[s];
+                                const int r = E.raw(d, idx);
+                                float y, ey;
+                                if (y_uvhit(d, E.Lp->elem[r], E.Lp->drift[r], 0, x01, tx1, y, ey)) {
+                                    pres |= 1u << s;
+                                    raw[s] = r; pos[s] = E.Lp->pos[r]; drift[s] = E.Lp->drift[r];
+                                    const float z = G(F_Z, d);
+                                    B0 += ey * y; B1 += (ey * y) * z; A0 += ey; A1 += ey * z; A3 += (ey * z) * z;
+                                    if (s < 4) ++nuh;
+                                    else ++nvh;
                                 }
-                                cand = true;
+                            }
+                            if (nuh > 0 && nvh > 0) {
+                                reached = true;
+                                const Fit g = fit_finish(A0, A1, A3, B0, B1);
+                                fy0 = g.p0; fty = g.p1; fey0 = g.e0; fety = g.e1;
+                                if (!(fabsf(fy0) > Y0_MAX + 2.f * fey0 || fabsf(fty) > TY_MAX + 2.f * fety)) {
+                                    int dets[6] = {4, 3, 2, 1, 6, 5};
+                                    lr_slots_new<6>(x01, tx1, fy0, fty, x01, tx1, fey0, fety, pres, dets, pos, drift, sign,
+                                                    150.f);
+                                    if (cross) wrote = sign[1] != 2;
+                                    if (!(fabsf(fy0) > Y0_MAX || fabsf(fty) > TY_MAX)) {
+                                        // station-2/3 residuals with (x0, tx) of the tracklet and the new y line
+                                        for (int m = 0; m < n23; ++m) {
+                                            const int d = T->det[m];
+                                            const float res = G(F_RESOLUTION, d);
+                                            const float dca = dca_wire(d, E.Lp->elem[T->hraw[m]], x0, fy0, tx, fty);
+                                            const float r23 = E.Lp->drift[T->hraw[m]] * (float)T->sign[m] - dca;
+                                            chi23 += (r23 * r23) / (res * res);
+                                        }
+                                        cand = true;
+                                    }
+                                }
                             }
                         }
+                        // sign of a lone X' hit: last writer at or before this lane, else the running state
+                        if (cross) {
+                            const unsigned long long wm = ballot(wrote);
+                            const unsigned long long le = wm & (lane == 63 ? ~0ull : ((2ull << lane) - 1ull));
+                            const int src = le ? 63 - __clzll((long long)le) : 0;
+                            const int sv = __shfl(sign[1], src, WAVE);
+                            if (iuv < ncomb) sign[1] = le ? sv : xs_state;
+                            if (wm) xs_state = __shfl(sign[1], 63 - __clzll((long long)wm), WAVE);
+                        }
+                        unsigned long long key = ~0ull;
+                        if (cand) {
+                            float c1 = 0.f;
+        #pragma unroll
+                            for (int s = 0; s < 6; ++s) {
+                                if (!((pres >> s) & 1u)) continue;
+                                const int d = det6[s];
+                                const float res = G(F_RESOLUTION, d);
+                                const float r1 = drift[s] * (float)sign[s] - dca_wire(d, E.Lp->elem[raw[s]], x01, fy0, tx1, fty);
+                                c1 += ((r1 * r1) / res) / res;
+                            }
+                            const float chi2 = chi23 + c1;
+                            if (!(chi2 > 10000.f) && chi2 < 10000.1f)
+                                key = ((unsigned long long)__float_as_uint(chi2) << 32) | (uint32_t)(ix * ncomb + iuv);
+                        }
+                        const unsigned long long cmin = wave_min_u64(key);
+                        if (cmin < best_key) {
+                            if (key == cmin) {
+                                GBest& B = *best;
+                                B.chi2 = __uint_as_float((uint32_t)(key >> 32));
+                                B.y0 = fy0; B.ty = fty; B.ey0 = fey0; B.ety = fety; B.invp = invp; B.einvp = einvp;
+                                B.x01 = x01; B.tx1 = tx1; B.q = q; B.pres = pres;
+        #pragma unroll
+                                for (int s = 0; s < 6; ++s) { B.raw[s] = raw[s]; B.sign[s] = sign[s]; }
+                            }
+                            best_key = cmin;
+                        }
+                        // y0/ty for the next x pair: the last (u, v) pair that reached the y fit
+                        const unsigned long long rm = ballot(reached);
+                        if (rm) {
+                            const int src = 63 - __clzll((long long)rm);
+                            y0 = __shfl(fy0, src, WAVE);
+                            ty = __shfl(fty, src, WAVE);
+                        }
                     }
                 }
-                // sign of a lone X' hit: last writer at or before this lane, else the running state
-                if (cross) {
-                    const unsigned long long wm = ballot(wrote);
-                    const unsigned long long le = wm & (lane == 63 ? ~0ull : ((2ull << lane) - 1ull));
-                    const int src = le ? 63 - __clzll((long long)le) : 0;
-                    const int sv = __shfl(sign[1], src, WAVE);
-                    if (iuv < ncomb) sign[1] = le ? sv : xs_state;
-                    if (wm) xs_state = __shfl(sign[1], 63 - __clzll((long long)wm), WAVE);
-                }
-                unsigned long long key = ~0ull;
-                if (cand) {
-                    float c1 = 0.f;
-#pragma unroll
+                __syncthreads();
+                OR_SUB(7);
+                if (best_key != ~0ull && lane == 0) {
+                    const GBest& B = *best;
+                    T->stid = 6;
+                    T->chi2 = B.chi2; T->y0 = B.y0; T->ty = B.ty; T->ey0 = B.ey0; T->ety = B.ety;
+                    T->invp = B.invp; T->einvp = B.einvp; T->charge = (int8_t)B.q;
+                    for (int m = 0; m < n23; ++m) {
+                        const int d = T->det[m], r = T->hraw[m];
+                        T->resid[m] = E.Lp->drift[r] * (float)T->sign[m] - dca_wire(d, E.Lp->elem[r], x0, B.y0, tx, B.ty);
+                    }
+                    int n1 = 0;
                     for (int s = 0; s < 6; ++s) {
-                        if (!((pres >> s) & 1u)) continue;
-                        const int d = det6[s];
-                        const float res = G(F_RESOLUTION, d);
-                        const float r1 = drift[s] * (float)sign[s] - dca_wire(d, E.Lp->elem[raw[s]], x01, fy0, tx1, fty);
-                        c1 += ((r1 * r1) / res) / res;
+                        if (!((B.pres >> s) & 1u)) continue;
+                        const int d = det6[s], r = B.raw[s];
+                        T->resid[n23 + n1] = E.Lp->drift[r] * (float)B.sign[s] -
+                                             dca_wire(d, E.Lp->elem[r], B.x01, B.y0, B.tx1, B.ty);
+                        T->hraw[n23 + n1] = (uint16_t)r;
+                        T->det[n23 + n1] = (uint8_t)d;
+                        T->sign[n23 + n1] = (int8_t)B.sign[s];
+                        ++n1;
                     }
-                    const float chi2 = chi23 + c1;
-                    if (!(chi2 > 10000.f) && chi2 < 10000.1f)
-                        key = ((unsigned long long)__float_as_uint(chi2) << 32) | (uint32_t)(ix * ncomb + iuv);
+                    T->n1 = (uint8_t)n1;
                 }
-                const unsigned long long cmin = wave_min_u64(key);
-                if (cmin < best_key) {
-                    if (key == cmin) {
-                        GBest& B = *best;
-                        B.chi2 = __uint_as_float((uint32_t)(key >> 32));
-                        B.y0 = fy0; B.ty = fty; B.ey0 = fey0; B.ety = fety; B.invp = invp; B.einvp = einvp;
-                        B.x01 = x01; B.tx1 = tx1; B.q = q; B.pres = pres;
-#pragma unroll
-                        for (int s = 0; s < 6; ++s) { B.raw[s] = raw[s]; B.sign[s] = sign[s]; }
-                    }
-                    best_key = cmin;
-                }
-                // y0/ty for the next x pair: the last (u, v) pair that reached the y fit
-                const unsigned long long rm = ballot(reached);
-                if (rm) {
-                    const int src = 63 - __clzll((long long)rm);
-                    y0 = __shfl(fy0, src, WAVE);
-                    ty = __shfl(fty, src, WAVE);
-                }
+                __syncthreads();
+                OR_SUB(8);
             }
         }
-        __syncthreads();
-        OR_SUB(7);
-        if (best_key != ~0ull && lane == 0) {
-            const GBest& B = *best;
-            T->stid = 6;
-            T->chi2 = B.chi2; T->y0 = B.y0; T->ty = B.ty; T->ey0 = B.ey0; T->ety = B.ety;
-            T->invp = B.invp; T->einvp = B.einvp; T->charge = (int8_t)B.q;
-            for (int m = 0; m < n23; ++m) {
-                const int d = T->det[m], r = T->hraw[m];
-                T->resid[m] = E.Lp->drift[r] * (float)T->sign[m] - dca_wire(d, E.Lp->elem[r], x0, B.y0, tx, B.ty);
-            }
-            int n1 = 0;
-            for (int s = 0; s < 6; ++s) {
-                if (!((B.pres >> s) & 1u)) continue;
-                const int d = det6[s], r = B.raw[s];
-                T->resid[n23 + n1] = E.Lp->drift[r] * (float)B.sign[s] -
-                                     dca_wire(d, E.Lp->elem[r], B.x01, B.y0, B.tx1, B.ty);
-                T->hraw[n23 + n1] = (uint16_t)r;
-                T->det[n23 + n1] = (uint8_t)d;
-                T->sign[n23 + n1] = (int8_t)B.sign[s];
-                ++n1;
-            }
-            T->n1 = (uint8_t)n1;
-        }
-        __syncthreads();
-        OR_SUB(8);
     }
 }
 
