@@ -1579,21 +1579,27 @@ STAGE void stage_clean(Tk* tks, const uint16_t* slot_order, int ntkl) {
 }
 
 // G: station-1 extension (reconstruction_kernels.cuh:1841-2534), one track at a
-// time, the wave's lanes spread over the track's (u, v) pairs.
+// time.
 //  * The three station-1 window lists (X, U, V) of a batch of tracks are built
 //    together, one lane per (track, view).
 //  * Per x pair the x part (x at the current y0/ty, KMAG-point fit, left/right,
 //    refit, 1/p window) is computed by every lane alike.
 //  * Hits sit in fixed slots X (4), X' (3), U (2), U' (1), V (6), V' (5), the
 //    reference's list order; see lr_slots_new.
-//  * (u, v) pairs of one x pair are independent except for one case: a lone X'
-//    hit next to a U hit forms a left/right pair whose sign writes carry to
-//    later (u, v) pairs.  That write does not depend on the current sign, so
+//  * Two passes.  Pass A walks the x pairs in order with the lanes spread over
+//    the (u, v) pairs: y sums, fit and first box cut only, listing the
+//    survivors.  y0/ty after an x pair are those of the last (u, v) pair that
+//    reached the y fit (the refit after left/right repeats the same fit); they
+//    feed the next x pair's x-at-y, so pass A is sequential over x pairs.
+//    Pass B evaluates the listed survivors, packed one per lane: left/right,
+//    second box cut, residual chi2.  (A full list is evaluated before pass A
+//    goes on.)
+//  * Survivors are independent except for one case: a lone X' hit next to a U
+//    hit forms a left/right pair whose sign writes carry to later (u, v) pairs
+//    of the same x pair.  That write does not depend on the current sign, so
 //    each lane records whether it writes (sentinel 2 in the sign slot) and a
-//    ballot scan hands each lane the sign left by the last writer before it.
-//  * y0/ty after an x pair are those of the last (u, v) pair that reached the
-//    y fit (the refit after left/right repeats the same fit); they feed the
-//    next x pair's x-at-y.
+//    scan over the list hands each lane the sign left by the last writer before
+//    it within its x pair, else the x pair's starting sign.
 //  * The best candidate is the minimum of (chi2, flat index ix*ncomb + iuv),
 //    which is the reference's strict-< first minimum; its residuals are
 //    recomputed from its parameters when the track is written.
@@ -1646,6 +1652,284 @@ __device__ __forceinline__ int global_window(const Ev<MAXH>& E, const Tk* T, int
     return pairs_window(E, 0, v, pexp - win * 1.0f, pexp + win * 1.0f, out, plane_cap, capw);
 }
 
+// x part of station-1 X entry qx at the current y0/ty (every lane alike);
+// false when the entry has no hit or fails the 1/p window
+struct GX {
+    float x01, tx1, invp;
+    int q;
+    unsigned xpres;
+    int xsign[2], xraw[2];
+    float xdr[2], xpos[2];
+};
+
+template <int MAXH>
+__device__ __forceinline__ bool global_xpart(const Ev<MAXH>& E, uint16_t qx, float x0, float tx, float ex0, float etx,
+                                             float y0, float ty, GX& X) {
+    constexpr int dx[2] = {4, 3};
+    const int xi[2] = {pa(qx), pb(qx)};
+    X.xpres = 0;
+    float Xp[2] = {0.f, 0.f}, Zx[2] = {0.f, 0.f};
+#pragma unroll
+    for (int w = 0; w < 2; ++w) {
+        X.xraw[w] = 0; X.xsign[w] = 0; X.xdr[w] = 0.f; X.xpos[w] = 0.f;
+        if (xi[w] < 0) continue;
+        const int r = E.raw(dx[w], xi[w]);
+        X.xpres |= 1u << w;
+        X.xraw[w] = r; X.xdr[w] = E.Lp->drift[r]; X.xpos[w] = E.Lp->pos[r];
+        xz_fy(dx[w], E.Lp->elem[r], X.xdr[w], 0, y0, ty, Xp[w], Zx[w]);
+    }
+    if (X.xpres == 0) return false;
+    const float xk = x0 + tx * Z_KMAG_BEND, exk = ex0 + etx * Z_KMAG_BEND;
+    auto xfit = [&](const float (&xv)[2]) {
+        float A0 = 0.f, A1 = 0.f, A3 = 0.f, B0 = 0.f, B1 = 0.f;
+#pragma unroll
+        for (int w = 0; w < 2; ++w) {
+            if (!((X.xpres >> w) & 1u)) continue;
+            const float e = G(F_RESOLUTION, dx[w]);
+            B0 += e * xv[w]; B1 += (e * xv[w]) * Zx[w]; A0 += e; A1 += e * Zx[w]; A3 += (e * Zx[w]) * Zx[w];
+        }
+        B0 += exk * xk; B1 += (exk * xk) * Z_KMAG_BEND; A0 += exk; A1 += exk * Z_KMAG_BEND;
+        A3 += (exk * Z_KMAG_BEND) * Z_KMAG_BEND;
+        return fit_finish(A0, A1, A3, B0, B1);
+    };
+    Fit f = xfit(Xp);
+    if (X.xpres == 3u) {   // X / X' pair, x-only decision with the fit's parameters as errors
+        const int h = lr_xpair_decide(f.p0, f.p1, f.p0, f.p1, 4, 3, Xp[0], Xp[1], X.xdr[0], X.xdr[1], 150.f);
+        if (h >= 0) { X.xsign[0] = lr_si(h); X.xsign[1] = lr_sj(h); }
+    }
+#pragma unroll
+    for (int w = 0; w < 2; ++w)
+        if (((X.xpres >> w) & 1u) && X.xsign[w] == 0) X.xsign[w] = G(F_Z, dx[w]) * f.p1 + f.p0 > Xp[w] ? 1 : -1;
+    float Xc[2];
+#pragma unroll
+    for (int w = 0; w < 2; ++w) Xc[w] = Xp[w] + (float)X.xsign[w] * X.xdr[w];
+    f = xfit(Xc);
+    X.x01 = f.p0; X.tx1 = f.p1;  // parameters used as errors (reconstruction_kernels.cuh:2194-2195)
+    X.q = (double)(-0.0033f) * copysign(1.0, (double)FMAGSTR) * (double)x0 < (double)tx ? 1 : -1;
+    X.invp = ((X.tx1 - tx) * (float)X.q) / PT_KICK_KMAG;
+    const float einvp = (etx - etx) / PT_KICK_KMAG;
+    return !(X.invp > INVP_MAX + einvp || X.invp < INVP_MIN - einvp);
+}
+
+// y fit of (u, v) pair iuv of an x entry: the station-2/3 sums As, then the
+// accepted U hits (a then b) and, when there is one, the accepted V hits.
+// False when the pair does not reach the fit (no accepted U or V hit).  With
+// SLOTS the hits go into slots 2-5.
+template <bool SLOTS, int MAXH>
+__device__ __forceinline__ bool global_yfit(const Ev<MAXH>& E, const uint16_t* lu, const uint16_t* lv, int nu1, int iuv,
+                                            const GX& X, const float (&As)[5], unsigned& pres, int (&raw)[6],
+                                            float (&pos)[6], float (&drift)[6], Fit& g) {
+    constexpr int det6[6] = {4, 3, 2, 1, 6, 5};
+    float A0 = As[0], A1 = As[1], A3 = As[2], B0 = As[3], B1 = As[4];
+    const int iu = iuv % nu1, iv = iuv / nu1;
+    const uint16_t qu = lu[iu], qv = lv[iv];
+    const int ui[4] = {pa(qu), pb(qu), pa(qv), pb(qv)};
+    int nuh = 0, nvh = 0;
+#pragma unroll
+    for (int s = 2; s < 6; ++s) {
+        const int idx = ui[s - 2];
+        if (idx < 0 || (s >= 4 && nuh == 0)) continue;
+        const int d = det6[s];
+        const int r = E.raw(d, idx);
+        float y, ey;
+        if (y_uvhit(d, E.Lp->elem[r], E.Lp->drift[r], 0, X.x01, X.tx1, y, ey)) {
+            if (SLOTS) {
+                pres |= 1u << s;
+                raw[s] = r; pos[s] = E.Lp->pos[r]; drift[s] = E.Lp->drift[r];
+            }
+            const float z = G(F_Z, d);
+            B0 += ey * y; B1 += (ey * y) * z; A0 += ey; A1 += ey * z; A3 += (ey * z) * z;
+            if (s < 4) ++nuh;
+            else ++nvh;
+        }
+    }
+    if (nuh == 0 || nvh == 0) return false;
+    g = fit_finish(A0, A1, A3, B0, B1);
+    return true;
+}
+
+constexpr int GLOBAL_LIST_CAP = 2048;   // pass-A survivors held before pass B runs
+
+// One track: pass A over its x pairs, pass B over the listed survivors, then
+// the best candidate is written.  Uses the x-part records (S.wy + 8), the
+// survivor list (S.plist) and the best-candidate record (S.wy).
+template <int MAXH>
+__device__ __attribute__((noinline)) void global_track(Scr S, Tk* T, const uint16_t* lx, const uint16_t* lu,
+                                                       const uint16_t* lv, int nx1, int nu1, int nv1) {
+    Ev<MAXH> E;
+    const int lane = lane_id();
+    constexpr int det6[6] = {4, 3, 2, 1, 6, 5};
+    GBest* best = reinterpret_cast<GBest*>(S.wy);
+    GX* xrec = reinterpret_cast<GX*>(S.wy + 8);
+    uint32_t* list = S.plist;
+    const int n23 = T->nx + T->nuv;
+    const float x0 = T->x0, tx = T->tx, ex0 = T->ex0, etx = T->etx;
+    const float einvp = (etx - etx) / PT_KICK_KMAG;
+    float y0 = T->y0, ty = T->ty;
+    OR_SUB_START
+    // y sums of the station-2/3 UV hits with their stored signs (reconstruction_kernels.cuh:2086-2102)
+    float As[5] = {0.f, 0.f, 0.f, 0.f, 0.f};   // A0, A1, A3, B0, B1
+    for (int m = 0; m < n23; ++m) {
+        const int d = T->det[m];
+        if ((double)G(F_COSTHETA, d) > 0.99) continue;
+        float y, ey;
+        if (y_uvhit(d, E.Lp->elem[T->hraw[m]], E.Lp->drift[T->hraw[m]], T->sign[m], x0, tx, y, ey)) {
+            const float z = G(F_Z, d);
+            As[3] += ey * y; As[4] += (ey * y) * z; As[0] += ey; As[1] += ey * z; As[2] += (ey * z) * z;
+        }
+    }
+    const int ncomb = nu1 * nv1;
+    unsigned long long best_key = ~0ull;
+    int carry_ix = -1, carry_sign = 0;   // last lone-X' sign writer evaluated so far
+    int ix = 0, base = 0;
+    bool have_x = false;
+    GX X;
+    while (ix < nx1) {
+        // ---- pass A: x pairs in order until done or the list is full
+        int np = 0;
+        while (ix < nx1) {
+            if (!have_x) {
+                if (!global_xpart(E, lx[ix], x0, tx, ex0, etx, y0, ty, X)) { ++ix; continue; }
+                have_x = true;
+                base = 0;
+                if (lane == 0) xrec[ix] = X;
+            }
+            if (np + WAVE > GLOBAL_LIST_CAP) break;
+            const int iuv = base + lane;
+            bool reached = false, pass = false;
+            Fit g = {0.f, 0.f, 0.f, 0.f};
+            if (iuv < ncomb) {
+                unsigned pres = 0;
+                int raw[6];
+                float pos[6], drift[6];
+                reached = global_yfit<false>(E, lu, lv, nu1, iuv, X, As, pres, raw, pos, drift, g);
+                pass = reached && !(fabsf(g.p0) > Y0_MAX + 2.f * g.e0 || fabsf(g.p1) > TY_MAX + 2.f * g.e1);
+            }
+            const unsigned long long pm = ballot(pass);
+            if (pass) list[np + rank_in(pm)] = ((uint32_t)ix << 16) | (uint32_t)iuv;
+            np += __popcll(pm);
+            // y0/ty for the next x pair: the last (u, v) pair that reached the y fit
+            const unsigned long long rm = ballot(reached);
+            if (rm) {
+                const int src = 63 - __clzll((long long)rm);
+                y0 = __shfl(g.p0, src, WAVE);
+                ty = __shfl(g.p1, src, WAVE);
+            }
+            base += WAVE;
+            if (base >= ncomb) { have_x = false; ++ix; }
+        }
+        __syncthreads();
+        OR_SUB(7);
+        // ---- pass B: the listed survivors, one per lane, in list order
+        for (int k0 = 0; k0 < np; k0 += WAVE) {
+            const int k = k0 + lane;
+            int kix = -1, sign[6] = {0, 0, 0, 0, 0, 0}, raw[6] = {0, 0, 0, 0, 0, 0};
+            float pos[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, drift[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+            bool cand = false, wrote = false, cross = false;
+            float fy0 = 0.f, fty = 0.f, fey0 = 0.f, fety = 0.f, chi23 = 0.f;
+            unsigned pres = 0;
+            GX Y;
+            if (k < np) {
+                const uint32_t code = list[k];
+                kix = (int)(code >> 16);
+                const int iuv = (int)(code & 0xFFFFu);
+                Y = xrec[kix];
+                cross = Y.xpres == 2u;   // lone X'
+                pres = Y.xpres;
+                sign[0] = Y.xsign[0]; raw[0] = Y.xraw[0]; pos[0] = Y.xpos[0]; drift[0] = Y.xdr[0];
+                sign[1] = cross ? 2 : Y.xsign[1]; raw[1] = Y.xraw[1]; pos[1] = Y.xpos[1]; drift[1] = Y.xdr[1];
+                Fit g;
+                global_yfit<true>(E, lu, lv, nu1, iuv, Y, As, pres, raw, pos, drift, g);
+                fy0 = g.p0; fty = g.p1; fey0 = g.e0; fety = g.e1;
+                int dets[6] = {4, 3, 2, 1, 6, 5};
+                lr_slots_new<6>(Y.x01, Y.tx1, fy0, fty, Y.x01, Y.tx1, fey0, fety, pres, dets, pos, drift, sign, 150.f);
+                if (cross) wrote = sign[1] != 2;
+                if (!(fabsf(fy0) > Y0_MAX || fabsf(fty) > TY_MAX)) {
+                    // station-2/3 residuals with (x0, tx) of the tracklet and the new y line
+                    for (int m = 0; m < n23; ++m) {
+                        const int d = T->det[m];
+                        const float res = G(F_RESOLUTION, d);
+                        const float dca = dca_wire(d, E.Lp->elem[T->hraw[m]], x0, fy0, tx, fty);
+                        const float r23 = E.Lp->drift[T->hraw[m]] * (float)T->sign[m] - dca;
+                        chi23 += (r23 * r23) / (res * res);
+                    }
+                    cand = true;
+                }
+            }
+            // sign of a lone X' hit: the last writer at or before this lane in the
+            // same x pair, else the x pair's starting sign
+            const unsigned long long wm = ballot(wrote);
+            {
+                const unsigned long long le = wm & (lane == 63 ? ~0ull : ((2ull << lane) - 1ull));
+                const int src = le ? 63 - __clzll((long long)le) : 0;
+                const int sv = __shfl(sign[1], src, WAVE), six = __shfl(kix, src, WAVE);
+                if (cross) {
+                    if (le) sign[1] = six == kix ? sv : Y.xsign[1];
+                    else sign[1] = carry_ix == kix ? carry_sign : Y.xsign[1];
+                }
+                if (wm) {
+                    const int last = 63 - __clzll((long long)wm);
+                    carry_ix = __shfl(kix, last, WAVE);
+                    carry_sign = __shfl(sign[1], last, WAVE);
+                }
+            }
+            unsigned long long key = ~0ull;
+            if (cand) {
+                float c1 = 0.f;
+#pragma unroll
+                for (int s = 0; s < 6; ++s) {
+                    if (!((pres >> s) & 1u)) continue;
+                    const int d = det6[s];
+                    const float res = G(F_RESOLUTION, d);
+                    const float r1 =
+                        drift[s] * (float)sign[s] - dca_wire(d, E.Lp->elem[raw[s]], Y.x01, fy0, Y.tx1, fty);
+                    c1 += ((r1 * r1) / res) / res;
+                }
+                const float chi2 = chi23 + c1;
+                if (!(chi2 > 10000.f) && chi2 < 10000.1f)
+                    key = ((unsigned long long)__float_as_uint(chi2) << 32) |
+                          (uint32_t)(kix * ncomb + (int)(list[k] & 0xFFFFu));
+            }
+            const unsigned long long cmin = wave_min_u64(key);
+            if (cmin < best_key) {
+                if (key == cmin) {
+                    GBest& B = *best;
+                    B.chi2 = __uint_as_float((uint32_t)(key >> 32));
+                    B.y0 = fy0; B.ty = fty; B.ey0 = fey0; B.ety = fety; B.invp = Y.invp; B.einvp = einvp;
+                    B.x01 = Y.x01; B.tx1 = Y.tx1; B.q = Y.q; B.pres = pres;
+#pragma unroll
+                    for (int s = 0; s < 6; ++s) { B.raw[s] = raw[s]; B.sign[s] = sign[s]; }
+                }
+                best_key = cmin;
+            }
+        }
+        __syncthreads();
+        OR_SUB(8);
+    }
+    if (best_key != ~0ull && lane == 0) {
+        const GBest& B = *best;
+        T->stid = 6;
+        T->chi2 = B.chi2; T->y0 = B.y0; T->ty = B.ty; T->ey0 = B.ey0; T->ety = B.ety;
+        T->invp = B.invp; T->einvp = B.einvp; T->charge = (int8_t)B.q;
+        for (int m = 0; m < n23; ++m) {
+            const int d = T->det[m], r = T->hraw[m];
+            T->resid[m] = E.Lp->drift[r] * (float)T->sign[m] - dca_wire(d, E.Lp->elem[r], x0, B.y0, tx, B.ty);
+        }
+        int n1 = 0;
+        for (int s = 0; s < 6; ++s) {
+            if (!((B.pres >> s) & 1u)) continue;
+            const int d = det6[s], r = B.raw[s];
+            T->resid[n23 + n1] = E.Lp->drift[r] * (float)B.sign[s] - dca_wire(d, E.Lp->elem[r], B.x01, B.y0, B.tx1, B.ty);
+            T->hraw[n23 + n1] = (uint16_t)r;
+            T->det[n23 + n1] = (uint8_t)d;
+            T->sign[n23 + n1] = (int8_t)B.sign[s];
+            ++n1;
+        }
+        T->n1 = (uint8_t)n1;
+    }
+    __syncthreads();
+}
+
 // The window lists depend only on the tracklet, and one overflow rejects the
 // event, so the lists of up to GLOBAL_BATCH tracklets (three each) are built
 // together, lane (tracklet, view); the tracklets are then extended one after
@@ -1659,9 +1943,7 @@ STAGE void stage_global(or_params P, Scr S, int ntkl) {
     const int lane = lane_id();
     Tk* tks = S.tks;
     int* nlf = &L.nl[0][0];
-    GBest* best = reinterpret_cast<GBest*>(S.wy);
     const int capw = P.max_pairs_window < WIN_CAP ? P.max_pairs_window : WIN_CAP;
-    const int det6[6] = {4, 3, 2, 1, 6, 5};
     for (int k0 = 0; k0 < ntkl; k0 += WAVE) {
         unsigned long long rest = ballot(k0 + lane < ntkl && tks[k0 + lane].stid >= 5);
         while (rest) {
@@ -1683,205 +1965,10 @@ STAGE void stage_global(or_params P, Scr S, int ntkl) {
             for (int b = 0; b < GLOBAL_BATCH && rest; ++b) {
                 const int k = k0 + __ffsll((long long)rest) - 1;
                 rest &= rest - 1;
-                Tk* T = &tks[k];
-                const int n23 = T->nx + T->nuv;
-                const float x0 = T->x0, tx = T->tx, ex0 = T->ex0, etx = T->etx;
-                float y0 = T->y0, ty = T->ty;
-                const uint16_t* lx = S.wl + (3 * b) * WIN_CAP;
-                const uint16_t* lu = S.wl + (3 * b + 1) * WIN_CAP;
-                const uint16_t* lv = S.wl + (3 * b + 2) * WIN_CAP;
-                OR_SUB_START
                 const int nx1 = nlf[3 * b], nu1 = nlf[3 * b + 1], nv1 = nlf[3 * b + 2];
                 if (nx1 == 0 || nu1 == 0 || nv1 == 0) continue;
-                // y sums of the station-2/3 UV hits with their stored signs (reconstruction_kernels.cuh:2086-2102)
-                float A0s = 0.f, A1s = 0.f, A3s = 0.f, B0s = 0.f, B1s = 0.f;
-                for (int m = 0; m < n23; ++m) {
-                    const int d = T->det[m];
-                    if ((double)G(F_COSTHETA, d) > 0.99) continue;
-                    float y, ey;
-                    if (y_uvhit(d, E.Lp->elem[T->hraw[m]], E.Lp->drift[T->hraw[m]], T->sign[m], x0, tx, y, ey)) {
-                        const float z = G(F_Z, d);
-                        B0s += ey * y; B1s += (ey * y) * z; A0s += ey; A1s += ey * z; A3s += (ey * z) * z;
-                    }
-                }
-                unsigned long long best_key = ~0ull;
-                const int ncomb = nu1 * nv1;
-                for (int ix = 0; ix < nx1; ++ix) {
-                    // ---- x part, every lane alike: slots X (4), X' (3) and the KMAG point
-                    const uint16_t qx = lx[ix];
-                    const int xi[2] = {pa(qx), pb(qx)};
-                    unsigned xpres = 0;
-                    int xraw[2] = {0, 0}, xsign[2] = {0, 0};
-                    float X[2] = {0.f, 0.f}, Zx[2] = {0.f, 0.f}, xdr[2] = {0.f, 0.f}, xpos[2] = {0.f, 0.f};
-        #pragma unroll
-                    for (int w = 0; w < 2; ++w) {
-                        if (xi[w] < 0) continue;
-                        const int d = det6[w];
-                        const int r = E.raw(d, xi[w]);
-                        xpres |= 1u << w;
-                        xraw[w] = r; xdr[w] = E.Lp->drift[r]; xpos[w] = E.Lp->pos[r];
-                        xz_fy(d, E.Lp->elem[r], xdr[w], 0, y0, ty, X[w], Zx[w]);
-                    }
-                    if (xpres == 0) continue;
-                    const float xk = x0 + tx * Z_KMAG_BEND, exk = ex0 + etx * Z_KMAG_BEND;
-                    auto xfit = [&](const float (&xv)[2]) {
-                        float A0 = 0.f, A1 = 0.f, A3 = 0.f, B0 = 0.f, B1 = 0.f;
-        #pragma unroll
-                        for (int w = 0; w < 2; ++w) {
-                            if (!((xpres >> w) & 1u)) continue;
-                            const float e = G(F_RESOLUTION, det6[w]);
-                            B0 += e * xv[w]; B1 += (e * xv[w]) * Zx[w]; A0 += e; A1 += e * Zx[w]; A3 += (e * Zx[w]) * Zx[w];
-                        }
-                        B0 += exk * xk; B1 += (exk * xk) * Z_KMAG_BEND; A0 += exk; A1 += exk * Z_KMAG_BEND;
-                        A3 += (exk * Z_KMAG_BEND) * Z_KMAG_BEND;
-                        return fit_finish(A0, A1, A3, B0, B1);
-                    };
-                    Fit f = xfit(X);
-                    if (xpres == 3u) {   // X / X' pair, x-only decision with the fit's parameters as errors
-                        const int h = lr_xpair_decide(f.p0, f.p1, f.p0, f.p1, 4, 3, X[0], X[1], xdr[0], xdr[1], 150.f);
-                        if (h >= 0) { xsign[0] = lr_si(h); xsign[1] = lr_sj(h); }
-                    }
-        #pragma unroll
-                    for (int w = 0; w < 2; ++w)
-                        if (((xpres >> w) & 1u) && xsign[w] == 0) xsign[w] = G(F_Z, det6[w]) * f.p1 + f.p0 > X[w] ? 1 : -1;
-                    float Xc[2];
-        #pragma unroll
-                    for (int w = 0; w < 2; ++w) Xc[w] = X[w] + (float)xsign[w] * xdr[w];
-                    f = xfit(Xc);
-                    const float x01 = f.p0, tx1 = f.p1;  // parameters used as errors (reconstruction_kernels.cuh:2194-2195)
-                    const int q = (double)(-0.0033f) * copysign(1.0, (double)FMAGSTR) * (double)x0 < (double)tx ? 1 : -1;
-                    const float invp = ((tx1 - tx) * (float)q) / PT_KICK_KMAG;
-                    const float einvp = (etx - etx) / PT_KICK_KMAG;
-                    if (invp > INVP_MAX + einvp || invp < INVP_MIN - einvp) continue;
-                    const bool cross = xpres == 2u;   // lone X'
-                    int xs_state = xsign[1];
-                    // ---- (u, v) pairs, one per lane
-                    for (int base = 0; base < ncomb; base += WAVE) {
-                        const int iuv = base + lane;
-                        bool reached = false, cand = false, wrote = false;
-                        float fy0 = 0.f, fty = 0.f, fey0 = 0.f, fety = 0.f, chi23 = 0.f;
-                        unsigned pres = xpres;
-                        int sign[6], raw[6];
-                        float pos[6], drift[6];
-                        sign[0] = xsign[0]; raw[0] = xraw[0]; pos[0] = xpos[0]; drift[0] = xdr[0];
-                        sign[1] = cross ? 2 : xsign[1]; raw[1] = xraw[1]; pos[1] = xpos[1]; drift[1] = xdr[1];
-        #pragma unroll
-                        for (int s = 2; s < 6; ++s) { sign[s] = 0; raw[s] = 0; pos[s] = 0.f; drift[s] = 0.f; }
-                        if (iuv < ncomb) {
-                            float A0 = A0s, A1 = A1s, A3 = A3s, B0 = B0s, B1 = B1s;
-                            const int iu = iuv % nu1, iv = iuv / nu1;
-                            const int ui[4] = {pa(lu[iu]), pb(lu[iu]), pa(lv[iv]), pb(lv[iv])};
-                            int nuh = 0, nvh = 0;
-        #pragma unroll
-                            for (int s = 2; s < 6; ++s) {
-                                const int idx = ui[s - 2];
-                                if (idx < 0 || (s >= 4 && nuh == 0)) continue;
-                                const int d = det6[s];
-                                const int r = E.raw(d, idx);
-                                float y, ey;
-                                if (y_uvhit(d, E.Lp->elem[r], E.Lp->drift[r], 0, x01, tx1, y, ey)) {
-                                    pres |= 1u << s;
-                                    raw[s] = r; pos[s] = E.Lp->pos[r]; drift[s] = E.Lp->drift[r];
-                                    const float z = G(F_Z, d);
-                                    B0 += ey * y; B1 += (ey * y) * z; A0 += ey; A1 += ey * z; A3 += (ey * z) * z;
-                                    if (s < 4) ++nuh;
-                                    else ++nvh;
-                                }
-                            }
-                            if (nuh > 0 && nvh > 0) {
-                                reached = true;
-                                const Fit g = fit_finish(A0, A1, A3, B0, B1);
-                                fy0 = g.p0; fty = g.p1; fey0 = g.e0; fety = g.e1;
-                                if (!(fabsf(fy0) > Y0_MAX + 2.f * fey0 || fabsf(fty) > TY_MAX + 2.f * fety)) {
-                                    int dets[6] = {4, 3, 2, 1, 6, 5};
-                                    lr_slots_new<6>(x01, tx1, fy0, fty, x01, tx1, fey0, fety, pres, dets, pos, drift, sign,
-                                                    150.f);
-                                    if (cross) wrote = sign[1] != 2;
-                                    if (!(fabsf(fy0) > Y0_MAX || fabsf(fty) > TY_MAX)) {
-                                        // station-2/3 residuals with (x0, tx) of the tracklet and the new y line
-                                        for (int m = 0; m < n23; ++m) {
-                                            const int d = T->det[m];
-                                            const float res = G(F_RESOLUTION, d);
-                                            const float dca = dca_wire(d, E.Lp->elem[T->hraw[m]], x0, fy0, tx, fty);
-                                            const float r23 = E.Lp->drift[T->hraw[m]] * (float)T->sign[m] - dca;
-                                            chi23 += (r23 * r23) / (res * res);
-                                        }
-                                        cand = true;
-                                    }
-                                }
-                            }
-                        }
-                        // sign of a lone X' hit: last writer at or before this lane, else the running state
-                        if (cross) {
-                            const unsigned long long wm = ballot(wrote);
-                            const unsigned long long le = wm & (lane == 63 ? ~0ull : ((2ull << lane) - 1ull));
-                            const int src = le ? 63 - __clzll((long long)le) : 0;
-                            const int sv = __shfl(sign[1], src, WAVE);
-                            if (iuv < ncomb) sign[1] = le ? sv : xs_state;
-                            if (wm) xs_state = __shfl(sign[1], 63 - __clzll((long long)wm), WAVE);
-                        }
-                        unsigned long long key = ~0ull;
-                        if (cand) {
-                            float c1 = 0.f;
-        #pragma unroll
-                            for (int s = 0; s < 6; ++s) {
-                                if (!((pres >> s) & 1u)) continue;
-                                const int d = det6[s];
-                                const float res = G(F_RESOLUTION, d);
-                                const float r1 = drift[s] * (float)sign[s] - dca_wire(d, E.Lp->elem[raw[s]], x01, fy0, tx1, fty);
-                                c1 += ((r1 * r1) / res) / res;
-                            }
-                            const float chi2 = chi23 + c1;
-                            if (!(chi2 > 10000.f) && chi2 < 10000.1f)
-                                key = ((unsigned long long)__float_as_uint(chi2) << 32) | (uint32_t)(ix * ncomb + iuv);
-                        }
-                        const unsigned long long cmin = wave_min_u64(key);
-                        if (cmin < best_key) {
-                            if (key == cmin) {
-                                GBest& B = *best;
-                                B.chi2 = __uint_as_float((uint32_t)(key >> 32));
-                                B.y0 = fy0; B.ty = fty; B.ey0 = fey0; B.ety = fety; B.invp = invp; B.einvp = einvp;
-                                B.x01 = x01; B.tx1 = tx1; B.q = q; B.pres = pres;
-        #pragma unroll
-                                for (int s = 0; s < 6; ++s) { B.raw[s] = raw[s]; B.sign[s] = sign[s]; }
-                            }
-                            best_key = cmin;
-                        }
-                        // y0/ty for the next x pair: the last (u, v) pair that reached the y fit
-                        const unsigned long long rm = ballot(reached);
-                        if (rm) {
-                            const int src = 63 - __clzll((long long)rm);
-                            y0 = __shfl(fy0, src, WAVE);
-                            ty = __shfl(fty, src, WAVE);
-                        }
-                    }
-                }
-                __syncthreads();
-                OR_SUB(7);
-                if (best_key != ~0ull && lane == 0) {
-                    const GBest& B = *best;
-                    T->stid = 6;
-                    T->chi2 = B.chi2; T->y0 = B.y0; T->ty = B.ty; T->ey0 = B.ey0; T->ety = B.ety;
-                    T->invp = B.invp; T->einvp = B.einvp; T->charge = (int8_t)B.q;
-                    for (int m = 0; m < n23; ++m) {
-                        const int d = T->det[m], r = T->hraw[m];
-                        T->resid[m] = E.Lp->drift[r] * (float)T->sign[m] - dca_wire(d, E.Lp->elem[r], x0, B.y0, tx, B.ty);
-                    }
-                    int n1 = 0;
-                    for (int s = 0; s < 6; ++s) {
-                        if (!((B.pres >> s) & 1u)) continue;
-                        const int d = det6[s], r = B.raw[s];
-                        T->resid[n23 + n1] = E.Lp->drift[r] * (float)B.sign[s] -
-                                             dca_wire(d, E.Lp->elem[r], B.x01, B.y0, B.tx1, B.ty);
-                        T->hraw[n23 + n1] = (uint16_t)r;
-                        T->det[n23 + n1] = (uint8_t)d;
-                        T->sign[n23 + n1] = (int8_t)B.sign[s];
-                        ++n1;
-                    }
-                    T->n1 = (uint8_t)n1;
-                }
-                __syncthreads();
-                OR_SUB(8);
+                global_track<MAXH>(S, &tks[k], S.wl + (3 * b) * WIN_CAP, S.wl + (3 * b + 1) * WIN_CAP,
+                                   S.wl + (3 * b + 2) * WIN_CAP, nx1, nu1, nv1);
             }
         }
     }

@@ -16,7 +16,7 @@ sys.path.insert(0, HERE)
 
 STAGES = ["load+reduce", "X pairs", "XZ", "YZ", "cleaning", "global", "outputs"]
 SUBS = ["YZ window lists", "YZ pass 1 (first fit)", "YZ pass 2 (seeds)", "YZ pass 3 (tails)", "YZ winners",
-        "YZ unit path", "st1 window lists", "st1 candidate loop", "st1 track write"]
+        "YZ unit path", "st1 window lists", "st1 pass A (y fits)", "st1 pass B + write"]
 
 
 def main():
