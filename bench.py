@@ -5,6 +5,11 @@ cleaning, station-1 extension, output records) over one batch of synthetic
 events already resident in HBM.  Default workload: 1,000,000 events, hits per
 event ~ Poisson(300) capped at 1000, 1-2 muon tracks, basic_geometry_e906.
 
+The same batch runs twice: the plain chain, then the chain with the shipped
+learned YZ unit pruner (e1039-gpu_amd/data/pruner_u3v3.npz).  The value is the
+pruned chain's events/s; the plain rate, the speedup and the pruned chain's
+track recall (plain-chain tracks it outputs identically) go beside it.
+
 Prints one JSON line with the throughput, the HBM roofline of the chain
 kernel (algorithmic bytes per event: 12 per input hit, 1 bit per hit of keep
 mask, 96 per output track) and the CPU oracle timed on a bounded sample.
@@ -49,6 +54,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="target CPU time of the CPU baseline sample (0 = skip the baseline)")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--plain", action="store_true", help="time the plain chain only (no pruner)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (reference for HBM byte counters) and exit")
     args = ap.parse_args()
@@ -82,26 +88,48 @@ def main():
     params = oc.default_params(max_hits_per_event=1024)
     sess = oc.Session(geo, params, device=0)
     dev_in = sess.upload(ev)
-    dev_out = sess.alloc_outputs(ev.n_events, ev.n_hits, tracklets=True)
     torch.cuda.synchronize()
 
-    for i in range(args.warmup):
-        sess.run_device(dev_in, dev_out)
+    def timed(label):
+        out = sess.alloc_outputs(ev.n_events, ev.n_hits, tracklets=True)
+        for i in range(args.warmup):
+            sess.run_device(dev_in, out)
+            torch.cuda.synchronize()
+            log(f"{label} warmup step {i} done: kernel {sess.last_kernel_ms():.1f} ms")
+        kms = []
         torch.cuda.synchronize()
-        log(f"warmup step {i} done: kernel {sess.last_kernel_ms():.1f} ms")
-    kms = []
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        sess.run_device(dev_in, dev_out)
-        kms.append(sess.last_kernel_ms())
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    ms_per_step = (t1 - t0) * 1e3 / max(1, args.steps)
-    kernel_ms = float(np.mean(kms)) if kms else float("nan")
-    log(f"timed {args.steps} steps: {ms_per_step:.2f} ms/step, chain kernel {kernel_ms:.2f} ms")
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            sess.run_device(dev_in, out)
+            kms.append(sess.last_kernel_ms())
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        ms = (t1 - t0) * 1e3 / max(1, args.steps)
+        km = float(np.mean(kms)) if kms else float("nan")
+        log(f"{label}: timed {args.steps} steps: {ms:.2f} ms/step, chain kernel {km:.2f} ms")
+        res = oc.Session.fetch(out, ev.n_events, params.max_tracks_out, params.max_tracklets_per_event)
+        return ms, km, res
 
-    res = oc.Session.fetch(dev_out, ev.n_events, params.max_tracks_out, params.max_tracklets_per_event)
+    plain_ms, plain_kms, plain_res = timed("plain")
+    prune = None
+    if not args.plain:
+        import pruner
+        if not os.path.exists(oc.DEFAULT_PRUNER):
+            log(f"no pruner weights at {oc.DEFAULT_PRUNER}; reporting the plain chain")
+        else:
+            flat, thr, meta = pruner.load()
+            sess.set_pruner(flat, thr)
+            pr_ms, pr_kms, pr_res = timed("pruned")
+            sess.set_pruner(None)
+            rec, found, total = pruner.recall(plain_res, pr_res)
+            extra = int(pr_res.n_tracks.sum()) - found
+            log(f"pruned chain: track recall {rec:.5f} ({found}/{total} plain tracks identical), {extra} other tracks; "
+                f"{plain_ms / pr_ms:.2f}x the plain chain")
+            prune = {"ms": pr_ms, "kms": pr_kms, "res": pr_res, "recall": rec, "found": found, "total": total,
+                     "extra": extra, "threshold": thr}
+    ms_per_step = prune["ms"] if prune else plain_ms
+    kernel_ms = prune["kms"] if prune else plain_kms
+    res = prune["res"] if prune else plain_res
     n_out = int(res.n_tracks.sum())
     rejected = int(((res.status & oc.EV_REJECT_MASK) != 0).sum())
     counts = np.diff(ev.offsets)
@@ -120,7 +148,8 @@ def main():
     log(f"host-buffer path (H2D + chain + D2H): {e2e:.0f} events/s on {n_e2e} events")
 
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", "chain_kernel_1M_poisson300_traffic.json")
+    tfile = os.path.join(ROOT, "profiles", "chain_kernel_1M_poisson300_pruned_traffic.json" if prune else
+                         "chain_kernel_1M_poisson300_traffic.json")
     if os.path.exists(tfile):
         with open(tfile) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
@@ -167,12 +196,20 @@ def main():
         "dtype": "fp32",
         "data": "synthetic",
         "config": {
-            "workload": f"plain chain (no learned pruning), {ev.n_events} events, hits/event Poisson({args.mean_hits:g}) "
-                        f"capped at {args.cap_hits}, 1-2 muon tracks, basic_geometry_e906, reference caps",
-            "chain": "plain",
+            "workload": f"{ev.n_events} events, hits/event Poisson({args.mean_hits:g}) capped at {args.cap_hits}, "
+                        f"1-2 muon tracks, basic_geometry_e906, reference caps",
+            "chain": "learned-pruned (YZ station-3 units, bf16 perceptron on the matrix cores)" if prune else "plain",
             "events": ev.n_events,
             "hits": int(ev.n_hits),
             "kernel_ms": kernel_ms,
+            "plain_events_per_s": ev.n_events / (plain_ms * 1e-3),
+            "plain_ms_per_step": plain_ms,
+            "speedup_vs_plain": (plain_ms / prune["ms"]) if prune else 1.0,
+            "track_recall": prune["recall"] if prune else 1.0,
+            "plain_tracks": prune["total"] if prune else int(plain_res.n_tracks.sum()),
+            "pruned_tracks_identical": prune["found"] if prune else None,
+            "pruned_other_tracks": prune["extra"] if prune else None,
+            "prune_threshold": prune["threshold"] if prune else None,
             "host_buffer_events_per_s": e2e,
         },
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -13,11 +13,13 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <new>
 
 #include "or_device.h"
+#include "or_prune.h"
 
 namespace ordev {
 __constant__ float c_geo[F_COUNT][ND];
@@ -38,7 +40,8 @@ constexpr int NCLASS_MAX = 5;    // event tile classes: 512, 1024, 2048, 4096, 8
 
 // Sub-stage cycle counters of the profiling build (-DOR_STAGE_TIMING); no-ops
 // otherwise.  Indices: YZ window lists, YZ pass 1, 2, 3, YZ winners, YZ unit
-// path, global window lists, global x/uv loop, global write.
+// path, global window lists, global pass A, global pass B and write, YZ unit
+// scoring.
 constexpr int N_SUB_TIMERS = 16;
 #ifdef OR_STAGE_TIMING
 __device__ unsigned long long g_sub_cycles[N_SUB_TIMERS];
@@ -122,6 +125,10 @@ struct Args {
     char* scratch;
     int64_t scratch_stride;
     int yz_list_cap;            // <= YZ_LIST_CAP; chunks with more candidates take the sequential path
+    const PrunerDev* pruner;    // learned YZ unit pruning (NULL: every unit)
+    float* unit_dump;           // optional per-unit feature rows (training aid)
+    int64_t dump_cap;
+    unsigned long long* dump_count;
 };
 
 // scratch layout per block
@@ -137,7 +144,10 @@ constexpr int64_t SCR_XP = ((SCR_YF + SCR_YF_BYTES + 255) / 256) * 256;         
 constexpr int64_t SCR_SO = SCR_XP + ((3 * 28 * BIN_CAP * 2 + 255) / 256) * 256;      // uint16 [MAXTKL] slot order
 constexpr int64_t SCR_PL = ((SCR_SO + MAXTKL * 2 + 255) / 256) * 256;               // uint32 [YZ_LIST_CAP] YZ items
 constexpr int64_t SCR_PB = SCR_PL + (int64_t)YZ_LIST_CAP * 4;                       // uint8 x4 [YZ_LIST_CAP] item state
-constexpr int64_t SCR_STRIDE = ((SCR_PB + (int64_t)YZ_LIST_CAP * 4 + 4095) / 4096) * 4096;
+constexpr int UNIT_CAP = 4096;                                                       // YZ units scored per chunk
+constexpr int64_t SCR_KL = SCR_PB + (int64_t)YZ_LIST_CAP * 4;                       // uint32 [UNIT_CAP] kept units
+constexpr int64_t SCR_KS = SCR_KL + (int64_t)UNIT_CAP * 4;                          // uint32 [UNIT_CAP] their first candidate
+constexpr int64_t SCR_STRIDE = ((SCR_KS + (int64_t)UNIT_CAP * 4 + 4095) / 4096) * 4096;
 
 __device__ __forceinline__ unsigned long long ballot(bool p) { return __ballot(p); }
 __device__ __forceinline__ int lane_id() { return threadIdx.x; }
@@ -626,18 +636,39 @@ __device__ __attribute__((noinline)) bool yz_unit(const Ev<MAXH>& E, Tk* Tp, con
 // each stage, and stage-level branches depend only on wave-uniform values.
 #define STAGE __device__ __attribute__((noinline))
 
-struct Scr {                  // per-wave scratch in HBM
-    Tk* tks;
-    uint32_t* celloff;
-    uint16_t* wl;
-    float4* wy;
-    uint8_t* wf;
-    uint16_t* xpairs;         // [3][28][BIN_CAP], (a | b << 8), 0xFF = none
-    uint16_t* slot_order;     // tracklet index by slot order
-    uint32_t* plist;          // YZ candidates passing the first box cut, in reference order
-    uchar4* pst;              // per candidate: x first-of-unit, y station-3 signs after its left/right
-                              // (first of unit), z sign written to the last station-3 hit (0: none),
-                              // w station-3 signs it starts from
+// Per-wave scratch in HBM: one base pointer (the regions sit at fixed
+// offsets), so the stage functions take it in one register pair.
+struct Scr {
+    char* base;
+    __device__ Tk* tks() const { return reinterpret_cast<Tk*>(base + SCR_TK); }
+    __device__ uint32_t* celloff() const { return reinterpret_cast<uint32_t*>(base + SCR_CELL); }
+    __device__ uint16_t* wl() const { return reinterpret_cast<uint16_t*>(base + SCR_LIST); }
+    __device__ float4* wy() const { return reinterpret_cast<float4*>(base + SCR_YV); }
+    __device__ uint8_t* wf() const { return reinterpret_cast<uint8_t*>(base + SCR_YF); }
+    // [3][28][BIN_CAP] X pairs (a | b << 8), 0xFF = none
+    __device__ uint16_t* xpairs() const { return reinterpret_cast<uint16_t*>(base + SCR_XP); }
+    // tracklet index by slot order
+    __device__ uint16_t* slot_order() const { return reinterpret_cast<uint16_t*>(base + SCR_SO); }
+    // YZ candidates passing the first box cut, in reference order
+    __device__ uint32_t* plist() const { return reinterpret_cast<uint32_t*>(base + SCR_PL); }
+    // per candidate: x first-of-unit, y station-3 signs after its left/right (first
+    // of unit), z sign written to the last station-3 hit (0: none), w station-3
+    // signs it starts from
+    __device__ uchar4* pst() const { return reinterpret_cast<uchar4*>(base + SCR_PB); }
+    // kept YZ units of a chunk (j << 16 | u3.v3 index), in order, and their first
+    // candidate in the kept enumeration
+    __device__ uint32_t* klist() const { return reinterpret_cast<uint32_t*>(base + SCR_KL); }
+    __device__ uint32_t* kstart() const { return reinterpret_cast<uint32_t*>(base + SCR_KS); }
+};
+
+// Learned unit pruning and unit rows for the YZ stage, by value from the
+// launch arguments.
+struct Prn {
+    const PrunerDev* prune;   // NULL: no pruning
+    float* dump;              // NULL: no unit rows
+    int64_t dump_cap;
+    unsigned long long* dump_count;
+    int64_t ev;               // event index (unit rows)
 };
 
 // A + B: load, group by detector, reduce.  Returns true when the keep bits are
@@ -914,8 +945,8 @@ STAGE int stage_xz(or_params P, Scr S) {
     Lds<MAXH>& L = g_lds<MAXH>;
     Ev<MAXH> E;
     const int lane = lane_id();
-    Tk* tks = S.tks;
-    uint32_t* celloff = S.celloff;
+    Tk* tks = S.tks();
+    uint32_t* celloff = S.celloff();
     int ntkl = 0;
     uint32_t run = 0;
         for (int c0 = 0; c0 < 1568; c0 += WAVE) {
@@ -956,8 +987,8 @@ STAGE int stage_xz(or_params P, Scr S) {
                 const int ix = (int)(g - celloff[c]);
                 const int ix2 = ix % nx2, ix3 = ix / nx2;
                 par = t & 1;
-                q2 = S.xpairs[b2 * BIN_CAP + ix2];
-                q3 = S.xpairs[(s3 * 28 + b3) * BIN_CAP + ix3];
+                q2 = S.xpairs()[b2 * BIN_CAP + ix2];
+                q3 = S.xpairs()[(s3 * 28 + b3) * BIN_CAP + ix3];
                 acc = xz_candidate(E, s3, q2, q3, nullptr);
             }
             const unsigned long long m = ballot(acc);
@@ -997,7 +1028,7 @@ STAGE int stage_xz(or_params P, Scr S) {
         }
         __syncthreads();
         if (L.status == 0)
-            for (int k = lane; k < ntkl; k += WAVE) S.slot_order[L.sstart[tks[k].slot] + tks[k].pos_in_slot] = (uint16_t)k;
+            for (int k = lane; k < ntkl; k += WAVE) S.slot_order()[L.sstart[tks[k].slot] + tks[k].pos_in_slot] = (uint16_t)k;
         __syncthreads();
         return ntkl;
 }
@@ -1021,9 +1052,9 @@ STAGE int stage_xz(or_params P, Scr S) {
 template <int MAXH>
 __device__ __forceinline__ void yz_lists_of(const Scr& S, int j, YzLists& W) {
     for (int v = 0; v < 4; ++v) {
-        W.l[v] = S.wl + (j * 4 + v) * WIN_CAP;
-        W.y[v] = S.wy + (j * 4 + v) * WIN_CAP;
-        W.f[v] = S.wf + (j * 4 + v) * WIN_CAP;
+        W.l[v] = S.wl() + (j * 4 + v) * WIN_CAP;
+        W.y[v] = S.wy() + (j * 4 + v) * WIN_CAP;
+        W.f[v] = S.wf() + (j * 4 + v) * WIN_CAP;
         W.n[v] = g_lds<MAXH>.nl[j][v];
     }
 }
@@ -1104,14 +1135,216 @@ __device__ __forceinline__ void yz_candidate(const Ev<MAXH>& E, const Scr& S, co
     yz_fill<6>(E, W, 1, it.iv2, H);
 }
 
+// Features of YZ unit (tracklet j, u3.v3 index iuv3) for the pruner: the
+// tracklet's x line and the y of the first and last accepted hit of the u3 and
+// v3 entries (window lists hold accepted entries only).
+__device__ __forceinline__ UnitFeatures unit_features(const Scr& S, const Tk& T, int j, int iuv3, int nu3) {
+    const int iu3 = iuv3 % nu3, iv3 = iuv3 / nu3;
+    const int qu = (j * 4 + 2) * WIN_CAP + iu3, qv = (j * 4 + 3) * WIN_CAP + iv3;
+    const float4 yu = S.wy()[qu], yw = S.wy()[qv];
+    const unsigned fu = S.wf()[qu], fv = S.wf()[qv];
+    const float ya = (fu & 1u) ? yu.x : yu.z, eya = (fu & 1u) ? yu.y : yu.w;
+    const float yb = (fu & 2u) ? yu.z : yu.x, eyb = (fu & 2u) ? yu.w : yu.y;
+    const float yc = (fv & 1u) ? yw.x : yw.z, eyc = (fv & 1u) ? yw.y : yw.w;
+    const float yd = (fv & 2u) ? yw.z : yw.x, eyd = (fv & 2u) ? yw.w : yw.y;
+    const float du = 0.5f * (ya + yb) - 0.5f * (yc + yd);
+    UnitFeatures f;
+    f.v[0] = T.x0 * 0.01f; f.v[1] = T.tx * 10.f; f.v[2] = (float)(T.st3 - 3);
+    f.v[3] = (float)(__popc(fu) - 1); f.v[4] = (float)(__popc(fv) - 1);
+    f.v[5] = ya * 0.02f; f.v[6] = yb * 0.02f; f.v[7] = yc * 0.02f; f.v[8] = yd * 0.02f;
+    f.v[9] = eya; f.v[10] = eyb; f.v[11] = eyc; f.v[12] = eyd;
+    f.v[13] = du * 0.1f; f.v[14] = fabsf(du) * 0.1f; f.v[15] = (ya - yb) * 0.1f;
+    return f;
+}
+
+// plane z of the YZ views: [u2, v2, u3, v3][station 3 is D3m][hit a, b]
+__device__ __forceinline__ void yz_plane_z(float (&zv)[4][2][2]) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+#pragma unroll
+        for (int m = 0; m < 2; ++m) {
+            const int sid = v < 2 ? super_id(2, v + 1) : super_id(3 + m, v - 1);
+            zv[v][m][0] = G(F_Z, sid * 2);
+            zv[v][m][1] = G(F_Z, sid * 2 - 1);
+        }
+}
+
+// first fit and box cut of YZ candidate (tracklet j of the chunk, list entries
+// e = u2, v2, u3, v3; m: station 3 is D3m), hits in the reference order u3,
+// v3, u2, v2, a then b
+__device__ __forceinline__ bool yz_first_cut(const Scr& S, int j, const int (&e)[4], unsigned m,
+                                             const float (&zv)[4][2][2]) {
+    uint8_t fl[4];
+    float4 yv[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const int q = (j * 4 + v) * WIN_CAP + e[v];
+        fl[v] = S.wf()[q];
+        yv[v] = S.wy()[q];
+    }
+    float A0 = 0.f, A1 = 0.f, A3 = 0.f, B0 = 0.f, B1 = 0.f;
+    bool ok = true;
+#pragma unroll
+    for (int o = 0; o < 4; ++o) {
+        const int v = o < 2 ? o + 2 : o - 2;
+        const float za = m ? zv[v][1][0] : zv[v][0][0], zb = m ? zv[v][1][1] : zv[v][0][1];
+        ok = ok && fl[v] != 0;
+        if (fl[v] & 1) {
+            B0 += yv[v].y * yv[v].x; B1 += (yv[v].y * yv[v].x) * za; A0 += yv[v].y; A1 += yv[v].y * za;
+            A3 += (yv[v].y * za) * za;
+        }
+        if (fl[v] & 2) {
+            B0 += yv[v].w * yv[v].z; B1 += (yv[v].w * yv[v].z) * zb; A0 += yv[v].w; A1 += yv[v].w * zb;
+            A3 += (yv[v].w * zb) * zb;
+        }
+    }
+    return ok && yz_box1(fit_finish(A0, A1, A3, B0, B1));
+}
+
+// Pass 1 by units, for learned pruning and unit rows: every (u3, v3) unit of
+// the chunk is scored (and its row written), the kept units are listed in
+// order with the start of their candidates, and the candidates of the kept
+// units go through the first fit.  Survivors keep their index in the full
+// enumeration, so the later passes see the plain chain's candidates with the
+// pruned units taken out.  np: survivors, -1 when they overflow the list, -2
+// when the chunk has more than UNIT_CAP units (then plain pass 1 runs).
+struct UnitPass {
+    int np;
+    int64_t dbase;   // first unit row of the chunk, -1: none written
+};
+
+template <int MAXH>
+__device__ __attribute__((noinline)) UnitPass yz_pass1_units(Scr S, Prn R, int kb, int ntkl, int cap) {
+    Lds<MAXH>& L = g_lds<MAXH>;
+    const int lane = lane_id();
+    Tk* tks = S.tks();
+    UnitPass res = {-2, -1};
+    OR_SUB_START
+    int uo[YZ_CHUNK + 1];
+    {
+        int units = 0;
+        if (lane < YZ_CHUNK && kb + lane < ntkl) {
+            const int* nl = L.nl[lane];
+            if (nl[0] > 0 && nl[1] > 0 && nl[2] > 0 && nl[3] > 0) units = nl[2] * nl[3];
+        }
+        const int inc = wave_scan(units);
+        uo[0] = 0;
+#pragma unroll
+        for (int t = 1; t <= YZ_CHUNK; ++t) uo[t] = __builtin_amdgcn_readfirstlane(__shfl(inc, t - 1, WAVE));
+    }
+    const int U = uo[YZ_CHUNK];
+    if (U > UNIT_CAP) return res;
+    if (R.dump) {
+        unsigned long long b = 0;
+        if (lane == 0) b = atomicAdd(R.dump_count, (unsigned long long)U);
+        const unsigned long long b0 = (unsigned)__shfl((int)(uint32_t)b, 0, WAVE);
+        const unsigned long long b1 = (unsigned)__shfl((int)(uint32_t)(b >> 32), 0, WAVE);
+        const unsigned long long bb = b0 | (b1 << 32);
+        if (bb + (unsigned long long)U <= (unsigned long long)R.dump_cap) res.dbase = (int64_t)bb;
+    }
+    // score every unit; the kept ones, in order, with the start of their
+    // candidates in the kept enumeration
+    int nk = 0;
+    uint32_t run = 0;
+    for (int u0 = 0; u0 < U; u0 += WAVE) {
+        const int u = u0 + lane;
+        const bool valid = u < U;
+        int j = 0;
+#pragma unroll
+        for (int t = 1; t < YZ_CHUNK; ++t) j += u >= uo[t] ? 1 : 0;
+        int ujs = uo[0];
+#pragma unroll
+        for (int t = 1; t < YZ_CHUNK; ++t) ujs = j == t ? uo[t] : ujs;
+        const int iuv3 = u - ujs;
+        UnitFeatures f;
+#pragma unroll
+        for (int q = 0; q < PRUNE_NF; ++q) f.v[q] = 0.f;
+        int nin = 0;
+        if (valid) {
+            const int4 nl = *reinterpret_cast<const int4*>(L.nl[j]);
+            f = unit_features(S, tks[kb + j], j, iuv3, nl.z);
+            nin = nl.x * nl.y;
+        }
+        bool keep = valid;
+        if (R.prune) {
+            const float score = prune_score(R.prune, f);
+            keep = valid && score >= R.prune->thr;
+        }
+        if (res.dbase >= 0 && valid) {
+            float* row = R.dump + (res.dbase + u) * OR_UNIT_ROW;
+            row[0] = (float)R.ev;
+            row[1] = (float)tks[kb + j].canon;
+            row[2] = (float)iuv3;
+            row[3] = 0.f;
+#pragma unroll
+            for (int q = 0; q < PRUNE_NF; ++q) row[4 + q] = f.v[q];
+        }
+        const unsigned long long km = ballot(keep);
+        const int c = keep ? nin : 0;
+        const int inc = wave_scan(c);
+        if (keep) {
+            const int r = nk + rank_in(km);
+            S.klist()[r] = ((uint32_t)j << 16) | (uint32_t)iuv3;
+            S.kstart()[r] = run + (uint32_t)(inc - c);
+        }
+        nk += __popcll(km);
+        run += (uint32_t)__shfl(inc, WAVE - 1, WAVE);
+    }
+    __syncthreads();
+    OR_SUB(9);
+    int off[YZ_CHUNK + 1];
+#pragma unroll
+    for (int t = 0; t <= YZ_CHUNK; ++t) off[t] = __builtin_amdgcn_readfirstlane(L.uoff[t]);
+    const unsigned d3m = (unsigned)ballot(lane < YZ_CHUNK && kb + lane < ntkl && tks[kb + lane].st3 == 4);
+    float zv[4][2][2];
+    yz_plane_z(zv);
+    int np = 0;
+    for (uint32_t i0 = 0; i0 < run; i0 += WAVE) {
+        const uint32_t i = i0 + (uint32_t)lane;
+        bool pass = false;
+        int iorig = 0;
+        if (i < run) {
+            int lo = 0, hi = nk;   // last kept unit starting at or before i
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (S.kstart()[mid] <= i) lo = mid;
+                else hi = mid;
+            }
+            const uint32_t code = S.klist()[lo];
+            const int j = (int)(code >> 16), iuv3 = (int)(code & 0xFFFFu);
+            const int inner = (int)(i - S.kstart()[lo]);
+            const int4 nl = *reinterpret_cast<const int4*>(L.nl[j]);
+            int ojs = off[0];
+#pragma unroll
+            for (int t = 1; t < YZ_CHUNK; ++t) ojs = j == t ? off[t] : ojs;
+            const int e[4] = {inner % nl.x, inner / nl.x, iuv3 % nl.z, iuv3 / nl.z};
+            iorig = ojs + iuv3 * (nl.x * nl.y) + inner;
+            pass = yz_first_cut(S, j, e, (d3m >> j) & 1, zv);
+        }
+        const unsigned long long m = ballot(pass);
+        const int c = __popcll(m);
+        if (np + c > cap) {
+            // the chunk goes the sequential way: its unit rows carry no winner flags
+            if (res.dbase >= 0)
+                for (int u = lane; u < U; u += WAVE) R.dump[(res.dbase + u) * OR_UNIT_ROW + 3] = -1.f;
+            res.np = -1;
+            return res;
+        }
+        if (pass) S.plist()[np + rank_in(m)] = (uint32_t)iorig;
+        np += c;
+    }
+    res.np = np;
+    return res;
+}
+
 // Two-pass form.  Returns false (nothing written) when the survivors of the
 // first box cut do not fit the list.
 template <int MAXH>
-__device__ __attribute__((noinline)) bool yz_chunk_par(Scr S, int kb, int ntkl, int cap) {
+__device__ __attribute__((noinline)) bool yz_chunk_par(Scr S, Prn R, int kb, int ntkl, int cap) {
     Lds<MAXH>& L = g_lds<MAXH>;
     Ev<MAXH> E;
     const int lane = lane_id();
-    Tk* tks = S.tks;
+    Tk* tks = S.tks();
     {
         const int j = lane, k = kb + j;
         int items = 0;
@@ -1129,71 +1362,48 @@ __device__ __attribute__((noinline)) bool yz_chunk_par(Scr S, int kb, int ntkl, 
     __syncthreads();
     const int I = L.uoff[YZ_CHUNK];
     OR_SUB_START
-    // pass 1: first fit and box cut of every candidate; survivors in order.
-    // Chunk constants go to scalar registers (candidate offsets, which
-    // tracklets are D3m, the plane z values of both station-3 choices) and the
-    // candidate's four list entries are loaded together.
-    int off[YZ_CHUNK + 1];
-#pragma unroll
-    for (int t = 0; t <= YZ_CHUNK; ++t) off[t] = __builtin_amdgcn_readfirstlane(L.uoff[t]);
-    const unsigned d3m = (unsigned)ballot(lane < YZ_CHUNK && kb + lane < ntkl && tks[kb + lane].st3 == 4);
-    float zv[4][2][2];   // [view u2, v2, u3, v3][station 3 is D3m][hit a, b]
-#pragma unroll
-    for (int v = 0; v < 4; ++v)
-#pragma unroll
-        for (int m = 0; m < 2; ++m) {
-            const int sid = v < 2 ? super_id(2, v + 1) : super_id(3 + m, v - 1);
-            zv[v][m][0] = G(F_Z, sid * 2);
-            zv[v][m][1] = G(F_Z, sid * 2 - 1);
-        }
     int np = 0;
-    for (int i0 = 0; i0 < I; i0 += WAVE) {
-        const int i = i0 + lane;
-        bool pass = false;
-        if (i < I) {
-            int j = 0;
+    int64_t dbase = -1;   // unit rows of this chunk (pruning path with the dump on)
+    if (R.prune || R.dump) {
+        const UnitPass up = yz_pass1_units<MAXH>(S, R, kb, ntkl, cap);
+        if (up.np == -1) return false;
+        np = up.np;
+        dbase = up.dbase;
+    }
+    if (!(R.prune || R.dump) || np == -2) {
+        np = 0;
+        // pass 1: first fit and box cut of every candidate; survivors in order.
+        // Chunk constants go to scalar registers (candidate offsets, which
+        // tracklets are D3m, the plane z values of both station-3 choices) and the
+        // candidate's four list entries are loaded together.
+        int off[YZ_CHUNK + 1];
 #pragma unroll
-            for (int t = 1; t < YZ_CHUNK; ++t) j += i >= off[t] ? 1 : 0;
-            const int4 nl = *reinterpret_cast<const int4*>(L.nl[j]);
-            int ojs = off[0];
+        for (int t = 0; t <= YZ_CHUNK; ++t) off[t] = __builtin_amdgcn_readfirstlane(L.uoff[t]);
+        const unsigned d3m = (unsigned)ballot(lane < YZ_CHUNK && kb + lane < ntkl && tks[kb + lane].st3 == 4);
+        float zv[4][2][2];
+        yz_plane_z(zv);
+        for (int i0 = 0; i0 < I; i0 += WAVE) {
+            const int i = i0 + lane;
+            bool pass = false;
+            if (i < I) {
+                int j = 0;
 #pragma unroll
-            for (int t = 1; t < YZ_CHUNK; ++t) ojs = j == t ? off[t] : ojs;
-            const int local = i - ojs, nin = nl.x * nl.y;
-            const int iuv3 = local / nin, iuv2 = local - iuv3 * nin;
-            const int e[4] = {iuv2 % nl.x, iuv2 / nl.x, iuv3 % nl.z, iuv3 / nl.z};  // u2, v2, u3, v3
-            const int m = (d3m >> j) & 1;
-            uint8_t fl[4];
-            float4 yv[4];
+                for (int t = 1; t < YZ_CHUNK; ++t) j += i >= off[t] ? 1 : 0;
+                const int4 nl = *reinterpret_cast<const int4*>(L.nl[j]);
+                int ojs = off[0];
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int q = (j * 4 + v) * WIN_CAP + e[v];
-                fl[v] = S.wf[q];
-                yv[v] = S.wy[q];
+                for (int t = 1; t < YZ_CHUNK; ++t) ojs = j == t ? off[t] : ojs;
+                const int local = i - ojs, nin = nl.x * nl.y;
+                const int iuv3 = local / nin, iuv2 = local - iuv3 * nin;
+                const int e[4] = {iuv2 % nl.x, iuv2 / nl.x, iuv3 % nl.z, iuv3 / nl.z};  // u2, v2, u3, v3
+                pass = yz_first_cut(S, j, e, (d3m >> j) & 1, zv);
             }
-            float A0 = 0.f, A1 = 0.f, A3 = 0.f, B0 = 0.f, B1 = 0.f;
-            bool ok = true;
-            // reference order u3, v3, u2, v2; hit a then b
-#pragma unroll
-            for (int o = 0; o < 4; ++o) {
-                const int v = o < 2 ? o + 2 : o - 2;
-                const float za = m ? zv[v][1][0] : zv[v][0][0], zb = m ? zv[v][1][1] : zv[v][0][1];
-                ok = ok && fl[v] != 0;
-                if (fl[v] & 1) {
-                    B0 += yv[v].y * yv[v].x; B1 += (yv[v].y * yv[v].x) * za; A0 += yv[v].y; A1 += yv[v].y * za;
-                    A3 += (yv[v].y * za) * za;
-                }
-                if (fl[v] & 2) {
-                    B0 += yv[v].w * yv[v].z; B1 += (yv[v].w * yv[v].z) * zb; A0 += yv[v].w; A1 += yv[v].w * zb;
-                    A3 += (yv[v].w * zb) * zb;
-                }
-            }
-            if (ok) pass = yz_box1(fit_finish(A0, A1, A3, B0, B1));
+            const unsigned long long m = ballot(pass);
+            const int c = __popcll(m);
+            if (np + c > cap) return false;
+            if (pass) S.plist()[np + rank_in(m)] = (uint32_t)i;
+            np += c;
         }
-        const unsigned long long m = ballot(pass);
-        const int c = __popcll(m);
-        if (np + c > cap) return false;
-        if (pass) S.plist[np + rank_in(m)] = (uint32_t)i;
-        np += c;
     }
     __syncthreads();
     OR_SUB(1);
@@ -1204,11 +1414,11 @@ __device__ __attribute__((noinline)) bool yz_chunk_par(Scr S, int kb, int ntkl, 
     for (int k0 = 0; k0 < np; k0 += WAVE) {
         const int k = k0 + lane;
         if (k < np) {
-            const YzItem it = yz_decode<MAXH>((int)S.plist[k]);
+            const YzItem it = yz_decode<MAXH>((int)S.plist()[k]);
             const int nin = L.nl[it.j][0] * L.nl[it.j][1];
             bool first = k == 0;
             if (!first) {
-                const YzItem pt = yz_decode<MAXH>((int)S.plist[k - 1]);
+                const YzItem pt = yz_decode<MAXH>((int)S.plist()[k - 1]);
                 first = pt.j != it.j || (pt.local / nin) != (it.local / nin);
             }
             const Tk* Tp = &tks[kb + it.j];
@@ -1226,7 +1436,7 @@ __device__ __attribute__((noinline)) bool yz_chunk_par(Scr S, int kb, int ntkl, 
                                              H.det[4], H.pos[3], H.pos[4], H.drift[3], H.drift[4], 150.f);
                 if (h >= 0) ps.z = (uint8_t)(lr_si(h) & 3);
             }
-            S.pst[k] = ps;
+            S.pst()[k] = ps;
         }
     }
     __syncthreads();
@@ -1238,7 +1448,7 @@ __device__ __attribute__((noinline)) bool yz_chunk_par(Scr S, int kb, int ntkl, 
         const int k = k0 + lane;
         const bool valid = k < np;
         uchar4 ps = make_uchar4(0, 0, 0, 0);
-        if (valid) ps = S.pst[k];
+        if (valid) ps = S.pst()[k];
         int fi = wave_max_scan(valid && ps.x ? k : -1);
         fi = max(fi, carry_f);
         const int wincl = wave_max_scan(valid && (ps.x || ps.z) ? k : -1);
@@ -1248,7 +1458,7 @@ __device__ __attribute__((noinline)) bool yz_chunk_par(Scr S, int kb, int ntkl, 
         carry_f = __shfl(fi, WAVE - 1, WAVE);
         carry_w = max(carry_w, __shfl(wincl, WAVE - 1, WAVE));
         if (valid) {
-            const YzItem it = yz_decode<MAXH>((int)S.plist[k]);
+            const YzItem it = yz_decode<MAXH>((int)S.plist()[k]);
             Tk* Tp = &tks[kb + it.j];
             YzSlots H;
             Fit f;
@@ -1257,11 +1467,11 @@ __device__ __attribute__((noinline)) bool yz_chunk_par(Scr S, int kb, int ntkl, 
             if (!ps.x) {
                 // seed of the unit with the v3' sign (slot 3) of the last writer; the
                 // first survivor of the unit is a writer, so wexcl >= fi
-                st = (uint8_t)((S.pst[fi].y & 0x3Fu) | ((unsigned)S.pst[wexcl].z << 6));
+                st = (uint8_t)((S.pst()[fi].y & 0x3Fu) | ((unsigned)S.pst()[wexcl].z << 6));
                 yz_unpack3(H, st);
             }
             ps.w = st;
-            S.pst[k] = ps;
+            S.pst()[k] = ps;
             float chi2;
             // the reference starts its minimum at 10000.1 (reconstruction_kernels.cuh:1240)
             if (yz_tail(E, Tp, H, f, chi2, nullptr) && chi2 < 10000.1f) {
@@ -1274,6 +1484,15 @@ __device__ __attribute__((noinline)) bool yz_chunk_par(Scr S, int kb, int ntkl, 
     __syncthreads();
     OR_SUB(3);
     // winners: replay the candidate with its starting signs and write it
+    int uo_lane = 0;   // first unit row of tracklet j = lane
+    if (dbase >= 0) {
+        int units = 0;
+        if (lane < YZ_CHUNK && kb + lane < ntkl) {
+            const int* nl = L.nl[lane];
+            if (nl[0] > 0 && nl[1] > 0 && nl[2] > 0 && nl[3] > 0) units = nl[2] * nl[3];
+        }
+        uo_lane = wave_scan(units) - units;
+    }
     if (lane < YZ_CHUNK && kb + lane < ntkl) {
         const int j = lane;
         Tk* Tp = &tks[kb + j];
@@ -1283,16 +1502,17 @@ __device__ __attribute__((noinline)) bool yz_chunk_par(Scr S, int kb, int ntkl, 
             int lo = 0, hi = np - 1;
             while (lo < hi) {
                 const int mid = (lo + hi) >> 1;
-                if ((int)S.plist[mid] < i) lo = mid + 1;
+                if ((int)S.plist()[mid] < i) lo = mid + 1;
                 else hi = mid;
             }
             const YzItem it = yz_decode<MAXH>(i);
             YzSlots H;
             Fit f;
             yz_candidate(E, S, Tp, it, H, f);
-            if (!S.pst[lo].x) yz_unpack3(H, S.pst[lo].w);
+            if (!S.pst()[lo].x) yz_unpack3(H, S.pst()[lo].w);
             float chi2;
             yz_tail(E, Tp, H, f, chi2, Tp);
+            if (dbase >= 0) R.dump[(dbase + uo_lane + it.local / (L.nl[j][0] * L.nl[j][1])) * OR_UNIT_ROW + 3] = 1.f;
         } else {
             Tp->stid = 3;
         }
@@ -1307,10 +1527,10 @@ __device__ __attribute__((noinline)) void yz_chunk_seq(Scr S, int kb, int ntkl) 
     Lds<MAXH>& L = g_lds<MAXH>;
     Ev<MAXH> E;
     const int lane = lane_id();
-    Tk* tks = S.tks;
-    uint16_t* wl = S.wl;
-    float4* wy = S.wy;
-    uint8_t* wf = S.wf;
+    Tk* tks = S.tks();
+    uint16_t* wl = S.wl();
+    float4* wy = S.wy();
+    uint8_t* wf = S.wf();
     {
         const int j = lane, k = kb + j;
         int u = 0;
@@ -1370,14 +1590,14 @@ __device__ __attribute__((noinline)) void yz_chunk_seq(Scr S, int kb, int ntkl) 
 }
 
 template <int MAXH>
-STAGE void stage_yz(or_params P, Scr S, int ntkl, int list_cap) {
+STAGE void stage_yz(or_params P, Scr S, Prn R, int ntkl, int list_cap) {
     Lds<MAXH>& L = g_lds<MAXH>;
     Ev<MAXH> E;
     const int lane = lane_id();
-    Tk* tks = S.tks;
-    uint16_t* wl = S.wl;
-    float4* wy = S.wy;
-    uint8_t* wf = S.wf;
+    Tk* tks = S.tks();
+    uint16_t* wl = S.wl();
+    float4* wy = S.wy();
+    uint8_t* wf = S.wf();
     const int capw = P.max_pairs_window < WIN_CAP ? P.max_pairs_window : WIN_CAP;
     for (int kb = 0; kb < ntkl; kb += YZ_CHUNK) {
         OR_SUB_START
@@ -1428,7 +1648,7 @@ STAGE void stage_yz(or_params P, Scr S, int ntkl, int list_cap) {
         __syncthreads();
         OR_SUB(0);
         if (L.status) break;
-        if (!yz_chunk_par<MAXH>(S, kb, ntkl, list_cap)) {
+        if (!yz_chunk_par<MAXH>(S, R, kb, ntkl, list_cap)) {
             __syncthreads();
             OR_SUB_START
             yz_chunk_seq<MAXH>(S, kb, ntkl);
@@ -1751,17 +1971,17 @@ __device__ __forceinline__ bool global_yfit(const Ev<MAXH>& E, const uint16_t* l
 constexpr int GLOBAL_LIST_CAP = 2048;   // pass-A survivors held before pass B runs
 
 // One track: pass A over its x pairs, pass B over the listed survivors, then
-// the best candidate is written.  Uses the x-part records (S.wy + 8), the
-// survivor list (S.plist) and the best-candidate record (S.wy).
+// the best candidate is written.  Uses the x-part records (S.wy() + 8), the
+// survivor list (S.plist()) and the best-candidate record (S.wy()).
 template <int MAXH>
 __device__ __attribute__((noinline)) void global_track(Scr S, Tk* T, const uint16_t* lx, const uint16_t* lu,
                                                        const uint16_t* lv, int nx1, int nu1, int nv1) {
     Ev<MAXH> E;
     const int lane = lane_id();
     constexpr int det6[6] = {4, 3, 2, 1, 6, 5};
-    GBest* best = reinterpret_cast<GBest*>(S.wy);
-    GX* xrec = reinterpret_cast<GX*>(S.wy + 8);
-    uint32_t* list = S.plist;
+    GBest* best = reinterpret_cast<GBest*>(S.wy());
+    GX* xrec = reinterpret_cast<GX*>(S.wy() + 8);
+    uint32_t* list = S.plist();
     const int n23 = T->nx + T->nuv;
     const float x0 = T->x0, tx = T->tx, ex0 = T->ex0, etx = T->etx;
     const float einvp = (etx - etx) / PT_KICK_KMAG;
@@ -1941,7 +2161,7 @@ STAGE void stage_global(or_params P, Scr S, int ntkl) {
     Lds<MAXH>& L = g_lds<MAXH>;
     Ev<MAXH> E;
     const int lane = lane_id();
-    Tk* tks = S.tks;
+    Tk* tks = S.tks();
     int* nlf = &L.nl[0][0];
     const int capw = P.max_pairs_window < WIN_CAP ? P.max_pairs_window : WIN_CAP;
     for (int k0 = 0; k0 < ntkl; k0 += WAVE) {
@@ -1954,7 +2174,7 @@ STAGE void stage_global(or_params P, Scr S, int ntkl) {
                 for (int t = 0; t < b && mm; ++t) mm &= mm - 1;
                 if (b < GLOBAL_BATCH && mm) {
                     const int k = k0 + __ffsll((long long)mm) - 1;
-                    const int nl = global_window(E, &tks[k], v, S.wl + lane * WIN_CAP, P.max_pair_plane_hits, capw);
+                    const int nl = global_window(E, &tks[k], v, S.wl() + lane * WIN_CAP, P.max_pair_plane_hits, capw);
                     if (nl < 0) atomicOr(&L.status, (uint32_t)OR_EV_PAIR_OVERFLOW);
                     nlf[lane] = nl;
                 }
@@ -1967,8 +2187,8 @@ STAGE void stage_global(or_params P, Scr S, int ntkl) {
                 rest &= rest - 1;
                 const int nx1 = nlf[3 * b], nu1 = nlf[3 * b + 1], nv1 = nlf[3 * b + 2];
                 if (nx1 == 0 || nu1 == 0 || nv1 == 0) continue;
-                global_track<MAXH>(S, &tks[k], S.wl + (3 * b) * WIN_CAP, S.wl + (3 * b + 1) * WIN_CAP,
-                                   S.wl + (3 * b + 2) * WIN_CAP, nx1, nu1, nv1);
+                global_track<MAXH>(S, &tks[k], S.wl() + (3 * b) * WIN_CAP, S.wl() + (3 * b + 1) * WIN_CAP,
+                                   S.wl() + (3 * b + 2) * WIN_CAP, nx1, nu1, nv1);
             }
         }
     }
@@ -2161,14 +2381,14 @@ STAGE void stage_out(const Args& a, Scr S, int64_t e, int ntkl) {
     Lds<MAXH>& L = g_lds<MAXH>;
     const int lane = lane_id();
     const or_params& P = a.p;
-    Tk* tks = S.tks;
+    Tk* tks = S.tks();
     const uint32_t st = L.status;
     const bool rejected = (st & OR_EV_REJECT_MASK) != 0;
     int nout = 0;
     if (!rejected) {
         for (int j0 = 0; j0 < ntkl; j0 += WAVE) {
             const int j = j0 + lane;
-            const Tk* Tp = j < ntkl ? &tks[S.slot_order[j]] : nullptr;
+            const Tk* Tp = j < ntkl ? &tks[S.slot_order()[j]] : nullptr;
             const bool is6 = Tp && Tp->stid >= (P.full_chain ? 7 : 6);
             const unsigned long long m = ballot(is6);
             if (is6) {
@@ -2211,7 +2431,7 @@ STAGE void stage_out(const Args& a, Scr S, int64_t e, int ntkl) {
             const int j = (int)(w / OR_TRACK_PARAMS), f = (int)(w % OR_TRACK_PARAMS);
             float v = 0.f;
             if (j < nrec) {
-                const Tk& T = tks[S.slot_order[j]];
+                const Tk& T = tks[S.slot_order()[j]];
                 const int nh = T.nx + T.nuv + T.n1;
                 switch (f) {
                     case 0: v = (float)(T.stid == 0 ? 3 : T.stid); break;
@@ -2274,15 +2494,7 @@ __global__ __launch_bounds__(WAVE, MAXH <= 512 ? OR_WAVES_PER_SIMD_512 : (MAXH <
     const int lane = lane_id();
     char* scr = a.scratch + (int64_t)blockIdx.x * a.scratch_stride;
     Scr S;
-    S.tks = reinterpret_cast<Tk*>(scr + SCR_TK);
-    S.celloff = reinterpret_cast<uint32_t*>(scr + SCR_CELL);
-    S.wl = reinterpret_cast<uint16_t*>(scr + SCR_LIST);
-    S.wy = reinterpret_cast<float4*>(scr + SCR_YV);
-    S.wf = reinterpret_cast<uint8_t*>(scr + SCR_YF);
-    S.xpairs = reinterpret_cast<uint16_t*>(scr + SCR_XP);
-    S.slot_order = reinterpret_cast<uint16_t*>(scr + SCR_SO);
-    S.plist = reinterpret_cast<uint32_t*>(scr + SCR_PL);
-    S.pst = reinterpret_cast<uchar4*>(scr + SCR_PB);
+    S.base = scr;
     const int64_t ne = a.ev_list ? (int64_t)*a.ev_count : a.n_events;
 #ifdef OR_STAGE_TIMING
     unsigned long long tacc[N_STAGE_TIMERS] = {0, 0, 0, 0, 0, 0, 0};
@@ -2302,17 +2514,20 @@ __global__ __launch_bounds__(WAVE, MAXH <= 512 ? OR_WAVES_PER_SIMD_512 : (MAXH <
         __syncthreads();
         OR_TIMER(0)
         int ntkl = 0;
-        if (L.status == 0) stage_xpairs<MAXH>(a.p, S.xpairs);
+        if (L.status == 0) stage_xpairs<MAXH>(a.p, S.xpairs());
         OR_TIMER(1)
         if (L.status == 0) ntkl = stage_xz<MAXH>(a.p, S);
         OR_TIMER(2)
-        if (L.status == 0) stage_yz<MAXH>(a.p, S, ntkl, a.yz_list_cap);
+        if (L.status == 0) {
+            const Prn R = {a.pruner, a.unit_dump, a.dump_cap, a.dump_count, e};
+            stage_yz<MAXH>(a.p, S, R, ntkl, a.yz_list_cap);
+        }
         OR_TIMER(3)
-        if (L.status == 0) stage_clean<MAXH>(S.tks, S.slot_order, ntkl);
+        if (L.status == 0) stage_clean<MAXH>(S.tks(), S.slot_order(), ntkl);
         OR_TIMER(4)
         if (L.status == 0) stage_global<MAXH>(a.p, S, ntkl);
-        if (L.status == 0 && a.p.full_chain) stage_postglobal<MAXH>(S.tks, S.slot_order, ntkl);
-        if (L.status == 0 && (a.vertex || a.p.full_chain)) stage_vertex<MAXH>(S.tks, ntkl);
+        if (L.status == 0 && a.p.full_chain) stage_postglobal<MAXH>(S.tks(), S.slot_order(), ntkl);
+        if (L.status == 0 && (a.vertex || a.p.full_chain)) stage_vertex<MAXH>(S.tks(), ntkl);
         OR_TIMER(5)
         stage_out<MAXH>(a, S, e, ntkl);
         __syncthreads();
@@ -2370,6 +2585,16 @@ __global__ void zero_u32(uint32_t* p, int64_t n) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = 0;
 }
 
+// one wave per 64 feature rows
+__global__ void prune_score_kernel(const PrunerDev* W, const float* feats, int64_t n, float* scores) {
+    const int64_t r = (int64_t)blockIdx.x * WAVE + threadIdx.x;
+    UnitFeatures f;
+#pragma unroll
+    for (int q = 0; q < PRUNE_NF; ++q) f.v[q] = r < n ? feats[r * PRUNE_NF + q] : 0.f;
+    const float sc = prune_score(W, f);
+    if (r < n) scores[r] = sc;
+}
+
 }  // namespace
 
 // ============================================================================
@@ -2397,9 +2622,14 @@ struct or_session {
     int grid_pct = 100;          // persistent grid, percent of the occupancy limit
     bool use_graph = false;      // replay the launch sequence as a captured hipGraph
     hipGraphExec_t graph_exec = nullptr;
+    PrunerDev* d_pruner = nullptr;   // packed pruner weights (first or_session_set_pruner)
+    bool prune_on = false;
+    float* dump = nullptr;           // unit rows (training aid)
+    int64_t dump_cap = 0;
+    unsigned long long* dump_count = nullptr;
     struct GraphKey {
-        const void* p[14];
-        int64_t n_events, n_hits;
+        const void* p[17];
+        int64_t n_events, n_hits, dump_cap;
         hipStream_t stream;
         int yz_list_cap, grid_pct;
         bool operator==(const GraphKey& o) const { return std::memcmp(this, &o, sizeof o) == 0; }
@@ -2551,6 +2781,10 @@ static or_status issue_chain(or_session* s, const or_hits* in, const or_result* 
     a.scratch = s->d_scratch;
     a.scratch_stride = SCR_STRIDE;
     a.yz_list_cap = s->yz_list_cap;
+    a.pruner = s->prune_on ? s->d_pruner : nullptr;
+    a.unit_dump = s->dump;
+    a.dump_cap = s->dump_cap;
+    a.dump_count = s->dump_count;
     const int64_t nw = (in->n_hits + 31) / 32;
     if (nw > 0) {
         hipLaunchKernelGGL(zero_u32, dim3(1024), dim3(256), 0, s->stream, out->keep_mask, nw);
@@ -2598,11 +2832,12 @@ static or_status run_device(or_session* s, const or_hits* in, const or_result* o
     } else {
         or_session::GraphKey key;
         std::memset(&key, 0, sizeof key);
-        const void* ptrs[14] = {in->det, in->elem, in->tdc, in->drift, in->flag, in->event_offsets, out->status,
+        const void* ptrs[17] = {in->det, in->elem, in->tdc, in->drift, in->flag, in->event_offsets, out->status,
                                 out->keep_mask, out->n_tracks, out->tracks, out->n_tracklets, out->records,
-                                out->vertex, s->d_lists};
+                                out->vertex, s->d_lists, s->prune_on ? s->d_pruner : nullptr, s->dump,
+                                s->dump_count};
         std::memcpy(key.p, ptrs, sizeof ptrs);
-        key.n_events = in->n_events; key.n_hits = in->n_hits; key.stream = s->stream;
+        key.n_events = in->n_events; key.n_hits = in->n_hits; key.dump_cap = s->dump_cap; key.stream = s->stream;
         key.yz_list_cap = s->yz_list_cap; key.grid_pct = s->grid_pct;
         if (!s->graph_exec || !(key == s->graph_key)) {
             if (s->graph_exec) { (void)hipGraphExecDestroy(s->graph_exec); s->graph_exec = nullptr; }
@@ -2704,6 +2939,46 @@ or_status or_reconstruct(or_session* s, const or_hits* in, or_result* out) {
     return OR_OK;
 }
 
+or_status or_session_set_pruner(or_session* s, const float* weights, int64_t n_weights, float threshold) {
+    if (!s) return OR_ERR_INVALID_ARG;
+    if (!weights) {
+        s->prune_on = false;
+        return OR_OK;
+    }
+    if (n_weights != PRUNE_NW || std::isnan(threshold)) return OR_ERR_INVALID_ARG;
+    for (int64_t i = 0; i < n_weights; ++i)
+        if (!std::isfinite(weights[i])) return OR_ERR_INVALID_ARG;
+    PrunerDev h;
+    prune_pack(weights, threshold, &h);
+    OR_HIP(hipSetDevice(s->device));
+    if (!s->d_pruner) OR_HIP(hipMalloc(&s->d_pruner, sizeof(PrunerDev)));
+    OR_HIP(hipStreamSynchronize(s->stream));
+    OR_HIP(hipMemcpy(s->d_pruner, &h, sizeof h, hipMemcpyHostToDevice));
+    s->prune_on = true;
+    return OR_OK;
+}
+
+or_status or_pruner_score(or_session* s, const float* features, int64_t n, float* scores) {
+    if (!s || n < 0 || (n > 0 && (!features || !scores)) || !s->prune_on) return OR_ERR_INVALID_ARG;
+    if (n == 0) return OR_OK;
+    OR_HIP(hipSetDevice(s->device));
+    hipLaunchKernelGGL(prune_score_kernel, dim3((unsigned)((n + WAVE - 1) / WAVE)), dim3(WAVE), 0, s->stream,
+                       s->d_pruner, features, n, scores);
+    OR_HIP(hipGetLastError());
+    return OR_OK;
+}
+
+or_status or_session_set_unit_dump(or_session* s, float* rows, int64_t cap, unsigned long long* count) {
+    if (!s) return OR_ERR_INVALID_ARG;
+    if (!rows) {
+        s->dump = nullptr; s->dump_cap = 0; s->dump_count = nullptr;
+        return OR_OK;
+    }
+    if (!count || cap < 0) return OR_ERR_INVALID_ARG;
+    s->dump = rows; s->dump_cap = cap; s->dump_count = count;
+    return OR_OK;
+}
+
 or_status or_session_autotune(or_session* s, const or_hits* in, or_result* out, int reps, float* best_ms) {
     if (!s || !in || !out || !in->on_device || !out->on_device || reps < 1) return OR_ERR_INVALID_ARG;
     static const int grid_pcts[] = {100, 75, 50};
@@ -2734,6 +3009,7 @@ void or_session_destroy(or_session* s) {
     if (s->d_scratch) (void)hipFree(s->d_scratch);
     if (s->d_lists) (void)hipFree(s->d_lists);
     if (s->d_counts) (void)hipFree(s->d_counts);
+    if (s->d_pruner) (void)hipFree(s->d_pruner);
     if (s->d_in) (void)hipFree(s->d_in);
     if (s->d_out) (void)hipFree(s->d_out);
     if (s->graph_exec) (void)hipGraphExecDestroy(s->graph_exec);

@@ -47,6 +47,10 @@ EV_REJECT_MASK = 0x7FF
 TUNE_YZ_LIST_CAP = 1
 TUNE_GRID_PERCENT = 2
 TUNE_GRAPH = 3
+PRUNE_WEIGHTS = 5313       # OR_PRUNE_WEIGHTS: w1[64][16], b1[64], w2[64][64], b2[64], w3[64], b3
+PRUNE_FEATURES = 16
+UNIT_ROW = 20              # OR_UNIT_ROW: event, tracklet_id, unit, winner flag, 16 features
+DEFAULT_PRUNER = os.path.join(HERE, "data", "pruner_u3v3.npz")
 
 PARAM_FIELDS = [
     "max_hits_chamber_plane", "max_hits_hodo_plane", "max_hits_prop_plane", "max_reduced_per_plane",
@@ -136,6 +140,12 @@ def load_library(path: str = LIB_PATH):
     lib.or_session_set_tuning.restype = i32
     lib.or_session_autotune.argtypes = [vp, C.POINTER(OrHits), C.POINTER(OrResult), C.c_int, C.POINTER(C.c_float)]
     lib.or_session_autotune.restype = i32
+    lib.or_session_set_pruner.argtypes = [vp, C.POINTER(C.c_float), i64, C.c_float]
+    lib.or_session_set_pruner.restype = i32
+    lib.or_pruner_score.argtypes = [vp, vp, i64, vp]
+    lib.or_pruner_score.restype = i32
+    lib.or_session_set_unit_dump.argtypes = [vp, vp, i64, vp]
+    lib.or_session_set_unit_dump.restype = i32
     lib.or_session_destroy.argtypes = [vp]
     lib.or_status_string.argtypes = [i32]
     lib.or_status_string.restype = C.c_char_p
@@ -297,6 +307,34 @@ class Session:
     def set_tuning(self, key: int, value: int):
         """or_session_set_tuning; key e.g. TUNE_YZ_LIST_CAP."""
         _check(self.lib.or_session_set_tuning(self.handle, key, value))
+
+    # --- learned YZ unit pruning -----------------------------------------------
+    def set_pruner(self, weights: np.ndarray | None, threshold: float = 0.0):
+        """or_session_set_pruner: flat fp32 weights (PRUNE_WEIGHTS), or None for the plain chain."""
+        if weights is None:
+            _check(self.lib.or_session_set_pruner(self.handle, None, 0, 0.0))
+            return
+        w = np.ascontiguousarray(weights, dtype=np.float32).ravel()
+        _check(self.lib.or_session_set_pruner(self.handle, w.ctypes.data_as(C.POINTER(C.c_float)), w.size,
+                                              float(threshold)))
+
+    def score_units(self, features):
+        """or_pruner_score on a (n, 16) float32 device tensor; returns the (n,) scores."""
+        import torch
+        f = features.contiguous()
+        out = torch.empty(f.shape[0], dtype=torch.float32, device=f.device)
+        stream = torch.cuda.current_stream(self.device)
+        _check(self.lib.or_session_set_stream(self.handle, C.c_void_p(stream.cuda_stream)))
+        _check(self.lib.or_pruner_score(self.handle, C.c_void_p(f.data_ptr()), f.shape[0], C.c_void_p(out.data_ptr())))
+        return out
+
+    def set_unit_dump(self, rows=None, count=None):
+        """or_session_set_unit_dump with a (cap, UNIT_ROW) float32 and a (1,) int64 device tensor (zero it first)."""
+        if rows is None:
+            _check(self.lib.or_session_set_unit_dump(self.handle, None, 0, None))
+            return
+        _check(self.lib.or_session_set_unit_dump(self.handle, C.c_void_p(rows.data_ptr()), rows.shape[0],
+                                                 C.c_void_p(count.data_ptr())))
 
     # --- device-resident path -------------------------------------------------
     def upload(self, ev: Events):

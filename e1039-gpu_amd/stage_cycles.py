@@ -1,11 +1,12 @@
 """Cycles per chain stage on the GPU, from the profiling build.
 
     make -C e1039-gpu_amd/csrc timing
-    python e1039-gpu_amd/stage_cycles.py [n_events] [mean_hits]
+    python e1039-gpu_amd/stage_cycles.py [n_events] [mean_hits] [pruned]
 
 Runs one warm-up and one measured launch of the chain on seeded events
 (hits/event ~ Poisson(mean_hits) capped at 1000) through libe1039or_timing.so
-and prints each stage's share of the summed wave cycles.
+and prints each stage's share of the summed wave cycles.  With "pruned" the
+shipped YZ unit pruner is on.
 """
 import ctypes as C
 import os
@@ -16,7 +17,8 @@ sys.path.insert(0, HERE)
 
 STAGES = ["load+reduce", "X pairs", "XZ", "YZ", "cleaning", "global", "outputs"]
 SUBS = ["YZ window lists", "YZ pass 1 (first fit)", "YZ pass 2 (seeds)", "YZ pass 3 (tails)", "YZ winners",
-        "YZ unit path", "st1 window lists", "st1 pass A (y fits)", "st1 pass B + write"]
+        "YZ unit path", "st1 window lists", "st1 pass A (y fits)", "st1 pass B + write",
+        "YZ unit scoring (in pass 1)"]
 
 
 def main():
@@ -33,6 +35,11 @@ def main():
     geo = oc.Geometry.load()
     ev = oc.generate(geo, n, seed=7, mult_mode=1, mult_mean=mean, mult_cap=1000)
     sess = oc.Session(geo, oc.default_params(max_hits_per_event=1024), device=0)
+    if len(sys.argv) > 3 and sys.argv[3] == "pruned":
+        import pruner
+        flat, thr, _ = pruner.load()
+        sess.set_pruner(flat, thr)
+        print(f"YZ unit pruner on (threshold {thr:.4f})")
     din = sess.upload(ev)
     dout = sess.alloc_outputs(ev.n_events, ev.n_hits, tracklets=True)
     sess.run_device(din, dout)

@@ -202,6 +202,38 @@ or_status or_session_set_tuning(or_session* s, or_tuning key, int64_t value);
  * counterpart. */
 or_status or_session_autotune(or_session* s, const or_hits* in, or_result* out, int reps, float* best_ms);
 
+/* ---- learned pruning of YZ station-3 units (no reference counterpart) -----
+ * A YZ unit is one (u3, v3) window-list pair of a tracklet: the reference
+ * evaluates every (u2, v2) pair under every unit (the i_uv3 loop,
+ * reconstruction_kernels.cuh:1320-1800).  With a pruner set, each unit is scored by a 16-64-64-1 ReLU
+ * perceptron on the matrix cores and units scoring below the threshold are
+ * skipped; everything else is the plain chain, so each track either comes out
+ * identical to the plain chain's or (when its unit was skipped) differs or is
+ * missing.  The 16 features are those of or_session_set_unit_dump.
+ * weights: OR_PRUNE_WEIGHTS floats, row-major (out x in): w1[64][16], b1[64],
+ * w2[64][64], b2[64], w3[64], b3; score = w3 . relu(w2 relu(w1 f + b1) + b2) + b3
+ * with bf16 weights and activations and fp32 sums.  weights == NULL turns
+ * pruning off.  Call between reconstructions (it waits for the session stream). */
+#define OR_PRUNE_WEIGHTS 5313
+or_status or_session_set_pruner(or_session* s, const float* weights, int64_t n_weights, float threshold);
+
+/* Scores n rows of 16 unit features with the session's pruner (device
+ * pointers; asynchronous on the session stream). */
+or_status or_pruner_score(or_session* s, const float* features, int64_t n, float* scores);
+
+/* Training aid: during device-resident calls, one row of OR_UNIT_ROW floats
+ * per scored unit is appended to rows (device memory, cap rows): event index,
+ * tracklet_id, u3.v3 index, winner flag (1: the tracklet's YZ winner is in this
+ * unit, 0: not, -1: unknown), then the 16 features: x0/100, 10 tx, station-3
+ * is D3m, u3 hits - 1, v3 hits - 1, y/50 of the first and last accepted u3 hit
+ * and of the v3 hits, their four y errors, (yu - yv)/10 and |yu - yv|/10 of the
+ * entry means, (ya - yb)/10 of the u3 hits.  *count (device uint64, zeroed by
+ * the caller) counts the rows wanted; rows beyond cap are not written.  Chunks
+ * with more than 4096 units are neither pruned nor dumped.  rows == NULL turns
+ * it off. */
+#define OR_UNIT_ROW 20
+or_status or_session_set_unit_dump(or_session* s, float* rows, int64_t cap, unsigned long long* count);
+
 /* Profiling: device cycles per chain stage summed over all waves since the last
  * reset, out[class * 7 + stage] for the 5 tile classes (512 ... 8192 hits) and
  * the stages load+reduce, X pairs, XZ, YZ, cleaning, global, outputs.  Counts

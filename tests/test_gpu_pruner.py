@@ -1,0 +1,109 @@
+"""Learned YZ unit pruning on the device, through the C ABI.
+
+* The matrix-core perceptron (or_pruner_score) against the torch fp32
+  restatement of the same arithmetic (pruner.score_reference).
+* The unit-list path with every unit kept (threshold -inf) against the CPU
+  oracle, bit for bit: enumerating kept units is the plain chain when nothing
+  is pruned.
+* The unit rows (or_session_set_unit_dump): one winner row per output track.
+* The shipped pruner keeps >= 99.5 % of the plain chain's tracks identical.
+"""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def psess(oc, geo):
+    s = oc.Session(geo, oc.default_params(max_hits_per_event=2048), device=0)
+    yield s
+    s.close()
+
+
+def random_weights(seed):
+    import pruner
+    rng = np.random.default_rng(seed)
+    return pruner.pack(rng.normal(0, 0.5, (64, 16)), rng.normal(0, 0.2, 64), rng.normal(0, 0.2, (64, 64)),
+                       rng.normal(0, 0.2, 64), rng.normal(0, 0.3, 64), 0.1)
+
+
+def test_scores_match_reference(psess):
+    import torch
+
+    import pruner
+    flat = random_weights(1)
+    rng = np.random.default_rng(2)
+    feats = rng.normal(0, 1, (1000, 16)).astype(np.float32)   # not a multiple of 64 rows
+    psess.set_pruner(flat, 0.0)
+    try:
+        dev = psess.score_units(torch.as_tensor(feats, device="cuda"))
+        torch.cuda.synchronize()
+        dev = dev.cpu().numpy()
+    finally:
+        psess.set_pruner(None)
+    ref = pruner.score_reference(flat, torch.as_tensor(feats)).numpy()
+    # fp32 sums in another order can move a bf16 rounding of a hidden unit by one step
+    err = np.abs(dev - ref) / (1.0 + np.abs(ref))
+    assert np.quantile(err, 0.999) < 1e-2, f"scores differ: {np.quantile(err, 0.999):.3e}"
+    assert err.max() < 5e-2, f"worst score differs by {err.max():.3e}"
+    assert np.std(ref) > 0.5   # the scores do depend on the features
+
+
+def same_outputs(got, ref):
+    assert np.array_equal(got.status, ref.status)
+    assert np.array_equal(got.keep_mask, ref.keep_mask)
+    assert np.array_equal(got.n_tracklets, ref.n_tracklets)
+    assert np.array_equal(got.records.view(np.uint32), ref.records.view(np.uint32))
+    assert np.array_equal(got.n_tracks, ref.n_tracks)
+    assert np.array_equal(got.valid_tracks().view(np.uint8), ref.valid_tracks().view(np.uint8))
+
+
+def test_keep_all_units_is_the_plain_chain(oc, orc, geo, ogeo, psess):
+    import torch
+    ev = oc.generate(geo, 1500, seed=21, mult_mode=1, mult_mean=300.0, mult_cap=1000)
+    rows = torch.empty((ev.n_events * 400, oc.UNIT_ROW), dtype=torch.float32, device="cuda")
+    count = torch.zeros(1, dtype=torch.int64, device="cuda")
+    psess.set_pruner(random_weights(3), float("-inf"))
+    psess.set_unit_dump(rows, count)
+    try:
+        got = psess.reconstruct(ev, records=True)
+    finally:
+        psess.set_unit_dump(None)
+        psess.set_pruner(None)
+    ref = orc.reconstruct(ogeo, psess.params, ev, records=True)
+    assert ref.n_tracks.sum() > 300
+    same_outputs(got, ref)
+    n = int(count.item())
+    assert 0 < n <= rows.shape[0]
+    r = rows[:n].cpu().numpy()
+    assert set(np.unique(r[:, 3])) <= {-1.0, 0.0, 1.0}
+    assert (r[:, 0] >= 0).all() and (r[:, 0] < ev.n_events).all()
+    # every output track's tracklet has exactly one winner unit
+    win = r[r[:, 3] == 1.0]
+    wkeys, wcount = np.unique(win[:, 0].astype(np.int64) * 4096 + win[:, 1].astype(np.int64), return_counts=True)
+    assert (wcount == 1).all()
+    unknown = set((r[r[:, 3] < 0][:, 0].astype(np.int64) * 4096 + r[r[:, 3] < 0][:, 1].astype(np.int64)).tolist())
+    for e in np.nonzero(got.n_tracks)[0]:
+        for k in range(got.n_tracks[e]):
+            key = int(e) * 4096 + int(got.tracks[e, k]["tracklet_id"])
+            assert key in set(wkeys.tolist()) or key in unknown
+
+
+def test_shipped_pruner_recall(oc, geo, psess):
+    import pruner
+    if not os.path.exists(oc.DEFAULT_PRUNER):
+        pytest.skip("no shipped pruner weights")
+    flat, thr, meta = pruner.load()
+    ev = oc.generate(geo, 20000, seed=0x5EED, mult_mode=1, mult_mean=300.0, mult_cap=1000)
+    plain = psess.reconstruct(ev)
+    psess.set_pruner(flat, thr)
+    try:
+        pr = psess.reconstruct(ev)
+    finally:
+        psess.set_pruner(None)
+    rec, found, total = pruner.recall(plain, pr)
+    assert total > 10000
+    assert rec >= 0.995, f"track recall {rec:.5f} ({found}/{total})"
