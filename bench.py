@@ -54,7 +54,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="target CPU time of the CPU baseline sample (0 = skip the baseline)")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--plain", action="store_true", help="time the plain chain only (no pruner)")
+    ap.add_argument("--chain", choices=["both", "plain", "pruned"], default="both",
+                    help="both: time plain and pruned; plain: plain only; pruned: plain once untimed (recall), pruned timed")
+    ap.add_argument("--no-e2e", action="store_true", help="skip the host-buffer (PCIe) leg")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (reference for HBM byte counters) and exit")
     args = ap.parse_args()
@@ -90,29 +92,34 @@ def main():
     dev_in = sess.upload(ev)
     torch.cuda.synchronize()
 
-    def timed(label):
+    def timed(label, steps=None, warmup=None):
+        steps = args.steps if steps is None else steps
+        warmup = args.warmup if warmup is None else warmup
         out = sess.alloc_outputs(ev.n_events, ev.n_hits, tracklets=True)
-        for i in range(args.warmup):
+        for i in range(warmup):
             sess.run_device(dev_in, out)
             torch.cuda.synchronize()
             log(f"{label} warmup step {i} done: kernel {sess.last_kernel_ms():.1f} ms")
         kms = []
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        for i in range(args.steps):
+        for i in range(steps):
             sess.run_device(dev_in, out)
             kms.append(sess.last_kernel_ms())
         torch.cuda.synchronize()
         t1 = time.perf_counter()
-        ms = (t1 - t0) * 1e3 / max(1, args.steps)
+        ms = (t1 - t0) * 1e3 / max(1, steps)
         km = float(np.mean(kms)) if kms else float("nan")
-        log(f"{label}: timed {args.steps} steps: {ms:.2f} ms/step, chain kernel {km:.2f} ms")
+        log(f"{label}: timed {steps} steps: {ms:.2f} ms/step, chain kernel {km:.2f} ms")
         res = oc.Session.fetch(out, ev.n_events, params.max_tracks_out, params.max_tracklets_per_event)
         return ms, km, res
 
-    plain_ms, plain_kms, plain_res = timed("plain")
+    if args.chain == "pruned":
+        plain_ms, plain_kms, plain_res = timed("plain (untimed reference)", steps=1, warmup=0)
+    else:
+        plain_ms, plain_kms, plain_res = timed("plain")
     prune = None
-    if not args.plain:
+    if args.chain != "plain":
         import pruner
         if not os.path.exists(oc.DEFAULT_PRUNER):
             log(f"no pruner weights at {oc.DEFAULT_PRUNER}; reporting the plain chain")
@@ -138,14 +145,25 @@ def main():
     log(f"tracks out {n_out}, tracklets {int(res.n_tracklets.sum())}, rejected events {rejected}; "
         f"algorithmic bytes {b_alg / 1e9:.3f} GB per launch")
 
+    # final straight-line fit of the output tracks against float64, on a slice (untimed)
+    import fitcheck
+    n_fit = min(ev.n_events, 200_000)
+    fsub = ev.subset(0, n_fit)
+    fit_rep = fitcheck.error_report(fsub, sess.reconstruct(fsub, fit=True), geo)
+    log(f"final fit vs float64 on {fit_rep['tracks']} tracks: " +
+        ", ".join(f"{q} max {fit_rep[q]['max_rel']:.1e}" for q in fitcheck.QUANTITIES) +
+        f"; FP64 solves {fit_rep['fp64_fraction'] * 100:.2f} %")
+
     # end to end through host buffers (PCIe both ways) on a slice, for reference only
-    n_e2e = min(ev.n_events, 200_000)
-    sub = ev.subset(0, n_e2e)
-    sess.reconstruct_host(sub)
-    t0 = time.perf_counter()
-    sess.reconstruct_host(sub)
-    e2e = n_e2e / (time.perf_counter() - t0)
-    log(f"host-buffer path (H2D + chain + D2H): {e2e:.0f} events/s on {n_e2e} events")
+    e2e = None
+    if not args.no_e2e:
+        n_e2e = min(ev.n_events, 200_000)
+        sub = ev.subset(0, n_e2e)
+        sess.reconstruct_host(sub)
+        t0 = time.perf_counter()
+        sess.reconstruct_host(sub)
+        e2e = n_e2e / (time.perf_counter() - t0)
+        log(f"host-buffer path (H2D + plain chain + D2H): {e2e:.0f} events/s on {n_e2e} events")
 
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "chain_kernel_1M_poisson300_pruned_traffic.json" if prune else
@@ -202,15 +220,16 @@ def main():
             "events": ev.n_events,
             "hits": int(ev.n_hits),
             "kernel_ms": kernel_ms,
-            "plain_events_per_s": ev.n_events / (plain_ms * 1e-3),
-            "plain_ms_per_step": plain_ms,
-            "speedup_vs_plain": (plain_ms / prune["ms"]) if prune else 1.0,
+            "plain_events_per_s": ev.n_events / (plain_ms * 1e-3) if args.chain != "pruned" else None,
+            "plain_ms_per_step": plain_ms if args.chain != "pruned" else None,
+            "speedup_vs_plain": (plain_ms / prune["ms"]) if prune and args.chain == "both" else None,
             "track_recall": prune["recall"] if prune else 1.0,
             "plain_tracks": prune["total"] if prune else int(plain_res.n_tracks.sum()),
             "pruned_tracks_identical": prune["found"] if prune else None,
             "pruned_other_tracks": prune["extra"] if prune else None,
             "prune_threshold": prune["threshold"] if prune else None,
             "host_buffer_events_per_s": e2e,
+            "final_fit_vs_fp64": dict(fit_rep, sample=f"output tracks of the first {n_fit} events, plain chain"),
         },
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},

@@ -23,6 +23,11 @@
 
 namespace ordev {
 __constant__ float c_geo[F_COUNT][ND];
+// per-plane constants of the final track fit (hi/lo float pairs, see final_fit)
+constexpr int FITP_M = 0, FITP_V = 20, FITP_H = 28, FITP_W = 36, FITP_N = 37;
+constexpr double FIT_ZREF = 1000.0;
+__constant__ float c_fitp[ND][FITP_N];
+__constant__ float c_fit_cond = OR_FIT_COND_FP32;   // FP32 / FP64 switch of the final fit
 }
 
 using namespace ordev;
@@ -122,6 +127,7 @@ struct Args {
     int32_t* n_tracklets;
     float* records;
     float* vertex;              // optional: swim global tracks to the beam line
+    float* fit;                 // optional: final straight-line fit of each output track
     char* scratch;
     int64_t scratch_stride;
     int yz_list_cap;            // <= YZ_LIST_CAP; chunks with more candidates take the sequential path
@@ -2376,6 +2382,225 @@ STAGE void stage_vertex(Tk* tks, int ntkl) {
 }
 
 // H: outputs
+// ---- final straight-line fit of an output track (no reference counterpart) ----
+// u = pos + sign * drift per hit, model u = cos (x0 + tx z) + sin (y0 + ty z),
+// weight 1/resolution^2, z taken from FIT_ZREF.  The normal matrix and
+// right-hand side are built from per-plane constants computed on the host in
+// double and held as hi/lo float pairs (w h_i h_j, w h_i, h_i with
+// h = (cos, cos z, sin, sin z)); the device sums them with Neumaier compensation
+// and forms the u terms with error-free products (fmaf), so the FP32 sums carry
+// the normal equations to about double precision.  The matrix is equilibrated
+// and Cholesky-factorised in FP32; one refinement step against the compensated
+// residual follows.  When (max/min pivot)^2 exceeds OR_FIT_COND_FP32, or a
+// pivot is not positive, the solve and chi2 run in FP64 from the same sums.
+
+// Neumaier compensated sum
+struct CSum {
+    float s = 0.f, c = 0.f;
+    __device__ __forceinline__ void add(float x) {
+        const float t = s + x;
+        c += fabsf(s) >= fabsf(x) ? (s - t) + x : (x - t) + s;
+        s = t;
+    }
+    __device__ __forceinline__ float f() const { return s + c; }
+    __device__ __forceinline__ double d() const { return (double)s + (double)c; }
+};
+
+// a + b = s + e exactly, a * b = p + e exactly
+__device__ __forceinline__ void two_sum(float a, float b, float& s, float& e) {
+    s = a + b;
+    const float bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
+__device__ __forceinline__ void two_prod(float a, float b, float& p, float& e) {
+    p = a * b;
+    e = fmaf(a, b, -p);
+}
+
+// In place: a (symmetric, lower part used) = L L^T.  cond = (max/min diagonal
+// of L)^2; false when a pivot is not positive.
+template <typename T>
+__device__ __forceinline__ bool chol4(T (&a)[4][4], T& cond) {
+    T lmin = 0, lmax = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        T v = a[k][k];
+#pragma unroll
+        for (int j = 0; j < k; ++j) v -= a[k][j] * a[k][j];
+        if (!(v > T(0))) return false;
+        const T l = sqrt(v);
+        a[k][k] = l;
+        lmin = k == 0 ? l : (l < lmin ? l : lmin);
+        lmax = k == 0 ? l : (l > lmax ? l : lmax);
+#pragma unroll
+        for (int i = k + 1; i < 4; ++i) {
+            T w = a[i][k];
+#pragma unroll
+            for (int j = 0; j < k; ++j) w -= a[i][j] * a[k][j];
+            a[i][k] = w / l;
+        }
+    }
+    cond = (lmax / lmin) * (lmax / lmin);
+    return true;
+}
+// b <- (L L^T)^-1 b
+template <typename T>
+__device__ __forceinline__ void chol4_solve(const T (&l)[4][4], T (&b)[4]) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        T v = b[i];
+#pragma unroll
+        for (int j = 0; j < i; ++j) v -= l[i][j] * b[j];
+        b[i] = v / l[i][i];
+    }
+#pragma unroll
+    for (int i = 3; i >= 0; --i) {
+        T v = b[i];
+#pragma unroll
+        for (int j = i + 1; j < 4; ++j) v -= l[j][i] * b[j];
+        b[i] = v / l[i][i];
+    }
+}
+
+// upper-triangle index of (i, j), i <= j
+__device__ __forceinline__ constexpr int fit_ut(int i, int j) { return i * 4 - i * (i - 1) / 2 + (j - i); }
+
+template <int MAXH>
+__device__ __attribute__((noinline)) void final_fit(const Tk& T, float* out) {
+    const Lds<MAXH>& L = g_lds<MAXH>;
+    const int nh = T.nx + T.nuv + T.n1;
+    CSum A[10], B[4];
+    for (int h = 0; h < nh; ++h) {
+        const int d = T.det[h], r = T.hraw[h];
+        float uh, ul;
+        two_sum(L.pos[r], (float)T.sign[h] * L.drift[r], uh, ul);
+#pragma unroll
+        for (int k = 0; k < 10; ++k) {
+            A[k].add(c_fitp[d][FITP_M + 2 * k]);
+            A[k].add(c_fitp[d][FITP_M + 2 * k + 1]);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const float vh = c_fitp[d][FITP_V + 2 * i], vl = c_fitp[d][FITP_V + 2 * i + 1];
+            float pp, pe;
+            two_prod(vh, uh, pp, pe);
+            B[i].add(pp); B[i].add(pe); B[i].add(vh * ul); B[i].add(vl * uh);
+        }
+    }
+    float res8[OR_FIT_FIELDS] = {0.f, 0.f, 0.f, 0.f, 0.f, (float)(nh - 4), INFINITY, 0.f};
+    if (nh >= 4) {
+        float sc[4], l[4][4], cond = INFINITY;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const float aii = A[fit_ut(i, i)].f();
+            sc[i] = aii > 0.f ? 1.f / sqrtf(aii) : 0.f;
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) l[i][j] = (A[i <= j ? fit_ut(i, j) : fit_ut(j, i)].f() * sc[i]) * sc[j];
+        const bool ok = chol4(l, cond);
+        if (ok && cond <= c_fit_cond) {
+            // solve, then one refinement step against r = b - A p in compensated FP32
+            float p[4], plo[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int i = 0; i < 4; ++i) p[i] = B[i].f() * sc[i];
+            chol4_solve(l, p);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) p[i] *= sc[i];
+            float rr[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                CSum acc;
+                acc.add(B[i].s); acc.add(B[i].c);
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const CSum& aij = A[i <= j ? fit_ut(i, j) : fit_ut(j, i)];
+                    float pp, pe;
+                    two_prod(aij.s, p[j], pp, pe);
+                    acc.add(-pp); acc.add(-pe); acc.add(-(aij.c * p[j]));
+                }
+                rr[i] = acc.f() * sc[i];
+            }
+            chol4_solve(l, rr);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                float sp, se;
+                two_sum(p[i], rr[i] * sc[i], sp, se);
+                p[i] = sp; plo[i] = se;
+            }
+            // to z = 0: x0 = x_ref - tx FIT_ZREF
+            const float zr = (float)FIT_ZREF;
+            auto ext = [&](float a, float alo, float t, float tlo) {
+                float pp, pe, ss, se;
+                two_prod(t, zr, pp, pe);
+                two_sum(a, -pp, ss, se);
+                return ss + ((se - pe) + (alo - tlo * zr));
+            };
+            res8[0] = ext(p[0], plo[0], p[1], plo[1]); res8[1] = p[1] + plo[1];
+            res8[2] = ext(p[2], plo[2], p[3], plo[3]); res8[3] = p[3] + plo[3];
+            // chi2 from double-float residuals u - h.p
+            CSum c2;
+            for (int h = 0; h < nh; ++h) {
+                const int d = T.det[h], r = T.hraw[h];
+                float uh, ul;
+                two_sum(L.pos[r], (float)T.sign[h] * L.drift[r], uh, ul);
+                CSum acc;
+                acc.add(uh); acc.add(ul);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const float hh = c_fitp[d][FITP_H + 2 * i], hl = c_fitp[d][FITP_H + 2 * i + 1];
+                    float pp, pe;
+                    two_prod(hh, p[i], pp, pe);
+                    acc.add(-pp); acc.add(-pe); acc.add(-(hh * plo[i])); acc.add(-(hl * p[i]));
+                }
+                const float rs = acc.f();
+                c2.add((c_fitp[d][FITP_W] * rs) * rs);
+            }
+            res8[4] = c2.f(); res8[6] = cond;
+        } else {
+            double ad[4][4], pd[4], condd = 0.0;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                pd[i] = B[i].d();
+#pragma unroll
+                for (int j = 0; j < 4; ++j) ad[i][j] = A[i <= j ? fit_ut(i, j) : fit_ut(j, i)].d();
+            }
+            double scd[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) scd[i] = ad[i][i] > 0.0 ? 1.0 / sqrt(ad[i][i]) : 0.0;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) ad[i][j] = (ad[i][j] * scd[i]) * scd[j];
+                pd[i] *= scd[i];
+            }
+            if (chol4(ad, condd)) {
+                chol4_solve(ad, pd);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) pd[i] *= scd[i];
+                double c2 = 0.0;
+                for (int h = 0; h < nh; ++h) {
+                    const int d = T.det[h], r = T.hraw[h];
+                    double m = 0.0;
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+                        m += ((double)c_fitp[d][FITP_H + 2 * i] + (double)c_fitp[d][FITP_H + 2 * i + 1]) * pd[i];
+                    const double rs = ((double)L.pos[r] + (double)T.sign[h] * (double)L.drift[r]) - m;
+                    c2 += (double)c_fitp[d][FITP_W] * rs * rs;
+                }
+                res8[0] = (float)(pd[0] - pd[1] * FIT_ZREF); res8[1] = (float)pd[1];
+                res8[2] = (float)(pd[2] - pd[3] * FIT_ZREF); res8[3] = (float)pd[3];
+                res8[4] = (float)c2;
+            }
+            res8[6] = ok ? cond : (float)condd;
+            res8[7] = 1.f;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < OR_FIT_FIELDS; ++q) out[q] = res8[q];
+}
+
 template <int MAXH>
 STAGE void stage_out(const Args& a, Scr S, int64_t e, int ntkl) {
     Lds<MAXH>& L = g_lds<MAXH>;
@@ -2412,6 +2637,7 @@ STAGE void stage_out(const Args& a, Scr S, int64_t e, int ntkl) {
                     a.tracks[e * P.max_tracks_out + o] = R;
                     if (a.vertex)
                         for (int c = 0; c < 6; ++c) a.vertex[(e * P.max_tracks_out + o) * 6 + c] = T.vtx[c];
+                    if (a.fit) final_fit<MAXH>(T, a.fit + (e * P.max_tracks_out + o) * OR_FIT_FIELDS);
                 }
             }
             nout += __popcll(m);
@@ -2628,7 +2854,7 @@ struct or_session {
     int64_t dump_cap = 0;
     unsigned long long* dump_count = nullptr;
     struct GraphKey {
-        const void* p[17];
+        const void* p[18];
         int64_t n_events, n_hits, dump_cap;
         hipStream_t stream;
         int yz_list_cap, grid_pct;
@@ -2706,6 +2932,31 @@ or_status or_session_create(const or_geometry* g, const or_params* p, int device
     OR_HIP(hipEventCreate(&s->ev0));
     OR_HIP(hipEventCreate(&s->ev1));
     OR_HIP(hipMemcpyToSymbol(HIP_SYMBOL(ordev::c_geo), g->f, sizeof g->f));
+    {
+        // final-fit plane constants in double, split into hi/lo floats
+        static_assert(ordev::FITP_N == 37, "final-fit constant layout");
+        float fp[OR_N_DETECTORS][ordev::FITP_N] = {};
+        auto put = [](float* dst, double x) {
+            const float hi = (float)x;
+            dst[0] = hi;
+            dst[1] = (float)(x - (double)hi);
+        };
+        for (int d = 0; d < OR_N_DETECTORS; ++d) {
+            const double c = g->f[F_COSTHETA][d], sn = g->f[F_SINTHETA][d];
+            const double z = (double)g->f[F_Z][d] - ordev::FIT_ZREF, r = g->f[F_RESOLUTION][d];
+            const double w = r > 0.0 ? 1.0 / (r * r) : 0.0;
+            const double hv[4] = {c, c * z, sn, sn * z};
+            int k = 0;
+            for (int i = 0; i < 4; ++i)
+                for (int j = i; j < 4; ++j, ++k) put(&fp[d][ordev::FITP_M + 2 * k], w * hv[i] * hv[j]);
+            for (int i = 0; i < 4; ++i) {
+                put(&fp[d][ordev::FITP_V + 2 * i], w * hv[i]);
+                put(&fp[d][ordev::FITP_H + 2 * i], hv[i]);
+            }
+            fp[d][ordev::FITP_W] = (float)w;
+        }
+        OR_HIP(hipMemcpyToSymbol(HIP_SYMBOL(ordev::c_fitp), fp, sizeof fp));
+    }
     OR_HIP(hipMalloc(&s->d_wire, sizeof g->wire));
     OR_HIP(hipMemcpy(s->d_wire, g->wire, sizeof g->wire, hipMemcpyHostToDevice));
     hipDeviceProp_t prop;
@@ -2777,7 +3028,7 @@ static or_status issue_chain(or_session* s, const or_hits* in, const or_result* 
     a.wire = s->d_wire;
     a.p = s->p;
     a.status = out->status; a.keep_mask = out->keep_mask; a.n_tracks = out->n_tracks; a.tracks = out->tracks;
-    a.n_tracklets = out->n_tracklets; a.records = out->records; a.vertex = out->vertex;
+    a.n_tracklets = out->n_tracklets; a.records = out->records; a.vertex = out->vertex; a.fit = out->fit;
     a.scratch = s->d_scratch;
     a.scratch_stride = SCR_STRIDE;
     a.yz_list_cap = s->yz_list_cap;
@@ -2832,9 +3083,9 @@ static or_status run_device(or_session* s, const or_hits* in, const or_result* o
     } else {
         or_session::GraphKey key;
         std::memset(&key, 0, sizeof key);
-        const void* ptrs[17] = {in->det, in->elem, in->tdc, in->drift, in->flag, in->event_offsets, out->status,
+        const void* ptrs[18] = {in->det, in->elem, in->tdc, in->drift, in->flag, in->event_offsets, out->status,
                                 out->keep_mask, out->n_tracks, out->tracks, out->n_tracklets, out->records,
-                                out->vertex, s->d_lists, s->prune_on ? s->d_pruner : nullptr, s->dump,
+                                out->vertex, out->fit, s->d_lists, s->prune_on ? s->d_pruner : nullptr, s->dump,
                                 s->dump_count};
         std::memcpy(key.p, ptrs, sizeof ptrs);
         key.n_events = in->n_events; key.n_hits = in->n_hits; key.dump_cap = s->dump_cap; key.stream = s->stream;
@@ -2886,7 +3137,8 @@ or_status or_reconstruct(or_session* s, const or_hits* in, or_result* out) {
     const size_t tk_bytes = (size_t)ne * s->p.max_tracks_out * sizeof(or_track);
     const size_t rec_bytes = out->records ? (size_t)ne * s->p.max_tracklets_per_event * OR_TRACK_PARAMS * 4 : 0;
     const size_t vtx_bytes = out->vertex ? (size_t)ne * s->p.max_tracks_out * 6 * 4 : 0;
-    const size_t out_bytes = (size_t)ne * 4 * 3 + (size_t)nw * 4 + tk_bytes + rec_bytes + vtx_bytes + 256 * 8;
+    const size_t fit_bytes = out->fit ? (size_t)ne * s->p.max_tracks_out * OR_FIT_FIELDS * 4 : 0;
+    const size_t out_bytes = (size_t)ne * 4 * 3 + (size_t)nw * 4 + tk_bytes + rec_bytes + vtx_bytes + fit_bytes + 256 * 8;
     if (s->d_in_bytes < in_bytes) {
         if (s->d_in) (void)hipFree(s->d_in);
         s->d_in = nullptr;
@@ -2925,6 +3177,7 @@ or_status or_reconstruct(or_session* s, const or_hits* in, or_result* out) {
     dout.tracks = (or_track*)carve(po, tk_bytes);
     dout.records = out->records ? (float*)carve(po, rec_bytes) : nullptr;
     dout.vertex = out->vertex ? (float*)carve(po, vtx_bytes) : nullptr;
+    dout.fit = out->fit ? (float*)carve(po, fit_bytes) : nullptr;
     dout.on_device = 1;
     or_status rs = run_device(s, &din, &dout, true);
     if (rs != OR_OK) return rs;
@@ -2935,6 +3188,7 @@ or_status or_reconstruct(or_session* s, const or_hits* in, or_result* out) {
     if (tk_bytes) OR_HIP(hipMemcpyAsync(out->tracks, dout.tracks, tk_bytes, hipMemcpyDeviceToHost, st));
     if (rec_bytes) OR_HIP(hipMemcpyAsync(out->records, dout.records, rec_bytes, hipMemcpyDeviceToHost, st));
     if (vtx_bytes) OR_HIP(hipMemcpyAsync(out->vertex, dout.vertex, vtx_bytes, hipMemcpyDeviceToHost, st));
+    if (fit_bytes) OR_HIP(hipMemcpyAsync(out->fit, dout.fit, fit_bytes, hipMemcpyDeviceToHost, st));
     OR_HIP(hipStreamSynchronize(st));
     return OR_OK;
 }
@@ -2955,6 +3209,14 @@ or_status or_session_set_pruner(or_session* s, const float* weights, int64_t n_w
     OR_HIP(hipStreamSynchronize(s->stream));
     OR_HIP(hipMemcpy(s->d_pruner, &h, sizeof h, hipMemcpyHostToDevice));
     s->prune_on = true;
+    return OR_OK;
+}
+
+or_status or_session_set_fit_cond_bound(or_session* s, float bound) {
+    if (!s || !(bound >= 0.f)) return OR_ERR_INVALID_ARG;
+    OR_HIP(hipSetDevice(s->device));
+    OR_HIP(hipStreamSynchronize(s->stream));
+    OR_HIP(hipMemcpyToSymbol(HIP_SYMBOL(ordev::c_fit_cond), &bound, sizeof bound));
     return OR_OK;
 }
 

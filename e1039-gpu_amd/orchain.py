@@ -51,6 +51,8 @@ PRUNE_WEIGHTS = 5313       # OR_PRUNE_WEIGHTS: w1[64][16], b1[64], w2[64][64], b
 PRUNE_FEATURES = 16
 UNIT_ROW = 20              # OR_UNIT_ROW: event, tracklet_id, unit, winner flag, 16 features
 DEFAULT_PRUNER = os.path.join(HERE, "data", "pruner_u3v3.npz")
+FIT_FIELDS = ["x0", "tx", "y0", "ty", "chi2", "ndf", "cond", "fp64"]   # OR_FIT_FIELDS per output track
+FIT_COND_FP32 = 1.0e4                                                   # OR_FIT_COND_FP32
 
 PARAM_FIELDS = [
     "max_hits_chamber_plane", "max_hits_hodo_plane", "max_hits_prop_plane", "max_reduced_per_plane",
@@ -75,7 +77,8 @@ class OrHits(C.Structure):
 
 class OrResult(C.Structure):
     _fields_ = [("status", C.c_void_p), ("keep_mask", C.c_void_p), ("n_tracks", C.c_void_p), ("tracks", C.c_void_p),
-                ("n_tracklets", C.c_void_p), ("records", C.c_void_p), ("vertex", C.c_void_p), ("on_device", C.c_int32)]
+                ("n_tracklets", C.c_void_p), ("records", C.c_void_p), ("vertex", C.c_void_p), ("fit", C.c_void_p),
+                ("on_device", C.c_int32)]
 
 
 class OrGenParams(C.Structure):
@@ -146,6 +149,8 @@ def load_library(path: str = LIB_PATH):
     lib.or_pruner_score.restype = i32
     lib.or_session_set_unit_dump.argtypes = [vp, vp, i64, vp]
     lib.or_session_set_unit_dump.restype = i32
+    lib.or_session_set_fit_cond_bound.argtypes = [vp, C.c_float]
+    lib.or_session_set_fit_cond_bound.restype = i32
     lib.or_session_destroy.argtypes = [vp]
     lib.or_status_string.argtypes = [i32]
     lib.or_status_string.restype = C.c_char_p
@@ -267,6 +272,7 @@ class Result:
     n_tracklets: np.ndarray | None = None
     records: np.ndarray | None = None  # float32 (n_events, max_tracklets_per_event, 148)
     vertex: np.ndarray | None = None   # float32 (n_events, max_tracks_out, 6): vx, vy, vz, px, py, pz
+    fit: np.ndarray | None = None      # float32 (n_events, max_tracks_out, 8): final fit (FIT_FIELDS)
 
     def valid_tracks(self) -> np.ndarray:
         """Tracks of all events, slots [0, n_tracks[e]) of each event in event
@@ -307,6 +313,11 @@ class Session:
     def set_tuning(self, key: int, value: int):
         """or_session_set_tuning; key e.g. TUNE_YZ_LIST_CAP."""
         _check(self.lib.or_session_set_tuning(self.handle, key, value))
+
+    def set_fit_cond_bound(self, bound: float):
+        """or_session_set_fit_cond_bound: condition bound of the final fit's FP32 solve
+        (0 sends every track to the FP64 path; per device, shared by its sessions)."""
+        _check(self.lib.or_session_set_fit_cond_bound(self.handle, float(bound)))
 
     # --- learned YZ unit pruning -----------------------------------------------
     def set_pruner(self, weights: np.ndarray | None, threshold: float = 0.0):
@@ -352,7 +363,7 @@ class Session:
         return t
 
     def alloc_outputs(self, n_events: int, n_hits: int, tracklets: bool = False, records: bool = False,
-                      vertex: bool = False):
+                      vertex: bool = False, fit: bool = False):
         import torch
         dev = torch.device("cuda", self.device)
         p = self.params
@@ -366,6 +377,8 @@ class Session:
                                     device=dev) if records else None),
             "vertex": (torch.empty(max(1, n_events * p.max_tracks_out * 6), dtype=torch.float32, device=dev)
                        if vertex else None),
+            "fit": (torch.empty(max(1, n_events * p.max_tracks_out * len(FIT_FIELDS)), dtype=torch.float32,
+                                device=dev) if fit else None),
         }
         return o
 
@@ -379,7 +392,8 @@ class Session:
         res = OrResult(o["status"].data_ptr(), o["keep_mask"].data_ptr(), o["n_tracks"].data_ptr(),
                        o["tracks"].data_ptr(), None if o["n_tracklets"] is None else o["n_tracklets"].data_ptr(),
                        None if o["records"] is None else o["records"].data_ptr(),
-                       None if o.get("vertex") is None else o["vertex"].data_ptr(), 1)
+                       None if o.get("vertex") is None else o["vertex"].data_ptr(),
+                       None if o.get("fit") is None else o["fit"].data_ptr(), 1)
         return hits, res
 
     def run_device(self, t, o):
@@ -407,18 +421,22 @@ class Session:
                      o["records"].cpu().numpy().reshape(n_events, max_tkl, TRACK_PARAMS)),
             vertex=(None if o.get("vertex") is None else
                     o["vertex"].cpu().numpy()[: n_events * max_tracks_out * 6].reshape(n_events, max_tracks_out, 6)),
+            fit=(None if o.get("fit") is None else
+                 o["fit"].cpu().numpy()[: n_events * max_tracks_out * len(FIT_FIELDS)].reshape(
+                     n_events, max_tracks_out, len(FIT_FIELDS))),
         )
 
-    def reconstruct(self, ev: Events, tracklets: bool = True, records: bool = False, vertex: bool = False) -> Result:
+    def reconstruct(self, ev: Events, tracklets: bool = True, records: bool = False, vertex: bool = False,
+                    fit: bool = False) -> Result:
         """Uploads, runs on the device, downloads."""
         import torch
         t = self.upload(ev)
-        o = self.alloc_outputs(ev.n_events, ev.n_hits, tracklets, records, vertex)
+        o = self.alloc_outputs(ev.n_events, ev.n_hits, tracklets, records, vertex, fit)
         self.run_device(t, o)
         torch.cuda.synchronize(self.device)
         return self.fetch(o, ev.n_events, self.params.max_tracks_out, self.params.max_tracklets_per_event)
 
-    def reconstruct_host(self, ev: Events, records: bool = False, vertex: bool = False) -> Result:
+    def reconstruct_host(self, ev: Events, records: bool = False, vertex: bool = False, fit: bool = False) -> Result:
         """Host buffers in and out; the library stages through device memory."""
         p = self.params
         ne, nh = ev.n_events, ev.n_hits
@@ -429,11 +447,13 @@ class Session:
         ntkl = np.zeros(ne, np.int32)
         rec = np.zeros((ne, p.max_tracklets_per_event, TRACK_PARAMS), np.float32) if records else None
         vtx = np.zeros((ne, p.max_tracks_out, 6), np.float32) if vertex else None
+        fitv = np.zeros((ne, p.max_tracks_out, len(FIT_FIELDS)), np.float32) if fit else None
         cont = [np.ascontiguousarray(a) for a in (ev.det, ev.elem, ev.tdc, ev.drift, ev.offsets)]
         flag = None if ev.flag is None else np.ascontiguousarray(ev.flag)
         hits = OrHits(ne, nh, cont[0].ctypes.data, cont[1].ctypes.data, cont[2].ctypes.data, cont[3].ctypes.data,
                       None if flag is None else flag.ctypes.data, cont[4].ctypes.data, 0)
         res = OrResult(status.ctypes.data, keep.ctypes.data, ntr.ctypes.data, tracks.ctypes.data, ntkl.ctypes.data,
-                       None if rec is None else rec.ctypes.data, None if vtx is None else vtx.ctypes.data, 0)
+                       None if rec is None else rec.ctypes.data, None if vtx is None else vtx.ctypes.data,
+                       None if fitv is None else fitv.ctypes.data, 0)
         _check(self.lib.or_reconstruct(self.handle, C.byref(hits), C.byref(res)))
-        return Result(status, keep, ntr, tracks, ntkl, rec, vtx)
+        return Result(status, keep, ntr, tracks, ntkl, rec, vtx, fitv)

@@ -136,8 +136,30 @@ typedef struct {
                                 when set, global tracks are swum to the beam line (stationID 7,
                                 gKernel_Vertexing, reconstruction_kernels.cuh:2547-2810) and records
                                 carry the same values in fields 16-21 */
+    float* fit;              /* optional, n_events * max_tracks_out * 8 floats per output track: final
+                                straight-line fit over all its hits (x0, tx, y0, ty, chi2, ndf, condition
+                                estimate, 1 when the 4x4 solve ran in FP64); see OR_FIT_* below.
+                                No reference counterpart (the reference's 4x4 inverse,
+                                trackfittingtools.cuh:84-349, is never called) */
     int32_t on_device;       /* 1: device pointers */
 } or_result;
+
+/* Final fit of an output track: measurement u = wire position + sign * drift of
+ * each hit, model u = cos(theta) (x0 + tx z) + sin(theta) (y0 + ty z) of its
+ * plane, weight 1/resolution^2, least squares over the 4 parameters.  The
+ * per-plane terms of the normal equations (w h h^T, w h and h, with
+ * h = (cos, cos z', sin, sin z') and z' = z - 1000 cm) are computed once per
+ * session in double and held as pairs of floats; per hit they are summed in
+ * FP32 with compensation (Neumaier), and the u terms are formed with
+ * error-free products.  The equilibrated 4x4 system is factorised (Cholesky)
+ * and solved in FP32 with one refinement step against the compensated
+ * residual, then extrapolated to z = 0 with error-free products; chi2 sums
+ * the residuals in double-float.  When the condition estimate (max/min
+ * Cholesky pivot)^2 exceeds the bound (OR_FIT_COND_FP32 unless
+ * or_session_set_fit_cond_bound changed it) or a pivot is not positive, the
+ * solve and chi2 run in FP64 from the same sums.  Fields per track: */
+#define OR_FIT_FIELDS 8          /* x0, tx, y0, ty, chi2, ndf (hits - 4), condition estimate, FP64 flag */
+#define OR_FIT_COND_FP32 1.0e4f
 
 /* ---- entry points ---------------------------------------------------------- */
 
@@ -246,6 +268,12 @@ or_status or_debug_stage_cycles(unsigned long long* out /* 35 */, int reset);
  * station-1 window lists, station-1 candidate loop, track write; 16 slots.
  * Same build rule as or_debug_stage_cycles.  No reference counterpart. */
 or_status or_debug_sub_cycles(unsigned long long* out /* 16 */, int reset);
+
+/* Sets the condition bound above which the final fit solves in FP64 (0 sends
+ * every track to the FP64 path).  The bound is held per device and shared by
+ * every session on the session's device; it applies to launches made after the
+ * call.  No reference counterpart. */
+or_status or_session_set_fit_cond_bound(or_session* s, float bound);
 
 void or_session_destroy(or_session* s);
 const char* or_status_string(or_status st);

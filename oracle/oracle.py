@@ -101,6 +101,6 @@ def reconstruct(geo: OracleGeometry, params, ev, records: bool = False, threads:
     hits = oc.OrHits(ne, nh, arrs[0].ctypes.data, arrs[1].ctypes.data, arrs[2].ctypes.data, arrs[3].ctypes.data,
                      None if flag is None else flag.ctypes.data, arrs[4].ctypes.data, 0)
     res = oc.OrResult(status.ctypes.data, keep.ctypes.data, ntr.ctypes.data, tracks.ctypes.data, ntkl.ctypes.data,
-                      None if rec is None else rec.ctypes.data, None if vtx is None else vtx.ctypes.data, 0)
+                      None if rec is None else rec.ctypes.data, None if vtx is None else vtx.ctypes.data, None, 0)
     load().orc_reconstruct(geo.handle, C.byref(params), C.byref(hits), C.byref(res), int(threads))
     return oc.Result(status, keep, ntr, tracks, ntkl, rec, vtx)
