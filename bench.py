@@ -154,6 +154,25 @@ def main():
         ", ".join(f"{q} max {fit_rep[q]['max_rel']:.1e}" for q in fitcheck.QUANTITIES) +
         f"; FP64 solves {fit_rep['fp64_fraction'] * 100:.2f} %")
 
+    # R-T calibration of every hit of the batch (or_rt_calibrate, device-resident):
+    # 2 B det + 4 B tdc + 4 B drift read, 4 B drift written per hit
+    sess.set_calibration(oc.Calibration.load())
+    rt_out = sess.rt_calibrate_device(dev_in)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 5
+    e0.record()
+    for _ in range(reps):
+        rt_out = sess.rt_calibrate_device(dev_in)
+    e1.record()
+    torch.cuda.synchronize()
+    sess.set_calibration(None)
+    rt_ms = e0.elapsed_time(e1) / reps
+    rt_gbs = 14.0 * ev.n_hits / (rt_ms * 1e-3) / 1e9
+    del rt_out
+    log(f"R-T calibration of {ev.n_hits} hits: {rt_ms:.3f} ms, {rt_gbs:.0f} GB/s algorithmic "
+        f"({rt_gbs / HBM_PEAK_GBS * 100:.1f} % of HBM peak)")
+
     # end to end through host buffers (PCIe both ways) on a slice, for reference only
     e2e = None
     if not args.no_e2e:
@@ -230,6 +249,8 @@ def main():
             "prune_threshold": prune["threshold"] if prune else None,
             "host_buffer_events_per_s": e2e,
             "final_fit_vs_fp64": dict(fit_rep, sample=f"output tracks of the first {n_fit} events, plain chain"),
+            "rt_calibrate": {"hits": int(ev.n_hits), "ms": rt_ms, "algorithmic_GB_per_s": rt_gbs,
+                             "frac_of_hbm_peak": rt_gbs / HBM_PEAK_GBS, "bytes_per_hit": 14},
         },
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},

@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <vector>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -28,6 +29,50 @@ constexpr int FITP_M = 0, FITP_V = 20, FITP_H = 28, FITP_W = 36, FITP_N = 37;
 constexpr double FIT_ZREF = 1000.0;
 __constant__ float c_fitp[ND][FITP_N];
 __constant__ float c_fit_cond = OR_FIT_COND_FP32;   // FP32 / FP64 switch of the final fit
+
+// R-T calibration on the device: per detector n knots x[off .. off+n) and the
+// cubic (y, b, c, d) of each knot interval at cf[off + k]; n = 0: not applied.
+struct RtDev {
+    int32_t n[ND], off[ND];
+    float tmin[ND], tmax[ND];
+    const double* x;
+    const double4* cf;
+};
+
+// Drift of one hit (online_reconstruction.cu:389-397): tdc < tmin -> cellwidth/2,
+// tdc > tmax -> 0, else the spline in FP64.  Knot interval k: t_k < t <= t_k+1,
+// the end intervals below the first and above the last knot (TSpline3's
+// FindX / Eval choice).
+__device__ __forceinline__ float rt_drift(const RtDev* R, int d, float tdc, float drift) {
+    const int n = R->n[d];
+    if (n <= 0) return drift;
+    if (tdc < R->tmin[d]) return c_geo[F_CELLWIDTH][d] * 0.5f;
+    if (tdc > R->tmax[d]) return 0.f;
+    const double t = (double)tdc;
+    const double* X = R->x + R->off[d];
+    int k = 0;
+    if (t >= X[n - 1]) {
+        k = n - 2;
+    } else if (t > X[0]) {
+        int hi = n - 1;
+        while (hi - k > 1) {
+            const int m = (k + hi) >> 1;
+            if (t > X[m]) k = m; else hi = m;
+        }
+    }
+    const double4 c = R->cf[R->off[d] + k];
+    const double dt = t - X[k];
+    return (float)(c.x + dt * (c.y + dt * (c.z + dt * c.w)));
+}
+
+__global__ void rt_calibrate_kernel(const RtDev* R, const int16_t* det, const float* tdc, const float* drift,
+                                    int64_t n, float* out) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int d = det[i];
+        const float dr = drift[i];
+        out[i] = (d >= 1 && d < ND) ? rt_drift(R, d, tdc[i], dr) : dr;
+    }
+}
 }
 
 using namespace ordev;
@@ -119,6 +164,7 @@ struct Args {
     const int32_t* ev_count;    // device count of ev_list
     int32_t* ev_head;           // work queue head (events handed out beyond the first gridDim.x), zeroed per launch
     const float* wire;          // 55 x OR_MAX_ELEM
+    const RtDev* rt;            // R-T calibration applied at load (NULL: input drift)
     or_params p;
     uint32_t* status;
     uint32_t* keep_mask;
@@ -731,7 +777,8 @@ STAGE bool stage_load_reduce(const Args& a, int64_t h0, int n) {
                 if (v) {
                     L.elem[dest] = (int16_t)el;
                     L.pos[dest] = a.wire[d * OR_MAX_ELEM + el];
-                    L.drift[dest] = fabsf(a.drift[h0 + h]);
+                    const float dr = a.drift[h0 + h];
+                    L.drift[dest] = fabsf(a.rt ? rt_drift(a.rt, d, a.tdc[h0 + h], dr) : dr);
                     L.orig[dest] = (uint16_t)h;
                     const int fl = a.flag ? (int)a.flag[h0 + h] : 2;
                     L.keep[dest] = (fl & 2) ? 1 : -1;
@@ -2853,8 +2900,11 @@ struct or_session {
     float* dump = nullptr;           // unit rows (training aid)
     int64_t dump_cap = 0;
     unsigned long long* dump_count = nullptr;
+    RtDev* d_rt = nullptr;           // R-T calibration table (or_session_set_calibration)
+    size_t d_rt_bytes = 0;
+    bool rt_on = false;
     struct GraphKey {
-        const void* p[18];
+        const void* p[19];
         int64_t n_events, n_hits, dump_cap;
         hipStream_t stream;
         int yz_list_cap, grid_pct;
@@ -3033,6 +3083,7 @@ static or_status issue_chain(or_session* s, const or_hits* in, const or_result* 
     a.scratch_stride = SCR_STRIDE;
     a.yz_list_cap = s->yz_list_cap;
     a.pruner = s->prune_on ? s->d_pruner : nullptr;
+    a.rt = s->rt_on ? s->d_rt : nullptr;
     a.unit_dump = s->dump;
     a.dump_cap = s->dump_cap;
     a.dump_count = s->dump_count;
@@ -3083,10 +3134,10 @@ static or_status run_device(or_session* s, const or_hits* in, const or_result* o
     } else {
         or_session::GraphKey key;
         std::memset(&key, 0, sizeof key);
-        const void* ptrs[18] = {in->det, in->elem, in->tdc, in->drift, in->flag, in->event_offsets, out->status,
+        const void* ptrs[19] = {in->det, in->elem, in->tdc, in->drift, in->flag, in->event_offsets, out->status,
                                 out->keep_mask, out->n_tracks, out->tracks, out->n_tracklets, out->records,
                                 out->vertex, out->fit, s->d_lists, s->prune_on ? s->d_pruner : nullptr, s->dump,
-                                s->dump_count};
+                                s->dump_count, s->rt_on ? s->d_rt : nullptr};
         std::memcpy(key.p, ptrs, sizeof ptrs);
         key.n_events = in->n_events; key.n_hits = in->n_hits; key.dump_cap = s->dump_cap; key.stream = s->stream;
         key.yz_list_cap = s->yz_list_cap; key.grid_pct = s->grid_pct;
@@ -3212,6 +3263,86 @@ or_status or_session_set_pruner(or_session* s, const float* weights, int64_t n_w
     return OR_OK;
 }
 
+or_status or_session_set_calibration(or_session* s, const or_calibration* c) {
+    if (!s) return OR_ERR_INVALID_ARG;
+    if (!c) {
+        s->rt_on = false;
+        return OR_OK;
+    }
+    // image: RtDev, then the knots (double), then the interval cubics (double4)
+    int64_t nk = 0;
+    for (int d = 0; d < OR_N_DETECTORS; ++d) nk += c->det[d].n;
+    const size_t head = (sizeof(RtDev) + 255) / 256 * 256;
+    const size_t xb = (size_t)std::max<int64_t>(nk, 1) * sizeof(double);
+    const size_t bytes = head + (xb + 31) / 32 * 32 + (size_t)std::max<int64_t>(nk, 1) * sizeof(double4);
+    std::vector<char> img(bytes, 0);
+    OR_HIP(hipSetDevice(s->device));
+    if (s->d_rt_bytes < bytes) {
+        OR_HIP(hipStreamSynchronize(s->stream));
+        if (s->d_rt) OR_HIP(hipFree(s->d_rt));
+        s->d_rt = nullptr;
+        s->d_rt_bytes = 0;
+        s->rt_on = false;
+        OR_HIP(hipMalloc(&s->d_rt, bytes));
+        s->d_rt_bytes = bytes;
+    }
+    char* base = reinterpret_cast<char*>(s->d_rt);
+    RtDev* h = reinterpret_cast<RtDev*>(img.data());
+    double* hx = reinterpret_cast<double*>(img.data() + head);
+    double4* hc = reinterpret_cast<double4*>(img.data() + head + (xb + 31) / 32 * 32);
+    h->x = reinterpret_cast<const double*>(base + head);
+    h->cf = reinterpret_cast<const double4*>(base + head + (xb + 31) / 32 * 32);
+    int64_t o = 0;
+    for (int d = 0; d < OR_N_DETECTORS; ++d) {
+        const or_calibration::Det& D = c->det[d];
+        h->n[d] = D.n; h->off[d] = (int32_t)o; h->tmin[d] = D.tmin; h->tmax[d] = D.tmax;
+        for (int k = 0; k < D.n; ++k) {
+            hx[o + k] = D.x[k];
+            hc[o + k] = k + 1 < D.n ? double4{D.y[k], D.b[k], D.c[k], D.d[k]} : double4{0.0, 0.0, 0.0, 0.0};
+        }
+        o += D.n;
+    }
+    OR_HIP(hipStreamSynchronize(s->stream));
+    OR_HIP(hipMemcpy(s->d_rt, img.data(), bytes, hipMemcpyHostToDevice));
+    s->rt_on = true;
+    return OR_OK;
+}
+
+or_status or_rt_calibrate(or_session* s, const or_hits* in, float* drift_out) {
+    if (!s || !in || !s->rt_on || in->n_hits < 0) return OR_ERR_INVALID_ARG;
+    if (in->n_hits == 0) return OR_OK;
+    if (!in->det || !in->tdc || !in->drift || !drift_out) return OR_ERR_INVALID_ARG;
+    OR_HIP(hipSetDevice(s->device));
+    const int64_t n = in->n_hits;
+    const unsigned grid = (unsigned)std::min<int64_t>(4096, (n + 255) / 256);
+    if (in->on_device) {
+        hipLaunchKernelGGL(rt_calibrate_kernel, dim3(grid), dim3(256), 0, s->stream, s->d_rt, in->det, in->tdc,
+                           in->drift, n, drift_out);
+        OR_HIP(hipGetLastError());
+        return OR_OK;
+    }
+    char* buf = nullptr;
+    const size_t b2 = ((size_t)n * 2 + 255) / 256 * 256, b4 = ((size_t)n * 4 + 255) / 256 * 256;
+    OR_HIP(hipMalloc(&buf, b2 + 3 * b4));
+    int16_t* dd = reinterpret_cast<int16_t*>(buf);
+    float* dt = reinterpret_cast<float*>(buf + b2);
+    float* dr = reinterpret_cast<float*>(buf + b2 + b4);
+    float* dout = reinterpret_cast<float*>(buf + b2 + 2 * b4);
+    hipError_t e = hipMemcpyAsync(dd, in->det, (size_t)n * 2, hipMemcpyHostToDevice, s->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dt, in->tdc, (size_t)n * 4, hipMemcpyHostToDevice, s->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dr, in->drift, (size_t)n * 4, hipMemcpyHostToDevice, s->stream);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(rt_calibrate_kernel, dim3(grid), dim3(256), 0, s->stream, s->d_rt, dd, dt, dr, n, dout);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(drift_out, dout, (size_t)n * 4, hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    const hipError_t ef = hipFree(buf);
+    OR_HIP(e);
+    OR_HIP(ef);
+    return OR_OK;
+}
+
 or_status or_session_set_fit_cond_bound(or_session* s, float bound) {
     if (!s || !(bound >= 0.f)) return OR_ERR_INVALID_ARG;
     OR_HIP(hipSetDevice(s->device));
@@ -3272,6 +3403,7 @@ void or_session_destroy(or_session* s) {
     if (s->d_lists) (void)hipFree(s->d_lists);
     if (s->d_counts) (void)hipFree(s->d_counts);
     if (s->d_pruner) (void)hipFree(s->d_pruner);
+    if (s->d_rt) (void)hipFree(s->d_rt);
     if (s->d_in) (void)hipFree(s->d_in);
     if (s->d_out) (void)hipFree(s->d_out);
     if (s->graph_exec) (void)hipGraphExecDestroy(s->graph_exec);

@@ -3,6 +3,7 @@
 #include "../../include/e1039_or.h"
 
 #include <cstdint>
+#include <vector>
 
 // Per-plane geometry fields, in the member order of the reference's gPlane
 // (reconstruction_classes.cuh:947-977).  nelem is held as a float.
@@ -16,4 +17,15 @@ struct or_geometry {
     float f[F_COUNT][OR_N_DETECTORS];          // plane table
     float wire[OR_N_DETECTORS][OR_MAX_ELEM];   // wire position per (detector, element), fp32
     float deltaW[OR_N_DETECTORS][9];           // alignment offsets read from the file
+};
+
+// R-T calibration: per detector the knots and the cubic of each knot interval,
+// value(t) = y + dt (b + dt (c + dt d)) with dt = t - x[k] (n - 1 intervals).
+struct or_calibration {
+    struct Det {
+        int n = 0;                          // knots (0: not calibrated)
+        float tmin = 0.f, tmax = 0.f;
+        std::vector<double> x, y, b, c, d;  // x: n knots; y..d: n - 1 intervals
+    };
+    Det det[OR_N_DETECTORS];
 };

@@ -51,6 +51,8 @@ PRUNE_WEIGHTS = 5313       # OR_PRUNE_WEIGHTS: w1[64][16], b1[64], w2[64][64], b
 PRUNE_FEATURES = 16
 UNIT_ROW = 20              # OR_UNIT_ROW: event, tracklet_id, unit, winner flag, 16 features
 DEFAULT_PRUNER = os.path.join(HERE, "data", "pruner_u3v3.npz")
+DEFAULT_CALIBRATION = os.path.join(HERE, "data", "calibration_e906.txt")
+RT_MAX_KNOTS = 512                                                      # OR_RT_MAX_KNOTS
 FIT_FIELDS = ["x0", "tx", "y0", "ty", "chi2", "ndf", "cond", "fp64"]   # OR_FIT_FIELDS per output track
 FIT_COND_FP32 = 1.0e4                                                   # OR_FIT_COND_FP32
 
@@ -149,6 +151,15 @@ def load_library(path: str = LIB_PATH):
     lib.or_pruner_score.restype = i32
     lib.or_session_set_unit_dump.argtypes = [vp, vp, i64, vp]
     lib.or_session_set_unit_dump.restype = i32
+    lib.or_calibration_load.argtypes = [C.c_char_p, C.POINTER(vp)]
+    lib.or_calibration_load.restype = i32
+    lib.or_calibration_create.argtypes = [C.c_int, vp, vp, vp, vp, vp, vp, C.POINTER(vp)]
+    lib.or_calibration_create.restype = i32
+    lib.or_calibration_free.argtypes = [vp]
+    lib.or_session_set_calibration.argtypes = [vp, vp]
+    lib.or_session_set_calibration.restype = i32
+    lib.or_rt_calibrate.argtypes = [vp, C.POINTER(OrHits), vp]
+    lib.or_rt_calibrate.restype = i32
     lib.or_session_set_fit_cond_bound.argtypes = [vp, C.c_float]
     lib.or_session_set_fit_cond_bound.restype = i32
     lib.or_session_destroy.argtypes = [vp]
@@ -200,6 +211,40 @@ class Geometry:
     def __del__(self):
         if getattr(self, "handle", None) and _lib is not None:
             _lib.or_geometry_free(self.handle)
+            self.handle = None
+
+
+class Calibration:
+    """R-T calibration (or_calibration_load / or_calibration_create)."""
+
+    def __init__(self, handle):
+        self.handle = handle
+
+    @classmethod
+    def load(cls, path: str = DEFAULT_CALIBRATION) -> "Calibration":
+        h = C.c_void_p()
+        _check(load_library().or_calibration_load(path.encode(), C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def create(cls, tables: dict) -> "Calibration":
+        """tables: detector id -> (tmin, tmax, knot times, values)."""
+        ids = np.array(sorted(tables), np.int32)
+        nk = np.array([len(tables[d][2]) for d in ids], np.int32)
+        tmin = np.array([tables[d][0] for d in ids], np.float32)
+        tmax = np.array([tables[d][1] for d in ids], np.float32)
+        t = np.ascontiguousarray(np.concatenate([np.asarray(tables[d][2], np.float64) for d in ids])
+                                 if len(ids) else np.zeros(1), np.float64)
+        r = np.ascontiguousarray(np.concatenate([np.asarray(tables[d][3], np.float64) for d in ids])
+                                 if len(ids) else np.zeros(1), np.float64)
+        h = C.c_void_p()
+        _check(load_library().or_calibration_create(len(ids), ids.ctypes.data, nk.ctypes.data, tmin.ctypes.data,
+                                                    tmax.ctypes.data, t.ctypes.data, r.ctypes.data, C.byref(h)))
+        return cls(h)
+
+    def __del__(self):
+        if getattr(self, "handle", None) and _lib is not None:
+            _lib.or_calibration_free(self.handle)
             self.handle = None
 
 
@@ -313,6 +358,30 @@ class Session:
     def set_tuning(self, key: int, value: int):
         """or_session_set_tuning; key e.g. TUNE_YZ_LIST_CAP."""
         _check(self.lib.or_session_set_tuning(self.handle, key, value))
+
+    def set_calibration(self, cal: "Calibration | None"):
+        """or_session_set_calibration: the chain derives drift from tdc for calibrated detectors (None: off)."""
+        _check(self.lib.or_session_set_calibration(self.handle, None if cal is None else cal.handle))
+
+    def rt_calibrate(self, ev: "Events") -> np.ndarray:
+        """or_rt_calibrate on host buffers: the drift of every hit under the session's calibration."""
+        cont = [np.ascontiguousarray(a) for a in (ev.det, ev.elem, ev.tdc, ev.drift, ev.offsets)]
+        hits = OrHits(ev.n_events, ev.n_hits, cont[0].ctypes.data, cont[1].ctypes.data, cont[2].ctypes.data,
+                      cont[3].ctypes.data, None, cont[4].ctypes.data, 0)
+        out = np.zeros(ev.n_hits, np.float32)
+        _check(self.lib.or_rt_calibrate(self.handle, C.byref(hits), out.ctypes.data))
+        return out
+
+    def rt_calibrate_device(self, t):
+        """or_rt_calibrate on a device-resident batch (upload()); returns a device tensor (asynchronous)."""
+        import torch
+        stream = torch.cuda.current_stream(self.device)
+        _check(self.lib.or_session_set_stream(self.handle, C.c_void_p(stream.cuda_stream)))
+        hits = OrHits(t["n_events"], t["n_hits"], t["det"].data_ptr(), t["elem"].data_ptr(), t["tdc"].data_ptr(),
+                      t["drift"].data_ptr(), None, t["offsets"].data_ptr(), 1)
+        out = torch.empty(max(1, t["n_hits"]), dtype=torch.float32, device=t["det"].device)
+        _check(self.lib.or_rt_calibrate(self.handle, C.byref(hits), C.c_void_p(out.data_ptr())))
+        return out[: t["n_hits"]]
 
     def set_fit_cond_bound(self, bound: float):
         """or_session_set_fit_cond_bound: condition bound of the final fit's FP32 solve

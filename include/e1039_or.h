@@ -224,6 +224,47 @@ or_status or_session_set_tuning(or_session* s, or_tuning key, int64_t value);
  * counterpart. */
 or_status or_session_autotune(or_session* s, const or_hits* in, or_result* out, int reps, float* best_ms);
 
+/* ---- R-T calibration: drift distance from the TDC time ---------------------
+ * Per detector a cubic spline R(T) through the calibration knots with zero
+ * slope at both ends (ROOT TSpline3 option "b1e1" as online_reconstruction.cu:240
+ * builds it), and the rule of online_reconstruction.cu:389-397 for chambers
+ * (1-30) and prop tubes (47-54): tdc < tmin -> cellwidth/2, tdc > tmax -> 0,
+ * else R(tdc), evaluated in FP64 and rounded to float.  Hodoscopes (31-46) and
+ * detectors without knots keep the input drift.  Interval choice as TSpline3's:
+ * knot interval k with t_k < tdc <= t_k+1; below the first knot the first
+ * interval's cubic, above the last knot the last interval's cubic (the shipped
+ * prop-tube calibration has knots 562.5..1197.5 inside tmin, tmax = 560, 1200). */
+typedef struct or_calibration or_calibration;
+#define OR_RT_MAX_KNOTS 512
+
+/* Parses a calibration_*.txt file: per detector a line "detectorID nBin tmin
+ * tmax name" followed by nBin lines "iBin T R".  Replaces
+ * online_reconstruction.cu:220-242. */
+or_status or_calibration_load(const char* path, or_calibration** out);
+
+/* Builds a calibration from tables: n_det detectors, det_ids[i] in 1..54,
+ * n_knots[i] in {0} or 2..OR_RT_MAX_KNOTS knots taken in order from t and r
+ * (strictly increasing t per detector), tmin/tmax per detector.  Entries for
+ * hodoscopes are accepted and not applied (as in the reference). */
+or_status or_calibration_create(int n_det, const int32_t* det_ids, const int32_t* n_knots, const float* tmin,
+                                const float* tmax, const double* t, const double* r, or_calibration** out);
+void or_calibration_free(or_calibration* c);
+
+/* Uploads c to the session's device; from the next or_reconstruct the chain
+ * derives the drift of calibrated detectors from in->tdc inside its event load
+ * (the input drift is read for the others).  c == NULL turns it off.  The
+ * session keeps its own copy; c may be freed afterwards.  Waits for the
+ * session stream. */
+or_status or_session_set_calibration(or_session* s, const or_calibration* c);
+
+/* Writes the drift of every input hit to drift_out (n_hits floats): the R-T
+ * rule for calibrated detectors, the input drift for the others, without the
+ * absolute value the chain takes.  Needs a calibration set on the session.
+ * With in->on_device, device pointers and asynchronous on the session stream;
+ * otherwise host pointers, returning when drift_out is written.  Replaces the
+ * per-hit drift_distance computation of online_reconstruction.cu:387-397. */
+or_status or_rt_calibrate(or_session* s, const or_hits* in, float* drift_out);
+
 /* ---- learned pruning of YZ station-3 units (no reference counterpart) -----
  * A YZ unit is one (u3, v3) window-list pair of a tracklet: the reference
  * evaluates every (u2, v2) pair under every unit (the i_uv3 loop,

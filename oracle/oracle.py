@@ -56,6 +56,9 @@ def load():
                                      f32, f32, f32, f32, f32, f32]
     lib.orc_prop_segment.restype = C.c_int
     lib.orc_global_cleaning.argtypes = [C.c_int, vp, vp, vp, vp, vp, vp]
+    lib.orc_rt_spline.argtypes = [C.c_int, vp, vp, vp, vp, vp]
+    lib.orc_rt_eval.argtypes = [C.c_int, vp, vp, C.c_int64, vp, vp]
+    lib.orc_rt_calibrate.argtypes = [vp, vp, vp, vp, vp, vp, C.c_int64, vp, vp, vp, vp]
     _lib = lib
     return lib
 
@@ -104,3 +107,82 @@ def reconstruct(geo: OracleGeometry, params, ev, records: bool = False, threads:
                       None if rec is None else rec.ctypes.data, None if vtx is None else vtx.ctypes.data, None, 0)
     load().orc_reconstruct(geo.handle, C.byref(params), C.byref(hits), C.byref(res), int(threads))
     return oc.Result(status, keep, ntr, tracks, ntkl, rec, vtx)
+
+
+# ---- R-T calibration (online_reconstruction.cu:220-242, 387-397) -------------
+
+class OracleCalibration:
+    """Per detector (tmin, tmax, T knots, R values) read from a calibration file
+    as online_reconstruction.cu:226-242 reads it: a header line "detectorID nBin
+    tmin tmax name", then nBin lines "iBin T R"; tmin/tmax kept as float32."""
+
+    def __init__(self, tables: dict):
+        self.tables = tables
+
+    @classmethod
+    def load(cls, path: str):
+        tables = {}
+        with open(path) as f:
+            lines = f.read().split("\n")
+        i = 0
+        while i < len(lines):
+            head = lines[i].split()
+            i += 1
+            if not head:
+                continue
+            d, n, t0, t1 = int(head[0]), int(head[1]), float(head[2]), float(head[3])
+            rows = [lines[i + k].split() for k in range(n)]
+            i += n
+            tables[d] = (np.float32(t0), np.float32(t1), np.array([float(r[1]) for r in rows]),
+                         np.array([float(r[2]) for r in rows]))
+        return cls(tables)
+
+    def flat(self):
+        """nk[55], tmin[55], tmax[55], concatenated knots and values."""
+        nk = np.zeros(55, np.int32)
+        tmin = np.zeros(55, np.float32)
+        tmax = np.zeros(55, np.float32)
+        ts, rs = [], []
+        for d in range(55):
+            if d in self.tables:
+                t0, t1, T, R = self.tables[d]
+                nk[d], tmin[d], tmax[d] = len(T), t0, t1
+                ts.append(T)
+                rs.append(R)
+        t = np.ascontiguousarray(np.concatenate(ts) if ts else np.zeros(1), np.float64)
+        r = np.ascontiguousarray(np.concatenate(rs) if rs else np.zeros(1), np.float64)
+        return nk, tmin, tmax, t, r
+
+
+def rt_spline(T, R):
+    """Coefficients (b, c, d) of the n-1 knot intervals of the "b1e1" spline."""
+    T = np.ascontiguousarray(T, np.float64)
+    R = np.ascontiguousarray(R, np.float64)
+    n = len(T)
+    b, c, d = (np.zeros(n - 1) for _ in range(3))
+    load().orc_rt_spline(n, T.ctypes.data, R.ctypes.data, b.ctypes.data, c.ctypes.data, d.ctypes.data)
+    return b, c, d
+
+
+def rt_eval(T, R, t):
+    """The "b1e1" spline through (T, R) at the points t (TSpline3::Eval's interval choice)."""
+    T = np.ascontiguousarray(T, np.float64)
+    R = np.ascontiguousarray(R, np.float64)
+    t = np.ascontiguousarray(t, np.float64)
+    out = np.zeros(len(t))
+    load().orc_rt_eval(len(T), T.ctypes.data, R.ctypes.data, len(t), t.ctypes.data, out.ctypes.data)
+    return out
+
+
+def rt_calibrate(cal: OracleCalibration, cellwidth, det, tdc, drift):
+    """Drift of every hit by the rule of online_reconstruction.cu:387-397."""
+    nk, tmin, tmax, t, r = cal.flat()
+    cw = np.ascontiguousarray(cellwidth, np.float32)
+    det = np.ascontiguousarray(det, np.int16)
+    tdc = np.ascontiguousarray(tdc, np.float32)
+    drift = np.ascontiguousarray(drift, np.float32)
+    out = np.zeros(len(det), np.float32)
+    load().orc_rt_calibrate(nk.ctypes.data, tmin.ctypes.data, tmax.ctypes.data, t.ctypes.data, r.ctypes.data,
+                            cw.ctypes.data, len(det), det.ctypes.data, tdc.ctypes.data, drift.ctypes.data,
+                            out.ctypes.data)
+    return out
