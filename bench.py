@@ -173,6 +173,30 @@ def main():
     log(f"R-T calibration of {ev.n_hits} hits: {rt_ms:.3f} ms, {rt_gbs:.0f} GB/s algorithmic "
         f"({rt_gbs / HBM_PEAK_GBS * 100:.1f} % of HBM peak)")
 
+    # dimuons and display histograms (or_dimuons) after the full chain with vertexing, on a slice
+    n_dim_ev = min(ev.n_events, 200_000)
+    dsub = ev.subset(0, n_dim_ev)
+    fparams = oc.default_params(max_hits_per_event=1024)
+    fparams.full_chain = 1
+    fsess = oc.Session(geo, fparams, device=0)
+    ft = fsess.upload(dsub)
+    fo = fsess.alloc_outputs(dsub.n_events, dsub.n_hits, vertex=True)
+    fsess.run_device(ft, fo)
+    dd = fsess.alloc_dimuons(dsub.n_events, histograms=True)
+    fsess.run_dimuons(ft, fo, dd)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        fsess.run_dimuons(ft, fo, dd)
+    e1.record()
+    torch.cuda.synchronize()
+    dim_ms = e0.elapsed_time(e1) / reps
+    dres = oc.Session.fetch_dimuons(dd, dsub.n_events)
+    n_dimuons = int(dres.n_dimuons.sum())
+    log(f"dimuons on {n_dim_ev} full-chain events: {n_dimuons} dimuons, {dim_ms:.3f} ms "
+        f"({n_dim_ev / (dim_ms * 1e-3):.3g} events/s)")
+    del fsess, ft, fo, dd
+
     # end to end through host buffers (PCIe both ways) on a slice, for reference only
     e2e = None
     if not args.no_e2e:
@@ -249,6 +273,9 @@ def main():
             "prune_threshold": prune["threshold"] if prune else None,
             "host_buffer_events_per_s": e2e,
             "final_fit_vs_fp64": dict(fit_rep, sample=f"output tracks of the first {n_fit} events, plain chain"),
+            "dimuons": {"events": n_dim_ev, "dimuons": n_dimuons, "kernel_ms": dim_ms,
+                        "events_per_s": n_dim_ev / (dim_ms * 1e-3),
+                        "sample": "first events of the batch, full chain with vertexing, histograms on"},
             "rt_calibrate": {"hits": int(ev.n_hits), "ms": rt_ms, "algorithmic_GB_per_s": rt_gbs,
                              "frac_of_hbm_peak": rt_gbs / HBM_PEAK_GBS, "bytes_per_hit": 14},
         },

@@ -2429,6 +2429,317 @@ STAGE void stage_vertex(Tk* tks, int ntkl) {
 }
 
 // H: outputs
+// ---- dimuons and display histograms (gKernel_DimuonBuilding, reconstruction_kernels.cuh:2820-3090;
+// FindDimuonClosestApproach, reconstruction_helper.cuh:1269-1476; gKernel_fill_display_histograms,
+// reconstruction_kernels.cuh:3100-3195) ----
+// One lane per event over the chain's output tracks (stationID >= 7, slot order).  Pairs are taken in the
+// reference's order (slot a <= slot b, then the tracks of a, then those of b) because momentum values the
+// swim leaves in the pair's 4-vectors carry over to the next partner of the same first track.  The swim
+// of the two tracks runs in lock step, keeping the closest-approach step on the fly instead of 603-float
+// arrays; every statement rounds where the reference's float arrays and double constants make it round.
+constexpr float D_MMU = 0.105658f, D_MP = 0.938272f, D_EBEAM = 120.f, D_PBEAM = 119.996331801f;
+constexpr float D_Z_DUMP = 42.f, D_CHI2DOF_MAX = 250.f;
+constexpr int D_NS = V_NSTEPS_FMAG + V_NSTEPS_TARGET;
+
+struct DimSwim {
+    float px, py, pz, mx, my, mz;
+};
+
+__device__ __forceinline__ float dim_e(const float* p) { return sqrtf(D_MMU * D_MMU + p[0] * p[0] + p[1] * p[1] + p[2] * p[2]); }
+
+// swim start at the FMAG end (reconstruction_kernels.cuh:2953-2977)
+__device__ DimSwim dim_start(const or_track& R, int det_first) {
+    const short q = (short)R.charge;
+    const float charge = (float)R.charge, invP = R.invp, ty = R.ty;
+    const float tx_st1 = R.tx + PT_KICK_KMAG * invP * q;
+    const float x0_st1 = R.tx * Z_KMAG_BEND + R.x0 - tx_st1 * Z_KMAG_BEND;
+    const float z_0 = G(F_Z, det_first);
+    const float s0 = charge * invP * sqrtf((1.f + tx_st1 * tx_st1) / (1.f + tx_st1 * tx_st1 + ty * ty));
+    const float s3 = z_0 * tx_st1 + x0_st1, s4 = z_0 * ty + R.y0;
+    DimSwim w;
+    w.px = (float)(s3 + tx_st1 * (V_FMAG_LENGTH - z_0));
+    w.py = (float)(s4 + ty * (V_FMAG_LENGTH - z_0));
+    w.pz = (float)V_FMAG_LENGTH;
+    const float pz_0 = 1.f / (fabsf(s0) * sqrtf(1.f + tx_st1 * tx_st1 + ty * ty));
+    w.mx = pz_0 * tx_st1; w.my = pz_0 * ty; w.mz = pz_0;
+    return w;
+}
+
+// one FMAG step; lin = linear coefficient of the first energy-loss term
+// (DEDX_UNIT_2 for the second track of a pair, as the reference has it)
+__device__ __forceinline__ void dim_fmag_step(DimSwim& w, float charge, float ty, double lin) {
+    const float tx_i = w.mx / w.mz;
+    const float tx_f = tx_i + 2.f * charge * V_PTKICK_UNIT * V_STEP_FMAG / sqrtf(w.mx * w.mx + w.mz * w.mz);
+    const float t1x = (float)(tx_i * V_STEP_FMAG), t1y = (float)(ty * V_STEP_FMAG), t1z = (float)V_STEP_FMAG;
+    const float bx = w.px - t1x, by = w.py - t1y, bz = w.pz - t1z;
+    const float ptot_i = sqrtf(w.mx * w.mx + w.my * w.my + w.mz * w.mz);
+    float ptot_b = ptot_i;
+    if (bz > V_FMAG_HOLE_LENGTH || bx * bx + by * by > V_FMAG_HOLE_RADIUS)
+        ptot_b += (V_DEDX0 + lin * ptot_i + V_DEDX2 * ptot_i * ptot_i + V_DEDX3 * ptot_i * ptot_i * ptot_i +
+                   V_DEDX4 * ptot_i * ptot_i * ptot_i * ptot_i) * sqrtf(t1x * t1x + t1y * t1y + t1z * t1z);
+    const float t2x = (float)(tx_f * V_STEP_FMAG), t2y = (float)(ty * V_STEP_FMAG), t2z = (float)V_STEP_FMAG;
+    w.px = bx - t2x; w.py = by - t2y; w.pz = bz - t2z;
+    float ptot_f = ptot_b;
+    if (w.pz > V_FMAG_HOLE_LENGTH || w.px * w.px + w.py * w.py > V_FMAG_HOLE_RADIUS)
+        ptot_f += (V_DEDX0 + V_DEDX1 * ptot_b + V_DEDX2 * ptot_b * ptot_b + V_DEDX3 * ptot_b * ptot_b * ptot_b +
+                   V_DEDX4 * ptot_b * ptot_b * ptot_b * ptot_b) * sqrtf(t2x * t2x + t2y * t2y + t2z * t2z);
+    const float pz_f = ptot_f / sqrtf(1.f + tx_f * tx_f + ty * ty);
+    w.mx = pz_f * tx_f; w.my = pz_f * ty; w.mz = pz_f;
+}
+
+__device__ __forceinline__ void dim_target_step(DimSwim& w, float ty) {
+    const float tx_i = w.mx / w.mz;
+    w.px = w.px - (float)(tx_i * V_STEP_TARGET);
+    w.py = w.py - (float)(ty * V_STEP_TARGET);
+    w.pz = w.pz - (float)V_STEP_TARGET;
+}
+
+// FindDimuonClosestApproach: v, returns the first track's z at the closest step
+// (Z_DUMP when no step qualifies); writes p_pos / p_neg xyz at that step.
+__device__ float dim_closest(float* v, float* p_pos, float* p_neg, float c1, float c2, float ty1, float ty2, DimSwim a,
+                             DimSwim b) {
+    float dmin = 1.e18f;
+    int best = -1;
+    DimSwim ba = a, bb = b;
+    float z200a = 0.f, z200b = 0.f;    // flat element NSTEPS_FMAG + NSTEPS_TARGET: z of step 66
+    auto consider = [&](int j) {
+        if (j == D_NS / 3) { z200a = a.pz; z200b = b.pz; }
+        if (FMAGSTR * c1 * a.mx < 0. && FMAGSTR * c2 * b.mx < 0.) return;
+        const float d2 = (a.px - b.px) * (a.px - b.px) + (a.py - b.py) * (a.py - b.py);
+        if (d2 < dmin) { dmin = d2; best = j; ba = a; bb = b; }
+    };
+    static_assert(D_NS % 3 == 2, "flat element 200 is a z coordinate");
+    consider(0);
+    int step = 1;
+    for (; step <= V_NSTEPS_FMAG; ++step) {
+        dim_fmag_step(a, c1, ty1, V_DEDX1);
+        dim_fmag_step(b, c2, ty2, V_DEDX2);
+        consider(step);
+    }
+    for (; step <= D_NS; ++step) {
+        dim_target_step(a, ty1);
+        dim_target_step(b, ty2);
+        consider(step);
+    }
+    if (best < 0) {
+        const float m = (float)((z200a + z200b) * 0.5);
+        v[0] = m; v[1] = m; v[2] = m;
+        return D_Z_DUMP;
+    }
+    const DimSwim& P = c1 > 0 ? ba : bb;
+    const DimSwim& N = c1 > 0 ? bb : ba;
+    p_pos[0] = P.mx; p_pos[1] = P.my; p_pos[2] = P.mz;
+    p_neg[0] = N.mx; p_neg[1] = N.my; p_neg[2] = N.mz;
+    v[0] = (float)((ba.px + bb.px) * 0.5);
+    v[1] = (float)((ba.py + bb.py) * 0.5);
+    v[2] = (float)((ba.pz + bb.pz) * 0.5);
+    return ba.pz;
+}
+
+// display histograms: 11 triggers x 18 variables, bins of (online_reconstruction.cu:670-726) with
+// reconstruction_constants.h:225-263; only bins of the variables' ranges are kept, packed per trigger
+constexpr int H_NTRG = 11, H_NVARS = 18, H_NVARS_TRK = 6;
+__constant__ short h_nb[H_NVARS] = {100, 100, 100, 100, 100, 200, 50, 50, 50, 50, 50, 50, 100, 100, 100, 100, 100, 200};
+__constant__ float h_min[H_NVARS] = {-20.0f, -20.0f, 0.0f, -20.0f, -20.0f, -500.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f,
+                                     -3.141592654f, -20.0f, -20.0f, 0.0f, -20.0f, -20.0f, -500.0f};
+__constant__ float h_max[H_NVARS] = {20.0f, 20.0f, 120.0f, 20.0f, 20.0f, 500.0f, 10.0f, 1.0f, 1.0f, 0.5f, 5.0f,
+                                     3.141592654f, 20.0f, 20.0f, 120.0f, 20.0f, 20.0f, 500.0f};
+__constant__ short h_off[H_NVARS + 1] = {0, 100, 200, 300, 400, 500, 700, 750, 800, 850, 900, 950, 1000, 1100,
+                                         1200, 1300, 1400, 1500, 1700};
+constexpr int H_PACKED = 1700;     // packed bins per trigger
+
+// Every bin j with x_j - hw <= v < x_j + hw (x_j = min + hw (2j + 1), hw = (max - min) / 2 / nbins, in
+// float as the reference computes them) gets one count.  Only bins within two of the estimate can
+// qualify, so those are tested with the reference's predicate.
+__device__ __forceinline__ void hist_fill(uint32_t* lh, int trg, int k, float v) {
+    if (!(fabsf(v) < INFINITY)) return;
+    const int nb = h_nb[k];
+    const float hw = (h_max[k] - h_min[k]) * 0.5f / nb;
+    float lf = (v - h_min[k]) / (2.f * hw);
+    lf = fminf(fmaxf(lf, -8.f), (float)nb + 8.f);
+    const int l0 = (int)floorf(lf);
+    for (int j = max(0, l0 - 2); j <= min(nb - 1, l0 + 2); ++j) {
+        const float x = h_min[k] + hw * (2.f * j + 1.f);
+        if (x - hw <= v && v < x + hw) atomicAdd(&lh[trg * H_PACKED + h_off[k] + j], 1u);
+    }
+}
+
+struct DimArgs {
+    int64_t n_events;
+    int max_out;
+    const int32_t* n_tracks;
+    const uint32_t* status;
+    const or_track* tracks;
+    const float* vertex;
+    const int16_t* det;
+    const int64_t* off;
+    const int32_t* trigger;
+    int32_t* n_dim;
+    float* dim;
+    float* hist;                 // OR_HIST_* layout, += counts (NULL: no histograms)
+    unsigned long long* lost;
+};
+
+constexpr int DIM_BLOCK = 256, DIM_MAX_TRACKS = 64;
+
+__global__ __launch_bounds__(DIM_BLOCK) void dimuon_kernel(DimArgs a) {
+    __shared__ uint32_t lh[H_NTRG * H_PACKED];
+    if (a.hist)
+        for (int i = threadIdx.x; i < H_NTRG * H_PACKED; i += DIM_BLOCK) lh[i] = 0;
+    __syncthreads();
+    const float p_beam[4] = {0, 0, D_PBEAM, D_EBEAM};
+    const float p_target[4] = {0, 0, 0, D_MP};
+    const float p_cms[4] = {0, 0, D_PBEAM, D_EBEAM + D_MP};
+    const float s = p_cms[3] * p_cms[3] - (p_cms[0] * p_cms[0] + p_cms[1] * p_cms[1] + p_cms[2] * p_cms[2]);
+    const float sqrt_s = sqrtf(s);
+    const float bz = (p_cms[2] / p_cms[3]) * -1;
+    const float t_cms = p_target[3] * p_cms[3] - (p_target[0] * p_cms[0] + p_target[1] * p_cms[1] + p_target[2] * p_cms[2]);
+    const float b_cms = p_beam[3] * p_cms[3] - (p_beam[0] * p_cms[0] + p_beam[1] * p_cms[1] + p_beam[2] * p_cms[2]);
+    for (int64_t e = (int64_t)blockIdx.x * DIM_BLOCK + threadIdx.x; e < a.n_events; e += (int64_t)gridDim.x * DIM_BLOCK) {
+        int ndim = 0;
+        const int nt = (a.status[e] & OR_EV_REJECT_MASK) ? 0 : min(min(a.n_tracks[e], a.max_out), DIM_MAX_TRACKS);
+        const or_track* TR = a.tracks + e * a.max_out;
+        const float* VX = a.vertex + e * a.max_out * 6;
+        // the event's tracks with stationID >= 7 (slot order), first-hit detectors (get_firsthitdetid,
+        // reconstruction_classes.cuh:517-532), slot membership as a bit mask
+        uint64_t use = 0;
+        unsigned slots = 0;
+        int dfirst[DIM_MAX_TRACKS];
+        for (int t = 0; t < nt; ++t) {
+            const or_track& R = TR[t];
+            if (R.station_id < 7) continue;
+            use |= 1ull << t;
+            slots |= 1u << (R.slot & 31);
+            int dmin = 30;
+            float zmin = 10000.f;
+            const int nh = min((int)R.nhits, OR_MAX_HITS_PER_TRACK);
+            for (int h = 0; h < nh; ++h) {
+                const int d = a.det[a.off[e] + R.hit_index[h]];
+                if (d > 0 && d < ND && G(F_Z, d) < zmin) { dmin = d; zmin = G(F_Z, d); }
+            }
+            dfirst[t] = dmin;
+        }
+        float p_pos[4] = {0, 0, 0, 0}, p_neg[4] = {0, 0, 0, 0}, pos_v[3] = {0, 0, 0}, neg_v[3] = {0, 0, 0};
+        float* out = a.dim + e * OR_DIMUONS_MAX * OR_DIMUON_FIELDS;
+        for (unsigned sa = slots; sa; sa &= sa - 1) {
+            const int ia = __ffs(sa) - 1;
+            for (unsigned sb = slots & ~((1u << ia) - 1u); sb; sb &= sb - 1) {
+                const int ib = __ffs(sb) - 1;
+                for (int i = 0; i < nt; ++i) {
+                    if (!((use >> i) & 1) || TR[i].slot != ia) continue;
+                    const or_track& A = TR[i];
+                    const float ca = (float)A.charge;
+                    if (ca > 0) {
+                        p_pos[0] = VX[i * 6 + 3]; p_pos[1] = VX[i * 6 + 4]; p_pos[2] = VX[i * 6 + 5]; p_pos[3] = dim_e(p_pos);
+                        pos_v[0] = VX[i * 6]; pos_v[1] = VX[i * 6 + 1]; pos_v[2] = VX[i * 6 + 2];
+                    } else {
+                        p_neg[0] = VX[i * 6 + 3]; p_neg[1] = VX[i * 6 + 4]; p_neg[2] = VX[i * 6 + 5]; p_neg[3] = dim_e(p_neg);
+                        neg_v[0] = VX[i * 6]; neg_v[1] = VX[i * 6 + 1]; neg_v[2] = VX[i * 6 + 2];
+                    }
+                    for (int j = 0; j < nt; ++j) {
+                        if (!((use >> j) & 1) || TR[j].slot != ib) continue;
+                        const or_track& B = TR[j];
+                        const float cb = (float)B.charge;
+                        if (cb * ca > 0) continue;
+                        if (cb > 0) {
+                            p_pos[0] = VX[j * 6 + 3]; p_pos[1] = VX[j * 6 + 4]; p_pos[2] = VX[j * 6 + 5]; p_pos[3] = dim_e(p_pos);
+                            pos_v[0] = VX[j * 6]; pos_v[1] = VX[j * 6 + 1]; pos_v[2] = VX[j * 6 + 2];
+                        } else {
+                            p_neg[0] = VX[j * 6 + 3]; p_neg[1] = VX[j * 6 + 4]; p_neg[2] = VX[j * 6 + 5]; p_neg[3] = dim_e(p_neg);
+                            neg_v[0] = VX[j * 6]; neg_v[1] = VX[j * 6 + 1]; neg_v[2] = VX[j * 6 + 2];
+                        }
+                        float ps[4], ns[4], v1[3], v2[3], dv[3], sum[4];
+                        for (int k = 0; k < 4; ++k) { ps[k] = p_pos[k]; ns[k] = p_neg[k]; }
+                        for (int k = 0; k < 3; ++k) v1[k] = (pos_v[k] + neg_v[k]) * 0.5f;
+                        const float vz2 = dim_closest(v2, p_pos, p_neg, ca, cb, A.ty, B.ty, dim_start(A, dfirst[i]),
+                                                      dim_start(B, dfirst[j]));
+                        p_pos[3] = dim_e(p_pos);
+                        p_neg[3] = dim_e(p_neg);
+                        if (fabsf(vz2 - v2[2]) < V_STEP_FMAG) {
+                            for (int k = 0; k < 4; ++k) sum[k] = p_pos[k] + p_neg[k];
+                            for (int k = 0; k < 3; ++k) dv[k] = v2[k];
+                        } else {
+                            for (int k = 0; k < 4; ++k) sum[k] = ps[k] + ns[k];
+                            for (int k = 0; k < 3; ++k) dv[k] = v1[k];
+                        }
+                        const float m = sqrtf(sum[3] * sum[3] - (sum[0] * sum[0] + sum[1] * sum[1] + sum[2] * sum[2]));
+                        const float pT = sqrtf(sum[0] * sum[0] + sum[1] * sum[1]);
+                        const float x1 = (p_target[3] * sum[3] - (p_target[0] * sum[0] + p_target[1] * sum[1] + p_target[2] * sum[2])) / t_cms;
+                        const float x2 = (p_beam[3] * sum[3] - (p_beam[0] * sum[0] + p_beam[1] * sum[1] + p_beam[2] * sum[2])) / b_cms;
+                        {   // Boost (reconstruction_helper.cuh:1244-1258) by -cms velocity (0, 0, bz)
+                            const float bx = 0.0f * -1, by = 0.0f * -1;
+                            const float b2 = bx * bx + by * by + bz * bz;
+                            const float gamma = 1.0f / sqrtf(1.0f - b2);
+                            const float bp = sum[0] * bx + sum[1] * by + sum[2] * bz;
+                            const float gamma2 = b2 > 0.0f ? (gamma - 1.0f) / b2 : 0.0f;
+                            sum[0] = sum[0] + gamma2 * bp * bx + gamma * bx * sum[3];
+                            sum[1] = sum[1] + gamma2 * bp * by + gamma * by * sum[3];
+                            sum[2] = sum[2] + gamma2 * bp * bz + gamma * bz * sum[3];
+                            sum[3] = gamma * (sum[3] + bp);
+                        }
+                        const float xF = 2.0f * sum[2] / (sqrt_s * (1.f - m * m / s));
+                        const float costheta = 2 * (p_neg[3] * p_pos[2] - p_neg[2] * p_pos[3]) / (m * sqrtf(m * m + pT * pT));
+                        const float phi = (float)atan2((double)(2 * sqrtf(m * m + pT * pT) * (p_neg[0] * p_pos[1] - p_neg[1] * p_pos[0])),
+                                                       (double)(m * (p_pos[0] * p_pos[0] - p_neg[0] * p_neg[0] + p_pos[1] * p_pos[1] -
+                                                                     p_neg[1] * p_neg[1])));
+                        if (FMAGSTR * (p_pos[0] - p_neg[0]) < 0.) continue;
+                        if (m < 0 || m > 10.f) continue;
+                        if (fabsf(xF) > 1.f) continue;
+                        if (x1 < 0.f || x1 > 1.f) continue;
+                        if (x2 < 0.f || x2 > 1.f) continue;
+                        if (fabsf(costheta) > 1.f) continue;
+                        if (p_pos[2] + p_neg[2] > 120.f || p_pos[2] + p_neg[2] < 30.f) continue;
+                        if (fabsf(p_pos[0] + p_neg[0]) > 3.f) continue;
+                        if (dv[0] * dv[0] + dv[1] * dv[1] > 225.f) continue;
+                        if (ndim >= OR_DIMUONS_MAX) {
+                            if (a.lost) atomicAdd(a.lost, 1ull);
+                            continue;
+                        }
+                        float* o = out + ndim * OR_DIMUON_FIELDS;
+                        o[0] = m; o[1] = pT; o[2] = xF; o[3] = x1; o[4] = x2; o[5] = costheta; o[6] = phi;
+                        o[7] = 0.f; o[8] = 0.f;
+                        for (int k = 0; k < 3; ++k) { o[9 + k] = dv[k]; o[12 + k] = pos_v[k]; o[15 + k] = neg_v[k]; }
+                        for (int k = 0; k < 4; ++k) { o[18 + k] = p_pos[k]; o[22 + k] = p_neg[k]; o[26 + k] = ps[k]; o[30 + k] = ns[k]; }
+                        o[34] = 0.f; o[35] = 0.f;
+                        ++ndim;
+                    }
+                }
+            }
+        }
+        a.n_dim[e] = ndim;
+        if (!a.hist) continue;
+        // trig_array[n] = TriggerBits & n (not a bit test, reconstruction_kernels.cuh:3108); the last is "all"
+        const int tb = a.trigger ? a.trigger[e] : 0;
+        for (int t = 0; t < nt; ++t) {
+            if (!((use >> t) & 1)) continue;
+            const or_track& R = TR[t];
+            if (R.chi2 / (float)((int)R.nhits - 5) > D_CHI2DOF_MAX) continue;
+            const float vars[H_NVARS_TRK] = {VX[t * 6 + 3], VX[t * 6 + 4], VX[t * 6 + 5], VX[t * 6], VX[t * 6 + 1], VX[t * 6 + 2]};
+            for (int trg = 0; trg < H_NTRG; ++trg)
+                if ((tb & trg) || trg == H_NTRG - 1)
+                    for (int k = 0; k < H_NVARS_TRK; ++k) hist_fill(lh, trg, k, vars[k]);
+        }
+        for (int i = 0; i < ndim; ++i) {
+            const float* o = out + i * OR_DIMUON_FIELDS;
+            const float vars[H_NVARS - H_NVARS_TRK] = {o[0], o[2], o[3], o[4], o[1], o[6], o[18] + o[22], o[19] + o[23],
+                                                       o[20] + o[24], o[9], o[10], o[11]};
+            for (int trg = 0; trg < H_NTRG; ++trg)
+                if ((tb & trg) || trg == H_NTRG - 1)
+                    for (int k = H_NVARS_TRK; k < H_NVARS; ++k) hist_fill(lh, trg, k, vars[k - H_NVARS_TRK]);
+        }
+    }
+    if (!a.hist) return;
+    __syncthreads();
+    for (int i = threadIdx.x; i < H_NTRG * H_PACKED; i += DIM_BLOCK) {
+        const uint32_t c = lh[i];
+        if (!c) continue;
+        const int trg = i / H_PACKED, r = i % H_PACKED;
+        int k = 0;
+        while (r >= h_off[k + 1]) ++k;
+        atomicAdd(&a.hist[(trg * H_NVARS + k) * OR_HIST_BINS + (r - h_off[k])], (float)c);
+    }
+}
+
 // ---- final straight-line fit of an output track (no reference counterpart) ----
 // u = pos + sign * drift per hit, model u = cos (x0 + tx z) + sin (y0 + ty z),
 // weight 1/resolution^2, z taken from FIT_ZREF.  The normal matrix and
@@ -3340,6 +3651,41 @@ or_status or_rt_calibrate(or_session* s, const or_hits* in, float* drift_out) {
     const hipError_t ef = hipFree(buf);
     OR_HIP(e);
     OR_HIP(ef);
+    return OR_OK;
+}
+
+or_status or_dimuons(or_session* s, const or_hits* in, const or_result* chain, const int32_t* trigger_bits,
+                     or_dimuon_result* out) {
+    if (!s || !in || !chain || !out || !in->on_device || !chain->on_device || in->n_events < 0) return OR_ERR_INVALID_ARG;
+    if (!in->det || !in->event_offsets || !chain->status || !chain->n_tracks || !chain->tracks || !chain->vertex ||
+        !out->n_dimuons || !out->dimuons)
+        return OR_ERR_INVALID_ARG;
+    if (s->p.max_tracks_out > DIM_MAX_TRACKS) return OR_ERR_CAPACITY;
+    if (in->n_events == 0) return OR_OK;
+    OR_HIP(hipSetDevice(s->device));
+    DimArgs a;
+    a.n_events = in->n_events;
+    a.max_out = s->p.max_tracks_out;
+    a.n_tracks = chain->n_tracks;
+    a.status = chain->status;
+    a.tracks = chain->tracks;
+    a.vertex = chain->vertex;
+    a.det = in->det;
+    a.off = in->event_offsets;
+    a.trigger = trigger_bits;
+    a.n_dim = out->n_dimuons;
+    a.dim = out->dimuons;
+    a.hist = out->histograms;
+    a.lost = out->lost;
+    int dev_cus = 0;
+    OR_HIP(hipDeviceGetAttribute(&dev_cus, hipDeviceAttributeMultiprocessorCount, s->device));
+    // one lane per event; with histograms, per-block LDS counts flushed once per block, so the grid is
+    // kept to a few blocks per CU
+    const int64_t want = (in->n_events + DIM_BLOCK - 1) / DIM_BLOCK;
+    const int64_t cap = out->histograms ? 4LL * std::max(1, dev_cus) : 65535;
+    const unsigned grid = (unsigned)std::max<int64_t>(1, std::min(want, cap));
+    hipLaunchKernelGGL(dimuon_kernel, dim3(grid), dim3(DIM_BLOCK), 0, s->stream, a);
+    OR_HIP(hipGetLastError());
     return OR_OK;
 }
 

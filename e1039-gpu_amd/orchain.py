@@ -160,6 +160,8 @@ def load_library(path: str = LIB_PATH):
     lib.or_session_set_calibration.restype = i32
     lib.or_rt_calibrate.argtypes = [vp, C.POINTER(OrHits), vp]
     lib.or_rt_calibrate.restype = i32
+    lib.or_dimuons.argtypes = [vp, C.POINTER(OrHits), C.POINTER(OrResult), vp, C.POINTER(OrDimuonResult)]
+    lib.or_dimuons.restype = i32
     lib.or_session_set_fit_cond_bound.argtypes = [vp, C.c_float]
     lib.or_session_set_fit_cond_bound.restype = i32
     lib.or_session_destroy.argtypes = [vp]
@@ -212,6 +214,23 @@ class Geometry:
         if getattr(self, "handle", None) and _lib is not None:
             _lib.or_geometry_free(self.handle)
             self.handle = None
+
+
+class OrDimuonResult(C.Structure):
+    _fields_ = [("n_dimuons", C.c_void_p), ("dimuons", C.c_void_p), ("histograms", C.c_void_p), ("lost", C.c_void_p)]
+
+
+DIMUON_FIELDS = 36        # OR_DIMUON_FIELDS
+DIMUONS_MAX = 16          # OR_DIMUONS_MAX
+HIST_TRIGGERS, HIST_VARS, HIST_BINS = 11, 18, 256
+
+
+@dataclass
+class DimuonResult:
+    n_dimuons: np.ndarray          # (n_events,) int32
+    dimuons: np.ndarray            # (n_events, DIMUONS_MAX, DIMUON_FIELDS) float32
+    histograms: np.ndarray | None  # (HIST_TRIGGERS * HIST_VARS * HIST_BINS,) float32
+    lost: int
 
 
 class Calibration:
@@ -382,6 +401,37 @@ class Session:
         out = torch.empty(max(1, t["n_hits"]), dtype=torch.float32, device=t["det"].device)
         _check(self.lib.or_rt_calibrate(self.handle, C.byref(hits), C.c_void_p(out.data_ptr())))
         return out[: t["n_hits"]]
+
+    # --- dimuons and display histograms ------------------------------------------
+    def alloc_dimuons(self, n_events: int, histograms: bool = True):
+        import torch
+        dev = torch.device("cuda", self.device)
+        return {
+            "n_dimuons": torch.empty(max(1, n_events), dtype=torch.int32, device=dev),
+            "dimuons": torch.empty(max(1, n_events * DIMUONS_MAX * DIMUON_FIELDS), dtype=torch.float32, device=dev),
+            "histograms": (torch.zeros(HIST_TRIGGERS * HIST_VARS * HIST_BINS, dtype=torch.float32, device=dev)
+                           if histograms else None),
+            "lost": torch.zeros(1, dtype=torch.int64, device=dev),
+        }
+
+    def run_dimuons(self, t, o, d, trigger=None):
+        """or_dimuons on device buffers: t from upload(), o the outputs of a full-chain run with vertex,
+        d from alloc_dimuons(), trigger an (n_events,) int32 device tensor or None (asynchronous)."""
+        hits, res = self._device_args(t, o)
+        out = OrDimuonResult(d["n_dimuons"].data_ptr(), d["dimuons"].data_ptr(),
+                             None if d["histograms"] is None else d["histograms"].data_ptr(), d["lost"].data_ptr())
+        trig = None if trigger is None else C.c_void_p(trigger.contiguous().data_ptr())
+        _check(self.lib.or_dimuons(self.handle, C.byref(hits), C.byref(res), trig, C.byref(out)))
+
+    @staticmethod
+    def fetch_dimuons(d, n_events: int) -> DimuonResult:
+        return DimuonResult(
+            n_dimuons=d["n_dimuons"].cpu().numpy()[:n_events],
+            dimuons=d["dimuons"].cpu().numpy()[: n_events * DIMUONS_MAX * DIMUON_FIELDS].reshape(
+                n_events, DIMUONS_MAX, DIMUON_FIELDS),
+            histograms=None if d["histograms"] is None else d["histograms"].cpu().numpy(),
+            lost=int(d["lost"].cpu().numpy()[0]),
+        )
 
     def set_fit_cond_bound(self, bound: float):
         """or_session_set_fit_cond_bound: condition bound of the final fit's FP32 solve

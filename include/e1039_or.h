@@ -265,6 +265,50 @@ or_status or_session_set_calibration(or_session* s, const or_calibration* c);
  * per-hit drift_distance computation of online_reconstruction.cu:387-397. */
 or_status or_rt_calibrate(or_session* s, const or_hits* in, float* drift_out);
 
+/* ---- dimuons and display histograms -----------------------------------------
+ * Replaces gKernel_DimuonBuilding (reconstruction_kernels.cuh:2820-3090, with
+ * FindDimuonClosestApproach, reconstruction_helper.cuh:1269-1476) and
+ * gKernel_fill_display_histograms (reconstruction_kernels.cuh:3100-3195), run
+ * on the outputs of a full-chain or_reconstruct (or_params.full_chain = 1,
+ * or_result.vertex set): opposite-charge pairs of the event's stationID >= 7
+ * tracks, taken in the reference's order (slot a <= slot b, tracks of a, tracks
+ * of b; a pair inside one slot appears in both orders), swum back through FMAG
+ * together to their closest approach, kinematics and the reference's cuts.
+ * Record of OR_DIMUON_FIELDS floats (reconstruction_classes.cuh:879-934):
+ * mass, pT, xF, x1, x2, costheta, phi, mass_single, chi2_single, vertex xyz,
+ * positive-track vertex xyz, negative-track vertex xyz, p+ (px, py, pz, E),
+ * p- (px, py, pz, E), single p+ (4), single p- (4), chi2_vtx, chi2_kf.  The
+ * reference never computes mass_single, chi2_single, chi2_vtx, chi2_kf; they
+ * are 0 here.  phi is atan2 in FP64 rounded to float (the reference's atan2f is
+ * within 2 ulp of it).  At most OR_DIMUONS_MAX per event (the reference writes
+ * a 17th past its array); further ones are counted in *lost.
+ * Histograms: OR_HIST_TRIGGERS x OR_HIST_VARS x OR_HIST_BINS floats, index
+ * (trigger * OR_HIST_VARS + variable) * OR_HIST_BINS + bin, the reference's
+ * bins (online_reconstruction.cu:670-726); counts are added to what the buffer
+ * holds.  Trigger n of an event counts when (trigger_bits & n) != 0, the last
+ * always (the reference's test, reconstruction_kernels.cuh:3108).  Variables:
+ * track px, py, pz, vx, vy, vz (stationID >= 7, chi2/(nhits-5) <= 250), dimuon
+ * mass, xF, x1, x2, pT, phi, px, py, pz, vx, vy, vz. */
+#define OR_DIMUON_FIELDS 36
+#define OR_DIMUONS_MAX 16
+#define OR_HIST_TRIGGERS 11
+#define OR_HIST_VARS 18
+#define OR_HIST_BINS 256
+
+typedef struct {
+    int32_t* n_dimuons;          /* n_events */
+    float* dimuons;              /* n_events * OR_DIMUONS_MAX * OR_DIMUON_FIELDS; rows [0, n_dimuons[e]) written */
+    float* histograms;           /* optional: OR_HIST_TRIGGERS * OR_HIST_VARS * OR_HIST_BINS, counts added */
+    unsigned long long* lost;    /* optional, one counter: dimuons beyond OR_DIMUONS_MAX, added */
+} or_dimuon_result;
+
+/* Device pointers only (in->on_device and chain->on_device), asynchronous on
+ * the session stream; the session must be the one that produced chain (its
+ * max_tracks_out, at most 64, else OR_ERR_CAPACITY).  trigger_bits: n_events
+ * int32, or NULL (all 0). */
+or_status or_dimuons(or_session* s, const or_hits* in, const or_result* chain, const int32_t* trigger_bits,
+                     or_dimuon_result* out);
+
 /* ---- learned pruning of YZ station-3 units (no reference counterpart) -----
  * A YZ unit is one (u3, v3) window-list pair of a tracklet: the reference
  * evaluates every (u2, v2) pair under every unit (the i_uv3 loop,

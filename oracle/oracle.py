@@ -56,6 +56,7 @@ def load():
                                      f32, f32, f32, f32, f32, f32]
     lib.orc_prop_segment.restype = C.c_int
     lib.orc_global_cleaning.argtypes = [C.c_int, vp, vp, vp, vp, vp, vp]
+    lib.orc_dimuons.argtypes = [C.c_int64, C.c_int, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
     lib.orc_rt_spline.argtypes = [C.c_int, vp, vp, vp, vp, vp]
     lib.orc_rt_eval.argtypes = [C.c_int, vp, vp, C.c_int64, vp, vp]
     lib.orc_rt_calibrate.argtypes = [vp, vp, vp, vp, vp, vp, C.c_int64, vp, vp, vp, vp]
@@ -186,3 +187,30 @@ def rt_calibrate(cal: OracleCalibration, cellwidth, det, tdc, drift):
                             cw.ctypes.data, len(det), det.ctypes.data, tdc.ctypes.data, drift.ctypes.data,
                             out.ctypes.data)
     return out
+
+
+# ---- dimuons and display histograms (reconstruction_kernels.cuh:2820-3195) ----
+
+def dimuons(chain, ev, z_array, max_out: int, trigger=None, histograms: bool = True):
+    """Dimuons of a full-chain result with vertex (chain: the Result of reconstruct(..., vertex=True))."""
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(HERE), "e1039-gpu_amd"))
+    import orchain as oc
+    ne = ev.n_events
+    n_dim = np.zeros(ne, np.int32)
+    dim = np.zeros((ne, oc.DIMUONS_MAX, oc.DIMUON_FIELDS), np.float32)
+    hist = np.zeros(oc.HIST_TRIGGERS * oc.HIST_VARS * oc.HIST_BINS, np.float32) if histograms else None
+    lost = np.zeros(1, np.int64)
+    tracks = np.ascontiguousarray(chain.tracks)
+    vtx = np.ascontiguousarray(chain.vertex, np.float32)
+    det = np.ascontiguousarray(ev.det, np.int16)
+    off = np.ascontiguousarray(ev.offsets, np.int64)
+    z = np.ascontiguousarray(z_array, np.float32)
+    status = np.ascontiguousarray(chain.status, np.uint32)
+    ntr = np.ascontiguousarray(chain.n_tracks, np.int32)
+    trig = None if trigger is None else np.ascontiguousarray(trigger, np.int32)
+    load().orc_dimuons(ne, max_out, ntr.ctypes.data, status.ctypes.data, tracks.ctypes.data, vtx.ctypes.data,
+                       det.ctypes.data, off.ctypes.data, z.ctypes.data, None if trig is None else trig.ctypes.data,
+                       n_dim.ctypes.data, dim.ctypes.data, None if hist is None else hist.ctypes.data,
+                       lost.ctypes.data)
+    return oc.DimuonResult(n_dim, dim, hist, int(lost[0]))
