@@ -32,45 +32,69 @@ __constant__ float c_fit_cond = OR_FIT_COND_FP32;   // FP32 / FP64 switch of the
 
 // R-T calibration on the device: per detector n knots x[off .. off+n) and the
 // cubic (y, b, c, d) of each knot interval at cf[off + k]; n = 0: not applied.
+// x0, xl: first and last knot; scale = (n - 1) / (xl - x0) for the interval guess.
+struct RtHead {
+    int32_t n, off;
+    float tmin, tmax;
+    double x0, xl, scale;
+};
 struct RtDev {
-    int32_t n[ND], off[ND];
-    float tmin[ND], tmax[ND];
+    RtHead h[ND];
     const double* x;
     const double4* cf;
 };
 
 // Drift of one hit (online_reconstruction.cu:389-397): tdc < tmin -> cellwidth/2,
-// tdc > tmax -> 0, else the spline in FP64.  Knot interval k: t_k < t <= t_k+1,
-// the end intervals below the first and above the last knot (TSpline3's
-// FindX / Eval choice).
-__device__ __forceinline__ float rt_drift(const RtDev* R, int d, float tdc, float drift) {
-    const int n = R->n[d];
+// tdc > tmax -> 0, else the spline in FP64.  Knot interval k: the largest with
+// t_k < t (TSpline3's binary search), the first interval below the first knot
+// and the last one above the last knot.  The interval is guessed from the
+// mean knot spacing and corrected against the knots, which gives the same k.
+__device__ __forceinline__ float rt_drift(const RtHead& H, const double* x, const double4* cf, int d, float tdc,
+                                          float drift) {
+    const int n = H.n;
     if (n <= 0) return drift;
-    if (tdc < R->tmin[d]) return c_geo[F_CELLWIDTH][d] * 0.5f;
-    if (tdc > R->tmax[d]) return 0.f;
+    if (tdc < H.tmin) return c_geo[F_CELLWIDTH][d] * 0.5f;
+    if (tdc > H.tmax) return 0.f;
     const double t = (double)tdc;
-    const double* X = R->x + R->off[d];
+    const double* X = x + H.off;
     int k = 0;
-    if (t >= X[n - 1]) {
+    if (t >= H.xl) {
         k = n - 2;
-    } else if (t > X[0]) {
-        int hi = n - 1;
-        while (hi - k > 1) {
-            const int m = (k + hi) >> 1;
-            if (t > X[m]) k = m; else hi = m;
-        }
+    } else if (t > H.x0) {
+        k = min(max((int)((t - H.x0) * H.scale), 0), n - 2);
+        while (k > 0 && !(X[k] < t)) --k;
+        while (k < n - 2 && X[k + 1] < t) ++k;
     }
-    const double4 c = R->cf[R->off[d] + k];
+    const double4 c = cf[H.off + k];
     const double dt = t - X[k];
     return (float)(c.x + dt * (c.y + dt * (c.z + dt * c.w)));
 }
 
-__global__ void rt_calibrate_kernel(const RtDev* R, const int16_t* det, const float* tdc, const float* drift,
-                                    int64_t n, float* out) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        const int d = det[i];
-        const float dr = drift[i];
-        out[i] = (d >= 1 && d < ND) ? rt_drift(R, d, tdc[i], dr) : dr;
+constexpr int RT_BLOCK = 256, RT_UNROLL = 4;
+
+// one hit per lane, RT_UNROLL hits per lane in flight; the per-detector heads in LDS
+__global__ __launch_bounds__(RT_BLOCK) void rt_calibrate_kernel(const RtDev* R, const int16_t* det, const float* tdc,
+                                                                const float* drift, int64_t n, float* out) {
+    __shared__ RtHead H[ND];
+    for (int i = threadIdx.x; i < ND; i += RT_BLOCK) H[i] = R->h[i];
+    const double* X = R->x;
+    const double4* CF = R->cf;
+    __syncthreads();
+    for (int64_t base = (int64_t)blockIdx.x * RT_BLOCK * RT_UNROLL; base < n;
+         base += (int64_t)gridDim.x * RT_BLOCK * RT_UNROLL) {
+        int d[RT_UNROLL];
+        float t[RT_UNROLL], dr[RT_UNROLL];
+#pragma unroll
+        for (int j = 0; j < RT_UNROLL; ++j) {
+            const int64_t i = base + j * RT_BLOCK + threadIdx.x;
+            d[j] = 0; t[j] = 0.f; dr[j] = 0.f;
+            if (i < n) { d[j] = det[i]; t[j] = tdc[i]; dr[j] = drift[i]; }
+        }
+#pragma unroll
+        for (int j = 0; j < RT_UNROLL; ++j) {
+            const int64_t i = base + j * RT_BLOCK + threadIdx.x;
+            if (i < n) out[i] = (d[j] >= 1 && d[j] < ND) ? rt_drift(H[d[j]], X, CF, d[j], t[j], dr[j]) : dr[j];
+        }
     }
 }
 }
@@ -778,7 +802,7 @@ STAGE bool stage_load_reduce(const Args& a, int64_t h0, int n) {
                     L.elem[dest] = (int16_t)el;
                     L.pos[dest] = a.wire[d * OR_MAX_ELEM + el];
                     const float dr = a.drift[h0 + h];
-                    L.drift[dest] = fabsf(a.rt ? rt_drift(a.rt, d, a.tdc[h0 + h], dr) : dr);
+                    L.drift[dest] = fabsf(a.rt ? rt_drift(a.rt->h[d], a.rt->x, a.rt->cf, d, a.tdc[h0 + h], dr) : dr);
                     L.orig[dest] = (uint16_t)h;
                     const int fl = a.flag ? (int)a.flag[h0 + h] : 2;
                     L.keep[dest] = (fl & 2) ? 1 : -1;
@@ -3606,7 +3630,11 @@ or_status or_session_set_calibration(or_session* s, const or_calibration* c) {
     int64_t o = 0;
     for (int d = 0; d < OR_N_DETECTORS; ++d) {
         const or_calibration::Det& D = c->det[d];
-        h->n[d] = D.n; h->off[d] = (int32_t)o; h->tmin[d] = D.tmin; h->tmax[d] = D.tmax;
+        RtHead& hd = h->h[d];
+        hd.n = D.n; hd.off = (int32_t)o; hd.tmin = D.tmin; hd.tmax = D.tmax;
+        hd.x0 = D.n ? D.x[0] : 0.0;
+        hd.xl = D.n ? D.x[D.n - 1] : 0.0;
+        hd.scale = D.n > 1 ? (double)(D.n - 1) / (D.x[D.n - 1] - D.x[0]) : 0.0;
         for (int k = 0; k < D.n; ++k) {
             hx[o + k] = D.x[k];
             hc[o + k] = k + 1 < D.n ? double4{D.y[k], D.b[k], D.c[k], D.d[k]} : double4{0.0, 0.0, 0.0, 0.0};
@@ -3625,9 +3653,12 @@ or_status or_rt_calibrate(or_session* s, const or_hits* in, float* drift_out) {
     if (!in->det || !in->tdc || !in->drift || !drift_out) return OR_ERR_INVALID_ARG;
     OR_HIP(hipSetDevice(s->device));
     const int64_t n = in->n_hits;
-    const unsigned grid = (unsigned)std::min<int64_t>(4096, (n + 255) / 256);
+    int cus = 0;
+    OR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s->device));
+    const int64_t per_block = (int64_t)RT_BLOCK * RT_UNROLL;
+    const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(16LL * std::max(1, cus), (n + per_block - 1) / per_block));
     if (in->on_device) {
-        hipLaunchKernelGGL(rt_calibrate_kernel, dim3(grid), dim3(256), 0, s->stream, s->d_rt, in->det, in->tdc,
+        hipLaunchKernelGGL(rt_calibrate_kernel, dim3(grid), dim3(RT_BLOCK), 0, s->stream, s->d_rt, in->det, in->tdc,
                            in->drift, n, drift_out);
         OR_HIP(hipGetLastError());
         return OR_OK;
@@ -3643,7 +3674,7 @@ or_status or_rt_calibrate(or_session* s, const or_hits* in, float* drift_out) {
     if (e == hipSuccess) e = hipMemcpyAsync(dt, in->tdc, (size_t)n * 4, hipMemcpyHostToDevice, s->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(dr, in->drift, (size_t)n * 4, hipMemcpyHostToDevice, s->stream);
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(rt_calibrate_kernel, dim3(grid), dim3(256), 0, s->stream, s->d_rt, dd, dt, dr, n, dout);
+        hipLaunchKernelGGL(rt_calibrate_kernel, dim3(grid), dim3(RT_BLOCK), 0, s->stream, s->d_rt, dd, dt, dr, n, dout);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipMemcpyAsync(drift_out, dout, (size_t)n * 4, hipMemcpyDeviceToHost, s->stream);

@@ -105,3 +105,45 @@ def test_chain_with_fused_calibration(oc, orc, geo, ogeo, session, cal):
     plain = session.reconstruct(ev)
     again = orc.reconstruct(ogeo, session.params, ev)
     assert np.array_equal(plain.valid_tracks().view(np.uint8), again.valid_tracks().view(np.uint8))
+
+
+def test_rt_calibrate_irregular_knots(oc, orc, geo, session):
+    """Uneven knot spacing, 2 to 60 knots, ranges wider and narrower than the
+    knots: the interval guess from the mean spacing must still land on the
+    interval the binary search defines."""
+    import torch
+    rng = np.random.default_rng(21)
+    tables = {}
+    for d in list(range(1, 31)) + list(range(47, 55)):
+        n = int(rng.integers(2, 61))
+        T = np.cumsum(rng.exponential(1.0, n) ** 3 + 1e-3) * 10.0 + 500.0
+        R = np.sort(rng.uniform(0.0, 2.0, n))[::-1].copy()
+        tables[d] = (np.float32(T[0] - rng.uniform(-5, 20)), np.float32(T[-1] + rng.uniform(-5, 20)), T, R)
+    cal = oc.Calibration.create(tables)
+    ocal = orc.OracleCalibration(tables)
+    n = 300_000
+    det = rng.integers(1, 55, n).astype(np.int16)
+    lo = np.array([tables[d][0] if d in tables else 0.0 for d in range(55)])
+    hi = np.array([tables[d][1] if d in tables else 1.0 for d in range(55)])
+    tdc = rng.uniform(lo[det] - 10.0, hi[det] + 10.0).astype(np.float32)
+    k = rng.integers(0, n, 5000)
+    for i in k:                                   # exact knot times
+        if int(det[i]) in tables:
+            T = tables[int(det[i])][2]
+            tdc[i] = np.float32(T[int(rng.integers(0, len(T)))])
+    ev = oc.Events(det, np.ones(n, np.int16), tdc, rng.uniform(0, 1, n).astype(np.float32), None,
+                   np.array([0, n], np.int64))
+    planes, _ = geo.export()
+    session.set_calibration(cal)
+    try:
+        t = session.upload(ev)
+        got = session.rt_calibrate_device(t)
+        torch.cuda.synchronize()
+        got = got.cpu().numpy()
+    finally:
+        session.set_calibration(None)
+    ref = orc.rt_calibrate(ocal, planes[2], ev.det, ev.tdc, ev.drift)
+    big = np.abs(ref) >= 1e-6
+    assert np.abs(got[~big] - ref[~big]).max(initial=0.0) <= 1e-12
+    assert ulp_diff(got[big], ref[big]).max() <= 1
+    assert (got[big] != ref[big]).sum() <= max(1, int(big.sum()) // 100_000)
