@@ -2,13 +2,18 @@
 
 One step = one pass of the whole chain (reduction, XZ, YZ, back-partial
 cleaning, station-1 extension, output records) over one batch of synthetic
-events already resident in HBM.  Default workload: 1,000,000 events, hits per
-event ~ Poisson(300) capped at 1000, 1-2 muon tracks, basic_geometry_e906.
+events already resident in HBM.  Default workload: the learned-pruning
+configuration, 4,000,000 events with 8 % noise occupancy per channel (about
+406 hits per event), 1-2 muon tracks, basic_geometry_e906 (--workload occ8);
+--workload poisson300 runs 1,000,000 events with Poisson(300) hits capped at
+1000 instead.
 
 The same batch runs twice: the plain chain, then the chain with the shipped
 learned YZ unit pruner (e1039-gpu_amd/data/pruner_u3v3.npz).  The value is the
 pruned chain's events/s; the plain rate, the speedup and the pruned chain's
-track recall (plain-chain tracks it outputs identically) go beside it.
+track recall (plain-chain tracks it outputs identically) go beside it, and the
+same three figures for the other workload (1M events each) go in
+config.other_workload.
 
 Prints one JSON line with the throughput, the HBM roofline of the chain
 kernel (algorithmic bytes per event: 12 per input hit, 1 bit per hit of keep
@@ -25,6 +30,13 @@ sys.path.insert(0, os.path.join(ROOT, "e1039-gpu_amd"))
 
 METRIC = "events/s/GPU, learned-pruned vs plain GPU chain, at ≥99.5% plain-track recall"
 HBM_PEAK_GBS = 8000.0
+# generator settings (include/e1039_or_gen.h) and default event counts
+WORKLOADS = {
+    "occ8": (dict(mult_mode=0, noise_occupancy=0.08), 4_000_000,
+             "8 % noise occupancy per channel (learned-pruning configuration)"),
+    "poisson300": (dict(mult_mode=1, mult_mean=300.0, mult_cap=1000), 1_000_000,
+                   "hits/event Poisson(300) capped at 1000"),
+}
 
 
 def log(msg):
@@ -47,9 +59,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--events", type=int, default=1_000_000)
-    ap.add_argument("--mean-hits", type=float, default=300.0)
-    ap.add_argument("--cap-hits", type=int, default=1000)
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="occ8")
+    ap.add_argument("--events", type=int, default=0, help="0: the workload's default")
+    ap.add_argument("--other", action=argparse.BooleanOptionalAction, default=True,
+                    help="also time plain and pruned on the other workload (1M events)")
     ap.add_argument("--seed", type=int, default=0xE1039)
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="target CPU time of the CPU baseline sample (0 = skip the baseline)")
@@ -84,7 +97,8 @@ def main():
         sys.exit(1)
     geo = oc.Geometry.load()
     t0 = time.time()
-    ev = oc.generate(geo, args.events, seed=args.seed, mult_mode=1, mult_mean=args.mean_hits, mult_cap=args.cap_hits)
+    gen, n_default, wl_text = WORKLOADS[args.workload]
+    ev = oc.generate(geo, args.events or n_default, seed=args.seed, **gen)
     log(f"generated {ev.n_events} events, {ev.n_hits} hits ({ev.n_hits / max(1, ev.n_events):.1f}/event) "
         f"in {time.time() - t0:.1f}s")
     params = oc.default_params(max_hits_per_event=1024)
@@ -92,7 +106,7 @@ def main():
     dev_in = sess.upload(ev)
     torch.cuda.synchronize()
 
-    def timed(label, steps=None, warmup=None):
+    def timed(label, steps=None, warmup=None, ev=ev, dev_in=dev_in):
         steps = args.steps if steps is None else steps
         warmup = args.warmup if warmup is None else warmup
         out = sess.alloc_outputs(ev.n_events, ev.n_hits, tracklets=True)
@@ -134,6 +148,26 @@ def main():
                 f"{plain_ms / pr_ms:.2f}x the plain chain")
             prune = {"ms": pr_ms, "kms": pr_kms, "res": pr_res, "recall": rec, "found": found, "total": total,
                      "extra": extra, "threshold": thr}
+    other = None
+    if args.other and prune:
+        # the other workload, 1M events: plain and pruned rates and the recall
+        oname = [w for w in WORKLOADS if w != args.workload][0]
+        ogen, _, otext = WORKLOADS[oname]
+        oev = oc.generate(geo, 1_000_000, seed=args.seed, **ogen)
+        odev = sess.upload(oev)
+        torch.cuda.synchronize()
+        o_plain_ms, _, o_plain = timed(f"{oname} plain", steps=3, warmup=1, ev=oev, dev_in=odev)
+        sess.set_pruner(flat, thr)
+        o_pr_ms, _, o_pr = timed(f"{oname} pruned", steps=3, warmup=1, ev=oev, dev_in=odev)
+        sess.set_pruner(None)
+        orec, ofound, ototal = pruner.recall(o_plain, o_pr)
+        other = {"workload": f"{oev.n_events} events, {otext}, {oev.n_hits / oev.n_events:.0f} hits/event",
+                 "plain_events_per_s": oev.n_events / (o_plain_ms * 1e-3),
+                 "pruned_events_per_s": oev.n_events / (o_pr_ms * 1e-3), "speedup_vs_plain": o_plain_ms / o_pr_ms,
+                 "track_recall": orec, "plain_tracks": ototal, "pruned_tracks_identical": ofound}
+        log(f"{oname}: plain {other['plain_events_per_s']:.4g} events/s, pruned {other['pruned_events_per_s']:.4g} "
+            f"({other['speedup_vs_plain']:.2f}x), recall {orec:.5f}")
+        del odev, oev
     ms_per_step = prune["ms"] if prune else plain_ms
     kernel_ms = prune["kms"] if prune else plain_kms
     res = prune["res"] if prune else plain_res
@@ -209,8 +243,9 @@ def main():
         log(f"host-buffer path (H2D + plain chain + D2H): {e2e:.0f} events/s on {n_e2e} events")
 
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", "chain_kernel_1M_poisson300_pruned_traffic.json" if prune else
-                         "chain_kernel_1M_poisson300_traffic.json")
+    tlabel = f"{ev.n_events // 1_000_000}M_{args.workload}"
+    tfile = os.path.join(ROOT, "profiles", f"chain_kernel_{tlabel}_pruned_traffic.json" if prune else
+                         f"chain_kernel_{tlabel}_traffic.json")
     if os.path.exists(tfile):
         with open(tfile) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
@@ -257,7 +292,7 @@ def main():
         "dtype": "fp32",
         "data": "synthetic",
         "config": {
-            "workload": f"{ev.n_events} events, hits/event Poisson({args.mean_hits:g}) capped at {args.cap_hits}, "
+            "workload": f"{ev.n_events} events, {wl_text}, {ev.n_hits / ev.n_events:.0f} hits/event, "
                         f"1-2 muon tracks, basic_geometry_e906, reference caps",
             "chain": "learned-pruned (YZ station-3 units, bf16 perceptron on the matrix cores)" if prune else "plain",
             "events": ev.n_events,
@@ -272,6 +307,7 @@ def main():
             "pruned_other_tracks": prune["extra"] if prune else None,
             "prune_threshold": prune["threshold"] if prune else None,
             "host_buffer_events_per_s": e2e,
+            "other_workload": other,
             "final_fit_vs_fp64": dict(fit_rep, sample=f"output tracks of the first {n_fit} events, plain chain"),
             "dimuons": {"events": n_dim_ev, "dimuons": n_dimuons, "kernel_ms": dim_ms,
                         "events_per_s": n_dim_ev / (dim_ms * 1e-3),

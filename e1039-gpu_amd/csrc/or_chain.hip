@@ -1376,20 +1376,35 @@ __device__ __attribute__((noinline)) UnitPass yz_pass1_units(Scr S, Prn R, int k
     float zv[4][2][2];
     yz_plane_z(zv);
     int np = 0;
+    int lo0 = 0;   // last kept unit starting at or before i0 (the same for every lane)
     for (uint32_t i0 = 0; i0 < run; i0 += WAVE) {
         const uint32_t i = i0 + (uint32_t)lane;
+        {
+            int hi = nk;
+            while (hi - lo0 > 1) {
+                const int mid = (lo0 + hi) >> 1;
+                if (S.kstart()[mid] <= i0) lo0 = mid;
+                else hi = mid;
+            }
+        }
+        // every kept unit has at least one candidate, so the wave's 64 candidates
+        // lie in units lo0 .. lo0 + 63: lane t holds the start of unit lo0 + t, and
+        // each lane finds the last start at or before its candidate by halving
+        const uint32_t st = lo0 + lane < nk ? S.kstart()[lo0 + lane] : 0xFFFFFFFFu;
+        int t = 0;
+#pragma unroll
+        for (int step = WAVE / 2; step > 0; step >>= 1) {
+            const uint32_t sv = __shfl(st, t + step, WAVE);
+            if (sv <= i) t += step;
+        }
+        const uint32_t st_t = __shfl(st, t, WAVE);
         bool pass = false;
         int iorig = 0;
         if (i < run) {
-            int lo = 0, hi = nk;   // last kept unit starting at or before i
-            while (hi - lo > 1) {
-                const int mid = (lo + hi) >> 1;
-                if (S.kstart()[mid] <= i) lo = mid;
-                else hi = mid;
-            }
+            const int lo = lo0 + t;
             const uint32_t code = S.klist()[lo];
             const int j = (int)(code >> 16), iuv3 = (int)(code & 0xFFFFu);
-            const int inner = (int)(i - S.kstart()[lo]);
+            const int inner = (int)(i - st_t);
             const int4 nl = *reinterpret_cast<const int4*>(L.nl[j]);
             int ojs = off[0];
 #pragma unroll

@@ -1,10 +1,11 @@
 """Cycles per chain stage on the GPU, from the profiling build.
 
     make -C e1039-gpu_amd/csrc timing
-    python e1039-gpu_amd/stage_cycles.py [n_events] [mean_hits] [pruned]
+    python e1039-gpu_amd/stage_cycles.py [n_events] [mean_hits | occ8] [pruned]
 
 Runs one warm-up and one measured launch of the chain on seeded events
-(hits/event ~ Poisson(mean_hits) capped at 1000) through libe1039or_timing.so
+(hits/event ~ Poisson(mean_hits) capped at 1000, or 8 % noise occupancy per
+channel with occ8) through libe1039or_timing.so
 and prints each stage's share of the summed wave cycles.  With "pruned" the
 shipped YZ unit pruner is on.
 """
@@ -28,12 +29,15 @@ def main():
     import orchain as oc
 
     n = int(sys.argv[1]) if len(sys.argv) > 1 else 200_000
-    mean = float(sys.argv[2]) if len(sys.argv) > 2 else 300.0
+    wl = sys.argv[2] if len(sys.argv) > 2 else "300"
+    # "occ8": 8 % noise occupancy per channel; a number: Poisson mean hits capped at 1000
+    gen = dict(mult_mode=0, noise_occupancy=0.08) if wl == "occ8" else dict(mult_mode=1, mult_mean=float(wl),
+                                                                              mult_cap=1000)
     lib = oc.load_library(os.path.join(HERE, "libe1039or_timing.so"))
     lib.or_debug_stage_cycles.argtypes = [C.c_void_p, C.c_int]
     lib.or_debug_stage_cycles.restype = C.c_int32
     geo = oc.Geometry.load()
-    ev = oc.generate(geo, n, seed=7, mult_mode=1, mult_mean=mean, mult_cap=1000)
+    ev = oc.generate(geo, n, seed=7, **gen)
     sess = oc.Session(geo, oc.default_params(max_hits_per_event=1024), device=0)
     if len(sys.argv) > 3 and sys.argv[3] == "pruned":
         import pruner
@@ -52,7 +56,8 @@ def main():
     lib.or_debug_sub_cycles(sub, 1)
     sess.run_device(din, dout)
     torch.cuda.synchronize()
-    print(f"{n} events, Poisson({mean:g}) hits/event: chain kernels {sess.last_kernel_ms():.1f} ms")
+    desc = "8 % noise occupancy" if wl == "occ8" else f"Poisson({float(wl):g}) hits/event"
+    print(f"{n} events, {desc} ({ev.n_hits / n:.0f} hits/event): chain kernels {sess.last_kernel_ms():.1f} ms")
     lib.or_debug_stage_cycles(buf, 0)
     lib.or_debug_sub_cycles(sub, 0)
     a = np.array(list(buf), dtype=np.float64).reshape(5, 7)

@@ -92,12 +92,14 @@ def test_keep_all_units_is_the_plain_chain(oc, orc, geo, ogeo, psess):
             assert key in set(wkeys.tolist()) or key in unknown
 
 
-def test_shipped_pruner_recall(oc, geo, psess):
+@pytest.mark.parametrize("gen", [dict(mult_mode=1, mult_mean=300.0, mult_cap=1000),
+                                 dict(mult_mode=0, noise_occupancy=0.08)], ids=["poisson300", "occ8"])
+def test_shipped_pruner_recall(oc, geo, psess, gen):
     import pruner
     if not os.path.exists(oc.DEFAULT_PRUNER):
         pytest.skip("no shipped pruner weights")
     flat, thr, meta = pruner.load()
-    ev = oc.generate(geo, 20000, seed=0x5EED, mult_mode=1, mult_mean=300.0, mult_cap=1000)
+    ev = oc.generate(geo, 20000, seed=0x5EED, **gen)
     plain = psess.reconstruct(ev)
     psess.set_pruner(flat, thr)
     try:
