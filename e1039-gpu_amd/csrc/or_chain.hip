@@ -126,9 +126,16 @@ __device__ unsigned long long g_sub_cycles[N_SUB_TIMERS];
         if (threadIdx.x == 0) atomicAdd(&g_sub_cycles[k], s_1 - s_0);          \
         s_0 = s_1;                                                             \
     } while (0)
+// counts instead of cycles in slots 10..14 (YZ candidates, units scored,
+// units kept, candidates of kept units, first-cut survivors)
+#define OR_COUNT(k, v)                                                         \
+    do {                                                                       \
+        if (threadIdx.x == 0) atomicAdd(&g_sub_cycles[k], (unsigned long long)(v)); \
+    } while (0)
 #else
 #define OR_SUB_START
 #define OR_SUB(k) do { } while (0)
+#define OR_COUNT(k, v) do { } while (0)
 #endif  // YZ candidates passing the first box cut, per chunk
 constexpr uint16_t NOHIT = 0xFFFF;
 
@@ -1369,6 +1376,9 @@ __device__ __attribute__((noinline)) UnitPass yz_pass1_units(Scr S, Prn R, int k
     }
     __syncthreads();
     OR_SUB(9);
+    OR_COUNT(11, U);
+    OR_COUNT(12, nk);
+    OR_COUNT(13, run);
     int off[YZ_CHUNK + 1];
 #pragma unroll
     for (int t = 0; t <= YZ_CHUNK; ++t) off[t] = __builtin_amdgcn_readfirstlane(L.uoff[t]);
@@ -1379,30 +1389,32 @@ __device__ __attribute__((noinline)) UnitPass yz_pass1_units(Scr S, Prn R, int k
     int lo0 = 0;   // last kept unit starting at or before i0 (the same for every lane)
     for (uint32_t i0 = 0; i0 < run; i0 += WAVE) {
         const uint32_t i = i0 + (uint32_t)lane;
-        {
-            int hi = nk;
-            while (hi - lo0 > 1) {
-                const int mid = (lo0 + hi) >> 1;
-                if (S.kstart()[mid] <= i0) lo0 = mid;
-                else hi = mid;
-            }
-        }
         // every kept unit has at least one candidate, so the wave's 64 candidates
-        // lie in units lo0 .. lo0 + 63: lane t holds the start of unit lo0 + t, and
-        // each lane finds the last start at or before its candidate by halving
-        const uint32_t st = lo0 + lane < nk ? S.kstart()[lo0 + lane] : 0xFFFFFFFFu;
-        int t = 0;
+        // lie in units lo0 .. lo0 + 63: lane t holds the start and code of unit
+        // lo0 + t, and each lane finds the last start at or before its candidate
+        // by halving
+        const bool inwin = lo0 + lane < nk;
+        const uint32_t st = inwin ? S.kstart()[lo0 + lane] : 0xFFFFFFFFu;
+        const uint32_t kl = inwin ? S.klist()[lo0 + lane] : 0u;
+        int tu = 0, tn = 0;
 #pragma unroll
         for (int step = WAVE / 2; step > 0; step >>= 1) {
-            const uint32_t sv = __shfl(st, t + step, WAVE);
-            if (sv <= i) t += step;
+            const uint32_t sv = __shfl(st, tu + step, WAVE);
+            if (sv <= i) tu += step;
+            const uint32_t sn = __shfl(st, tn + step, WAVE);
+            if (sn <= i0 + WAVE) tn += step;
         }
-        const uint32_t st_t = __shfl(st, t, WAVE);
+        const uint32_t st_t = __shfl(st, tu, WAVE);
+        const uint32_t code = __shfl(kl, tu, WAVE);
+        // the next window starts at the last unit starting at or before i0 + 64;
+        // a unit past this window can start there only when the window's last
+        // unit does (then unit lo0 started at i0)
+        lo0 += tn;
+        if (tn == WAVE - 1)
+            while (lo0 + 1 < nk && S.kstart()[lo0 + 1] <= i0 + WAVE) ++lo0;
         bool pass = false;
         int iorig = 0;
         if (i < run) {
-            const int lo = lo0 + t;
-            const uint32_t code = S.klist()[lo];
             const int j = (int)(code >> 16), iuv3 = (int)(code & 0xFFFFu);
             const int inner = (int)(i - st_t);
             const int4 nl = *reinterpret_cast<const int4*>(L.nl[j]);
@@ -1453,6 +1465,7 @@ __device__ __attribute__((noinline)) bool yz_chunk_par(Scr S, Prn R, int kb, int
     }
     __syncthreads();
     const int I = L.uoff[YZ_CHUNK];
+    OR_COUNT(10, I);
     OR_SUB_START
     int np = 0;
     int64_t dbase = -1;   // unit rows of this chunk (pruning path with the dump on)
@@ -1499,6 +1512,7 @@ __device__ __attribute__((noinline)) bool yz_chunk_par(Scr S, Prn R, int kb, int
     }
     __syncthreads();
     OR_SUB(1);
+    OR_COUNT(14, np);
     // pass 2: per survivor, first-of-unit flag; the first of a unit runs the
     // full left/right from zero signs (its station-3 result seeds the unit),
     // later ones only the decision of a v3' / u2 pair (the one pair that can

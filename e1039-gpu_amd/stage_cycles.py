@@ -19,7 +19,8 @@ sys.path.insert(0, HERE)
 STAGES = ["load+reduce", "X pairs", "XZ", "YZ", "cleaning", "global", "outputs"]
 SUBS = ["YZ window lists", "YZ pass 1 (first fit)", "YZ pass 2 (seeds)", "YZ pass 3 (tails)", "YZ winners",
         "YZ unit path", "st1 window lists", "st1 pass A (y fits)", "st1 pass B + write",
-        "YZ unit scoring (in pass 1)"]
+        "YZ unit scoring (in pass 1)", "YZ candidates (count)", "YZ units scored (count)",
+        "YZ units kept (count)", "YZ candidates of kept units (count)", "YZ first-cut survivors (count)"]
 
 
 def main():
@@ -68,7 +69,7 @@ def main():
         print(f"tile class {512 << c} hits: {tot:.3e} cycles")
         for k, name in enumerate(STAGES):
             print(f"  {name:12s} {a[c, k] / tot * 100:5.1f} %  {a[c, k] / n:12.0f} cycles/event")
-    print("sub-stages (all classes), cycles/event:")
+    print("sub-stages (all classes), cycles/event (counts per event for the count rows):")
     for k, name in enumerate(SUBS):
         print(f"  {name:22s} {sub[k] / n:12.0f}")
 
