@@ -13,6 +13,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <cstddef>
 #include <cstdint>
 
 namespace ordev {
@@ -30,9 +31,13 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 struct PrunerDev {
     bf16x4 w1[2][2][64];
     bf16x4 w2[2][8][64];
-    float b1[PRUNE_NH], b2[PRUNE_NH], w3[PRUNE_NH];
+    alignas(16) float b1[PRUNE_NH];
+    alignas(16) float b2[PRUNE_NH];
+    alignas(16) float w3[PRUNE_NH];
     float b3, thr;
 };
+static_assert(offsetof(PrunerDev, b1) % 16 == 0 && offsetof(PrunerDev, b2) % 16 == 0 &&
+                  offsetof(PrunerDev, w3) % 16 == 0, "bias rows are read as float4");
 
 // fp32 -> bf16, round to nearest even (finite inputs)
 __host__ __device__ inline uint16_t bf16_rne(float x) {
@@ -48,7 +53,10 @@ struct UnitFeatures {
 
 // Score of the unit whose features this lane holds.  Every lane of the wave
 // calls it (lanes without a unit pass zeros and ignore the result).
-__device__ __attribute__((noinline)) float prune_score(const PrunerDev* W, UnitFeatures f) {
+__device__ __attribute__((noinline)) float prune_score(const PrunerDev* Wp, UnitFeatures f) {
+    // the weights live in device global memory: global loads (not flat ones, which
+    // also count against the LDS counter the feature shuffles wait on)
+    const __attribute__((address_space(1))) PrunerDev* W = (const __attribute__((address_space(1))) PrunerDev*)Wp;
     const int lane = threadIdx.x & 63, half = lane >> 5, col = lane & 31;
     float total[2];
 #pragma unroll
@@ -69,17 +77,18 @@ __device__ __attribute__((noinline)) float prune_score(const PrunerDev* W, UnitF
 #pragma unroll
             for (int kk = 0; kk < 2; ++kk) h[mt] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(W->w1[mt][kk][lane], xb[kk], h[mt], 0, 0, 0);
         }
-        // bias, ReLU, bf16: the B operand of layer 2, step mt*4 + s
+        // bias, ReLU, bf16: the B operand of layer 2, step mt*4 + s (rows
+        // mt*32 + 8s + 4*half .. +3, one 16-byte load)
         bf16x4 hb[8];
 #pragma unroll
         for (int mt = 0; mt < 2; ++mt)
 #pragma unroll
-            for (int s = 0; s < 4; ++s)
+            for (int s = 0; s < 4; ++s) {
+                const int i0 = mt * 32 + 8 * s + 4 * half;   // 16-byte aligned row of 4
+                const float bq[4] = {W->b1[i0], W->b1[i0 + 1], W->b1[i0 + 2], W->b1[i0 + 3]};
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const int i = mt * 32 + 8 * s + 4 * half + q;
-                    hb[mt * 4 + s][q] = (short)bf16_rne(fmaxf(h[mt][4 * s + q] + W->b1[i], 0.f));
-                }
+                for (int q = 0; q < 4; ++q) hb[mt * 4 + s][q] = (short)bf16_rne(fmaxf(h[mt][4 * s + q] + bq[q], 0.f));
+            }
         float p = 0.f;
 #pragma unroll
         for (int mt = 0; mt < 2; ++mt) {
@@ -87,9 +96,12 @@ __device__ __attribute__((noinline)) float prune_score(const PrunerDev* W, UnitF
 #pragma unroll
             for (int kk = 0; kk < 8; ++kk) o = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(W->w2[mt][kk][lane], hb[kk], o, 0, 0, 0);
 #pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int i = mt * 32 + 8 * (r / 4) + 4 * half + (r % 4);
-                p += fmaxf(o[r] + W->b2[i], 0.f) * W->w3[i];
+            for (int g4 = 0; g4 < 4; ++g4) {
+                const int i0 = mt * 32 + 8 * g4 + 4 * half;
+                const float bq[4] = {W->b2[i0], W->b2[i0 + 1], W->b2[i0 + 2], W->b2[i0 + 3]};
+                const float wq[4] = {W->w3[i0], W->w3[i0 + 1], W->w3[i0 + 2], W->w3[i0 + 3]};
+#pragma unroll
+                for (int q = 0; q < 4; ++q) p += fmaxf(o[4 * g4 + q] + bq[q], 0.f) * wq[q];
             }
         }
         // lanes col and col + 32 hold the two halves of the rows
