@@ -540,6 +540,18 @@ __device__ __forceinline__ bool xz_candidate(const Ev<MAXH>& E, int s3, uint16_t
 //    v2' (the same sequence of float additions as the reference).
 //  * The refit after left/right (reconstruction_kernels.cuh:1628) has the same
 //    inputs as the first fit and is not repeated.
+// Loads from the wave's scratch (global memory) issued as global, not flat,
+// operations: flat ones also count against the LDS counter that shuffles and
+// LDS reads wait on.
+template <typename T>
+__device__ __forceinline__ T gld(const T* p) {
+    return *(const __attribute__((address_space(1))) T*)p;
+}
+__device__ __forceinline__ float4 gld4(const float4* p) {
+    const __attribute__((address_space(1))) float* f = (const __attribute__((address_space(1))) float*)p;
+    return make_float4(f[0], f[1], f[2], f[3]);
+}
+
 struct YzLists {
     const uint16_t* l[4];   // pair entries (a | b << 8), views u2, v2, u3, v3
     const float4* y[4];     // (y_a, ey_a, y_b, ey_b)
@@ -550,9 +562,9 @@ struct YzLists {
 // fit sums of one list entry's accepted hits, in order a then b
 __device__ __forceinline__ int yz_sum(const YzLists& W, int v, int k, float za, float zb, float& A0, float& A1,
                                       float& A3, float& B0, float& B1) {
-    const uint8_t fl = W.f[v][k];
+    const uint8_t fl = gld(W.f[v] + k);
     if (!fl) return 0;
-    const float4 yv = W.y[v][k];
+    const float4 yv = gld4(W.y[v] + k);
     int c = 0;
     if (fl & 1) {
         B0 += yv.y * yv.x; B1 += (yv.y * yv.x) * za; A0 += yv.y; A1 += yv.y * za; A3 += (yv.y * za) * za; ++c;
@@ -583,8 +595,8 @@ __device__ __forceinline__ void yz_dets(YzSlots& H, int st3) {
 // accepted hits of entry k of view v into slots S0 (hit a) and S0 + 1 (hit b), signs 0
 template <int S0, int MAXH>
 __device__ __forceinline__ void yz_fill(const Ev<MAXH>& E, const YzLists& W, int v, int k, YzSlots& H) {
-    const uint16_t q = W.l[v][k];
-    const unsigned fl = W.f[v][k];
+    const uint16_t q = gld(W.l[v] + k);
+    const unsigned fl = gld(W.f[v] + k);
     H.pres &= ~(3u << S0);
     H.sign[S0] = 0;
     H.sign[S0 + 1] = 0;
@@ -1225,8 +1237,8 @@ __device__ __forceinline__ void yz_candidate(const Ev<MAXH>& E, const Scr& S, co
 __device__ __forceinline__ UnitFeatures unit_features(const Scr& S, const Tk& T, int j, int iuv3, int nu3) {
     const int iu3 = iuv3 % nu3, iv3 = iuv3 / nu3;
     const int qu = (j * 4 + 2) * WIN_CAP + iu3, qv = (j * 4 + 3) * WIN_CAP + iv3;
-    const float4 yu = S.wy()[qu], yw = S.wy()[qv];
-    const unsigned fu = S.wf()[qu], fv = S.wf()[qv];
+    const float4 yu = gld4(S.wy() + qu), yw = gld4(S.wy() + qv);
+    const unsigned fu = gld(S.wf() + qu), fv = gld(S.wf() + qv);
     const float ya = (fu & 1u) ? yu.x : yu.z, eya = (fu & 1u) ? yu.y : yu.w;
     const float yb = (fu & 2u) ? yu.z : yu.x, eyb = (fu & 2u) ? yu.w : yu.y;
     const float yc = (fv & 1u) ? yw.x : yw.z, eyc = (fv & 1u) ? yw.y : yw.w;
@@ -1263,8 +1275,8 @@ __device__ __forceinline__ bool yz_first_cut(const Scr& S, int j, const int (&e)
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         const int q = (j * 4 + v) * WIN_CAP + e[v];
-        fl[v] = S.wf()[q];
-        yv[v] = S.wy()[q];
+        fl[v] = gld(S.wf() + q);
+        yv[v] = gld4(S.wy() + q);
     }
     float A0 = 0.f, A1 = 0.f, A3 = 0.f, B0 = 0.f, B1 = 0.f;
     bool ok = true;
