@@ -105,6 +105,18 @@ def main():
     sess = oc.Session(geo, params, device=0)
     dev_in = sess.upload(ev)
     torch.cuda.synchronize()
+    # the autotuner (YZ candidate-list size x persistent grid size) on a slice of the batch;
+    # results never depend on these settings
+    n_tune = min(ev.n_events, 200_000)
+    tsub = ev.subset(0, n_tune)
+    tt = sess.upload(tsub)
+    to = sess.alloc_outputs(tsub.n_events, tsub.n_hits, tracklets=True)
+    tune_ms = sess.autotune(tt, to, reps=2)
+    tuning = {"grid_percent": sess.get_tuning(oc.TUNE_GRID_PERCENT),
+              "yz_list_cap": sess.get_tuning(oc.TUNE_YZ_LIST_CAP),
+              "tuned_on": f"first {n_tune} events, or_session_autotune", "best_slice_ms": tune_ms}
+    del tt, to
+    log(f"autotune: grid {tuning['grid_percent']} %, YZ list cap {tuning['yz_list_cap']} ({tune_ms:.2f} ms on the slice)")
 
     def timed(label, steps=None, warmup=None, ev=ev, dev_in=dev_in):
         steps = args.steps if steps is None else steps
@@ -306,6 +318,7 @@ def main():
             "pruned_tracks_identical": prune["found"] if prune else None,
             "pruned_other_tracks": prune["extra"] if prune else None,
             "prune_threshold": prune["threshold"] if prune else None,
+            "tuning": tuning,
             "host_buffer_events_per_s": e2e,
             "other_workload": other,
             "final_fit_vs_fp64": dict(fit_rep, sample=f"output tracks of the first {n_fit} events, plain chain"),

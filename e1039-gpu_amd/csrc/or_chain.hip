@@ -3443,6 +3443,16 @@ or_status or_session_set_tuning(or_session* s, or_tuning key, int64_t value) {
     return OR_ERR_INVALID_ARG;
 }
 
+or_status or_session_get_tuning(const or_session* s, or_tuning key, int64_t* value) {
+    if (!s || !value) return OR_ERR_INVALID_ARG;
+    switch (key) {
+        case OR_TUNE_YZ_LIST_CAP: *value = s->yz_list_cap; return OR_OK;
+        case OR_TUNE_GRID_PERCENT: *value = s->grid_pct; return OR_OK;
+        case OR_TUNE_GRAPH: *value = s->use_graph ? 1 : 0; return OR_OK;
+    }
+    return OR_ERR_INVALID_ARG;
+}
+
 // The launch sequence of one call: keep-mask fill, class counters fill,
 // classification, one persistent launch per tile class.  No memset nodes, so
 // it can be captured into a graph.
@@ -3793,22 +3803,27 @@ or_status or_session_set_unit_dump(or_session* s, float* rows, int64_t cap, unsi
 or_status or_session_autotune(or_session* s, const or_hits* in, or_result* out, int reps, float* best_ms) {
     if (!s || !in || !out || !in->on_device || !out->on_device || reps < 1) return OR_ERR_INVALID_ARG;
     static const int grid_pcts[] = {100, 75, 50};
-    int best_pct = s->grid_pct;
+    static const int list_caps[] = {YZ_LIST_CAP, YZ_LIST_CAP / 2, YZ_LIST_CAP / 4};
+    int best_pct = s->grid_pct, best_cap = s->yz_list_cap;
     float best = -1.f;
-    for (int pct : grid_pcts) {
-        s->grid_pct = pct;
-        or_status st = or_reconstruct(s, in, out);   // warm-up at this setting
-        if (st != OR_OK) return st;
-        float tot = 0.f;
-        for (int r = 0; r < reps; ++r) {
-            st = or_reconstruct(s, in, out);
+    for (int cap : list_caps) {
+        for (int pct : grid_pcts) {
+            s->grid_pct = pct;
+            s->yz_list_cap = cap;
+            or_status st = or_reconstruct(s, in, out);   // warm-up at this setting
             if (st != OR_OK) return st;
-            tot += or_session_last_kernel_ms(s);
+            float tot = 0.f;
+            for (int r = 0; r < reps; ++r) {
+                st = or_reconstruct(s, in, out);
+                if (st != OR_OK) return st;
+                tot += or_session_last_kernel_ms(s);
+            }
+            const float ms = tot / (float)reps;
+            if (best < 0.f || ms < best) { best = ms; best_pct = pct; best_cap = cap; }
         }
-        const float ms = tot / (float)reps;
-        if (best < 0.f || ms < best) { best = ms; best_pct = pct; }
     }
     s->grid_pct = best_pct;
+    s->yz_list_cap = best_cap;
     if (best_ms) *best_ms = best;
     return OR_OK;
 }

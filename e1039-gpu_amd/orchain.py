@@ -143,6 +143,8 @@ def load_library(path: str = LIB_PATH):
     lib.or_session_last_kernel_ms.restype = C.c_float
     lib.or_session_set_tuning.argtypes = [vp, C.c_int, i64]
     lib.or_session_set_tuning.restype = i32
+    lib.or_session_get_tuning.argtypes = [vp, C.c_int, C.POINTER(i64)]
+    lib.or_session_get_tuning.restype = i32
     lib.or_session_autotune.argtypes = [vp, C.POINTER(OrHits), C.POINTER(OrResult), C.c_int, C.POINTER(C.c_float)]
     lib.or_session_autotune.restype = i32
     lib.or_session_set_pruner.argtypes = [vp, C.POINTER(C.c_float), i64, C.c_float]
@@ -377,6 +379,12 @@ class Session:
     def set_tuning(self, key: int, value: int):
         """or_session_set_tuning; key e.g. TUNE_YZ_LIST_CAP."""
         _check(self.lib.or_session_set_tuning(self.handle, key, value))
+
+    def get_tuning(self, key: int) -> int:
+        """or_session_get_tuning."""
+        v = C.c_int64(0)
+        _check(self.lib.or_session_get_tuning(self.handle, key, C.byref(v)))
+        return int(v.value)
 
     def set_calibration(self, cal: "Calibration | None"):
         """or_session_set_calibration: the chain derives drift from tdc for calibrated detectors (None: off)."""

@@ -217,11 +217,17 @@ typedef enum {
 } or_tuning;
 or_status or_session_set_tuning(or_session* s, or_tuning key, int64_t value);
 
-/* Times the device-resident batch (in, out) at each persistent grid size
- * (100, 75, 50 % of the occupancy limit; reps timed calls after one warm-up
- * each), keeps the fastest setting in the session and returns its mean chain
- * time in *best_ms.  Outputs are those of the last call.  No reference
- * counterpart. */
+/* Reads a tuning knob of the session into *value. */
+or_status or_session_get_tuning(const or_session* s, or_tuning key, int64_t* value);
+
+/* Times the device-resident batch (in, out) at each combination of YZ
+ * candidate-list size (4096, 2048, 1024 per chunk of 16 tracklets: the tile
+ * of candidates evaluated in parallel) and persistent grid size (100, 75, 50 %
+ * of the occupancy limit); reps timed calls after one warm-up each.  Keeps the
+ * fastest combination in the session and returns its mean chain time in
+ * *best_ms.  Outputs are those of the last call.  One wave per event is fixed
+ * by the design (the event's hits, lists and tracklets are staged for one
+ * wave), so it is not swept.  No reference counterpart. */
 or_status or_session_autotune(or_session* s, const or_hits* in, or_result* out, int reps, float* best_ms);
 
 /* ---- R-T calibration: drift distance from the TDC time ---------------------

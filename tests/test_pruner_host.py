@@ -70,3 +70,5 @@ def test_abi_argument_checks(oc):
     assert lib.or_session_set_pruner(None, w.ctypes.data_as(C.POINTER(C.c_float)), 5313, 0.0) == 1
     assert lib.or_pruner_score(None, None, 0, None) == 1
     assert lib.or_session_set_unit_dump(None, None, 0, None) == 1
+    v = C.c_int64(0)
+    assert lib.or_session_get_tuning(None, 1, C.byref(v)) == 1
