@@ -230,7 +230,8 @@ constexpr int64_t SCR_PB = SCR_PL + (int64_t)YZ_LIST_CAP * 4;                   
 constexpr int UNIT_CAP = 4096;                                                       // YZ units scored per chunk
 constexpr int64_t SCR_KL = SCR_PB + (int64_t)YZ_LIST_CAP * 4;                       // uint32 [UNIT_CAP] kept units
 constexpr int64_t SCR_KS = SCR_KL + (int64_t)UNIT_CAP * 4;                          // uint32 [UNIT_CAP] their first candidate
-constexpr int64_t SCR_STRIDE = ((SCR_KS + (int64_t)UNIT_CAP * 4 + 4095) / 4096) * 4096;
+constexpr int64_t SCR_YC = ((SCR_KS + (int64_t)UNIT_CAP * 4 + 255) / 256) * 256;     // uint4 x3 [YZ_LIST_CAP] candidate cache
+constexpr int64_t SCR_STRIDE = ((SCR_YC + (int64_t)YZ_LIST_CAP * 48 + 4095) / 4096) * 4096;
 
 __device__ __forceinline__ unsigned long long ballot(bool p) { return __ballot(p); }
 __device__ __forceinline__ int lane_id() { return threadIdx.x; }
@@ -754,6 +755,9 @@ struct Scr {
     // candidate in the kept enumeration
     __device__ uint32_t* klist() const { return reinterpret_cast<uint32_t*>(base + SCR_KL); }
     __device__ uint32_t* kstart() const { return reinterpret_cast<uint32_t*>(base + SCR_KS); }
+    // per box-cut survivor of a chunk: its hits (raw LDS indices), which slots
+    // hold one, and its first fit, as pass 2 built them (3 x 16 bytes)
+    __device__ uint4* ycache() const { return reinterpret_cast<uint4*>(base + SCR_YC); }
 };
 
 // Learned unit pruning and unit rows for the YZ stage, by value from the
@@ -1453,6 +1457,45 @@ __device__ __attribute__((noinline)) UnitPass yz_pass1_units(Scr S, Prn R, int k
     return res;
 }
 
+// Candidate cache of pass 2 -> pass 3: raw LDS index of each slot's hit
+// (16 bits each), the slot presence bits and the first fit.
+__device__ __forceinline__ void yz_cache_put(uint4* c, const YzSlots& H, const Fit& f) {
+    uint32_t r[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const uint32_t lo = ((H.pres >> (2 * t)) & 1u) ? (uint32_t)H.raw[2 * t] : 0u;
+        const uint32_t hi = ((H.pres >> (2 * t + 1)) & 1u) ? (uint32_t)H.raw[2 * t + 1] : 0u;
+        r[t] = lo | (hi << 16);
+    }
+    c[0] = make_uint4(r[0], r[1], r[2], r[3]);
+    c[1] = make_uint4(H.pres, __float_as_uint(f.p0), __float_as_uint(f.p1), __float_as_uint(f.e0));
+    c[2] = make_uint4(__float_as_uint(f.e1), 0u, 0u, 0u);
+}
+
+// The candidate exactly as yz_candidate builds it, from the cache: slot
+// detectors of the station-3 choice, hits of the present slots from LDS, signs 0.
+template <int MAXH>
+__device__ __forceinline__ void yz_cache_get(const Ev<MAXH>& E, const uint4* c, int st3, YzSlots& H, Fit& f) {
+    const uint4 a = c[0], b = c[1], d = c[2];
+    const uint32_t r[4] = {a.x, a.y, a.z, a.w};
+    yz_dets(H, st3);
+    H.pres = b.x;
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+        H.sign[s] = 0;
+        if ((H.pres >> s) & 1u) {
+            const int raw = (int)((r[s >> 1] >> (16 * (s & 1))) & 0xFFFFu);
+            H.raw[s] = raw;
+            H.pos[s] = E.Lp->pos[raw];
+            H.drift[s] = E.Lp->drift[raw];
+        }
+    }
+    f.p0 = __uint_as_float(b.y);
+    f.p1 = __uint_as_float(b.z);
+    f.e0 = __uint_as_float(b.w);
+    f.e1 = __uint_as_float(d.x);
+}
+
 // Two-pass form.  Returns false (nothing written) when the survivors of the
 // first box cut do not fit the list.
 template <int MAXH>
@@ -1543,6 +1586,7 @@ __device__ __attribute__((noinline)) bool yz_chunk_par(Scr S, Prn R, int kb, int
             YzSlots H;
             Fit f;
             yz_candidate(E, S, Tp, it, H, f);
+            yz_cache_put(S.ycache() + 3 * k, H, f);
             uchar4 ps = make_uchar4(first ? 1 : 0, 0, 0, 0);
             if (first) {
                 lr_slots_new<8>(Tp->x0, Tp->tx, f.p0, f.p1, Tp->ex0, Tp->etx, f.e0, f.e1, H.pres, H.det, H.pos,
@@ -1580,7 +1624,7 @@ __device__ __attribute__((noinline)) bool yz_chunk_par(Scr S, Prn R, int kb, int
             Tk* Tp = &tks[kb + it.j];
             YzSlots H;
             Fit f;
-            yz_candidate(E, S, Tp, it, H, f);
+            yz_cache_get(E, S.ycache() + 3 * k, Tp->st3, H, f);
             uint8_t st = 0;
             if (!ps.x) {
                 // seed of the unit with the v3' sign (slot 3) of the last writer; the
@@ -1626,7 +1670,7 @@ __device__ __attribute__((noinline)) bool yz_chunk_par(Scr S, Prn R, int kb, int
             const YzItem it = yz_decode<MAXH>(i);
             YzSlots H;
             Fit f;
-            yz_candidate(E, S, Tp, it, H, f);
+            yz_cache_get(E, S.ycache() + 3 * lo, Tp->st3, H, f);
             if (!S.pst()[lo].x) yz_unpack3(H, S.pst()[lo].w);
             float chi2;
             yz_tail(E, Tp, H, f, chi2, Tp);
