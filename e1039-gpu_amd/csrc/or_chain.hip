@@ -231,7 +231,8 @@ constexpr int UNIT_CAP = 4096;                                                  
 constexpr int64_t SCR_KL = SCR_PB + (int64_t)YZ_LIST_CAP * 4;                       // uint32 [UNIT_CAP] kept units
 constexpr int64_t SCR_KS = SCR_KL + (int64_t)UNIT_CAP * 4;                          // uint32 [UNIT_CAP] their first candidate
 constexpr int64_t SCR_YC = ((SCR_KS + (int64_t)UNIT_CAP * 4 + 255) / 256) * 256;     // uint4 x3 [YZ_LIST_CAP] candidate cache
-constexpr int64_t SCR_STRIDE = ((SCR_YC + (int64_t)YZ_LIST_CAP * 48 + 4095) / 4096) * 4096;
+constexpr int64_t SCR_QL = SCR_YC + (int64_t)YZ_LIST_CAP * 48;                       // uint32 [YZ_LIST_CAP] pass-3 list
+constexpr int64_t SCR_STRIDE = ((SCR_QL + (int64_t)YZ_LIST_CAP * 4 + 4095) / 4096) * 4096;
 
 __device__ __forceinline__ unsigned long long ballot(bool p) { return __ballot(p); }
 __device__ __forceinline__ int lane_id() { return threadIdx.x; }
@@ -627,17 +628,12 @@ __device__ __forceinline__ unsigned lr_slot_starts(unsigned pres, const int (&de
     return m;
 }
 
-// The rest of a candidate after the first box cut: left/right on the UV hits
-// (station-3 signs in H as they stand), second box cut, prop-tube Y and
-// hodoscope matches, chi2 over the x hits (stored signs) and the UV hits
-// (reconstruction_helper.cuh:1182-1204).  With Tw set the candidate is written
-// into the tracklet.
+// The tests of the tail that need no left/right sign: second box cut, prop-tube
+// Y and hodoscope matches, on the x line and the first fit alone.
 template <int MAXH>
-__device__ __forceinline__ bool yz_tail(const Ev<MAXH>& E, const Tk* Tp, YzSlots& H, const Fit& f, float& chi2_out,
-                                        Tk* Tw) {
+__device__ __forceinline__ bool yz_cuts(const Ev<MAXH>& E, const Tk* Tp, const Fit& f) {
     const float x0 = Tp->x0, tx = Tp->tx, ex0 = Tp->ex0, etx = Tp->etx;
     const float y0 = f.p0, ty = f.p1, ey0 = f.e0, ety = f.e1;
-    lr_slots_new<8>(x0, tx, y0, ty, ex0, etx, ey0, ety, H.pres, H.det, H.pos, H.drift, H.sign, 150.f);
     if (fabsf(y0) > Y0_MAX || fabsf(ty) > TY_MAX) return false;
     // prop Y: P1Y (48 then 47) within 5.08, P2Y (54 then 53) within 10.16
     const int nprop = prop_hit(E, 48, 47, ty, y0, 5.08f) + prop_hit(E, 54, 53, ty, y0, 10.16f);
@@ -651,6 +647,16 @@ __device__ __forceinline__ bool yz_tail(const Ev<MAXH>& E, const Tk* Tp, YzSlots
     if (!hodo_tkl(E, 3, 45, x0, y0, tx, ty, ex0, ey0, etx, ety) &&
         !hodo_tkl(E, 3, 46, x0, y0, tx, ty, ex0, ey0, etx, ety))
         return false;
+    return true;
+}
+
+// chi2 over the x hits (stored signs) and the UV hits (signs in H); with Tw set
+// the candidate is written into the tracklet
+template <int MAXH>
+__device__ __forceinline__ void yz_chi2(const Ev<MAXH>& E, const Tk* Tp, const YzSlots& H, const Fit& f,
+                                        float& chi2_out, Tk* Tw) {
+    const float x0 = Tp->x0, tx = Tp->tx;
+    const float y0 = f.p0, ty = f.p1, ey0 = f.e0, ety = f.e1;
     float chi2 = 0.f;
     const int nx = Tp->nx;
     for (int k = 0; k < nx; ++k) {
@@ -682,6 +688,20 @@ __device__ __forceinline__ bool yz_tail(const Ev<MAXH>& E, const Tk* Tp, YzSlots
         Tw->chi2 = chi2; Tw->y0 = y0; Tw->ty = ty; Tw->ey0 = ey0; Tw->ety = ety;
     }
     chi2_out = chi2;
+}
+
+// The rest of a candidate after the first box cut: left/right on the UV hits
+// (station-3 signs in H as they stand), second box cut, prop-tube Y and
+// hodoscope matches, chi2 over the x hits (stored signs) and the UV hits
+// (reconstruction_helper.cuh:1182-1204).  With Tw set the candidate is written
+// into the tracklet.
+template <int MAXH>
+__device__ __forceinline__ bool yz_tail(const Ev<MAXH>& E, const Tk* Tp, YzSlots& H, const Fit& f, float& chi2_out,
+                                        Tk* Tw) {
+    lr_slots_new<8>(Tp->x0, Tp->tx, f.p0, f.p1, Tp->ex0, Tp->etx, f.e0, f.e1, H.pres, H.det, H.pos, H.drift, H.sign,
+                    150.f);
+    if (!yz_cuts(E, Tp, f)) return false;
+    yz_chi2(E, Tp, H, f, chi2_out, Tw);
     return true;
 }
 
@@ -758,6 +778,8 @@ struct Scr {
     // per box-cut survivor of a chunk: its hits (raw LDS indices), which slots
     // hold one, and its first fit, as pass 2 built them (3 x 16 bytes)
     __device__ uint4* ycache() const { return reinterpret_cast<uint4*>(base + SCR_YC); }
+    // survivors of a chunk that pass the sign-free tests, in order
+    __device__ uint32_t* qlist() const { return reinterpret_cast<uint32_t*>(base + SCR_QL); }
 };
 
 // Learned unit pruning and unit rows for the YZ stage, by value from the
@@ -1604,8 +1626,9 @@ __device__ __attribute__((noinline)) bool yz_chunk_par(Scr S, Prn R, int kb, int
     __syncthreads();
     OR_SUB(2);
     // pass 3: starting signs by scan (the unit's seed, and the last writer of
-    // the v3' sign before this candidate), then the rest of the candidate
-    int carry_f = -1, carry_w = -1;
+    // the v3' sign before this candidate) and the tests that need no sign; the
+    // candidates that pass them are listed and run the rest (left/right, chi2)
+    int carry_f = -1, carry_w = -1, nq = 0;
     for (int k0 = 0; k0 < np; k0 += WAVE) {
         const int k = k0 + lane;
         const bool valid = k < np;
@@ -1619,24 +1642,47 @@ __device__ __attribute__((noinline)) bool yz_chunk_par(Scr S, Prn R, int kb, int
         wexcl = max(wexcl, carry_w);
         carry_f = __shfl(fi, WAVE - 1, WAVE);
         carry_w = max(carry_w, __shfl(wincl, WAVE - 1, WAVE));
+        bool go = false;
         if (valid) {
             const YzItem it = yz_decode<MAXH>((int)S.plist()[k]);
-            Tk* Tp = &tks[kb + it.j];
-            YzSlots H;
+            const uint4 b = S.ycache()[3 * k + 1];
             Fit f;
-            yz_cache_get(E, S.ycache() + 3 * k, Tp->st3, H, f);
+            f.p0 = __uint_as_float(b.y);
+            f.p1 = __uint_as_float(b.z);
+            f.e0 = __uint_as_float(b.w);
+            f.e1 = __uint_as_float(S.ycache()[3 * k + 2].x);
             uint8_t st = 0;
             if (!ps.x) {
                 // seed of the unit with the v3' sign (slot 3) of the last writer; the
                 // first survivor of the unit is a writer, so wexcl >= fi
                 st = (uint8_t)((S.pst()[fi].y & 0x3Fu) | ((unsigned)S.pst()[wexcl].z << 6));
-                yz_unpack3(H, st);
             }
             ps.w = st;
             S.pst()[k] = ps;
+            go = yz_cuts(E, &tks[kb + it.j], f);
+        }
+        const unsigned long long m = ballot(go);
+        if (go) S.qlist()[nq + rank_in(m)] = (uint32_t)k;
+        nq += __popcll(m);
+    }
+    __syncthreads();
+    for (int q0 = 0; q0 < nq; q0 += WAVE) {
+        const int q = q0 + lane;
+        if (q < nq) {
+            const int k = (int)S.qlist()[q];
+            const YzItem it = yz_decode<MAXH>((int)S.plist()[k]);
+            const uchar4 ps = S.pst()[k];
+            const Tk* Tp = &tks[kb + it.j];
+            YzSlots H;
+            Fit f;
+            yz_cache_get(E, S.ycache() + 3 * k, Tp->st3, H, f);
+            if (!ps.x) yz_unpack3(H, ps.w);
+            lr_slots_new<8>(Tp->x0, Tp->tx, f.p0, f.p1, Tp->ex0, Tp->etx, f.e0, f.e1, H.pres, H.det, H.pos, H.drift,
+                            H.sign, 150.f);
             float chi2;
+            yz_chi2(E, Tp, H, f, chi2, nullptr);
             // the reference starts its minimum at 10000.1 (reconstruction_kernels.cuh:1240)
-            if (yz_tail(E, Tp, H, f, chi2, nullptr) && chi2 < 10000.1f) {
+            if (chi2 < 10000.1f) {
                 const unsigned long long key =
                     ((unsigned long long)__float_as_uint(chi2) << 32) | (unsigned long long)(uint32_t)it.local;
                 atomicMin(&L.best[it.j], key);
