@@ -2139,6 +2139,12 @@ __device__ __forceinline__ bool global_xpart(const Ev<MAXH>& E, uint16_t qx, flo
     return !(X.invp > INVP_MAX + einvp || X.invp < INVP_MIN - einvp);
 }
 
+// one accepted UV hit into the y-fit sums
+__device__ __forceinline__ void y_acc(float& A0, float& A1, float& A3, float& B0, float& B1, float y, float ey,
+                                      float z) {
+    B0 += ey * y; B1 += (ey * y) * z; A0 += ey; A1 += ey * z; A3 += (ey * z) * z;
+}
+
 // y fit of (u, v) pair iuv of an x entry: the station-2/3 sums As, then the
 // accepted U hits (a then b) and, when there is one, the accepted V hits.
 // False when the pair does not reach the fit (no accepted U or V hit).  With
@@ -2165,8 +2171,7 @@ __device__ __forceinline__ bool global_yfit(const Ev<MAXH>& E, const uint16_t* l
                 pres |= 1u << s;
                 raw[s] = r; pos[s] = E.Lp->pos[r]; drift[s] = E.Lp->drift[r];
             }
-            const float z = G(F_Z, d);
-            B0 += ey * y; B1 += (ey * y) * z; A0 += ey; A1 += ey * z; A3 += (ey * z) * z;
+            y_acc(A0, A1, A3, B0, B1, y, ey, G(F_Z, d));
             if (s < 4) ++nuh;
             else ++nvh;
         }
@@ -2176,11 +2181,82 @@ __device__ __forceinline__ bool global_yfit(const Ev<MAXH>& E, const uint16_t* l
     return true;
 }
 
+// Pass-A tables of one x entry: per U entry the sums As with its accepted hits
+// added (a then b) and their count; per V entry its accepted hits (y, ey) and
+// which of a, b they are.  A (u, v) pair then adds the V hits to the U row,
+// the same operations in the same order as global_yfit.
+struct GTab {
+    float4* u;      // [WIN_CAP][2]: A0, A1, A3, B0 | B1, U hits
+    float4* v;      // [WIN_CAP]: y a, ey a, y b, ey b
+    uint32_t* vf;   // [WIN_CAP]: bit 0 hit a accepted, bit 1 hit b
+};
+
+template <int MAXH>
+__device__ __forceinline__ void global_tables(const Ev<MAXH>& E, const GTab& T, const uint16_t* lu,
+                                              const uint16_t* lv, int nu1, int nv1, const GX& X,
+                                              const float (&As)[5]) {
+    constexpr int det6[6] = {4, 3, 2, 1, 6, 5};
+    const int lane = lane_id();
+    for (int iu = lane; iu < nu1; iu += WAVE) {
+        float A0 = As[0], A1 = As[1], A3 = As[2], B0 = As[3], B1 = As[4];
+        const uint16_t q = lu[iu];
+        const int ui[2] = {pa(q), pb(q)};
+        int nuh = 0;
+#pragma unroll
+        for (int s = 2; s < 4; ++s) {
+            const int idx = ui[s - 2];
+            if (idx < 0) continue;
+            const int d = det6[s];
+            const int r = E.raw(d, idx);
+            float y, ey;
+            if (y_uvhit(d, E.Lp->elem[r], E.Lp->drift[r], 0, X.x01, X.tx1, y, ey)) {
+                y_acc(A0, A1, A3, B0, B1, y, ey, G(F_Z, d));
+                ++nuh;
+            }
+        }
+        T.u[2 * iu] = make_float4(A0, A1, A3, B0);
+        T.u[2 * iu + 1] = make_float4(B1, __int_as_float(nuh), 0.f, 0.f);
+    }
+    for (int iv = lane; iv < nv1; iv += WAVE) {
+        const uint16_t q = lv[iv];
+        const int ui[2] = {pa(q), pb(q)};
+        float yv[2] = {0.f, 0.f}, eyv[2] = {0.f, 0.f};
+        uint32_t fl = 0;
+#pragma unroll
+        for (int s = 4; s < 6; ++s) {
+            const int idx = ui[s - 4];
+            if (idx < 0) continue;
+            const int d = det6[s];
+            const int r = E.raw(d, idx);
+            if (y_uvhit(d, E.Lp->elem[r], E.Lp->drift[r], 0, X.x01, X.tx1, yv[s - 4], eyv[s - 4])) fl |= 1u << (s - 4);
+        }
+        T.v[iv] = make_float4(yv[0], eyv[0], yv[1], eyv[1]);
+        T.vf[iv] = fl;
+    }
+}
+
+// global_yfit<false> of pair iuv from the tables
+__device__ __forceinline__ bool global_yfit_tab(const GTab& T, int nu1, int iuv, Fit& g) {
+    constexpr int det6[6] = {4, 3, 2, 1, 6, 5};
+    const int iu = iuv % nu1, iv = iuv / nu1;
+    const float4 a = gld4(T.u + 2 * iu), b = gld4(T.u + 2 * iu + 1);
+    if (__float_as_int(b.y) == 0) return false;
+    float A0 = a.x, A1 = a.y, A3 = a.z, B0 = a.w, B1 = b.x;
+    const uint32_t fl = gld(T.vf + iv);
+    if (fl == 0) return false;
+    const float4 v = gld4(T.v + iv);
+    if (fl & 1u) y_acc(A0, A1, A3, B0, B1, v.x, v.y, G(F_Z, det6[4]));
+    if (fl & 2u) y_acc(A0, A1, A3, B0, B1, v.z, v.w, G(F_Z, det6[5]));
+    g = fit_finish(A0, A1, A3, B0, B1);
+    return true;
+}
+
 constexpr int GLOBAL_LIST_CAP = 2048;   // pass-A survivors held before pass B runs
 
 // One track: pass A over its x pairs, pass B over the listed survivors, then
 // the best candidate is written.  Uses the x-part records (S.wy() + 8), the
-// survivor list (S.plist()) and the best-candidate record (S.wy()).
+// survivor list (S.plist()), the best-candidate record (S.wy()) and the
+// pass-A tables (S.ycache()).
 template <int MAXH>
 __device__ __attribute__((noinline)) void global_track(Scr S, Tk* T, const uint16_t* lx, const uint16_t* lu,
                                                        const uint16_t* lv, int nx1, int nu1, int nv1) {
@@ -2190,6 +2266,10 @@ __device__ __attribute__((noinline)) void global_track(Scr S, Tk* T, const uint1
     GBest* best = reinterpret_cast<GBest*>(S.wy());
     GX* xrec = reinterpret_cast<GX*>(S.wy() + 8);
     uint32_t* list = S.plist();
+    GTab tab;
+    tab.u = reinterpret_cast<float4*>(S.ycache());
+    tab.v = tab.u + 2 * WIN_CAP;
+    tab.vf = reinterpret_cast<uint32_t*>(tab.v + WIN_CAP);
     const int n23 = T->nx + T->nuv;
     const float x0 = T->x0, tx = T->tx, ex0 = T->ex0, etx = T->etx;
     const float einvp = (etx - etx) / PT_KICK_KMAG;
@@ -2221,16 +2301,15 @@ __device__ __attribute__((noinline)) void global_track(Scr S, Tk* T, const uint1
                 have_x = true;
                 base = 0;
                 if (lane == 0) xrec[ix] = X;
+                global_tables(E, tab, lu, lv, nu1, nv1, X, As);
+                __syncthreads();
             }
             if (np + WAVE > GLOBAL_LIST_CAP) break;
             const int iuv = base + lane;
             bool reached = false, pass = false;
             Fit g = {0.f, 0.f, 0.f, 0.f};
             if (iuv < ncomb) {
-                unsigned pres = 0;
-                int raw[6];
-                float pos[6], drift[6];
-                reached = global_yfit<false>(E, lu, lv, nu1, iuv, X, As, pres, raw, pos, drift, g);
+                reached = global_yfit_tab(tab, nu1, iuv, g);
                 pass = reached && !(fabsf(g.p0) > Y0_MAX + 2.f * g.e0 || fabsf(g.p1) > TY_MAX + 2.f * g.e1);
             }
             const unsigned long long pm = ballot(pass);
