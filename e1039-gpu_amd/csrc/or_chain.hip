@@ -2278,10 +2278,11 @@ __device__ __attribute__((noinline)) void global_track(Scr S, Tk* T, const uint1
     // y sums of the station-2/3 UV hits with their stored signs (reconstruction_kernels.cuh:2086-2102)
     float As[5] = {0.f, 0.f, 0.f, 0.f, 0.f};   // A0, A1, A3, B0, B1
     for (int m = 0; m < n23; ++m) {
-        const int d = T->det[m];
+        const int d = __builtin_amdgcn_readfirstlane(T->det[m]);
         if ((double)G(F_COSTHETA, d) > 0.99) continue;
+        const int h = __builtin_amdgcn_readfirstlane(T->hraw[m]);
         float y, ey;
-        if (y_uvhit(d, E.Lp->elem[T->hraw[m]], E.Lp->drift[T->hraw[m]], T->sign[m], x0, tx, y, ey)) {
+        if (y_uvhit(d, E.Lp->elem[h], E.Lp->drift[h], T->sign[m], x0, tx, y, ey)) {
             const float z = G(F_Z, d);
             As[3] += ey * y; As[4] += (ey * y) * z; As[0] += ey; As[1] += ey * z; As[2] += (ey * z) * z;
         }
@@ -2352,12 +2353,15 @@ __device__ __attribute__((noinline)) void global_track(Scr S, Tk* T, const uint1
                 lr_slots_new<6>(Y.x01, Y.tx1, fy0, fty, Y.x01, Y.tx1, fey0, fety, pres, dets, pos, drift, sign, 150.f);
                 if (cross) wrote = sign[1] != 2;
                 if (!(fabsf(fy0) > Y0_MAX || fabsf(fty) > TY_MAX)) {
-                    // station-2/3 residuals with (x0, tx) of the tracklet and the new y line
+                    // station-2/3 residuals with (x0, tx) of the tracklet and the new y line;
+                    // the track is the wave's own, so its hits are wave-uniform and the
+                    // geometry of their planes comes through scalar loads
                     for (int m = 0; m < n23; ++m) {
-                        const int d = T->det[m];
+                        const int d = __builtin_amdgcn_readfirstlane(T->det[m]);
+                        const int h = __builtin_amdgcn_readfirstlane(T->hraw[m]);
                         const float res = G(F_RESOLUTION, d);
-                        const float dca = dca_wire(d, E.Lp->elem[T->hraw[m]], x0, fy0, tx, fty);
-                        const float r23 = E.Lp->drift[T->hraw[m]] * (float)T->sign[m] - dca;
+                        const float dca = dca_wire(d, E.Lp->elem[h], x0, fy0, tx, fty);
+                        const float r23 = E.Lp->drift[h] * (float)T->sign[m] - dca;
                         chi23 += (r23 * r23) / (res * res);
                     }
                     cand = true;
