@@ -17,6 +17,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 
 #include "or_device.h"
@@ -104,12 +105,19 @@ using namespace ordev;
 namespace {
 
 constexpr int WAVE = 64;
-constexpr int MAXTKL = 512;   // tracklets per event (datasizes::TrackSizeMax)
-constexpr int BIN_CAP = 20;   // pairs per XZ bin (globalconsts::MaxHitsProj_X)
-constexpr int WIN_CAP = 100;  // pairs per window list (hitpairs_*[100])
+// Bounds set by index encodings, not by buffers (buffers are sized from the
+// session's or_params): in-plane hit indices are 8 bits (0xFF = none), the
+// (u, v) and (u3, v3) enumeration indices of a list pair are 16 bits, slot
+// order entries are 16 bits.
+constexpr int PAIRS_MAX = 255;        // max_pairs_per_bin, max_pairs_window
+constexpr int TKL_MAX = 16384;        // max_tracklets_per_event
 constexpr int PLANE_CAP = 400;
 constexpr int YZ_CHUNK = 16;  // tracklets whose window lists are built together
-constexpr int YZ_LIST_CAP = 4096;
+constexpr int YZ_LIST_MIN = 4096;     // survivor list entries per chunk (buffer; longer chunks stream)
+constexpr int YZ_LIST_MAX = 1 << 20;
+// a tracklet whose four window lists span more than this many (u3, v3, u2, v2)
+// candidates is an enumeration overflow (keeps chunk sums within 32 bits)
+constexpr int64_t YZ_ENUM_MAX = 1ll << 27;
 constexpr int NCLASS_MAX = 5;    // event tile classes: 512, 1024, 2048, 4096, 8192 hits
 
 // Sub-stage cycle counters of the profiling build (-DOR_STAGE_TIMING); no-ops
@@ -153,10 +161,14 @@ struct __align__(16) Tk {
     uint32_t canon;         // position in the canonical XZ list
     int8_t stid;            // 0 after XZ, 3 rejected/demoted, 5 YZ, 6 global
     uint8_t nx, nuv, n1;
-    uint8_t t, st3, slot, pos_in_slot;
+    uint8_t t, st3, slot;
     int8_t charge;
-    uint8_t pad2[3];
+    uint16_t pos_in_slot;
+    uint8_t pad2[2];
 };
+static_assert(sizeof(Tk) == 256, "tracklet record size");
+
+constexpr int CLEAN_WORDS = 64;   // lane-parallel cleaning of up to 64 x 64 live tracklets
 
 // The event tile.  Hits are staged grouped by detector; after the reducer the
 // kept ones are compacted in place, so plane d's reduced hits are
@@ -176,11 +188,43 @@ struct Lds {
     union alignas(16) {
         int nl[YZ_CHUNK][4];      // YZ / global: window list lengths (read as int4)
         uint16_t live[WAVE];      // cleaning: tracklets at stationID >= 5, slot order
+        struct {
+            unsigned long long alive[CLEAN_WORDS];   // cleaning of more than 64: alive bits by live position
+            int lstart[33];                          // first live position of each slot
+        } cl;
     };
     int uoff[YZ_CHUNK + 1];   // prefix of (u3, v3) units or of candidates per tracklet of the chunk
     unsigned long long best[YZ_CHUNK];
     int sstart[33];
     uint32_t status;
+};
+
+// Per-wave scratch in HBM: one block per resident wave, regions at offsets the
+// host computes from the session's caps (ScrLayout); each wave keeps the
+// addresses of its regions in LDS (g_cfg) so the stage functions take no
+// pointer arguments for them.
+enum ScrRegion {
+    R_TK,     // Tk[maxtkl]: the event's tracklets
+    R_CELL,   // uint32[1569]: XZ cell offsets
+    R_LIST,   // uint16[64][win_cap]: window pair lists (YZ: tracklet x view; station 1: lane)
+    R_YV,     // float4[64][win_cap]: y values of YZ list entries; station-1 records
+    R_YF,     // uint8[64][win_cap]: accepted-hit flags of YZ list entries
+    R_XP,     // uint16[3][28][bin_cap]: binned X pairs
+    R_SO,     // uint16[maxtkl]: tracklet index by slot order
+    R_PL,     // uint32[list_cap]: YZ first-box survivors of a batch; station-1 pass-A list
+    R_PB,     // uchar4[list_cap]: survivor sign state
+    R_KL,     // uint32[unit_cap]: kept YZ units of a chunk
+    R_KS,     // uint32[unit_cap]: their first candidate in the kept enumeration
+    R_YC,     // uint4[3 * list_cap]: survivor candidate cache; station-1 pass-A tables
+    R_QL,     // uint32[list_cap]: survivors passing the sign-free tests
+    R_BR,     // uint4[YZ_CHUNK][4]: best candidate of each tracklet of a chunk so far
+    R_CL,     // uint4[3 * maxtkl]: cleaning, hit per detector 13-30 of each live tracklet
+    R_COUNT
+};
+struct ScrLayout {
+    int64_t off[R_COUNT];
+    int64_t stride;
+    int32_t maxtkl, bin_cap, win_cap, list_cap, unit_cap;
 };
 
 struct Args {
@@ -206,33 +250,20 @@ struct Args {
     float* vertex;              // optional: swim global tracks to the beam line
     float* fit;                 // optional: final straight-line fit of each output track
     char* scratch;
-    int64_t scratch_stride;
-    int yz_list_cap;            // <= YZ_LIST_CAP; chunks with more candidates take the sequential path
+    ScrLayout lay;              // scratch layout of this launch's tile class
+    int yz_list_cap;            // <= lay.list_cap: survivor batch size of the YZ passes
     const PrunerDev* pruner;    // learned YZ unit pruning (NULL: every unit)
     float* unit_dump;           // optional per-unit feature rows (training aid)
     int64_t dump_cap;
     unsigned long long* dump_count;
 };
 
-// scratch layout per block
-constexpr int64_t SCR_TK = 0;
-constexpr int64_t SCR_CELL = SCR_TK + (int64_t)sizeof(Tk) * MAXTKL;                 // uint32[1569]
-constexpr int64_t SCR_LIST = SCR_CELL + ((1569 * 4 + 255) / 256) * 256;             // uint16 lists
-constexpr int64_t SCR_LIST_BYTES = (int64_t)WAVE * 4 * WIN_CAP * 2;
-constexpr int64_t SCR_YV = ((SCR_LIST + SCR_LIST_BYTES + 255) / 256) * 256;         // float4 per YZ list entry
-constexpr int64_t SCR_YV_BYTES = (int64_t)YZ_CHUNK * 4 * WIN_CAP * 16;
-constexpr int64_t SCR_YF = SCR_YV + SCR_YV_BYTES;                                   // uint8 per YZ list entry
-constexpr int64_t SCR_YF_BYTES = (int64_t)YZ_CHUNK * 4 * WIN_CAP;
-constexpr int64_t SCR_XP = ((SCR_YF + SCR_YF_BYTES + 255) / 256) * 256;           // uint16 [3][28][BIN_CAP] X pairs
-constexpr int64_t SCR_SO = SCR_XP + ((3 * 28 * BIN_CAP * 2 + 255) / 256) * 256;      // uint16 [MAXTKL] slot order
-constexpr int64_t SCR_PL = ((SCR_SO + MAXTKL * 2 + 255) / 256) * 256;               // uint32 [YZ_LIST_CAP] YZ items
-constexpr int64_t SCR_PB = SCR_PL + (int64_t)YZ_LIST_CAP * 4;                       // uint8 x4 [YZ_LIST_CAP] item state
-constexpr int UNIT_CAP = 4096;                                                       // YZ units scored per chunk
-constexpr int64_t SCR_KL = SCR_PB + (int64_t)YZ_LIST_CAP * 4;                       // uint32 [UNIT_CAP] kept units
-constexpr int64_t SCR_KS = SCR_KL + (int64_t)UNIT_CAP * 4;                          // uint32 [UNIT_CAP] their first candidate
-constexpr int64_t SCR_YC = ((SCR_KS + (int64_t)UNIT_CAP * 4 + 255) / 256) * 256;     // uint4 x3 [YZ_LIST_CAP] candidate cache
-constexpr int64_t SCR_QL = SCR_YC + (int64_t)YZ_LIST_CAP * 48;                       // uint32 [YZ_LIST_CAP] pass-3 list
-constexpr int64_t SCR_STRIDE = ((SCR_QL + (int64_t)YZ_LIST_CAP * 4 + 4095) / 4096) * 4096;
+// region addresses and caps of the wave (written at kernel start)
+struct WaveCfg {
+    char* rp[R_COUNT];
+    int32_t maxtkl, bin_cap, win_cap, list_cap, unit_cap;
+};
+__shared__ WaveCfg g_cfg;
 
 __device__ __forceinline__ unsigned long long ballot(bool p) { return __ballot(p); }
 __device__ __forceinline__ int lane_id() { return threadIdx.x; }
@@ -553,6 +584,10 @@ __device__ __forceinline__ float4 gld4(const float4* p) {
     const __attribute__((address_space(1))) float* f = (const __attribute__((address_space(1))) float*)p;
     return make_float4(f[0], f[1], f[2], f[3]);
 }
+__device__ __forceinline__ uint4 gld4(const uint4* p) {
+    const __attribute__((address_space(1))) uint32_t* f = (const __attribute__((address_space(1))) uint32_t*)p;
+    return make_uint4(f[0], f[1], f[2], f[3]);
+}
 
 struct YzLists {
     const uint16_t* l[4];   // pair entries (a | b << 8), views u2, v2, u3, v3
@@ -752,34 +787,37 @@ __device__ __attribute__((noinline)) bool yz_unit(const Ev<MAXH>& E, Tk* Tp, con
 // each stage, and stage-level branches depend only on wave-uniform values.
 #define STAGE __device__ __attribute__((noinline))
 
-// Per-wave scratch in HBM: one base pointer (the regions sit at fixed
-// offsets), so the stage functions take it in one register pair.
+// Accessors of the wave's scratch regions (addresses from g_cfg).
 struct Scr {
-    char* base;
-    __device__ Tk* tks() const { return reinterpret_cast<Tk*>(base + SCR_TK); }
-    __device__ uint32_t* celloff() const { return reinterpret_cast<uint32_t*>(base + SCR_CELL); }
-    __device__ uint16_t* wl() const { return reinterpret_cast<uint16_t*>(base + SCR_LIST); }
-    __device__ float4* wy() const { return reinterpret_cast<float4*>(base + SCR_YV); }
-    __device__ uint8_t* wf() const { return reinterpret_cast<uint8_t*>(base + SCR_YF); }
-    // [3][28][BIN_CAP] X pairs (a | b << 8), 0xFF = none
-    __device__ uint16_t* xpairs() const { return reinterpret_cast<uint16_t*>(base + SCR_XP); }
+    __device__ Tk* tks() const { return reinterpret_cast<Tk*>(g_cfg.rp[R_TK]); }
+    __device__ uint32_t* celloff() const { return reinterpret_cast<uint32_t*>(g_cfg.rp[R_CELL]); }
+    __device__ uint16_t* wl() const { return reinterpret_cast<uint16_t*>(g_cfg.rp[R_LIST]); }
+    __device__ float4* wy() const { return reinterpret_cast<float4*>(g_cfg.rp[R_YV]); }
+    __device__ uint8_t* wf() const { return reinterpret_cast<uint8_t*>(g_cfg.rp[R_YF]); }
+    // [3][28][bin_cap] X pairs (a | b << 8), 0xFF = none
+    __device__ uint16_t* xpairs() const { return reinterpret_cast<uint16_t*>(g_cfg.rp[R_XP]); }
     // tracklet index by slot order
-    __device__ uint16_t* slot_order() const { return reinterpret_cast<uint16_t*>(base + SCR_SO); }
+    __device__ uint16_t* slot_order() const { return reinterpret_cast<uint16_t*>(g_cfg.rp[R_SO]); }
     // YZ candidates passing the first box cut, in reference order
-    __device__ uint32_t* plist() const { return reinterpret_cast<uint32_t*>(base + SCR_PL); }
+    __device__ uint32_t* plist() const { return reinterpret_cast<uint32_t*>(g_cfg.rp[R_PL]); }
     // per candidate: x first-of-unit, y station-3 signs after its left/right (first
     // of unit), z sign written to the last station-3 hit (0: none), w station-3
     // signs it starts from
-    __device__ uchar4* pst() const { return reinterpret_cast<uchar4*>(base + SCR_PB); }
+    __device__ uchar4* pst() const { return reinterpret_cast<uchar4*>(g_cfg.rp[R_PB]); }
     // kept YZ units of a chunk (j << 16 | u3.v3 index), in order, and their first
     // candidate in the kept enumeration
-    __device__ uint32_t* klist() const { return reinterpret_cast<uint32_t*>(base + SCR_KL); }
-    __device__ uint32_t* kstart() const { return reinterpret_cast<uint32_t*>(base + SCR_KS); }
-    // per box-cut survivor of a chunk: its hits (raw LDS indices), which slots
+    __device__ uint32_t* klist() const { return reinterpret_cast<uint32_t*>(g_cfg.rp[R_KL]); }
+    __device__ uint32_t* kstart() const { return reinterpret_cast<uint32_t*>(g_cfg.rp[R_KS]); }
+    // per box-cut survivor of a batch: its hits (raw LDS indices), which slots
     // hold one, and its first fit, as pass 2 built them (3 x 16 bytes)
-    __device__ uint4* ycache() const { return reinterpret_cast<uint4*>(base + SCR_YC); }
-    // survivors of a chunk that pass the sign-free tests, in order
-    __device__ uint32_t* qlist() const { return reinterpret_cast<uint32_t*>(base + SCR_QL); }
+    __device__ uint4* ycache() const { return reinterpret_cast<uint4*>(g_cfg.rp[R_YC]); }
+    // survivors of a batch that pass the sign-free tests, in order
+    __device__ uint32_t* qlist() const { return reinterpret_cast<uint32_t*>(g_cfg.rp[R_QL]); }
+    // per tracklet of the chunk: the cache entry and sign state of its best candidate so far
+    __device__ uint4* brec() const { return reinterpret_cast<uint4*>(g_cfg.rp[R_BR]); }
+    // cleaning: per live tracklet, the raw hit of each of detectors 13-30 (uint16, 0xFFFF none)
+    __device__ uint4* clean_hits() const { return reinterpret_cast<uint4*>(g_cfg.rp[R_CL]); }
+    __device__ int win() const { return g_cfg.win_cap; }
 };
 
 // Learned unit pruning and unit rows for the YZ stage, by value from the
@@ -1022,12 +1060,12 @@ STAGE void stage_xpairs(or_params P, uint16_t* xpairs) {
             for (int b = 0; b < 28; ++b) L.xcnt[s][b] = 0;
             {
                 const float sp = spacing_plane(super_id(2 + s, 0));
-                const int cap = P.max_pairs_per_bin < BIN_CAP ? P.max_pairs_per_bin : BIN_CAP;
+                const int cap = g_cfg.bin_cap;
                 auto put = [&](int c, int ia, int ib) {
                     const int b = xbin(s, c);
                     if (b < 0) return;
                     const int k = L.xcnt[s][b];
-                    if (k < cap) xpairs[(s * 28 + b) * BIN_CAP + k] = (uint16_t)((ia & 0xFF) | ((ib & 0xFF) << 8));
+                    if (k < cap) xpairs[(s * 28 + b) * cap + k] = (uint16_t)((ia & 0xFF) | ((ib & 0xFF) << 8));
                     L.xcnt[s][b] = k + 1;
                 };
                 auto close = [&](int i, int j) { return !(fabsf(L.pos[E.raw(d1, i)] - L.pos[E.raw(d2, j)]) > sp); };
@@ -1109,8 +1147,8 @@ STAGE int stage_xz(or_params P, Scr S) {
                 const int ix = (int)(g - celloff[c]);
                 const int ix2 = ix % nx2, ix3 = ix / nx2;
                 par = t & 1;
-                q2 = S.xpairs()[b2 * BIN_CAP + ix2];
-                q3 = S.xpairs()[(s3 * 28 + b3) * BIN_CAP + ix3];
+                q2 = S.xpairs()[b2 * g_cfg.bin_cap + ix2];
+                q3 = S.xpairs()[(s3 * 28 + b3) * g_cfg.bin_cap + ix3];
                 acc = xz_candidate(E, s3, q2, q3, nullptr);
             }
             const unsigned long long m = ballot(acc);
@@ -1120,14 +1158,14 @@ STAGE int stage_xz(or_params P, Scr S) {
                 // accepted: evaluate again writing straight into the tracklet slot
                 const int k = ntkl + rank_in(m);
                 const int kp = par == 0 ? np + rank_in(mp) : nm + rank_in(mm);
-                if (k < MAXTKL) {
+                if (k < g_cfg.maxtkl) {
                     Tk& T = tks[k];
                     xz_candidate(E, s3, q2, q3, &T);
                     T.t = (uint8_t)t;
                     T.st3 = (uint8_t)(3 + par);
                     T.canon = (uint32_t)k;
                     T.slot = (uint8_t)(2 * (kp % 16) + par);
-                    T.pos_in_slot = (uint8_t)(kp / 16 < 255 ? kp / 16 : 255);
+                    T.pos_in_slot = (uint16_t)(kp / 16);
                 }
             }
             ntkl += __popcll(m);
@@ -1136,7 +1174,7 @@ STAGE int stage_xz(or_params P, Scr S) {
         }
         if (lane == 0) {
             if (np > 16 * P.max_tracklets_per_slot || nm > 16 * P.max_tracklets_per_slot ||
-                ntkl >= P.max_tracklets_per_event || ntkl > MAXTKL)
+                ntkl >= P.max_tracklets_per_event || ntkl > g_cfg.maxtkl)
                 atomicOr(&L.status, (uint32_t)OR_EV_TRACKLET_OVERFLOW);
         }
         // slot order: slot s = 2q + p holds items kp = q, q+16, ...
@@ -1146,7 +1184,7 @@ STAGE int stage_xz(or_params P, Scr S) {
             const int cs = (lane < 32 && npar > q) ? (npar - q + 15) / 16 : 0;
             const int inc = wave_scan(cs);
             if (lane < 32) L.sstart[lane] = inc - cs;
-            if (lane == 32) L.sstart[32] = ntkl < MAXTKL ? ntkl : MAXTKL;
+            if (lane == 32) L.sstart[32] = ntkl < g_cfg.maxtkl ? ntkl : g_cfg.maxtkl;
         }
         __syncthreads();
         if (L.status == 0)
@@ -1174,9 +1212,9 @@ STAGE int stage_xz(or_params P, Scr S) {
 template <int MAXH>
 __device__ __forceinline__ void yz_lists_of(const Scr& S, int j, YzLists& W) {
     for (int v = 0; v < 4; ++v) {
-        W.l[v] = S.wl() + (j * 4 + v) * WIN_CAP;
-        W.y[v] = S.wy() + (j * 4 + v) * WIN_CAP;
-        W.f[v] = S.wf() + (j * 4 + v) * WIN_CAP;
+        W.l[v] = S.wl() + (j * 4 + v) * g_cfg.win_cap;
+        W.y[v] = S.wy() + (j * 4 + v) * g_cfg.win_cap;
+        W.f[v] = S.wf() + (j * 4 + v) * g_cfg.win_cap;
         W.n[v] = g_lds<MAXH>.nl[j][v];
     }
 }
@@ -1262,7 +1300,7 @@ __device__ __forceinline__ void yz_candidate(const Ev<MAXH>& E, const Scr& S, co
 // v3 entries (window lists hold accepted entries only).
 __device__ __forceinline__ UnitFeatures unit_features(const Scr& S, const Tk& T, int j, int iuv3, int nu3) {
     const int iu3 = iuv3 % nu3, iv3 = iuv3 / nu3;
-    const int qu = (j * 4 + 2) * WIN_CAP + iu3, qv = (j * 4 + 3) * WIN_CAP + iv3;
+    const int qu = (j * 4 + 2) * g_cfg.win_cap + iu3, qv = (j * 4 + 3) * g_cfg.win_cap + iv3;
     const float4 yu = gld4(S.wy() + qu), yw = gld4(S.wy() + qv);
     const unsigned fu = gld(S.wf() + qu), fv = gld(S.wf() + qv);
     const float ya = (fu & 1u) ? yu.x : yu.z, eya = (fu & 1u) ? yu.y : yu.w;
@@ -1300,7 +1338,7 @@ __device__ __forceinline__ bool yz_first_cut(const Scr& S, int j, const int (&e)
     float4 yv[4];
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-        const int q = (j * 4 + v) * WIN_CAP + e[v];
+        const int q = (j * 4 + v) * g_cfg.win_cap + e[v];
         fl[v] = gld(S.wf() + q);
         yv[v] = gld4(S.wy() + q);
     }
@@ -1323,24 +1361,32 @@ __device__ __forceinline__ bool yz_first_cut(const Scr& S, int j, const int (&e)
     return ok && yz_box1(fit_finish(A0, A1, A3, B0, B1));
 }
 
-// Pass 1 by units, for learned pruning and unit rows: every (u3, v3) unit of
-// the chunk is scored (and its row written), the kept units are listed in
-// order with the start of their candidates, and the candidates of the kept
-// units go through the first fit.  Survivors keep their index in the full
+// Pass 1 streams in batches.  The survivors of the first box cut go into a
+// list of cap entries; when the list fills, the batch ends before the first
+// (u3, v3) unit that does not fit whole, passes 2 and 3 run on it, and the next
+// batch starts at that unit.  The sign state of the YZ passes never crosses a
+// unit, so batches cut at unit boundaries give every candidate the state it
+// has in one pass over all of them.  A unit with more survivors than the list
+// holds sends the chunk to the unit-by-unit path.
+//
+// Learned pruning and unit rows: every (u3, v3) unit of the chunk is scored
+// (and its row written) first, the kept units are listed in order with the
+// start of their candidates in the kept enumeration, and pass 1 walks the
+// candidates of the kept units only.  Survivors keep their index in the full
 // enumeration, so the later passes see the plain chain's candidates with the
-// pruned units taken out.  np: survivors, -1 when they overflow the list, -2
-// when the chunk has more than UNIT_CAP units (then plain pass 1 runs).
-struct UnitPass {
-    int np;
+// pruned units taken out.
+struct UnitScore {
+    int nk;          // kept units; -1: more units than the list holds (plain pass 1 runs)
+    uint32_t run;    // candidates of the kept units
     int64_t dbase;   // first unit row of the chunk, -1: none written
 };
 
 template <int MAXH>
-__device__ __attribute__((noinline)) UnitPass yz_pass1_units(Scr S, Prn R, int kb, int ntkl, int cap) {
+__device__ __attribute__((noinline)) UnitScore yz_score_units(Scr S, Prn R, int kb, int ntkl) {
     Lds<MAXH>& L = g_lds<MAXH>;
     const int lane = lane_id();
     Tk* tks = S.tks();
-    UnitPass res = {-2, -1};
+    UnitScore res = {-1, 0u, -1};
     OR_SUB_START
     int uo[YZ_CHUNK + 1];
     {
@@ -1355,7 +1401,7 @@ __device__ __attribute__((noinline)) UnitPass yz_pass1_units(Scr S, Prn R, int k
         for (int t = 1; t <= YZ_CHUNK; ++t) uo[t] = __builtin_amdgcn_readfirstlane(__shfl(inc, t - 1, WAVE));
     }
     const int U = uo[YZ_CHUNK];
-    if (U > UNIT_CAP) return res;
+    if (U > g_cfg.unit_cap) return res;
     if (R.dump) {
         unsigned long long b = 0;
         if (lane == 0) b = atomicAdd(R.dump_count, (unsigned long long)U);
@@ -1417,15 +1463,42 @@ __device__ __attribute__((noinline)) UnitPass yz_pass1_units(Scr S, Prn R, int k
     OR_COUNT(11, U);
     OR_COUNT(12, nk);
     OR_COUNT(13, run);
+    res.nk = nk;
+    res.run = run;
+    return res;
+}
+
+// number of entries of the sorted list p[0, n) below v (every lane alike)
+__device__ __forceinline__ int lower_bound_u32(const uint32_t* p, int n, uint32_t v) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (gld(p + mid) < v) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// One batch of pass 1 over the kept units' candidates from kept-enumeration
+// index cursor (the start of kept unit lo0).  Returns the survivors listed; on
+// return cursor and lo0 name the next batch's first unit (cursor == run: done).
+// -1: the unit at cursor alone has more survivors than the list holds.
+template <int MAXH>
+__device__ __attribute__((noinline)) int yz_pass1_kept(Scr S, int kb, int ntkl, int cap, int nk, uint32_t run,
+                                                       uint32_t& cursor, int& lo_unit) {
+    Lds<MAXH>& L = g_lds<MAXH>;
+    const int lane = lane_id();
+    Tk* tks = S.tks();
     int off[YZ_CHUNK + 1];
 #pragma unroll
     for (int t = 0; t <= YZ_CHUNK; ++t) off[t] = __builtin_amdgcn_readfirstlane(L.uoff[t]);
     const unsigned d3m = (unsigned)ballot(lane < YZ_CHUNK && kb + lane < ntkl && tks[kb + lane].st3 == 4);
     float zv[4][2][2];
     yz_plane_z(zv);
+    const uint32_t start = cursor;
     int np = 0;
-    int lo0 = 0;   // last kept unit starting at or before i0 (the same for every lane)
-    for (uint32_t i0 = 0; i0 < run; i0 += WAVE) {
+    int lo0 = lo_unit;   // last kept unit starting at or before i0 (the same for every lane)
+    for (uint32_t i0 = start; i0 < run; i0 += WAVE) {
         const uint32_t i = i0 + (uint32_t)lane;
         // every kept unit has at least one candidate, so the wave's 64 candidates
         // lie in units lo0 .. lo0 + 63: lane t holds the start and code of unit
@@ -1444,6 +1517,7 @@ __device__ __attribute__((noinline)) UnitPass yz_pass1_units(Scr S, Prn R, int k
         }
         const uint32_t st_t = __shfl(st, tu, WAVE);
         const uint32_t code = __shfl(kl, tu, WAVE);
+        const int ku = lo0 + tu;   // this lane's kept unit
         // the next window starts at the last unit starting at or before i0 + 64;
         // a unit past this window can start there only when the window's last
         // unit does (then unit lo0 started at i0)
@@ -1451,7 +1525,7 @@ __device__ __attribute__((noinline)) UnitPass yz_pass1_units(Scr S, Prn R, int k
         if (tn == WAVE - 1)
             while (lo0 + 1 < nk && S.kstart()[lo0 + 1] <= i0 + WAVE) ++lo0;
         bool pass = false;
-        int iorig = 0;
+        int iorig = 0, ustart = 0;
         if (i < run) {
             const int j = (int)(code >> 16), iuv3 = (int)(code & 0xFFFFu);
             const int inner = (int)(i - st_t);
@@ -1460,23 +1534,90 @@ __device__ __attribute__((noinline)) UnitPass yz_pass1_units(Scr S, Prn R, int k
 #pragma unroll
             for (int t = 1; t < YZ_CHUNK; ++t) ojs = j == t ? off[t] : ojs;
             const int e[4] = {inner % nl.x, inner / nl.x, iuv3 % nl.z, iuv3 / nl.z};
-            iorig = ojs + iuv3 * (nl.x * nl.y) + inner;
+            ustart = ojs + iuv3 * (nl.x * nl.y);
+            iorig = ustart + inner;
             pass = yz_first_cut(S, j, e, (d3m >> j) & 1, zv);
         }
         const unsigned long long m = ballot(pass);
         const int c = __popcll(m);
         if (np + c > cap) {
-            // the chunk goes the sequential way: its unit rows carry no winner flags
-            if (res.dbase >= 0)
-                for (int u = lane; u < U; u += WAVE) R.dump[(res.dbase + u) * OR_UNIT_ROW + 3] = -1.f;
-            res.np = -1;
-            return res;
+            // cut before the unit of the first survivor that does not fit
+            const unsigned long long over = ballot(pass && rank_in(m) == cap - np);
+            const int lo_l = __ffsll((long long)over) - 1;
+            const uint32_t ucut_k = __shfl(st_t, lo_l, WAVE);
+            const int ucut_o = __shfl(ustart, lo_l, WAVE), ku_cut = __shfl(ku, lo_l, WAVE);
+            if (ucut_k == start) return -1;
+            if (pass && iorig < ucut_o) S.plist()[np + rank_in(m)] = (uint32_t)iorig;
+            const int nb = __popcll(ballot(pass && iorig < ucut_o));
+            __syncthreads();   // the list writes above are read by other lanes below
+            // survivors listed before this iteration that belong to the cut unit go too
+            const int keep = lower_bound_u32(S.plist(), np + nb, (uint32_t)ucut_o);
+            cursor = ucut_k;
+            lo_unit = ku_cut;
+            return keep;
         }
         if (pass) S.plist()[np + rank_in(m)] = (uint32_t)iorig;
         np += c;
     }
-    res.np = np;
-    return res;
+    cursor = run;
+    lo_unit = nk;
+    return np;
+}
+
+// One batch of plain pass 1 from candidate cursor (a unit start): first fit and
+// box cut of every candidate, survivors in order.  Chunk constants go to scalar
+// registers (candidate offsets, which tracklets are D3m, the plane z values of
+// both station-3 choices) and a candidate's four list entries are loaded
+// together.  Returns as yz_pass1_kept.
+template <int MAXH>
+__device__ __attribute__((noinline)) int yz_pass1_plain(Scr S, int kb, int ntkl, int cap, int I, int& cursor) {
+    Lds<MAXH>& L = g_lds<MAXH>;
+    const int lane = lane_id();
+    Tk* tks = S.tks();
+    int off[YZ_CHUNK + 1];
+#pragma unroll
+    for (int t = 0; t <= YZ_CHUNK; ++t) off[t] = __builtin_amdgcn_readfirstlane(L.uoff[t]);
+    const unsigned d3m = (unsigned)ballot(lane < YZ_CHUNK && kb + lane < ntkl && tks[kb + lane].st3 == 4);
+    float zv[4][2][2];
+    yz_plane_z(zv);
+    const int start = cursor;
+    int np = 0;
+    for (int i0 = start; i0 < I; i0 += WAVE) {
+        const int i = i0 + lane;
+        bool pass = false;
+        int ustart = 0;
+        if (i < I) {
+            int j = 0;
+#pragma unroll
+            for (int t = 1; t < YZ_CHUNK; ++t) j += i >= off[t] ? 1 : 0;
+            const int4 nl = *reinterpret_cast<const int4*>(L.nl[j]);
+            int ojs = off[0];
+#pragma unroll
+            for (int t = 1; t < YZ_CHUNK; ++t) ojs = j == t ? off[t] : ojs;
+            const int local = i - ojs, nin = nl.x * nl.y;
+            const int iuv3 = local / nin, iuv2 = local - iuv3 * nin;
+            ustart = ojs + iuv3 * nin;
+            const int e[4] = {iuv2 % nl.x, iuv2 / nl.x, iuv3 % nl.z, iuv3 / nl.z};  // u2, v2, u3, v3
+            pass = yz_first_cut(S, j, e, (d3m >> j) & 1, zv);
+        }
+        const unsigned long long m = ballot(pass);
+        const int c = __popcll(m);
+        if (np + c > cap) {
+            const unsigned long long over = ballot(pass && rank_in(m) == cap - np);
+            const int ucut = __shfl(ustart, __ffsll((long long)over) - 1, WAVE);
+            if (ucut == start) return -1;
+            if (pass && i < ucut) S.plist()[np + rank_in(m)] = (uint32_t)i;
+            const int nb = __popcll(ballot(pass && i < ucut));
+            __syncthreads();   // the list writes above are read by other lanes below
+            const int keep = lower_bound_u32(S.plist(), np + nb, (uint32_t)ucut);
+            cursor = ucut;
+            return keep;
+        }
+        if (pass) S.plist()[np + rank_in(m)] = (uint32_t)i;
+        np += c;
+    }
+    cursor = I;
+    return np;
 }
 
 // Candidate cache of pass 2 -> pass 3: raw LDS index of each slot's hit
@@ -1518,8 +1659,9 @@ __device__ __forceinline__ void yz_cache_get(const Ev<MAXH>& E, const uint4* c, 
     f.e1 = __uint_as_float(d.x);
 }
 
-// Two-pass form.  Returns false (nothing written) when the survivors of the
-// first box cut do not fit the list.
+// Parallel form over batches of first-box survivors (see yz_pass1_kept).
+// Returns false (nothing written) when one unit alone overflows the list; the
+// chunk then takes the unit-by-unit path.
 template <int MAXH>
 __device__ __attribute__((noinline)) bool yz_chunk_par(Scr S, Prn R, int kb, int ntkl, int cap) {
     Lds<MAXH>& L = g_lds<MAXH>;
@@ -1531,9 +1673,10 @@ __device__ __attribute__((noinline)) bool yz_chunk_par(Scr S, Prn R, int kb, int
         int items = 0;
         if (j < YZ_CHUNK && k < ntkl) {
             const int* nl = L.nl[j];
+            // at most YZ_ENUM_MAX each (stage_yz), so the sum stays below 2^31
             if (nl[0] > 0 && nl[1] > 0 && nl[2] > 0 && nl[3] > 0) items = nl[0] * nl[1] * nl[2] * nl[3];
         }
-        const int inc = wave_scan(items);   // at most 16 * 100^4 < 2^31
+        const int inc = wave_scan(items);
         if (j < YZ_CHUNK) {
             L.best[j] = ~0ull;
             L.uoff[j + 1] = inc;
@@ -1543,154 +1686,154 @@ __device__ __attribute__((noinline)) bool yz_chunk_par(Scr S, Prn R, int kb, int
     __syncthreads();
     const int I = L.uoff[YZ_CHUNK];
     OR_COUNT(10, I);
-    OR_SUB_START
-    int np = 0;
-    int64_t dbase = -1;   // unit rows of this chunk (pruning path with the dump on)
-    if (R.prune || R.dump) {
-        const UnitPass up = yz_pass1_units<MAXH>(S, R, kb, ntkl, cap);
-        if (up.np == -1) return false;
-        np = up.np;
-        dbase = up.dbase;
-    }
-    if (!(R.prune || R.dump) || np == -2) {
-        np = 0;
-        // pass 1: first fit and box cut of every candidate; survivors in order.
-        // Chunk constants go to scalar registers (candidate offsets, which
-        // tracklets are D3m, the plane z values of both station-3 choices) and the
-        // candidate's four list entries are loaded together.
-        int off[YZ_CHUNK + 1];
-#pragma unroll
-        for (int t = 0; t <= YZ_CHUNK; ++t) off[t] = __builtin_amdgcn_readfirstlane(L.uoff[t]);
-        const unsigned d3m = (unsigned)ballot(lane < YZ_CHUNK && kb + lane < ntkl && tks[kb + lane].st3 == 4);
-        float zv[4][2][2];
-        yz_plane_z(zv);
-        for (int i0 = 0; i0 < I; i0 += WAVE) {
-            const int i = i0 + lane;
-            bool pass = false;
-            if (i < I) {
-                int j = 0;
-#pragma unroll
-                for (int t = 1; t < YZ_CHUNK; ++t) j += i >= off[t] ? 1 : 0;
-                const int4 nl = *reinterpret_cast<const int4*>(L.nl[j]);
-                int ojs = off[0];
-#pragma unroll
-                for (int t = 1; t < YZ_CHUNK; ++t) ojs = j == t ? off[t] : ojs;
-                const int local = i - ojs, nin = nl.x * nl.y;
-                const int iuv3 = local / nin, iuv2 = local - iuv3 * nin;
-                const int e[4] = {iuv2 % nl.x, iuv2 / nl.x, iuv3 % nl.z, iuv3 / nl.z};  // u2, v2, u3, v3
-                pass = yz_first_cut(S, j, e, (d3m >> j) & 1, zv);
+    UnitScore us = {-1, 0u, -1};
+    if (R.prune || R.dump) us = yz_score_units<MAXH>(S, R, kb, ntkl);
+    const bool kept = us.nk >= 0;
+    const int64_t dbase = us.dbase;
+    uint32_t cursor = 0;
+    int icursor = 0, lo_unit = 0;
+    unsigned long long saved = ~0ull;   // lane j < 16: best key of tracklet j already in its record
+    while (kept ? cursor < us.run : icursor < I) {
+        OR_SUB_START
+        const int np = kept ? yz_pass1_kept<MAXH>(S, kb, ntkl, cap, us.nk, us.run, cursor, lo_unit)
+                            : yz_pass1_plain<MAXH>(S, kb, ntkl, cap, I, icursor);
+        if (np < 0) {
+            // the chunk goes the unit-by-unit way: its unit rows carry no winner flags
+            if (dbase >= 0) {
+                int units = 0;
+                if (lane < YZ_CHUNK && kb + lane < ntkl) {
+                    const int* nl = L.nl[lane];
+                    if (nl[0] > 0 && nl[1] > 0 && nl[2] > 0 && nl[3] > 0) units = nl[2] * nl[3];
+                }
+                const int U = wave_sum(units);
+                for (int u = lane; u < U; u += WAVE) R.dump[(dbase + u) * OR_UNIT_ROW + 3] = -1.f;
             }
-            const unsigned long long m = ballot(pass);
-            const int c = __popcll(m);
-            if (np + c > cap) return false;
-            if (pass) S.plist()[np + rank_in(m)] = (uint32_t)i;
-            np += c;
+            return false;
         }
-    }
-    __syncthreads();
-    OR_SUB(1);
-    OR_COUNT(14, np);
-    // pass 2: per survivor, first-of-unit flag; the first of a unit runs the
-    // full left/right from zero signs (its station-3 result seeds the unit),
-    // later ones only the decision of a v3' / u2 pair (the one pair that can
-    // change a station-3 sign once all are set)
-    for (int k0 = 0; k0 < np; k0 += WAVE) {
-        const int k = k0 + lane;
-        if (k < np) {
-            const YzItem it = yz_decode<MAXH>((int)S.plist()[k]);
-            const int nin = L.nl[it.j][0] * L.nl[it.j][1];
-            bool first = k == 0;
-            if (!first) {
-                const YzItem pt = yz_decode<MAXH>((int)S.plist()[k - 1]);
-                first = pt.j != it.j || (pt.local / nin) != (it.local / nin);
+        __syncthreads();
+        OR_SUB(1);
+        OR_COUNT(14, np);
+        // pass 2: per survivor, first-of-unit flag; the first of a unit runs the
+        // full left/right from zero signs (its station-3 result seeds the unit),
+        // later ones only the decision of a v3' / u2 pair (the one pair that can
+        // change a station-3 sign once all are set).  A batch starts at a unit
+        // boundary, so its first survivor is the first of its unit.
+        for (int k0 = 0; k0 < np; k0 += WAVE) {
+            const int k = k0 + lane;
+            if (k < np) {
+                const YzItem it = yz_decode<MAXH>((int)S.plist()[k]);
+                const int nin = L.nl[it.j][0] * L.nl[it.j][1];
+                bool first = k == 0;
+                if (!first) {
+                    const YzItem pt = yz_decode<MAXH>((int)S.plist()[k - 1]);
+                    first = pt.j != it.j || (pt.local / nin) != (it.local / nin);
+                }
+                const Tk* Tp = &tks[kb + it.j];
+                YzSlots H;
+                Fit f;
+                yz_candidate(E, S, Tp, it, H, f);
+                yz_cache_put(S.ycache() + 3 * k, H, f);
+                uchar4 ps = make_uchar4(first ? 1 : 0, 0, 0, 0);
+                if (first) {
+                    lr_slots_new<8>(Tp->x0, Tp->tx, f.p0, f.p1, Tp->ex0, Tp->etx, f.e0, f.e1, H.pres, H.det, H.pos,
+                                    H.drift, H.sign, 150.f);
+                    ps.y = yz_pack3(H);
+                    ps.z = (uint8_t)(H.sign[3] & 3);
+                } else if ((lr_slot_starts<8>(H.pres, H.det) >> 3) & 1u) {
+                    const int h = lr_pair_decide(Tp->x0, Tp->tx, f.p0, f.p1, Tp->ex0, Tp->etx, f.e0, f.e1, H.det[3],
+                                                 H.det[4], H.pos[3], H.pos[4], H.drift[3], H.drift[4], 150.f);
+                    if (h >= 0) ps.z = (uint8_t)(lr_si(h) & 3);
+                }
+                S.pst()[k] = ps;
             }
-            const Tk* Tp = &tks[kb + it.j];
-            YzSlots H;
-            Fit f;
-            yz_candidate(E, S, Tp, it, H, f);
-            yz_cache_put(S.ycache() + 3 * k, H, f);
-            uchar4 ps = make_uchar4(first ? 1 : 0, 0, 0, 0);
-            if (first) {
+        }
+        __syncthreads();
+        OR_SUB(2);
+        // pass 3: starting signs by scan (the unit's seed, and the last writer of
+        // the v3' sign before this candidate) and the tests that need no sign; the
+        // candidates that pass them are listed and run the rest (left/right, chi2)
+        int carry_f = -1, carry_w = -1, nq = 0;
+        for (int k0 = 0; k0 < np; k0 += WAVE) {
+            const int k = k0 + lane;
+            const bool valid = k < np;
+            uchar4 ps = make_uchar4(0, 0, 0, 0);
+            if (valid) ps = S.pst()[k];
+            int fi = wave_max_scan(valid && ps.x ? k : -1);
+            fi = max(fi, carry_f);
+            const int wincl = wave_max_scan(valid && (ps.x || ps.z) ? k : -1);
+            int wexcl = __shfl_up(wincl, 1, WAVE);
+            if (lane == 0) wexcl = -1;
+            wexcl = max(wexcl, carry_w);
+            carry_f = __shfl(fi, WAVE - 1, WAVE);
+            carry_w = max(carry_w, __shfl(wincl, WAVE - 1, WAVE));
+            bool go = false;
+            if (valid) {
+                const YzItem it = yz_decode<MAXH>((int)S.plist()[k]);
+                const uint4 b = S.ycache()[3 * k + 1];
+                Fit f;
+                f.p0 = __uint_as_float(b.y);
+                f.p1 = __uint_as_float(b.z);
+                f.e0 = __uint_as_float(b.w);
+                f.e1 = __uint_as_float(S.ycache()[3 * k + 2].x);
+                uint8_t st = 0;
+                if (!ps.x) {
+                    // seed of the unit with the v3' sign (slot 3) of the last writer; the
+                    // first survivor of the unit is a writer, so wexcl >= fi
+                    st = (uint8_t)((S.pst()[fi].y & 0x3Fu) | ((unsigned)S.pst()[wexcl].z << 6));
+                }
+                ps.w = st;
+                S.pst()[k] = ps;
+                go = yz_cuts(E, &tks[kb + it.j], f);
+            }
+            const unsigned long long m = ballot(go);
+            if (go) S.qlist()[nq + rank_in(m)] = (uint32_t)k;
+            nq += __popcll(m);
+        }
+        __syncthreads();
+        for (int q0 = 0; q0 < nq; q0 += WAVE) {
+            const int q = q0 + lane;
+            if (q < nq) {
+                const int k = (int)S.qlist()[q];
+                const YzItem it = yz_decode<MAXH>((int)S.plist()[k]);
+                const uchar4 ps = S.pst()[k];
+                const Tk* Tp = &tks[kb + it.j];
+                YzSlots H;
+                Fit f;
+                yz_cache_get(E, S.ycache() + 3 * k, Tp->st3, H, f);
+                if (!ps.x) yz_unpack3(H, ps.w);
                 lr_slots_new<8>(Tp->x0, Tp->tx, f.p0, f.p1, Tp->ex0, Tp->etx, f.e0, f.e1, H.pres, H.det, H.pos,
                                 H.drift, H.sign, 150.f);
-                ps.y = yz_pack3(H);
-                ps.z = (uint8_t)(H.sign[3] & 3);
-            } else if ((lr_slot_starts<8>(H.pres, H.det) >> 3) & 1u) {
-                const int h = lr_pair_decide(Tp->x0, Tp->tx, f.p0, f.p1, Tp->ex0, Tp->etx, f.e0, f.e1, H.det[3],
-                                             H.det[4], H.pos[3], H.pos[4], H.drift[3], H.drift[4], 150.f);
-                if (h >= 0) ps.z = (uint8_t)(lr_si(h) & 3);
-            }
-            S.pst()[k] = ps;
-        }
-    }
-    __syncthreads();
-    OR_SUB(2);
-    // pass 3: starting signs by scan (the unit's seed, and the last writer of
-    // the v3' sign before this candidate) and the tests that need no sign; the
-    // candidates that pass them are listed and run the rest (left/right, chi2)
-    int carry_f = -1, carry_w = -1, nq = 0;
-    for (int k0 = 0; k0 < np; k0 += WAVE) {
-        const int k = k0 + lane;
-        const bool valid = k < np;
-        uchar4 ps = make_uchar4(0, 0, 0, 0);
-        if (valid) ps = S.pst()[k];
-        int fi = wave_max_scan(valid && ps.x ? k : -1);
-        fi = max(fi, carry_f);
-        const int wincl = wave_max_scan(valid && (ps.x || ps.z) ? k : -1);
-        int wexcl = __shfl_up(wincl, 1, WAVE);
-        if (lane == 0) wexcl = -1;
-        wexcl = max(wexcl, carry_w);
-        carry_f = __shfl(fi, WAVE - 1, WAVE);
-        carry_w = max(carry_w, __shfl(wincl, WAVE - 1, WAVE));
-        bool go = false;
-        if (valid) {
-            const YzItem it = yz_decode<MAXH>((int)S.plist()[k]);
-            const uint4 b = S.ycache()[3 * k + 1];
-            Fit f;
-            f.p0 = __uint_as_float(b.y);
-            f.p1 = __uint_as_float(b.z);
-            f.e0 = __uint_as_float(b.w);
-            f.e1 = __uint_as_float(S.ycache()[3 * k + 2].x);
-            uint8_t st = 0;
-            if (!ps.x) {
-                // seed of the unit with the v3' sign (slot 3) of the last writer; the
-                // first survivor of the unit is a writer, so wexcl >= fi
-                st = (uint8_t)((S.pst()[fi].y & 0x3Fu) | ((unsigned)S.pst()[wexcl].z << 6));
-            }
-            ps.w = st;
-            S.pst()[k] = ps;
-            go = yz_cuts(E, &tks[kb + it.j], f);
-        }
-        const unsigned long long m = ballot(go);
-        if (go) S.qlist()[nq + rank_in(m)] = (uint32_t)k;
-        nq += __popcll(m);
-    }
-    __syncthreads();
-    for (int q0 = 0; q0 < nq; q0 += WAVE) {
-        const int q = q0 + lane;
-        if (q < nq) {
-            const int k = (int)S.qlist()[q];
-            const YzItem it = yz_decode<MAXH>((int)S.plist()[k]);
-            const uchar4 ps = S.pst()[k];
-            const Tk* Tp = &tks[kb + it.j];
-            YzSlots H;
-            Fit f;
-            yz_cache_get(E, S.ycache() + 3 * k, Tp->st3, H, f);
-            if (!ps.x) yz_unpack3(H, ps.w);
-            lr_slots_new<8>(Tp->x0, Tp->tx, f.p0, f.p1, Tp->ex0, Tp->etx, f.e0, f.e1, H.pres, H.det, H.pos, H.drift,
-                            H.sign, 150.f);
-            float chi2;
-            yz_chi2(E, Tp, H, f, chi2, nullptr);
-            // the reference starts its minimum at 10000.1 (reconstruction_kernels.cuh:1240)
-            if (chi2 < 10000.1f) {
-                const unsigned long long key =
-                    ((unsigned long long)__float_as_uint(chi2) << 32) | (unsigned long long)(uint32_t)it.local;
-                atomicMin(&L.best[it.j], key);
+                float chi2;
+                yz_chi2(E, Tp, H, f, chi2, nullptr);
+                // the reference starts its minimum at 10000.1 (reconstruction_kernels.cuh:1240)
+                if (chi2 < 10000.1f) {
+                    const unsigned long long key =
+                        ((unsigned long long)__float_as_uint(chi2) << 32) | (unsigned long long)(uint32_t)it.local;
+                    atomicMin(&L.best[it.j], key);
+                }
             }
         }
+        __syncthreads();
+        // a tracklet whose best improved in this batch keeps that candidate's
+        // cache entry and sign state, since the next batch reuses the lists
+        if (lane < YZ_CHUNK && kb + lane < ntkl) {
+            const unsigned long long key = L.best[lane];
+            if (key != saved) {
+                const uint32_t iw = (uint32_t)L.uoff[lane] + (uint32_t)key;
+                const int k = min(lower_bound_u32(S.plist(), np, iw), np - 1);   // present: set in this batch
+                uint4* br = S.brec() + 4 * lane;
+                const uint4* c = S.ycache() + 3 * k;
+                br[0] = c[0]; br[1] = c[1]; br[2] = c[2];
+                const uchar4 ps = S.pst()[k];
+                br[3] = make_uint4((uint32_t)ps.x | ((uint32_t)ps.y << 8) | ((uint32_t)ps.z << 16) | ((uint32_t)ps.w << 24),
+                                   0u, 0u, 0u);
+                saved = key;
+            }
+        }
+        __syncthreads();
+        OR_SUB(3);
     }
-    __syncthreads();
-    OR_SUB(3);
+    OR_SUB_START
     // winners: replay the candidate with its starting signs and write it
     int uo_lane = 0;   // first unit row of tracklet j = lane
     if (dbase >= 0) {
@@ -1706,21 +1849,16 @@ __device__ __attribute__((noinline)) bool yz_chunk_par(Scr S, Prn R, int kb, int
         Tk* Tp = &tks[kb + j];
         const unsigned long long key = L.best[j];
         if (key != ~0ull) {
-            const int i = L.uoff[j] + (int)(uint32_t)key;
-            int lo = 0, hi = np - 1;
-            while (lo < hi) {
-                const int mid = (lo + hi) >> 1;
-                if ((int)S.plist()[mid] < i) lo = mid + 1;
-                else hi = mid;
-            }
-            const YzItem it = yz_decode<MAXH>(i);
+            const uint4* br = S.brec() + 4 * j;
+            const uint32_t psw = br[3].x;
             YzSlots H;
             Fit f;
-            yz_cache_get(E, S.ycache() + 3 * lo, Tp->st3, H, f);
-            if (!S.pst()[lo].x) yz_unpack3(H, S.pst()[lo].w);
+            yz_cache_get(E, br, Tp->st3, H, f);
+            if (!(psw & 0xFFu)) yz_unpack3(H, psw >> 24);
             float chi2;
             yz_tail(E, Tp, H, f, chi2, Tp);
-            if (dbase >= 0) R.dump[(dbase + uo_lane + it.local / (L.nl[j][0] * L.nl[j][1])) * OR_UNIT_ROW + 3] = 1.f;
+            const int local = (int)(uint32_t)key;
+            if (dbase >= 0) R.dump[(dbase + uo_lane + local / (L.nl[j][0] * L.nl[j][1])) * OR_UNIT_ROW + 3] = 1.f;
         } else {
             Tp->stid = 3;
         }
@@ -1763,8 +1901,8 @@ __device__ __attribute__((noinline)) void yz_chunk_seq(Scr S, int kb, int ntkl) 
             const int iuv3 = u - L.uoff[j];
             YzLists W;
             for (int v = 0; v < 4; ++v) {
-                W.l[v] = wl + (j * 4 + v) * WIN_CAP; W.y[v] = wy + (j * 4 + v) * WIN_CAP;
-                W.f[v] = wf + (j * 4 + v) * WIN_CAP; W.n[v] = L.nl[j][v];
+                W.l[v] = wl + (j * 4 + v) * g_cfg.win_cap; W.y[v] = wy + (j * 4 + v) * g_cfg.win_cap;
+                W.f[v] = wf + (j * 4 + v) * g_cfg.win_cap; W.n[v] = L.nl[j][v];
             }
             float bc = 10000.1f;
             int bi = -1;
@@ -1784,8 +1922,8 @@ __device__ __attribute__((noinline)) void yz_chunk_seq(Scr S, int kb, int ntkl) 
         if (key != ~0ull) {
             YzLists W;
             for (int v = 0; v < 4; ++v) {
-                W.l[v] = wl + (j * 4 + v) * WIN_CAP; W.y[v] = wy + (j * 4 + v) * WIN_CAP;
-                W.f[v] = wf + (j * 4 + v) * WIN_CAP; W.n[v] = L.nl[j][v];
+                W.l[v] = wl + (j * 4 + v) * g_cfg.win_cap; W.y[v] = wy + (j * 4 + v) * g_cfg.win_cap;
+                W.f[v] = wf + (j * 4 + v) * g_cfg.win_cap; W.n[v] = L.nl[j][v];
             }
             const int flat = (int)(uint32_t)key, nin = W.n[0] * W.n[1];
             float bc = 0.f;
@@ -1806,13 +1944,14 @@ STAGE void stage_yz(or_params P, Scr S, Prn R, int ntkl, int list_cap) {
     uint16_t* wl = S.wl();
     float4* wy = S.wy();
     uint8_t* wf = S.wf();
-    const int capw = P.max_pairs_window < WIN_CAP ? P.max_pairs_window : WIN_CAP;
+    const int capw = g_cfg.win_cap;
     for (int kb = 0; kb < ntkl; kb += YZ_CHUNK) {
         OR_SUB_START
         // window lists: lane = (tracklet j, view v)
         {
             const int j = lane >> 2, v = lane & 3;
             const int k = kb + j;
+            unsigned long long raw = 1;   // list length before the accepted-hit compaction
             if (k < ntkl) {
                 const Tk& T = tks[k];
                 const int st = v < 2 ? 2 : T.st3, pj = (v & 1) ? 2 : 1;
@@ -1821,9 +1960,10 @@ STAGE void stage_yz(or_params P, Scr S, Prn R, int ntkl, int list_cap) {
                 const float za = G(F_Z, da), zb = G(F_Z, db);
                 const float xmin = fminf((T.tx - T.etx) * za, (T.tx - T.etx) * zb) + T.x0 - T.ex0 - G(F_SPACING, da);
                 const float xmax = fmaxf((T.tx + T.etx) * za, (T.tx + T.etx) * zb) + T.x0 + T.ex0 + G(F_SPACING, da);
-                uint16_t* lst = wl + (j * 4 + v) * WIN_CAP;
+                uint16_t* lst = wl + (j * 4 + v) * g_cfg.win_cap;
                 const int nl = pairs_window(E, st, pj, xmin, xmax, lst, P.max_pair_plane_hits, capw);
                 if (nl < 0) atomicOr(&L.status, (uint32_t)OR_EV_PAIR_OVERFLOW);
+                raw = nl > 0 ? (unsigned long long)nl : 0ull;
                 // y of each listed hit from the tracklet's x line (calculate_y_uvhit,
                 // sign 0).  An entry with no accepted hit ends every candidate that
                 // contains it before its fit (reconstruction_kernels.cuh:1390, 1457,
@@ -1845,13 +1985,17 @@ STAGE void stage_yz(or_params P, Scr S, Prn R, int ntkl, int list_cap) {
                     }
                     if (fl) {
                         lst[nacc] = lst[q];
-                        wy[(j * 4 + v) * WIN_CAP + nacc] = yv;
-                        wf[(j * 4 + v) * WIN_CAP + nacc] = fl;
+                        wy[(j * 4 + v) * g_cfg.win_cap + nacc] = yv;
+                        wf[(j * 4 + v) * g_cfg.win_cap + nacc] = fl;
                         ++nacc;
                     }
                 }
                 L.nl[j][v] = nacc;
             }
+            // enumeration bound: u2 x v2 x u3 x v3 list entries of one tracklet
+            raw *= (unsigned long long)__shfl_xor((long long)raw, 1, WAVE);
+            raw *= (unsigned long long)__shfl_xor((long long)raw, 2, WAVE);
+            if (raw > (unsigned long long)YZ_ENUM_MAX) atomicOr(&L.status, (uint32_t)OR_EV_PAIR_OVERFLOW);
         }
         __syncthreads();
         OR_SUB(0);
@@ -1882,6 +2026,139 @@ __device__ __forceinline__ int clean_shared(const int (&ha)[18], const int (&hb)
 #pragma unroll
     for (int t = 0; t < 18; ++t) c += (ha[t] >= 0 && ha[t] == hb[t]) ? 1 : 0;
     return c;
+}
+
+// More than 64 live tracklets: each is held in scratch as 9 words of hits (the
+// raw hit of detectors 13-30, 16 bits each, 0xFFFF none), then tracklet | slot
+// << 16 | nhits << 24 and chi2/(nhits - 4); alive bits by live position in LDS.
+// An i-step runs over its j range in chunks of 64 with the shared-hit prefix
+// sum carried from chunk to chunk.
+struct CleanEnt {
+    uint32_t h[9];
+    uint32_t kn;
+    float ratio;
+};
+__device__ __forceinline__ CleanEnt clean_load(const uint4* ent, int p) {
+    const uint4 a = gld4(ent + 3 * p), b = gld4(ent + 3 * p + 1), c = gld4(ent + 3 * p + 2);
+    CleanEnt e;
+    e.h[0] = a.x; e.h[1] = a.y; e.h[2] = a.z; e.h[3] = a.w;
+    e.h[4] = b.x; e.h[5] = b.y; e.h[6] = b.z; e.h[7] = b.w;
+    e.h[8] = c.x; e.kn = c.y; e.ratio = __uint_as_float(c.z);
+    return e;
+}
+__device__ __forceinline__ int clean_shared9(const uint32_t (&a)[9], const uint32_t (&b)[9]) {
+    int c = 0;
+#pragma unroll
+    for (int w = 0; w < 9; ++w) {
+        const uint32_t x = a[w] ^ b[w];
+        c += ((x & 0xFFFFu) == 0u && (a[w] & 0xFFFFu) != 0xFFFFu) ? 1 : 0;
+        c += ((x >> 16) == 0u && (a[w] >> 16) != 0xFFFFu) ? 1 : 0;
+    }
+    return c;
+}
+
+template <int MAXH>
+__device__ __attribute__((noinline)) void clean_many(Tk* tks, const uint16_t* slot_order, int ntkl, int nlive) {
+    Lds<MAXH>& L = g_lds<MAXH>;
+    const int lane = lane_id();
+    uint4* ent = Scr().clean_hits();
+    if (lane < 33) L.cl.lstart[lane] = 0;
+    for (int w = lane; w < CLEAN_WORDS; w += WAVE) {
+        const int b0 = w * 64;
+        L.cl.alive[w] = nlive >= b0 + 64 ? ~0ull : (nlive > b0 ? ((1ull << (nlive - b0)) - 1ull) : 0ull);
+    }
+    __syncthreads();
+    // entries in slot order, live tracklets per slot
+    int np = 0;
+    for (int j0 = 0; j0 < ntkl; j0 += WAVE) {
+        const int j = j0 + lane;
+        const int k = j < ntkl ? slot_order[j] : 0;
+        const bool live = j < ntkl && tks[k].stid >= 5;
+        const unsigned long long m = ballot(live);
+        if (live) {
+            const Tk& T = tks[k];
+            const int n = T.nx + T.nuv;
+            int hm[18];
+#pragma unroll
+            for (int t = 0; t < 18; ++t) hm[t] = 0xFFFF;
+            for (int h = 0; h < 12; ++h) {
+                if (h >= n) break;
+                const int d = T.det[h], r = T.hraw[h];
+#pragma unroll
+                for (int t = 0; t < 18; ++t)
+                    if (d == 13 + t) hm[t] = r;
+            }
+            const int p = np + rank_in(m);
+            const float ratio = T.chi2 / (float)(n - 4);
+            uint4* e = ent + 3 * p;
+            e[0] = make_uint4(hm[0] | (hm[1] << 16), hm[2] | (hm[3] << 16), hm[4] | (hm[5] << 16), hm[6] | (hm[7] << 16));
+            e[1] = make_uint4(hm[8] | (hm[9] << 16), hm[10] | (hm[11] << 16), hm[12] | (hm[13] << 16),
+                              hm[14] | (hm[15] << 16));
+            e[2] = make_uint4(hm[16] | (hm[17] << 16), (uint32_t)k | ((uint32_t)T.slot << 16) | ((uint32_t)n << 24),
+                              __float_as_uint(ratio), 0u);
+            atomicAdd(&L.cl.lstart[T.slot + 1], 1);
+        }
+        np += __popcll(m);
+    }
+    __syncthreads();
+    {
+        const int c = lane < 33 ? L.cl.lstart[lane] : 0;
+        const int inc = wave_scan(c);
+        __syncthreads();
+        if (lane < 33) L.cl.lstart[lane] = inc;
+    }
+    __syncthreads();
+    // one i-step: A at live position pa against the alive positions of [jb, je)
+    auto step = [&](int pa, int jb, int je) {
+        const CleanEnt A = clean_load(ent, pa);
+        const int na = (int)(A.kn >> 24);
+        int carry = 0;
+        bool di = false;
+        for (int c0 = jb; c0 < je; c0 += WAVE) {
+            const int j = c0 + lane;
+            const bool part = j < je && ((L.cl.alive[j >> 6] >> (j & 63)) & 1ull);
+            int c = 0;
+            CleanEnt B;
+            if (part) {
+                B = clean_load(ent, j);
+                c = clean_shared9(A.h, B.h);
+            }
+            const int common = carry + wave_scan(c);
+            bool dj = false, dl = false;
+            if (part) {
+                if (A.ratio < B.ratio) dj = (float)common > (float)na * 0.3333f;
+                else dl = (float)common > (float)(int)(B.kn >> 24) * 0.3333f;
+            }
+            __syncthreads();
+            if (dj) atomicAnd(&L.cl.alive[j >> 6], ~(1ull << (j & 63)));
+            di = di || ballot(dl) != 0ull;
+            carry = __shfl(common, WAVE - 1, WAVE);
+        }
+        if (di && lane == 0) atomicAnd(&L.cl.alive[pa >> 6], ~(1ull << (pa & 63)));
+        __syncthreads();
+    };
+    // within each slot (reconstruction_helper.cuh:1789-1843)
+    for (int pa = 0; pa < nlive; ++pa) {
+        if (!((L.cl.alive[pa >> 6] >> (pa & 63)) & 1ull)) continue;
+        const int sa = (int)((gld4(ent + 3 * pa + 2).y >> 16) & 0xFFu);
+        step(pa, pa + 1, L.cl.lstart[sa + 1]);
+    }
+    // slot against slot (reconstruction_helper.cuh:1846-1909): for s, for o != s, for i in s, for j in o
+    for (int sidx = 0; sidx < 32; ++sidx) {
+        const int sb = L.cl.lstart[sidx], se = L.cl.lstart[sidx + 1];
+        if (sb == se) continue;
+        for (int oidx = 0; oidx < 32; ++oidx) {
+            const int ob = L.cl.lstart[oidx], oe = L.cl.lstart[oidx + 1];
+            if (oidx == sidx || ob == oe) continue;
+            for (int pa = sb; pa < se; ++pa) {
+                if (!((L.cl.alive[pa >> 6] >> (pa & 63)) & 1ull)) continue;
+                step(pa, ob, oe);
+            }
+        }
+    }
+    for (int p = lane; p < nlive; p += WAVE)
+        if (!((L.cl.alive[p >> 6] >> (p & 63)) & 1ull)) tks[gld4(ent + 3 * p + 2).y & 0xFFFFu].stid = 3;
+    __syncthreads();
 }
 
 template <int MAXH>
@@ -1969,6 +2246,10 @@ STAGE void stage_clean(Tk* tks, const uint16_t* slot_order, int ntkl) {
         }
         if (mine && !((alive >> lane) & 1ull)) tks[k].stid = 3;
         __syncthreads();
+        return;
+    }
+    if (nlive <= WAVE * CLEAN_WORDS) {
+        clean_many<MAXH>(tks, slot_order, ntkl, nlive);
         return;
     }
     if (lane == 0) {
@@ -2186,9 +2467,9 @@ __device__ __forceinline__ bool global_yfit(const Ev<MAXH>& E, const uint16_t* l
 // which of a, b they are.  A (u, v) pair then adds the V hits to the U row,
 // the same operations in the same order as global_yfit.
 struct GTab {
-    float4* u;      // [WIN_CAP][2]: A0, A1, A3, B0 | B1, U hits
-    float4* v;      // [WIN_CAP]: y a, ey a, y b, ey b
-    uint32_t* vf;   // [WIN_CAP]: bit 0 hit a accepted, bit 1 hit b
+    float4* u;      // [win_cap][2]: A0, A1, A3, B0 | B1, U hits
+    float4* v;      // [win_cap]: y a, ey a, y b, ey b
+    uint32_t* vf;   // [win_cap]: bit 0 hit a accepted, bit 1 hit b
 };
 
 template <int MAXH>
@@ -2251,7 +2532,7 @@ __device__ __forceinline__ bool global_yfit_tab(const GTab& T, int nu1, int iuv,
     return true;
 }
 
-constexpr int GLOBAL_LIST_CAP = 2048;   // pass-A survivors held before pass B runs
+constexpr int GLOBAL_LIST_CAP = 2048;   // pass-A survivors held before pass B runs (<= list_cap)
 
 // One track: pass A over its x pairs, pass B over the listed survivors, then
 // the best candidate is written.  Uses the x-part records (S.wy() + 8), the
@@ -2268,8 +2549,8 @@ __device__ __attribute__((noinline)) void global_track(Scr S, Tk* T, const uint1
     uint32_t* list = S.plist();
     GTab tab;
     tab.u = reinterpret_cast<float4*>(S.ycache());
-    tab.v = tab.u + 2 * WIN_CAP;
-    tab.vf = reinterpret_cast<uint32_t*>(tab.v + WIN_CAP);
+    tab.v = tab.u + 2 * g_cfg.win_cap;
+    tab.vf = reinterpret_cast<uint32_t*>(tab.v + g_cfg.win_cap);
     const int n23 = T->nx + T->nuv;
     const float x0 = T->x0, tx = T->tx, ex0 = T->ex0, etx = T->etx;
     const float einvp = (etx - etx) / PT_KICK_KMAG;
@@ -2454,7 +2735,7 @@ STAGE void stage_global(or_params P, Scr S, int ntkl) {
     const int lane = lane_id();
     Tk* tks = S.tks();
     int* nlf = &L.nl[0][0];
-    const int capw = P.max_pairs_window < WIN_CAP ? P.max_pairs_window : WIN_CAP;
+    const int capw = g_cfg.win_cap;
     for (int k0 = 0; k0 < ntkl; k0 += WAVE) {
         unsigned long long rest = ballot(k0 + lane < ntkl && tks[k0 + lane].stid >= 5);
         while (rest) {
@@ -2465,7 +2746,7 @@ STAGE void stage_global(or_params P, Scr S, int ntkl) {
                 for (int t = 0; t < b && mm; ++t) mm &= mm - 1;
                 if (b < GLOBAL_BATCH && mm) {
                     const int k = k0 + __ffsll((long long)mm) - 1;
-                    const int nl = global_window(E, &tks[k], v, S.wl() + lane * WIN_CAP, P.max_pair_plane_hits, capw);
+                    const int nl = global_window(E, &tks[k], v, S.wl() + lane * g_cfg.win_cap, P.max_pair_plane_hits, capw);
                     if (nl < 0) atomicOr(&L.status, (uint32_t)OR_EV_PAIR_OVERFLOW);
                     nlf[lane] = nl;
                 }
@@ -2478,8 +2759,8 @@ STAGE void stage_global(or_params P, Scr S, int ntkl) {
                 rest &= rest - 1;
                 const int nx1 = nlf[3 * b], nu1 = nlf[3 * b + 1], nv1 = nlf[3 * b + 2];
                 if (nx1 == 0 || nu1 == 0 || nv1 == 0) continue;
-                global_track<MAXH>(S, &tks[k], S.wl() + (3 * b) * WIN_CAP, S.wl() + (3 * b + 1) * WIN_CAP,
-                                   S.wl() + (3 * b + 2) * WIN_CAP, nx1, nu1, nv1);
+                global_track<MAXH>(S, &tks[k], S.wl() + (3 * b) * g_cfg.win_cap, S.wl() + (3 * b + 1) * g_cfg.win_cap,
+                                   S.wl() + (3 * b + 2) * g_cfg.win_cap, nx1, nu1, nv1);
             }
         }
     }
@@ -3314,9 +3595,16 @@ template <int MAXH>
 __global__ __launch_bounds__(WAVE, MAXH <= 512 ? OR_WAVES_PER_SIMD_512 : (MAXH <= 1024 ? 2 : 1)) void chain_kernel(Args a) {
     Lds<MAXH>& L = g_lds<MAXH>;
     const int lane = lane_id();
-    char* scr = a.scratch + (int64_t)blockIdx.x * a.scratch_stride;
+    if (lane < R_COUNT) g_cfg.rp[lane] = a.scratch + (int64_t)blockIdx.x * a.lay.stride + a.lay.off[lane];
+    if (lane == 0) {
+        g_cfg.maxtkl = a.lay.maxtkl;
+        g_cfg.bin_cap = a.lay.bin_cap;
+        g_cfg.win_cap = a.lay.win_cap;
+        g_cfg.list_cap = a.lay.list_cap;
+        g_cfg.unit_cap = a.lay.unit_cap;
+    }
+    __syncthreads();
     Scr S;
-    S.base = scr;
     const int64_t ne = a.ev_list ? (int64_t)*a.ev_count : a.n_events;
 #ifdef OR_STAGE_TIMING
     unsigned long long tacc[N_STAGE_TIMERS] = {0, 0, 0, 0, 0, 0, 0};
@@ -3424,10 +3712,11 @@ __global__ void prune_score_kernel(const PrunerDev* W, const float* feats, int64
 // ============================================================================
 struct or_session {
     int device = 0;
+    bool geo_registered = false;   // counted in g_dev_geo[device].users
     or_params p;
     float* d_wire = nullptr;
     char* d_scratch = nullptr;
-    int64_t scratch_blocks = 0;
+    ScrLayout lay;               // per-wave scratch layout, from the caps and the YZ batch capacity
     int grid = 0;
     int maxh = 1024;
     int nclass = 2;
@@ -3440,7 +3729,7 @@ struct or_session {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     mutable float last_ms = 0.f;
     mutable bool timing_pending = false;   // ev0/ev1 recorded, elapsed time not read yet
-    int yz_list_cap = YZ_LIST_CAP;
+    int yz_list_cap = YZ_LIST_MIN;   // survivor batch size of the YZ passes (<= lay.list_cap)
     int grid_pct = 100;          // persistent grid, percent of the occupancy limit
     bool use_graph = false;      // replay the launch sequence as a captured hipGraph
     hipGraphExec_t graph_exec = nullptr;
@@ -3465,6 +3754,55 @@ struct or_session {
     void* d_out = nullptr;
     size_t d_out_bytes = 0;
 };
+
+// Scratch layout of one wave from the session's caps: every region sized by
+// the cap that bounds it.
+static ScrLayout make_layout(const or_params& p, int list_cap) {
+    ScrLayout L;
+    std::memset(&L, 0, sizeof L);
+    const int64_t maxtkl = p.max_tracklets_per_event, win = p.max_pairs_window, bin = p.max_pairs_per_bin;
+    const int64_t unit_cap = std::max<int64_t>(list_cap, 16384);
+    int64_t o = 0;
+    auto put = [&](int r, int64_t bytes) {
+        L.off[r] = o;
+        o = (o + std::max<int64_t>(bytes, 1) + 255) / 256 * 256;
+    };
+    put(R_TK, (int64_t)sizeof(Tk) * maxtkl);
+    put(R_CELL, 1569 * 4);
+    put(R_LIST, 64 * win * 2);
+    put(R_YV, std::max<int64_t>(64 * win * 16, 8 * 16 + win * (int64_t)sizeof(GX)));   // station 1: best + x records
+    put(R_YF, 64 * win);
+    put(R_XP, 3 * 28 * bin * 2);
+    put(R_SO, maxtkl * 2);
+    put(R_PL, (int64_t)std::max(list_cap, GLOBAL_LIST_CAP) * 4);
+    put(R_PB, (int64_t)list_cap * 4);
+    put(R_KL, unit_cap * 4);
+    put(R_KS, unit_cap * 4);
+    put(R_YC, std::max<int64_t>((int64_t)list_cap * 48, win * 52));                     // station 1: pass-A tables
+    put(R_QL, (int64_t)list_cap * 4);
+    put(R_BR, YZ_CHUNK * 64);
+    put(R_CL, std::min<int64_t>(maxtkl, (int64_t)WAVE * CLEAN_WORDS) * 48);
+    L.stride = (o + 4095) / 4096 * 4096;
+    L.maxtkl = (int32_t)maxtkl;
+    L.bin_cap = (int32_t)bin;
+    L.win_cap = (int32_t)win;
+    L.list_cap = list_cap;
+    L.unit_cap = (int32_t)unit_cap;
+    return L;
+}
+
+// The plane table and the final-fit constants sit in constant memory, one copy
+// per device: a session may only be created on a device whose live sessions use
+// the same geometry (the tables are uploaded when no session is live).
+namespace {
+struct DevGeo {
+    bool set = false;
+    int users = 0;
+    float f[F_COUNT][OR_N_DETECTORS];
+};
+std::mutex g_geo_mu;
+DevGeo g_dev_geo[64];
+}  // namespace
 
 #define OR_HIP(x)                                                          \
     do {                                                                   \
@@ -3512,11 +3850,13 @@ or_status or_session_create(const or_geometry* g, const or_params* p, int device
     or_params prm;
     if (p) prm = *p;
     else or_params_default(&prm);
-    if (prm.max_pairs_per_bin > BIN_CAP || prm.max_pairs_window > WIN_CAP ||
-        prm.max_tracklets_per_event > MAXTKL || prm.max_hits_per_event > MAX_EVENT_HITS ||
+    if (prm.max_pairs_per_bin < 1 || prm.max_pairs_window < 1 || prm.max_tracklets_per_event < 1 ||
+        prm.max_tracklets_per_slot < 1 || prm.max_hits_per_event < 1 || prm.max_tracks_out < 0)
+        return OR_ERR_INVALID_ARG;
+    if (prm.max_pairs_per_bin > PAIRS_MAX || prm.max_pairs_window > PAIRS_MAX ||
+        prm.max_tracklets_per_event > TKL_MAX || prm.max_hits_per_event > MAX_EVENT_HITS ||
         prm.max_hits_chamber_plane > PLANE_CAP || prm.max_hits_prop_plane > PLANE_CAP ||
-        prm.max_hits_hodo_plane > PLANE_CAP || prm.max_pair_plane_hits > 254 || prm.max_reduced_per_plane > 254 ||
-        prm.max_tracks_out < 0)
+        prm.max_hits_hodo_plane > PLANE_CAP || prm.max_pair_plane_hits > 254 || prm.max_reduced_per_plane > 254)
         return OR_ERR_CAPACITY;
     auto* s = new (std::nothrow) or_session;
     if (!s) return OR_ERR_NOMEM;
@@ -3530,6 +3870,16 @@ or_status or_session_create(const or_geometry* g, const or_params* p, int device
     s->stream = s->own;
     OR_HIP(hipEventCreate(&s->ev0));
     OR_HIP(hipEventCreate(&s->ev1));
+    std::unique_lock<std::mutex> geo_lock(g_geo_mu);
+    DevGeo& dg = g_dev_geo[device & 63];
+    if (dg.users > 0 && std::memcmp(dg.f, g->f, sizeof g->f) != 0) {
+        (void)hipEventDestroy(s->ev0);
+        (void)hipEventDestroy(s->ev1);
+        (void)hipStreamDestroy(s->own);
+        delete s;
+        return OR_ERR_CONFLICT;
+    }
+    if (dg.users == 0) {
     OR_HIP(hipMemcpyToSymbol(HIP_SYMBOL(ordev::c_geo), g->f, sizeof g->f));
     {
         // final-fit plane constants in double, split into hi/lo floats
@@ -3556,6 +3906,12 @@ or_status or_session_create(const or_geometry* g, const or_params* p, int device
         }
         OR_HIP(hipMemcpyToSymbol(HIP_SYMBOL(ordev::c_fitp), fp, sizeof fp));
     }
+    std::memcpy(dg.f, g->f, sizeof g->f);
+    dg.set = true;
+    }
+    ++dg.users;
+    s->geo_registered = true;
+    geo_lock.unlock();
     OR_HIP(hipMalloc(&s->d_wire, sizeof g->wire));
     OR_HIP(hipMemcpy(s->d_wire, g->wire, sizeof g->wire, hipMemcpyHostToDevice));
     hipDeviceProp_t prop;
@@ -3567,9 +3923,9 @@ or_status or_session_create(const or_geometry* g, const or_params* p, int device
         s->grid_c[c] = prop.multiProcessorCount * occ[c];
         s->grid = std::max(s->grid, s->grid_c[c]);
     }
-    s->scratch_blocks = s->grid;
+    s->lay = make_layout(prm, YZ_LIST_MIN);
     OR_HIP(hipMalloc(&s->d_counts, sizeof(int32_t) * 2 * NCLASS));
-    OR_HIP(hipMalloc(&s->d_scratch, (size_t)SCR_STRIDE * s->scratch_blocks));
+    OR_HIP(hipMalloc(&s->d_scratch, (size_t)s->lay.stride * s->grid));
     *out = s;
     return OR_OK;
 }
@@ -3601,7 +3957,18 @@ or_status or_session_set_tuning(or_session* s, or_tuning key, int64_t value) {
     if (!s) return OR_ERR_INVALID_ARG;
     switch (key) {
         case OR_TUNE_YZ_LIST_CAP:
-            if (value < 1 || value > YZ_LIST_CAP) return OR_ERR_INVALID_ARG;
+            if (value < 1 || value > YZ_LIST_MAX) return OR_ERR_INVALID_ARG;
+            if (value > s->lay.list_cap) {
+                // a larger batch than the scratch holds: re-size the scratch
+                const ScrLayout nl = make_layout(s->p, (int)value);
+                OR_HIP(hipSetDevice(s->device));
+                OR_HIP(hipStreamSynchronize(s->stream));
+                if (s->graph_exec) { (void)hipGraphExecDestroy(s->graph_exec); s->graph_exec = nullptr; }
+                OR_HIP(hipFree(s->d_scratch));
+                s->d_scratch = nullptr;
+                OR_HIP(hipMalloc(&s->d_scratch, (size_t)nl.stride * s->grid));
+                s->lay = nl;
+            }
             s->yz_list_cap = (int)value;
             return OR_OK;
         case OR_TUNE_GRID_PERCENT:
@@ -3639,8 +4006,8 @@ static or_status issue_chain(or_session* s, const or_hits* in, const or_result* 
     a.status = out->status; a.keep_mask = out->keep_mask; a.n_tracks = out->n_tracks; a.tracks = out->tracks;
     a.n_tracklets = out->n_tracklets; a.records = out->records; a.vertex = out->vertex; a.fit = out->fit;
     a.scratch = s->d_scratch;
-    a.scratch_stride = SCR_STRIDE;
-    a.yz_list_cap = s->yz_list_cap;
+    a.lay = s->lay;
+    a.yz_list_cap = std::min(s->yz_list_cap, s->lay.list_cap);
     a.pruner = s->prune_on ? s->d_pruner : nullptr;
     a.rt = s->rt_on ? s->d_rt : nullptr;
     a.unit_dump = s->dump;
@@ -3976,7 +4343,7 @@ or_status or_session_set_unit_dump(or_session* s, float* rows, int64_t cap, unsi
 or_status or_session_autotune(or_session* s, const or_hits* in, or_result* out, int reps, float* best_ms) {
     if (!s || !in || !out || !in->on_device || !out->on_device || reps < 1) return OR_ERR_INVALID_ARG;
     static const int grid_pcts[] = {100, 75, 50};
-    static const int list_caps[] = {YZ_LIST_CAP, YZ_LIST_CAP / 2, YZ_LIST_CAP / 4};
+    const int list_caps[] = {s->lay.list_cap, s->lay.list_cap / 2, s->lay.list_cap / 4};
     int best_pct = s->grid_pct, best_cap = s->yz_list_cap;
     float best = -1.f;
     for (int cap : list_caps) {
@@ -4004,6 +4371,11 @@ or_status or_session_autotune(or_session* s, const or_hits* in, or_result* out, 
 void or_session_destroy(or_session* s) {
     if (!s) return;
     (void)hipSetDevice(s->device);
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    if (s->geo_registered) {
+        std::lock_guard<std::mutex> lk(g_geo_mu);
+        --g_dev_geo[s->device & 63].users;
+    }
     if (s->d_wire) (void)hipFree(s->d_wire);
     if (s->d_scratch) (void)hipFree(s->d_scratch);
     if (s->d_lists) (void)hipFree(s->d_lists);

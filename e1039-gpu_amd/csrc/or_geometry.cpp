@@ -147,7 +147,8 @@ const char* or_status_string(or_status st) {
         case OR_ERR_DEVICE: return "HIP runtime error";
         case OR_ERR_NOMEM: return "out of memory";
         case OR_ERR_NO_DEVICE: return "no GPU device visible";
-        case OR_ERR_CAPACITY: return "parameter exceeds a compiled capacity";
+        case OR_ERR_CAPACITY: return "parameter exceeds the bound of an index encoding";
+        case OR_ERR_CONFLICT: return "a live session on this device uses a different geometry";
     }
     return "unknown status";
 }

@@ -194,6 +194,24 @@ def default_params(**overrides) -> OrParams:
     return p
 
 
+# Caps for high-occupancy workloads (hundreds to thousands of hits per event):
+# the reference's staging, multiplicity, pair-list and tracklet-storage caps
+# (reconstruction_constants.h:154,173-191; reconstruction_kernels.cuh:16,225,896)
+# lifted to the bounds of the library's index encodings, so that the plain
+# method runs on every event instead of rejecting it.
+HIGH_OCCUPANCY_CAPS = dict(
+    max_hits_chamber_plane=400, max_hits_hodo_plane=400, max_hits_prop_plane=400,
+    max_reduced_per_plane=254, max_pair_plane_hits=254, max_pairs_per_bin=255, max_pairs_window=255,
+    max_tracklets_per_slot=16384, max_tracklets_per_event=2048, max_d0_mult=1 << 30, max_d2_mult=1 << 30,
+    max_d3_mult=1 << 30, max_tracks_out=64, max_hits_per_event=8192,
+)
+
+
+def high_occupancy_params(**overrides) -> OrParams:
+    """or_params with HIGH_OCCUPANCY_CAPS, then overrides."""
+    return default_params(**dict(HIGH_OCCUPANCY_CAPS, **overrides))
+
+
 class Geometry:
     """Parsed geometry + wire table (or_geometry_load)."""
 

@@ -1,13 +1,14 @@
 """Cycles per chain stage on the GPU, from the profiling build.
 
     make -C e1039-gpu_amd/csrc timing
-    python e1039-gpu_amd/stage_cycles.py [n_events] [mean_hits | occ8] [pruned]
+    python e1039-gpu_amd/stage_cycles.py [n_events] [mean_hits | occ8] [pruned] [high]
 
 Runs one warm-up and one measured launch of the chain on seeded events
 (hits/event ~ Poisson(mean_hits) capped at 1000, or 8 % noise occupancy per
 channel with occ8) through libe1039or_timing.so
 and prints each stage's share of the summed wave cycles.  With "pruned" the
-shipped YZ unit pruner is on.
+shipped YZ unit pruner is on; with "high" the caps are orchain.HIGH_OCCUPANCY_CAPS
+and the hit count is capped at 6000 instead of 1000.
 """
 import ctypes as C
 import os
@@ -32,15 +33,17 @@ def main():
     n = int(sys.argv[1]) if len(sys.argv) > 1 else 200_000
     wl = sys.argv[2] if len(sys.argv) > 2 else "300"
     # "occ8": 8 % noise occupancy per channel; a number: Poisson mean hits capped at 1000
+    high = "high" in sys.argv[3:]
     gen = dict(mult_mode=0, noise_occupancy=0.08) if wl == "occ8" else dict(mult_mode=1, mult_mean=float(wl),
-                                                                              mult_cap=1000)
+                                                                              mult_cap=6000 if high else 1000)
     lib = oc.load_library(os.path.join(HERE, "libe1039or_timing.so"))
     lib.or_debug_stage_cycles.argtypes = [C.c_void_p, C.c_int]
     lib.or_debug_stage_cycles.restype = C.c_int32
     geo = oc.Geometry.load()
     ev = oc.generate(geo, n, seed=7, **gen)
-    sess = oc.Session(geo, oc.default_params(max_hits_per_event=1024), device=0)
-    if len(sys.argv) > 3 and sys.argv[3] == "pruned":
+    params = oc.high_occupancy_params() if high else oc.default_params(max_hits_per_event=1024)
+    sess = oc.Session(geo, params, device=0)
+    if "pruned" in sys.argv[3:]:
         import pruner
         flat, thr, _ = pruner.load()
         sess.set_pruner(flat, thr)

@@ -32,7 +32,8 @@ typedef enum {
     OR_ERR_DEVICE = 4,        /* HIP runtime error                          */
     OR_ERR_NOMEM = 5,
     OR_ERR_NO_DEVICE = 6,     /* no GPU visible                             */
-    OR_ERR_CAPACITY = 7       /* a parameter exceeds a compiled bound       */
+    OR_ERR_CAPACITY = 7,      /* a parameter exceeds the bound of an index encoding (see or_params) */
+    OR_ERR_CONFLICT = 8       /* a live session on the device uses a different geometry */
 } or_status;
 
 /* ---- per-event status bits (the reference's HasTooManyHits, split by cause)
@@ -46,7 +47,9 @@ enum {
     OR_EV_XZ_BIN_OVERFLOW= 1u << 4,  /* > 20 X pairs in one bin (reconstruction_kernels.cuh:359-375) */
     OR_EV_TRACKLET_OVERFLOW = 1u << 5, /* tracklet slots exhausted (reconstruction_kernels.cuh:719,824,835) */
     OR_EV_PLANE_OVERFLOW = 1u << 6,  /* more hits in a plane than the staging arrays hold (reconstruction_constants.h:173-175) */
-    OR_EV_PAIR_OVERFLOW  = 1u << 7,  /* > 100 hits in a pair-builder plane or > 100 pairs in a window list (reconstruction_kernels.cuh:225,896) */
+    OR_EV_PAIR_OVERFLOW  = 1u << 7,  /* > max_pair_plane_hits in a pair-builder plane, > max_pairs_window pairs in a window list
+                                        (reconstruction_kernels.cuh:225,896), or the four YZ window lists of one tracklet
+                                        spanning more than 2^27 (u3, v3, u2, v2) combinations */
     OR_EV_CLUSTER_OVERFLOW = 1u << 8,/* chamber cluster longer than 192 hits (reconstruction_helper.cuh:237) */
     OR_EV_EVENT_TOO_LARGE= 1u << 9,  /* event has more hits than the session's per-event bound */
     OR_EV_BAD_HIT        = 1u << 10, /* detectorID < 1 or elementID outside [1, nelem] */
@@ -65,7 +68,13 @@ typedef struct or_session or_session;     /* opaque */
 
 /* Runtime parameters: caps and cuts of the reference as values, so that
  * high-occupancy workloads can lift them.  or_params_default() fills the
- * reference values. */
+ * reference values.  The session sizes its device scratch from them (tracklet
+ * storage from max_tracklets_per_event, X pair bins from max_pairs_per_bin,
+ * window lists from max_pairs_window).  Bounds come from index encodings only:
+ * max_pairs_per_bin and max_pairs_window <= 255, max_tracklets_per_event <=
+ * 16384, per-plane hit caps <= 400, max_pair_plane_hits and
+ * max_reduced_per_plane <= 254, max_hits_per_event <= 8192; beyond them
+ * or_session_create returns OR_ERR_CAPACITY. */
 typedef struct {
     int32_t max_hits_chamber_plane;   /* 200  datasizes::NMaxHitsChambers (reconstruction_constants.h:173) */
     int32_t max_hits_hodo_plane;      /* 60   (reconstruction_constants.h:174) */
@@ -177,8 +186,11 @@ void or_geometry_free(or_geometry* g);
  * into caller buffers; either pointer may be NULL. */
 or_status or_geometry_export(const or_geometry* g, float* planes_27x55, float* wire_table);
 
-/* Creates a device session: uploads geometry, sizes the per-wave scratch.
- * Replaces the cudaMalloc/cudaMemcpy block (online_reconstruction.cu:652-757). */
+/* Creates a device session: uploads geometry, sizes the per-wave scratch from
+ * the caps in p.  Replaces the cudaMalloc/cudaMemcpy block
+ * (online_reconstruction.cu:652-757).  The plane table lives in constant
+ * memory, one copy per device: while sessions are live on a device, a session
+ * with a different geometry is refused (OR_ERR_CONFLICT). */
 or_status or_session_create(const or_geometry* g, const or_params* p, int device, or_session** out);
 
 /* Runs the chain on a batch of events.  Replaces the kernel sequence
@@ -205,9 +217,12 @@ float or_session_last_kernel_ms(const or_session* s);
 /* Tuning knobs of a session; results do not depend on them.  No reference
  * counterpart (the reference hard-codes its launch shapes). */
 typedef enum {
-    /* YZ candidates surviving the first box cut that one chunk of 16
-     * tracklets keeps for parallel evaluation, 1..4096 (default 4096); a chunk
-     * with more is evaluated unit by unit instead */
+    /* YZ candidates surviving the first box cut that are evaluated together
+     * (one batch), 1..2^20 (default 4096).  A chunk of 16 tracklets with more
+     * survivors runs in several batches cut at (u3, v3) unit boundaries; a
+     * chunk in which one unit alone has more is evaluated unit by unit.
+     * A value above the session's current batch capacity re-sizes its scratch
+     * (waits for the session stream). */
     OR_TUNE_YZ_LIST_CAP = 1,
     /* persistent grid size in percent of the occupancy limit, 1..100 (default 100) */
     OR_TUNE_GRID_PERCENT = 2,

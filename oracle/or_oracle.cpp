@@ -652,7 +652,9 @@ static bool run_yz_one(Ctx& C, Tkl& T) {
     R[0] = 3;
     /* views: 0 u2, 1 v2, 2 u3, 3 v3 (reconstruction_kernels.cuh:1252-1283) */
     const int vst[4] = {2, 2, st3, st3}, vpj[4] = {1, 2, 1, 2};
-    Pair lists[4][128];
+    const int wcap = std::max(C.p->max_pairs_window, 1);
+    std::vector<Pair> lists[4];
+    for (auto& l : lists) l.resize(wcap);
     int nl[4];
     for (int v = 0; v < 4; ++v) {
         int sid = kSuperId[vst[v]][vpj[v]];
@@ -660,10 +662,13 @@ static bool run_yz_one(Ctx& C, Tkl& T) {
         float za = g.z[da], zb = g.z[db];
         float xmin = std::fmin((tx - etx) * za, (tx - etx) * zb) + x0 - ex0 - g.spacing[da];
         float xmax = std::fmax((tx + etx) * za, (tx + etx) * zb) + x0 + ex0 + g.spacing[da];
-        nl[v] = pairs_window(C, vst[v], vpj[v], xmin, xmax, lists[v], C.p->max_pairs_window);
+        nl[v] = pairs_window(C, vst[v], vpj[v], xmin, xmax, lists[v].data(), C.p->max_pairs_window);
         if (nl[v] < 0) { C.status |= OR_EV_PAIR_OVERFLOW; return false; }
     }
     const int nu2 = nl[0], nv2 = nl[1], nu3 = nl[2], nv3 = nl[3];
+    /* enumeration bound of the device (32-bit candidate indices): more than
+     * 2^27 (u3, v3, u2, v2) list combinations for one tracklet is an overflow */
+    if ((long long)nu2 * nv2 * nu3 * nv3 > (1ll << 27)) { C.status |= OR_EV_PAIR_OVERFLOW; return false; }
     if (nu2 == 0 || nu3 == 0) return true;
     if (nv2 == 0 || nv3 == 0) return true;
     float dd[12], res[12], p1x[12], p1y[12], p1z[12], dpx[12], dpy[12], dpz[12], rsd[12];
@@ -916,7 +921,9 @@ static bool run_global_one(Ctx& C, Tkl& T) {
             win[idx] = 0.5 * (pmx - pmn);
         }
     }
-    Pair lx[128], lu[128], lv[128];
+    const int wcap = std::max(C.p->max_pairs_window, 1);
+    std::vector<Pair> lxv(wcap), luv(wcap), lvv(wcap);
+    Pair *lx = lxv.data(), *lu = luv.data(), *lv = lvv.data();
     int nx1 = pairs_window(C, 0, 0, pexp[0] - win[0] * 1.0f, pexp[0] + win[0] * 1.0f, lx, C.p->max_pairs_window);
     int nu1 = pairs_window(C, 0, 1, pexp[1] - win[1] * 1.0f, pexp[1] + win[1] * 1.0f, lu, C.p->max_pairs_window);
     int nv1 = pairs_window(C, 0, 2, pexp[2] - win[2] * 1.0f, pexp[2] + win[2] * 1.0f, lv, C.p->max_pairs_window);
