@@ -14,6 +14,7 @@
 
 #include <algorithm>
 #include <vector>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -194,6 +195,7 @@ struct Lds {
         } cl;
     };
     int uoff[YZ_CHUNK + 1];   // prefix of (u3, v3) units or of candidates per tracklet of the chunk
+    int umax[YZ_CHUNK];       // pruning: best unit score of each tracklet of the chunk (order-preserving int)
     unsigned long long best[YZ_CHUNK];
     int sstart[33];
     uint32_t status;
@@ -280,6 +282,19 @@ __device__ __forceinline__ int wave_scan(int v) {
     }
     return v;
 }
+// Orders the lanes' LDS and memory accesses of one wave without a workgroup
+// barrier: writes made by any lane before it are seen by every lane after it.
+__device__ __forceinline__ void wsync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+// float <-> int with the same order (for integer atomic max of float values)
+__device__ __forceinline__ int float_order(float f) {
+    const int i = __float_as_int(f);
+    return i >= 0 ? i : i ^ 0x7FFFFFFF;
+}
+__device__ __forceinline__ float float_unorder(int i) { return __int_as_float(i >= 0 ? i : i ^ 0x7FFFFFFF); }
 __device__ __forceinline__ int wave_sum(int v) {
 #pragma unroll
     for (int o = WAVE / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
@@ -1295,25 +1310,102 @@ __device__ __forceinline__ void yz_candidate(const Ev<MAXH>& E, const Scr& S, co
     yz_fill<6>(E, W, 1, it.iv2, H);
 }
 
-// Features of YZ unit (tracklet j, u3.v3 index iuv3) for the pruner: the
-// tracklet's x line and the y of the first and last accepted hit of the u3 and
-// v3 entries (window lists hold accepted entries only).
-__device__ __forceinline__ UnitFeatures unit_features(const Scr& S, const Tk& T, int j, int iuv3, int nu3) {
-    const int iu3 = iuv3 % nu3, iv3 = iuv3 / nu3;
-    const int qu = (j * 4 + 2) * g_cfg.win_cap + iu3, qv = (j * 4 + 3) * g_cfg.win_cap + iv3;
+// Features of YZ unit (tracklet j, u3.v3 index iuv3) for the pruner.  The
+// window lists hold accepted entries only; a view's y is the mean of its
+// entry's accepted hits.  With y3 the mean y of the unit's station-3 hits at
+// the mean z3 of its U and V planes, the line through (z3, y3) and the beam
+// axis at z = 0 predicts the y of any later hit; the box cut |y0| <= 50 cm at
+// z = 0 keeps every candidate's line near it.  Features: 10 tx, y3/50, the
+// signed and absolute U - V difference at station 3, the a - b difference
+// within the U3 and the V3 entry, U3 and V3 hit counts - 1, per station-2
+// view the distance of the nearest list entry to the predicted y, then for the
+// representative candidate (the unit's station-3 entries with the nearest U2
+// and V2 entries, evaluated as every candidate is) log(1 + chi2)/4 and how far
+// it gets (0: fails the first box cut, 1: fails a sign-free test, 2: passes),
+// the nearest U2 minus nearest V2 y, the distance of the nearest P1 Y and P2 Y
+// prop-tube hit to the predicted y, and the candidate's |y0|/50 (distances in
+// units of 10 cm; every feature clamped at 10).
+template <int MAXH>
+__device__ __forceinline__ UnitFeatures unit_features(const Scr& S, const Tk& T, int j, int iuv3, const int4 nl) {
+    const Ev<MAXH> E;
+    const int win = S.win();
+    const int iu3 = iuv3 % nl.z, iv3 = iuv3 / nl.z;
+    const int qu = (j * 4 + 2) * win + iu3, qv = (j * 4 + 3) * win + iv3;
     const float4 yu = gld4(S.wy() + qu), yw = gld4(S.wy() + qv);
     const unsigned fu = gld(S.wf() + qu), fv = gld(S.wf() + qv);
-    const float ya = (fu & 1u) ? yu.x : yu.z, eya = (fu & 1u) ? yu.y : yu.w;
-    const float yb = (fu & 2u) ? yu.z : yu.x, eyb = (fu & 2u) ? yu.w : yu.y;
-    const float yc = (fv & 1u) ? yw.x : yw.z, eyc = (fv & 1u) ? yw.y : yw.w;
-    const float yd = (fv & 2u) ? yw.z : yw.x, eyd = (fv & 2u) ? yw.w : yw.y;
-    const float du = 0.5f * (ya + yb) - 0.5f * (yc + yd);
+    const float ya = (fu & 1u) ? yu.x : yu.z, yb = (fu & 2u) ? yu.z : yu.x;
+    const float yc = (fv & 1u) ? yw.x : yw.z, yd = (fv & 2u) ? yw.z : yw.x;
+    const float yum = 0.5f * (ya + yb), yvm = 0.5f * (yc + yd);
+    const float y3 = 0.5f * (yum + yvm);
+    const int st3 = T.st3;
+    const float z3 = 0.5f * (G(F_Z, super_id(st3, 1) * 2) + G(F_Z, super_id(st3, 2) * 2));
+    const float k3 = y3 / z3;   // predicted y per cm of z
+    float d2[2], yb2[2];
+    int k2[2];
+#pragma unroll
+    for (int v = 0; v < 2; ++v) {
+        const float yp = k3 * G(F_Z, super_id(2, v + 1) * 2);
+        float dmin = 100.f, ybest = yp;
+        int kb = 0;
+        const int n = v == 0 ? nl.x : nl.y;
+        for (int k = 0; k < n; ++k) {
+            const int q = (j * 4 + v) * win + k;
+            const float4 e = gld4(S.wy() + q);
+            const unsigned fl = gld(S.wf() + q);
+            const float y = (fl & 1u) ? ((fl & 2u) ? 0.5f * (e.x + e.z) : e.x) : e.z;
+            const float d = fabsf(y - yp);
+            if (d < dmin) { dmin = d; ybest = y; kb = k; }
+        }
+        d2[v] = dmin; yb2[v] = ybest; k2[v] = kb;
+    }
+    // representative candidate: the unit's station-3 entries with the nearest U2
+    // and V2 entries, evaluated as the chain evaluates a candidate: first fit,
+    // first box cut, left/right, the sign-free tests (second box cut, prop-tube Y,
+    // hodoscopes) and chi2
+    float chi = 1.0e6f, rep_y0 = 0.f;
+    int rep_pass = 0;
+    if (nl.x > 0 && nl.y > 0) {
+        YzItem it;
+        it.j = j; it.local = 0; it.iu3 = iu3; it.iv3 = iv3; it.iu2 = k2[0]; it.iv2 = k2[1];
+        YzSlots H;
+        Fit fr;
+        yz_candidate(E, S, &T, it, H, fr);
+        rep_y0 = fr.p0;
+        if (yz_box1(fr)) {
+            rep_pass = 1;
+            lr_slots_new<8>(T.x0, T.tx, fr.p0, fr.p1, T.ex0, T.etx, fr.e0, fr.e1, H.pres, H.det, H.pos, H.drift,
+                            H.sign, 150.f);
+            if (yz_cuts(E, &T, fr)) rep_pass = 2;
+            yz_chi2(E, &T, H, fr, chi, nullptr);
+        }
+    }
+    float dp[2];
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+        const int da = p == 0 ? 48 : 54, db = da - 1;
+        const float ypa = k3 * G(F_Z, da), ypb = k3 * G(F_Z, db);
+        float dmin = 100.f;
+        for (int i = 0; i < E.nred(da); ++i) dmin = fminf(dmin, fabsf(E.Lp->pos[E.raw(da, i)] - ypa));
+        for (int i = 0; i < E.nred(db); ++i) dmin = fminf(dmin, fabsf(E.Lp->pos[E.raw(db, i)] - ypb));
+        dp[p] = dmin;
+    }
     UnitFeatures f;
-    f.v[0] = T.x0 * 0.01f; f.v[1] = T.tx * 10.f; f.v[2] = (float)(T.st3 - 3);
-    f.v[3] = (float)(__popc(fu) - 1); f.v[4] = (float)(__popc(fv) - 1);
-    f.v[5] = ya * 0.02f; f.v[6] = yb * 0.02f; f.v[7] = yc * 0.02f; f.v[8] = yd * 0.02f;
-    f.v[9] = eya; f.v[10] = eyb; f.v[11] = eyc; f.v[12] = eyd;
-    f.v[13] = du * 0.1f; f.v[14] = fabsf(du) * 0.1f; f.v[15] = (ya - yb) * 0.1f;
+    f.v[0] = T.tx * 10.f;
+    f.v[1] = y3 * 0.02f;
+    f.v[2] = (yum - yvm) * 0.1f;
+    f.v[3] = fabsf(yum - yvm) * 0.1f;
+    f.v[4] = (ya - yb) * 0.1f;
+    f.v[5] = (yc - yd) * 0.1f;
+    f.v[6] = (float)(__popc(fu) - 1);
+    f.v[7] = (float)(__popc(fv) - 1);
+    f.v[8] = fminf(d2[0] * 0.1f, 10.f);
+    f.v[9] = fminf(d2[1] * 0.1f, 10.f);
+    f.v[10] = fminf(log1pf(fmaxf(chi, 0.f)) * 0.25f, 10.f);
+    f.v[11] = (float)rep_pass;
+    f.v[12] = fminf(fmaxf((yb2[0] - yb2[1]) * 0.1f, -10.f), 10.f);
+    f.v[13] = fminf(dp[0] * 0.1f, 10.f);
+    f.v[14] = fminf(dp[1] * 0.1f, 10.f);
+    f.v[15] = fminf(fabsf(rep_y0) * 0.02f, 10.f);
     return f;
 }
 
@@ -1410,10 +1502,12 @@ __device__ __attribute__((noinline)) UnitScore yz_score_units(Scr S, Prn R, int 
         const unsigned long long bb = b0 | (b1 << 32);
         if (bb + (unsigned long long)U <= (unsigned long long)R.dump_cap) res.dbase = (int64_t)bb;
     }
-    // score every unit; the kept ones, in order, with the start of their
-    // candidates in the kept enumeration
-    int nk = 0;
-    uint32_t run = 0;
+    // score every unit (scores held in the candidate-cache region, which pass 2
+    // fills only later), each tracklet's best score in LDS
+    const bool rel = R.prune && R.prune->margin < INFINITY;
+    float* sc = reinterpret_cast<float*>(S.ycache());
+    if (lane < YZ_CHUNK) L.umax[lane] = INT_MIN;
+    wsync();
     for (int u0 = 0; u0 < U; u0 += WAVE) {
         const int u = u0 + lane;
         const bool valid = u < U;
@@ -1427,16 +1521,13 @@ __device__ __attribute__((noinline)) UnitScore yz_score_units(Scr S, Prn R, int 
         UnitFeatures f;
 #pragma unroll
         for (int q = 0; q < PRUNE_NF; ++q) f.v[q] = 0.f;
-        int nin = 0;
-        if (valid) {
-            const int4 nl = *reinterpret_cast<const int4*>(L.nl[j]);
-            f = unit_features(S, tks[kb + j], j, iuv3, nl.z);
-            nin = nl.x * nl.y;
-        }
-        bool keep = valid;
+        if (valid) f = unit_features<MAXH>(S, tks[kb + j], j, iuv3, *reinterpret_cast<const int4*>(L.nl[j]));
         if (R.prune) {
             const float score = prune_score(R.prune, f);
-            keep = valid && score >= R.prune->thr;
+            if (valid) {
+                sc[u] = score;
+                if (rel) atomicMax(&L.umax[j], float_order(score));
+            }
         }
         if (res.dbase >= 0 && valid) {
             float* row = R.dump + (res.dbase + u) * OR_UNIT_ROW;
@@ -1446,6 +1537,33 @@ __device__ __attribute__((noinline)) UnitScore yz_score_units(Scr S, Prn R, int 
             row[3] = 0.f;
 #pragma unroll
             for (int q = 0; q < PRUNE_NF; ++q) row[4 + q] = f.v[q];
+        }
+    }
+    wsync();
+    // keep a unit when it reaches the threshold and lies within the margin of its
+    // tracklet's best; the kept ones, in order, with the start of their candidates
+    // in the kept enumeration
+    int nk = 0;
+    uint32_t run = 0;
+    for (int u0 = 0; u0 < U; u0 += WAVE) {
+        const int u = u0 + lane;
+        const bool valid = u < U;
+        int j = 0;
+#pragma unroll
+        for (int t = 1; t < YZ_CHUNK; ++t) j += u >= uo[t] ? 1 : 0;
+        int ujs = uo[0];
+#pragma unroll
+        for (int t = 1; t < YZ_CHUNK; ++t) ujs = j == t ? uo[t] : ujs;
+        const int iuv3 = u - ujs;
+        int nin = 0;
+        bool keep = valid;
+        if (valid) {
+            const int4 nl = *reinterpret_cast<const int4*>(L.nl[j]);
+            nin = nl.x * nl.y;
+            if (R.prune) {
+                const float score = sc[u];
+                keep = score >= R.prune->thr && (!rel || score >= float_unorder(L.umax[j]) - R.prune->margin);
+            }
         }
         const unsigned long long km = ballot(keep);
         const int c = keep ? nin : 0;
@@ -3735,6 +3853,7 @@ struct or_session {
     hipGraphExec_t graph_exec = nullptr;
     PrunerDev* d_pruner = nullptr;   // packed pruner weights (first or_session_set_pruner)
     bool prune_on = false;
+    float prune_margin = INFINITY;   // per-tracklet score margin (or_session_set_pruner_margin)
     float* dump = nullptr;           // unit rows (training aid)
     int64_t dump_cap = 0;
     unsigned long long* dump_count = nullptr;
@@ -3993,10 +4112,18 @@ or_status or_session_get_tuning(const or_session* s, or_tuning key, int64_t* val
     return OR_ERR_INVALID_ARG;
 }
 
-// The launch sequence of one call: keep-mask fill, class counters fill,
-// classification, one persistent launch per tile class.  No memset nodes, so
-// it can be captured into a graph.
-static or_status issue_chain(or_session* s, const or_hits* in, const or_result* out) {
+// Device buffers of one call's launch sequence: per-wave scratch (waves_max
+// blocks of the session's stride), the class event lists (NCLASS x
+// list_stride) and the class counters (2 x NCLASS).
+struct ChainBufs {
+    char* scratch;
+    int64_t waves_max;
+    int32_t* lists;
+    int64_t list_stride;
+    int32_t* counts;
+};
+
+static or_status issue_chain_with(or_session* s, const or_hits* in, const or_result* out, const ChainBufs& B) {
     Args a;
     a.det = in->det; a.elem = in->elem; a.tdc = in->tdc; a.drift = in->drift; a.flag = in->flag;
     a.off = in->event_offsets; a.n_events = in->n_events;
@@ -4005,7 +4132,7 @@ static or_status issue_chain(or_session* s, const or_hits* in, const or_result* 
     a.p = s->p;
     a.status = out->status; a.keep_mask = out->keep_mask; a.n_tracks = out->n_tracks; a.tracks = out->tracks;
     a.n_tracklets = out->n_tracklets; a.records = out->records; a.vertex = out->vertex; a.fit = out->fit;
-    a.scratch = s->d_scratch;
+    a.scratch = B.scratch;
     a.lay = s->lay;
     a.yz_list_cap = std::min(s->yz_list_cap, s->lay.list_cap);
     a.pruner = s->prune_on ? s->d_pruner : nullptr;
@@ -4018,21 +4145,29 @@ static or_status issue_chain(or_session* s, const or_hits* in, const or_result* 
         hipLaunchKernelGGL(zero_u32, dim3(1024), dim3(256), 0, s->stream, out->keep_mask, nw);
         OR_HIP(hipGetLastError());
     }
-    hipLaunchKernelGGL(zero_u32, dim3(1), dim3(64), 0, s->stream, reinterpret_cast<uint32_t*>(s->d_counts),
+    hipLaunchKernelGGL(zero_u32, dim3(1), dim3(64), 0, s->stream, reinterpret_cast<uint32_t*>(B.counts),
                        (int64_t)(2 * NCLASS));
     OR_HIP(hipGetLastError());
     hipLaunchKernelGGL(classify_events, dim3(std::min<int64_t>(1024, (in->n_events + 255) / 256)), dim3(256), 0,
-                       s->stream, in->event_offsets, in->n_events, s->nclass, s->d_lists, s->list_cap, s->d_counts);
+                       s->stream, in->event_offsets, in->n_events, s->nclass, B.lists, B.list_stride, B.counts);
     OR_HIP(hipGetLastError());
     for (int c = 0; c < s->nclass; ++c) {
-        a.ev_list = s->d_lists + c * s->list_cap;
-        a.ev_count = s->d_counts + c;
-        a.ev_head = s->d_counts + NCLASS + c;
+        a.ev_list = B.lists + c * B.list_stride;
+        a.ev_count = B.counts + c;
+        a.ev_head = B.counts + NCLASS + c;
         const int64_t g = std::max<int64_t>(1, (int64_t)s->grid_c[c] * s->grid_pct / 100);
-        const int grid = (int)std::min<int64_t>(g, in->n_events);
+        const int grid = (int)std::min<int64_t>(std::min<int64_t>(g, in->n_events), B.waves_max);
         OR_HIP(launch_class(c, a, grid, s->stream));
     }
     return OR_OK;
+}
+
+// The launch sequence of one call: keep-mask fill, class counters fill,
+// classification, one persistent launch per tile class.  No memset nodes, so
+// it can be captured into a graph.
+static or_status issue_chain(or_session* s, const or_hits* in, const or_result* out) {
+    const ChainBufs B = {s->d_scratch, s->grid, s->d_lists, s->list_cap, s->d_counts};
+    return issue_chain_with(s, in, out, B);
 }
 
 static or_status run_device(or_session* s, const or_hits* in, const or_result* out, bool timed) {
@@ -4180,12 +4315,24 @@ or_status or_session_set_pruner(or_session* s, const float* weights, int64_t n_w
     for (int64_t i = 0; i < n_weights; ++i)
         if (!std::isfinite(weights[i])) return OR_ERR_INVALID_ARG;
     PrunerDev h;
-    prune_pack(weights, threshold, &h);
+    prune_pack(weights, threshold, s->prune_margin, &h);
     OR_HIP(hipSetDevice(s->device));
     if (!s->d_pruner) OR_HIP(hipMalloc(&s->d_pruner, sizeof(PrunerDev)));
     OR_HIP(hipStreamSynchronize(s->stream));
     OR_HIP(hipMemcpy(s->d_pruner, &h, sizeof h, hipMemcpyHostToDevice));
     s->prune_on = true;
+    return OR_OK;
+}
+
+or_status or_session_set_pruner_margin(or_session* s, float margin) {
+    if (!s || std::isnan(margin) || margin < 0.f) return OR_ERR_INVALID_ARG;
+    s->prune_margin = margin;
+    if (s->d_pruner) {
+        OR_HIP(hipSetDevice(s->device));
+        OR_HIP(hipStreamSynchronize(s->stream));
+        OR_HIP(hipMemcpy(reinterpret_cast<char*>(s->d_pruner) + offsetof(PrunerDev, margin), &margin, sizeof margin,
+                         hipMemcpyHostToDevice));
+    }
     return OR_OK;
 }
 

@@ -27,7 +27,7 @@ typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 // Weights in per-lane A-operand order ([row tile][k step][lane]), biases and
-// the output layer in fp32, and the keep threshold.
+// the output layer in fp32, the keep threshold and the per-tracklet margin.
 struct PrunerDev {
     bf16x4 w1[2][2][64];
     bf16x4 w2[2][8][64];
@@ -35,6 +35,7 @@ struct PrunerDev {
     alignas(16) float b2[PRUNE_NH];
     alignas(16) float w3[PRUNE_NH];
     float b3, thr;
+    float margin;   // keep a unit only within margin of its tracklet's best score (INFINITY: off)
 };
 static_assert(offsetof(PrunerDev, b1) % 16 == 0 && offsetof(PrunerDev, b2) % 16 == 0 &&
                   offsetof(PrunerDev, w3) % 16 == 0, "bias rows are read as float4");
@@ -111,7 +112,7 @@ __device__ __attribute__((noinline)) float prune_score(const PrunerDev* Wp, Unit
 }
 
 // Host-side packing of the flat weight vector (layout above) into PrunerDev.
-inline void prune_pack(const float* w, float thr, PrunerDev* out) {
+inline void prune_pack(const float* w, float thr, float margin, PrunerDev* out) {
     const float* w1 = w;
     const float* b1 = w1 + PRUNE_NH * PRUNE_NF;
     const float* w2 = b1 + PRUNE_NH;
@@ -131,6 +132,7 @@ inline void prune_pack(const float* w, float thr, PrunerDev* out) {
     for (int i = 0; i < PRUNE_NH; ++i) { out->b1[i] = b1[i]; out->b2[i] = b2[i]; out->w3[i] = w3[i]; }
     out->b3 = b3;
     out->thr = thr;
+    out->margin = margin;
 }
 
 }  // namespace ordev

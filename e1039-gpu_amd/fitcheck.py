@@ -84,3 +84,64 @@ def error_report(ev, res, geo, max_tracks: int | None = None, cond_bound: float 
         fp32 = dev[:, 7] < 0.5
         out[q]["max_rel_fp32_path"] = float(err[fp32].max()) if fp32.any() else 0.0
     return out
+
+
+# ---- the chain's own 2-parameter XZ fit -------------------------------------
+# fit_2D_track_drift (trackanalyticalminimizer.cuh:78-119): points x at y = 0
+# from the wire end points (calculate_xz_fy with y0 = ty = 0,
+# reconstruction_helper.cuh:140-158) plus drift * sign, weights w = resolution
+# (the reference weights by sigma, not 1/sigma^2), normal equations
+# A = [sum w, sum w z; sum w z, sum w z^2], B = [sum w x, sum w x z],
+# parameters A^-1 B, errors sqrt(|A^-1_jj|).  Output tracks keep these
+# (x0, tx, err_x0, err_tx) from the XZ stage; their X hits lead the hit list.
+F_P1X, F_P1Y, F_P1Z, F_DELTAPX, F_DELTAPY, F_DELTAPZ, F_DP1X, F_DP1Y, F_DP1Z = range(16, 25)
+X_DETS = (15, 16, 21, 22, 27, 28)
+XZ_QUANTITIES = ["x0", "tx", "err_x0", "err_tx"]
+XZ_FLOORS = {"x0": 1.0, "tx": 1e-3, "err_x0": 1e-3, "err_tx": 1e-6}
+
+
+def xz_fit_fp64(det, elem, drift, sign, mask, planes):
+    """float64 solve of the reference's XZ fit per row (arrays (T, 4))."""
+    e1 = elem.astype(np.float64) - 1.0
+    p1x = planes[F_P1X][det] + planes[F_DP1X][det] * e1
+    p1y = planes[F_P1Y][det] + planes[F_DP1Y][det] * e1
+    p1z = planes[F_P1Z][det] + planes[F_DP1Z][det] * e1
+    dpx, dpy, dpz = planes[F_DELTAPX][det], planes[F_DELTAPY][det], planes[F_DELTAPZ][det]
+    x = p1x + (-p1y * dpx) / dpy + drift * sign
+    z = p1z + (-p1y * dpz) / dpy
+    w = np.where(mask, planes[F_RESOLUTION][det].astype(np.float64), 0.0)
+    a00, a01, a11 = w.sum(1), (w * z).sum(1), (w * z * z).sum(1)
+    b0, b1 = (w * x).sum(1), (w * x * z).sum(1)
+    det2 = a00 * a11 - a01 * a01
+    i00, i01, i11 = a11 / det2, -a01 / det2, a00 / det2
+    return np.stack([i00 * b0 + i01 * b1, i01 * b0 + i11 * b1, np.sqrt(np.abs(i00)), np.sqrt(np.abs(i11))], axis=1)
+
+
+def xz_error_report(ev, res, geo) -> dict:
+    """Per-quantity error of the output tracks' XZ fit (x0, tx, err_x0, err_tx)
+    against the float64 solve of the same problem."""
+    planes, _ = geo.export()
+    planes = planes.astype(np.float64)
+    nt = res.n_tracks.astype(np.int64)
+    valid = np.arange(res.tracks.shape[1])[None, :] < nt[:, None]
+    ev_of = np.broadcast_to(np.arange(len(nt), dtype=np.int64)[:, None], valid.shape)[valid]
+    tr = res.tracks[valid]
+    k = np.arange(4)[None, :]
+    gidx = ev.offsets[ev_of][:, None] + tr["hit_index"][:, :4].astype(np.int64)
+    det = ev.det[gidx].astype(np.int64)
+    mask = np.isin(det, X_DETS) & (k < tr["nhits"][:, None])
+    det = np.where(mask, det, 16)
+    elem = np.where(mask, ev.elem[gidx].astype(np.int64), 1)
+    drift = np.where(mask, np.abs(ev.drift[gidx].astype(np.float64)), 0.0)
+    sign = (((tr["sign_plus"][:, None] >> k) & 1).astype(np.float64) -
+            ((tr["sign_minus"][:, None] >> k) & 1).astype(np.float64))
+    ref = xz_fit_fp64(det, elem, drift, sign, mask, planes)
+    dev = np.stack([tr["x0"], tr["tx"], tr["err_x0"], tr["err_tx"]], axis=1).astype(np.float64)
+    out = {"tracks": int(len(dev)), "floors": XZ_FLOORS,
+           "fit": "fit_2D_track_drift, FP32 in the chain vs float64 (trackanalyticalminimizer.cuh:78-119)"}
+    for i, q in enumerate(XZ_QUANTITIES):
+        err = np.abs(dev[:, i] - ref[:, i]) / np.maximum(np.abs(ref[:, i]), XZ_FLOORS[q])
+        out[q] = {"max_rel": float(err.max()) if len(err) else 0.0,
+                  "p99_rel": float(np.quantile(err, 0.99)) if len(err) else 0.0,
+                  "median_rel": float(np.median(err)) if len(err) else 0.0}
+    return out

@@ -149,6 +149,8 @@ def load_library(path: str = LIB_PATH):
     lib.or_session_autotune.restype = i32
     lib.or_session_set_pruner.argtypes = [vp, C.POINTER(C.c_float), i64, C.c_float]
     lib.or_session_set_pruner.restype = i32
+    lib.or_session_set_pruner_margin.argtypes = [vp, C.c_float]
+    lib.or_session_set_pruner_margin.restype = i32
     lib.or_pruner_score.argtypes = [vp, vp, i64, vp]
     lib.or_pruner_score.restype = i32
     lib.or_session_set_unit_dump.argtypes = [vp, vp, i64, vp]
@@ -473,6 +475,10 @@ class Session:
         w = np.ascontiguousarray(weights, dtype=np.float32).ravel()
         _check(self.lib.or_session_set_pruner(self.handle, w.ctypes.data_as(C.POINTER(C.c_float)), w.size,
                                               float(threshold)))
+
+    def set_pruner_margin(self, margin: float = float("inf")):
+        """or_session_set_pruner_margin: keep units within margin of their tracklet's best score."""
+        _check(self.lib.or_session_set_pruner_margin(self.handle, float(margin)))
 
     def score_units(self, features):
         """or_pruner_score on a (n, 16) float32 device tensor; returns the (n,) scores."""

@@ -17,9 +17,12 @@ import os
 import numpy as np
 
 NF, NH = 16, 64
+THRESHOLD_OFF = -3.0e38   # absolute score threshold that keeps every unit (the margin decides)
 NW = NH * NF + NH + NH * NH + NH + NH + 1   # 5313 = OR_PRUNE_WEIGHTS
-FEATURE_NAMES = ["x0/100", "10 tx", "st3 is D3m", "u3 hits - 1", "v3 hits - 1", "y_u3a/50", "y_u3b/50", "y_v3a/50",
-                 "y_v3b/50", "ey_u3a", "ey_u3b", "ey_v3a", "ey_v3b", "(yu - yv)/10", "|yu - yv|/10", "(y_u3a - y_u3b)/10"]
+FEATURE_NAMES = ["10 tx", "y3/50", "(yU3 - yV3)/10", "|yU3 - yV3|/10", "(yU3a - yU3b)/10", "(yV3a - yV3b)/10",
+                 "U3 hits - 1", "V3 hits - 1", "U2 nearest distance/10", "V2 nearest distance/10",
+                 "representative log(1 + chi2)/4", "representative stage reached (0-2)", "(nearest U2 - nearest V2)/10",
+                 "P1 Y nearest distance/10", "P2 Y nearest distance/10", "representative |y0|/50"]
 
 
 def pack(w1, b1, w2, b2, w3, b3) -> np.ndarray:
@@ -54,18 +57,24 @@ def bf16_round(x):
     return u.view(torch.float32)
 
 
-def score_reference(flat: np.ndarray, feats):
+def score_reference(flat: np.ndarray, feats, chunk: int = 1 << 22):
     """torch fp32 restatement of the device scoring: bf16 weights, features and
-    layer-1 activations, fp32 sums and output layer.  feats: (n, 16) tensor."""
+    layer-1 activations, fp32 sums and output layer.  feats: (n, 16) tensor;
+    evaluated in chunks of rows (keeps every intermediate tensor far below 2^31
+    elements)."""
     import torch
     w1, b1, w2, b2, w3, b3 = (torch.as_tensor(np.asarray(a, np.float32)) if not isinstance(a, float) else a
                               for a in unpack(flat))
     dev = feats.device
     w1, b1, w2, b2, w3 = (t.to(dev) for t in (w1, b1, w2, b2, w3))
-    x = bf16_round(feats.float())
-    h1 = torch.relu(x @ bf16_round(w1).T + b1)
-    h2 = torch.relu(bf16_round(h1) @ bf16_round(w2).T + b2)
-    return h2 @ w3 + b3
+    w1r, w2r = bf16_round(w1).T, bf16_round(w2).T
+    out = []
+    for a in range(0, feats.shape[0], chunk):
+        x = bf16_round(feats[a:a + chunk].float())
+        h1 = torch.relu(x @ w1r + b1)
+        h2 = torch.relu(bf16_round(h1) @ w2r + b2)
+        out.append(h2 @ w3 + b3)
+    return torch.cat(out) if out else torch.zeros(0, device=dev)
 
 
 def save(path: str, flat: np.ndarray, threshold: float, meta: dict):
@@ -143,3 +152,91 @@ def train(feats: np.ndarray, labels: np.ndarray, epochs: int = 12, seed: int = 7
         b1 = (l1.bias.cpu() - (l1.weight.cpu() * (mu / sd)).sum(1)).numpy()
         return pack(w1, b1, l2.weight.cpu().numpy(), l2.bias.cpu().numpy(), l3.weight.cpu().numpy().ravel(),
                     float(l3.bias.cpu()[0]))
+
+
+def train_rank(feats, group, winner, epochs: int = 12, seed: int = 7, device: str = "cuda", batch_rows: int = 1 << 22,
+               tune_epochs: int = 4, tune_temp: float = 0.3, log=print) -> np.ndarray:
+    """Fits the perceptron as a ranking of the units of each tracklet: a softmax
+    over the units of a group (a tracklet) with the group's YZ winner as the
+    target (listwise cross-entropy), then tune_epochs more at softmax
+    temperature tune_temp, where the loss tends to the winner's distance below
+    the best unit of its group (what the keep margin has to cover).  feats (n,
+    16), group (n,) non-decreasing group ids (rows of a group contiguous),
+    winner (n,) bool, at most one per group; groups without a winner do not
+    contribute.  Returns flat weights for raw features (the input
+    standardisation folded into layer 1)."""
+    import torch
+    torch.manual_seed(seed)
+    x = torch.as_tensor(feats, device=device)
+    g = torch.as_tensor(group, dtype=torch.int64, device=device)
+    w = torch.as_tensor(winner, dtype=torch.bool, device=device)
+    n = x.shape[0]
+    mu = torch.zeros(NF, dtype=torch.float64, device=device)
+    sq = torch.zeros(NF, dtype=torch.float64, device=device)
+    for a in range(0, n, batch_rows):
+        xb = x[a:a + batch_rows].double()
+        mu += xb.sum(0)
+        sq += (xb * xb).sum(0)
+    mu = mu / n
+    sd = (torch.sqrt(torch.clamp(sq / n - mu * mu, min=0)) + 1e-3).float()
+    mu = mu.float()
+    # batches of whole groups
+    starts = torch.nonzero(torch.cat([torch.ones(1, dtype=torch.bool, device=device), g[1:] != g[:-1]])).squeeze(1)
+    bounds = torch.cat([starts, torch.tensor([n], device=device)]).cpu().numpy()
+    cuts = [0]
+    for b in bounds[1:]:
+        if b - cuts[-1] >= batch_rows:
+            cuts.append(int(b))
+    if cuts[-1] != n:
+        cuts.append(n)
+    net = torch.nn.Sequential(torch.nn.Linear(NF, NH), torch.nn.ReLU(), torch.nn.Linear(NH, NH), torch.nn.ReLU(),
+                              torch.nn.Linear(NH, 1)).to(device)
+    order = np.arange(len(cuts) - 1)
+    rng = np.random.default_rng(seed)
+
+    def run(nep, temp, lr):
+        opt = torch.optim.Adam(net.parameters(), lr=lr)
+        sched = torch.optim.lr_scheduler.OneCycleLR(opt, max_lr=lr, total_steps=max(nep * (len(cuts) - 1), 1))
+        for ep in range(nep):
+            rng.shuffle(order)
+            tot, cnt = 0.0, 0
+            for k in order:
+                a, b = cuts[k], cuts[k + 1]
+                wb = w[a:b]
+                gb = g[a:b] - g[a]
+                ng = int(gb[-1]) + 1
+                s = net((x[a:b].float() - mu) / sd).squeeze(1) / temp
+                top = s.max().detach()            # one shift for the batch (no per-group reduction)
+                z = torch.zeros(ng, device=device).index_add(0, gb, torch.exp(s - top))
+                lse = torch.log(z) + top
+                loss = temp * (lse[gb[wb]] - s[wb]).mean()
+                opt.zero_grad()
+                loss.backward()
+                opt.step()
+                sched.step()
+                tot += float(loss.detach()) * int(wb.sum())
+                cnt += int(wb.sum())
+            log(f"epoch {ep} (temperature {temp}): loss {tot / max(cnt, 1):.4f}")
+
+    run(epochs, 1.0, 3e-3)
+    if tune_epochs > 0:
+        run(tune_epochs, tune_temp, 1e-3)
+    l1, l2, l3 = net[0], net[2], net[4]
+    with torch.no_grad():
+        w1 = (l1.weight.cpu() / sd.cpu()).numpy()
+        b1 = (l1.bias.cpu() - (l1.weight.cpu() * (mu.cpu() / sd.cpu())).sum(1)).numpy()
+        return pack(w1, b1, l2.weight.cpu().numpy(), l2.bias.cpu().numpy(), l3.weight.cpu().numpy().ravel(),
+                    float(l3.bias.cpu()[0]))
+
+
+def margin_gaps(scores, group, winner):
+    """Per group with a winner: best score of the group minus the winner's score
+    (>= 0; the margin a unit must be within for the winner to be kept), and the
+    best score of every group.  Computed on the host."""
+    import torch
+    s = torch.as_tensor(scores).float().cpu()
+    g = torch.as_tensor(group, dtype=torch.int64).cpu()
+    w = torch.as_tensor(winner, dtype=torch.bool).cpu()
+    ng = int(g.max()) + 1 if g.numel() else 0
+    m = torch.full((ng,), -3.0e38).scatter_reduce(0, g, s, "amax", include_self=True)
+    return (m[g[w]] - s[w]), m

@@ -1,20 +1,26 @@
 """Trains the YZ unit pruner on the plain chain's own decisions (needs the GPU).
 
-    python e1039-gpu_amd/train_pruner.py [--train 100000] [--out e1039-gpu_amd/data/pruner_u3v3.npz]
+    python e1039-gpu_amd/train_pruner.py [--train 20000] [--out e1039-gpu_amd/data/pruner_u3v3.npz]
 
-1. Runs the plain chain on seeded synthetic events of each workload (hits per
-   event Poisson(300) capped at 1000, and 8 % noise occupancy per channel)
-   with the unit dump on (or_session_set_unit_dump): one row of 16 features
-   per YZ unit, with a flag saying whether the tracklet's YZ winner lies in
-   that unit.
-2. Labels a unit positive when it holds the winner of a tracklet that ends as
-   an output track (or_track.tracklet_id).
-3. Fits one perceptron on the rows of both workloads (pruner.train), then picks
-   the threshold: candidates are validation scores at several positive-unit
-   recalls; each is run through the pruned chain on separate event sets of
-   both workloads, and the largest one whose track recall (tracks output
-   identically, pruner.recall) reaches --target on every workload is kept.
-4. Saves weights, threshold and the measured figures.
+Workload: hits per event Poisson(1200) capped at 4000 with the high-occupancy
+caps (orchain.HIGH_OCCUPANCY_CAPS), the learned-pruning configuration.
+
+1. Runs the plain chain on seeded synthetic events with the unit dump on
+   (or_session_set_unit_dump): one row of 16 features per YZ unit, with a
+   flag saying whether the tracklet's YZ winner lies in that unit.  Rows of a
+   tracklet are contiguous.
+2. Fits the perceptron as a ranking of each tracklet's units with its YZ
+   winner as the target (pruner.train_rank).  Every tracklet's winner counts,
+   not only those of tracklets that end as tracks: back-partial cleaning
+   compares each surviving tracklet with the others, so a changed winner of
+   any tracklet can change which tracks come out.
+3. The device keeps a unit when its score is within a margin of the best
+   score of its tracklet (or_session_set_pruner_margin).  Candidate margins
+   are validation quantiles of (best score - winner score); each is run
+   through the pruned chain on a separate event set, and the smallest whose
+   track recall (tracks output identically, pruner.recall) reaches --target
+   is kept.
+4. Saves weights, threshold (off), margin and the measured figures.
 """
 import argparse
 import json
@@ -25,11 +31,7 @@ import time
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
-# name -> generator settings (include/e1039_or_gen.h)
-WORKLOADS = {
-    "poisson300": dict(mult_mode=1, mult_mean=300.0, mult_cap=1000),
-    "occ8": dict(mult_mode=0, noise_occupancy=0.08),
-}
+WORKLOAD = dict(mult_mode=1, mult_mean=1200.0, mult_cap=4000)
 
 
 def log(msg):
@@ -38,11 +40,12 @@ def log(msg):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--train", type=int, default=100_000, help="training events per workload")
-    ap.add_argument("--valid", type=int, default=30_000, help="validation events per workload")
-    ap.add_argument("--test", type=int, default=100_000, help="events per workload for the track-recall check")
-    ap.add_argument("--epochs", type=int, default=20)
-    ap.add_argument("--target", type=float, default=0.995, help="track recall the threshold must reach")
+    ap.add_argument("--train", type=int, default=20_000, help="training events")
+    ap.add_argument("--valid", type=int, default=2_000, help="validation events (margin candidates)")
+    ap.add_argument("--test", type=int, default=8_000, help="events for the track-recall check")
+    ap.add_argument("--epochs", type=int, default=16)
+    ap.add_argument("--shard", type=int, default=1000, help="events per unit dump")
+    ap.add_argument("--target", type=float, default=0.996, help="track recall the margin must reach")
     ap.add_argument("--out", default=os.path.join(HERE, "data", "pruner_u3v3.npz"))
     args = ap.parse_args()
 
@@ -53,114 +56,97 @@ def main():
     import pruner
 
     geo = oc.Geometry.load()
-    params = oc.default_params(max_hits_per_event=1024)
+    params = oc.high_occupancy_params()
     sess = oc.Session(geo, params, device=0)
     dev = torch.device("cuda", 0)
 
-    def events(name, n, seed):
-        return oc.generate(geo, n, seed=seed, **WORKLOADS[name])
+    def dump(ev):
+        """plain chain with unit rows, in shards; returns (features, group id, winner) on the device"""
+        feats, groups, wins = [], [], []
+        gbase = 0
+        for s0 in range(0, ev.n_events, args.shard):
+            sub = ev.subset(s0, min(ev.n_events, s0 + args.shard))
+            cap = sub.n_events * 200_000
+            rows = torch.empty((cap, oc.UNIT_ROW), dtype=torch.float32, device=dev)
+            count = torch.zeros(1, dtype=torch.int64, device=dev)
+            sess.set_pruner(None)
+            sess.set_unit_dump(rows, count)
+            t = sess.upload(sub)
+            o = sess.alloc_outputs(sub.n_events, sub.n_hits, tracklets=True)
+            sess.run_device(t, o)
+            torch.cuda.synchronize()
+            sess.set_unit_dump(None)
+            n = int(count.item())
+            if n > cap:
+                raise RuntimeError(f"unit rows: {n} wanted, {cap} held")
+            r = rows[:n]
+            r = r[r[:, 3] >= 0]          # chunks that went unit by unit carry no winner flags
+            key = r[:, 0].long() * (1 << 20) + r[:, 1].long()
+            chg = torch.cat([torch.ones(1, dtype=torch.bool, device=dev), key[1:] != key[:-1]])
+            g = torch.cumsum(chg.long(), 0) - 1 + gbase
+            gbase = int(g[-1]) + 1 if g.numel() else gbase
+            feats.append(r[:, 4:4 + pruner.NF].clone())
+            groups.append(g)
+            wins.append(r[:, 3] > 0.5)
+            del rows, r, o, t
+        return torch.cat(feats), torch.cat(groups), torch.cat(wins)
 
-    def dump_rows(ev):
-        """plain chain with unit rows; returns (rows, result)"""
-        cap = ev.n_events * 1024
-        rows = torch.empty((cap, oc.UNIT_ROW), dtype=torch.float32, device=dev)
-        count = torch.zeros(1, dtype=torch.int64, device=dev)
-        sess.set_pruner(None)
-        sess.set_unit_dump(rows, count)
+    def run(ev, flat=None, margin=float("inf")):
+        sess.set_pruner_margin(margin)
+        sess.set_pruner(None) if flat is None else sess.set_pruner(flat, pruner.THRESHOLD_OFF)
         t = sess.upload(ev)
         o = sess.alloc_outputs(ev.n_events, ev.n_hits, tracklets=True)
         sess.run_device(t, o)
         torch.cuda.synchronize()
-        sess.set_unit_dump(None)
-        n = int(count.item())
-        if n > cap:
-            raise RuntimeError(f"unit rows: {n} wanted, {cap} held")
         res = oc.Session.fetch(o, ev.n_events, params.max_tracks_out, params.max_tracklets_per_event)
-        out = rows[:n].cpu()
-        del rows
-        return out, res
-
-    def label(rows, res):
-        r = rows.numpy()
-        good = r[:, 3] >= 0
-        r = r[good]
-        # (event, tracklet_id) of every output track, as one int64 key
-        nt = res.n_tracks.astype(np.int64)
-        slot = np.arange(res.tracks.shape[1])[None, :] < nt[:, None]
-        ev_of = np.broadcast_to(np.arange(len(nt))[:, None], slot.shape)[slot]
-        final = ev_of.astype(np.int64) * 4096 + res.tracks["tracklet_id"][slot].astype(np.int64)
-        key = r[:, 0].astype(np.int64) * 4096 + r[:, 1].astype(np.int64)
-        y = (r[:, 3] > 0.5) & np.isin(key, final)
-        return r[:, 4:4 + pruner.NF].astype(np.float32), y
-
-    t0 = time.time()
-    xs, ys = [], []
-    for i, name in enumerate(WORKLOADS):
-        rows, res = dump_rows(events(name, args.train, 0x7A11 + 16 * i))
-        x, y = label(rows, res)
-        log(f"{name}: training rows {len(x)} ({len(x) / args.train:.1f}/event), positive {y.mean():.4f}")
-        xs.append(x)
-        ys.append(y)
-    xtr, ytr = np.concatenate(xs), np.concatenate(ys)
-    del xs, ys
-    flat = pruner.train(xtr, ytr, epochs=args.epochs, device="cuda", log=log)
-    log(f"trained on {len(xtr)} rows in {time.time() - t0:.1f}s")
-
-    pos_scores, all_scores = [], []
-    for i, name in enumerate(WORKLOADS):
-        rows, res = dump_rows(events(name, args.valid, 0x7A12 + 16 * i))
-        xva, yva = label(rows, res)
-        s_ref = pruner.score_reference(flat, torch.as_tensor(xva, device=dev)).cpu().numpy()
-        pos_scores.append(s_ref[yva])
-        all_scores.append(s_ref)
-    pos = np.concatenate(pos_scores)
-    unit_recalls = [0.995, 0.998, 0.999, 0.9995, 0.9999, 1.0]
-    cands = sorted({float(np.quantile(pos, 1 - r)) if r < 1 else float(pos.min()) for r in unit_recalls},
-                   reverse=True)
-    cands.append(float(pos.min()) - 1.0)
-
-    tests = {}
-    for i, name in enumerate(WORKLOADS):
-        te = events(name, args.test, 0x7A13 + 16 * i)
-        t = sess.upload(te)
-        o = sess.alloc_outputs(te.n_events, te.n_hits, tracklets=True)
-        sess.set_pruner(None)
         sess.run_device(t, o)
         torch.cuda.synchronize()
-        plain = oc.Session.fetch(o, te.n_events, params.max_tracks_out, params.max_tracklets_per_event)
-        sess.run_device(t, o)
-        tests[name] = (te, t, o, plain, sess.last_kernel_ms())
-    chosen = None
-    trials = []
-    for thr in cands:
-        sess.set_pruner(flat, thr)
-        entry = {"threshold": thr, "workloads": {}}
-        ok = True
-        for i, name in enumerate(WORKLOADS):
-            te, t, o, plain, plain_ms = tests[name]
-            sess.run_device(t, o)
-            torch.cuda.synchronize()
-            pr = oc.Session.fetch(o, te.n_events, params.max_tracks_out, params.max_tracklets_per_event)
-            sess.run_device(t, o)
-            ms = sess.last_kernel_ms()
-            rec, found, total = pruner.recall(plain, pr)
-            kept = float((all_scores[i] >= thr).mean())
-            entry["workloads"][name] = {"track_recall": rec, "tracks": total, "kept_units": kept,
-                                        "plain_ms": plain_ms, "pruned_ms": ms, "speedup": plain_ms / ms}
-            log(f"threshold {thr:.4f} {name}: track recall {rec:.5f} ({found}/{total}), kept units {kept:.3f}, "
-                f"chain {ms:.2f} ms vs plain {plain_ms:.2f} ms ({plain_ms / ms:.2f}x)")
-            ok = ok and rec >= args.target
+        return res, sess.last_kernel_ms()
+
+    t0 = time.time()
+    x, g, w = dump(oc.generate(geo, args.train, seed=0x7A31, **WORKLOAD))
+    log(f"training: {args.train} events, {x.shape[0]} units ({x.shape[0] / args.train:.0f}/event), "
+        f"{int(g[-1]) + 1} tracklets, {int(w.sum())} winners; dumped in {time.time() - t0:.1f}s")
+    t0 = time.time()
+    flat = pruner.train_rank(x, g, w, epochs=args.epochs, log=log)
+    log(f"trained in {time.time() - t0:.1f}s")
+    del x, g, w
+    torch.cuda.empty_cache()
+
+    xv, gv, wv = dump(oc.generate(geo, args.valid, seed=0x7A32, **WORKLOAD))
+    sv = pruner.score_reference(flat, xv).reshape(-1)
+    gaps, gmax = pruner.margin_gaps(sv, gv, wv)
+    gcpu = gv.cpu()
+    scpu = sv.float().cpu()
+    qs = [0.999, 0.9992, 0.9994, 0.9996, 0.9998, 0.9999, 1.0]
+    gs = np.sort(gaps.numpy())
+    cands = sorted({float(gs[min(len(gs) - 1, int(q * len(gs)))]) * 1.02 + 1e-3 for q in qs})
+    below = (gmax[gcpu] - scpu).numpy()
+    log("validation: " + ", ".join(f"margin {mg:.3f}: kept units {float((below <= mg).mean()):.4f}, "
+                                   f"winners kept {float((gs <= mg).mean()):.5f}" for mg in cands))
+    del xv, gv, wv, sv
+
+    te = oc.generate(geo, args.test, seed=0x7A33, **WORKLOAD)
+    plain, plain_ms = run(te)
+    chosen, trials = None, []
+    for mg in cands:
+        pr, ms = run(te, flat, mg)
+        rec, found, total = pruner.recall(plain, pr)
+        entry = {"margin": mg, "track_recall": rec, "tracks": total, "tracks_identical": found,
+                 "plain_ms": plain_ms, "pruned_ms": ms, "speedup": plain_ms / ms}
         trials.append(entry)
-        if ok:
+        log(f"margin {mg:.3f}: track recall {rec:.5f} ({found}/{total}), chain {ms:.1f} ms vs plain "
+            f"{plain_ms:.1f} ms ({plain_ms / ms:.2f}x)")
+        if chosen is None and rec >= args.target:
             chosen = entry
-            break
     sess.set_pruner(None)
     if chosen is None:
-        raise RuntimeError("no threshold reaches the target recall")
+        raise RuntimeError("no margin reaches the target recall")
     meta = {"features": pruner.FEATURE_NAMES, "train_events": args.train, "valid_events": args.valid,
-            "test_events": args.test, "workloads": WORKLOADS, "seeds": "0x7A11/12/13 + 16 * workload index",
-            "positive_fraction": float(ytr.mean()), "target": args.target, "chosen": chosen, "trials": trials}
-    pruner.save(args.out, flat, chosen["threshold"], meta)
+            "test_events": args.test, "workload": WORKLOAD, "caps": oc.HIGH_OCCUPANCY_CAPS,
+            "seeds": "train 0x7A31, valid 0x7A32, test 0x7A33", "loss": "listwise (softmax over a tracklet's units)",
+            "margin": chosen["margin"], "target": args.target, "chosen": chosen, "trials": trials}
+    pruner.save(args.out, flat, pruner.THRESHOLD_OFF, meta)
     log(json.dumps({"saved": args.out, **chosen}))
 
 

@@ -345,6 +345,13 @@ or_status or_dimuons(or_session* s, const or_hits* in, const or_result* chain, c
 #define OR_PRUNE_WEIGHTS 5313
 or_status or_session_set_pruner(or_session* s, const float* weights, int64_t n_weights, float threshold);
 
+/* Relative pruning: a unit is kept only when its score is also within margin
+ * of the best score among the units of its tracklet (the units compete for
+ * the tracklet's one YZ winner).  INFINITY (the default) turns it off; the
+ * margin is kept across or_session_set_pruner calls.  Waits for the session
+ * stream.  No reference counterpart. */
+or_status or_session_set_pruner_margin(or_session* s, float margin);
+
 /* Scores n rows of 16 unit features with the session's pruner (device
  * pointers; asynchronous on the session stream). */
 or_status or_pruner_score(or_session* s, const float* features, int64_t n, float* scores);
@@ -352,13 +359,21 @@ or_status or_pruner_score(or_session* s, const float* features, int64_t n, float
 /* Training aid: during device-resident calls, one row of OR_UNIT_ROW floats
  * per scored unit is appended to rows (device memory, cap rows): event index,
  * tracklet_id, u3.v3 index, winner flag (1: the tracklet's YZ winner is in this
- * unit, 0: not, -1: unknown), then the 16 features: x0/100, 10 tx, station-3
- * is D3m, u3 hits - 1, v3 hits - 1, y/50 of the first and last accepted u3 hit
- * and of the v3 hits, their four y errors, (yu - yv)/10 and |yu - yv|/10 of the
- * entry means, (ya - yb)/10 of the u3 hits.  *count (device uint64, zeroed by
- * the caller) counts the rows wanted; rows beyond cap are not written.  Chunks
- * with more than 4096 units are neither pruned nor dumped.  rows == NULL turns
- * it off. */
+ * unit, 0: not, -1: unknown), then the 16 features.  A unit's station-3 y is
+ * the mean of the y of its accepted U3 and V3 hits (sign 0); the line through
+ * it and the beam axis at z = 0 predicts later hits (distances in 10 cm):
+ * 10 tx; y3/50; (yU3 - yV3)/10 and |yU3 - yV3|/10; (yU3a - yU3b)/10 and
+ * (yV3a - yV3b)/10; U3 and V3 hits - 1; the distance of the nearest U2 and of
+ * the nearest V2 list entry to the prediction; for the representative
+ * candidate (the unit with those nearest U2 and V2 entries, evaluated as the
+ * chain evaluates every candidate) log(1 + chi2)/4 and the stage it reaches
+ * (0: fails the first box cut, 1: a sign-free test, 2: passes); nearest U2 y
+ * minus nearest V2 y; the distance of the nearest P1 Y and P2 Y prop-tube hit
+ * to the prediction; the representative's |y0|/50.  Each is clamped at 10.
+ * Rows of one tracklet are contiguous.  *count (device uint64, zeroed by the
+ * caller) counts the rows wanted; rows beyond cap are not written.  Chunks
+ * whose units overflow the session's unit list are neither pruned nor
+ * dumped.  rows == NULL turns it off. */
 #define OR_UNIT_ROW 20
 or_status or_session_set_unit_dump(or_session* s, float* rows, int64_t cap, unsigned long long* count);
 

@@ -1443,7 +1443,7 @@ int orc_reconstruct(const orc_geometry* G, const or_params* P, const or_hits* in
 #pragma omp parallel
     {
         orc::Ctx* C = new orc::Ctx;
-#pragma omp for schedule(dynamic, 64)
+#pragma omp for schedule(dynamic, 1)
         for (int64_t e = 0; e < n; ++e) orc::run_event(G->g, *P, *in, e, *out, kb, *C);
         delete C;
     }

@@ -35,7 +35,7 @@ def test_library_exports_every_declared_function(oc):
 
 def test_status_strings(oc):
     lib = oc.load_library()
-    for st in range(8):
+    for st in range(9):
         assert lib.or_status_string(st)
     assert b"unknown" in lib.or_status_string(99)
 

@@ -92,20 +92,41 @@ def test_keep_all_units_is_the_plain_chain(oc, orc, geo, ogeo, psess):
             assert key in set(wkeys.tolist()) or key in unknown
 
 
-@pytest.mark.parametrize("gen", [dict(mult_mode=1, mult_mean=300.0, mult_cap=1000),
-                                 dict(mult_mode=0, noise_occupancy=0.08)], ids=["poisson300", "occ8"])
-def test_shipped_pruner_recall(oc, geo, psess, gen):
+def test_margin_keeping_everything_is_the_plain_chain(oc, orc, geo, ogeo):
+    # relative pruning with a margin wider than any score spread keeps every unit
+    import pruner
+    sess = oc.Session(geo, oc.high_occupancy_params(), device=0)
+    ev = oc.generate(geo, 60, seed=23, mult_mode=1, mult_mean=1200.0, mult_cap=4000)
+    sess.set_pruner_margin(1.0e30)
+    sess.set_pruner(random_weights(4), pruner.THRESHOLD_OFF)
+    try:
+        got = sess.reconstruct(ev, records=True)
+    finally:
+        sess.set_pruner(None)
+        sess.set_pruner_margin()
+    ref = orc.reconstruct(ogeo, sess.params, ev, records=True)
+    same_outputs(got, ref)
+    sess.close()
+
+
+def test_shipped_pruner_recall(oc, geo):
+    # the learned-pruning configuration: hits Poisson(1200), high-occupancy caps,
+    # events of a seed the training never saw
     import pruner
     if not os.path.exists(oc.DEFAULT_PRUNER):
         pytest.skip("no shipped pruner weights")
     flat, thr, meta = pruner.load()
-    ev = oc.generate(geo, 20000, seed=0x5EED, **gen)
-    plain = psess.reconstruct(ev)
-    psess.set_pruner(flat, thr)
+    sess = oc.Session(geo, oc.high_occupancy_params(), device=0)
+    ev = oc.generate(geo, 6000, seed=0x5EED, mult_mode=1, mult_mean=1200.0, mult_cap=4000)
+    plain = sess.reconstruct(ev)
+    sess.set_pruner_margin(meta["margin"])
+    sess.set_pruner(flat, thr)
     try:
-        pr = psess.reconstruct(ev)
+        pr = sess.reconstruct(ev)
     finally:
-        psess.set_pruner(None)
+        sess.set_pruner(None)
+        sess.set_pruner_margin()
     rec, found, total = pruner.recall(plain, pr)
-    assert total > 10000
+    assert total > 50000
     assert rec >= 0.995, f"track recall {rec:.5f} ({found}/{total})"
+    sess.close()

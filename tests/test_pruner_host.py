@@ -60,8 +60,9 @@ def test_shipped_weights_file(oc, pruner):
         pytest.skip("no shipped pruner weights")
     flat, thr, meta = pruner.load()
     assert flat.size == pruner.NW and np.isfinite(flat).all() and np.isfinite(thr)
-    assert all(w["track_recall"] >= 0.995 for w in meta["chosen"]["workloads"].values())
-    assert len(meta["features"]) == pruner.NF
+    assert meta["chosen"]["track_recall"] >= 0.995 and meta["chosen"]["tracks"] > 50000
+    assert 0.0 < meta["margin"] < 1.0e6
+    assert meta["features"] == pruner.FEATURE_NAMES and len(meta["features"]) == pruner.NF
 
 
 def test_abi_argument_checks(oc):
