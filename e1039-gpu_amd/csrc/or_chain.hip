@@ -3913,7 +3913,18 @@ static ScrLayout make_layout(const or_params& p, int list_cap) {
 // The plane table and the final-fit constants sit in constant memory, one copy
 // per device: a session may only be created on a device whose live sessions use
 // the same geometry (the tables are uploaded when no session is live).
+// Scratch of or_reconstruct_noalloc: one fixed array per device, holding the
+// class counters, the class event lists (OR_NOALLOC_MAX_EVENTS per class) and
+// as many waves' scratch blocks as fit after them.  Host code uses the device
+// address looked up once per device at session creation (g_noalloc_ptr), never
+// the array's name.
+constexpr int64_t NOALLOC_BYTES = (int64_t)OR_NOALLOC_SCRATCH_MB << 20;
+constexpr int64_t NOALLOC_LISTS = 256;
+constexpr int64_t NOALLOC_WAVES = ((NOALLOC_LISTS + (int64_t)NCLASS_MAX * OR_NOALLOC_MAX_EVENTS * 4 + 4095) / 4096) * 4096;
+static __device__ __align__(256) char scratch[NOALLOC_BYTES];
+
 namespace {
+char* g_noalloc_ptr[64];   // device address of scratch per device (set under g_geo_mu)
 struct DevGeo {
     bool set = false;
     int users = 0;
@@ -4030,6 +4041,15 @@ or_status or_session_create(const or_geometry* g, const or_params* p, int device
     }
     ++dg.users;
     s->geo_registered = true;
+    if (!g_noalloc_ptr[device & 63]) {
+        void* ptr = nullptr;
+        const hipError_t se = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (se != hipSuccess || !ptr) {
+            std::fprintf(stderr, "libe1039or: hipGetSymbolAddress(scratch) failed: %s\n", hipGetErrorString(se));
+            return OR_ERR_DEVICE;
+        }
+        g_noalloc_ptr[device & 63] = static_cast<char*>(ptr);
+    }
     geo_lock.unlock();
     OR_HIP(hipMalloc(&s->d_wire, sizeof g->wire));
     OR_HIP(hipMemcpy(s->d_wire, g->wire, sizeof g->wire, hipMemcpyHostToDevice));
@@ -4302,6 +4322,42 @@ or_status or_reconstruct(or_session* s, const or_hits* in, or_result* out) {
     if (vtx_bytes) OR_HIP(hipMemcpyAsync(out->vertex, dout.vertex, vtx_bytes, hipMemcpyDeviceToHost, st));
     if (fit_bytes) OR_HIP(hipMemcpyAsync(out->fit, dout.fit, fit_bytes, hipMemcpyDeviceToHost, st));
     OR_HIP(hipStreamSynchronize(st));
+    return OR_OK;
+}
+
+or_status or_reconstruct_noalloc(or_session* s, const or_hits* in, or_result* out) {
+    if (!s || !in || !out || in->n_events < 0 || in->n_hits < 0) return OR_ERR_INVALID_ARG;
+    if (!in->on_device || !out->on_device) return OR_ERR_INVALID_ARG;
+    if (!in->det || !in->elem || !in->tdc || !in->drift || !in->event_offsets) return OR_ERR_INVALID_ARG;
+    if (!out->status || !out->keep_mask || !out->n_tracks || (!out->tracks && s->p.max_tracks_out > 0))
+        return OR_ERR_INVALID_ARG;
+    if (in->n_events > OR_NOALLOC_MAX_EVENTS) return OR_ERR_CAPACITY;
+    char* base = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_geo_mu);
+        base = g_noalloc_ptr[s->device & 63];
+    }
+    if (!base) return OR_ERR_DEVICE;
+    const int64_t waves = (NOALLOC_BYTES - NOALLOC_WAVES) / s->lay.stride;
+    if (waves < 1) return OR_ERR_CAPACITY;   // the session's per-wave scratch does not fit
+    OR_HIP(hipSetDevice(s->device));
+    if (in->n_events == 0) {
+        const int64_t nw = (in->n_hits + 31) / 32;
+        if (nw > 0) {
+            hipLaunchKernelGGL(zero_u32, dim3(1024), dim3(256), 0, s->stream, out->keep_mask, nw);
+            OR_HIP(hipGetLastError());
+        }
+        s->last_ms = 0.f;
+        s->timing_pending = false;
+        return OR_OK;
+    }
+    const ChainBufs B = {base + NOALLOC_WAVES, waves, reinterpret_cast<int32_t*>(base + NOALLOC_LISTS),
+                         (int64_t)OR_NOALLOC_MAX_EVENTS, reinterpret_cast<int32_t*>(base)};
+    OR_HIP(hipEventRecord(s->ev0, s->stream));
+    const or_status st = issue_chain_with(s, in, out, B);
+    if (st != OR_OK) return st;
+    OR_HIP(hipEventRecord(s->ev1, s->stream));
+    s->timing_pending = true;
     return OR_OK;
 }
 

@@ -47,6 +47,7 @@ EV_REJECT_MASK = 0x7FF
 TUNE_YZ_LIST_CAP = 1
 TUNE_GRID_PERCENT = 2
 TUNE_GRAPH = 3
+NOALLOC_MAX_EVENTS = 262144     # OR_NOALLOC_MAX_EVENTS
 PRUNE_WEIGHTS = 5313       # OR_PRUNE_WEIGHTS: w1[64][16], b1[64], w2[64][64], b2[64], w3[64], b3
 PRUNE_FEATURES = 16
 UNIT_ROW = 20              # OR_UNIT_ROW: event, tracklet_id, unit, winner flag, 16 features
@@ -135,6 +136,8 @@ def load_library(path: str = LIB_PATH):
     lib.or_session_create.restype = i32
     lib.or_reconstruct.argtypes = [vp, C.POINTER(OrHits), C.POINTER(OrResult)]
     lib.or_reconstruct.restype = i32
+    lib.or_reconstruct_noalloc.argtypes = [vp, C.POINTER(OrHits), C.POINTER(OrResult)]
+    lib.or_reconstruct_noalloc.restype = i32
     lib.or_session_sync.argtypes = [vp]
     lib.or_session_sync.restype = i32
     lib.or_session_set_stream.argtypes = [vp, vp]
@@ -551,6 +554,11 @@ class Session:
         """Device call on torch's current stream: inputs and outputs already in HBM (asynchronous)."""
         hits, res = self._device_args(t, o)
         _check(self.lib.or_reconstruct(self.handle, C.byref(hits), C.byref(res)))
+
+    def run_device_noalloc(self, t, o):
+        """or_reconstruct_noalloc: the device call without any device allocation (fixed per-device scratch)."""
+        hits, res = self._device_args(t, o)
+        _check(self.lib.or_reconstruct_noalloc(self.handle, C.byref(hits), C.byref(res)))
 
     def autotune(self, t, o, reps: int = 2) -> float:
         """or_session_autotune on a device-resident batch; returns the best mean chain time in ms."""

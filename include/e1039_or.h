@@ -202,6 +202,24 @@ or_status or_session_create(const or_geometry* g, const or_params* p, int device
  * through device buffers and returns when results are on the host. */
 or_status or_reconstruct(or_session* s, const or_hits* in, or_result* out);
 
+/* or_reconstruct for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid allocation).
+ * No counterpart in the reference.  Same results as or_reconstruct; device
+ * pointers only (in->on_device and out->on_device), asynchronous on the
+ * session stream, never captured into the session's graph.  Calls no
+ * hipMalloc, hipFree or allocating routine: its per-call device memory (class
+ * counters, class event lists, per-wave scratch) is one fixed array of
+ * OR_NOALLOC_SCRATCH_MB MiB per device, held until the process ends, so fewer
+ * waves run at once than in or_reconstruct.  Limits: at most
+ * OR_NOALLOC_MAX_EVENTS events per call, and the session's per-wave scratch
+ * (set by its caps) must fit the array; otherwise OR_ERR_CAPACITY.  Every
+ * call on one GPU uses that GPU's one array: calls that overlap on a GPU
+ * (other streams, other threads, other sessions) race and must be ordered by
+ * the caller. */
+#define OR_NOALLOC_SCRATCH_MB 128
+#define OR_NOALLOC_MAX_EVENTS 262144
+or_status or_reconstruct_noalloc(or_session* s, const or_hits* in, or_result* out);
+
 /* Waits for the session stream. */
 or_status or_session_sync(or_session* s);
 
