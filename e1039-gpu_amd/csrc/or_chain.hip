@@ -338,15 +338,25 @@ __device__ bool hodo_xz(const Ev<MAXH>& E, int st, int d, float x0, float tx, fl
     const float xh = G(F_Z, d) * tx + x0;
     const float f = hodo_fudge(st), cw = hodo_xcell(st);
     float ex = 3.f * (fabsf(G(F_Z, d) * etx) + ex0);
-    for (int i = 0; i < n; ++i) {
-        const float p = E.Lp->pos[E.raw(d, i)];
-        float xmin = p - cw * 0.5f, xmax = p + cw * 0.5f;
-        xmin -= (xmax - xmin) * f;
-        xmax += (xmax - xmin) * f;
-        ex += (xmax - xmin) * 0.15f;
-        xmin -= ex;
-        xmax += ex;
-        if (xmin <= xh && xh <= xmax) return true;
+    // hits in order (the window widening accumulates over them); positions
+    // read four at a time so one LDS wait covers four hits
+    const int r0 = E.raw(d, 0);
+    for (int i0 = 0; i0 < n; i0 += 4) {
+        float pv[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) pv[k] = E.Lp->pos[r0 + min(i0 + k, n - 1)];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (i0 + k >= n) break;
+            const float p = pv[k];
+            float xmin = p - cw * 0.5f, xmax = p + cw * 0.5f;
+            xmin -= (xmax - xmin) * f;
+            xmax += (xmax - xmin) * f;
+            ex += (xmax - xmin) * 0.15f;
+            xmin -= ex;
+            xmax += ex;
+            if (xmin <= xh && xh <= xmax) return true;
+        }
     }
     return false;
 }
@@ -362,34 +372,52 @@ __device__ bool hodo_tkl(const Ev<MAXH>& E, int st, int d, float x0, float y0, f
     const float xh = G(F_Z, d) * tx + x0, yh = G(F_Z, d) * ty + y0;
     float ex = 3.f * (fabsf(G(F_Z, d) * etx) + ex0);
     const float ey = 3.f * (fabsf(G(F_Z, d) * ety) + ey0);
-    for (int i = 0; i < n; ++i) {
-        const float p = E.Lp->pos[E.raw(d, i)];
-        float xmin = p - cw * 0.5f, xmax = p + cw * 0.5f;
-        float ymin = G(F_Y1, d), ymax = G(F_Y2, d);
-        xmin -= (xmax - xmin) * f;
-        xmax += (xmax - xmin) * f;
-        ymin -= (ymax - ymin) * f;
-        ymax += (ymax - ymin) * f;
-        ex += (xmax - xmin) * 0.15f;
-        xmin -= ex;
-        xmax += ex;
-        ymin -= ey;
-        ymax += ey;
-        if (xmin <= xh && xh <= xmax && ymin <= yh && yh <= ymax) return true;
+    const int r0 = E.raw(d, 0);
+    for (int i0 = 0; i0 < n; i0 += 4) {
+        float pv[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) pv[k] = E.Lp->pos[r0 + min(i0 + k, n - 1)];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (i0 + k >= n) break;
+            const float p = pv[k];
+            float xmin = p - cw * 0.5f, xmax = p + cw * 0.5f;
+            float ymin = G(F_Y1, d), ymax = G(F_Y2, d);
+            xmin -= (xmax - xmin) * f;
+            xmax += (xmax - xmin) * f;
+            ymin -= (ymax - ymin) * f;
+            ymax += (ymax - ymin) * f;
+            ex += (xmax - xmin) * 0.15f;
+            xmin -= ex;
+            xmax += ex;
+            ymin -= ey;
+            ymax += ey;
+            if (xmin <= xh && xh <= xmax && ymin <= yh && yh <= ymax) return true;
+        }
     }
     return false;
 }
 
-// first prop-tube hit within win of the expected coordinate, plane a then b
+// a prop-tube hit within win of the expected coordinate, plane a then b; hit
+// positions read four at a time so one LDS wait covers four hits
+template <int MAXH>
+__device__ __forceinline__ bool prop_plane_hit(const Ev<MAXH>& E, int d, float expect, float win) {
+    const int n = E.nred(d), r0 = E.raw(d, 0);
+    for (int m0 = 0; m0 < n; m0 += 4) {
+        float pv[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) pv[k] = E.Lp->pos[r0 + min(m0 + k, n - 1)];
+        bool hit = false;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) hit = hit || (m0 + k < n && fabsf(pv[k] - expect) < win);
+        if (hit) return true;
+    }
+    return false;
+}
 template <int MAXH>
 __device__ __forceinline__ int prop_hit(const Ev<MAXH>& E, int da, int db, float slope, float icpt, float win) {
-    const float za = G(F_Z, da);
-    for (int m = 0; m < E.nred(da); ++m)
-        if (fabsf(E.Lp->pos[E.raw(da, m)] - (slope * za + icpt)) < win) return 1;
-    const float zb = G(F_Z, db);
-    for (int m = 0; m < E.nred(db); ++m)
-        if (fabsf(E.Lp->pos[E.raw(db, m)] - (slope * zb + icpt)) < win) return 1;
-    return 0;
+    if (prop_plane_hit(E, da, slope * G(F_Z, da) + icpt, win)) return 1;
+    return prop_plane_hit(E, db, slope * G(F_Z, db) + icpt, win) ? 1 : 0;
 }
 
 // ---- window pair list (make_hitpairs_in_station, reconstruction_helper.cuh:464-549)
@@ -1348,13 +1376,23 @@ __device__ __forceinline__ UnitFeatures unit_features(const Scr& S, const Tk& T,
         float dmin = 100.f, ybest = yp;
         int kb = 0;
         const int n = v == 0 ? nl.x : nl.y;
-        for (int k = 0; k < n; ++k) {
-            const int q = (j * 4 + v) * win + k;
-            const float4 e = gld4(S.wy() + q);
-            const unsigned fl = gld(S.wf() + q);
-            const float y = (fl & 1u) ? ((fl & 2u) ? 0.5f * (e.x + e.z) : e.x) : e.z;
-            const float d = fabsf(y - yp);
-            if (d < dmin) { dmin = d; ybest = y; kb = k; }
+        const int q0 = (j * 4 + v) * win;
+        for (int k0 = 0; k0 < n; k0 += 4) {
+            float4 e[4];
+            unsigned fl[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const int q = q0 + min(k0 + t, n - 1);
+                e[t] = gld4(S.wy() + q);
+                fl[t] = gld(S.wf() + q);
+            }
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                if (k0 + t >= n) break;
+                const float y = (fl[t] & 1u) ? ((fl[t] & 2u) ? 0.5f * (e[t].x + e[t].z) : e[t].x) : e[t].z;
+                const float d = fabsf(y - yp);
+                if (d < dmin) { dmin = d; ybest = y; kb = k0 + t; }
+            }
         }
         d2[v] = dmin; yb2[v] = ybest; k2[v] = kb;
     }
@@ -1385,7 +1423,9 @@ __device__ __forceinline__ UnitFeatures unit_features(const Scr& S, const Tk& T,
         const int da = p == 0 ? 48 : 54, db = da - 1;
         const float ypa = k3 * G(F_Z, da), ypb = k3 * G(F_Z, db);
         float dmin = 100.f;
+#pragma unroll 4
         for (int i = 0; i < E.nred(da); ++i) dmin = fminf(dmin, fabsf(E.Lp->pos[E.raw(da, i)] - ypa));
+#pragma unroll 4
         for (int i = 0; i < E.nred(db); ++i) dmin = fminf(dmin, fabsf(E.Lp->pos[E.raw(db, i)] - ypb));
         dp[p] = dmin;
     }

@@ -8,7 +8,7 @@ Runs one warm-up and one measured launch of the chain on seeded events
 channel with occ8) through libe1039or_timing.so
 and prints each stage's share of the summed wave cycles.  With "pruned" the
 shipped YZ unit pruner is on; with "high" the caps are orchain.HIGH_OCCUPANCY_CAPS
-and the hit count is capped at 6000 instead of 1000.
+and the hit count is capped at 4000 instead of 1000.
 """
 import ctypes as C
 import os
@@ -35,7 +35,7 @@ def main():
     # "occ8": 8 % noise occupancy per channel; a number: Poisson mean hits capped at 1000
     high = "high" in sys.argv[3:]
     gen = dict(mult_mode=0, noise_occupancy=0.08) if wl == "occ8" else dict(mult_mode=1, mult_mean=float(wl),
-                                                                              mult_cap=6000 if high else 1000)
+                                                                              mult_cap=4000 if high else 1000)
     lib = oc.load_library(os.path.join(HERE, "libe1039or_timing.so"))
     lib.or_debug_stage_cycles.argtypes = [C.c_void_p, C.c_int]
     lib.or_debug_stage_cycles.restype = C.c_int32
@@ -45,9 +45,10 @@ def main():
     sess = oc.Session(geo, params, device=0)
     if "pruned" in sys.argv[3:]:
         import pruner
-        flat, thr, _ = pruner.load()
+        flat, thr, meta = pruner.load()
+        sess.set_pruner_margin(meta.get("margin", float("inf")))
         sess.set_pruner(flat, thr)
-        print(f"YZ unit pruner on (threshold {thr:.4f})")
+        print(f"YZ unit pruner on (keep margin {meta.get('margin', float('inf')):.4f})")
     din = sess.upload(ev)
     dout = sess.alloc_outputs(ev.n_events, ev.n_hits, tracklets=True)
     sess.run_device(din, dout)
