@@ -3522,7 +3522,9 @@ __device__ __attribute__((noinline)) void final_fit(const Tk& T, float* out) {
             B[i].add(pp); B[i].add(pe); B[i].add(vh * ul); B[i].add(vl * uh);
         }
     }
-    float res8[OR_FIT_FIELDS] = {0.f, 0.f, 0.f, 0.f, 0.f, (float)(nh - 4), INFINITY, 0.f};
+    // a fit that cannot be solved (fewer than 4 hits, or the FP64 factorisation
+    // fails) keeps NaN parameters and chi2 and an infinite condition estimate
+    float res8[OR_FIT_FIELDS] = {NAN, NAN, NAN, NAN, NAN, (float)(nh - 4), INFINITY, 0.f};
     if (nh >= 4) {
         float sc[4], l[4][4], cond = INFINITY;
 #pragma unroll
@@ -3628,7 +3630,9 @@ __device__ __attribute__((noinline)) void final_fit(const Tk& T, float* out) {
                 res8[2] = (float)(pd[2] - pd[3] * FIT_ZREF); res8[3] = (float)pd[3];
                 res8[4] = (float)c2;
             }
-            res8[6] = ok ? cond : (float)condd;
+            // FP32 factorisation succeeded above the bound: its estimate; failed: the FP64 one,
+            // and infinity when that fails too
+            res8[6] = ok ? cond : (std::isnan(res8[0]) ? INFINITY : (float)condd);
             res8[7] = 1.f;
         }
     }

@@ -167,7 +167,9 @@ typedef struct {
  * Cholesky pivot)^2 exceeds the bound (OR_FIT_COND_FP32 unless
  * or_session_set_fit_cond_bound changed it) or a pivot is not positive, the
  * solve and chi2 run in FP64 from the same sums.  Fields per track: */
-#define OR_FIT_FIELDS 8          /* x0, tx, y0, ty, chi2, ndf (hits - 4), condition estimate, FP64 flag */
+#define OR_FIT_FIELDS 8          /* x0, tx, y0, ty, chi2, ndf (hits - 4), condition estimate, FP64 flag;
+                                    a track whose fit cannot be solved (fewer than 4 hits, or the FP64
+                                    factorisation fails) gets NaN x0..chi2 and an infinite condition estimate */
 #define OR_FIT_COND_FP32 1.0e4f
 
 /* ---- entry points ---------------------------------------------------------- */
