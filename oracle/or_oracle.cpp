@@ -36,6 +36,7 @@
 #include <fstream>
 #include <sstream>
 #include <string>
+#include <mutex>
 #include <vector>
 #include <omp.h>
 
@@ -420,6 +421,7 @@ struct Tkl {             /* one tracklet: the reference 148-float record + bookk
 struct Ctx {
     const Geo* g;
     const or_params* p;
+    int64_t ev = 0;
     Plane pl[ND];
     std::vector<Tkl> tk;
     uint32_t status;
@@ -611,6 +613,21 @@ static void run_xz(Ctx& C) {
  * station-3/station-2 cross pair took a left/right decision.  When enabled,
  * run_yz_one compares that prediction with the carried state before and after
  * every left/right step.  Counters: checks, mismatches, cross-pair cases. */
+/* ---- candidate trace (test infrastructure): every YZ and station-1 candidate
+ * that reaches its chi2, in evaluation order, each tracklet's list sizes and
+ * its winner.  kind 0 YZ candidate (iu3, iv3, iu2, iv2), 1 YZ winner, 4 YZ
+ * list sizes (nu3, nv3, nu2, nv2); 2 station-1 candidate (ix, iu, iv), 3 its
+ * winner, 5 station-1 list sizes (nx1, nu1, nv1). */
+struct TraceRow { int kind; int64_t ev; int canon; int i[4]; float chi2; };
+static std::atomic<int> g_trace_on{0};
+static std::mutex g_trace_mu;
+static std::vector<TraceRow> g_trace;
+static void trace(int kind, const Ctx& C, int canon, int a, int b, int c, int d, float chi2) {
+    if (!g_trace_on.load(std::memory_order_relaxed)) return;
+    std::lock_guard<std::mutex> lk(g_trace_mu);
+    g_trace.push_back({kind, C.ev, canon, {a, b, c, d}, chi2});
+}
+
 static std::atomic<int> g_yzcheck{0};
 static std::atomic<long long> g_yzcnt[3];
 
@@ -666,6 +683,7 @@ static bool run_yz_one(Ctx& C, Tkl& T) {
         if (nl[v] < 0) { C.status |= OR_EV_PAIR_OVERFLOW; return false; }
     }
     const int nu2 = nl[0], nv2 = nl[1], nu3 = nl[2], nv3 = nl[3];
+    trace(4, C, T.canon, nu3, nv3, nu2, nv2, 0.f);
     /* enumeration bound of the device (32-bit candidate indices): more than
      * 2^27 (u3, v3, u2, v2) list combinations for one tracklet is an overflow */
     if ((long long)nu2 * nv2 * nu3 * nv3 > (1ll << 27)) { C.status |= OR_EV_PAIR_OVERFLOW; return false; }
@@ -705,6 +723,7 @@ static bool run_yz_one(Ctx& C, Tkl& T) {
     bool found = false;
     float best[NP];
     int best_orig[8];
+    int win_i[4] = {-1, -1, -1, -1};
     const int P1a = 48, P1b = 47, P2a = 54, P2b = 53;
     for (int iuv3 = 0; iuv3 < nu3 * nv3; ++iuv3) {
         int iu3 = iuv3 % nu3, iv3 = iuv3 / nu3;
@@ -796,9 +815,11 @@ static bool run_yz_one(Ctx& C, Tkl& T) {
                 rsd[k] = dd[k] * hs[k] - dca;
                 chi2 += ((rsd[k] * rsd[k]) / res[k]) / res[k];
             }
+            trace(0, C, T.canon, iu3, iv3, iu2, iv2, chi2);
             if (chi2 < chi2min) {
                 chi2min = chi2;
                 found = true;
+                win_i[0] = iu3; win_i[1] = iv3; win_i[2] = iu2; win_i[3] = iv2;
                 best[2] = nuv; best[3] = chi2; best[6] = ty; best[8] = y0; best[11] = ety; best[13] = ey0;
                 for (int h = 0; h < nuv; ++h) {
                     best[16 + h] = det[h]; best[34 + h] = el[h]; best[52 + h] = pos[h]; best[70 + h] = drift[h];
@@ -811,6 +832,7 @@ static bool run_yz_one(Ctx& C, Tkl& T) {
         }
     }
     if (found) {
+        trace(1, C, T.canon, win_i[0], win_i[1], win_i[2], win_i[3], chi2min);
         int nb = (int)best[2];
         R[0] = 5;
         R[2] = (float)(nx + nb);
@@ -885,6 +907,31 @@ static void run_cleaning(Ctx& C) {
 }
 
 /* ---- global: reconstruction_kernels.cuh:1841-2534, one track --------------- */
+/* SagittaRatioInStation1, reconstruction_helper.cuh:556-613: expected position
+ * and half-width of the station-1 window of each view (0 X, 1 U, 2 V) from the
+ * track's projections at station 2 and at its last hit, combined for the
+ * target and the dump hypotheses */
+static void sagitta_windows(const Geo& g, float x0, float tx, float y0, float ty, int dlast, float* pexp, float* win) {
+    float z3 = g.z[dlast], x3 = x0 + tx * z3, y3 = y0 + ty * z3;
+    for (int i = 0; i < 3; ++i) {
+        int d = 2 * i + 2;
+        int idx = kPlaneType[d / 2 - 1];
+        int d2 = kSuperId[2][idx] * 2;
+        float ps3 = x3 * g.costheta[d2] + y3 * g.sintheta[d2];
+        float z1 = g.z[d], z2 = g.z[d2];
+        float x2 = x0 + tx * z2, y2 = y0 + ty * z2;
+        float ps2 = x2 * g.costheta[d2] + y2 * g.sintheta[d2];
+        float s2t = ps2 - (ps3 * (z2 - Z_TARGET)) / (z3 - Z_TARGET);
+        float s2d = ps2 - (ps3 * (z2 - Z_DUMP)) / (z3 - Z_DUMP);
+        float pet = SAG_T_C * s2t + (ps3 * (z1 - Z_TARGET)) / (z3 - Z_TARGET);
+        float ped = SAG_D_C * s2d + (ps3 * (z1 - Z_DUMP)) / (z3 - Z_DUMP);
+        float wt = std::fabs(s2t * SAG_T_W), wd = std::fabs(s2d * SAG_D_W);
+        float pmn = std::fmin(pet - wt, ped - wd), pmx = std::fmax(pet + wt, ped + wd);
+        pexp[idx] = 0.5 * (pmx + pmn);
+        win[idx] = 0.5 * (pmx - pmn);
+    }
+}
+
 static bool run_global_one(Ctx& C, Tkl& T) {
     const Geo& g = *C.g;
     float* R = T.rec;
@@ -901,26 +948,7 @@ static bool run_global_one(Ctx& C, Tkl& T) {
     }
     /* SagittaRatioInStation1, reconstruction_helper.cuh:556-613 */
     float pexp[3], win[3];
-    {
-        float z3 = g.z[dlast], x3 = x0 + tx * z3, y3 = y0 + ty * z3;
-        for (int i = 0; i < 3; ++i) {
-            int d = 2 * i + 2;
-            int idx = kPlaneType[d / 2 - 1];
-            int d2 = kSuperId[2][idx] * 2;
-            float ps3 = x3 * g.costheta[d2] + y3 * g.sintheta[d2];
-            float z1 = g.z[d], z2 = g.z[d2];
-            float x2 = x0 + tx * z2, y2 = y0 + ty * z2;
-            float ps2 = x2 * g.costheta[d2] + y2 * g.sintheta[d2];
-            float s2t = ps2 - (ps3 * (z2 - Z_TARGET)) / (z3 - Z_TARGET);
-            float s2d = ps2 - (ps3 * (z2 - Z_DUMP)) / (z3 - Z_DUMP);
-            float pet = SAG_T_C * s2t + (ps3 * (z1 - Z_TARGET)) / (z3 - Z_TARGET);
-            float ped = SAG_D_C * s2d + (ps3 * (z1 - Z_DUMP)) / (z3 - Z_DUMP);
-            float wt = std::fabs(s2t * SAG_T_W), wd = std::fabs(s2d * SAG_D_W);
-            float pmn = std::fmin(pet - wt, ped - wd), pmx = std::fmax(pet + wt, ped + wd);
-            pexp[idx] = 0.5 * (pmx + pmn);
-            win[idx] = 0.5 * (pmx - pmn);
-        }
-    }
+    sagitta_windows(g, x0, tx, y0, ty, dlast, pexp, win);
     const int wcap = std::max(C.p->max_pairs_window, 1);
     std::vector<Pair> lxv(wcap), luv(wcap), lvv(wcap);
     Pair *lx = lxv.data(), *lu = luv.data(), *lv = lvv.data();
@@ -929,6 +957,7 @@ static bool run_global_one(Ctx& C, Tkl& T) {
     int nv1 = pairs_window(C, 0, 2, pexp[2] - win[2] * 1.0f, pexp[2] + win[2] * 1.0f, lv, C.p->max_pairs_window);
     if (nx1 < 0 || nu1 < 0 || nv1 < 0) { C.status |= OR_EV_PAIR_OVERFLOW; return false; }
     if (nx1 == 0 || nu1 == 0 || nv1 == 0) return true;
+    trace(5, C, T.canon, nx1, nu1, nv1, 0, 0.f);
     /* y of the station-2/3 UV hits with their signs (reconstruction_kernels.cuh:2086-2102) */
     short d23[12], c23[12], s23[12];
     float dr23[12], res23[12], r23[12];
@@ -952,6 +981,7 @@ static bool run_global_one(Ctx& C, Tkl& T) {
     bool found = false;
     float best[NP];
     int best_orig[6];
+    int gwin[3] = {-1, -1, -1};
     for (int ix = 0; ix < nx1; ++ix) {
         int nx = 0;
         auto addx = [&](int d, int idx) {
@@ -1031,10 +1061,12 @@ static bool run_global_one(Ctx& C, Tkl& T) {
                 c1 += ((rsd[k] * rsd[k]) / res[k]) / res[k];
             }
             chi2 += c1;
+            trace(2, C, T.canon, ix, iu, iv, 0, chi2);
             if (chi2 > 10000.f) continue;
             if (chi2 < chi2min) {
                 chi2min = chi2;
                 found = true;
+                gwin[0] = ix; gwin[1] = iu; gwin[2] = iv;
                 best[2] = nx + nuv; best[3] = chi2; best[6] = ty; best[8] = y0; best[9] = invp; best[11] = ety;
                 best[13] = ey0; best[14] = einvp; best[15] = q;
                 for (int m = 0; m < n23; ++m) best[124 + m] = r23[m];
@@ -1047,6 +1079,7 @@ static bool run_global_one(Ctx& C, Tkl& T) {
         }
     }
     if (found) {
+        trace(3, C, T.canon, gwin[0], gwin[1], gwin[2], 0, chi2min);
         int n1 = (int)best[2];
         R[0] = 6;
         R[2] = (float)(n23 + n1);
@@ -1253,7 +1286,7 @@ static void run_event(const Geo& g, const or_params& P, const or_hits& in, int64
                       std::vector<uint8_t>& keep_bytes, Ctx& C) {
     const int64_t h0 = in.event_offsets[e], h1 = in.event_offsets[e + 1];
     const int nh = (int)(h1 - h0);
-    C.g = &g; C.p = &P; C.status = 0; C.tk.clear();
+    C.g = &g; C.p = &P; C.status = 0; C.tk.clear(); C.ev = e;
     for (int d = 0; d < ND; ++d) C.pl[d].n = 0;
     uint8_t* keep = keep_bytes.data() + h0;
     for (int h = 0; h < nh; ++h) keep[h] = 0;
@@ -1483,9 +1516,60 @@ void orc_xz_fy(const orc_geometry* G, int d, int e, float drift, int hs, float y
 }
 int orc_hodo_xz(const orc_geometry* G, int stid, int d, int n, const float* hpos, float x0, float tx, float ex0,
                 float etx) { return orc::hodo_xz(stid, d, n, hpos, x0, tx, ex0, etx, G->g) ? 1 : 0; }
+/* left/right resolution on explicit hit lists (reconstruction_helper.cuh:714-892) */
 void orc_lr_xhits(const orc_geometry* G, float x0, float tx, float ex0, float etx, int n, const short* det,
-                  const float* pos, const float* drift, short* sign) {
-    orc::lr_xhits(x0, tx, ex0, etx, n, det, pos, drift, sign, G->g, 150.f);
+                  const float* pos, const float* drift, short* sign, float thr) {
+    orc::lr_xhits(x0, tx, ex0, etx, n, det, pos, drift, sign, G->g, thr);
+}
+void orc_lr_newhits(const orc_geometry* G, float x0, float tx, float y0, float ty, float ex0, float etx, float ey0,
+                    float ety, int n, const short* det, const float* pos, const float* drift, short* sign, float thr) {
+    orc::lr_newhits(x0, tx, y0, ty, ex0, etx, ey0, ety, n, det, pos, drift, sign, G->g, thr);
+}
+void orc_lr_single_xhits(const orc_geometry* G, float x0, float tx, int n, const short* det, const float* pos,
+                         short* sign) {
+    orc::lr_single_xhits(x0, tx, n, det, pos, sign, G->g);
+}
+void orc_lr_single_newhits(const orc_geometry* G, float x0, float tx, float y0, float ty, int n, const short* det,
+                           const float* pos, short* sign) {
+    orc::lr_single_newhits(x0, tx, y0, ty, n, det, pos, sign, G->g);
+}
+/* station-1 windows of a track (SagittaRatioInStation1): pexp[3], win[3] by view X, U, V */
+void orc_sagitta(const orc_geometry* G, float x0, float tx, float y0, float ty, int dlast, float* pexp, float* win) {
+    orc::sagitta_windows(G->g, x0, tx, y0, ty, dlast, pexp, win);
+}
+/* back-partial cleaning of n explicit tracklets (148-float records, storage
+ * slot each), in slot order as the chain runs it; stid_out[i] = stationID of
+ * tracklet i afterwards */
+void orc_back_cleaning(int n, const float* recs, const int* slot, float* stid_out) {
+    orc::Ctx* C = new orc::Ctx;
+    C->status = 0;
+    C->tk.resize(n);
+    for (int i = 0; i < n; ++i) {
+        std::memcpy(C->tk[i].rec, recs + (size_t)i * orc::NP, sizeof(float) * orc::NP);
+        C->tk[i].slot = slot[i];
+        C->tk[i].canon = i;
+        C->tk[i].pos_in_slot = i;
+    }
+    std::stable_sort(C->tk.begin(), C->tk.end(), [](const orc::Tkl& a, const orc::Tkl& b) { return a.slot < b.slot; });
+    orc::run_cleaning(*C);
+    for (const auto& T : C->tk) stid_out[T.canon] = T.rec[0];
+    delete C;
+}
+/* candidate trace: enable (and clear) / fetch up to cap rows; returns the row count */
+void orc_trace(int enable) {
+    std::lock_guard<std::mutex> lk(orc::g_trace_mu);
+    orc::g_trace.clear();
+    orc::g_trace_on.store(enable);
+}
+long long orc_trace_fetch(long long cap, int* kind, long long* ev, int* canon, int* idx4, float* chi2) {
+    std::lock_guard<std::mutex> lk(orc::g_trace_mu);
+    const long long n = (long long)orc::g_trace.size();
+    for (long long r = 0; r < n && r < cap; ++r) {
+        const auto& T = orc::g_trace[r];
+        kind[r] = T.kind; ev[r] = T.ev; canon[r] = T.canon; chi2[r] = T.chi2;
+        for (int k = 0; k < 4; ++k) idx4[4 * r + k] = T.i[k];
+    }
+    return n;
 }
 /* Binned X pair builder on explicit planes: pos/chan arrays; out: 28*cap pairs (a,b), cnt[28] */
 void orc_pairs_binned(int stid, int n1, const float* chan1, const float* pos1, int n2, const float* chan2,

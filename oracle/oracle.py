@@ -48,7 +48,15 @@ def load():
     lib.orc_xz_fy.argtypes = [vp, C.c_int, C.c_int, f32, C.c_int, f32, f32, C.POINTER(f32), C.POINTER(f32)]
     lib.orc_hodo_xz.argtypes = [vp, C.c_int, C.c_int, C.c_int, vp, f32, f32, f32, f32]
     lib.orc_hodo_xz.restype = C.c_int
-    lib.orc_lr_xhits.argtypes = [vp, f32, f32, f32, f32, C.c_int, vp, vp, vp, vp]
+    lib.orc_lr_xhits.argtypes = [vp, f32, f32, f32, f32, C.c_int, vp, vp, vp, vp, f32]
+    lib.orc_lr_newhits.argtypes = [vp, f32, f32, f32, f32, f32, f32, f32, f32, C.c_int, vp, vp, vp, vp, f32]
+    lib.orc_lr_single_xhits.argtypes = [vp, f32, f32, C.c_int, vp, vp, vp]
+    lib.orc_lr_single_newhits.argtypes = [vp, f32, f32, f32, f32, C.c_int, vp, vp, vp]
+    lib.orc_sagitta.argtypes = [vp, f32, f32, f32, f32, C.c_int, vp, vp]
+    lib.orc_back_cleaning.argtypes = [C.c_int, vp, vp, vp]
+    lib.orc_trace.argtypes = [C.c_int]
+    lib.orc_trace_fetch.argtypes = [C.c_longlong, vp, vp, vp, vp, vp]
+    lib.orc_trace_fetch.restype = C.c_longlong
     lib.orc_pairs_binned.argtypes = [C.c_int, C.c_int, vp, vp, C.c_int, vp, vp, C.c_int, vp, vp]
     lib.orc_yz_state_check.argtypes = [C.c_int, vp]
     f32 = C.c_float
