@@ -132,14 +132,14 @@ __device__ unsigned long long g_sub_cycles[N_SUB_TIMERS];
 #define OR_SUB(k)                                                              \
     do {                                                                       \
         s_1 = clock64();                                                       \
-        if (threadIdx.x == 0) atomicAdd(&g_sub_cycles[k], s_1 - s_0);          \
+        if ((threadIdx.x & 63) == 0) atomicAdd(&g_sub_cycles[k], s_1 - s_0);          \
         s_0 = s_1;                                                             \
     } while (0)
 // counts instead of cycles in slots 10..14 (YZ candidates, units scored,
 // units kept, candidates of kept units, first-cut survivors)
 #define OR_COUNT(k, v)                                                         \
     do {                                                                       \
-        if (threadIdx.x == 0) atomicAdd(&g_sub_cycles[k], (unsigned long long)(v)); \
+        if ((threadIdx.x & 63) == 0) atomicAdd(&g_sub_cycles[k], (unsigned long long)(v)); \
     } while (0)
 #else
 #define OR_SUB_START
@@ -171,6 +171,21 @@ static_assert(sizeof(Tk) == 256, "tracklet record size");
 
 constexpr int CLEAN_WORDS = 64;   // lane-parallel cleaning of up to 64 x 64 live tracklets
 
+// Waves per event: a tile class runs with 1, 2 or 4 waves per workgroup (one
+// event per workgroup).  The YZ stage hands its tracklet chunks to all of
+// them; the other stages run on the first wave.
+constexpr int WPE_MAX = 4;
+__host__ __device__ constexpr int lds_waves(int maxh) { return maxh <= 512 ? 1 : WPE_MAX; }
+
+// YZ state of one wave's chunk: window list lengths, candidate or unit offsets
+// per tracklet, best unit score (pruning) and best candidate key per tracklet.
+struct YzWave {
+    int nl[YZ_CHUNK][4];      // window list lengths (read as int4)
+    int uoff[YZ_CHUNK + 1];   // prefix of (u3, v3) units or of candidates per tracklet of the chunk
+    int umax[YZ_CHUNK];       // pruning: best unit score of each tracklet of the chunk (order-preserving int)
+    unsigned long long best[YZ_CHUNK];
+};
+
 // The event tile.  Hits are staged grouped by detector; after the reducer the
 // kept ones are compacted in place, so plane d's reduced hits are
 // [rstart[d], rstart[d] + rcnt[d]) of elem/pos/drift/orig.  Sized so that the
@@ -187,23 +202,23 @@ struct Lds {
     int16_t pstart[56], cursor[56], rcnt[56], rstart[56];
     uint16_t xcnt[3][28];
     union alignas(16) {
-        int nl[YZ_CHUNK][4];      // YZ / global: window list lengths (read as int4)
+        YzWave yw[lds_waves(MAXH)];   // YZ: one per wave; station 1 reads yw[0].nl as 64 list lengths
         uint16_t live[WAVE];      // cleaning: tracklets at stationID >= 5, slot order
         struct {
             unsigned long long alive[CLEAN_WORDS];   // cleaning of more than 64: alive bits by live position
             int lstart[33];                          // first live position of each slot
         } cl;
     };
-    int uoff[YZ_CHUNK + 1];   // prefix of (u3, v3) units or of candidates per tracklet of the chunk
-    int umax[YZ_CHUNK];       // pruning: best unit score of each tracklet of the chunk (order-preserving int)
-    unsigned long long best[YZ_CHUNK];
     int sstart[33];
     uint32_t status;
+    int ntkl;                 // tracklets after XZ (read by every wave of the event)
+    int yz_next;              // next YZ chunk to hand out
+    int next_ev;              // work queue: the event the group takes next
 };
 
 // Per-wave scratch in HBM: one block per resident wave, regions at offsets the
 // host computes from the session's caps (ScrLayout); each wave keeps the
-// addresses of its regions in LDS (g_cfg) so the stage functions take no
+// addresses of its regions in LDS (g_cfgw) so the stage functions take no
 // pointer arguments for them.
 enum ScrRegion {
     R_TK,     // Tk[maxtkl]: the event's tracklets
@@ -260,15 +275,23 @@ struct Args {
     unsigned long long* dump_count;
 };
 
-// region addresses and caps of the wave (written at kernel start)
+// region addresses and caps of each wave (written at kernel start)
 struct WaveCfg {
     char* rp[R_COUNT];
     int32_t maxtkl, bin_cap, win_cap, list_cap, unit_cap;
 };
-__shared__ WaveCfg g_cfg;
+__shared__ WaveCfg g_cfgw[WPE_MAX];
+
+// regions of the event, shared by its waves; the others are per wave
+__host__ __device__ constexpr bool scr_shared(int r) {
+    return r == R_TK || r == R_CELL || r == R_XP || r == R_SO || r == R_CL;
+}
 
 __device__ __forceinline__ unsigned long long ballot(bool p) { return __ballot(p); }
-__device__ __forceinline__ int lane_id() { return threadIdx.x; }
+__device__ __forceinline__ int lane_id() { return threadIdx.x & (WAVE - 1); }
+// wave of the workgroup (wave-uniform, in a scalar register)
+__device__ __forceinline__ int wave_idx() { return __builtin_amdgcn_readfirstlane((int)threadIdx.x) >> 6; }
+__device__ __forceinline__ WaveCfg& cfg() { return g_cfgw[wave_idx()]; }
 __device__ __forceinline__ int rank_in(unsigned long long m) {
     return __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
 }
@@ -308,7 +331,11 @@ __device__ __forceinline__ int wave_sum(int v) {
 template <int MAXH>
 __shared__ Lds<MAXH> g_lds;
 
+// the calling wave's YZ state
 template <int MAXH>
+__device__ __forceinline__ YzWave& yzw() { return g_lds<MAXH>.yw[lds_waves(MAXH) == 1 ? 0 : wave_idx()]; }
+
+template <int MAXH, int WPE>
 struct Ev {
     struct Ref {
         __device__ __forceinline__ Lds<MAXH>* operator->() const { return &g_lds<MAXH>; }
@@ -330,8 +357,8 @@ __device__ __forceinline__ int xbin(int s, int c) {
 }
 
 // hodoscope X match for an XZ candidate (reconstruction_helper.cuh:1090-1144)
-template <int MAXH>
-__device__ bool hodo_xz(const Ev<MAXH>& E, int st, int d, float x0, float tx, float ex0, float etx) {
+template <int MAXH, int WPE>
+__device__ bool hodo_xz(const Ev<MAXH, WPE>& E, int st, int d, float x0, float tx, float ex0, float etx) {
     const int n = E.nred(d);
     if (n == 0) return false;
     if (d > 40) st = 3;
@@ -362,8 +389,8 @@ __device__ bool hodo_xz(const Ev<MAXH>& E, int st, int d, float x0, float tx, fl
 }
 
 // hodoscope match for a tracklet (reconstruction_helper.cuh:1002-1087)
-template <int MAXH>
-__device__ bool hodo_tkl(const Ev<MAXH>& E, int st, int d, float x0, float y0, float tx, float ty, float ex0, float ey0,
+template <int MAXH, int WPE>
+__device__ bool hodo_tkl(const Ev<MAXH, WPE>& E, int st, int d, float x0, float y0, float tx, float ty, float ex0, float ey0,
                          float etx, float ety) {
     const int n = E.nred(d);
     if (n == 0) return false;
@@ -400,8 +427,8 @@ __device__ bool hodo_tkl(const Ev<MAXH>& E, int st, int d, float x0, float y0, f
 
 // a prop-tube hit within win of the expected coordinate, plane a then b; hit
 // positions read four at a time so one LDS wait covers four hits
-template <int MAXH>
-__device__ __forceinline__ bool prop_plane_hit(const Ev<MAXH>& E, int d, float expect, float win) {
+template <int MAXH, int WPE>
+__device__ __forceinline__ bool prop_plane_hit(const Ev<MAXH, WPE>& E, int d, float expect, float win) {
     const int n = E.nred(d), r0 = E.raw(d, 0);
     for (int m0 = 0; m0 < n; m0 += 4) {
         float pv[4];
@@ -414,16 +441,16 @@ __device__ __forceinline__ bool prop_plane_hit(const Ev<MAXH>& E, int d, float e
     }
     return false;
 }
-template <int MAXH>
-__device__ __forceinline__ int prop_hit(const Ev<MAXH>& E, int da, int db, float slope, float icpt, float win) {
+template <int MAXH, int WPE>
+__device__ __forceinline__ int prop_hit(const Ev<MAXH, WPE>& E, int da, int db, float slope, float icpt, float win) {
     if (prop_plane_hit(E, da, slope * G(F_Z, da) + icpt, win)) return 1;
     return prop_plane_hit(E, db, slope * G(F_Z, db) + icpt, win) ? 1 : 0;
 }
 
 // ---- window pair list (make_hitpairs_in_station, reconstruction_helper.cuh:464-549)
 // Returns the number of pairs, or -1 when a cap is exceeded.
-template <int MAXH>
-__device__ __attribute__((noinline)) int pairs_window(const Ev<MAXH>& E, int st, int proj, float xmin, float xmax,
+template <int MAXH, int WPE>
+__device__ __attribute__((noinline)) int pairs_window(const Ev<MAXH, WPE>& E, int st, int proj, float xmin, float xmax,
                                                       uint16_t* out, int cap_plane, int cap_pairs) {
     const int sid = super_id(st, proj);
     const int d1 = sid * 2, d2 = sid * 2 - 1;
@@ -510,8 +537,8 @@ __device__ __forceinline__ int pb(uint16_t q) { int b = q >> 8; return b == 0xFF
 
 // ---------------------------------------------------------------------------
 // XZ candidate (reconstruction_kernels.cuh:465-672)
-template <int MAXH>
-__device__ __forceinline__ bool xz_candidate(const Ev<MAXH>& E, int s3, uint16_t q2, uint16_t q3, Tk* out) {
+template <int MAXH, int WPE>
+__device__ __forceinline__ bool xz_candidate(const Ev<MAXH, WPE>& E, int s3, uint16_t q2, uint16_t q3, Tk* out) {
     // hits in fixed slots D2 X, X', D3 X, X' (the reference's list order); see lr_slots_new
     const int d3x = s3 == 1 ? 22 : 28;
     int det[4] = {16, 15, d3x, d3x - 1};
@@ -673,8 +700,8 @@ __device__ __forceinline__ void yz_dets(YzSlots& H, int st3) {
 }
 
 // accepted hits of entry k of view v into slots S0 (hit a) and S0 + 1 (hit b), signs 0
-template <int S0, int MAXH>
-__device__ __forceinline__ void yz_fill(const Ev<MAXH>& E, const YzLists& W, int v, int k, YzSlots& H) {
+template <int S0, int MAXH, int WPE>
+__device__ __forceinline__ void yz_fill(const Ev<MAXH, WPE>& E, const YzLists& W, int v, int k, YzSlots& H) {
     const uint16_t q = gld(W.l[v] + k);
     const unsigned fl = gld(W.f[v] + k);
     H.pres &= ~(3u << S0);
@@ -708,8 +735,8 @@ __device__ __forceinline__ unsigned lr_slot_starts(unsigned pres, const int (&de
 
 // The tests of the tail that need no left/right sign: second box cut, prop-tube
 // Y and hodoscope matches, on the x line and the first fit alone.
-template <int MAXH>
-__device__ __forceinline__ bool yz_cuts(const Ev<MAXH>& E, const Tk* Tp, const Fit& f) {
+template <int MAXH, int WPE>
+__device__ __forceinline__ bool yz_cuts(const Ev<MAXH, WPE>& E, const Tk* Tp, const Fit& f) {
     const float x0 = Tp->x0, tx = Tp->tx, ex0 = Tp->ex0, etx = Tp->etx;
     const float y0 = f.p0, ty = f.p1, ey0 = f.e0, ety = f.e1;
     if (fabsf(y0) > Y0_MAX || fabsf(ty) > TY_MAX) return false;
@@ -730,8 +757,8 @@ __device__ __forceinline__ bool yz_cuts(const Ev<MAXH>& E, const Tk* Tp, const F
 
 // chi2 over the x hits (stored signs) and the UV hits (signs in H); with Tw set
 // the candidate is written into the tracklet
-template <int MAXH>
-__device__ __forceinline__ void yz_chi2(const Ev<MAXH>& E, const Tk* Tp, const YzSlots& H, const Fit& f,
+template <int MAXH, int WPE>
+__device__ __forceinline__ void yz_chi2(const Ev<MAXH, WPE>& E, const Tk* Tp, const YzSlots& H, const Fit& f,
                                         float& chi2_out, Tk* Tw) {
     const float x0 = Tp->x0, tx = Tp->tx;
     const float y0 = f.p0, ty = f.p1, ey0 = f.e0, ety = f.e1;
@@ -773,8 +800,8 @@ __device__ __forceinline__ void yz_chi2(const Ev<MAXH>& E, const Tk* Tp, const Y
 // hodoscope matches, chi2 over the x hits (stored signs) and the UV hits
 // (reconstruction_helper.cuh:1182-1204).  With Tw set the candidate is written
 // into the tracklet.
-template <int MAXH>
-__device__ __forceinline__ bool yz_tail(const Ev<MAXH>& E, const Tk* Tp, YzSlots& H, const Fit& f, float& chi2_out,
+template <int MAXH, int WPE>
+__device__ __forceinline__ bool yz_tail(const Ev<MAXH, WPE>& E, const Tk* Tp, YzSlots& H, const Fit& f, float& chi2_out,
                                         Tk* Tw) {
     lr_slots_new<8>(Tp->x0, Tp->tx, f.p0, f.p1, Tp->ex0, Tp->etx, f.e0, f.e1, H.pres, H.det, H.pos, H.drift, H.sign,
                     150.f);
@@ -787,8 +814,8 @@ __device__ __forceinline__ bool yz_tail(const Ev<MAXH>& E, const Tk* Tp, YzSlots
 // order with the station-3 signs carried from candidate to candidate, as the
 // reference does.  search (stop < 0): the unit's best (chi2, i_uv2).  replay
 // (stop >= 0): writes the candidate at i_uv2 == stop into the tracklet.
-template <int MAXH>
-__device__ __attribute__((noinline)) bool yz_unit(const Ev<MAXH>& E, Tk* Tp, const YzLists& W, int iuv3, int stop,
+template <int MAXH, int WPE>
+__device__ __attribute__((noinline)) bool yz_unit(const Ev<MAXH, WPE>& E, Tk* Tp, const YzLists& W, int iuv3, int stop,
                                                   float& best_chi2, int& best_i) {
     const int st3 = Tp->st3;
     const int su2 = super_id(2, 1) * 2, sv2 = super_id(2, 2) * 2;
@@ -830,37 +857,37 @@ __device__ __attribute__((noinline)) bool yz_unit(const Ev<MAXH>& E, Tk* Tp, con
 // each stage, and stage-level branches depend only on wave-uniform values.
 #define STAGE __device__ __attribute__((noinline))
 
-// Accessors of the wave's scratch regions (addresses from g_cfg).
+// Accessors of the wave's scratch regions (addresses from g_cfgw).
 struct Scr {
-    __device__ Tk* tks() const { return reinterpret_cast<Tk*>(g_cfg.rp[R_TK]); }
-    __device__ uint32_t* celloff() const { return reinterpret_cast<uint32_t*>(g_cfg.rp[R_CELL]); }
-    __device__ uint16_t* wl() const { return reinterpret_cast<uint16_t*>(g_cfg.rp[R_LIST]); }
-    __device__ float4* wy() const { return reinterpret_cast<float4*>(g_cfg.rp[R_YV]); }
-    __device__ uint8_t* wf() const { return reinterpret_cast<uint8_t*>(g_cfg.rp[R_YF]); }
+    __device__ Tk* tks() const { return reinterpret_cast<Tk*>(cfg().rp[R_TK]); }
+    __device__ uint32_t* celloff() const { return reinterpret_cast<uint32_t*>(cfg().rp[R_CELL]); }
+    __device__ uint16_t* wl() const { return reinterpret_cast<uint16_t*>(cfg().rp[R_LIST]); }
+    __device__ float4* wy() const { return reinterpret_cast<float4*>(cfg().rp[R_YV]); }
+    __device__ uint8_t* wf() const { return reinterpret_cast<uint8_t*>(cfg().rp[R_YF]); }
     // [3][28][bin_cap] X pairs (a | b << 8), 0xFF = none
-    __device__ uint16_t* xpairs() const { return reinterpret_cast<uint16_t*>(g_cfg.rp[R_XP]); }
+    __device__ uint16_t* xpairs() const { return reinterpret_cast<uint16_t*>(cfg().rp[R_XP]); }
     // tracklet index by slot order
-    __device__ uint16_t* slot_order() const { return reinterpret_cast<uint16_t*>(g_cfg.rp[R_SO]); }
+    __device__ uint16_t* slot_order() const { return reinterpret_cast<uint16_t*>(cfg().rp[R_SO]); }
     // YZ candidates passing the first box cut, in reference order
-    __device__ uint32_t* plist() const { return reinterpret_cast<uint32_t*>(g_cfg.rp[R_PL]); }
+    __device__ uint32_t* plist() const { return reinterpret_cast<uint32_t*>(cfg().rp[R_PL]); }
     // per candidate: x first-of-unit, y station-3 signs after its left/right (first
     // of unit), z sign written to the last station-3 hit (0: none), w station-3
     // signs it starts from
-    __device__ uchar4* pst() const { return reinterpret_cast<uchar4*>(g_cfg.rp[R_PB]); }
+    __device__ uchar4* pst() const { return reinterpret_cast<uchar4*>(cfg().rp[R_PB]); }
     // kept YZ units of a chunk (j << 16 | u3.v3 index), in order, and their first
     // candidate in the kept enumeration
-    __device__ uint32_t* klist() const { return reinterpret_cast<uint32_t*>(g_cfg.rp[R_KL]); }
-    __device__ uint32_t* kstart() const { return reinterpret_cast<uint32_t*>(g_cfg.rp[R_KS]); }
+    __device__ uint32_t* klist() const { return reinterpret_cast<uint32_t*>(cfg().rp[R_KL]); }
+    __device__ uint32_t* kstart() const { return reinterpret_cast<uint32_t*>(cfg().rp[R_KS]); }
     // per box-cut survivor of a batch: its hits (raw LDS indices), which slots
     // hold one, and its first fit, as pass 2 built them (3 x 16 bytes)
-    __device__ uint4* ycache() const { return reinterpret_cast<uint4*>(g_cfg.rp[R_YC]); }
+    __device__ uint4* ycache() const { return reinterpret_cast<uint4*>(cfg().rp[R_YC]); }
     // survivors of a batch that pass the sign-free tests, in order
-    __device__ uint32_t* qlist() const { return reinterpret_cast<uint32_t*>(g_cfg.rp[R_QL]); }
+    __device__ uint32_t* qlist() const { return reinterpret_cast<uint32_t*>(cfg().rp[R_QL]); }
     // per tracklet of the chunk: the cache entry and sign state of its best candidate so far
-    __device__ uint4* brec() const { return reinterpret_cast<uint4*>(g_cfg.rp[R_BR]); }
+    __device__ uint4* brec() const { return reinterpret_cast<uint4*>(cfg().rp[R_BR]); }
     // cleaning: per live tracklet, the raw hit of each of detectors 13-30 (uint16, 0xFFFF none)
-    __device__ uint4* clean_hits() const { return reinterpret_cast<uint4*>(g_cfg.rp[R_CL]); }
-    __device__ int win() const { return g_cfg.win_cap; }
+    __device__ uint4* clean_hits() const { return reinterpret_cast<uint4*>(cfg().rp[R_CL]); }
+    __device__ int win() const { return cfg().win_cap; }
 };
 
 // Learned unit pruning and unit rows for the YZ stage, by value from the
@@ -875,7 +902,7 @@ struct Prn {
 
 // A + B: load, group by detector, reduce.  Returns true when the keep bits are
 // valid (staging and reduction caps passed); station cuts set status bits.
-template <int MAXH>
+template <int MAXH, int WPE>
 STAGE bool stage_load_reduce(const Args& a, int64_t h0, int n) {
     Lds<MAXH>& L = g_lds<MAXH>;
     const int lane = lane_id();
@@ -891,7 +918,7 @@ STAGE bool stage_load_reduce(const Args& a, int64_t h0, int n) {
             if (d < 1 || el < 1 || el > (int)G(F_NELEM, d) || el >= OR_MAX_ELEM) atomicOr(&L.status, (uint32_t)OR_EV_BAD_HIT);
             else atomicAdd(&L.pcnt[d], 1);
         }
-        __syncthreads();
+        wsync();
         const bool badhit = L.status != 0;
         {
             const int d = lane;
@@ -903,7 +930,7 @@ STAGE bool stage_load_reduce(const Args& a, int64_t h0, int n) {
             const int inc = wave_scan(c);
             if (lane < 56) { L.pstart[lane] = (int16_t)(inc - c); L.cursor[lane] = (int16_t)(inc - c); }
         }
-        __syncthreads();
+        wsync();
         if (L.status == 0) {
             // stable scatter by detector
             for (int base = 0; base < n; base += WAVE) {
@@ -919,9 +946,9 @@ STAGE bool stage_load_reduce(const Args& a, int64_t h0, int n) {
                     const unsigned long long m = ballot(v && d == dl);
                     const int cur = L.cursor[dl];
                     if (v && d == dl) dest = cur + rank_in(m);
-                    __syncthreads();
+                    wsync();
                     if (lane == leader) L.cursor[dl] = (int16_t)(cur + __popcll(m));
-                    __syncthreads();
+                    wsync();
                     todo &= ~m;
                 }
                 if (v) {
@@ -935,7 +962,7 @@ STAGE bool stage_load_reduce(const Args& a, int64_t h0, int n) {
                 }
             }
             for (int w = lane; w < MAXH / 32; w += WAVE) L.kbits[w] = 0;
-            __syncthreads();
+            wsync();
             // ---- B: reduction, one lane per plane (reconstruction_helper.cuh:168-361)
             int nred = 0;
             bool clover = false;
@@ -1019,7 +1046,7 @@ STAGE bool stage_load_reduce(const Args& a, int64_t h0, int n) {
                 const int m2 = wave_sum((d >= 13 && d <= 18) ? nred : 0);
                 const int m3 = wave_sum((d >= 19 && d <= 24) ? nred : 0);
                 const int m4 = wave_sum((d >= 25 && d <= 30) ? nred : 0);
-                __syncthreads();
+                wsync();
                 if (lane == 0 && L.status == 0) {
                     uint32_t st = 0;
                     if (m0 > P.max_d0_mult) st |= OR_EV_D0_MULT;
@@ -1030,7 +1057,7 @@ STAGE bool stage_load_reduce(const Args& a, int64_t h0, int n) {
                     staged = true;
                 }
             }
-            __syncthreads();
+            wsync();
             // compact the kept hits in place (planes stay in detector order, so the
             // k-th kept hit of plane d lands at rstart[d] + k); destinations never
             // pass their sources, so chunks taken in order read before any write
@@ -1045,21 +1072,21 @@ STAGE bool stage_load_reduce(const Args& a, int64_t h0, int n) {
                 uint16_t og = 0;
                 if (kept) { el = L.elem[h]; ps = L.pos[h]; dr = L.drift[h]; og = L.orig[h]; }
                 const unsigned long long m = ballot(kept);
-                __syncthreads();
+                wsync();
                 if (kept) {
                     const int t = base + rank_in(m);
                     L.elem[t] = el; L.pos[t] = ps; L.drift[t] = dr; L.orig[t] = og;
                     atomicOr(&L.kbits[og >> 5], 1u << (og & 31));
                 }
                 base += __popcll(m);
-                __syncthreads();
+                wsync();
             }
         }
     }
     return __shfl(staged ? 1 : 0, 0, WAVE) != 0;
 }
 
-template <int MAXH>
+template <int MAXH, int WPE>
 STAGE void stage_keep_bits(const Args& a, int64_t h0, int n) {
     Lds<MAXH>& L = g_lds<MAXH>;
     const int lane = lane_id();
@@ -1086,10 +1113,10 @@ STAGE void stage_keep_bits(const Args& a, int64_t h0, int n) {
     }
 
 // C: binned X pairs for station 2, D3p, D3m
-template <int MAXH>
+template <int MAXH, int WPE>
 STAGE void stage_xpairs(or_params P, uint16_t* xpairs) {
     Lds<MAXH>& L = g_lds<MAXH>;
-    Ev<MAXH> E;
+    Ev<MAXH, WPE> E;
     const int lane = lane_id();
     // hitflag1/2[100] bound (reconstruction_kernels.cuh:225-226) on all six planes first
         const int pc = (lane < 6) ? L.rcnt[lane < 2 ? 16 - lane : (lane < 4 ? 24 - lane : 32 - lane)] : 0;
@@ -1103,7 +1130,7 @@ STAGE void stage_xpairs(or_params P, uint16_t* xpairs) {
             for (int b = 0; b < 28; ++b) L.xcnt[s][b] = 0;
             {
                 const float sp = spacing_plane(super_id(2 + s, 0));
-                const int cap = g_cfg.bin_cap;
+                const int cap = cfg().bin_cap;
                 auto put = [&](int c, int ia, int ib) {
                     const int b = xbin(s, c);
                     if (b < 0) return;
@@ -1139,14 +1166,14 @@ STAGE void stage_xpairs(or_params P, uint16_t* xpairs) {
                     if (L.xcnt[s][b] > P.max_pairs_per_bin) atomicOr(&L.status, (uint32_t)OR_EV_XZ_BIN_OVERFLOW);
             }
         }
-        __syncthreads();
+        wsync();
     }
 
 // D: XZ combinations in canonical order, slot order.  Returns the tracklet count.
-template <int MAXH>
+template <int MAXH, int WPE>
 STAGE int stage_xz(or_params P, Scr S) {
     Lds<MAXH>& L = g_lds<MAXH>;
-    Ev<MAXH> E;
+    Ev<MAXH, WPE> E;
     const int lane = lane_id();
     Tk* tks = S.tks();
     uint32_t* celloff = S.celloff();
@@ -1166,7 +1193,7 @@ STAGE int stage_xz(or_params P, Scr S) {
             if (c == 0) celloff[0] = 0;
             run += (uint32_t)__shfl(inc, WAVE - 1, WAVE);
         }
-        __syncthreads();
+        wsync();
         const uint32_t total = run;
         int np = 0, nm = 0;
         for (uint32_t g0 = 0; g0 < total; g0 += WAVE) {
@@ -1190,8 +1217,8 @@ STAGE int stage_xz(or_params P, Scr S) {
                 const int ix = (int)(g - celloff[c]);
                 const int ix2 = ix % nx2, ix3 = ix / nx2;
                 par = t & 1;
-                q2 = S.xpairs()[b2 * g_cfg.bin_cap + ix2];
-                q3 = S.xpairs()[(s3 * 28 + b3) * g_cfg.bin_cap + ix3];
+                q2 = S.xpairs()[b2 * cfg().bin_cap + ix2];
+                q3 = S.xpairs()[(s3 * 28 + b3) * cfg().bin_cap + ix3];
                 acc = xz_candidate(E, s3, q2, q3, nullptr);
             }
             const unsigned long long m = ballot(acc);
@@ -1201,7 +1228,7 @@ STAGE int stage_xz(or_params P, Scr S) {
                 // accepted: evaluate again writing straight into the tracklet slot
                 const int k = ntkl + rank_in(m);
                 const int kp = par == 0 ? np + rank_in(mp) : nm + rank_in(mm);
-                if (k < g_cfg.maxtkl) {
+                if (k < cfg().maxtkl) {
                     Tk& T = tks[k];
                     xz_candidate(E, s3, q2, q3, &T);
                     T.t = (uint8_t)t;
@@ -1217,7 +1244,7 @@ STAGE int stage_xz(or_params P, Scr S) {
         }
         if (lane == 0) {
             if (np > 16 * P.max_tracklets_per_slot || nm > 16 * P.max_tracklets_per_slot ||
-                ntkl >= P.max_tracklets_per_event || ntkl > g_cfg.maxtkl)
+                ntkl >= P.max_tracklets_per_event || ntkl > cfg().maxtkl)
                 atomicOr(&L.status, (uint32_t)OR_EV_TRACKLET_OVERFLOW);
         }
         // slot order: slot s = 2q + p holds items kp = q, q+16, ...
@@ -1227,12 +1254,12 @@ STAGE int stage_xz(or_params P, Scr S) {
             const int cs = (lane < 32 && npar > q) ? (npar - q + 15) / 16 : 0;
             const int inc = wave_scan(cs);
             if (lane < 32) L.sstart[lane] = inc - cs;
-            if (lane == 32) L.sstart[32] = ntkl < g_cfg.maxtkl ? ntkl : g_cfg.maxtkl;
+            if (lane == 32) L.sstart[32] = ntkl < cfg().maxtkl ? ntkl : cfg().maxtkl;
         }
-        __syncthreads();
+        wsync();
         if (L.status == 0)
             for (int k = lane; k < ntkl; k += WAVE) S.slot_order()[L.sstart[tks[k].slot] + tks[k].pos_in_slot] = (uint16_t)k;
-        __syncthreads();
+        wsync();
         return ntkl;
 }
 
@@ -1252,31 +1279,31 @@ STAGE int stage_xz(or_params P, Scr S) {
 // evaluated in parallel (yz_chunk_par).  yz_chunk_seq walks units one lane
 // each, in reference order, for chunks with more survivors than the list holds.
 
-template <int MAXH>
+template <int MAXH, int WPE>
 __device__ __forceinline__ void yz_lists_of(const Scr& S, int j, YzLists& W) {
     for (int v = 0; v < 4; ++v) {
-        W.l[v] = S.wl() + (j * 4 + v) * g_cfg.win_cap;
-        W.y[v] = S.wy() + (j * 4 + v) * g_cfg.win_cap;
-        W.f[v] = S.wf() + (j * 4 + v) * g_cfg.win_cap;
-        W.n[v] = g_lds<MAXH>.nl[j][v];
+        W.l[v] = S.wl() + (j * 4 + v) * cfg().win_cap;
+        W.y[v] = S.wy() + (j * 4 + v) * cfg().win_cap;
+        W.f[v] = S.wf() + (j * 4 + v) * cfg().win_cap;
+        W.n[v] = yzw<MAXH>().nl[j][v];
     }
 }
 
 struct YzItem { int j, local, iu3, iv3, iu2, iv2; };
 
 // candidate index within the chunk -> tracklet and list entries
-template <int MAXH>
+template <int MAXH, int WPE>
 __device__ __forceinline__ YzItem yz_decode(int i) {
-    const Lds<MAXH>& L = g_lds<MAXH>;
+    const YzWave& Y = yzw<MAXH>();
     YzItem it;
     int j = 0;
-    while (L.uoff[j + 1] <= i) ++j;
+    while (Y.uoff[j + 1] <= i) ++j;
     it.j = j;
-    it.local = i - L.uoff[j];
-    const int n0 = L.nl[j][0], nin = n0 * L.nl[j][1];
+    it.local = i - Y.uoff[j];
+    const int n0 = Y.nl[j][0], nin = n0 * Y.nl[j][1];
     const int iuv3 = it.local / nin, iuv2 = it.local - iuv3 * nin;
     it.iu2 = iuv2 % n0; it.iv2 = iuv2 / n0;
-    it.iu3 = iuv3 % L.nl[j][2]; it.iv3 = iuv3 / L.nl[j][2];
+    it.iu3 = iuv3 % Y.nl[j][2]; it.iv3 = iuv3 / Y.nl[j][2];
     return it;
 }
 
@@ -1324,11 +1351,11 @@ __device__ __forceinline__ int wave_max_scan(int v) {
 }
 
 // candidate slots and first fit of candidate i of the chunk
-template <int MAXH>
-__device__ __forceinline__ void yz_candidate(const Ev<MAXH>& E, const Scr& S, const Tk* Tp, const YzItem& it,
+template <int MAXH, int WPE>
+__device__ __forceinline__ void yz_candidate(const Ev<MAXH, WPE>& E, const Scr& S, const Tk* Tp, const YzItem& it,
                                              YzSlots& H, Fit& f) {
     YzLists W;
-    yz_lists_of<MAXH>(S, it.j, W);
+    yz_lists_of<MAXH, WPE>(S, it.j, W);
     yz_fit(W, Tp->st3, it, f);
     yz_dets(H, Tp->st3);
     H.pres = 0;
@@ -1353,9 +1380,9 @@ __device__ __forceinline__ void yz_candidate(const Ev<MAXH>& E, const Scr& S, co
 // the nearest U2 minus nearest V2 y, the distance of the nearest P1 Y and P2 Y
 // prop-tube hit to the predicted y, and the candidate's |y0|/50 (distances in
 // units of 10 cm; every feature clamped at 10).
-template <int MAXH>
+template <int MAXH, int WPE>
 __device__ __forceinline__ UnitFeatures unit_features(const Scr& S, const Tk& T, int j, int iuv3, const int4 nl) {
-    const Ev<MAXH> E;
+    const Ev<MAXH, WPE> E;
     const int win = S.win();
     const int iu3 = iuv3 % nl.z, iv3 = iuv3 / nl.z;
     const int qu = (j * 4 + 2) * win + iu3, qv = (j * 4 + 3) * win + iv3;
@@ -1470,7 +1497,7 @@ __device__ __forceinline__ bool yz_first_cut(const Scr& S, int j, const int (&e)
     float4 yv[4];
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-        const int q = (j * 4 + v) * g_cfg.win_cap + e[v];
+        const int q = (j * 4 + v) * cfg().win_cap + e[v];
         fl[v] = gld(S.wf() + q);
         yv[v] = gld4(S.wy() + q);
     }
@@ -1513,9 +1540,10 @@ struct UnitScore {
     int64_t dbase;   // first unit row of the chunk, -1: none written
 };
 
-template <int MAXH>
+template <int MAXH, int WPE>
 __device__ __attribute__((noinline)) UnitScore yz_score_units(Scr S, Prn R, int kb, int ntkl) {
     Lds<MAXH>& L = g_lds<MAXH>;
+    YzWave& Y = yzw<MAXH>();
     const int lane = lane_id();
     Tk* tks = S.tks();
     UnitScore res = {-1, 0u, -1};
@@ -1524,7 +1552,7 @@ __device__ __attribute__((noinline)) UnitScore yz_score_units(Scr S, Prn R, int 
     {
         int units = 0;
         if (lane < YZ_CHUNK && kb + lane < ntkl) {
-            const int* nl = L.nl[lane];
+            const int* nl = Y.nl[lane];
             if (nl[0] > 0 && nl[1] > 0 && nl[2] > 0 && nl[3] > 0) units = nl[2] * nl[3];
         }
         const int inc = wave_scan(units);
@@ -1533,7 +1561,7 @@ __device__ __attribute__((noinline)) UnitScore yz_score_units(Scr S, Prn R, int 
         for (int t = 1; t <= YZ_CHUNK; ++t) uo[t] = __builtin_amdgcn_readfirstlane(__shfl(inc, t - 1, WAVE));
     }
     const int U = uo[YZ_CHUNK];
-    if (U > g_cfg.unit_cap) return res;
+    if (U > cfg().unit_cap) return res;
     if (R.dump) {
         unsigned long long b = 0;
         if (lane == 0) b = atomicAdd(R.dump_count, (unsigned long long)U);
@@ -1546,7 +1574,7 @@ __device__ __attribute__((noinline)) UnitScore yz_score_units(Scr S, Prn R, int 
     // fills only later), each tracklet's best score in LDS
     const bool rel = R.prune && R.prune->margin < INFINITY;
     float* sc = reinterpret_cast<float*>(S.ycache());
-    if (lane < YZ_CHUNK) L.umax[lane] = INT_MIN;
+    if (lane < YZ_CHUNK) Y.umax[lane] = INT_MIN;
     wsync();
     for (int u0 = 0; u0 < U; u0 += WAVE) {
         const int u = u0 + lane;
@@ -1561,12 +1589,12 @@ __device__ __attribute__((noinline)) UnitScore yz_score_units(Scr S, Prn R, int 
         UnitFeatures f;
 #pragma unroll
         for (int q = 0; q < PRUNE_NF; ++q) f.v[q] = 0.f;
-        if (valid) f = unit_features<MAXH>(S, tks[kb + j], j, iuv3, *reinterpret_cast<const int4*>(L.nl[j]));
+        if (valid) f = unit_features<MAXH, WPE>(S, tks[kb + j], j, iuv3, *reinterpret_cast<const int4*>(Y.nl[j]));
         if (R.prune) {
             const float score = prune_score(R.prune, f);
             if (valid) {
                 sc[u] = score;
-                if (rel) atomicMax(&L.umax[j], float_order(score));
+                if (rel) atomicMax(&Y.umax[j], float_order(score));
             }
         }
         if (res.dbase >= 0 && valid) {
@@ -1598,11 +1626,11 @@ __device__ __attribute__((noinline)) UnitScore yz_score_units(Scr S, Prn R, int 
         int nin = 0;
         bool keep = valid;
         if (valid) {
-            const int4 nl = *reinterpret_cast<const int4*>(L.nl[j]);
+            const int4 nl = *reinterpret_cast<const int4*>(Y.nl[j]);
             nin = nl.x * nl.y;
             if (R.prune) {
                 const float score = sc[u];
-                keep = score >= R.prune->thr && (!rel || score >= float_unorder(L.umax[j]) - R.prune->margin);
+                keep = score >= R.prune->thr && (!rel || score >= float_unorder(Y.umax[j]) - R.prune->margin);
             }
         }
         const unsigned long long km = ballot(keep);
@@ -1616,7 +1644,7 @@ __device__ __attribute__((noinline)) UnitScore yz_score_units(Scr S, Prn R, int 
         nk += __popcll(km);
         run += (uint32_t)__shfl(inc, WAVE - 1, WAVE);
     }
-    __syncthreads();
+    wsync();
     OR_SUB(9);
     OR_COUNT(11, U);
     OR_COUNT(12, nk);
@@ -1641,15 +1669,16 @@ __device__ __forceinline__ int lower_bound_u32(const uint32_t* p, int n, uint32_
 // index cursor (the start of kept unit lo0).  Returns the survivors listed; on
 // return cursor and lo0 name the next batch's first unit (cursor == run: done).
 // -1: the unit at cursor alone has more survivors than the list holds.
-template <int MAXH>
+template <int MAXH, int WPE>
 __device__ __attribute__((noinline)) int yz_pass1_kept(Scr S, int kb, int ntkl, int cap, int nk, uint32_t run,
                                                        uint32_t& cursor, int& lo_unit) {
     Lds<MAXH>& L = g_lds<MAXH>;
+    YzWave& Y = yzw<MAXH>();
     const int lane = lane_id();
     Tk* tks = S.tks();
     int off[YZ_CHUNK + 1];
 #pragma unroll
-    for (int t = 0; t <= YZ_CHUNK; ++t) off[t] = __builtin_amdgcn_readfirstlane(L.uoff[t]);
+    for (int t = 0; t <= YZ_CHUNK; ++t) off[t] = __builtin_amdgcn_readfirstlane(Y.uoff[t]);
     const unsigned d3m = (unsigned)ballot(lane < YZ_CHUNK && kb + lane < ntkl && tks[kb + lane].st3 == 4);
     float zv[4][2][2];
     yz_plane_z(zv);
@@ -1687,7 +1716,7 @@ __device__ __attribute__((noinline)) int yz_pass1_kept(Scr S, int kb, int ntkl, 
         if (i < run) {
             const int j = (int)(code >> 16), iuv3 = (int)(code & 0xFFFFu);
             const int inner = (int)(i - st_t);
-            const int4 nl = *reinterpret_cast<const int4*>(L.nl[j]);
+            const int4 nl = *reinterpret_cast<const int4*>(Y.nl[j]);
             int ojs = off[0];
 #pragma unroll
             for (int t = 1; t < YZ_CHUNK; ++t) ojs = j == t ? off[t] : ojs;
@@ -1707,7 +1736,7 @@ __device__ __attribute__((noinline)) int yz_pass1_kept(Scr S, int kb, int ntkl, 
             if (ucut_k == start) return -1;
             if (pass && iorig < ucut_o) S.plist()[np + rank_in(m)] = (uint32_t)iorig;
             const int nb = __popcll(ballot(pass && iorig < ucut_o));
-            __syncthreads();   // the list writes above are read by other lanes below
+            wsync();   // the list writes above are read by other lanes below
             // survivors listed before this iteration that belong to the cut unit go too
             const int keep = lower_bound_u32(S.plist(), np + nb, (uint32_t)ucut_o);
             cursor = ucut_k;
@@ -1727,14 +1756,15 @@ __device__ __attribute__((noinline)) int yz_pass1_kept(Scr S, int kb, int ntkl, 
 // registers (candidate offsets, which tracklets are D3m, the plane z values of
 // both station-3 choices) and a candidate's four list entries are loaded
 // together.  Returns as yz_pass1_kept.
-template <int MAXH>
+template <int MAXH, int WPE>
 __device__ __attribute__((noinline)) int yz_pass1_plain(Scr S, int kb, int ntkl, int cap, int I, int& cursor) {
     Lds<MAXH>& L = g_lds<MAXH>;
+    YzWave& Y = yzw<MAXH>();
     const int lane = lane_id();
     Tk* tks = S.tks();
     int off[YZ_CHUNK + 1];
 #pragma unroll
-    for (int t = 0; t <= YZ_CHUNK; ++t) off[t] = __builtin_amdgcn_readfirstlane(L.uoff[t]);
+    for (int t = 0; t <= YZ_CHUNK; ++t) off[t] = __builtin_amdgcn_readfirstlane(Y.uoff[t]);
     const unsigned d3m = (unsigned)ballot(lane < YZ_CHUNK && kb + lane < ntkl && tks[kb + lane].st3 == 4);
     float zv[4][2][2];
     yz_plane_z(zv);
@@ -1748,7 +1778,7 @@ __device__ __attribute__((noinline)) int yz_pass1_plain(Scr S, int kb, int ntkl,
             int j = 0;
 #pragma unroll
             for (int t = 1; t < YZ_CHUNK; ++t) j += i >= off[t] ? 1 : 0;
-            const int4 nl = *reinterpret_cast<const int4*>(L.nl[j]);
+            const int4 nl = *reinterpret_cast<const int4*>(Y.nl[j]);
             int ojs = off[0];
 #pragma unroll
             for (int t = 1; t < YZ_CHUNK; ++t) ojs = j == t ? off[t] : ojs;
@@ -1766,7 +1796,7 @@ __device__ __attribute__((noinline)) int yz_pass1_plain(Scr S, int kb, int ntkl,
             if (ucut == start) return -1;
             if (pass && i < ucut) S.plist()[np + rank_in(m)] = (uint32_t)i;
             const int nb = __popcll(ballot(pass && i < ucut));
-            __syncthreads();   // the list writes above are read by other lanes below
+            wsync();   // the list writes above are read by other lanes below
             const int keep = lower_bound_u32(S.plist(), np + nb, (uint32_t)ucut);
             cursor = ucut;
             return keep;
@@ -1795,8 +1825,8 @@ __device__ __forceinline__ void yz_cache_put(uint4* c, const YzSlots& H, const F
 
 // The candidate exactly as yz_candidate builds it, from the cache: slot
 // detectors of the station-3 choice, hits of the present slots from LDS, signs 0.
-template <int MAXH>
-__device__ __forceinline__ void yz_cache_get(const Ev<MAXH>& E, const uint4* c, int st3, YzSlots& H, Fit& f) {
+template <int MAXH, int WPE>
+__device__ __forceinline__ void yz_cache_get(const Ev<MAXH, WPE>& E, const uint4* c, int st3, YzSlots& H, Fit& f) {
     const uint4 a = c[0], b = c[1], d = c[2];
     const uint32_t r[4] = {a.x, a.y, a.z, a.w};
     yz_dets(H, st3);
@@ -1820,32 +1850,33 @@ __device__ __forceinline__ void yz_cache_get(const Ev<MAXH>& E, const uint4* c, 
 // Parallel form over batches of first-box survivors (see yz_pass1_kept).
 // Returns false (nothing written) when one unit alone overflows the list; the
 // chunk then takes the unit-by-unit path.
-template <int MAXH>
+template <int MAXH, int WPE>
 __device__ __attribute__((noinline)) bool yz_chunk_par(Scr S, Prn R, int kb, int ntkl, int cap) {
     Lds<MAXH>& L = g_lds<MAXH>;
-    Ev<MAXH> E;
+    YzWave& Y = yzw<MAXH>();
+    Ev<MAXH, WPE> E;
     const int lane = lane_id();
     Tk* tks = S.tks();
     {
         const int j = lane, k = kb + j;
         int items = 0;
         if (j < YZ_CHUNK && k < ntkl) {
-            const int* nl = L.nl[j];
+            const int* nl = Y.nl[j];
             // at most YZ_ENUM_MAX each (stage_yz), so the sum stays below 2^31
             if (nl[0] > 0 && nl[1] > 0 && nl[2] > 0 && nl[3] > 0) items = nl[0] * nl[1] * nl[2] * nl[3];
         }
         const int inc = wave_scan(items);
         if (j < YZ_CHUNK) {
-            L.best[j] = ~0ull;
-            L.uoff[j + 1] = inc;
+            Y.best[j] = ~0ull;
+            Y.uoff[j + 1] = inc;
         }
-        if (j == 0) L.uoff[0] = 0;
+        if (j == 0) Y.uoff[0] = 0;
     }
-    __syncthreads();
-    const int I = L.uoff[YZ_CHUNK];
+    wsync();
+    const int I = Y.uoff[YZ_CHUNK];
     OR_COUNT(10, I);
     UnitScore us = {-1, 0u, -1};
-    if (R.prune || R.dump) us = yz_score_units<MAXH>(S, R, kb, ntkl);
+    if (R.prune || R.dump) us = yz_score_units<MAXH, WPE>(S, R, kb, ntkl);
     const bool kept = us.nk >= 0;
     const int64_t dbase = us.dbase;
     uint32_t cursor = 0;
@@ -1853,14 +1884,14 @@ __device__ __attribute__((noinline)) bool yz_chunk_par(Scr S, Prn R, int kb, int
     unsigned long long saved = ~0ull;   // lane j < 16: best key of tracklet j already in its record
     while (kept ? cursor < us.run : icursor < I) {
         OR_SUB_START
-        const int np = kept ? yz_pass1_kept<MAXH>(S, kb, ntkl, cap, us.nk, us.run, cursor, lo_unit)
-                            : yz_pass1_plain<MAXH>(S, kb, ntkl, cap, I, icursor);
+        const int np = kept ? yz_pass1_kept<MAXH, WPE>(S, kb, ntkl, cap, us.nk, us.run, cursor, lo_unit)
+                            : yz_pass1_plain<MAXH, WPE>(S, kb, ntkl, cap, I, icursor);
         if (np < 0) {
             // the chunk goes the unit-by-unit way: its unit rows carry no winner flags
             if (dbase >= 0) {
                 int units = 0;
                 if (lane < YZ_CHUNK && kb + lane < ntkl) {
-                    const int* nl = L.nl[lane];
+                    const int* nl = Y.nl[lane];
                     if (nl[0] > 0 && nl[1] > 0 && nl[2] > 0 && nl[3] > 0) units = nl[2] * nl[3];
                 }
                 const int U = wave_sum(units);
@@ -1868,7 +1899,7 @@ __device__ __attribute__((noinline)) bool yz_chunk_par(Scr S, Prn R, int kb, int
             }
             return false;
         }
-        __syncthreads();
+        wsync();
         OR_SUB(1);
         OR_COUNT(14, np);
         // pass 2: per survivor, first-of-unit flag; the first of a unit runs the
@@ -1879,11 +1910,11 @@ __device__ __attribute__((noinline)) bool yz_chunk_par(Scr S, Prn R, int kb, int
         for (int k0 = 0; k0 < np; k0 += WAVE) {
             const int k = k0 + lane;
             if (k < np) {
-                const YzItem it = yz_decode<MAXH>((int)S.plist()[k]);
-                const int nin = L.nl[it.j][0] * L.nl[it.j][1];
+                const YzItem it = yz_decode<MAXH, WPE>((int)S.plist()[k]);
+                const int nin = Y.nl[it.j][0] * Y.nl[it.j][1];
                 bool first = k == 0;
                 if (!first) {
-                    const YzItem pt = yz_decode<MAXH>((int)S.plist()[k - 1]);
+                    const YzItem pt = yz_decode<MAXH, WPE>((int)S.plist()[k - 1]);
                     first = pt.j != it.j || (pt.local / nin) != (it.local / nin);
                 }
                 const Tk* Tp = &tks[kb + it.j];
@@ -1905,7 +1936,7 @@ __device__ __attribute__((noinline)) bool yz_chunk_par(Scr S, Prn R, int kb, int
                 S.pst()[k] = ps;
             }
         }
-        __syncthreads();
+        wsync();
         OR_SUB(2);
         // pass 3: starting signs by scan (the unit's seed, and the last writer of
         // the v3' sign before this candidate) and the tests that need no sign; the
@@ -1926,7 +1957,7 @@ __device__ __attribute__((noinline)) bool yz_chunk_par(Scr S, Prn R, int kb, int
             carry_w = max(carry_w, __shfl(wincl, WAVE - 1, WAVE));
             bool go = false;
             if (valid) {
-                const YzItem it = yz_decode<MAXH>((int)S.plist()[k]);
+                const YzItem it = yz_decode<MAXH, WPE>((int)S.plist()[k]);
                 const uint4 b = S.ycache()[3 * k + 1];
                 Fit f;
                 f.p0 = __uint_as_float(b.y);
@@ -1947,12 +1978,12 @@ __device__ __attribute__((noinline)) bool yz_chunk_par(Scr S, Prn R, int kb, int
             if (go) S.qlist()[nq + rank_in(m)] = (uint32_t)k;
             nq += __popcll(m);
         }
-        __syncthreads();
+        wsync();
         for (int q0 = 0; q0 < nq; q0 += WAVE) {
             const int q = q0 + lane;
             if (q < nq) {
                 const int k = (int)S.qlist()[q];
-                const YzItem it = yz_decode<MAXH>((int)S.plist()[k]);
+                const YzItem it = yz_decode<MAXH, WPE>((int)S.plist()[k]);
                 const uchar4 ps = S.pst()[k];
                 const Tk* Tp = &tks[kb + it.j];
                 YzSlots H;
@@ -1967,17 +1998,17 @@ __device__ __attribute__((noinline)) bool yz_chunk_par(Scr S, Prn R, int kb, int
                 if (chi2 < 10000.1f) {
                     const unsigned long long key =
                         ((unsigned long long)__float_as_uint(chi2) << 32) | (unsigned long long)(uint32_t)it.local;
-                    atomicMin(&L.best[it.j], key);
+                    atomicMin(&Y.best[it.j], key);
                 }
             }
         }
-        __syncthreads();
+        wsync();
         // a tracklet whose best improved in this batch keeps that candidate's
         // cache entry and sign state, since the next batch reuses the lists
         if (lane < YZ_CHUNK && kb + lane < ntkl) {
-            const unsigned long long key = L.best[lane];
+            const unsigned long long key = Y.best[lane];
             if (key != saved) {
-                const uint32_t iw = (uint32_t)L.uoff[lane] + (uint32_t)key;
+                const uint32_t iw = (uint32_t)Y.uoff[lane] + (uint32_t)key;
                 const int k = min(lower_bound_u32(S.plist(), np, iw), np - 1);   // present: set in this batch
                 uint4* br = S.brec() + 4 * lane;
                 const uint4* c = S.ycache() + 3 * k;
@@ -1988,7 +2019,7 @@ __device__ __attribute__((noinline)) bool yz_chunk_par(Scr S, Prn R, int kb, int
                 saved = key;
             }
         }
-        __syncthreads();
+        wsync();
         OR_SUB(3);
     }
     OR_SUB_START
@@ -1997,7 +2028,7 @@ __device__ __attribute__((noinline)) bool yz_chunk_par(Scr S, Prn R, int kb, int
     if (dbase >= 0) {
         int units = 0;
         if (lane < YZ_CHUNK && kb + lane < ntkl) {
-            const int* nl = L.nl[lane];
+            const int* nl = Y.nl[lane];
             if (nl[0] > 0 && nl[1] > 0 && nl[2] > 0 && nl[3] > 0) units = nl[2] * nl[3];
         }
         uo_lane = wave_scan(units) - units;
@@ -2005,7 +2036,7 @@ __device__ __attribute__((noinline)) bool yz_chunk_par(Scr S, Prn R, int kb, int
     if (lane < YZ_CHUNK && kb + lane < ntkl) {
         const int j = lane;
         Tk* Tp = &tks[kb + j];
-        const unsigned long long key = L.best[j];
+        const unsigned long long key = Y.best[j];
         if (key != ~0ull) {
             const uint4* br = S.brec() + 4 * j;
             const uint32_t psw = br[3].x;
@@ -2016,7 +2047,7 @@ __device__ __attribute__((noinline)) bool yz_chunk_par(Scr S, Prn R, int kb, int
             float chi2;
             yz_tail(E, Tp, H, f, chi2, Tp);
             const int local = (int)(uint32_t)key;
-            if (dbase >= 0) R.dump[(dbase + uo_lane + local / (L.nl[j][0] * L.nl[j][1])) * OR_UNIT_ROW + 3] = 1.f;
+            if (dbase >= 0) R.dump[(dbase + uo_lane + local / (Y.nl[j][0] * Y.nl[j][1])) * OR_UNIT_ROW + 3] = 1.f;
         } else {
             Tp->stid = 3;
         }
@@ -2026,10 +2057,11 @@ __device__ __attribute__((noinline)) bool yz_chunk_par(Scr S, Prn R, int kb, int
 }
 
 // Unit loop: one lane per (tracklet, u3.v3) unit, candidates in reference order.
-template <int MAXH>
+template <int MAXH, int WPE>
 __device__ __attribute__((noinline)) void yz_chunk_seq(Scr S, int kb, int ntkl) {
     Lds<MAXH>& L = g_lds<MAXH>;
-    Ev<MAXH> E;
+    YzWave& Y = yzw<MAXH>();
+    Ev<MAXH, WPE> E;
     const int lane = lane_id();
     Tk* tks = S.tks();
     uint16_t* wl = S.wl();
@@ -2039,28 +2071,28 @@ __device__ __attribute__((noinline)) void yz_chunk_seq(Scr S, int kb, int ntkl) 
         const int j = lane, k = kb + j;
         int u = 0;
         if (j < YZ_CHUNK && k < ntkl) {
-            const int* nl = L.nl[j];
+            const int* nl = Y.nl[j];
             if (nl[0] > 0 && nl[1] > 0 && nl[2] > 0 && nl[3] > 0) u = nl[2] * nl[3];
         }
         const int inc = wave_scan(u);
         if (j < YZ_CHUNK) {
-            L.best[j] = ~0ull;
-            L.uoff[j + 1] = inc;
+            Y.best[j] = ~0ull;
+            Y.uoff[j + 1] = inc;
         }
-        if (j == 0) L.uoff[0] = 0;
+        if (j == 0) Y.uoff[0] = 0;
     }
-    __syncthreads();
-    const int U = L.uoff[YZ_CHUNK];
+    wsync();
+    const int U = Y.uoff[YZ_CHUNK];
     for (int u0 = 0; u0 < U; u0 += WAVE) {
         const int u = u0 + lane;
         if (u < U) {
             int j = 0;
-            while (L.uoff[j + 1] <= u) ++j;
-            const int iuv3 = u - L.uoff[j];
+            while (Y.uoff[j + 1] <= u) ++j;
+            const int iuv3 = u - Y.uoff[j];
             YzLists W;
             for (int v = 0; v < 4; ++v) {
-                W.l[v] = wl + (j * 4 + v) * g_cfg.win_cap; W.y[v] = wy + (j * 4 + v) * g_cfg.win_cap;
-                W.f[v] = wf + (j * 4 + v) * g_cfg.win_cap; W.n[v] = L.nl[j][v];
+                W.l[v] = wl + (j * 4 + v) * cfg().win_cap; W.y[v] = wy + (j * 4 + v) * cfg().win_cap;
+                W.f[v] = wf + (j * 4 + v) * cfg().win_cap; W.n[v] = Y.nl[j][v];
             }
             float bc = 10000.1f;
             int bi = -1;
@@ -2068,20 +2100,20 @@ __device__ __attribute__((noinline)) void yz_chunk_seq(Scr S, int kb, int ntkl) 
                 const unsigned long long key =
                     ((unsigned long long)__float_as_uint(bc) << 32) |
                     (unsigned long long)(uint32_t)(iuv3 * (W.n[0] * W.n[1]) + bi);
-                atomicMin(&L.best[j], key);
+                atomicMin(&Y.best[j], key);
             }
         }
     }
-    __syncthreads();
+    wsync();
     if (lane < YZ_CHUNK && kb + lane < ntkl) {
         const int j = lane;
         Tk* Tp = &tks[kb + j];
-        const unsigned long long key = L.best[j];
+        const unsigned long long key = Y.best[j];
         if (key != ~0ull) {
             YzLists W;
             for (int v = 0; v < 4; ++v) {
-                W.l[v] = wl + (j * 4 + v) * g_cfg.win_cap; W.y[v] = wy + (j * 4 + v) * g_cfg.win_cap;
-                W.f[v] = wf + (j * 4 + v) * g_cfg.win_cap; W.n[v] = L.nl[j][v];
+                W.l[v] = wl + (j * 4 + v) * cfg().win_cap; W.y[v] = wy + (j * 4 + v) * cfg().win_cap;
+                W.f[v] = wf + (j * 4 + v) * cfg().win_cap; W.n[v] = Y.nl[j][v];
             }
             const int flat = (int)(uint32_t)key, nin = W.n[0] * W.n[1];
             float bc = 0.f;
@@ -2093,17 +2125,23 @@ __device__ __attribute__((noinline)) void yz_chunk_seq(Scr S, int kb, int ntkl) 
     }
 }
 
-template <int MAXH>
+template <int MAXH, int WPE>
 STAGE void stage_yz(or_params P, Scr S, Prn R, int ntkl, int list_cap) {
     Lds<MAXH>& L = g_lds<MAXH>;
-    Ev<MAXH> E;
+    YzWave& Y = yzw<MAXH>();
+    Ev<MAXH, WPE> E;
     const int lane = lane_id();
     Tk* tks = S.tks();
     uint16_t* wl = S.wl();
     float4* wy = S.wy();
     uint8_t* wf = S.wf();
-    const int capw = g_cfg.win_cap;
-    for (int kb = 0; kb < ntkl; kb += YZ_CHUNK) {
+    const int capw = cfg().win_cap;
+    // chunks of YZ_CHUNK tracklets, handed to the event's waves as they come free
+    for (;;) {
+        int kb = 0;
+        if (lane == 0) kb = atomicAdd(&L.yz_next, YZ_CHUNK);
+        kb = __builtin_amdgcn_readfirstlane(__shfl(kb, 0, WAVE));
+        if (kb >= ntkl) break;
         OR_SUB_START
         // window lists: lane = (tracklet j, view v)
         {
@@ -2118,7 +2156,7 @@ STAGE void stage_yz(or_params P, Scr S, Prn R, int ntkl, int list_cap) {
                 const float za = G(F_Z, da), zb = G(F_Z, db);
                 const float xmin = fminf((T.tx - T.etx) * za, (T.tx - T.etx) * zb) + T.x0 - T.ex0 - G(F_SPACING, da);
                 const float xmax = fmaxf((T.tx + T.etx) * za, (T.tx + T.etx) * zb) + T.x0 + T.ex0 + G(F_SPACING, da);
-                uint16_t* lst = wl + (j * 4 + v) * g_cfg.win_cap;
+                uint16_t* lst = wl + (j * 4 + v) * cfg().win_cap;
                 const int nl = pairs_window(E, st, pj, xmin, xmax, lst, P.max_pair_plane_hits, capw);
                 if (nl < 0) atomicOr(&L.status, (uint32_t)OR_EV_PAIR_OVERFLOW);
                 raw = nl > 0 ? (unsigned long long)nl : 0ull;
@@ -2143,28 +2181,28 @@ STAGE void stage_yz(or_params P, Scr S, Prn R, int ntkl, int list_cap) {
                     }
                     if (fl) {
                         lst[nacc] = lst[q];
-                        wy[(j * 4 + v) * g_cfg.win_cap + nacc] = yv;
-                        wf[(j * 4 + v) * g_cfg.win_cap + nacc] = fl;
+                        wy[(j * 4 + v) * cfg().win_cap + nacc] = yv;
+                        wf[(j * 4 + v) * cfg().win_cap + nacc] = fl;
                         ++nacc;
                     }
                 }
-                L.nl[j][v] = nacc;
+                Y.nl[j][v] = nacc;
             }
             // enumeration bound: u2 x v2 x u3 x v3 list entries of one tracklet
             raw *= (unsigned long long)__shfl_xor((long long)raw, 1, WAVE);
             raw *= (unsigned long long)__shfl_xor((long long)raw, 2, WAVE);
             if (raw > (unsigned long long)YZ_ENUM_MAX) atomicOr(&L.status, (uint32_t)OR_EV_PAIR_OVERFLOW);
         }
-        __syncthreads();
+        wsync();
         OR_SUB(0);
-        if (L.status) break;
-        if (!yz_chunk_par<MAXH>(S, R, kb, ntkl, list_cap)) {
-            __syncthreads();
+        if (__atomic_load_n(&L.status, __ATOMIC_RELAXED)) break;   // set by any wave: the event is rejected
+        if (!yz_chunk_par<MAXH, WPE>(S, R, kb, ntkl, list_cap)) {
+            wsync();
             OR_SUB_START
-            yz_chunk_seq<MAXH>(S, kb, ntkl);
+            yz_chunk_seq<MAXH, WPE>(S, kb, ntkl);
             OR_SUB(5);
         }
-        __syncthreads();
+        wsync();
     }
 }
 
@@ -2215,7 +2253,7 @@ __device__ __forceinline__ int clean_shared9(const uint32_t (&a)[9], const uint3
     return c;
 }
 
-template <int MAXH>
+template <int MAXH, int WPE>
 __device__ __attribute__((noinline)) void clean_many(Tk* tks, const uint16_t* slot_order, int ntkl, int nlive) {
     Lds<MAXH>& L = g_lds<MAXH>;
     const int lane = lane_id();
@@ -2225,7 +2263,7 @@ __device__ __attribute__((noinline)) void clean_many(Tk* tks, const uint16_t* sl
         const int b0 = w * 64;
         L.cl.alive[w] = nlive >= b0 + 64 ? ~0ull : (nlive > b0 ? ((1ull << (nlive - b0)) - 1ull) : 0ull);
     }
-    __syncthreads();
+    wsync();
     // entries in slot order, live tracklets per slot
     int np = 0;
     for (int j0 = 0; j0 < ntkl; j0 += WAVE) {
@@ -2258,14 +2296,14 @@ __device__ __attribute__((noinline)) void clean_many(Tk* tks, const uint16_t* sl
         }
         np += __popcll(m);
     }
-    __syncthreads();
+    wsync();
     {
         const int c = lane < 33 ? L.cl.lstart[lane] : 0;
         const int inc = wave_scan(c);
-        __syncthreads();
+        wsync();
         if (lane < 33) L.cl.lstart[lane] = inc;
     }
-    __syncthreads();
+    wsync();
     // one i-step: A at live position pa against the alive positions of [jb, je)
     auto step = [&](int pa, int jb, int je) {
         const CleanEnt A = clean_load(ent, pa);
@@ -2287,13 +2325,13 @@ __device__ __attribute__((noinline)) void clean_many(Tk* tks, const uint16_t* sl
                 if (A.ratio < B.ratio) dj = (float)common > (float)na * 0.3333f;
                 else dl = (float)common > (float)(int)(B.kn >> 24) * 0.3333f;
             }
-            __syncthreads();
+            wsync();
             if (dj) atomicAnd(&L.cl.alive[j >> 6], ~(1ull << (j & 63)));
             di = di || ballot(dl) != 0ull;
             carry = __shfl(common, WAVE - 1, WAVE);
         }
         if (di && lane == 0) atomicAnd(&L.cl.alive[pa >> 6], ~(1ull << (pa & 63)));
-        __syncthreads();
+        wsync();
     };
     // within each slot (reconstruction_helper.cuh:1789-1843)
     for (int pa = 0; pa < nlive; ++pa) {
@@ -2316,10 +2354,10 @@ __device__ __attribute__((noinline)) void clean_many(Tk* tks, const uint16_t* sl
     }
     for (int p = lane; p < nlive; p += WAVE)
         if (!((L.cl.alive[p >> 6] >> (p & 63)) & 1ull)) tks[gld4(ent + 3 * p + 2).y & 0xFFFFu].stid = 3;
-    __syncthreads();
+    wsync();
 }
 
-template <int MAXH>
+template <int MAXH, int WPE>
 STAGE void stage_clean(Tk* tks, const uint16_t* slot_order, int ntkl) {
     Lds<MAXH>& L = g_lds<MAXH>;
     const int lane = lane_id();
@@ -2333,7 +2371,7 @@ STAGE void stage_clean(Tk* tks, const uint16_t* slot_order, int ntkl) {
         if (live && nlive + rank_in(m) < WAVE) L.live[nlive + rank_in(m)] = (uint16_t)k;
         nlive += __popcll(m);
     }
-    __syncthreads();
+    wsync();
     if (nlive == 0) return;
     if (nlive <= WAVE) {
         const bool mine = lane < nlive;
@@ -2403,11 +2441,11 @@ STAGE void stage_clean(Tk* tks, const uint16_t* slot_order, int ntkl) {
             }
         }
         if (mine && !((alive >> lane) & 1ull)) tks[k].stid = 3;
-        __syncthreads();
+        wsync();
         return;
     }
     if (nlive <= WAVE * CLEAN_WORDS) {
-        clean_many<MAXH>(tks, slot_order, ntkl, nlive);
+        clean_many<MAXH, WPE>(tks, slot_order, ntkl, nlive);
         return;
     }
     if (lane == 0) {
@@ -2442,7 +2480,7 @@ STAGE void stage_clean(Tk* tks, const uint16_t* slot_order, int ntkl) {
             for (int o = 0; o < 32; ++o)
                 if (o != s) pass(L.sstart[s], L.sstart[s + 1], 0, L.sstart[o], L.sstart[o + 1]);
     }
-    __syncthreads();
+    wsync();
 }
 
 // G: station-1 extension (reconstruction_kernels.cuh:1841-2534), one track at a
@@ -2488,8 +2526,8 @@ __device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long v)
 
 // window list of view v (0 X, 1 U, 2 V) of station 1 from the sagitta ratio
 // (reconstruction_helper.cuh:556-613); -1 on overflow
-template <int MAXH>
-__device__ __forceinline__ int global_window(const Ev<MAXH>& E, const Tk* T, int v, uint16_t* out, int plane_cap,
+template <int MAXH, int WPE>
+__device__ __forceinline__ int global_window(const Ev<MAXH, WPE>& E, const Tk* T, int v, uint16_t* out, int plane_cap,
                                              int capw) {
     const int n23 = T->nx + T->nuv;
     const float x0 = T->x0, tx = T->tx, y0 = T->y0, ty = T->ty;
@@ -2529,8 +2567,8 @@ struct GX {
     float xdr[2], xpos[2];
 };
 
-template <int MAXH>
-__device__ __forceinline__ bool global_xpart(const Ev<MAXH>& E, uint16_t qx, float x0, float tx, float ex0, float etx,
+template <int MAXH, int WPE>
+__device__ __forceinline__ bool global_xpart(const Ev<MAXH, WPE>& E, uint16_t qx, float x0, float tx, float ex0, float etx,
                                              float y0, float ty, GX& X) {
     constexpr int dx[2] = {4, 3};
     const int xi[2] = {pa(qx), pb(qx)};
@@ -2588,8 +2626,8 @@ __device__ __forceinline__ void y_acc(float& A0, float& A1, float& A3, float& B0
 // accepted U hits (a then b) and, when there is one, the accepted V hits.
 // False when the pair does not reach the fit (no accepted U or V hit).  With
 // SLOTS the hits go into slots 2-5.
-template <bool SLOTS, int MAXH>
-__device__ __forceinline__ bool global_yfit(const Ev<MAXH>& E, const uint16_t* lu, const uint16_t* lv, int nu1, int iuv,
+template <bool SLOTS, int MAXH, int WPE>
+__device__ __forceinline__ bool global_yfit(const Ev<MAXH, WPE>& E, const uint16_t* lu, const uint16_t* lv, int nu1, int iuv,
                                             const GX& X, const float (&As)[5], unsigned& pres, int (&raw)[6],
                                             float (&pos)[6], float (&drift)[6], Fit& g) {
     constexpr int det6[6] = {4, 3, 2, 1, 6, 5};
@@ -2630,8 +2668,8 @@ struct GTab {
     uint32_t* vf;   // [win_cap]: bit 0 hit a accepted, bit 1 hit b
 };
 
-template <int MAXH>
-__device__ __forceinline__ void global_tables(const Ev<MAXH>& E, const GTab& T, const uint16_t* lu,
+template <int MAXH, int WPE>
+__device__ __forceinline__ void global_tables(const Ev<MAXH, WPE>& E, const GTab& T, const uint16_t* lu,
                                               const uint16_t* lv, int nu1, int nv1, const GX& X,
                                               const float (&As)[5]) {
     constexpr int det6[6] = {4, 3, 2, 1, 6, 5};
@@ -2696,10 +2734,10 @@ constexpr int GLOBAL_LIST_CAP = 2048;   // pass-A survivors held before pass B r
 // the best candidate is written.  Uses the x-part records (S.wy() + 8), the
 // survivor list (S.plist()), the best-candidate record (S.wy()) and the
 // pass-A tables (S.ycache()).
-template <int MAXH>
+template <int MAXH, int WPE>
 __device__ __attribute__((noinline)) void global_track(Scr S, Tk* T, const uint16_t* lx, const uint16_t* lu,
                                                        const uint16_t* lv, int nx1, int nu1, int nv1) {
-    Ev<MAXH> E;
+    Ev<MAXH, WPE> E;
     const int lane = lane_id();
     constexpr int det6[6] = {4, 3, 2, 1, 6, 5};
     GBest* best = reinterpret_cast<GBest*>(S.wy());
@@ -2707,8 +2745,8 @@ __device__ __attribute__((noinline)) void global_track(Scr S, Tk* T, const uint1
     uint32_t* list = S.plist();
     GTab tab;
     tab.u = reinterpret_cast<float4*>(S.ycache());
-    tab.v = tab.u + 2 * g_cfg.win_cap;
-    tab.vf = reinterpret_cast<uint32_t*>(tab.v + g_cfg.win_cap);
+    tab.v = tab.u + 2 * cfg().win_cap;
+    tab.vf = reinterpret_cast<uint32_t*>(tab.v + cfg().win_cap);
     const int n23 = T->nx + T->nuv;
     const float x0 = T->x0, tx = T->tx, ex0 = T->ex0, etx = T->etx;
     const float einvp = (etx - etx) / PT_KICK_KMAG;
@@ -2742,7 +2780,7 @@ __device__ __attribute__((noinline)) void global_track(Scr S, Tk* T, const uint1
                 base = 0;
                 if (lane == 0) xrec[ix] = X;
                 global_tables(E, tab, lu, lv, nu1, nv1, X, As);
-                __syncthreads();
+                wsync();
             }
             if (np + WAVE > GLOBAL_LIST_CAP) break;
             const int iuv = base + lane;
@@ -2765,7 +2803,7 @@ __device__ __attribute__((noinline)) void global_track(Scr S, Tk* T, const uint1
             base += WAVE;
             if (base >= ncomb) { have_x = false; ++ix; }
         }
-        __syncthreads();
+        wsync();
         OR_SUB(7);
         // ---- pass B: the listed survivors, one per lane, in list order
         for (int k0 = 0; k0 < np; k0 += WAVE) {
@@ -2853,7 +2891,7 @@ __device__ __attribute__((noinline)) void global_track(Scr S, Tk* T, const uint1
                 best_key = cmin;
             }
         }
-        __syncthreads();
+        wsync();
         OR_SUB(8);
     }
     if (best_key != ~0ull && lane == 0) {
@@ -2877,7 +2915,7 @@ __device__ __attribute__((noinline)) void global_track(Scr S, Tk* T, const uint1
         }
         T->n1 = (uint8_t)n1;
     }
-    __syncthreads();
+    wsync();
 }
 
 // The window lists depend only on the tracklet, and one overflow rejects the
@@ -2886,14 +2924,15 @@ __device__ __attribute__((noinline)) void global_track(Scr S, Tk* T, const uint1
 // the other, each independent of the others.
 constexpr int GLOBAL_BATCH = WAVE / 3;
 
-template <int MAXH>
+template <int MAXH, int WPE>
 STAGE void stage_global(or_params P, Scr S, int ntkl) {
     Lds<MAXH>& L = g_lds<MAXH>;
-    Ev<MAXH> E;
+    YzWave& Y = yzw<MAXH>();
+    Ev<MAXH, WPE> E;
     const int lane = lane_id();
     Tk* tks = S.tks();
-    int* nlf = &L.nl[0][0];
-    const int capw = g_cfg.win_cap;
+    int* nlf = &Y.nl[0][0];
+    const int capw = cfg().win_cap;
     for (int k0 = 0; k0 < ntkl; k0 += WAVE) {
         unsigned long long rest = ballot(k0 + lane < ntkl && tks[k0 + lane].stid >= 5);
         while (rest) {
@@ -2904,12 +2943,12 @@ STAGE void stage_global(or_params P, Scr S, int ntkl) {
                 for (int t = 0; t < b && mm; ++t) mm &= mm - 1;
                 if (b < GLOBAL_BATCH && mm) {
                     const int k = k0 + __ffsll((long long)mm) - 1;
-                    const int nl = global_window(E, &tks[k], v, S.wl() + lane * g_cfg.win_cap, P.max_pair_plane_hits, capw);
+                    const int nl = global_window(E, &tks[k], v, S.wl() + lane * cfg().win_cap, P.max_pair_plane_hits, capw);
                     if (nl < 0) atomicOr(&L.status, (uint32_t)OR_EV_PAIR_OVERFLOW);
                     nlf[lane] = nl;
                 }
             }
-            __syncthreads();
+            wsync();
             OR_SUB(6);
             if (L.status) return;
             for (int b = 0; b < GLOBAL_BATCH && rest; ++b) {
@@ -2917,8 +2956,8 @@ STAGE void stage_global(or_params P, Scr S, int ntkl) {
                 rest &= rest - 1;
                 const int nx1 = nlf[3 * b], nu1 = nlf[3 * b + 1], nv1 = nlf[3 * b + 2];
                 if (nx1 == 0 || nu1 == 0 || nv1 == 0) continue;
-                global_track<MAXH>(S, &tks[k], S.wl() + (3 * b) * g_cfg.win_cap, S.wl() + (3 * b + 1) * g_cfg.win_cap,
-                                   S.wl() + (3 * b + 2) * g_cfg.win_cap, nx1, nu1, nv1);
+                global_track<MAXH, WPE>(S, &tks[k], S.wl() + (3 * b) * cfg().win_cap, S.wl() + (3 * b + 1) * cfg().win_cap,
+                                   S.wl() + (3 * b + 2) * cfg().win_cap, nx1, nu1, nv1);
             }
         }
     }
@@ -3016,10 +3055,10 @@ __device__ void vertex_track(Tk& T) {
 //    chisq / (nhits - 5), merge threshold 0.015 on the signed relative 1/p
 //    difference, same-sign charges; the within-slot else branch compares 1/p(i)
 //    with itself, as the reference does.
-template <int MAXH>
+template <int MAXH, int WPE>
 __device__ bool prop_segment(int p1a, int p1b, int p2a, int p2b, float slope, float icpt, float win1, float win2,
                              float smax, float imax) {
-    Ev<MAXH> E;
+    Ev<MAXH, WPE> E;
     const float cut = 0.03f;
     const int first[2] = {p1a, p1b}, second[2] = {p2a, p2b};
     for (int f = 0; f < 2; ++f) {
@@ -3048,17 +3087,17 @@ __device__ bool prop_segment(int p1a, int p1b, int p2a, int p2b, float slope, fl
     return false;
 }
 
-template <int MAXH>
+template <int MAXH, int WPE>
 STAGE void stage_postglobal(Tk* tks, const uint16_t* slot_order, int ntkl) {
     Lds<MAXH>& L = g_lds<MAXH>;
     const int lane = lane_id();
     for (int k = lane; k < ntkl; k += WAVE) {
         Tk& T = tks[k];
-        if (!prop_segment<MAXH>(50, 49, 52, 51, T.tx, T.x0, 5.08f, 7.62f, TX_MAX, X0_MAX) ||
-            !prop_segment<MAXH>(48, 47, 54, 53, T.ty, T.y0, 5.08f, 10.16f, TY_MAX, Y0_MAX))
+        if (!prop_segment<MAXH, WPE>(50, 49, 52, 51, T.tx, T.x0, 5.08f, 7.62f, TX_MAX, X0_MAX) ||
+            !prop_segment<MAXH, WPE>(48, 47, 54, 53, T.ty, T.y0, 5.08f, 10.16f, TY_MAX, Y0_MAX))
             T.stid = 5;
     }
-    __syncthreads();
+    wsync();
     if (lane == 0) {
         const float thr = 0.015f;
         auto ndf = [](const Tk& T) { return T.chi2 / ((float)(T.nx + T.nuv + T.n1) - 5.f); };
@@ -3095,14 +3134,14 @@ STAGE void stage_postglobal(Tk* tks, const uint16_t* slot_order, int ntkl) {
                 }
             }
     }
-    __syncthreads();
+    wsync();
 }
 
-template <int MAXH>
+template <int MAXH, int WPE>
 STAGE void stage_vertex(Tk* tks, int ntkl) {
     for (int k = lane_id(); k < ntkl; k += WAVE)
         if (tks[k].stid == 6) vertex_track(tks[k]);
-    __syncthreads();
+    wsync();
 }
 
 // H: outputs
@@ -3500,7 +3539,7 @@ __device__ __forceinline__ void chol4_solve(const T (&l)[4][4], T (&b)[4]) {
 // upper-triangle index of (i, j), i <= j
 __device__ __forceinline__ constexpr int fit_ut(int i, int j) { return i * 4 - i * (i - 1) / 2 + (j - i); }
 
-template <int MAXH>
+template <int MAXH, int WPE>
 __device__ __attribute__((noinline)) void final_fit(const Tk& T, float* out) {
     const Lds<MAXH>& L = g_lds<MAXH>;
     const int nh = T.nx + T.nuv + T.n1;
@@ -3640,7 +3679,7 @@ __device__ __attribute__((noinline)) void final_fit(const Tk& T, float* out) {
     for (int q = 0; q < OR_FIT_FIELDS; ++q) out[q] = res8[q];
 }
 
-template <int MAXH>
+template <int MAXH, int WPE>
 STAGE void stage_out(const Args& a, Scr S, int64_t e, int ntkl) {
     Lds<MAXH>& L = g_lds<MAXH>;
     const int lane = lane_id();
@@ -3676,7 +3715,7 @@ STAGE void stage_out(const Args& a, Scr S, int64_t e, int ntkl) {
                     a.tracks[e * P.max_tracks_out + o] = R;
                     if (a.vertex)
                         for (int c = 0; c < 6; ++c) a.vertex[(e * P.max_tracks_out + o) * 6 + c] = T.vtx[c];
-                    if (a.fit) final_fit<MAXH>(T, a.fit + (e * P.max_tracks_out + o) * OR_FIT_FIELDS);
+                    if (a.fit) final_fit<MAXH, WPE>(T, a.fit + (e * P.max_tracks_out + o) * OR_FIT_FIELDS);
                 }
             }
             nout += __popcll(m);
@@ -3750,20 +3789,39 @@ __device__ unsigned long long g_stage_cycles[NCLASS_MAX][N_STAGE_TIMERS];
 #define OR_TIMER(k)
 #endif
 
-template <int MAXH>
 #ifndef OR_WAVES_PER_SIMD_512
 #define OR_WAVES_PER_SIMD_512 4
 #endif
-__global__ __launch_bounds__(WAVE, MAXH <= 512 ? OR_WAVES_PER_SIMD_512 : (MAXH <= 1024 ? 2 : 1)) void chain_kernel(Args a) {
+// Waves per SIMD a tile class is compiled for (the register budget): as many
+// as the workgroups that fit the CU's 160 KB of LDS bring, at most 4.
+template <int MAXH, int WPE>
+constexpr int chain_waves_per_simd() {
+    if (MAXH <= 512) return OR_WAVES_PER_SIMD_512;
+    constexpr int lds = (int)sizeof(Lds<MAXH>) + (int)sizeof(WaveCfg) * WPE_MAX;
+    constexpr int w = (160 * 1024 / lds) * WPE / 4;
+    return w < 1 ? 1 : (w > 4 ? 4 : w);
+}
+
+// The chain: one event per workgroup of WPE waves, persistent over a work queue.
+// The first wave runs every stage but YZ; YZ runs on all waves, chunk by chunk.
+template <int MAXH, int WPE>
+__global__ __launch_bounds__(WAVE * WPE, (chain_waves_per_simd<MAXH, WPE>())) void chain_kernel(Args a) {
+    static_assert(WPE >= 1 && WPE <= lds_waves(MAXH), "waves per event of this tile class");
     Lds<MAXH>& L = g_lds<MAXH>;
     const int lane = lane_id();
-    if (lane < R_COUNT) g_cfg.rp[lane] = a.scratch + (int64_t)blockIdx.x * a.lay.stride + a.lay.off[lane];
-    if (lane == 0) {
-        g_cfg.maxtkl = a.lay.maxtkl;
-        g_cfg.bin_cap = a.lay.bin_cap;
-        g_cfg.win_cap = a.lay.win_cap;
-        g_cfg.list_cap = a.lay.list_cap;
-        g_cfg.unit_cap = a.lay.unit_cap;
+    const int w = WPE == 1 ? 0 : wave_idx();
+    {
+        // the event's regions from the first wave's block, the others from the wave's own
+        const int64_t blk = (int64_t)blockIdx.x * WPE;
+        if (lane < R_COUNT)
+            g_cfgw[w].rp[lane] = a.scratch + (blk + (scr_shared(lane) ? 0 : w)) * a.lay.stride + a.lay.off[lane];
+        if (lane == 0) {
+            g_cfgw[w].maxtkl = a.lay.maxtkl;
+            g_cfgw[w].bin_cap = a.lay.bin_cap;
+            g_cfgw[w].win_cap = a.lay.win_cap;
+            g_cfgw[w].list_cap = a.lay.list_cap;
+            g_cfgw[w].unit_cap = a.lay.unit_cap;
+        }
     }
     __syncthreads();
     Scr S;
@@ -3771,50 +3829,58 @@ __global__ __launch_bounds__(WAVE, MAXH <= 512 ? OR_WAVES_PER_SIMD_512 : (MAXH <
 #ifdef OR_STAGE_TIMING
     unsigned long long tacc[N_STAGE_TIMERS] = {0, 0, 0, 0, 0, 0, 0};
 #endif
-    // persistent work queue: the first event of each wave is its block index,
-    // later ones come from an atomic head, so waves that drew cheap events take more
+    // persistent work queue: the first event of each workgroup is its block
+    // index, later ones come from an atomic head, so groups that drew cheap
+    // events take more
     for (int64_t i = blockIdx.x; i < ne;) {
         const int64_t e = a.ev_list ? (int64_t)a.ev_list[i] : i;
-        const int64_t h0 = a.off[e];
-        const int n = (int)(a.off[e + 1] - h0);
         OR_TIMER_START
-        if (lane == 0) L.status = 0;
-        if (lane < 56) L.pcnt[lane] = 0;
+        if (w == 0) {
+            const int64_t h0 = a.off[e];
+            const int n = (int)(a.off[e + 1] - h0);
+            if (lane == 0) L.status = 0;
+            if (lane < 56) L.pcnt[lane] = 0;
+            wsync();
+            const bool staged = stage_load_reduce<MAXH, WPE>(a, h0, n);
+            if (staged) stage_keep_bits<MAXH, WPE>(a, h0, n);
+            wsync();
+            OR_TIMER(0)
+            int nt = 0;
+            if (L.status == 0) stage_xpairs<MAXH, WPE>(a.p, S.xpairs());
+            OR_TIMER(1)
+            if (L.status == 0) nt = stage_xz<MAXH, WPE>(a.p, S);
+            OR_TIMER(2)
+            if (lane == 0) {
+                L.ntkl = nt;
+                L.yz_next = 0;
+            }
+        }
         __syncthreads();
-        const bool staged = stage_load_reduce<MAXH>(a, h0, n);
-        if (staged) stage_keep_bits<MAXH>(a, h0, n);
-        __syncthreads();
-        OR_TIMER(0)
-        int ntkl = 0;
-        if (L.status == 0) stage_xpairs<MAXH>(a.p, S.xpairs());
-        OR_TIMER(1)
-        if (L.status == 0) ntkl = stage_xz<MAXH>(a.p, S);
-        OR_TIMER(2)
+        const int ntkl = L.ntkl;
         if (L.status == 0) {
             const Prn R = {a.pruner, a.unit_dump, a.dump_cap, a.dump_count, e};
-            stage_yz<MAXH>(a.p, S, R, ntkl, a.yz_list_cap);
+            stage_yz<MAXH, WPE>(a.p, S, R, ntkl, a.yz_list_cap);
         }
-        OR_TIMER(3)
-        if (L.status == 0) stage_clean<MAXH>(S.tks(), S.slot_order(), ntkl);
-        OR_TIMER(4)
-        if (L.status == 0) stage_global<MAXH>(a.p, S, ntkl);
-        if (L.status == 0 && a.p.full_chain) stage_postglobal<MAXH>(S.tks(), S.slot_order(), ntkl);
-        if (L.status == 0 && (a.vertex || a.p.full_chain)) stage_vertex<MAXH>(S.tks(), ntkl);
-        OR_TIMER(5)
-        stage_out<MAXH>(a, S, e, ntkl);
         __syncthreads();
-        OR_TIMER(6)
-        if (a.ev_head) {
-            int next = 0;
-            if (lane == 0) next = atomicAdd(a.ev_head, 1);
-            i = (int64_t)gridDim.x + __shfl(next, 0, WAVE);
-        } else {
-            i += gridDim.x;
+        OR_TIMER(3)
+        if (w == 0) {
+            if (L.status == 0) stage_clean<MAXH, WPE>(S.tks(), S.slot_order(), ntkl);
+            OR_TIMER(4)
+            if (L.status == 0) stage_global<MAXH, WPE>(a.p, S, ntkl);
+            if (L.status == 0 && a.p.full_chain) stage_postglobal<MAXH, WPE>(S.tks(), S.slot_order(), ntkl);
+            if (L.status == 0 && (a.vertex || a.p.full_chain)) stage_vertex<MAXH, WPE>(S.tks(), ntkl);
+            OR_TIMER(5)
+            stage_out<MAXH, WPE>(a, S, e, ntkl);
+            wsync();
+            OR_TIMER(6)
+            if (a.ev_head && lane == 0) L.next_ev = atomicAdd(a.ev_head, 1);
         }
+        __syncthreads();
+        i = a.ev_head ? (int64_t)gridDim.x + L.next_ev : i + gridDim.x;
     }
 #ifdef OR_STAGE_TIMING
     constexpr int cls = MAXH == 512 ? 0 : MAXH == 1024 ? 1 : MAXH == 2048 ? 2 : MAXH == 4096 ? 3 : 4;
-    if (lane == 0)
+    if (w == 0 && lane == 0)
         for (int k = 0; k < N_STAGE_TIMERS; ++k) atomicAdd(&g_stage_cycles[cls][k], tacc[k]);
 #endif
 }
@@ -3879,10 +3945,12 @@ struct or_session {
     float* d_wire = nullptr;
     char* d_scratch = nullptr;
     ScrLayout lay;               // per-wave scratch layout, from the caps and the YZ batch capacity
-    int grid = 0;
+    int grid = 0;                 // scratch blocks (waves) held: the most any class and waves-per-event setting uses
     int maxh = 1024;
     int nclass = 2;
-    int grid_c[NCLASS] = {0, 0, 0, 0, 0};
+    int cus = 0;                  // compute units of the device
+    int occ[NCLASS][3] = {};      // resident workgroups per CU by class and waves per event (1, 2, 4)
+    int wpe_log = 2;              // waves per event 1 << wpe_log for the tile classes above 512 hits
     int32_t* d_lists = nullptr;   // NCLASS x list_cap event indices
     int32_t* d_counts = nullptr;  // NCLASS class counts, then NCLASS work queue heads
     int64_t list_cap = 0;
@@ -3908,7 +3976,7 @@ struct or_session {
         const void* p[19];
         int64_t n_events, n_hits, dump_cap;
         hipStream_t stream;
-        int yz_list_cap, grid_pct;
+        int yz_list_cap, grid_pct, wpe_log;
         bool operator==(const GraphKey& o) const { return std::memcmp(this, &o, sizeof o) == 0; }
     } graph_key;
     // host staging (host-pointer calls)
@@ -3989,26 +4057,52 @@ DevGeo g_dev_geo[64];
 
 namespace {
 
-template <int MAXH>
+template <int MAXH, int WPE>
 int occupancy_blocks() {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, chain_kernel<MAXH>, WAVE, 0) != hipSuccess) nb = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, chain_kernel<MAXH, WPE>, WAVE * WPE, 0) != hipSuccess) nb = 1;
     return nb < 1 ? 1 : nb;
 }
 
+// resident workgroups per CU of a tile class at 1, 2, 4 waves per event (0: not built)
 template <int MAXH>
+void occupancy_of(int (&o)[3]) {
+    o[0] = occupancy_blocks<MAXH, 1>();
+    if constexpr (lds_waves(MAXH) > 1) {
+        o[1] = occupancy_blocks<MAXH, 2>();
+        o[2] = occupancy_blocks<MAXH, 4>();
+    } else {
+        o[1] = o[2] = 0;
+    }
+}
+
+template <int MAXH, int WPE>
 hipError_t launch_chain(const Args& a, int grid, hipStream_t s) {
-    hipLaunchKernelGGL(chain_kernel<MAXH>, dim3(grid), dim3(WAVE), 0, s, a);
+    hipLaunchKernelGGL((chain_kernel<MAXH, WPE>), dim3(grid), dim3(WAVE * WPE), 0, s, a);
     return hipGetLastError();
 }
 
-hipError_t launch_class(int c, const Args& a, int grid, hipStream_t s) {
+// k: waves per event 1 << k
+template <int MAXH>
+hipError_t launch_wpe(int k, const Args& a, int grid, hipStream_t s) {
+    if constexpr (lds_waves(MAXH) == 1) {
+        return launch_chain<MAXH, 1>(a, grid, s);
+    } else {
+        switch (k) {
+            case 0: return launch_chain<MAXH, 1>(a, grid, s);
+            case 1: return launch_chain<MAXH, 2>(a, grid, s);
+            default: return launch_chain<MAXH, 4>(a, grid, s);
+        }
+    }
+}
+
+hipError_t launch_class(int c, int k, const Args& a, int grid, hipStream_t s) {
     switch (c) {
-        case 0: return launch_chain<512>(a, grid, s);
-        case 1: return launch_chain<1024>(a, grid, s);
-        case 2: return launch_chain<2048>(a, grid, s);
-        case 3: return launch_chain<4096>(a, grid, s);
-        default: return launch_chain<8192>(a, grid, s);
+        case 0: return launch_wpe<512>(k, a, grid, s);
+        case 1: return launch_wpe<1024>(k, a, grid, s);
+        case 2: return launch_wpe<2048>(k, a, grid, s);
+        case 3: return launch_wpe<4096>(k, a, grid, s);
+        default: return launch_wpe<8192>(k, a, grid, s);
     }
 }
 
@@ -4099,13 +4193,15 @@ or_status or_session_create(const or_geometry* g, const or_params* p, int device
     OR_HIP(hipMemcpy(s->d_wire, g->wire, sizeof g->wire, hipMemcpyHostToDevice));
     hipDeviceProp_t prop;
     OR_HIP(hipGetDeviceProperties(&prop, device));
-    const int occ[NCLASS] = {occupancy_blocks<512>(), occupancy_blocks<1024>(), occupancy_blocks<2048>(),
-                             occupancy_blocks<4096>(), occupancy_blocks<8192>()};
+    occupancy_of<512>(s->occ[0]);
+    occupancy_of<1024>(s->occ[1]);
+    occupancy_of<2048>(s->occ[2]);
+    occupancy_of<4096>(s->occ[3]);
+    occupancy_of<8192>(s->occ[4]);
+    s->cus = prop.multiProcessorCount;
     s->grid = 0;
-    for (int c = 0; c < s->nclass; ++c) {
-        s->grid_c[c] = prop.multiProcessorCount * occ[c];
-        s->grid = std::max(s->grid, s->grid_c[c]);
-    }
+    for (int c = 0; c < s->nclass; ++c)
+        for (int k = 0; k < 3; ++k) s->grid = std::max(s->grid, s->cus * s->occ[c][k] << k);
     s->lay = make_layout(prm, YZ_LIST_MIN);
     OR_HIP(hipMalloc(&s->d_counts, sizeof(int32_t) * 2 * NCLASS));
     OR_HIP(hipMalloc(&s->d_scratch, (size_t)s->lay.stride * s->grid));
@@ -4162,6 +4258,10 @@ or_status or_session_set_tuning(or_session* s, or_tuning key, int64_t value) {
             if (value != 0 && value != 1) return OR_ERR_INVALID_ARG;
             s->use_graph = value != 0;
             return OR_OK;
+        case OR_TUNE_WAVES_PER_EVENT:
+            if (value != 1 && value != 2 && value != 4) return OR_ERR_INVALID_ARG;
+            s->wpe_log = value == 1 ? 0 : (value == 2 ? 1 : 2);
+            return OR_OK;
     }
     return OR_ERR_INVALID_ARG;
 }
@@ -4172,12 +4272,13 @@ or_status or_session_get_tuning(const or_session* s, or_tuning key, int64_t* val
         case OR_TUNE_YZ_LIST_CAP: *value = s->yz_list_cap; return OR_OK;
         case OR_TUNE_GRID_PERCENT: *value = s->grid_pct; return OR_OK;
         case OR_TUNE_GRAPH: *value = s->use_graph ? 1 : 0; return OR_OK;
+        case OR_TUNE_WAVES_PER_EVENT: *value = (int64_t)1 << s->wpe_log; return OR_OK;
     }
     return OR_ERR_INVALID_ARG;
 }
 
 // Device buffers of one call's launch sequence: per-wave scratch (waves_max
-// blocks of the session's stride), the class event lists (NCLASS x
+// blocks of the session's stride; a workgroup of W waves takes W consecutive ones), the class event lists (NCLASS x
 // list_stride) and the class counters (2 x NCLASS).
 struct ChainBufs {
     char* scratch;
@@ -4219,9 +4320,11 @@ static or_status issue_chain_with(or_session* s, const or_hits* in, const or_res
         a.ev_list = B.lists + c * B.list_stride;
         a.ev_count = B.counts + c;
         a.ev_head = B.counts + NCLASS + c;
-        const int64_t g = std::max<int64_t>(1, (int64_t)s->grid_c[c] * s->grid_pct / 100);
-        const int grid = (int)std::min<int64_t>(std::min<int64_t>(g, in->n_events), B.waves_max);
-        OR_HIP(launch_class(c, a, grid, s->stream));
+        int k = c == 0 ? 0 : s->wpe_log;
+        while (k > 0 && ((int64_t)1 << k) > B.waves_max) --k;   // a small fixed scratch: fewer waves per event
+        const int64_t g = std::max<int64_t>(1, (int64_t)s->cus * s->occ[c][k] * s->grid_pct / 100);
+        const int grid = (int)std::min<int64_t>(std::min<int64_t>(g, in->n_events), B.waves_max >> k);
+        OR_HIP(launch_class(c, k, a, grid, s->stream));
     }
     return OR_OK;
 }
@@ -4265,7 +4368,7 @@ static or_status run_device(or_session* s, const or_hits* in, const or_result* o
                                 s->dump_count, s->rt_on ? s->d_rt : nullptr};
         std::memcpy(key.p, ptrs, sizeof ptrs);
         key.n_events = in->n_events; key.n_hits = in->n_hits; key.dump_cap = s->dump_cap; key.stream = s->stream;
-        key.yz_list_cap = s->yz_list_cap; key.grid_pct = s->grid_pct;
+        key.yz_list_cap = s->yz_list_cap; key.grid_pct = s->grid_pct; key.wpe_log = s->wpe_log;
         if (!s->graph_exec || !(key == s->graph_key)) {
             if (s->graph_exec) { (void)hipGraphExecDestroy(s->graph_exec); s->graph_exec = nullptr; }
             // captured on the session's own stream (the null stream cannot capture),
@@ -4589,28 +4692,45 @@ or_status or_session_set_unit_dump(or_session* s, float* rows, int64_t cap, unsi
 
 or_status or_session_autotune(or_session* s, const or_hits* in, or_result* out, int reps, float* best_ms) {
     if (!s || !in || !out || !in->on_device || !out->on_device || reps < 1) return OR_ERR_INVALID_ARG;
-    static const int grid_pcts[] = {100, 75, 50};
+    const int wpe_logs[] = {0, 1, 2};
     const int list_caps[] = {s->lay.list_cap, s->lay.list_cap / 2, s->lay.list_cap / 4};
-    int best_pct = s->grid_pct, best_cap = s->yz_list_cap;
+    static const int grid_pcts[] = {75, 50};
+    int best_w = s->wpe_log, best_cap = s->yz_list_cap, best_pct = s->grid_pct;
     float best = -1.f;
-    for (int cap : list_caps) {
-        for (int pct : grid_pcts) {
-            s->grid_pct = pct;
-            s->yz_list_cap = cap;
-            or_status st = or_reconstruct(s, in, out);   // warm-up at this setting
+    auto time_it = [&](float& ms) -> or_status {
+        or_status st = or_reconstruct(s, in, out);   // warm-up at this setting
+        if (st != OR_OK) return st;
+        float tot = 0.f;
+        for (int r = 0; r < reps; ++r) {
+            st = or_reconstruct(s, in, out);
             if (st != OR_OK) return st;
-            float tot = 0.f;
-            for (int r = 0; r < reps; ++r) {
-                st = or_reconstruct(s, in, out);
-                if (st != OR_OK) return st;
-                tot += or_session_last_kernel_ms(s);
-            }
-            const float ms = tot / (float)reps;
-            if (best < 0.f || ms < best) { best = ms; best_pct = pct; best_cap = cap; }
+            tot += or_session_last_kernel_ms(s);
+        }
+        ms = tot / (float)reps;
+        return OR_OK;
+    };
+    // waves per event x YZ survivor batch on the full grid, then smaller grids at the best pair
+    s->grid_pct = 100;
+    for (int wl : wpe_logs) {
+        for (int cap : list_caps) {
+            s->wpe_log = wl;
+            s->yz_list_cap = cap;
+            float ms = 0.f;
+            const or_status st = time_it(ms);
+            if (st != OR_OK) return st;
+            if (best < 0.f || ms < best) { best = ms; best_w = wl; best_cap = cap; best_pct = 100; }
         }
     }
-    s->grid_pct = best_pct;
+    s->wpe_log = best_w;
     s->yz_list_cap = best_cap;
+    for (int pct : grid_pcts) {
+        s->grid_pct = pct;
+        float ms = 0.f;
+        const or_status st = time_it(ms);
+        if (st != OR_OK) return st;
+        if (ms < best) { best = ms; best_pct = pct; }
+    }
+    s->grid_pct = best_pct;
     if (best_ms) *best_ms = best;
     return OR_OK;
 }

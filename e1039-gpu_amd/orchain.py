@@ -47,6 +47,7 @@ EV_REJECT_MASK = 0x7FF
 TUNE_YZ_LIST_CAP = 1
 TUNE_GRID_PERCENT = 2
 TUNE_GRAPH = 3
+TUNE_WAVES_PER_EVENT = 4
 NOALLOC_MAX_EVENTS = 262144     # OR_NOALLOC_MAX_EVENTS
 PRUNE_WEIGHTS = 5313       # OR_PRUNE_WEIGHTS: w1[64][16], b1[64], w2[64][64], b2[64], w3[64], b3
 PRUNE_FEATURES = 16

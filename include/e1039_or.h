@@ -248,21 +248,25 @@ typedef enum {
     OR_TUNE_GRID_PERCENT = 2,
     /* 1: capture the launch sequence of a call into a hipGraph and replay it
      * while the buffers, sizes and stream stay the same; 0 (default): launch */
-    OR_TUNE_GRAPH = 3
+    OR_TUNE_GRAPH = 3,
+    /* waves per event, 1, 2 or 4 (default 4), for the tile classes above 512
+     * hits (the 512-hit class always runs one): the waves of an event share its
+     * LDS tile and split its YZ tracklet chunks; the other stages run on one
+     * wave.  Results do not depend on it. */
+    OR_TUNE_WAVES_PER_EVENT = 4
 } or_tuning;
 or_status or_session_set_tuning(or_session* s, or_tuning key, int64_t value);
 
 /* Reads a tuning knob of the session into *value. */
 or_status or_session_get_tuning(const or_session* s, or_tuning key, int64_t* value);
 
-/* Times the device-resident batch (in, out) at each combination of YZ
- * candidate-list size (4096, 2048, 1024 per chunk of 16 tracklets: the tile
- * of candidates evaluated in parallel) and persistent grid size (100, 75, 50 %
- * of the occupancy limit); reps timed calls after one warm-up each.  Keeps the
- * fastest combination in the session and returns its mean chain time in
- * *best_ms.  Outputs are those of the last call.  One wave per event is fixed
- * by the design (the event's hits, lists and tracklets are staged for one
- * wave), so it is not swept.  No reference counterpart. */
+/* Times the device-resident batch (in, out) at each combination of waves per
+ * event (1, 2, 4) and YZ survivor batch (the session's batch capacity, a half
+ * and a quarter of it), on the full persistent grid, then the best pair at 75
+ * and 50 % of the occupancy limit; reps timed calls after one warm-up each.
+ * Keeps the fastest setting in the session and returns its mean chain time in
+ * *best_ms.  Outputs are those of the last call.  No reference counterpart
+ * (the reference launches one block per event, online_reconstruction.cu:776). */
 or_status or_session_autotune(or_session* s, const or_hits* in, or_result* out, int reps, float* best_ms);
 
 /* ---- R-T calibration: drift distance from the TDC time ---------------------

@@ -95,3 +95,41 @@ def test_geometry_conflict(oc, geo, tmp_path):
     assert np.array_equal(before.n_tracks, after.n_tracks)
     assert np.array_equal(before.valid_tracks().view(np.uint8), after.valid_tracks().view(np.uint8))
     sess.close()
+
+
+@pytest.mark.parametrize("mean,cap,n", [(800.0, 1000, 30), (1200.0, 4000, 30), (2500.0, 6000, 8)])
+def test_waves_per_event(oc, orc, geo, ogeo, mean, cap, n):
+    # an event's YZ chunks split over 1, 2 or 4 waves (tile classes 1024, 2048,
+    # 4096 hits), with full and small survivor batches: the oracle's result at each
+    print(f"case: {n} events, hits Poisson({mean:g}) cap {cap}, waves per event 1/2/4", flush=True)
+    ev = oc.generate(geo, n, seed=0x3E + int(mean), mult_mode=1, mult_mean=mean, mult_cap=cap)
+    params = oc.high_occupancy_params(max_tracklets_per_event=4096)
+    ref = orc.reconstruct(ogeo, params, ev, records=True)
+    assert ref.n_tracklets.sum() > 100 * n
+    sess = oc.Session(geo, params, device=0)
+    for wpe in (1, 2, 4):
+        sess.set_tuning(oc.TUNE_WAVES_PER_EVENT, wpe)
+        assert sess.get_tuning(oc.TUNE_WAVES_PER_EVENT) == wpe
+        for batch in (4096, 150):
+            sess.set_tuning(oc.TUNE_YZ_LIST_CAP, batch)
+            got = sess.reconstruct(ev, records=True)
+            assert_same(got, ref, ev, f"{wpe} waves per event, batch {batch}")
+    sess.close()
+
+
+def test_waves_per_event_pruned(oc, geo):
+    # learned pruning gives the same tracks whichever wave takes a chunk
+    import pruner
+    flat, thr, meta = pruner.load()
+    print("case: 300 events, hits Poisson(1200), pruned, waves per event 1 vs 4", flush=True)
+    ev = oc.generate(geo, 300, seed=0x3F, mult_mode=1, mult_mean=1200.0, mult_cap=4000)
+    sess = oc.Session(geo, oc.high_occupancy_params(), device=0)
+    sess.set_pruner_margin(meta["margin"])
+    sess.set_pruner(flat, thr)
+    outs = []
+    for wpe in (1, 4, 2):
+        sess.set_tuning(oc.TUNE_WAVES_PER_EVENT, wpe)
+        outs.append(sess.reconstruct(ev, records=True))
+    for o in outs[1:]:
+        assert_same(o, outs[0], ev, "pruned, waves per event")
+    sess.close()
