@@ -172,8 +172,8 @@ static_assert(sizeof(Tk) == 256, "tracklet record size");
 constexpr int CLEAN_WORDS = 64;   // lane-parallel cleaning of up to 64 x 64 live tracklets
 
 // Waves per event: a tile class runs with 1, 2 or 4 waves per workgroup (one
-// event per workgroup).  The YZ stage hands its tracklet chunks to all of
-// them; the other stages run on the first wave.
+// event per workgroup).  The YZ and station-1 stages hand their tracklet
+// chunks to all of them; the other stages run on the first wave.
 constexpr int WPE_MAX = 4;
 __host__ __device__ constexpr int lds_waves(int maxh) { return maxh <= 512 ? 1 : WPE_MAX; }
 
@@ -213,6 +213,7 @@ struct Lds {
     uint32_t status;
     int ntkl;                 // tracklets after XZ (read by every wave of the event)
     int yz_next;              // next YZ chunk to hand out
+    int gl_next;              // next block of 64 tracklets of the station-1 stage
     int next_ev;              // work queue: the event the group takes next
 };
 
@@ -2933,7 +2934,12 @@ STAGE void stage_global(or_params P, Scr S, int ntkl) {
     Tk* tks = S.tks();
     int* nlf = &Y.nl[0][0];
     const int capw = cfg().win_cap;
-    for (int k0 = 0; k0 < ntkl; k0 += WAVE) {
+    // blocks of 64 tracklets, handed to the event's waves as they come free
+    for (;;) {
+        int k0 = 0;
+        if (lane == 0) k0 = atomicAdd(&L.gl_next, WAVE);
+        k0 = __builtin_amdgcn_readfirstlane(__shfl(k0, 0, WAVE));
+        if (k0 >= ntkl) break;
         unsigned long long rest = ballot(k0 + lane < ntkl && tks[k0 + lane].stid >= 5);
         while (rest) {
             OR_SUB_START
@@ -2950,7 +2956,7 @@ STAGE void stage_global(or_params P, Scr S, int ntkl) {
             }
             wsync();
             OR_SUB(6);
-            if (L.status) return;
+            if (__atomic_load_n(&L.status, __ATOMIC_RELAXED)) return;   // set by any wave: the event is rejected
             for (int b = 0; b < GLOBAL_BATCH && rest; ++b) {
                 const int k = k0 + __ffsll((long long)rest) - 1;
                 rest &= rest - 1;
@@ -3803,7 +3809,8 @@ constexpr int chain_waves_per_simd() {
 }
 
 // The chain: one event per workgroup of WPE waves, persistent over a work queue.
-// The first wave runs every stage but YZ; YZ runs on all waves, chunk by chunk.
+// YZ (by chunks of tracklets) and station 1 (by blocks of 64 tracklets) run on
+// all waves; the other stages on the first.
 template <int MAXH, int WPE>
 __global__ __launch_bounds__(WAVE * WPE, (chain_waves_per_simd<MAXH, WPE>())) void chain_kernel(Args a) {
     static_assert(WPE >= 1 && WPE <= lds_waves(MAXH), "waves per event of this tile class");
@@ -3865,8 +3872,13 @@ __global__ __launch_bounds__(WAVE * WPE, (chain_waves_per_simd<MAXH, WPE>())) vo
         OR_TIMER(3)
         if (w == 0) {
             if (L.status == 0) stage_clean<MAXH, WPE>(S.tks(), S.slot_order(), ntkl);
-            OR_TIMER(4)
-            if (L.status == 0) stage_global<MAXH, WPE>(a.p, S, ntkl);
+            if (lane == 0) L.gl_next = 0;
+        }
+        __syncthreads();
+        OR_TIMER(4)
+        if (L.status == 0) stage_global<MAXH, WPE>(a.p, S, ntkl);
+        __syncthreads();
+        if (w == 0) {
             if (L.status == 0 && a.p.full_chain) stage_postglobal<MAXH, WPE>(S.tks(), S.slot_order(), ntkl);
             if (L.status == 0 && (a.vertex || a.p.full_chain)) stage_vertex<MAXH, WPE>(S.tks(), ntkl);
             OR_TIMER(5)

@@ -251,8 +251,8 @@ typedef enum {
     OR_TUNE_GRAPH = 3,
     /* waves per event, 1, 2 or 4 (default 4), for the tile classes above 512
      * hits (the 512-hit class always runs one): the waves of an event share its
-     * LDS tile and split its YZ tracklet chunks; the other stages run on one
-     * wave.  Results do not depend on it. */
+     * LDS tile and split its YZ and station-1 work by tracklets; the other
+     * stages run on one wave.  Results do not depend on it. */
     OR_TUNE_WAVES_PER_EVENT = 4
 } or_tuning;
 or_status or_session_set_tuning(or_session* s, or_tuning key, int64_t value);
