@@ -76,7 +76,7 @@ class OrParams(C.Structure):
 class OrHits(C.Structure):
     _fields_ = [("n_events", C.c_int64), ("n_hits", C.c_int64), ("det", C.c_void_p), ("elem", C.c_void_p),
                 ("tdc", C.c_void_p), ("drift", C.c_void_p), ("flag", C.c_void_p), ("event_offsets", C.c_void_p),
-                ("on_device", C.c_int32)]
+                ("on_device", C.c_int32), ("event_id", C.c_void_p)]
 
 
 class OrResult(C.Structure):
@@ -172,6 +172,12 @@ def load_library(path: str = LIB_PATH):
     lib.or_dimuons.restype = i32
     lib.or_session_set_fit_cond_bound.argtypes = [vp, C.c_float]
     lib.or_session_set_fit_cond_bound.restype = i32
+    lib.or_output_open.argtypes = [C.c_char_p, C.POINTER(vp)]
+    lib.or_output_open.restype = i32
+    lib.or_output_fill.argtypes = [vp, vp, C.POINTER(OrHits), C.POINTER(OrResult), i32, vp, i32]
+    lib.or_output_fill.restype = i32
+    lib.or_output_close.argtypes = [vp]
+    lib.or_output_close.restype = i32
     lib.or_session_destroy.argtypes = [vp]
     lib.or_status_string.argtypes = [i32]
     lib.or_status_string.restype = C.c_char_p
@@ -617,3 +623,52 @@ class Session:
                        None if fitv is None else fitv.ctypes.data, 0)
         _check(self.lib.or_reconstruct(self.handle, C.byref(hits), C.byref(res)))
         return Result(status, keep, ntr, tracks, ntkl, rec, vtx, fitv)
+
+
+# ---- columnar event output (or_output_*, the reference's OROutput tree) -------
+
+OUTPUT_MIN_STATION_ID = 7   # the reference writes tracks of stationID >= 7 (online_reconstruction.cu:1132)
+
+
+class OutputWriter:
+    """or_output_open / or_output_fill / or_output_close: one .npz archive with
+    one array per OROutput branch (OROutput.cxx:24-102), filled from host-side
+    results batch by batch.  Use as a context manager or call close()."""
+
+    def __init__(self, path: str):
+        self.lib = load_library()
+        self.handle = C.c_void_p()
+        _check(self.lib.or_output_open(str(path).encode(), C.byref(self.handle)))
+
+    def fill(self, geo: Geometry, ev: Events, res: Result, max_tracklets_per_event: int,
+             dim: "DimuonResult | None" = None, event_id=None, min_station_id: int = OUTPUT_MIN_STATION_ID):
+        if res.records is None or res.n_tracklets is None:
+            raise ValueError("the writer needs the tracklet records and counts (records=True)")
+        keep = []   # arrays referenced by pointer until the call returns
+        def ptr(a, dt):
+            a = np.ascontiguousarray(a, dt)
+            keep.append(a)
+            return a.ctypes.data
+        hits = OrHits(ev.n_events, ev.n_hits, ptr(ev.det, np.int16), ptr(ev.elem, np.int16), ptr(ev.tdc, np.float32),
+                      ptr(ev.drift, np.float32), None, ptr(ev.offsets, np.int64), 0,
+                      None if event_id is None else ptr(event_id, np.int32))
+        out = OrResult(ptr(res.status, np.uint32), ptr(res.keep_mask, np.uint32), None, None,
+                       ptr(res.n_tracklets, np.int32), ptr(res.records, np.float32), None, None, 0)
+        dptr = None
+        if dim is not None:
+            d = OrDimuonResult(ptr(dim.n_dimuons, np.int32), ptr(dim.dimuons, np.float32), None, None)
+            keep.append(d)
+            dptr = C.addressof(d)
+        _check(self.lib.or_output_fill(self.handle, geo.handle, C.byref(hits), C.byref(out),
+                                       int(max_tracklets_per_event), dptr, int(min_station_id)))
+
+    def close(self):
+        if self.handle:
+            h, self.handle = self.handle, C.c_void_p()
+            _check(self.lib.or_output_close(h))
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

@@ -27,7 +27,7 @@ extern "C" {
 typedef enum {
     OR_OK = 0,
     OR_ERR_INVALID_ARG = 1,
-    OR_ERR_IO = 2,            /* geometry file cannot be opened            */
+    OR_ERR_IO = 2,            /* a file cannot be opened, read or written  */
     OR_ERR_PARSE = 3,         /* geometry line with a detector id out of range */
     OR_ERR_DEVICE = 4,        /* HIP runtime error                          */
     OR_ERR_NOMEM = 5,
@@ -114,6 +114,9 @@ typedef struct {
     const uint8_t* flag;            /* may be NULL */
     const int64_t* event_offsets;   /* n_events+1 */
     int32_t on_device;              /* 1: the pointers above are device pointers */
+    const int32_t* event_id;        /* optional, n_events: the events' IDs (LoadInput.h EventID), carried to
+                                       the outputs that name events (or_output_fill, host memory); the chain
+                                       does not read it.  NULL: the event index */
 } or_hits;
 
 /* Compact output track, 96 bytes.  Fields map to the reference record
@@ -353,6 +356,36 @@ typedef struct {
  * int32, or NULL (all 0). */
 or_status or_dimuons(or_session* s, const or_hits* in, const or_result* chain, const int32_t* trigger_bits,
                      or_dimuon_result* out);
+
+/* ---- columnar event output (the reference's OROutput tree) ----------------
+ * One row per written event and one array per branch, with the branch names
+ * and types of OROutput.cxx:24-102 (Int_t and vector<int> as int32,
+ * vector<float> as float32).  A vector branch holds the events' values
+ * concatenated; its length per event is the event's counter branch (nhits for
+ * hit.*, ntracks for track.* but track.hit*, whose lengths are track.nhits,
+ * ndimuons for dimuon.*).  Filled as online_reconstruction.cu:1063-1222 fills
+ * the tree: events without hits or failing the station multiplicity cuts
+ * (OR_EV_D0_MULT .. OR_EV_D3M_MULT, the reference's HasTooManyHits) are not
+ * written; hit.* are the hits the reducer kept (keep_mask), detector by
+ * detector 1..54 in staging order, chan = elementID, pos = the wire position,
+ * tdc and |drift| as given; track.* come from the tracklet records (slot
+ * order) whose stationID is at least min_station_id (the reference writes 7
+ * and above, the full chain's tracks); dimuon.* from the or_dimuon_result rows,
+ * dimuon.px/py/pz being the sums of the two muons' momenta as the reference
+ * fills them.  track.invp_tgt stays empty (the reference fills it only when
+ * built with TEST_MOMENTUM).  One branch beyond the reference's, status
+ * (uint32), holds each written event's OR_EV_* bits.
+ * The archive is an uncompressed .npz (numpy.load reads it) written by
+ * or_output_close; each array below 4 GiB. */
+typedef struct or_output or_output;   /* opaque */
+or_status or_output_open(const char* path, or_output** out);
+/* Appends the events of one batch, all host pointers: in (event_id optional),
+ * chain (status, keep_mask, n_tracklets and records required) from a session
+ * with the given max_tracklets_per_event, dim optional (NULL: no dimuons). */
+or_status or_output_fill(or_output* w, const or_geometry* g, const or_hits* in, const or_result* chain,
+                         int32_t max_tracklets_per_event, const or_dimuon_result* dim, int32_t min_station_id);
+/* Writes the archive and frees the writer (also when it fails). */
+or_status or_output_close(or_output* w);
 
 /* ---- learned pruning of YZ station-3 units (no reference counterpart) -----
  * A YZ unit is one (u3, v3) window-list pair of a tracklet: the reference
