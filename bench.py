@@ -39,6 +39,7 @@ sys.path.insert(0, os.path.join(ROOT, "e1039-gpu_amd"))
 
 METRIC = "events/s/GPU, learned-pruned vs plain GPU chain, at ≥99.5% plain-track recall"
 HBM_PEAK_GBS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0   # MI355X dense bf16 matrix-core peak
 # name -> generator settings (include/e1039_or_gen.h), caps, default batch, description
 WORKLOADS = {
     "c3": dict(gen=dict(mult_mode=1, mult_mean=1200.0, mult_cap=4000), caps="high", events=40_000,
@@ -77,6 +78,15 @@ def lib_sha(path):
     return h.hexdigest()[:16]
 
 
+def kernel_sha():
+    """sha256 (16 hex digits) over the chain kernel's sources, as rocprof_summary.py stamps its counter files"""
+    h = hashlib.sha256()
+    for n in ("or_chain.hip", "or_device.h", "or_prune.h"):
+        with open(os.path.join(ROOT, "e1039-gpu_amd", "csrc", n), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()[:16]
+
+
 def params_for(oc, name):
     wl = WORKLOADS[name]
     if wl["caps"] == "reference":
@@ -103,6 +113,9 @@ def main():
     ap.add_argument("--chain", choices=["both", "plain", "pruned"], default="both",
                     help="both: time plain and pruned; plain: plain only; pruned: plain once untimed (recall), pruned timed")
     ap.add_argument("--no-e2e", action="store_true", help="skip the host-buffer (PCIe) leg")
+    ap.add_argument("--waves-per-event", type=int, default=0, choices=[0, 1, 2, 4],
+                    help="waves per event of the tile classes above 512 hits (0: the library default)")
+    ap.add_argument("--yz-batch", type=int, default=0, help="YZ survivor batch (0: the library default)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (reference for HBM byte counters) and exit")
     args = ap.parse_args()
@@ -173,6 +186,12 @@ def main():
     ev = make(name, args.events or WORKLOADS[name]["events"])
     params = params_for(oc, name)
     sess = oc.Session(geo, params, device=0)
+    if args.waves_per_event:
+        sess.set_tuning(oc.TUNE_WAVES_PER_EVENT, args.waves_per_event)
+    if args.yz_batch:
+        sess.set_tuning(oc.TUNE_YZ_LIST_CAP, args.yz_batch)
+    wpe = sess.get_tuning(oc.TUNE_WAVES_PER_EVENT)
+    yz_batch = sess.get_tuning(oc.TUNE_YZ_LIST_CAP)
     dev_in = sess.upload(ev)
     torch.cuda.synchronize()
     if args.chain == "pruned":
@@ -258,6 +277,30 @@ def main():
     del dev_in
     torch.cuda.empty_cache()
 
+    # ---- the pruner's perceptron alone on the matrix cores (or_pruner_score) --------
+    # 16-64-64-1: 2 x (16*64 + 64*64 + 64) = 10,368 FLOP per unit, bf16 operands, fp32 sums
+    mlp = None
+    if flat is not None:
+        n_rows = 1 << 24
+        frows = torch.randn((n_rows, 16), dtype=torch.float32, device="cuda")
+        sess.set_pruner(flat, thr)
+        sess.score_units(frows)
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(reps):
+            sess.score_units(frows)
+        e1.record()
+        torch.cuda.synchronize()
+        sess.set_pruner(None)
+        mlp_ms = e0.elapsed_time(e1) / reps
+        tflops = 10368.0 * n_rows / (mlp_ms * 1e-3) / 1e12
+        mlp = {"units": n_rows, "ms": mlp_ms, "achieved": tflops, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+               "frac": tflops / BF16_PEAK_TFLOPS, "flop_per_unit": 10368,
+               "note": "or_pruner_score over 16M random feature rows (64 B read, 4 B written per row)"}
+        del frows
+        log(f"pruner perceptron alone: {n_rows} units in {mlp_ms:.2f} ms, {tflops:.1f} TFLOP/s "
+            f"({tflops / BF16_PEAK_TFLOPS * 100:.2f} % of the dense bf16 peak)")
+
     # ---- the other configurations -------------------------------------------------
     others = {}
     dimuons = None
@@ -324,17 +367,33 @@ def main():
             del odev, oev
             torch.cuda.empty_cache()
 
-    # ---- HBM traffic from the committed rocprofv3 counters, when they match this build ----
-    traffic, traffic_note = None, None
+    # ---- HBM traffic and VALU use from the committed rocprofv3 counters, when they
+    # were taken on these kernel sources, settings and batch size ----
+    traffic, traffic_note, valu = None, None, None
     lib_hash = lib_sha(oc.LIB_PATH)
-    tfile = os.path.join(ROOT, "profiles", f"chain_kernel_{name}_{'pruned' if prune else 'plain'}_traffic.json")
+    ksha = kernel_sha()
+    mode = "pruned" if prune else "plain"
+    tfile = os.path.join(ROOT, "profiles", f"chain_kernel_{name}_{mode}_counters.json")
     if os.path.exists(tfile):
         with open(tfile) as f:
             tj = json.load(f)
-        if tj.get("library_sha256_16") == lib_hash and tj.get("events") == ev.n_events:
+        if (tj.get("kernel_source_sha256_16") == ksha and tj.get("events") == ev.n_events
+                and tj.get("waves_per_event") == wpe and tj.get("yz_batch") == yz_batch):
             traffic = tj.get("hbm_bytes_per_launch")
+            traffic_note = (f"{os.path.basename(tfile)}: FETCH_SIZE + WRITE_SIZE of this launch, copy-calibrated; "
+                            f"{tj['launch_ms']:.1f} ms under the profiler")
         else:
-            traffic_note = f"{os.path.basename(tfile)} was measured on another build or batch; not used"
+            traffic_note = f"{os.path.basename(tfile)} was measured on other kernel sources, settings or batch; not used"
+        pfile = os.path.join(ROOT, "profiles", f"chain_kernel_{name}_plain_counters.json")
+        if os.path.exists(pfile):
+            with open(pfile) as f:
+                pj = json.load(f)
+            if pj.get("kernel_source_sha256_16") == ksha and pj.get("derived"):
+                valu = {"busy_frac": pj["derived"]["valu_busy_frac"],
+                        "valu_insts_per_event": pj["derived"]["valu_insts_per_event"],
+                        "lds_util_frac": pj["derived"]["lds_util_frac"],
+                        "lds_bank_conflict_frac": pj["derived"]["lds_bank_conflict_frac"],
+                        "source": os.path.basename(pfile) + " (plain chain, SQ counters)"}
     else:
         traffic_note = "no counter file for this workload"
 
@@ -389,6 +448,8 @@ def main():
                       f"{margin:.3f} below each tracklet's best unit)") if prune else "plain",
             "events": ev.n_events,
             "hits": int(ev.n_hits),
+            "waves_per_event": wpe,
+            "yz_batch": yz_batch,
             "kernel_ms": kernel_ms,
             "rejected_events": rejected,
             "rejected_fraction": rejected / ev.n_events,
@@ -411,7 +472,7 @@ def main():
         },
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_note": traffic_note,
-                     "measured_copy_GBs": copy_gbs},
+                     "measured_copy_GBs": copy_gbs, "valu": valu, "pruner_mfma": mlp},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -114,7 +114,8 @@ constexpr int PAIRS_MAX = 255;        // max_pairs_per_bin, max_pairs_window
 constexpr int TKL_MAX = 16384;        // max_tracklets_per_event
 constexpr int PLANE_CAP = 400;
 constexpr int YZ_CHUNK = 16;  // tracklets whose window lists are built together
-constexpr int YZ_LIST_MIN = 4096;     // survivor list entries per chunk (buffer; longer chunks stream)
+constexpr int YZ_LIST_MIN = 4096;     // survivor list capacity per wave at session creation (longer chunks stream)
+constexpr int YZ_BATCH_DEFAULT = 1024;   // survivor batch of the YZ passes (fastest at 4 waves per event)
 constexpr int YZ_LIST_MAX = 1 << 20;
 // a tracklet whose four window lists span more than this many (u3, v3, u2, v2)
 // candidates is an enumeration overflow (keeps chunk sums within 32 bits)
@@ -3935,13 +3936,15 @@ __global__ void zero_u32(uint32_t* p, int64_t n) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = 0;
 }
 
-// one wave per 64 feature rows
-__global__ void prune_score_kernel(const PrunerDev* W, const float* feats, int64_t n, float* scores) {
-    const int64_t r = (int64_t)blockIdx.x * WAVE + threadIdx.x;
+// one wave per 64 feature rows; with pre1/pre2 also the hidden layers' pre-activations
+__global__ void prune_score_kernel(const PrunerDev* W, const float* feats, int64_t n, float* scores, float* pre1,
+                                   float* pre2) {
+    const int64_t r0 = (int64_t)blockIdx.x * WAVE;
+    const int64_t r = r0 + threadIdx.x;
     UnitFeatures f;
 #pragma unroll
     for (int q = 0; q < PRUNE_NF; ++q) f.v[q] = r < n ? feats[r * PRUNE_NF + q] : 0.f;
-    const float sc = prune_score(W, f);
+    const float sc = pre1 ? prune_eval<true>(W, f, pre1, pre2, r0, n) : prune_score(W, f);
     if (r < n) scores[r] = sc;
 }
 
@@ -3971,7 +3974,7 @@ struct or_session {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     mutable float last_ms = 0.f;
     mutable bool timing_pending = false;   // ev0/ev1 recorded, elapsed time not read yet
-    int yz_list_cap = YZ_LIST_MIN;   // survivor batch size of the YZ passes (<= lay.list_cap)
+    int yz_list_cap = YZ_BATCH_DEFAULT;   // survivor batch size of the YZ passes (<= lay.list_cap)
     int grid_pct = 100;          // persistent grid, percent of the occupancy limit
     bool use_graph = false;      // replay the launch sequence as a captured hipGraph
     hipGraphExec_t graph_exec = nullptr;
@@ -4686,7 +4689,17 @@ or_status or_pruner_score(or_session* s, const float* features, int64_t n, float
     if (n == 0) return OR_OK;
     OR_HIP(hipSetDevice(s->device));
     hipLaunchKernelGGL(prune_score_kernel, dim3((unsigned)((n + WAVE - 1) / WAVE)), dim3(WAVE), 0, s->stream,
-                       s->d_pruner, features, n, scores);
+                       s->d_pruner, features, n, scores, nullptr, nullptr);
+    OR_HIP(hipGetLastError());
+    return OR_OK;
+}
+
+or_status or_pruner_layers(or_session* s, const float* features, int64_t n, float* scores, float* pre1, float* pre2) {
+    if (!s || n < 0 || (n > 0 && (!features || !scores || !pre1 || !pre2)) || !s->prune_on) return OR_ERR_INVALID_ARG;
+    if (n == 0) return OR_OK;
+    OR_HIP(hipSetDevice(s->device));
+    hipLaunchKernelGGL(prune_score_kernel, dim3((unsigned)((n + WAVE - 1) / WAVE)), dim3(WAVE), 0, s->stream,
+                       s->d_pruner, features, n, scores, pre1, pre2);
     OR_HIP(hipGetLastError());
     return OR_OK;
 }
@@ -4705,7 +4718,7 @@ or_status or_session_set_unit_dump(or_session* s, float* rows, int64_t cap, unsi
 or_status or_session_autotune(or_session* s, const or_hits* in, or_result* out, int reps, float* best_ms) {
     if (!s || !in || !out || !in->on_device || !out->on_device || reps < 1) return OR_ERR_INVALID_ARG;
     const int wpe_logs[] = {0, 1, 2};
-    const int list_caps[] = {s->lay.list_cap, s->lay.list_cap / 2, s->lay.list_cap / 4};
+    const int list_caps[] = {s->lay.list_cap, s->lay.list_cap / 2, s->lay.list_cap / 4, s->lay.list_cap / 8};
     static const int grid_pcts[] = {75, 50};
     int best_w = s->wpe_log, best_cap = s->yz_list_cap, best_pct = s->grid_pct;
     float best = -1.f;

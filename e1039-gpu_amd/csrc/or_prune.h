@@ -53,8 +53,13 @@ struct UnitFeatures {
 };
 
 // Score of the unit whose features this lane holds.  Every lane of the wave
-// calls it (lanes without a unit pass zeros and ignore the result).
-__device__ __attribute__((noinline)) float prune_score(const PrunerDev* Wp, UnitFeatures f) {
+// calls it (lanes without a unit pass zeros and ignore the result).  With
+// LAYERS, the pre-activations (bias added, before ReLU) of both hidden layers
+// of unit u = row0 + (the lane's unit) are also written to pre1/pre2[u * 64 + i]
+// for rows below nrow (a diagnostic of the matrix-core lane layout).
+template <bool LAYERS>
+__device__ __attribute__((noinline)) float prune_eval(const PrunerDev* Wp, UnitFeatures f, float* pre1, float* pre2,
+                                                      int64_t row0, int64_t nrow) {
     // the weights live in device global memory: global loads (not flat ones, which
     // also count against the LDS counter the feature shuffles wait on)
     const __attribute__((address_space(1))) PrunerDev* W = (const __attribute__((address_space(1))) PrunerDev*)Wp;
@@ -88,7 +93,11 @@ __device__ __attribute__((noinline)) float prune_score(const PrunerDev* Wp, Unit
                 const int i0 = mt * 32 + 8 * s + 4 * half;   // 16-byte aligned row of 4
                 const float bq[4] = {W->b1[i0], W->b1[i0 + 1], W->b1[i0 + 2], W->b1[i0 + 3]};
 #pragma unroll
-                for (int q = 0; q < 4; ++q) hb[mt * 4 + s][q] = (short)bf16_rne(fmaxf(h[mt][4 * s + q] + bq[q], 0.f));
+                for (int q = 0; q < 4; ++q) {
+                    const float pre = h[mt][4 * s + q] + bq[q];
+                    if (LAYERS && row0 + nt * 32 + col < nrow) pre1[(row0 + nt * 32 + col) * PRUNE_NH + i0 + q] = pre;
+                    hb[mt * 4 + s][q] = (short)bf16_rne(fmaxf(pre, 0.f));
+                }
             }
         float p = 0.f;
 #pragma unroll
@@ -102,13 +111,21 @@ __device__ __attribute__((noinline)) float prune_score(const PrunerDev* Wp, Unit
                 const float bq[4] = {W->b2[i0], W->b2[i0 + 1], W->b2[i0 + 2], W->b2[i0 + 3]};
                 const float wq[4] = {W->w3[i0], W->w3[i0 + 1], W->w3[i0 + 2], W->w3[i0 + 3]};
 #pragma unroll
-                for (int q = 0; q < 4; ++q) p += fmaxf(o[4 * g4 + q] + bq[q], 0.f) * wq[q];
+                for (int q = 0; q < 4; ++q) {
+                    const float pre = o[4 * g4 + q] + bq[q];
+                    if (LAYERS && row0 + nt * 32 + col < nrow) pre2[(row0 + nt * 32 + col) * PRUNE_NH + i0 + q] = pre;
+                    p += fmaxf(pre, 0.f) * wq[q];
+                }
             }
         }
         // lanes col and col + 32 hold the two halves of the rows
         total[nt] = p + __shfl_xor(p, 32, 64);
     }
     return (half ? total[1] : total[0]) + W->b3;
+}
+
+__device__ __forceinline__ float prune_score(const PrunerDev* W, UnitFeatures f) {
+    return prune_eval<false>(W, f, nullptr, nullptr, 0, 0);
 }
 
 // Host-side packing of the flat weight vector (layout above) into PrunerDev.

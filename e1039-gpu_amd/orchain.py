@@ -157,6 +157,8 @@ def load_library(path: str = LIB_PATH):
     lib.or_session_set_pruner_margin.restype = i32
     lib.or_pruner_score.argtypes = [vp, vp, i64, vp]
     lib.or_pruner_score.restype = i32
+    lib.or_pruner_layers.argtypes = [vp, vp, i64, vp, vp, vp]
+    lib.or_pruner_layers.restype = i32
     lib.or_session_set_unit_dump.argtypes = [vp, vp, i64, vp]
     lib.or_session_set_unit_dump.restype = i32
     lib.or_calibration_load.argtypes = [C.c_char_p, C.POINTER(vp)]
@@ -499,6 +501,20 @@ class Session:
         _check(self.lib.or_session_set_stream(self.handle, C.c_void_p(stream.cuda_stream)))
         _check(self.lib.or_pruner_score(self.handle, C.c_void_p(f.data_ptr()), f.shape[0], C.c_void_p(out.data_ptr())))
         return out
+
+    def score_layers(self, features):
+        """or_pruner_layers: (scores, layer-1 pre-activations, layer-2 pre-activations), (n,), (n, 64), (n, 64)."""
+        import torch
+        f = features.contiguous()
+        n = f.shape[0]
+        out = torch.empty(n, dtype=torch.float32, device=f.device)
+        p1 = torch.empty((n, 64), dtype=torch.float32, device=f.device)
+        p2 = torch.empty((n, 64), dtype=torch.float32, device=f.device)
+        stream = torch.cuda.current_stream(self.device)
+        _check(self.lib.or_session_set_stream(self.handle, C.c_void_p(stream.cuda_stream)))
+        _check(self.lib.or_pruner_layers(self.handle, C.c_void_p(f.data_ptr()), n, C.c_void_p(out.data_ptr()),
+                                         C.c_void_p(p1.data_ptr()), C.c_void_p(p2.data_ptr())))
+        return out, p1, p2
 
     def set_unit_dump(self, rows=None, count=None):
         """or_session_set_unit_dump with a (cap, UNIT_ROW) float32 and a (1,) int64 device tensor (zero it first)."""

@@ -241,7 +241,7 @@ float or_session_last_kernel_ms(const or_session* s);
  * counterpart (the reference hard-codes its launch shapes). */
 typedef enum {
     /* YZ candidates surviving the first box cut that are evaluated together
-     * (one batch), 1..2^20 (default 4096).  A chunk of 16 tracklets with more
+     * (one batch), 1..2^20 (default 1024; capacity 4096 at creation).  A chunk of 16 tracklets with more
      * survivors runs in several batches cut at (u3, v3) unit boundaries; a
      * chunk in which one unit alone has more is evaluated unit by unit.
      * A value above the session's current batch capacity re-sizes its scratch
@@ -264,8 +264,8 @@ or_status or_session_set_tuning(or_session* s, or_tuning key, int64_t value);
 or_status or_session_get_tuning(const or_session* s, or_tuning key, int64_t* value);
 
 /* Times the device-resident batch (in, out) at each combination of waves per
- * event (1, 2, 4) and YZ survivor batch (the session's batch capacity, a half
- * and a quarter of it), on the full persistent grid, then the best pair at 75
+ * event (1, 2, 4) and YZ survivor batch (the session's batch capacity, a half,
+ * a quarter and an eighth of it), on the full persistent grid, then the best pair at 75
  * and 50 % of the occupancy limit; reps timed calls after one warm-up each.
  * Keeps the fastest setting in the session and returns its mean chain time in
  * *best_ms.  Outputs are those of the last call.  No reference counterpart
@@ -412,6 +412,12 @@ or_status or_session_set_pruner_margin(or_session* s, float margin);
 /* Scores n rows of 16 unit features with the session's pruner (device
  * pointers; asynchronous on the session stream). */
 or_status or_pruner_score(or_session* s, const float* features, int64_t n, float* scores);
+
+/* As or_pruner_score, and also writes both hidden layers' pre-activations
+ * (bias added, before ReLU) of each row: pre1 and pre2 hold n x 64 floats,
+ * row-major (device pointers).  A check of the matrix-core arithmetic; no
+ * reference counterpart. */
+or_status or_pruner_layers(or_session* s, const float* features, int64_t n, float* scores, float* pre1, float* pre2);
 
 /* Training aid: during device-resident calls, one row of OR_UNIT_ROW floats
  * per scored unit is appended to rows (device memory, cap rows): event index,

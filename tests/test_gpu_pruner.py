@@ -1,7 +1,8 @@
 """Learned YZ unit pruning on the device, through the C ABI.
 
 * The matrix-core perceptron (or_pruner_score) against the torch fp32
-  restatement of the same arithmetic (pruner.score_reference).
+  restatement of the same arithmetic (pruner.score_reference), and each of its
+  layers (or_pruner_layers) against float64 within the fp32 summation bound.
 * The unit-list path with every unit kept (threshold -inf) against the CPU
   oracle, bit for bit: enumerating kept units is the plain chain when nothing
   is pruned.
@@ -50,6 +51,63 @@ def test_scores_match_reference(psess):
     assert np.quantile(err, 0.999) < 1e-2, f"scores differ: {np.quantile(err, 0.999):.3e}"
     assert err.max() < 5e-2, f"worst score differs by {err.max():.3e}"
     assert np.std(ref) > 0.5   # the scores do depend on the features
+
+
+def bf16(x):
+    """float32 -> bfloat16 (round to nearest even) -> float64, as the device rounds operands"""
+    u = np.ascontiguousarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def test_hidden_layers_exact(psess):
+    # Each layer of the matrix-core perceptron against float64 on the same
+    # bf16 operands, fed with the device's own previous layer: bf16 x bf16
+    # products are exact in fp32, so the only error is the fp32 summation
+    # (at most 64 terms), bounded by 64 * 2^-24 of the sum of |terms|.  A
+    # lane-layout slip (a transposed tile, a wrong row group) moves values by
+    # the order of the terms themselves.
+    import torch
+    flat = random_weights(4).astype(np.float64)
+    rng = np.random.default_rng(5)
+    n = 1000                                                     # not a multiple of 64 rows
+    feats = rng.normal(0, 1, (n, 16)).astype(np.float32)
+    psess.set_pruner(flat.astype(np.float32), 0.0)
+    try:
+        sc, p1, p2 = psess.score_layers(torch.as_tensor(feats, device="cuda"))
+        torch.cuda.synchronize()
+        sc, p1, p2 = sc.cpu().numpy().astype(np.float64), p1.cpu().numpy().astype(np.float64), p2.cpu().numpy().astype(np.float64)
+    finally:
+        psess.set_pruner(None)
+    o = 0
+    w1 = flat[o:o + 1024].reshape(64, 16); o += 1024
+    b1 = flat[o:o + 64]; o += 64
+    w2 = flat[o:o + 4096].reshape(64, 64); o += 4096
+    b2 = flat[o:o + 64]; o += 64
+    w3 = flat[o:o + 64]; o += 64
+    b3 = flat[o]
+    f32 = lambda a: a.astype(np.float32).astype(np.float64)   # biases and the output layer are fp32 on the device
+    x, W1, W2 = bf16(feats), bf16(w1), bf16(w2)
+    ref1 = x @ W1.T + f32(b1)
+    bound1 = 64 * 2.0 ** -24 * (np.abs(x) @ np.abs(W1.T) + np.abs(f32(b1))) + 1e-30
+    assert (np.abs(p1 - ref1) <= bound1).all(), f"layer 1: worst {np.max(np.abs(p1 - ref1) / bound1):.3g} of the bound"
+    h = bf16(np.maximum(p1, 0.0))
+    ref2 = h @ W2.T + f32(b2)
+    bound2 = 64 * 2.0 ** -24 * (np.abs(h) @ np.abs(W2.T) + np.abs(f32(b2))) + 1e-30
+    assert (np.abs(p2 - ref2) <= bound2).all(), f"layer 2: worst {np.max(np.abs(p2 - ref2) / bound2):.3g} of the bound"
+    r = np.maximum(p2, 0.0)
+    ref3 = r @ f32(w3) + f32(np.array(b3))
+    bound3 = 70 * 2.0 ** -24 * (r @ np.abs(f32(w3)) + abs(f32(np.array(b3)))) + 1e-30
+    assert (np.abs(sc - ref3) <= bound3).all(), f"output: worst {np.max(np.abs(sc - ref3) / bound3):.3g} of the bound"
+    # the layers are not degenerate: both signs before ReLU, scores spread
+    assert (p1 > 0).mean() > 0.2 and (p1 < 0).mean() > 0.2 and (p2 > 0).mean() > 0.1 and np.std(sc) > 0.5
+    # and or_pruner_score gives the same scores
+    psess.set_pruner(flat.astype(np.float32), 0.0)
+    try:
+        only = psess.score_units(torch.as_tensor(feats, device="cuda")).cpu().numpy()
+    finally:
+        psess.set_pruner(None)
+    assert np.array_equal(only.view(np.uint32), sc.astype(np.float32).view(np.uint32))
 
 
 def same_outputs(got, ref):
