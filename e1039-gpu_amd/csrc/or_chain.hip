@@ -114,6 +114,7 @@ constexpr int PAIRS_MAX = 255;        // max_pairs_per_bin, max_pairs_window
 constexpr int TKL_MAX = 16384;        // max_tracklets_per_event
 constexpr int PLANE_CAP = 400;
 constexpr int YZ_CHUNK = 16;  // tracklets whose window lists are built together
+static_assert((YZ_CHUNK & (YZ_CHUNK - 1)) == 0, "YZ_CHUNK: a power of two (binary searches over a chunk)");
 constexpr int YZ_LIST_MIN = 4096;     // survivor list capacity per wave at session creation (longer chunks stream)
 constexpr int YZ_BATCH_DEFAULT = 1024;   // survivor batch of the YZ passes (fastest at 4 waves per event)
 constexpr int YZ_LIST_MAX = 1 << 20;
@@ -185,7 +186,18 @@ struct YzWave {
     int uoff[YZ_CHUNK + 1];   // prefix of (u3, v3) units or of candidates per tracklet of the chunk
     int umax[YZ_CHUNK];       // pruning: best unit score of each tracklet of the chunk (order-preserving int)
     unsigned long long best[YZ_CHUNK];
+    // 1/d rounded to double of the enumeration radices (see qdiv): u2 list
+    // length, (u2, v2) pairs per unit, u3 list length
+    double rx[YZ_CHUNK], rin[YZ_CHUNK], rz[YZ_CHUNK];
 };
+
+// floor(a / d) for 0 <= a < 2^27 and 1 <= d < 2^16, from r = 1/d rounded to
+// double.  a*r + 2^-20 (one rounding, fused) lies within 2^-25 of
+// a/d + 2^-20: above floor(a/d), and below floor(a/d) + 1 because the fraction
+// of a/d is 0 or at most 1 - 1/d < 1 - 2^-16.  The YZ enumeration indices stay
+// below 2^27 (the enumeration bound) and the radices below 255^2.
+__device__ __forceinline__ int qdiv(int a, double r) { return (int)__builtin_fma((double)a, r, 0x1p-20); }
+__device__ __forceinline__ double qrcp(int d) { return 1.0 / (double)(d > 0 ? d : 1); }
 
 // The event tile.  Hits are staged grouped by detector; after the reducer the
 // kept ones are compacted in place, so plane d's reduced hits are
@@ -1298,14 +1310,18 @@ template <int MAXH, int WPE>
 __device__ __forceinline__ YzItem yz_decode(int i) {
     const YzWave& Y = yzw<MAXH>();
     YzItem it;
+    // the last tracklet whose candidates start at or before i (empty ones start
+    // where the next does, so the one holding i is the last)
     int j = 0;
-    while (Y.uoff[j + 1] <= i) ++j;
+#pragma unroll
+    for (int st = YZ_CHUNK / 2; st > 0; st >>= 1)
+        if (Y.uoff[j + st] <= i) j += st;
     it.j = j;
     it.local = i - Y.uoff[j];
     const int n0 = Y.nl[j][0], nin = n0 * Y.nl[j][1];
-    const int iuv3 = it.local / nin, iuv2 = it.local - iuv3 * nin;
-    it.iu2 = iuv2 % n0; it.iv2 = iuv2 / n0;
-    it.iu3 = iuv3 % Y.nl[j][2]; it.iv3 = iuv3 / Y.nl[j][2];
+    const int iuv3 = qdiv(it.local, Y.rin[j]), iuv2 = it.local - iuv3 * nin;
+    it.iv2 = qdiv(iuv2, Y.rx[j]); it.iu2 = iuv2 - it.iv2 * n0;
+    it.iv3 = qdiv(iuv3, Y.rz[j]); it.iu3 = iuv3 - it.iv3 * Y.nl[j][2];
     return it;
 }
 
@@ -1386,7 +1402,7 @@ template <int MAXH, int WPE>
 __device__ __forceinline__ UnitFeatures unit_features(const Scr& S, const Tk& T, int j, int iuv3, const int4 nl) {
     const Ev<MAXH, WPE> E;
     const int win = S.win();
-    const int iu3 = iuv3 % nl.z, iv3 = iuv3 / nl.z;
+    const int iv3 = qdiv(iuv3, yzw<MAXH>().rz[j]), iu3 = iuv3 - iv3 * nl.z;
     const int qu = (j * 4 + 2) * win + iu3, qv = (j * 4 + 3) * win + iv3;
     const float4 yu = gld4(S.wy() + qu), yw = gld4(S.wy() + qv);
     const unsigned fu = gld(S.wf() + qu), fv = gld(S.wf() + qv);
@@ -1722,7 +1738,8 @@ __device__ __attribute__((noinline)) int yz_pass1_kept(Scr S, int kb, int ntkl, 
             int ojs = off[0];
 #pragma unroll
             for (int t = 1; t < YZ_CHUNK; ++t) ojs = j == t ? off[t] : ojs;
-            const int e[4] = {inner % nl.x, inner / nl.x, iuv3 % nl.z, iuv3 / nl.z};
+            const int v2 = qdiv(inner, Y.rx[j]), v3 = qdiv(iuv3, Y.rz[j]);
+            const int e[4] = {inner - v2 * nl.x, v2, iuv3 - v3 * nl.z, v3};
             ustart = ojs + iuv3 * (nl.x * nl.y);
             iorig = ustart + inner;
             pass = yz_first_cut(S, j, e, (d3m >> j) & 1, zv);
@@ -1785,9 +1802,10 @@ __device__ __attribute__((noinline)) int yz_pass1_plain(Scr S, int kb, int ntkl,
 #pragma unroll
             for (int t = 1; t < YZ_CHUNK; ++t) ojs = j == t ? off[t] : ojs;
             const int local = i - ojs, nin = nl.x * nl.y;
-            const int iuv3 = local / nin, iuv2 = local - iuv3 * nin;
+            const int iuv3 = qdiv(local, Y.rin[j]), iuv2 = local - iuv3 * nin;
             ustart = ojs + iuv3 * nin;
-            const int e[4] = {iuv2 % nl.x, iuv2 / nl.x, iuv3 % nl.z, iuv3 / nl.z};  // u2, v2, u3, v3
+            const int v2 = qdiv(iuv2, Y.rx[j]), v3 = qdiv(iuv3, Y.rz[j]);
+            const int e[4] = {iuv2 - v2 * nl.x, v2, iuv3 - v3 * nl.z, v3};  // u2, v2, u3, v3
             pass = yz_first_cut(S, j, e, (d3m >> j) & 1, zv);
         }
         const unsigned long long m = ballot(pass);
@@ -1871,6 +1889,10 @@ __device__ __attribute__((noinline)) bool yz_chunk_par(Scr S, Prn R, int kb, int
         if (j < YZ_CHUNK) {
             Y.best[j] = ~0ull;
             Y.uoff[j + 1] = inc;
+            const int* nl = Y.nl[j];
+            Y.rx[j] = qrcp(nl[0]);
+            Y.rin[j] = qrcp(nl[0] * nl[1]);
+            Y.rz[j] = qrcp(nl[2]);
         }
         if (j == 0) Y.uoff[0] = 0;
     }
@@ -1913,11 +1935,10 @@ __device__ __attribute__((noinline)) bool yz_chunk_par(Scr S, Prn R, int kb, int
             const int k = k0 + lane;
             if (k < np) {
                 const YzItem it = yz_decode<MAXH, WPE>((int)S.plist()[k]);
-                const int nin = Y.nl[it.j][0] * Y.nl[it.j][1];
                 bool first = k == 0;
                 if (!first) {
                     const YzItem pt = yz_decode<MAXH, WPE>((int)S.plist()[k - 1]);
-                    first = pt.j != it.j || (pt.local / nin) != (it.local / nin);
+                    first = pt.j != it.j || pt.iu3 != it.iu3 || pt.iv3 != it.iv3;
                 }
                 const Tk* Tp = &tks[kb + it.j];
                 YzSlots H;
