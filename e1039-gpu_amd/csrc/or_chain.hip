@@ -2362,17 +2362,57 @@ __device__ __attribute__((noinline)) void clean_many(Tk* tks, const uint16_t* sl
         const int sa = (int)((gld4(ent + 3 * pa + 2).y >> 16) & 0xFFu);
         step(pa, pa + 1, L.cl.lstart[sa + 1]);
     }
-    // slot against slot (reconstruction_helper.cuh:1846-1909): for s, for o != s, for i in s, for j in o
+    // slot against slot (reconstruction_helper.cuh:1846-1909): for s, for o != s,
+    // for i in s, for j in o.  A step (i, o) reads and changes only the alive
+    // bits of i and of o's tracklets, so steps of different i and different o
+    // commute: any order that keeps the steps of one i in o order and those of
+    // one o in i order gives the same result.  Here, for s, for i in s, all o at
+    // once: the j of every other slot in one pass (positions are in slot order),
+    // shared hits summed per slot by a segmented scan, and the first slot whose
+    // step demotes i is i's last (the j demotions of the slots after it are not
+    // applied, as those steps do not run for a dead i).
+    auto cross = [&](int pa, int sb, int se) {
+        const CleanEnt A = clean_load(ent, pa);
+        const int na = (int)(A.kn >> 24);
+        int run = 0, base_carry = 0, mlim = 32;
+        for (int c0 = 0; c0 < nlive; c0 += WAVE) {
+            const int j = c0 + lane;
+            const bool in = j < nlive && (j < sb || j >= se);
+            const bool al = in && ((L.cl.alive[j >> 6] >> (j & 63)) & 1ull);
+            int c = 0, o = 32;
+            CleanEnt B;
+            if (in) {
+                B = clean_load(ent, j);
+                o = (int)((B.kn >> 16) & 0xFFu);
+            }
+            // past the slot that demoted i: nothing is left
+            if (mlim < 32 && ballot(in && o <= mlim) == 0ull) break;
+            if (al) c = clean_shared9(A.h, B.h);
+            const int incl = run + wave_scan(c);
+            const bool head = in && j == L.cl.lstart[o];
+            const int base = max(base_carry, wave_max_scan(head ? incl - c : -1));
+            const int common = incl - base;
+            bool dj = false, dl = false;
+            if (al) {
+                if (A.ratio < B.ratio) dj = (float)common > (float)na * 0.3333f;
+                else dl = (float)common > (float)(int)(B.kn >> 24) * 0.3333f;
+            }
+            const unsigned long long dlm = ballot(dl && o <= mlim);
+            if (dlm) mlim = min(mlim, __shfl(o, __ffsll((long long)dlm) - 1, WAVE));
+            wsync();
+            if (dj && o <= mlim) atomicAnd(&L.cl.alive[j >> 6], ~(1ull << (j & 63)));
+            run = __shfl(incl, WAVE - 1, WAVE);
+            base_carry = __shfl(base, WAVE - 1, WAVE);
+        }
+        if (mlim < 32 && lane == 0) atomicAnd(&L.cl.alive[pa >> 6], ~(1ull << (pa & 63)));
+        wsync();
+    };
     for (int sidx = 0; sidx < 32; ++sidx) {
         const int sb = L.cl.lstart[sidx], se = L.cl.lstart[sidx + 1];
-        if (sb == se) continue;
-        for (int oidx = 0; oidx < 32; ++oidx) {
-            const int ob = L.cl.lstart[oidx], oe = L.cl.lstart[oidx + 1];
-            if (oidx == sidx || ob == oe) continue;
-            for (int pa = sb; pa < se; ++pa) {
-                if (!((L.cl.alive[pa >> 6] >> (pa & 63)) & 1ull)) continue;
-                step(pa, ob, oe);
-            }
+        if (sb == se || se - sb == nlive) continue;
+        for (int pa = sb; pa < se; ++pa) {
+            if (!((L.cl.alive[pa >> 6] >> (pa & 63)) & 1ull)) continue;
+            cross(pa, sb, se);
         }
     }
     for (int p = lane; p < nlive; p += WAVE)
