@@ -124,10 +124,11 @@ constexpr int64_t YZ_ENUM_MAX = 1ll << 27;
 constexpr int NCLASS_MAX = 5;    // event tile classes: 512, 1024, 2048, 4096, 8192 hits
 
 // Sub-stage cycle counters of the profiling build (-DOR_STAGE_TIMING); no-ops
-// otherwise.  Indices: YZ window lists, YZ pass 1, 2, 3, YZ winners, YZ unit
-// path, global window lists, global pass A, global pass B and write, YZ unit
-// scoring.
-constexpr int N_SUB_TIMERS = 16;
+// otherwise.  Indices: YZ window lists, YZ pass 1, 2, 3 (its left/right and
+// chi2 part), YZ winners, YZ unit path, global window lists, global pass A,
+// global pass B and write, YZ unit scoring, then counts (see OR_COUNT), pass 3's
+// sign-free tests (cycles) and the survivors passing them (count).
+constexpr int N_SUB_TIMERS = 18;
 #ifdef OR_STAGE_TIMING
 __device__ unsigned long long g_sub_cycles[N_SUB_TIMERS];
 #define OR_SUB_START unsigned long long s_0 = clock64(), s_1;
@@ -1945,7 +1946,10 @@ __device__ __attribute__((noinline)) bool yz_chunk_par(Scr S, Prn R, int kb, int
                 Fit f;
                 yz_candidate(E, S, Tp, it, H, f);
                 yz_cache_put(S.ycache() + 3 * k, H, f);
-                uchar4 ps = make_uchar4(first ? 1 : 0, 0, 0, 0);
+                // the tests that need no sign (second box cut, prop-tube Y,
+                // hodoscopes) depend on the first fit and the tracklet only
+                const bool go = yz_cuts(E, Tp, f);
+                uchar4 ps = make_uchar4((first ? 1 : 0) | (go ? 2 : 0), 0, 0, 0);
                 if (first) {
                     lr_slots_new<8>(Tp->x0, Tp->tx, f.p0, f.p1, Tp->ex0, Tp->etx, f.e0, f.e1, H.pres, H.det, H.pos,
                                     H.drift, H.sign, 150.f);
@@ -1962,46 +1966,37 @@ __device__ __attribute__((noinline)) bool yz_chunk_par(Scr S, Prn R, int kb, int
         wsync();
         OR_SUB(2);
         // pass 3: starting signs by scan (the unit's seed, and the last writer of
-        // the v3' sign before this candidate) and the tests that need no sign; the
-        // candidates that pass them are listed and run the rest (left/right, chi2)
+        // the v3' sign before this candidate); the candidates that passed the
+        // sign-free tests in pass 2 are listed and run the rest (left/right, chi2)
         int carry_f = -1, carry_w = -1, nq = 0;
         for (int k0 = 0; k0 < np; k0 += WAVE) {
             const int k = k0 + lane;
             const bool valid = k < np;
             uchar4 ps = make_uchar4(0, 0, 0, 0);
             if (valid) ps = S.pst()[k];
-            int fi = wave_max_scan(valid && ps.x ? k : -1);
+            const bool first = (ps.x & 1u) != 0;
+            int fi = wave_max_scan(valid && first ? k : -1);
             fi = max(fi, carry_f);
-            const int wincl = wave_max_scan(valid && (ps.x || ps.z) ? k : -1);
+            const int wincl = wave_max_scan(valid && (first || ps.z) ? k : -1);
             int wexcl = __shfl_up(wincl, 1, WAVE);
             if (lane == 0) wexcl = -1;
             wexcl = max(wexcl, carry_w);
             carry_f = __shfl(fi, WAVE - 1, WAVE);
             carry_w = max(carry_w, __shfl(wincl, WAVE - 1, WAVE));
-            bool go = false;
-            if (valid) {
-                const YzItem it = yz_decode<MAXH, WPE>((int)S.plist()[k]);
-                const uint4 b = S.ycache()[3 * k + 1];
-                Fit f;
-                f.p0 = __uint_as_float(b.y);
-                f.p1 = __uint_as_float(b.z);
-                f.e0 = __uint_as_float(b.w);
-                f.e1 = __uint_as_float(S.ycache()[3 * k + 2].x);
-                uint8_t st = 0;
-                if (!ps.x) {
-                    // seed of the unit with the v3' sign (slot 3) of the last writer; the
-                    // first survivor of the unit is a writer, so wexcl >= fi
-                    st = (uint8_t)((S.pst()[fi].y & 0x3Fu) | ((unsigned)S.pst()[wexcl].z << 6));
-                }
-                ps.w = st;
+            const bool go = valid && (ps.x & 2u);
+            if (go && !first) {
+                // seed of the unit with the v3' sign (slot 3) of the last writer; the
+                // first survivor of the unit is a writer, so wexcl >= fi
+                ps.w = (uint8_t)((S.pst()[fi].y & 0x3Fu) | ((unsigned)S.pst()[wexcl].z << 6));
                 S.pst()[k] = ps;
-                go = yz_cuts(E, &tks[kb + it.j], f);
             }
             const unsigned long long m = ballot(go);
             if (go) S.qlist()[nq + rank_in(m)] = (uint32_t)k;
             nq += __popcll(m);
         }
         wsync();
+        OR_SUB(15);
+        OR_COUNT(16, nq);
         for (int q0 = 0; q0 < nq; q0 += WAVE) {
             const int q = q0 + lane;
             if (q < nq) {
@@ -2012,7 +2007,7 @@ __device__ __attribute__((noinline)) bool yz_chunk_par(Scr S, Prn R, int kb, int
                 YzSlots H;
                 Fit f;
                 yz_cache_get(E, S.ycache() + 3 * k, Tp->st3, H, f);
-                if (!ps.x) yz_unpack3(H, ps.w);
+                if (!(ps.x & 1u)) yz_unpack3(H, ps.w);
                 lr_slots_new<8>(Tp->x0, Tp->tx, f.p0, f.p1, Tp->ex0, Tp->etx, f.e0, f.e1, H.pres, H.det, H.pos,
                                 H.drift, H.sign, 150.f);
                 float chi2;
@@ -2066,7 +2061,7 @@ __device__ __attribute__((noinline)) bool yz_chunk_par(Scr S, Prn R, int kb, int
             YzSlots H;
             Fit f;
             yz_cache_get(E, br, Tp->st3, H, f);
-            if (!(psw & 0xFFu)) yz_unpack3(H, psw >> 24);
+            if (!(psw & 1u)) yz_unpack3(H, psw >> 24);
             float chi2;
             yz_tail(E, Tp, H, f, chi2, Tp);
             const int local = (int)(uint32_t)key;

@@ -21,7 +21,8 @@ STAGES = ["load+reduce", "X pairs", "XZ", "YZ", "cleaning", "global", "outputs"]
 SUBS = ["YZ window lists", "YZ pass 1 (first fit)", "YZ pass 2 (seeds)", "YZ pass 3 (tails)", "YZ winners",
         "YZ unit path", "st1 window lists", "st1 pass A (y fits)", "st1 pass B + write",
         "YZ unit scoring (in pass 1)", "YZ candidates (count)", "YZ units scored (count)",
-        "YZ units kept (count)", "YZ candidates of kept units (count)", "YZ first-cut survivors (count)"]
+        "YZ units kept (count)", "YZ candidates of kept units (count)", "YZ first-cut survivors (count)",
+        "YZ pass 3 sign-free tests (in pass 3)", "YZ pass 3 survivors of them (count)"]
 
 
 def main():
@@ -56,7 +57,7 @@ def main():
     lib.or_debug_sub_cycles.argtypes = [C.c_void_p, C.c_int]
     lib.or_debug_sub_cycles.restype = C.c_int32
     buf = (C.c_ulonglong * 35)()
-    sub = (C.c_ulonglong * 16)()
+    sub = (C.c_ulonglong * 18)()
     assert lib.or_debug_stage_cycles(buf, 1) == 0, "not the profiling build"
     lib.or_debug_sub_cycles(sub, 1)
     sess.run_device(din, dout)

@@ -451,7 +451,7 @@ or_status or_debug_stage_cycles(unsigned long long* out /* 35 */, int reset);
  * tile classes): YZ window lists, YZ passes 1-3, YZ winners, YZ unit path,
  * station-1 window lists, station-1 candidate loop, track write; 16 slots.
  * Same build rule as or_debug_stage_cycles.  No reference counterpart. */
-or_status or_debug_sub_cycles(unsigned long long* out /* 16 */, int reset);
+or_status or_debug_sub_cycles(unsigned long long* out /* 18 */, int reset);
 
 /* Sets the condition bound above which the final fit solves in FP64 (0 sends
  * every track to the FP64 path).  The bound is held per device and shared by
