@@ -187,6 +187,7 @@ struct YzWave {
     int uoff[YZ_CHUNK + 1];   // prefix of (u3, v3) units or of candidates per tracklet of the chunk
     int umax[YZ_CHUNK];       // pruning: best unit score of each tracklet of the chunk (order-preserving int)
     unsigned long long best[YZ_CHUNK];
+    uint32_t hx[YZ_CHUNK];    // hodoscope x-match mask of each tracklet of the chunk (yz_cuts_hx)
     // 1/d rounded to double of the enumeration radices (see qdiv): u2 list
     // length, (u2, v2) pairs per unit, u3 list length
     double rx[YZ_CHUNK], rin[YZ_CHUNK], rz[YZ_CHUNK];
@@ -204,6 +205,12 @@ __device__ __forceinline__ double qrcp(int d) { return 1.0 / (double)(d > 0 ? d 
 // kept ones are compacted in place, so plane d's reduced hits are
 // [rstart[d], rstart[d] + rcnt[d]) of elem/pos/drift/orig.  Sized so that the
 // 512-hit tile stays within 8 KB (20 waves per CU).
+// Prop-tube Y planes of the YZ stage's tests (P1Y 48, 47; P2Y 54, 53): their
+// reduced hit positions sorted, per event, when a plane holds at most
+// psort_cap hits (else the plane is scanned hit by hit)
+__host__ __device__ constexpr int psort_cap(int maxh) { return maxh <= 512 ? 32 : 128; }
+__device__ __forceinline__ constexpr int psort_plane(int p) { return p == 0 ? 48 : p == 1 ? 47 : p == 2 ? 54 : 53; }
+
 template <int MAXH>
 struct Lds {
     int16_t elem[MAXH];
@@ -215,6 +222,8 @@ struct Lds {
     int pcnt[56];
     int16_t pstart[56], cursor[56], rcnt[56], rstart[56];
     uint16_t xcnt[3][28];
+    float psort[4][psort_cap(MAXH)];   // sorted positions of plane psort_plane(p)
+    int pscnt[4];                      // their count, -1: more than psort_cap hits
     union alignas(16) {
         YzWave yw[lds_waves(MAXH)];   // YZ: one per wave; station 1 reads yw[0].nl as 64 list lengths
         uint16_t live[WAVE];      // cleaning: tracklets at stationID >= 5, slot order
@@ -440,6 +449,59 @@ __device__ bool hodo_tkl(const Ev<MAXH, WPE>& E, int st, int d, float x0, float 
     return false;
 }
 
+// The x half of hodo_tkl: whether some hit of plane d matches the tracklet's
+// x line (x0, tx and their errors) with the window widening accumulated over
+// the hits in order.  hodo_tkl's y window is the same for every hit of a plane
+// (plane ends widened by the fudge factor and the y error), so hodo_tkl is
+// (this) && hodo_y; in the YZ stage the x line is the tracklet's and fixed, so
+// this half is computed once per tracklet and plane.
+template <int MAXH, int WPE>
+__device__ bool hodo_x(const Ev<MAXH, WPE>& E, int st, int d, float x0, float tx, float ex0, float etx) {
+    const int n = E.nred(d);
+    if (n == 0) return false;
+    if (d > 40) st = 3;
+    const float f = hodo_fudge(st), cw = G(F_CELLWIDTH, d);
+    const float xh = G(F_Z, d) * tx + x0;
+    float ex = 3.f * (fabsf(G(F_Z, d) * etx) + ex0);
+    const int r0 = E.raw(d, 0);
+    for (int i0 = 0; i0 < n; i0 += 4) {
+        float pv[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) pv[k] = E.Lp->pos[r0 + min(i0 + k, n - 1)];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (i0 + k >= n) break;
+            const float p = pv[k];
+            float xmin = p - cw * 0.5f, xmax = p + cw * 0.5f;
+            xmin -= (xmax - xmin) * f;
+            xmax += (xmax - xmin) * f;
+            ex += (xmax - xmin) * 0.15f;
+            xmin -= ex;
+            xmax += ex;
+            if (xmin <= xh && xh <= xmax) return true;
+        }
+    }
+    return false;
+}
+
+// the y half of hodo_tkl (the window every hit of plane d shares)
+__device__ __forceinline__ bool hodo_y(int st, int d, float y0, float ty, float ey0, float ety) {
+    if (d > 40) st = 3;
+    const float f = hodo_fudge(st);
+    const float yh = G(F_Z, d) * ty + y0;
+    const float ey = 3.f * (fabsf(G(F_Z, d) * ety) + ey0);
+    float ymin = G(F_Y1, d), ymax = G(F_Y2, d);
+    ymin -= (ymax - ymin) * f;
+    ymax += (ymax - ymin) * f;
+    ymin -= ey;
+    ymax += ey;
+    return ymin <= yh && yh <= ymax;
+}
+
+// hodoscope planes of the YZ stage's matches, in the order they are tested:
+// bit b of a tracklet's x-match mask is plane yz_hodo(b) (37, 38, 39, 40, 45, 46)
+__device__ __forceinline__ constexpr int yz_hodo(int b) { return b < 4 ? 37 + b : 41 + b; }
+
 // a prop-tube hit within win of the expected coordinate, plane a then b; hit
 // positions read four at a time so one LDS wait covers four hits
 template <int MAXH, int WPE>
@@ -460,6 +522,38 @@ template <int MAXH, int WPE>
 __device__ __forceinline__ int prop_hit(const Ev<MAXH, WPE>& E, int da, int db, float slope, float icpt, float win) {
     if (prop_plane_hit(E, da, slope * G(F_Z, da) + icpt, win)) return 1;
     return prop_plane_hit(E, db, slope * G(F_Z, db) + icpt, win) ? 1 : 0;
+}
+
+// prop_plane_hit over the sorted positions s[0, n) of a plane (n wave-uniform).
+// fl(p - e) is non-decreasing in p, so the smallest |fl(p - e)| over the plane
+// is that of the last position below e or of the first at or above it: the
+// test of those two is the test of every hit.  NaN or infinite e fails both,
+// as it fails every hit.
+__device__ __forceinline__ bool prop_sorted_hit(const float* s, int n, float e, float win) {
+    int lo = 0, len = n;
+    while (len > 0) {
+        const int half = len >> 1;
+        if (s[lo + half] < e) {
+            lo += half + 1;
+            len -= half + 1;
+        } else {
+            len = half;
+        }
+    }
+    const float below = s[max(lo - 1, 0)], above = s[min(lo, max(n - 1, 0))];
+    return (lo > 0 && fabsf(below - e) < win) || (lo < n && fabsf(above - e) < win);
+}
+
+// prop_hit for the planes psort_plane(2 pp) then psort_plane(2 pp + 1)
+template <int MAXH, int WPE>
+__device__ __forceinline__ int prop_hit_y(const Ev<MAXH, WPE>& E, int pp, float slope, float icpt, float win) {
+    const Lds<MAXH>& L = g_lds<MAXH>;
+    const int da = psort_plane(2 * pp), db = psort_plane(2 * pp + 1);
+    const int na = L.pscnt[2 * pp], nb = L.pscnt[2 * pp + 1];
+    const float ea = slope * G(F_Z, da) + icpt;
+    if (na >= 0 ? prop_sorted_hit(L.psort[2 * pp], na, ea, win) : prop_plane_hit(E, da, ea, win)) return 1;
+    const float eb = slope * G(F_Z, db) + icpt;
+    return (nb >= 0 ? prop_sorted_hit(L.psort[2 * pp + 1], nb, eb, win) : prop_plane_hit(E, db, eb, win)) ? 1 : 0;
 }
 
 // ---- window pair list (make_hitpairs_in_station, reconstruction_helper.cuh:464-549)
@@ -767,6 +861,22 @@ __device__ __forceinline__ bool yz_cuts(const Ev<MAXH, WPE>& E, const Tk* Tp, co
     if (!hodo_tkl(E, 3, 45, x0, y0, tx, ty, ex0, ey0, etx, ety) &&
         !hodo_tkl(E, 3, 46, x0, y0, tx, ty, ex0, ey0, etx, ety))
         return false;
+    return true;
+}
+
+// yz_cuts with the hodoscope x halves given as the tracklet's mask (bit b:
+// plane yz_hodo(b) matches the x line): the same decision
+template <int MAXH, int WPE>
+__device__ __forceinline__ bool yz_cuts_hx(const Ev<MAXH, WPE>& E, unsigned hx, const Fit& f) {
+    const float y0 = f.p0, ty = f.p1, ey0 = f.e0, ety = f.e1;
+    if (fabsf(y0) > Y0_MAX || fabsf(ty) > TY_MAX) return false;
+    const int nprop = prop_hit_y(E, 0, ty, y0, 5.08f) + prop_hit_y(E, 1, ty, y0, 10.16f);
+    if (nprop < 2) return false;
+#pragma unroll
+    for (int s = 0; s < 3; ++s) {
+        const bool a = ((hx >> (2 * s)) & 1u) && hodo_y(s + 1, yz_hodo(2 * s), y0, ty, ey0, ety);
+        if (!a && !(((hx >> (2 * s + 1)) & 1u) && hodo_y(s + 1, yz_hodo(2 * s + 1), y0, ty, ey0, ety))) return false;
+    }
     return true;
 }
 
@@ -1294,6 +1404,36 @@ STAGE int stage_xz(or_params P, Scr S) {
 // evaluated in parallel (yz_chunk_par).  yz_chunk_seq walks units one lane
 // each, in reference order, for chunks with more survivors than the list holds.
 
+// Sorted prop-tube Y positions of the event (rank sort, ties in plane order),
+// for the YZ stage's tests; one wave.
+template <int MAXH, int WPE>
+STAGE void stage_prop_sort() {
+    Lds<MAXH>& L = g_lds<MAXH>;
+    Ev<MAXH, WPE> E;
+    const int lane = lane_id();
+    for (int p = 0; p < 4; ++p) {
+        const int d = psort_plane(p), n = E.nred(d), r0 = E.raw(d, 0);
+        if (n > psort_cap(MAXH)) {
+            if (lane == 0) L.pscnt[p] = -1;
+            continue;
+        }
+        for (int i0 = 0; i0 < n; i0 += WAVE) {
+            const int i = i0 + lane;
+            if (i < n) {
+                const float v = L.pos[r0 + i];
+                int rank = 0;
+                for (int k = 0; k < n; ++k) {
+                    const float u = L.pos[r0 + k];
+                    rank += (u < v || (u == v && k < i)) ? 1 : 0;
+                }
+                L.psort[p][rank] = v;
+            }
+        }
+        if (lane == 0) L.pscnt[p] = n;
+    }
+    wsync();
+}
+
 template <int MAXH, int WPE>
 __device__ __forceinline__ void yz_lists_of(const Scr& S, int j, YzLists& W) {
     for (int v = 0; v < 4; ++v) {
@@ -1459,7 +1599,7 @@ __device__ __forceinline__ UnitFeatures unit_features(const Scr& S, const Tk& T,
             rep_pass = 1;
             lr_slots_new<8>(T.x0, T.tx, fr.p0, fr.p1, T.ex0, T.etx, fr.e0, fr.e1, H.pres, H.det, H.pos, H.drift,
                             H.sign, 150.f);
-            if (yz_cuts(E, &T, fr)) rep_pass = 2;
+            if (yz_cuts_hx(E, yzw<MAXH>().hx[j], fr)) rep_pass = 2;
             yz_chi2(E, &T, H, fr, chi, nullptr);
         }
     }
@@ -1868,6 +2008,152 @@ __device__ __forceinline__ void yz_cache_get(const Ev<MAXH, WPE>& E, const uint4
     f.e1 = __uint_as_float(d.x);
 }
 
+// Pass 2 over the survivors k < np of a batch: per survivor, first-of-unit
+// flag, the candidate cache entry, the sign-free tests and the v3' / u2 pair
+// decision.  Returns the number of first survivors, listed in the pass-3 list
+// (free until then) for yz_pass2_firsts.
+template <int MAXH, int WPE>
+__device__ __attribute__((noinline)) int yz_pass2(Scr S, int kb, int np) {
+    Lds<MAXH>& L = g_lds<MAXH>;
+    YzWave& Y = yzw<MAXH>();
+    Ev<MAXH, WPE> E;
+    const int lane = lane_id();
+    Tk* tks = S.tks();
+    int nf = 0;
+    for (int k0 = 0; k0 < np; k0 += WAVE) {
+        const int k = k0 + lane;
+        bool first = false;
+        if (k < np) {
+            const YzItem it = yz_decode<MAXH, WPE>((int)S.plist()[k]);
+            first = k == 0;
+            if (!first) {
+                const YzItem pt = yz_decode<MAXH, WPE>((int)S.plist()[k - 1]);
+                first = pt.j != it.j || pt.iu3 != it.iu3 || pt.iv3 != it.iv3;
+            }
+            const Tk* Tp = &tks[kb + it.j];
+            YzSlots H;
+            Fit f;
+            yz_candidate(E, S, Tp, it, H, f);
+            yz_cache_put(S.ycache() + 3 * k, H, f);
+            // the tests that need no sign (second box cut, prop-tube Y,
+            // hodoscopes) depend on the first fit and the tracklet only
+            const bool go = yz_cuts_hx(E, Y.hx[it.j], f);
+            uchar4 ps = make_uchar4((first ? 1 : 0) | (go ? 2 : 0), 0, 0, 0);
+            if (!first && ((lr_slot_starts<8>(H.pres, H.det) >> 3) & 1u)) {
+                const int h = lr_pair_decide(Tp->x0, Tp->tx, f.p0, f.p1, Tp->ex0, Tp->etx, f.e0, f.e1, H.det[3],
+                                             H.det[4], H.pos[3], H.pos[4], H.drift[3], H.drift[4], 150.f);
+                if (h >= 0) ps.z = (uint8_t)(lr_si(h) & 3);
+            }
+            S.pst()[k] = ps;
+        }
+        const unsigned long long fm = ballot(first);
+        if (first) S.qlist()[nf + rank_in(fm)] = (uint32_t)k;
+        nf += __popcll(fm);
+    }
+    return nf;
+}
+
+// The first survivors of the units (listed by yz_pass2) run the full
+// left/right from zero signs; their station-3 signs seed the unit.
+template <int MAXH, int WPE>
+__device__ __attribute__((noinline)) void yz_pass2_firsts(Scr S, int kb, int nf) {
+    Lds<MAXH>& L = g_lds<MAXH>;
+    YzWave& Y = yzw<MAXH>();
+    Ev<MAXH, WPE> E;
+    const int lane = lane_id();
+    Tk* tks = S.tks();
+    for (int q0 = 0; q0 < nf; q0 += WAVE) {
+        const int q = q0 + lane;
+        if (q < nf) {
+            const int k = (int)S.qlist()[q];
+            const YzItem it = yz_decode<MAXH, WPE>((int)S.plist()[k]);
+            const Tk* Tp = &tks[kb + it.j];
+            YzSlots H;
+            Fit f;
+            yz_cache_get(E, S.ycache() + 3 * k, Tp->st3, H, f);
+            lr_slots_new<8>(Tp->x0, Tp->tx, f.p0, f.p1, Tp->ex0, Tp->etx, f.e0, f.e1, H.pres, H.det, H.pos,
+                            H.drift, H.sign, 150.f);
+            uchar4 ps = S.pst()[k];
+            ps.y = yz_pack3(H);
+            ps.z = (uint8_t)(H.sign[3] & 3);
+            S.pst()[k] = ps;
+        }
+    }
+}
+
+// Pass 3 scan: each survivor's starting signs (the unit's seed, and the last
+// writer of the v3' sign before it); the survivors that passed the sign-free
+// tests are listed.  Returns their number.
+template <int MAXH, int WPE>
+__device__ __attribute__((noinline)) int yz_pass3_scan(Scr S, int np) {
+    const int lane = lane_id();
+    // pass 3: starting signs by scan (the unit's seed, and the last writer of
+    // the v3' sign before this candidate); the candidates that passed the
+    // sign-free tests in pass 2 are listed and run the rest (left/right, chi2)
+    int carry_f = -1, carry_w = -1, nq = 0;
+    for (int k0 = 0; k0 < np; k0 += WAVE) {
+        const int k = k0 + lane;
+        const bool valid = k < np;
+        uchar4 ps = make_uchar4(0, 0, 0, 0);
+        if (valid) ps = S.pst()[k];
+        const bool first = (ps.x & 1u) != 0;
+        int fi = wave_max_scan(valid && first ? k : -1);
+        fi = max(fi, carry_f);
+        const int wincl = wave_max_scan(valid && (first || ps.z) ? k : -1);
+        int wexcl = __shfl_up(wincl, 1, WAVE);
+        if (lane == 0) wexcl = -1;
+        wexcl = max(wexcl, carry_w);
+        carry_f = __shfl(fi, WAVE - 1, WAVE);
+        carry_w = max(carry_w, __shfl(wincl, WAVE - 1, WAVE));
+        const bool go = valid && (ps.x & 2u);
+        if (go && !first) {
+            // seed of the unit with the v3' sign (slot 3) of the last writer; the
+            // first survivor of the unit is a writer, so wexcl >= fi
+            ps.w = (uint8_t)((S.pst()[fi].y & 0x3Fu) | ((unsigned)S.pst()[wexcl].z << 6));
+            S.pst()[k] = ps;
+        }
+        const unsigned long long m = ballot(go);
+        if (go) S.qlist()[nq + rank_in(m)] = (uint32_t)k;
+        nq += __popcll(m);
+    }
+    return nq;
+}
+
+// Pass 3 over the listed survivors: left/right from the starting signs, chi2,
+// and the tracklet's best key (64-bit LDS atomicMin of chi2 bits and index).
+template <int MAXH, int WPE>
+__device__ __attribute__((noinline)) void yz_pass3(Scr S, int kb, int nq) {
+    Lds<MAXH>& L = g_lds<MAXH>;
+    YzWave& Y = yzw<MAXH>();
+    Ev<MAXH, WPE> E;
+    const int lane = lane_id();
+    Tk* tks = S.tks();
+    for (int q0 = 0; q0 < nq; q0 += WAVE) {
+        const int q = q0 + lane;
+        if (q < nq) {
+            const int k = (int)S.qlist()[q];
+            const YzItem it = yz_decode<MAXH, WPE>((int)S.plist()[k]);
+            const uchar4 ps = S.pst()[k];
+            const Tk* Tp = &tks[kb + it.j];
+            YzSlots H;
+            Fit f;
+            yz_cache_get(E, S.ycache() + 3 * k, Tp->st3, H, f);
+            if (!(ps.x & 1u)) yz_unpack3(H, ps.w);
+            lr_slots_new<8>(Tp->x0, Tp->tx, f.p0, f.p1, Tp->ex0, Tp->etx, f.e0, f.e1, H.pres, H.det, H.pos,
+                            H.drift, H.sign, 150.f);
+            float chi2;
+            yz_chi2(E, Tp, H, f, chi2, nullptr);
+            // the reference starts its minimum at 10000.1 (reconstruction_kernels.cuh:1240)
+            if (chi2 < 10000.1f) {
+                const unsigned long long key =
+                    ((unsigned long long)__float_as_uint(chi2) << 32) | (unsigned long long)(uint32_t)it.local;
+                atomicMin(&Y.best[it.j], key);
+            }
+        }
+    }
+    wsync();
+}
+
 // Parallel form over batches of first-box survivors (see yz_pass1_kept).
 // Returns false (nothing written) when one unit alone overflows the list; the
 // chunk then takes the unit-by-unit path.
@@ -1931,95 +2217,21 @@ __device__ __attribute__((noinline)) bool yz_chunk_par(Scr S, Prn R, int kb, int
         // full left/right from zero signs (its station-3 result seeds the unit),
         // later ones only the decision of a v3' / u2 pair (the one pair that can
         // change a station-3 sign once all are set).  A batch starts at a unit
-        // boundary, so its first survivor is the first of its unit.
-        for (int k0 = 0; k0 < np; k0 += WAVE) {
-            const int k = k0 + lane;
-            if (k < np) {
-                const YzItem it = yz_decode<MAXH, WPE>((int)S.plist()[k]);
-                bool first = k == 0;
-                if (!first) {
-                    const YzItem pt = yz_decode<MAXH, WPE>((int)S.plist()[k - 1]);
-                    first = pt.j != it.j || pt.iu3 != it.iu3 || pt.iv3 != it.iv3;
-                }
-                const Tk* Tp = &tks[kb + it.j];
-                YzSlots H;
-                Fit f;
-                yz_candidate(E, S, Tp, it, H, f);
-                yz_cache_put(S.ycache() + 3 * k, H, f);
-                // the tests that need no sign (second box cut, prop-tube Y,
-                // hodoscopes) depend on the first fit and the tracklet only
-                const bool go = yz_cuts(E, Tp, f);
-                uchar4 ps = make_uchar4((first ? 1 : 0) | (go ? 2 : 0), 0, 0, 0);
-                if (first) {
-                    lr_slots_new<8>(Tp->x0, Tp->tx, f.p0, f.p1, Tp->ex0, Tp->etx, f.e0, f.e1, H.pres, H.det, H.pos,
-                                    H.drift, H.sign, 150.f);
-                    ps.y = yz_pack3(H);
-                    ps.z = (uint8_t)(H.sign[3] & 3);
-                } else if ((lr_slot_starts<8>(H.pres, H.det) >> 3) & 1u) {
-                    const int h = lr_pair_decide(Tp->x0, Tp->tx, f.p0, f.p1, Tp->ex0, Tp->etx, f.e0, f.e1, H.det[3],
-                                                 H.det[4], H.pos[3], H.pos[4], H.drift[3], H.drift[4], 150.f);
-                    if (h >= 0) ps.z = (uint8_t)(lr_si(h) & 3);
-                }
-                S.pst()[k] = ps;
-            }
-        }
+        // boundary, so its first survivor is the first of its unit.  The first
+        // survivors run their left/right in a pass of their own, so that the
+        // other lanes of a wave do not wait through it.
+        const int nf = yz_pass2<MAXH, WPE>(S, kb, np);
+        wsync();
+        yz_pass2_firsts<MAXH, WPE>(S, kb, nf);
         wsync();
         OR_SUB(2);
-        // pass 3: starting signs by scan (the unit's seed, and the last writer of
-        // the v3' sign before this candidate); the candidates that passed the
-        // sign-free tests in pass 2 are listed and run the rest (left/right, chi2)
-        int carry_f = -1, carry_w = -1, nq = 0;
-        for (int k0 = 0; k0 < np; k0 += WAVE) {
-            const int k = k0 + lane;
-            const bool valid = k < np;
-            uchar4 ps = make_uchar4(0, 0, 0, 0);
-            if (valid) ps = S.pst()[k];
-            const bool first = (ps.x & 1u) != 0;
-            int fi = wave_max_scan(valid && first ? k : -1);
-            fi = max(fi, carry_f);
-            const int wincl = wave_max_scan(valid && (first || ps.z) ? k : -1);
-            int wexcl = __shfl_up(wincl, 1, WAVE);
-            if (lane == 0) wexcl = -1;
-            wexcl = max(wexcl, carry_w);
-            carry_f = __shfl(fi, WAVE - 1, WAVE);
-            carry_w = max(carry_w, __shfl(wincl, WAVE - 1, WAVE));
-            const bool go = valid && (ps.x & 2u);
-            if (go && !first) {
-                // seed of the unit with the v3' sign (slot 3) of the last writer; the
-                // first survivor of the unit is a writer, so wexcl >= fi
-                ps.w = (uint8_t)((S.pst()[fi].y & 0x3Fu) | ((unsigned)S.pst()[wexcl].z << 6));
-                S.pst()[k] = ps;
-            }
-            const unsigned long long m = ballot(go);
-            if (go) S.qlist()[nq + rank_in(m)] = (uint32_t)k;
-            nq += __popcll(m);
-        }
+        // pass 3: starting signs by scan; the candidates that passed the
+        // sign-free tests in pass 2 run the rest (left/right, chi2)
+        const int nq = yz_pass3_scan<MAXH, WPE>(S, np);
         wsync();
         OR_SUB(15);
         OR_COUNT(16, nq);
-        for (int q0 = 0; q0 < nq; q0 += WAVE) {
-            const int q = q0 + lane;
-            if (q < nq) {
-                const int k = (int)S.qlist()[q];
-                const YzItem it = yz_decode<MAXH, WPE>((int)S.plist()[k]);
-                const uchar4 ps = S.pst()[k];
-                const Tk* Tp = &tks[kb + it.j];
-                YzSlots H;
-                Fit f;
-                yz_cache_get(E, S.ycache() + 3 * k, Tp->st3, H, f);
-                if (!(ps.x & 1u)) yz_unpack3(H, ps.w);
-                lr_slots_new<8>(Tp->x0, Tp->tx, f.p0, f.p1, Tp->ex0, Tp->etx, f.e0, f.e1, H.pres, H.det, H.pos,
-                                H.drift, H.sign, 150.f);
-                float chi2;
-                yz_chi2(E, Tp, H, f, chi2, nullptr);
-                // the reference starts its minimum at 10000.1 (reconstruction_kernels.cuh:1240)
-                if (chi2 < 10000.1f) {
-                    const unsigned long long key =
-                        ((unsigned long long)__float_as_uint(chi2) << 32) | (unsigned long long)(uint32_t)it.local;
-                    atomicMin(&Y.best[it.j], key);
-                }
-            }
-        }
+        yz_pass3<MAXH, WPE>(S, kb, nq);
         wsync();
         // a tracklet whose best improved in this batch keeps that candidate's
         // cache entry and sign state, since the next batch reuses the lists
@@ -2206,6 +2418,18 @@ STAGE void stage_yz(or_params P, Scr S, Prn R, int ntkl, int list_cap) {
                 }
                 Y.nl[j][v] = nacc;
             }
+            // hodoscope x halves of the tracklet's matches: lane (j, v < 3) takes
+            // station v + 1's two planes
+            unsigned hb = 0;
+            if (k < ntkl && v < 3) {
+                const Tk& T = tks[k];
+                hb = (hodo_x(E, v + 1, yz_hodo(2 * v), T.x0, T.tx, T.ex0, T.etx) ? 1u : 0u) |
+                     (hodo_x(E, v + 1, yz_hodo(2 * v + 1), T.x0, T.tx, T.ex0, T.etx) ? 2u : 0u);
+                hb <<= 2 * v;
+            }
+            hb |= (unsigned)__shfl_xor((int)hb, 1, WAVE);
+            hb |= (unsigned)__shfl_xor((int)hb, 2, WAVE);
+            if (v == 0) Y.hx[j] = hb;
             // enumeration bound: u2 x v2 x u3 x v3 list entries of one tracklet
             raw *= (unsigned long long)__shfl_xor((long long)raw, 1, WAVE);
             raw *= (unsigned long long)__shfl_xor((long long)raw, 2, WAVE);
@@ -3913,6 +4137,7 @@ __global__ __launch_bounds__(WAVE * WPE, (chain_waves_per_simd<MAXH, WPE>())) vo
             if (L.status == 0) stage_xpairs<MAXH, WPE>(a.p, S.xpairs());
             OR_TIMER(1)
             if (L.status == 0) nt = stage_xz<MAXH, WPE>(a.p, S);
+            if (L.status == 0 && nt > 0) stage_prop_sort<MAXH, WPE>();
             OR_TIMER(2)
             if (lane == 0) {
                 L.ntkl = nt;
