@@ -248,7 +248,8 @@ enum ScrRegion {
     R_TK,     // Tk[maxtkl]: the event's tracklets
     R_CELL,   // uint32[1569]: XZ cell offsets
     R_LIST,   // uint16[64][win_cap]: window pair lists (YZ: tracklet x view; station 1: lane)
-    R_YV,     // float4[64][win_cap]: y values of YZ list entries; station-1 records
+    R_YV,     // float4[64][win_cap]: fit terms (w y, w y z) of YZ list entries' hits a, b; station-1 records
+    R_YY,     // float2[64][win_cap]: y of YZ list entries' hits a, b
     R_YF,     // uint8[64][win_cap]: accepted-hit flags of YZ list entries
     R_XP,     // uint16[3][28][bin_cap]: binned X pairs
     R_SO,     // uint16[maxtkl]: tracklet index by slot order
@@ -763,6 +764,10 @@ __device__ __forceinline__ float4 gld4(const float4* p) {
     const __attribute__((address_space(1))) float* f = (const __attribute__((address_space(1))) float*)p;
     return make_float4(f[0], f[1], f[2], f[3]);
 }
+__device__ __forceinline__ float2 gld2(const float2* p) {
+    const __attribute__((address_space(1))) float* f = (const __attribute__((address_space(1))) float*)p;
+    return make_float2(f[0], f[1]);
+}
 __device__ __forceinline__ uint4 gld4(const uint4* p) {
     const __attribute__((address_space(1))) uint32_t* f = (const __attribute__((address_space(1))) uint32_t*)p;
     return make_uint4(f[0], f[1], f[2], f[3]);
@@ -770,23 +775,30 @@ __device__ __forceinline__ uint4 gld4(const uint4* p) {
 
 struct YzLists {
     const uint16_t* l[4];   // pair entries (a | b << 8), views u2, v2, u3, v3
-    const float4* y[4];     // (y_a, ey_a, y_b, ey_b)
+    const float4* y[4];     // (w y, w y z) of hit a, then of hit b
     const uint8_t* f[4];    // bit 0: hit a present and accepted, bit 1: hit b
     int n[4];
 };
 
-// fit sums of one list entry's accepted hits, in order a then b
-__device__ __forceinline__ int yz_sum(const YzLists& W, int v, int k, float za, float zb, float& A0, float& A1,
-                                      float& A3, float& B0, float& B1) {
+// Weight of a UV hit of plane d in the YZ fits: the y error calculate_y_uvhit
+// gives it (reconstruction_helper.cuh), resolution x |dy/dx| of the wire, a
+// constant of the plane.
+__device__ __forceinline__ float yz_w(int d) { return G(F_RESOLUTION, d) * fabsf(G(F_DELTAPY, d) / G(F_DELTAPX, d)); }
+
+// fit sums of one list entry's accepted hits (planes da, db), in order a then b
+__device__ __forceinline__ int yz_sum(const YzLists& W, int v, int k, int da, int db, float& A0, float& A1, float& A3,
+                                      float& B0, float& B1) {
     const uint8_t fl = gld(W.f[v] + k);
     if (!fl) return 0;
     const float4 yv = gld4(W.y[v] + k);
     int c = 0;
     if (fl & 1) {
-        B0 += yv.y * yv.x; B1 += (yv.y * yv.x) * za; A0 += yv.y; A1 += yv.y * za; A3 += (yv.y * za) * za; ++c;
+        const float w = yz_w(da), z = G(F_Z, da);
+        B0 += yv.x; B1 += yv.y; A0 += w; A1 += w * z; A3 += (w * z) * z; ++c;
     }
     if (fl & 2) {
-        B0 += yv.w * yv.z; B1 += (yv.w * yv.z) * zb; A0 += yv.w; A1 += yv.w * zb; A3 += (yv.w * zb) * zb; ++c;
+        const float w = yz_w(db), z = G(F_Z, db);
+        B0 += yv.z; B1 += yv.w; A0 += w; A1 += w * z; A3 += (w * z) * z; ++c;
     }
     return c;
 }
@@ -945,12 +957,11 @@ __device__ __attribute__((noinline)) bool yz_unit(const Ev<MAXH, WPE>& E, Tk* Tp
     const int st3 = Tp->st3;
     const int su2 = super_id(2, 1) * 2, sv2 = super_id(2, 2) * 2;
     const int su3 = super_id(st3, 1) * 2, sv3 = super_id(st3, 2) * 2;
-    const float zu2 = G(F_Z, su2), zu2p = G(F_Z, su2 - 1), zv2 = G(F_Z, sv2), zv2p = G(F_Z, sv2 - 1);
     const int nu2 = W.n[0], nv2 = W.n[1], nu3 = W.n[2];
     const int iu3 = iuv3 % nu3, iv3 = iuv3 / nu3;
     float sA0 = 0.f, sA1 = 0.f, sA3 = 0.f, sB0 = 0.f, sB1 = 0.f;
-    if (yz_sum(W, 2, iu3, G(F_Z, su3), G(F_Z, su3 - 1), sA0, sA1, sA3, sB0, sB1) == 0) return false;
-    if (yz_sum(W, 3, iv3, G(F_Z, sv3), G(F_Z, sv3 - 1), sA0, sA1, sA3, sB0, sB1) == 0) return false;
+    if (yz_sum(W, 2, iu3, su3, su3 - 1, sA0, sA1, sA3, sB0, sB1) == 0) return false;
+    if (yz_sum(W, 3, iv3, sv3, sv3 - 1, sA0, sA1, sA3, sB0, sB1) == 0) return false;
     YzSlots H;
     yz_dets(H, st3);
     H.pres = 0;
@@ -961,8 +972,8 @@ __device__ __attribute__((noinline)) bool yz_unit(const Ev<MAXH, WPE>& E, Tk* Tp
     for (int iuv2 = 0; iuv2 <= last; ++iuv2) {
         const int iu2 = iuv2 % nu2, iv2 = iuv2 / nu2;
         float A0 = sA0, A1 = sA1, A3 = sA3, B0 = sB0, B1 = sB1;
-        if (yz_sum(W, 0, iu2, zu2, zu2p, A0, A1, A3, B0, B1) == 0) continue;
-        if (yz_sum(W, 1, iv2, zv2, zv2p, A0, A1, A3, B0, B1) == 0) continue;
+        if (yz_sum(W, 0, iu2, su2, su2 - 1, A0, A1, A3, B0, B1) == 0) continue;
+        if (yz_sum(W, 1, iv2, sv2, sv2 - 1, A0, A1, A3, B0, B1) == 0) continue;
         const Fit f = fit_finish(A0, A1, A3, B0, B1);
         if (fabsf(f.p0) > Y0_MAX + 2.f * f.e0 || fabsf(f.p1) > TY_MAX + 2.f * f.e1) continue;
         yz_fill<4>(E, W, 0, iu2, H);
@@ -987,7 +998,10 @@ struct Scr {
     __device__ Tk* tks() const { return reinterpret_cast<Tk*>(cfg().rp[R_TK]); }
     __device__ uint32_t* celloff() const { return reinterpret_cast<uint32_t*>(cfg().rp[R_CELL]); }
     __device__ uint16_t* wl() const { return reinterpret_cast<uint16_t*>(cfg().rp[R_LIST]); }
+    // per YZ list entry: (w y, w y z) of hit a, then of hit b, w the plane's
+    // weight and z its position; the y values themselves in wyy
     __device__ float4* wy() const { return reinterpret_cast<float4*>(cfg().rp[R_YV]); }
+    __device__ float2* wyy() const { return reinterpret_cast<float2*>(cfg().rp[R_YY]); }
     __device__ uint8_t* wf() const { return reinterpret_cast<uint8_t*>(cfg().rp[R_YF]); }
     // [3][28][bin_cap] X pairs (a | b << 8), 0xFF = none
     __device__ uint16_t* xpairs() const { return reinterpret_cast<uint16_t*>(cfg().rp[R_XP]); }
@@ -1468,15 +1482,80 @@ __device__ __forceinline__ YzItem yz_decode(int i) {
 
 // first fit of a candidate over u3, u3', v3, v3', u2, u2', v2, v2' (the
 // reference's float order); false when a view has no accepted hit
+// YZ first-fit table.  A hit's weight and z are constants of its plane, so the
+// normal matrix (A0, A1, A3) of a YZ candidate, its inverse and the parameter
+// errors depend only on which hits of its four list entries are present (per
+// entry: a, b or both) and on station 3 being D3p or D3m: 2 x 81 patterns.
+// Entry (m, pattern): Ai[0..3], then e0, e1, computed by the fit's own
+// arithmetic (sums in the order u3, v3, u2, v2, hit a before b; matinv2;
+// square roots) once per device when the geometry is set; a candidate then
+// needs only its B sums.  pattern = 27 (fu3 - 1) + 9 (fv3 - 1) + 3 (fu2 - 1) + fv2 - 1.
+__device__ float4 g_yz_atab[2 * 81 * 2];
+
+__device__ __forceinline__ int yz_pattern(unsigned fu2, unsigned fv2, unsigned fu3, unsigned fv3) {
+    return (int)(27 * (fu3 - 1) + 9 * (fv3 - 1) + 3 * (fu2 - 1) + (fv2 - 1));
+}
+
+__global__ __launch_bounds__(256) void yz_atab_kernel() {
+    const int t = threadIdx.x;
+    if (t >= 2 * 81) return;
+    const int m = t / 81, pt = t % 81;
+    const unsigned fl[4] = {(unsigned)(pt / 3 % 3 + 1), (unsigned)(pt % 3 + 1), (unsigned)(pt / 27 + 1),
+                            (unsigned)(pt / 9 % 3 + 1)};   // u2, v2, u3, v3
+    float A0 = 0.f, A1 = 0.f, A3 = 0.f;
+#pragma unroll
+    for (int o = 0; o < 4; ++o) {
+        const int v = o < 2 ? o + 2 : o - 2;
+        const int sid = v < 2 ? super_id(2, v + 1) : super_id(3 + m, v - 1);
+        const int da = sid * 2, db = sid * 2 - 1;
+        if (fl[v] & 1) {
+            const float w = yz_w(da), z = G(F_Z, da);
+            A0 += w; A1 += w * z; A3 += (w * z) * z;
+        }
+        if (fl[v] & 2) {
+            const float w = yz_w(db), z = G(F_Z, db);
+            A0 += w; A1 += w * z; A3 += (w * z) * z;
+        }
+    }
+    float Ai[4];
+    matinv2(A0, A1, A1, A3, Ai);
+    g_yz_atab[2 * t] = make_float4(Ai[0], Ai[1], Ai[2], Ai[3]);
+    g_yz_atab[2 * t + 1] = make_float4(sqrtf(fabsf(Ai[0])), sqrtf(fabsf(Ai[3])), 0.f, 0.f);
+}
+
+// fit_finish's parameters from the table entry and the B sums
+__device__ __forceinline__ Fit yz_fit_tab(int m, int pattern, float B0, float B1) {
+    const float4 ai = g_yz_atab[2 * (m * 81 + pattern)], ee = g_yz_atab[2 * (m * 81 + pattern) + 1];
+    Fit f;
+    f.e0 = ee.x;
+    f.e1 = ee.y;
+    float q = 0.0f;
+    q += ai.x * B0;
+    q += ai.y * B1;
+    f.p0 = q;
+    q = 0.0f;
+    q += ai.z * B0;
+    q += ai.w * B1;
+    f.p1 = q;
+    return f;
+}
+
+// B sums of an entry's present hits, a then b
+__device__ __forceinline__ void yz_bsum(unsigned fl, const float4& t, float& B0, float& B1) {
+    if (fl & 1) { B0 += t.x; B1 += t.y; }
+    if (fl & 2) { B0 += t.z; B1 += t.w; }
+}
+
 __device__ __forceinline__ bool yz_fit(const YzLists& W, int st3, const YzItem& it, Fit& f) {
-    const int su2 = super_id(2, 1) * 2, sv2 = super_id(2, 2) * 2;
-    const int su3 = super_id(st3, 1) * 2, sv3 = super_id(st3, 2) * 2;
-    float A0 = 0.f, A1 = 0.f, A3 = 0.f, B0 = 0.f, B1 = 0.f;
-    if (!yz_sum(W, 2, it.iu3, G(F_Z, su3), G(F_Z, su3 - 1), A0, A1, A3, B0, B1)) return false;
-    if (!yz_sum(W, 3, it.iv3, G(F_Z, sv3), G(F_Z, sv3 - 1), A0, A1, A3, B0, B1)) return false;
-    if (!yz_sum(W, 0, it.iu2, G(F_Z, su2), G(F_Z, su2 - 1), A0, A1, A3, B0, B1)) return false;
-    if (!yz_sum(W, 1, it.iv2, G(F_Z, sv2), G(F_Z, sv2 - 1), A0, A1, A3, B0, B1)) return false;
-    f = fit_finish(A0, A1, A3, B0, B1);
+    const unsigned fu3 = gld(W.f[2] + it.iu3), fv3 = gld(W.f[3] + it.iv3);
+    const unsigned fu2 = gld(W.f[0] + it.iu2), fv2 = gld(W.f[1] + it.iv2);
+    if (!fu3 || !fv3 || !fu2 || !fv2) return false;
+    float B0 = 0.f, B1 = 0.f;
+    yz_bsum(fu3, gld4(W.y[2] + it.iu3), B0, B1);
+    yz_bsum(fv3, gld4(W.y[3] + it.iv3), B0, B1);
+    yz_bsum(fu2, gld4(W.y[0] + it.iu2), B0, B1);
+    yz_bsum(fv2, gld4(W.y[1] + it.iv2), B0, B1);
+    f = yz_fit_tab(st3 == 4 ? 1 : 0, yz_pattern(fu2, fv2, fu3, fv3), B0, B1);
     return true;
 }
 
@@ -1545,10 +1624,10 @@ __device__ __forceinline__ UnitFeatures unit_features(const Scr& S, const Tk& T,
     const int win = S.win();
     const int iv3 = qdiv(iuv3, yzw<MAXH>().rz[j]), iu3 = iuv3 - iv3 * nl.z;
     const int qu = (j * 4 + 2) * win + iu3, qv = (j * 4 + 3) * win + iv3;
-    const float4 yu = gld4(S.wy() + qu), yw = gld4(S.wy() + qv);
+    const float2 yu = gld2(S.wyy() + qu), yw = gld2(S.wyy() + qv);
     const unsigned fu = gld(S.wf() + qu), fv = gld(S.wf() + qv);
-    const float ya = (fu & 1u) ? yu.x : yu.z, yb = (fu & 2u) ? yu.z : yu.x;
-    const float yc = (fv & 1u) ? yw.x : yw.z, yd = (fv & 2u) ? yw.z : yw.x;
+    const float ya = (fu & 1u) ? yu.x : yu.y, yb = (fu & 2u) ? yu.y : yu.x;
+    const float yc = (fv & 1u) ? yw.x : yw.y, yd = (fv & 2u) ? yw.y : yw.x;
     const float yum = 0.5f * (ya + yb), yvm = 0.5f * (yc + yd);
     const float y3 = 0.5f * (yum + yvm);
     const int st3 = T.st3;
@@ -1564,18 +1643,18 @@ __device__ __forceinline__ UnitFeatures unit_features(const Scr& S, const Tk& T,
         const int n = v == 0 ? nl.x : nl.y;
         const int q0 = (j * 4 + v) * win;
         for (int k0 = 0; k0 < n; k0 += 4) {
-            float4 e[4];
+            float2 e[4];
             unsigned fl[4];
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
                 const int q = q0 + min(k0 + t, n - 1);
-                e[t] = gld4(S.wy() + q);
+                e[t] = gld2(S.wyy() + q);
                 fl[t] = gld(S.wf() + q);
             }
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
                 if (k0 + t >= n) break;
-                const float y = (fl[t] & 1u) ? ((fl[t] & 2u) ? 0.5f * (e[t].x + e[t].z) : e[t].x) : e[t].z;
+                const float y = (fl[t] & 1u) ? ((fl[t] & 2u) ? 0.5f * (e[t].x + e[t].y) : e[t].x) : e[t].y;
                 const float d = fabsf(y - yp);
                 if (d < dmin) { dmin = d; ybest = y; kb = k0 + t; }
             }
@@ -1635,48 +1714,25 @@ __device__ __forceinline__ UnitFeatures unit_features(const Scr& S, const Tk& T,
     return f;
 }
 
-// plane z of the YZ views: [u2, v2, u3, v3][station 3 is D3m][hit a, b]
-__device__ __forceinline__ void yz_plane_z(float (&zv)[4][2][2]) {
-#pragma unroll
-    for (int v = 0; v < 4; ++v)
-#pragma unroll
-        for (int m = 0; m < 2; ++m) {
-            const int sid = v < 2 ? super_id(2, v + 1) : super_id(3 + m, v - 1);
-            zv[v][m][0] = G(F_Z, sid * 2);
-            zv[v][m][1] = G(F_Z, sid * 2 - 1);
-        }
-}
-
 // first fit and box cut of YZ candidate (tracklet j of the chunk, list entries
-// e = u2, v2, u3, v3; m: station 3 is D3m), hits in the reference order u3,
-// v3, u2, v2, a then b
-__device__ __forceinline__ bool yz_first_cut(const Scr& S, int j, const int (&e)[4], unsigned m,
-                                             const float (&zv)[4][2][2]) {
-    uint8_t fl[4];
-    float4 yv[4];
+// e = u2, v2, u3, v3; m: station 3 is D3m): B sums in the reference order u3,
+// v3, u2, v2, a then b, the rest from the first-fit table
+__device__ __forceinline__ bool yz_first_cut(const Scr& S, int j, const int (&e)[4], unsigned m) {
+    unsigned fl[4];
+    float4 t[4];
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         const int q = (j * 4 + v) * cfg().win_cap + e[v];
         fl[v] = gld(S.wf() + q);
-        yv[v] = gld4(S.wy() + q);
+        t[v] = gld4(S.wy() + q);
     }
-    float A0 = 0.f, A1 = 0.f, A3 = 0.f, B0 = 0.f, B1 = 0.f;
-    bool ok = true;
-#pragma unroll
-    for (int o = 0; o < 4; ++o) {
-        const int v = o < 2 ? o + 2 : o - 2;
-        const float za = m ? zv[v][1][0] : zv[v][0][0], zb = m ? zv[v][1][1] : zv[v][0][1];
-        ok = ok && fl[v] != 0;
-        if (fl[v] & 1) {
-            B0 += yv[v].y * yv[v].x; B1 += (yv[v].y * yv[v].x) * za; A0 += yv[v].y; A1 += yv[v].y * za;
-            A3 += (yv[v].y * za) * za;
-        }
-        if (fl[v] & 2) {
-            B0 += yv[v].w * yv[v].z; B1 += (yv[v].w * yv[v].z) * zb; A0 += yv[v].w; A1 += yv[v].w * zb;
-            A3 += (yv[v].w * zb) * zb;
-        }
-    }
-    return ok && yz_box1(fit_finish(A0, A1, A3, B0, B1));
+    if (!fl[0] || !fl[1] || !fl[2] || !fl[3]) return false;
+    float B0 = 0.f, B1 = 0.f;
+    yz_bsum(fl[2], t[2], B0, B1);
+    yz_bsum(fl[3], t[3], B0, B1);
+    yz_bsum(fl[0], t[0], B0, B1);
+    yz_bsum(fl[1], t[1], B0, B1);
+    return yz_box1(yz_fit_tab((int)m, yz_pattern(fl[0], fl[1], fl[2], fl[3]), B0, B1));
 }
 
 // Pass 1 streams in batches.  The survivors of the first box cut go into a
@@ -1839,8 +1895,6 @@ __device__ __attribute__((noinline)) int yz_pass1_kept(Scr S, int kb, int ntkl, 
 #pragma unroll
     for (int t = 0; t <= YZ_CHUNK; ++t) off[t] = __builtin_amdgcn_readfirstlane(Y.uoff[t]);
     const unsigned d3m = (unsigned)ballot(lane < YZ_CHUNK && kb + lane < ntkl && tks[kb + lane].st3 == 4);
-    float zv[4][2][2];
-    yz_plane_z(zv);
     const uint32_t start = cursor;
     int np = 0;
     int lo0 = lo_unit;   // last kept unit starting at or before i0 (the same for every lane)
@@ -1883,7 +1937,7 @@ __device__ __attribute__((noinline)) int yz_pass1_kept(Scr S, int kb, int ntkl, 
             const int e[4] = {inner - v2 * nl.x, v2, iuv3 - v3 * nl.z, v3};
             ustart = ojs + iuv3 * (nl.x * nl.y);
             iorig = ustart + inner;
-            pass = yz_first_cut(S, j, e, (d3m >> j) & 1, zv);
+            pass = yz_first_cut(S, j, e, (d3m >> j) & 1);
         }
         const unsigned long long m = ballot(pass);
         const int c = __popcll(m);
@@ -1926,8 +1980,6 @@ __device__ __attribute__((noinline)) int yz_pass1_plain(Scr S, int kb, int ntkl,
 #pragma unroll
     for (int t = 0; t <= YZ_CHUNK; ++t) off[t] = __builtin_amdgcn_readfirstlane(Y.uoff[t]);
     const unsigned d3m = (unsigned)ballot(lane < YZ_CHUNK && kb + lane < ntkl && tks[kb + lane].st3 == 4);
-    float zv[4][2][2];
-    yz_plane_z(zv);
     const int start = cursor;
     int np = 0;
     for (int i0 = start; i0 < I; i0 += WAVE) {
@@ -1947,7 +1999,7 @@ __device__ __attribute__((noinline)) int yz_pass1_plain(Scr S, int kb, int ntkl,
             ustart = ojs + iuv3 * nin;
             const int v2 = qdiv(iuv2, Y.rx[j]), v3 = qdiv(iuv3, Y.rz[j]);
             const int e[4] = {iuv2 - v2 * nl.x, v2, iuv3 - v3 * nl.z, v3};  // u2, v2, u3, v3
-            pass = yz_first_cut(S, j, e, (d3m >> j) & 1, zv);
+            pass = yz_first_cut(S, j, e, (d3m >> j) & 1);
         }
         const unsigned long long m = ballot(pass);
         const int c = __popcll(m);
@@ -2364,6 +2416,7 @@ STAGE void stage_yz(or_params P, Scr S, Prn R, int ntkl, int list_cap) {
     Tk* tks = S.tks();
     uint16_t* wl = S.wl();
     float4* wy = S.wy();
+    float2* wyy = S.wyy();
     uint8_t* wf = S.wf();
     const int capw = cfg().win_cap;
     // chunks of YZ_CHUNK tracklets, handed to the event's waves as they come free
@@ -2410,9 +2463,12 @@ STAGE void stage_yz(or_params P, Scr S, Prn R, int ntkl, int list_cap) {
                         if (y_uvhit(db, L.elem[r], L.drift[r], 0, T.x0, T.tx, yv.z, yv.w)) fl |= 2;
                     }
                     if (fl) {
+                        const int o = (j * 4 + v) * cfg().win_cap + nacc;
                         lst[nacc] = lst[q];
-                        wy[(j * 4 + v) * cfg().win_cap + nacc] = yv;
-                        wf[(j * 4 + v) * cfg().win_cap + nacc] = fl;
+                        // fit terms w y and (w y) z of each hit, as the fit sums them
+                        wy[o] = make_float4(yv.y * yv.x, (yv.y * yv.x) * za, yv.w * yv.z, (yv.w * yv.z) * zb);
+                        wyy[o] = make_float2(yv.x, yv.z);
+                        wf[o] = fl;
                         ++nacc;
                     }
                 }
@@ -4298,6 +4354,7 @@ static ScrLayout make_layout(const or_params& p, int list_cap) {
     put(R_CELL, 1569 * 4);
     put(R_LIST, 64 * win * 2);
     put(R_YV, std::max<int64_t>(64 * win * 16, 8 * 16 + win * (int64_t)sizeof(GX)));   // station 1: best + x records
+    put(R_YY, 64 * win * 8);
     put(R_YF, 64 * win);
     put(R_XP, 3 * 28 * bin * 2);
     put(R_SO, maxtkl * 2);
@@ -4470,6 +4527,10 @@ or_status or_session_create(const or_geometry* g, const or_params* p, int device
         }
         OR_HIP(hipMemcpyToSymbol(HIP_SYMBOL(ordev::c_fitp), fp, sizeof fp));
     }
+    // the YZ first-fit table of this geometry (null stream, before any session's work)
+    yz_atab_kernel<<<1, 256>>>();
+    OR_HIP(hipGetLastError());
+    OR_HIP(hipDeviceSynchronize());
     std::memcpy(dg.f, g->f, sizeof g->f);
     dg.set = true;
     }
