@@ -1717,22 +1717,30 @@ __device__ __forceinline__ UnitFeatures unit_features(const Scr& S, const Tk& T,
 // first fit and box cut of YZ candidate (tracklet j of the chunk, list entries
 // e = u2, v2, u3, v3; m: station 3 is D3m): B sums in the reference order u3,
 // v3, u2, v2, a then b, the rest from the first-fit table
-__device__ __forceinline__ bool yz_first_cut(const Scr& S, int j, const int (&e)[4], unsigned m) {
+// Pass 1 splits it in two: the loads of a candidate (YzFirst) are issued one
+// wave iteration ahead of its evaluation.
+struct YzFirst {
     unsigned fl[4];
     float4 t[4];
+    unsigned m;
+    int ustart, index;   // first candidate of its unit, its own index (-1: no candidate)
+};
+__device__ __forceinline__ void yz_first_load(const Scr& S, int j, const int (&e)[4], YzFirst& c) {
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         const int q = (j * 4 + v) * cfg().win_cap + e[v];
-        fl[v] = gld(S.wf() + q);
-        t[v] = gld4(S.wy() + q);
+        c.fl[v] = gld(S.wf() + q);
+        c.t[v] = gld4(S.wy() + q);
     }
-    if (!fl[0] || !fl[1] || !fl[2] || !fl[3]) return false;
+}
+__device__ __forceinline__ bool yz_first_eval(const YzFirst& c) {
+    if (c.index < 0 || !c.fl[0] || !c.fl[1] || !c.fl[2] || !c.fl[3]) return false;
     float B0 = 0.f, B1 = 0.f;
-    yz_bsum(fl[2], t[2], B0, B1);
-    yz_bsum(fl[3], t[3], B0, B1);
-    yz_bsum(fl[0], t[0], B0, B1);
-    yz_bsum(fl[1], t[1], B0, B1);
-    return yz_box1(yz_fit_tab((int)m, yz_pattern(fl[0], fl[1], fl[2], fl[3]), B0, B1));
+    yz_bsum(c.fl[2], c.t[2], B0, B1);
+    yz_bsum(c.fl[3], c.t[3], B0, B1);
+    yz_bsum(c.fl[0], c.t[0], B0, B1);
+    yz_bsum(c.fl[1], c.t[1], B0, B1);
+    return yz_box1(yz_fit_tab((int)c.m, yz_pattern(c.fl[0], c.fl[1], c.fl[2], c.fl[3]), B0, B1));
 }
 
 // Pass 1 streams in batches.  The survivors of the first box cut go into a
@@ -1891,14 +1899,13 @@ __device__ __attribute__((noinline)) int yz_pass1_kept(Scr S, int kb, int ntkl, 
     YzWave& Y = yzw<MAXH>();
     const int lane = lane_id();
     Tk* tks = S.tks();
-    int off[YZ_CHUNK + 1];
-#pragma unroll
-    for (int t = 0; t <= YZ_CHUNK; ++t) off[t] = __builtin_amdgcn_readfirstlane(Y.uoff[t]);
     const unsigned d3m = (unsigned)ballot(lane < YZ_CHUNK && kb + lane < ntkl && tks[kb + lane].st3 == 4);
     const uint32_t start = cursor;
     int np = 0;
     int lo0 = lo_unit;   // last kept unit starting at or before i0 (the same for every lane)
-    for (uint32_t i0 = start; i0 < run; i0 += WAVE) {
+    // candidate i0 + lane of the kept enumeration: its kept unit (ku), that
+    // unit's start in the kept enumeration (ks), and its loads; advances lo0
+    auto load = [&](uint32_t i0, YzFirst& c, uint32_t& ks, int& ku) {
         const uint32_t i = i0 + (uint32_t)lane;
         // every kept unit has at least one candidate, so the wave's 64 candidates
         // lie in units lo0 .. lo0 + 63: lane t holds the start and code of unit
@@ -1917,28 +1924,40 @@ __device__ __attribute__((noinline)) int yz_pass1_kept(Scr S, int kb, int ntkl, 
         }
         const uint32_t st_t = __shfl(st, tu, WAVE);
         const uint32_t code = __shfl(kl, tu, WAVE);
-        const int ku = lo0 + tu;   // this lane's kept unit
+        ks = st_t;
+        ku = lo0 + tu;
         // the next window starts at the last unit starting at or before i0 + 64;
         // a unit past this window can start there only when the window's last
         // unit does (then unit lo0 started at i0)
         lo0 += tn;
         if (tn == WAVE - 1)
             while (lo0 + 1 < nk && S.kstart()[lo0 + 1] <= i0 + WAVE) ++lo0;
-        bool pass = false;
-        int iorig = 0, ustart = 0;
+        c.index = -1;
+        c.ustart = 0;
         if (i < run) {
             const int j = (int)(code >> 16), iuv3 = (int)(code & 0xFFFFu);
             const int inner = (int)(i - st_t);
             const int4 nl = *reinterpret_cast<const int4*>(Y.nl[j]);
-            int ojs = off[0];
-#pragma unroll
-            for (int t = 1; t < YZ_CHUNK; ++t) ojs = j == t ? off[t] : ojs;
+            const int ojs = Y.uoff[j];
             const int v2 = qdiv(inner, Y.rx[j]), v3 = qdiv(iuv3, Y.rz[j]);
             const int e[4] = {inner - v2 * nl.x, v2, iuv3 - v3 * nl.z, v3};
-            ustart = ojs + iuv3 * (nl.x * nl.y);
-            iorig = ustart + inner;
-            pass = yz_first_cut(S, j, e, (d3m >> j) & 1);
+            c.ustart = ojs + iuv3 * (nl.x * nl.y);
+            c.index = c.ustart + inner;
+            c.m = (d3m >> j) & 1;
+            yz_first_load(S, j, e, c);
         }
+    };
+    YzFirst nxt;
+    uint32_t nks = 0;
+    int nku = 0;
+    load(start, nxt, nks, nku);
+    for (uint32_t i0 = start; i0 < run; i0 += WAVE) {
+        const YzFirst cur = nxt;
+        const uint32_t st_t = nks;
+        const int ku = nku;
+        if (i0 + WAVE < run) load(i0 + WAVE, nxt, nks, nku);
+        const bool pass = yz_first_eval(cur);
+        const int iorig = cur.index, ustart = cur.ustart;
         const unsigned long long m = ballot(pass);
         const int c = __popcll(m);
         if (np + c > cap) {
@@ -1982,25 +2001,34 @@ __device__ __attribute__((noinline)) int yz_pass1_plain(Scr S, int kb, int ntkl,
     const unsigned d3m = (unsigned)ballot(lane < YZ_CHUNK && kb + lane < ntkl && tks[kb + lane].st3 == 4);
     const int start = cursor;
     int np = 0;
+    // candidate i0 + lane: tracklet, list entries and unit, and its loads
+    auto load = [&](int i0, YzFirst& c) {
+        const int i = i0 + lane;
+        c.index = -1;
+        c.ustart = 0;
+        if (i >= I) return;
+        int j = 0;
+#pragma unroll
+        for (int t = 1; t < YZ_CHUNK; ++t) j += i >= off[t] ? 1 : 0;
+        const int4 nl = *reinterpret_cast<const int4*>(Y.nl[j]);
+        const int ojs = Y.uoff[j];
+        const int local = i - ojs, nin = nl.x * nl.y;
+        const int iuv3 = qdiv(local, Y.rin[j]), iuv2 = local - iuv3 * nin;
+        const int v2 = qdiv(iuv2, Y.rx[j]), v3 = qdiv(iuv3, Y.rz[j]);
+        const int e[4] = {iuv2 - v2 * nl.x, v2, iuv3 - v3 * nl.z, v3};  // u2, v2, u3, v3
+        c.index = i;
+        c.ustart = ojs + iuv3 * nin;
+        c.m = (d3m >> j) & 1;
+        yz_first_load(S, j, e, c);
+    };
+    YzFirst nxt;
+    load(start, nxt);
     for (int i0 = start; i0 < I; i0 += WAVE) {
         const int i = i0 + lane;
-        bool pass = false;
-        int ustart = 0;
-        if (i < I) {
-            int j = 0;
-#pragma unroll
-            for (int t = 1; t < YZ_CHUNK; ++t) j += i >= off[t] ? 1 : 0;
-            const int4 nl = *reinterpret_cast<const int4*>(Y.nl[j]);
-            int ojs = off[0];
-#pragma unroll
-            for (int t = 1; t < YZ_CHUNK; ++t) ojs = j == t ? off[t] : ojs;
-            const int local = i - ojs, nin = nl.x * nl.y;
-            const int iuv3 = qdiv(local, Y.rin[j]), iuv2 = local - iuv3 * nin;
-            ustart = ojs + iuv3 * nin;
-            const int v2 = qdiv(iuv2, Y.rx[j]), v3 = qdiv(iuv3, Y.rz[j]);
-            const int e[4] = {iuv2 - v2 * nl.x, v2, iuv3 - v3 * nl.z, v3};  // u2, v2, u3, v3
-            pass = yz_first_cut(S, j, e, (d3m >> j) & 1);
-        }
+        const YzFirst cur = nxt;
+        if (i0 + WAVE < I) load(i0 + WAVE, nxt);
+        const bool pass = yz_first_eval(cur);
+        const int ustart = cur.ustart;
         const unsigned long long m = ballot(pass);
         const int c = __popcll(m);
         if (np + c > cap) {
