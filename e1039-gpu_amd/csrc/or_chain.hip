@@ -185,6 +185,7 @@ __host__ __device__ constexpr int lds_waves(int maxh) { return maxh <= 512 ? 1 :
 struct YzWave {
     int nl[YZ_CHUNK][4];      // window list lengths (read as int4)
     int uoff[YZ_CHUNK + 1];   // prefix of (u3, v3) units or of candidates per tracklet of the chunk
+    int unoff[YZ_CHUNK + 1];  // prefix of (u3, v3) units per tracklet of the chunk (scoring)
     int umax[YZ_CHUNK];       // pruning: best unit score of each tracklet of the chunk (order-preserving int)
     unsigned long long best[YZ_CHUNK];
     uint32_t hx[YZ_CHUNK];    // hodoscope x-match mask of each tracklet of the chunk (yz_cuts_hx)
@@ -249,7 +250,7 @@ enum ScrRegion {
     R_CELL,   // uint32[1569]: XZ cell offsets
     R_LIST,   // uint16[64][win_cap]: window pair lists (YZ: tracklet x view; station 1: lane)
     R_YV,     // float4[64][win_cap]: fit terms (w y, w y z) of YZ list entries' hits a, b; station-1 records
-    R_YY,     // float2[64][win_cap]: y of YZ list entries' hits a, b
+    R_YY,     // float2[64][win_cap]: y of YZ list entries' hits a, b (station 2: the entry's mean y, 0)
     R_YF,     // uint8[64][win_cap]: accepted-hit flags of YZ list entries
     R_XP,     // uint16[3][28][bin_cap]: binned X pairs
     R_SO,     // uint16[maxtkl]: tracklet index by slot order
@@ -543,6 +544,25 @@ __device__ __forceinline__ bool prop_sorted_hit(const float* s, int n, float e, 
     }
     const float below = s[max(lo - 1, 0)], above = s[min(lo, max(n - 1, 0))];
     return (lo > 0 && fabsf(below - e) < win) || (lo < n && fabsf(above - e) < win);
+}
+
+// min(dmin, |p - e|) over the sorted positions s[0, n) of a plane: by the same
+// monotonicity, the smallest |fl(p - e)| is that of one of e's two neighbours
+__device__ __forceinline__ float prop_sorted_nearest(const float* s, int n, float e, float dmin) {
+    int lo = 0, len = n;
+    while (len > 0) {
+        const int half = len >> 1;
+        if (s[lo + half] < e) {
+            lo += half + 1;
+            len -= half + 1;
+        } else {
+            len = half;
+        }
+    }
+    const float below = s[max(lo - 1, 0)], above = s[min(lo, max(n - 1, 0))];
+    if (lo > 0) dmin = fminf(dmin, fabsf(below - e));
+    if (lo < n) dmin = fminf(dmin, fabsf(above - e));
+    return dmin;
 }
 
 // prop_hit for the planes psort_plane(2 pp) then psort_plane(2 pp + 1)
@@ -1001,6 +1021,7 @@ struct Scr {
     // per YZ list entry: (w y, w y z) of hit a, then of hit b, w the plane's
     // weight and z its position; the y values themselves in wyy
     __device__ float4* wy() const { return reinterpret_cast<float4*>(cfg().rp[R_YV]); }
+    // (station 2: x = the entry's y, the mean of its accepted hits)
     __device__ float2* wyy() const { return reinterpret_cast<float2*>(cfg().rp[R_YY]); }
     __device__ uint8_t* wf() const { return reinterpret_cast<uint8_t*>(cfg().rp[R_YF]); }
     // [3][28][bin_cap] X pairs (a | b << 8), 0xFF = none
@@ -1643,18 +1664,13 @@ __device__ __forceinline__ UnitFeatures unit_features(const Scr& S, const Tk& T,
         const int n = v == 0 ? nl.x : nl.y;
         const int q0 = (j * 4 + v) * win;
         for (int k0 = 0; k0 < n; k0 += 4) {
-            float2 e[4];
-            unsigned fl[4];
+            float ye[4];
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                const int q = q0 + min(k0 + t, n - 1);
-                e[t] = gld2(S.wyy() + q);
-                fl[t] = gld(S.wf() + q);
-            }
+            for (int t = 0; t < 4; ++t) ye[t] = gld(reinterpret_cast<const float*>(S.wyy() + q0 + min(k0 + t, n - 1)));
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
                 if (k0 + t >= n) break;
-                const float y = (fl[t] & 1u) ? ((fl[t] & 2u) ? 0.5f * (e[t].x + e[t].y) : e[t].x) : e[t].y;
+                const float y = ye[t];
                 const float d = fabsf(y - yp);
                 if (d < dmin) { dmin = d; ybest = y; kb = k0 + t; }
             }
@@ -1683,15 +1699,24 @@ __device__ __forceinline__ UnitFeatures unit_features(const Scr& S, const Tk& T,
         }
     }
     float dp[2];
+    const Lds<MAXH>& L = g_lds<MAXH>;
 #pragma unroll
     for (int p = 0; p < 2; ++p) {
-        const int da = p == 0 ? 48 : 54, db = da - 1;
+        const int da = psort_plane(2 * p), db = psort_plane(2 * p + 1);   // 48, 47; 54, 53
         const float ypa = k3 * G(F_Z, da), ypb = k3 * G(F_Z, db);
         float dmin = 100.f;
+        if (L.pscnt[2 * p] >= 0) {
+            dmin = prop_sorted_nearest(L.psort[2 * p], L.pscnt[2 * p], ypa, dmin);
+        } else {
 #pragma unroll 4
-        for (int i = 0; i < E.nred(da); ++i) dmin = fminf(dmin, fabsf(E.Lp->pos[E.raw(da, i)] - ypa));
+            for (int i = 0; i < E.nred(da); ++i) dmin = fminf(dmin, fabsf(E.Lp->pos[E.raw(da, i)] - ypa));
+        }
+        if (L.pscnt[2 * p + 1] >= 0) {
+            dmin = prop_sorted_nearest(L.psort[2 * p + 1], L.pscnt[2 * p + 1], ypb, dmin);
+        } else {
 #pragma unroll 4
-        for (int i = 0; i < E.nred(db); ++i) dmin = fminf(dmin, fabsf(E.Lp->pos[E.raw(db, i)] - ypb));
+            for (int i = 0; i < E.nred(db); ++i) dmin = fminf(dmin, fabsf(E.Lp->pos[E.raw(db, i)] - ypb));
+        }
         dp[p] = dmin;
     }
     UnitFeatures f;
@@ -1717,6 +1742,30 @@ __device__ __forceinline__ UnitFeatures unit_features(const Scr& S, const Tk& T,
 // first fit and box cut of YZ candidate (tracklet j of the chunk, list entries
 // e = u2, v2, u3, v3; m: station 3 is D3m): B sums in the reference order u3,
 // v3, u2, v2, a then b, the rest from the first-fit table
+// the chunk's tracklet holding (u3, v3) unit u of the scoring enumeration
+template <int MAXH>
+__device__ __forceinline__ int unit_tracklet(int u) {
+    const YzWave& Y = yzw<MAXH>();
+    int j = 0;
+#pragma unroll
+    for (int t = 1; t < YZ_CHUNK; ++t) j += u >= Y.unoff[t] ? 1 : 0;
+    return j;
+}
+
+// features of unit u of the chunk starting at tracklet kb (zeros when u >= U)
+template <int MAXH, int WPE>
+__device__ __attribute__((noinline)) UnitFeatures unit_features_at(Scr S, int kb, int u, int U) {
+    UnitFeatures f;
+#pragma unroll
+    for (int q = 0; q < PRUNE_NF; ++q) f.v[q] = 0.f;
+    if (u < U) {
+        const YzWave& Y = yzw<MAXH>();
+        const int j = unit_tracklet<MAXH>(u);
+        f = unit_features<MAXH, WPE>(S, S.tks()[kb + j], j, u - Y.unoff[j], *reinterpret_cast<const int4*>(Y.nl[j]));
+    }
+    return f;
+}
+
 // Pass 1 splits it in two: the loads of a candidate (YzFirst) are issued one
 // wave iteration ahead of its evaluation.
 struct YzFirst {
@@ -1782,6 +1831,8 @@ __device__ __attribute__((noinline)) UnitScore yz_score_units(Scr S, Prn R, int 
         uo[0] = 0;
 #pragma unroll
         for (int t = 1; t <= YZ_CHUNK; ++t) uo[t] = __builtin_amdgcn_readfirstlane(__shfl(inc, t - 1, WAVE));
+        if (lane < YZ_CHUNK) Y.unoff[lane + 1] = inc;
+        if (lane == 0) Y.unoff[0] = 0;
     }
     const int U = uo[YZ_CHUNK];
     if (U > cfg().unit_cap) return res;
@@ -1802,25 +1853,16 @@ __device__ __attribute__((noinline)) UnitScore yz_score_units(Scr S, Prn R, int 
     for (int u0 = 0; u0 < U; u0 += WAVE) {
         const int u = u0 + lane;
         const bool valid = u < U;
-        int j = 0;
-#pragma unroll
-        for (int t = 1; t < YZ_CHUNK; ++t) j += u >= uo[t] ? 1 : 0;
-        int ujs = uo[0];
-#pragma unroll
-        for (int t = 1; t < YZ_CHUNK; ++t) ujs = j == t ? uo[t] : ujs;
-        const int iuv3 = u - ujs;
-        UnitFeatures f;
-#pragma unroll
-        for (int q = 0; q < PRUNE_NF; ++q) f.v[q] = 0.f;
-        if (valid) f = unit_features<MAXH, WPE>(S, tks[kb + j], j, iuv3, *reinterpret_cast<const int4*>(Y.nl[j]));
+        const UnitFeatures f = unit_features_at<MAXH, WPE>(S, kb, u, U);
         if (R.prune) {
             const float score = prune_score(R.prune, f);
             if (valid) {
                 sc[u] = score;
-                if (rel) atomicMax(&Y.umax[j], float_order(score));
+                if (rel) atomicMax(&Y.umax[unit_tracklet<MAXH>(u)], float_order(score));
             }
         }
         if (res.dbase >= 0 && valid) {
+            const int j = unit_tracklet<MAXH>(u), iuv3 = u - Y.unoff[j];
             float* row = R.dump + (res.dbase + u) * OR_UNIT_ROW;
             row[0] = (float)R.ev;
             row[1] = (float)tks[kb + j].canon;
@@ -2495,7 +2537,10 @@ STAGE void stage_yz(or_params P, Scr S, Prn R, int ntkl, int list_cap) {
                         lst[nacc] = lst[q];
                         // fit terms w y and (w y) z of each hit, as the fit sums them
                         wy[o] = make_float4(yv.y * yv.x, (yv.y * yv.x) * za, yv.w * yv.z, (yv.w * yv.z) * zb);
-                        wyy[o] = make_float2(yv.x, yv.z);
+                        // station 3: the y of hits a and b; station 2: the entry's y
+                        // (the mean of its accepted hits) for the pruner's features
+                        wyy[o] = v >= 2 ? make_float2(yv.x, yv.z)
+                                        : make_float2((fl & 1u) ? ((fl & 2u) ? 0.5f * (yv.x + yv.z) : yv.x) : yv.z, 0.f);
                         wf[o] = fl;
                         ++nacc;
                     }
