@@ -18,7 +18,9 @@ json when its kernel-source hash, batch and settings match the timed run.
 import argparse
 import hashlib
 import json
+import csv
 import os
+import re
 import sqlite3
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -140,7 +142,8 @@ def main():
     with open(base + "_counters.json", "w") as f:
         json.dump(res, f, indent=1)
     cmds = open(a.commands).read().strip() if a.commands else ""
-    lines = [f"# {dom.split('(')[0].replace('void ', '')}, {a.workload}, {a.mode} chain, {a.events} events",
+    kname = re.search(r"chain_kernel<[^>]*>", dom).group(0)
+    lines = [f"# {kname}, {a.workload}, {a.mode} chain, {a.events} events",
              "", f"Settings: {a.waves_per_event} waves per event, YZ survivor batch {a.yz_batch}; kernel sources "
              f"sha256 {res['kernel_source_sha256_16']} ({', '.join(KERNEL_SOURCES)}).", "",
              "Commands (repository root on the GPU box as `$REPO`; one counter block per run):", "", "```", cmds,
@@ -148,7 +151,16 @@ def main():
              f"Launch of the {a.mode} chain: **{ms:.1f} ms**, {res['workgroups']} workgroups of "
              f"{res['workgroup_size']} lanes, {launch['lds_size']} B LDS, {launch['scratch_size']} B/lane scratch. "
              f"All launches of this kernel in the run (ms): {', '.join(f'{x:.1f}' for x in res['all_launch_ms'])}.",
-             "", "## HBM bytes", "",
+             "", "rocprofv3 --stats of the same run (kernel-trace run's run_kernel_stats.csv), chain kernels:", "",
+             "| kernel | calls | total ms | average ms |", "|---|---|---|---|"]
+    stats = os.path.join(a.dir, f"kt_{a.mode}", "run_kernel_stats.csv")
+    if os.path.exists(stats):
+        with open(stats) as f:
+            for r in csv.DictReader(f):
+                if "chain_kernel" in r["Name"]:
+                    lines.append(f"| {re.search(r'chain_kernel<[^>]*>', r['Name']).group(0)} | {r['Calls']} | "
+                                 f"{float(r['TotalDurationNs']) / 1e6:.1f} | {float(r['AverageNs']) / 1e6:.1f} |")
+    lines += ["", "## HBM bytes", "",
              f"FETCH_SIZE {fetch_kib:,.0f} KiB, WRITE_SIZE {write_kib:,.0f} KiB per launch. Calibration: the 1 GiB "
              f"device copy reads {cfk:,.1f} KiB and writes {cwk:,.1f} KiB by the same counters, so FETCH is scaled by "
              f"{f_scale:.4f} and WRITE by {w_scale:.4f}.", "",
