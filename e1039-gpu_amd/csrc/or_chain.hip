@@ -251,7 +251,7 @@ enum ScrRegion {
     R_LIST,   // uint16[64][win_cap]: window pair lists (YZ: tracklet x view; station 1: lane)
     R_YV,     // float4[64][win_cap]: fit terms (w y, w y z) of YZ list entries' hits a, b; station-1 records
     R_YY,     // float2[64][win_cap]: y of YZ list entries' hits a, b (station 2: the entry's mean y, 0)
-    R_YF,     // uint8[64][win_cap]: accepted-hit flags of YZ list entries
+    R_YF,     // uint8[64][win_cap]: accepted-hit flags of YZ list entries | first-fit pattern term << 2
     R_XP,     // uint16[3][28][bin_cap]: binned X pairs
     R_SO,     // uint16[maxtkl]: tracklet index by slot order
     R_PL,     // uint32[list_cap]: YZ first-box survivors of a batch; station-1 pass-A list
@@ -796,7 +796,7 @@ __device__ __forceinline__ uint4 gld4(const uint4* p) {
 struct YzLists {
     const uint16_t* l[4];   // pair entries (a | b << 8), views u2, v2, u3, v3
     const float4* y[4];     // (w y, w y z) of hit a, then of hit b
-    const uint8_t* f[4];    // bit 0: hit a present and accepted, bit 1: hit b
+    const uint8_t* f[4];    // bit 0: hit a present and accepted, bit 1: hit b; bits 2-7: pattern term (yz_pattern)
     int n[4];
 };
 
@@ -1513,8 +1513,9 @@ __device__ __forceinline__ YzItem yz_decode(int i) {
 // needs only its B sums.  pattern = 27 (fu3 - 1) + 9 (fv3 - 1) + 3 (fu2 - 1) + fv2 - 1.
 __device__ float4 g_yz_atab[2 * 81 * 2];
 
-__device__ __forceinline__ int yz_pattern(unsigned fu2, unsigned fv2, unsigned fu3, unsigned fv3) {
-    return (int)(27 * (fu3 - 1) + 9 * (fv3 - 1) + 3 * (fu2 - 1) + (fv2 - 1));
+// The pattern index from list flag bytes, which carry each entry's term in bits 2-7
+__device__ __forceinline__ int yz_pattern_b(unsigned bu2, unsigned bv2, unsigned bu3, unsigned bv3) {
+    return (int)((bu2 >> 2) + (bv2 >> 2) + (bu3 >> 2) + (bv3 >> 2));
 }
 
 __global__ __launch_bounds__(256) void yz_atab_kernel() {
@@ -1576,7 +1577,7 @@ __device__ __forceinline__ bool yz_fit(const YzLists& W, int st3, const YzItem& 
     yz_bsum(fv3, gld4(W.y[3] + it.iv3), B0, B1);
     yz_bsum(fu2, gld4(W.y[0] + it.iu2), B0, B1);
     yz_bsum(fv2, gld4(W.y[1] + it.iv2), B0, B1);
-    f = yz_fit_tab(st3 == 4 ? 1 : 0, yz_pattern(fu2, fv2, fu3, fv3), B0, B1);
+    f = yz_fit_tab(st3 == 4 ? 1 : 0, yz_pattern_b(fu2, fv2, fu3, fv3), B0, B1);
     return true;
 }
 
@@ -1646,7 +1647,7 @@ __device__ __forceinline__ UnitFeatures unit_features(const Scr& S, const Tk& T,
     const int iv3 = qdiv(iuv3, yzw<MAXH>().rz[j]), iu3 = iuv3 - iv3 * nl.z;
     const int qu = (j * 4 + 2) * win + iu3, qv = (j * 4 + 3) * win + iv3;
     const float2 yu = gld2(S.wyy() + qu), yw = gld2(S.wyy() + qv);
-    const unsigned fu = gld(S.wf() + qu), fv = gld(S.wf() + qv);
+    const unsigned fu = gld(S.wf() + qu) & 3u, fv = gld(S.wf() + qv) & 3u;
     const float ya = (fu & 1u) ? yu.x : yu.y, yb = (fu & 2u) ? yu.y : yu.x;
     const float yc = (fv & 1u) ? yw.x : yw.y, yd = (fv & 2u) ? yw.y : yw.x;
     const float yum = 0.5f * (ya + yb), yvm = 0.5f * (yc + yd);
@@ -1789,7 +1790,7 @@ __device__ __forceinline__ bool yz_first_eval(const YzFirst& c) {
     yz_bsum(c.fl[3], c.t[3], B0, B1);
     yz_bsum(c.fl[0], c.t[0], B0, B1);
     yz_bsum(c.fl[1], c.t[1], B0, B1);
-    return yz_box1(yz_fit_tab((int)c.m, yz_pattern(c.fl[0], c.fl[1], c.fl[2], c.fl[3]), B0, B1));
+    return yz_box1(yz_fit_tab((int)c.m, yz_pattern_b(c.fl[0], c.fl[1], c.fl[2], c.fl[3]), B0, B1));
 }
 
 // Pass 1 streams in batches.  The survivors of the first box cut go into a
@@ -1982,22 +1983,14 @@ __device__ __attribute__((noinline)) int yz_pass1_kept(Scr S, int kb, int ntkl, 
             const int4 nl = *reinterpret_cast<const int4*>(Y.nl[j]);
             const int ojs = Y.uoff[j];
             const int v2 = qdiv(inner, Y.rx[j]), v3 = qdiv(iuv3, Y.rz[j]);
-            const int e[4] = {inner - v2 * nl.x, v2, iuv3 - v3 * nl.z, v3};
-            c.ustart = ojs + iuv3 * (nl.x * nl.y);
+            const int e[4] = {inner - __umul24(v2, nl.x), v2, iuv3 - __umul24(v3, nl.z), v3};
+            c.ustart = ojs + __umul24(iuv3, __umul24(nl.x, nl.y));
             c.index = c.ustart + inner;
             c.m = (d3m >> j) & 1;
             yz_first_load(S, j, e, c);
         }
     };
-    YzFirst nxt;
-    uint32_t nks = 0;
-    int nku = 0;
-    load(start, nxt, nks, nku);
-    for (uint32_t i0 = start; i0 < run; i0 += WAVE) {
-        const YzFirst cur = nxt;
-        const uint32_t st_t = nks;
-        const int ku = nku;
-        if (i0 + WAVE < run) load(i0 + WAVE, nxt, nks, nku);
+    auto take = [&](uint32_t i0, const YzFirst& cur, uint32_t st_t, int ku) -> int {
         const bool pass = yz_first_eval(cur);
         const int iorig = cur.index, ustart = cur.ustart;
         const unsigned long long m = ballot(pass);
@@ -2020,6 +2013,21 @@ __device__ __attribute__((noinline)) int yz_pass1_kept(Scr S, int kb, int ntkl, 
         }
         if (pass) S.plist()[np + rank_in(m)] = (uint32_t)iorig;
         np += c;
+        return INT_MIN;
+    };
+    // two candidate buffers (see yz_pass1_plain)
+    YzFirst ca, cb;
+    uint32_t ksa = 0, ksb = 0;
+    int kua = 0, kub = 0;
+    load(start, ca, ksa, kua);
+    for (uint32_t i0 = start; i0 < run; i0 += 2 * WAVE) {
+        if (i0 + WAVE < run) load(i0 + WAVE, cb, ksb, kub);
+        int r = take(i0, ca, ksa, kua);
+        if (r != INT_MIN) return r;
+        if (i0 + WAVE >= run) break;
+        if (i0 + 2 * WAVE < run) load(i0 + 2 * WAVE, ca, ksa, kua);
+        r = take(i0 + WAVE, cb, ksb, kub);
+        if (r != INT_MIN) return r;
     }
     cursor = run;
     lo_unit = nk;
@@ -2044,31 +2052,43 @@ __device__ __attribute__((noinline)) int yz_pass1_plain(Scr S, int kb, int ntkl,
     const int start = cursor;
     int np = 0;
     // candidate i0 + lane: tracklet, list entries and unit, and its loads
+    auto fill = [&](int i, int j, YzFirst& c) {
+        const int4 nl = *reinterpret_cast<const int4*>(Y.nl[j]);
+        const int ojs = Y.uoff[j];
+        // enumeration indices stay below 2^27 and radices below 2^16: 24-bit products are exact
+        const int local = i - ojs, nin = __umul24(nl.x, nl.y);
+        const int iuv3 = qdiv(local, Y.rin[j]), iuv2 = local - __umul24(iuv3, nin);
+        const int v2 = qdiv(iuv2, Y.rx[j]), v3 = qdiv(iuv3, Y.rz[j]);
+        const int e[4] = {iuv2 - __umul24(v2, nl.x), v2, iuv3 - __umul24(v3, nl.z), v3};  // u2, v2, u3, v3
+        c.index = i;
+        c.ustart = ojs + __umul24(iuv3, nin);
+        c.m = (d3m >> j) & 1;
+        yz_first_load(S, j, e, c);
+    };
     auto load = [&](int i0, YzFirst& c) {
         const int i = i0 + lane;
         c.index = -1;
         c.ustart = 0;
-        if (i >= I) return;
-        int j = 0;
+        // the tracklet of the wave's first candidate; when every candidate of the
+        // wave lies in it, the tracklet index is wave-uniform
+        int j0 = 0;
 #pragma unroll
-        for (int t = 1; t < YZ_CHUNK; ++t) j += i >= off[t] ? 1 : 0;
-        const int4 nl = *reinterpret_cast<const int4*>(Y.nl[j]);
-        const int ojs = Y.uoff[j];
-        const int local = i - ojs, nin = nl.x * nl.y;
-        const int iuv3 = qdiv(local, Y.rin[j]), iuv2 = local - iuv3 * nin;
-        const int v2 = qdiv(iuv2, Y.rx[j]), v3 = qdiv(iuv3, Y.rz[j]);
-        const int e[4] = {iuv2 - v2 * nl.x, v2, iuv3 - v3 * nl.z, v3};  // u2, v2, u3, v3
-        c.index = i;
-        c.ustart = ojs + iuv3 * nin;
-        c.m = (d3m >> j) & 1;
-        yz_first_load(S, j, e, c);
+        for (int t = 1; t < YZ_CHUNK; ++t) j0 += i0 >= off[t] ? 1 : 0;
+        const int end0 = __builtin_amdgcn_readfirstlane(Y.uoff[j0 + 1]);
+        if (i0 + WAVE <= end0 || I <= end0) {
+            if (i < I) fill(i, j0, c);
+        } else if (i < I) {
+            int j = 0;
+#pragma unroll
+            for (int t = 1; t < YZ_CHUNK; ++t) j += i >= off[t] ? 1 : 0;
+            fill(i, j, c);
+        }
     };
-    YzFirst nxt;
-    load(start, nxt);
-    for (int i0 = start; i0 < I; i0 += WAVE) {
+    // one wave iteration: evaluate, list the survivors; a batch that fills
+    // ends before the first unit that does not fit (returns its survivor count,
+    // or -1 when that unit is the batch's first); INT_MIN: go on
+    auto take = [&](int i0, const YzFirst& cur) -> int {
         const int i = i0 + lane;
-        const YzFirst cur = nxt;
-        if (i0 + WAVE < I) load(i0 + WAVE, nxt);
         const bool pass = yz_first_eval(cur);
         const int ustart = cur.ustart;
         const unsigned long long m = ballot(pass);
@@ -2086,6 +2106,20 @@ __device__ __attribute__((noinline)) int yz_pass1_plain(Scr S, int kb, int ntkl,
         }
         if (pass) S.plist()[np + rank_in(m)] = (uint32_t)i;
         np += c;
+        return INT_MIN;
+    };
+    // two candidate buffers: the loads of the next wave iteration are in flight
+    // while the current one is evaluated
+    YzFirst ca, cb;
+    load(start, ca);
+    for (int i0 = start; i0 < I; i0 += 2 * WAVE) {
+        if (i0 + WAVE < I) load(i0 + WAVE, cb);
+        int r = take(i0, ca);
+        if (r != INT_MIN) return r;
+        if (i0 + WAVE >= I) break;
+        if (i0 + 2 * WAVE < I) load(i0 + 2 * WAVE, ca);
+        r = take(i0 + WAVE, cb);
+        if (r != INT_MIN) return r;
     }
     cursor = I;
     return np;
@@ -2541,7 +2575,8 @@ STAGE void stage_yz(or_params P, Scr S, Prn R, int ntkl, int list_cap) {
                         // (the mean of its accepted hits) for the pruner's features
                         wyy[o] = v >= 2 ? make_float2(yv.x, yv.z)
                                         : make_float2((fl & 1u) ? ((fl & 2u) ? 0.5f * (yv.x + yv.z) : yv.x) : yv.z, 0.f);
-                        wf[o] = fl;
+                        // the entry's term of the first-fit pattern index (yz_pattern)
+                        wf[o] = (uint8_t)(fl | ((fl - 1u) * (v == 0 ? 3u : v == 1 ? 1u : v == 2 ? 27u : 9u) << 2));
                         ++nacc;
                     }
                 }
