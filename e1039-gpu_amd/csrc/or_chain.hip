@@ -1957,13 +1957,19 @@ __device__ __attribute__((noinline)) int yz_pass1_kept(Scr S, int kb, int ntkl, 
         const bool inwin = lo0 + lane < nk;
         const uint32_t st = inwin ? S.kstart()[lo0 + lane] : 0xFFFFFFFFu;
         const uint32_t kl = inwin ? S.klist()[lo0 + lane] : 0u;
-        int tu = 0, tn = 0;
+        // the starts are increasing over the lanes, so the units starting at or
+        // before i0 + 64 are a prefix of them
+        const int tn = __popcll(ballot(st <= i0 + WAVE)) - 1;
+        int tu = 0;
+        if ((uint32_t)__builtin_amdgcn_readlane((int)st, 2) > i0 + WAVE - 1) {
+            // the wave's candidates lie in units lo0 and lo0 + 1 (the common case)
+            tu = i >= (uint32_t)__builtin_amdgcn_readlane((int)st, 1) ? 1 : 0;
+        } else {
 #pragma unroll
-        for (int step = WAVE / 2; step > 0; step >>= 1) {
-            const uint32_t sv = __shfl(st, tu + step, WAVE);
-            if (sv <= i) tu += step;
-            const uint32_t sn = __shfl(st, tn + step, WAVE);
-            if (sn <= i0 + WAVE) tn += step;
+            for (int step = WAVE / 2; step > 0; step >>= 1) {
+                const uint32_t sv = __shfl(st, tu + step, WAVE);
+                if (sv <= i) tu += step;
+            }
         }
         const uint32_t st_t = __shfl(st, tu, WAVE);
         const uint32_t code = __shfl(kl, tu, WAVE);
