@@ -233,3 +233,30 @@ def test_full_chain(oc, orc, geo, ogeo):
         same = np.array_equal(got.vertex[k].view(np.uint32), ref.vertex[k].view(np.uint32))
         assert same, "vertex differs"
 
+
+
+def test_dense_prop_planes(oc, orc, geo, ogeo, session):
+    # The YZ prop-tube Y tests search each plane's sorted positions when the
+    # plane holds at most psort_cap hits (32 in the 512-hit tile) and scan it
+    # hit by hit otherwise: events of the 512-hit tile with 0 to 60 extra hits on
+    # each of planes 47, 48, 53, 54 take both forms, in one batch
+    rng = np.random.default_rng(21)
+    base = oc.generate(geo, 600, seed=22, mult_mode=1, mult_mean=250.0, mult_cap=270)
+    evs = []
+    for e in range(base.n_events):
+        sub = base.subset(e, e + 1)
+        parts = [(sub.det, sub.elem, sub.tdc, sub.drift, sub.flag)]
+        extra = (e % 4) * 20   # 0, 20, 40, 60 per plane
+        for d in (47, 48, 53, 54):
+            if extra:
+                parts.append((np.full(extra, d, np.int16), rng.integers(1, 73, extra).astype(np.int16),
+                              rng.uniform(0, 100, extra).astype(np.float32), rng.uniform(0, 2.5, extra).astype(np.float32),
+                              np.full(extra, 2, np.uint8)))
+        evs.append(tuple(np.concatenate([p[i] for p in parts]) for i in range(5)))
+    offs = np.concatenate([[0], np.cumsum([len(x[0]) for x in evs])]).astype(np.int64)
+    assert np.diff(offs).max() <= 512
+    cat = lambda i: np.concatenate([x[i] for x in evs])
+    ev = oc.Events(cat(0), cat(1), cat(2), cat(3), cat(4), offs)
+    got, ref = run_both(oc, orc, session, ogeo, ev)
+    assert ref.n_tracks.sum() > 100
+    assert_same(got, ref, ev, "dense prop planes")
