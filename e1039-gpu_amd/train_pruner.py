@@ -1,6 +1,7 @@
 """Trains the YZ unit pruner on the plain chain's own decisions (needs the GPU).
 
     python e1039-gpu_amd/train_pruner.py [--train 20000] [--out e1039-gpu_amd/data/pruner_u3v3.npz]
+    python e1039-gpu_amd/train_pruner.py --refine-margin 1.9 2.125 0.025   (weights kept, margin re-picked)
 
 Workload: hits per event Poisson(1200) capped at 4000 with the high-occupancy
 caps (orchain.HIGH_OCCUPANCY_CAPS), the learned-pruning configuration.
@@ -47,6 +48,9 @@ def main():
     ap.add_argument("--shard", type=int, default=1000, help="events per unit dump")
     ap.add_argument("--target", type=float, default=0.996, help="track recall the margin must reach")
     ap.add_argument("--out", default=os.path.join(HERE, "data", "pruner_u3v3.npz"))
+    ap.add_argument("--refine-margin", nargs=3, type=float, metavar=("LO", "HI", "STEP"),
+                    help="keep the weights of --out and re-pick its margin on the test events from the grid "
+                         "LO, LO + STEP, ... <= HI (the smallest reaching --target)")
     args = ap.parse_args()
 
     import numpy as np
@@ -102,6 +106,31 @@ def main():
         sess.run_device(t, o)
         torch.cuda.synchronize()
         return res, sess.last_kernel_ms()
+
+    if args.refine_margin:
+        flat, _, meta = pruner.load(args.out)
+        te = oc.generate(geo, args.test, seed=0x7A33, **WORKLOAD)
+        plain, plain_ms = run(te)
+        lo, hi, step = args.refine_margin
+        chosen, trials = None, []
+        for k in range(int(round((hi - lo) / step)) + 1):
+            mg = round(lo + k * step, 6)
+            pr, ms = run(te, flat, mg)
+            rec, found, total = pruner.recall(plain, pr)
+            entry = {"margin": mg, "track_recall": rec, "tracks": total, "tracks_identical": found,
+                     "plain_ms": plain_ms, "pruned_ms": ms, "speedup": plain_ms / ms}
+            trials.append(entry)
+            log(json.dumps(entry))
+            if chosen is None and rec >= args.target:
+                chosen = entry
+        sess.set_pruner(None)
+        if chosen is None:
+            raise RuntimeError("no margin of the grid reaches the target recall")
+        meta.update({"margin": chosen["margin"], "chosen": chosen,
+                     "margin_refinement": {"grid": [lo, hi, step], "target": args.target, "trials": trials}})
+        pruner.save(args.out, flat, pruner.THRESHOLD_OFF, meta)
+        log(json.dumps({"saved": args.out, **chosen}))
+        return
 
     t0 = time.time()
     x, g, w = dump(oc.generate(geo, args.train, seed=0x7A31, **WORKLOAD))
