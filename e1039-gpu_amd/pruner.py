@@ -200,7 +200,9 @@ def train_rank(feats, group, winner, epochs: int = 12, seed: int = 7, device: st
         for ep in range(nep):
             rng.shuffle(order)
             tot, cnt = 0.0, 0
-            for k in order:
+            for it, k in enumerate(order):
+                if it % 50 == 49:
+                    log(f"  epoch {ep}: {it + 1} of {len(order)} batches")
                 a, b = cuts[k], cuts[k + 1]
                 wb = w[a:b]
                 gb = g[a:b] - g[a]

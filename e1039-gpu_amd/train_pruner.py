@@ -46,6 +46,11 @@ def main():
     ap.add_argument("--test", type=int, default=8_000, help="events for the track-recall check")
     ap.add_argument("--epochs", type=int, default=16)
     ap.add_argument("--shard", type=int, default=1000, help="events per unit dump")
+    ap.add_argument("--sample", type=int, default=1,
+                    help="keep the units of one tracklet in SAMPLE (a fixed hash of event and tracklet), so that "
+                         "more events fit the same number of rows")
+    ap.add_argument("--train-only", action="store_true",
+                    help="save the trained weights without picking a margin (then --refine-margin)")
     ap.add_argument("--target", type=float, default=0.996, help="track recall the margin must reach")
     ap.add_argument("--out", default=os.path.join(HERE, "data", "pruner_u3v3.npz"))
     ap.add_argument("--refine-margin", nargs=3, type=float, metavar=("LO", "HI", "STEP"),
@@ -64,12 +69,15 @@ def main():
     sess = oc.Session(geo, params, device=0)
     dev = torch.device("cuda", 0)
 
-    def dump(ev):
-        """plain chain with unit rows, in shards; returns (features, group id, winner) on the device"""
+    def dump(n_events, seed, sample=1):
+        """plain chain with unit rows on n_events generated events, in shards (shard k
+        generated with seed 2^20 seed + k, apart from the test events' seed); with
+        sample > 1 only the units of one tracklet in sample are kept; returns
+        (features, group id, winner) on the device"""
         feats, groups, wins = [], [], []
         gbase = 0
-        for s0 in range(0, ev.n_events, args.shard):
-            sub = ev.subset(s0, min(ev.n_events, s0 + args.shard))
+        for k, s0 in enumerate(range(0, n_events, args.shard)):
+            sub = oc.generate(geo, min(n_events, s0 + args.shard) - s0, seed=(seed << 20) + k, **WORKLOAD)
             cap = sub.n_events * 200_000
             rows = torch.empty((cap, oc.UNIT_ROW), dtype=torch.float32, device=dev)
             count = torch.zeros(1, dtype=torch.int64, device=dev)
@@ -86,6 +94,10 @@ def main():
             r = rows[:n]
             r = r[r[:, 3] >= 0]          # chunks that went unit by unit carry no winner flags
             key = r[:, 0].long() * (1 << 20) + r[:, 1].long()
+            if sample > 1:
+                h = ((key + (s0 << 20)) * 2654435761) % 4294967296
+                keep = (h >> 16) % sample == 0
+                r, key = r[keep], key[keep]
             chg = torch.cat([torch.ones(1, dtype=torch.bool, device=dev), key[1:] != key[:-1]])
             g = torch.cumsum(chg.long(), 0) - 1 + gbase
             gbase = int(g[-1]) + 1 if g.numel() else gbase
@@ -93,6 +105,8 @@ def main():
             groups.append(g)
             wins.append(r[:, 3] > 0.5)
             del rows, r, o, t
+            if k % 50 == 49:
+                log(f"  dumped {s0 + sub.n_events} of {n_events} events, {sum(f.shape[0] for f in feats)} units")
         return torch.cat(feats), torch.cat(groups), torch.cat(wins)
 
     def run(ev, flat=None, margin=float("inf")):
@@ -133,16 +147,24 @@ def main():
         return
 
     t0 = time.time()
-    x, g, w = dump(oc.generate(geo, args.train, seed=0x7A31, **WORKLOAD))
-    log(f"training: {args.train} events, {x.shape[0]} units ({x.shape[0] / args.train:.0f}/event), "
+    x, g, w = dump(args.train, 0x7A31, args.sample)
+    log(f"training: {args.train} events (one tracklet in {args.sample}), {x.shape[0]} units, "
         f"{int(g[-1]) + 1} tracklets, {int(w.sum())} winners; dumped in {time.time() - t0:.1f}s")
     t0 = time.time()
     flat = pruner.train_rank(x, g, w, epochs=args.epochs, log=log)
     log(f"trained in {time.time() - t0:.1f}s")
     del x, g, w
     torch.cuda.empty_cache()
+    if args.train_only:
+        meta = {"features": pruner.FEATURE_NAMES, "train_events": args.train, "train_sample": args.sample,
+                "workload": WORKLOAD, "caps": oc.HIGH_OCCUPANCY_CAPS, "seeds": "train 2^20 0x7A31 + shard, test 0x7A33",
+                "loss": "listwise (softmax over a tracklet's units)", "epochs": [args.epochs, 4],
+                "margin": float("inf")}
+        pruner.save(args.out, flat, pruner.THRESHOLD_OFF, meta)
+        log(json.dumps({"saved": args.out, "trained": True}))
+        return
 
-    xv, gv, wv = dump(oc.generate(geo, args.valid, seed=0x7A32, **WORKLOAD))
+    xv, gv, wv = dump(args.valid, 0x7A32)
     sv = pruner.score_reference(flat, xv).reshape(-1)
     gaps, gmax = pruner.margin_gaps(sv, gv, wv)
     gcpu = gv.cpu()
@@ -171,9 +193,10 @@ def main():
     sess.set_pruner(None)
     if chosen is None:
         raise RuntimeError("no margin reaches the target recall")
-    meta = {"features": pruner.FEATURE_NAMES, "train_events": args.train, "valid_events": args.valid,
+    meta = {"features": pruner.FEATURE_NAMES, "train_events": args.train, "train_sample": args.sample,
+            "valid_events": args.valid,
             "test_events": args.test, "workload": WORKLOAD, "caps": oc.HIGH_OCCUPANCY_CAPS,
-            "seeds": "train 0x7A31, valid 0x7A32, test 0x7A33", "loss": "listwise (softmax over a tracklet's units)",
+            "seeds": "train 2^20 0x7A31 + shard, valid 2^20 0x7A32 + shard, test 0x7A33", "loss": "listwise (softmax over a tracklet's units)",
             "margin": chosen["margin"], "target": args.target, "chosen": chosen, "trials": trials}
     pruner.save(args.out, flat, pruner.THRESHOLD_OFF, meta)
     log(json.dumps({"saved": args.out, **chosen}))
