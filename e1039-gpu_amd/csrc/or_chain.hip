@@ -1356,19 +1356,33 @@ STAGE int stage_xz(or_params P, Scr S) {
         wsync();
         const uint32_t total = run;
         int np = 0, nm = 0;
+        int cb = 0;   // the cell of combination g0 (the same for every lane)
         for (uint32_t g0 = 0; g0 < total; g0 += WAVE) {
             const uint32_t g = g0 + lane;
             bool acc = false;
             int par = 0, t = 0, s3 = 0;
             uint16_t q2 = 0, q3 = 0;
+            // lane t holds the end of cell cb + t; the cells ending at or before a
+            // combination are a prefix of the lanes (cell offsets never decrease)
+            const uint32_t ce = cb + 1 + lane <= 1568 ? celloff[cb + 1 + lane] : 0xFFFFFFFFu;
+            const bool inwin = (uint32_t)__builtin_amdgcn_readlane((int)ce, WAVE - 1) > g0 + WAVE - 1;
+            int tu = 0;
+            if (inwin) {
+#pragma unroll
+                for (int step = WAVE / 2; step > 0; step >>= 1)
+                    if ((uint32_t)__shfl((int)ce, tu + step - 1, WAVE) <= g) tu += step;
+            }
             if (g < total) {
-                int lo = 0, hi = 1568;  // largest c with celloff[c] <= g
-                while (hi - lo > 1) {
-                    const int mid = (lo + hi) >> 1;
-                    if (celloff[mid] <= g) lo = mid;
-                    else hi = mid;
+                int c = cb + tu;
+                if (!inwin) {
+                    int lo = 0, hi = 1568;  // largest c with celloff[c] <= g
+                    while (hi - lo > 1) {
+                        const int mid = (lo + hi) >> 1;
+                        if (celloff[mid] <= g) lo = mid;
+                        else hi = mid;
+                    }
+                    c = lo;
                 }
-                const int c = lo;
                 t = c / 49;
                 const int i = c % 49;
                 s3 = 1 + (t & 1);
@@ -1401,6 +1415,19 @@ STAGE int stage_xz(or_params P, Scr S) {
             ntkl += __popcll(m);
             np += __popcll(mp);
             nm += __popcll(mm);
+            // the cell of the next wave's first combination
+            const unsigned long long adv = ballot(ce <= g0 + WAVE);
+            if (adv != ~0ull) {
+                cb += __popcll(adv);
+            } else {
+                int lo = cb, hi = 1568;
+                while (hi - lo > 1) {
+                    const int mid = (lo + hi) >> 1;
+                    if (celloff[mid] <= g0 + WAVE) lo = mid;
+                    else hi = mid;
+                }
+                cb = lo;
+            }
         }
         if (lane == 0) {
             if (np > 16 * P.max_tracklets_per_slot || nm > 16 * P.max_tracklets_per_slot ||
