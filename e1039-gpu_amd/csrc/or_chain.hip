@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 #include <climits>
 #include <cmath>
@@ -189,6 +190,10 @@ struct YzWave {
     int umax[YZ_CHUNK];       // pruning: best unit score of each tracklet of the chunk (order-preserving int)
     unsigned long long best[YZ_CHUNK];
     uint32_t hx[YZ_CHUNK];    // hodoscope x-match mask of each tracklet of the chunk (yz_cuts_hx)
+    // x hits of each tracklet of the chunk by fixed slot D2 X, X', D3 X, X' (raw
+    // index, NOHIT: none) and their signs (yz_chi2_st)
+    uint16_t xh[YZ_CHUNK][4];
+    int8_t xs[YZ_CHUNK][4];
     // 1/d rounded to double of the enumeration radices (see qdiv): u2 list
     // length, (u2, v2) pairs per unit, u3 list length
     double rx[YZ_CHUNK], rin[YZ_CHUNK], rz[YZ_CHUNK];
@@ -1030,7 +1035,8 @@ struct Scr {
     __device__ uint16_t* slot_order() const { return reinterpret_cast<uint16_t*>(cfg().rp[R_SO]); }
     // YZ candidates passing the first box cut, in reference order
     __device__ uint32_t* plist() const { return reinterpret_cast<uint32_t*>(cfg().rp[R_PL]); }
-    // per candidate: x first-of-unit, y station-3 signs after its left/right (first
+    // per candidate: x first-of-unit (bit 0), passed the sign-free tests (bit 1),
+    // station 3 is D3m (bit 2); y station-3 signs after its left/right (first
     // of unit), z sign written to the last station-3 hit (0: none), w station-3
     // signs it starts from
     __device__ uchar4* pst() const { return reinterpret_cast<uchar4*>(cfg().rp[R_PB]); }
@@ -2227,7 +2233,7 @@ __device__ __attribute__((noinline)) int yz_pass2(Scr S, int kb, int np) {
             // the tests that need no sign (second box cut, prop-tube Y,
             // hodoscopes) depend on the first fit and the tracklet only
             const bool go = yz_cuts_hx(E, Y.hx[it.j], f);
-            uchar4 ps = make_uchar4((first ? 1 : 0) | (go ? 2 : 0), 0, 0, 0);
+            uchar4 ps = make_uchar4((first ? 1 : 0) | (go ? 2 : 0) | (Tp->st3 == 4 ? 4 : 0), 0, 0, 0);
             if (!first && ((lr_slot_starts<8>(H.pres, H.det) >> 3) & 1u)) {
                 const int h = lr_pair_decide(Tp->x0, Tp->tx, f.p0, f.p1, Tp->ex0, Tp->etx, f.e0, f.e1, H.det[3],
                                              H.det[4], H.pos[3], H.pos[4], H.drift[3], H.drift[4], 150.f);
@@ -2274,12 +2280,15 @@ __device__ __attribute__((noinline)) void yz_pass2_firsts(Scr S, int kb, int nf)
 // writer of the v3' sign before it); the survivors that passed the sign-free
 // tests are listed.  Returns their number.
 template <int MAXH, int WPE>
-__device__ __attribute__((noinline)) int yz_pass3_scan(Scr S, int np) {
+__device__ __attribute__((noinline)) int2 yz_pass3_scan(Scr S, int np) {
     const int lane = lane_id();
     // pass 3: starting signs by scan (the unit's seed, and the last writer of
     // the v3' sign before this candidate); the candidates that passed the
-    // sign-free tests in pass 2 are listed and run the rest (left/right, chi2)
-    int carry_f = -1, carry_w = -1, nq = 0;
+    // sign-free tests in pass 2 are listed and run the rest (left/right, chi2):
+    // those of D3p tracklets from the front of the list, those of D3m ones from
+    // its back, so that each part has one set of planes
+    int carry_f = -1, carry_w = -1, nq = 0, nq4 = 0;
+    const int cap = cfg().list_cap;
     for (int k0 = 0; k0 < np; k0 += WAVE) {
         const int k = k0 + lane;
         const bool valid = k < np;
@@ -2301,44 +2310,86 @@ __device__ __attribute__((noinline)) int yz_pass3_scan(Scr S, int np) {
             ps.w = (uint8_t)((S.pst()[fi].y & 0x3Fu) | ((unsigned)S.pst()[wexcl].z << 6));
             S.pst()[k] = ps;
         }
-        const unsigned long long m = ballot(go);
-        if (go) S.qlist()[nq + rank_in(m)] = (uint32_t)k;
+        const bool d3m = (ps.x & 4u) != 0;
+        const unsigned long long m = ballot(go && !d3m), m4 = ballot(go && d3m);
+        if (go && !d3m) S.qlist()[nq + rank_in(m)] = (uint32_t)k;
+        if (go && d3m) S.qlist()[cap - 1 - (nq4 + rank_in(m4))] = (uint32_t)k;
         nq += __popcll(m);
+        nq4 += __popcll(m4);
     }
-    return nq;
+    return make_int2(nq, nq4);
+}
+
+// chi2 of a YZ candidate of a tracklet whose station 3 is ST3 (yz_chi2 without
+// the write): the planes of the x hits (D2 X, X', D3 X, X', absent ones left
+// out, the order xz_candidate stores them in) and of the UV slots are then
+// wave-uniform, so their constants come through scalar loads; the sum runs in
+// the same order
+template <int ST3, int MAXH, int WPE>
+__device__ __forceinline__ float yz_chi2_st(const Ev<MAXH, WPE>& E, const Tk* Tp, int j, const YzSlots& H,
+                                            const Fit& f) {
+    const YzWave& Y = yzw<MAXH>();
+    const float x0 = Tp->x0, tx = Tp->tx;
+    const float y0 = f.p0, ty = f.p1;
+    const int d3x = ST3 == 3 ? 22 : 28;   // as xz_candidate sets it
+    const int xp[4] = {16, 15, d3x, d3x - 1};
+    float chi2 = 0.f;
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        const int d = xp[s], r = Y.xh[j][s];
+        if (r != NOHIT) {
+            const float res = G(F_RESOLUTION, d);
+            const float rs = E.Lp->drift[r] * (float)Y.xs[j][s] - dca_wire(d, E.Lp->elem[r], x0, y0, tx, ty);
+            chi2 += ((rs * rs) / res) / res;
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+        if (!((H.pres >> s) & 1u)) continue;
+        const int d = H.det[s];
+        const float res = G(F_RESOLUTION, d);
+        const float rs = H.drift[s] * (float)H.sign[s] - dca_wire(d, E.Lp->elem[H.raw[s]], x0, y0, tx, ty);
+        chi2 += ((rs * rs) / res) / res;
+    }
+    return chi2;
 }
 
 // Pass 3 over the listed survivors: left/right from the starting signs, chi2,
-// and the tracklet's best key (64-bit LDS atomicMin of chi2 bits and index).
+// and the tracklet's best key (64-bit LDS atomicMin of chi2 bits and index);
+// the D3p part of the list (nq from its front), then the D3m part (nq4 from its back)
 template <int MAXH, int WPE>
-__device__ __attribute__((noinline)) void yz_pass3(Scr S, int kb, int nq) {
-    Lds<MAXH>& L = g_lds<MAXH>;
+__device__ __attribute__((noinline)) void yz_pass3(Scr S, int kb, int nq, int nq4) {
     YzWave& Y = yzw<MAXH>();
     Ev<MAXH, WPE> E;
     const int lane = lane_id();
     Tk* tks = S.tks();
+    const int cap = cfg().list_cap;
+    auto eval = [&](auto st3c, int k) {
+        constexpr int ST3 = decltype(st3c)::value;
+        const YzItem it = yz_decode<MAXH, WPE>((int)S.plist()[k]);
+        const uchar4 ps = S.pst()[k];
+        const Tk* Tp = &tks[kb + it.j];
+        YzSlots H;
+        Fit f;
+        yz_cache_get(E, S.ycache() + 3 * k, ST3, H, f);
+        if (!(ps.x & 1u)) yz_unpack3(H, ps.w);
+        lr_slots_new<8>(Tp->x0, Tp->tx, f.p0, f.p1, Tp->ex0, Tp->etx, f.e0, f.e1, H.pres, H.det, H.pos,
+                        H.drift, H.sign, 150.f);
+        const float chi2 = yz_chi2_st<ST3>(E, Tp, it.j, H, f);
+        // the reference starts its minimum at 10000.1 (reconstruction_kernels.cuh:1240)
+        if (chi2 < 10000.1f) {
+            const unsigned long long key =
+                ((unsigned long long)__float_as_uint(chi2) << 32) | (unsigned long long)(uint32_t)it.local;
+            atomicMin(&Y.best[it.j], key);
+        }
+    };
     for (int q0 = 0; q0 < nq; q0 += WAVE) {
         const int q = q0 + lane;
-        if (q < nq) {
-            const int k = (int)S.qlist()[q];
-            const YzItem it = yz_decode<MAXH, WPE>((int)S.plist()[k]);
-            const uchar4 ps = S.pst()[k];
-            const Tk* Tp = &tks[kb + it.j];
-            YzSlots H;
-            Fit f;
-            yz_cache_get(E, S.ycache() + 3 * k, Tp->st3, H, f);
-            if (!(ps.x & 1u)) yz_unpack3(H, ps.w);
-            lr_slots_new<8>(Tp->x0, Tp->tx, f.p0, f.p1, Tp->ex0, Tp->etx, f.e0, f.e1, H.pres, H.det, H.pos,
-                            H.drift, H.sign, 150.f);
-            float chi2;
-            yz_chi2(E, Tp, H, f, chi2, nullptr);
-            // the reference starts its minimum at 10000.1 (reconstruction_kernels.cuh:1240)
-            if (chi2 < 10000.1f) {
-                const unsigned long long key =
-                    ((unsigned long long)__float_as_uint(chi2) << 32) | (unsigned long long)(uint32_t)it.local;
-                atomicMin(&Y.best[it.j], key);
-            }
-        }
+        if (q < nq) eval(std::integral_constant<int, 3>{}, (int)S.qlist()[q]);
+    }
+    for (int q0 = 0; q0 < nq4; q0 += WAVE) {
+        const int q = q0 + lane;
+        if (q < nq4) eval(std::integral_constant<int, 4>{}, (int)S.qlist()[cap - 1 - q]);
     }
     wsync();
 }
@@ -2416,11 +2467,11 @@ __device__ __attribute__((noinline)) bool yz_chunk_par(Scr S, Prn R, int kb, int
         OR_SUB(2);
         // pass 3: starting signs by scan; the candidates that passed the
         // sign-free tests in pass 2 run the rest (left/right, chi2)
-        const int nq = yz_pass3_scan<MAXH, WPE>(S, np);
+        const int2 nq = yz_pass3_scan<MAXH, WPE>(S, np);
         wsync();
         OR_SUB(15);
-        OR_COUNT(16, nq);
-        yz_pass3<MAXH, WPE>(S, kb, nq);
+        OR_COUNT(16, nq.x + nq.y);
+        yz_pass3<MAXH, WPE>(S, kb, nq.x, nq.y);
         wsync();
         // a tracklet whose best improved in this batch keeps that candidate's
         // cache entry and sign state, since the next batch reuses the lists
@@ -2627,6 +2678,18 @@ STAGE void stage_yz(or_params P, Scr S, Prn R, int ntkl, int list_cap) {
             hb |= (unsigned)__shfl_xor((int)hb, 1, WAVE);
             hb |= (unsigned)__shfl_xor((int)hb, 2, WAVE);
             if (v == 0) Y.hx[j] = hb;
+            // x hit of slot v (D2 X, X', D3 X, X'), as xz_candidate stores them in order
+            if (k < ntkl) {
+                const Tk& T = tks[k];
+                const int d3x = T.st3 == 3 ? 22 : 28;
+                const int dv = v == 0 ? 16 : v == 1 ? 15 : v == 2 ? d3x : d3x - 1;
+                uint16_t xr = NOHIT;
+                int8_t xsg = 0;
+                for (int h = 0; h < T.nx; ++h)
+                    if (T.det[h] == dv) { xr = T.hraw[h]; xsg = T.sign[h]; }
+                Y.xh[j][v] = xr;
+                Y.xs[j][v] = xsg;
+            }
             // enumeration bound: u2 x v2 x u3 x v3 list entries of one tracklet
             raw *= (unsigned long long)__shfl_xor((long long)raw, 1, WAVE);
             raw *= (unsigned long long)__shfl_xor((long long)raw, 2, WAVE);
