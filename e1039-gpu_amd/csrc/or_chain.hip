@@ -254,7 +254,7 @@ enum ScrRegion {
     R_TK,     // Tk[maxtkl]: the event's tracklets
     R_CELL,   // uint32[1569]: XZ cell offsets
     R_LIST,   // uint16[64][win_cap]: window pair lists (YZ: tracklet x view; station 1: lane)
-    R_YV,     // float4[64][win_cap]: fit terms (w y, w y z) of YZ list entries' hits a, b; station-1 records
+    R_YV,     // float4[64][win_cap]: per YZ list entry (w y of hit a, w y of hit b, flag byte, 0); station-1 records
     R_YY,     // float2[64][win_cap]: y of YZ list entries' hits a, b (station 2: the entry's mean y, 0)
     R_YF,     // uint8[64][win_cap]: accepted-hit flags of YZ list entries | first-fit pattern term << 2
     R_XP,     // uint16[3][28][bin_cap]: binned X pairs
@@ -800,7 +800,7 @@ __device__ __forceinline__ uint4 gld4(const uint4* p) {
 
 struct YzLists {
     const uint16_t* l[4];   // pair entries (a | b << 8), views u2, v2, u3, v3
-    const float4* y[4];     // (w y, w y z) of hit a, then of hit b
+    const float4* y[4];     // (w y of hit a, w y of hit b, flag byte bits, 0)
     const uint8_t* f[4];    // bit 0: hit a present and accepted, bit 1: hit b; bits 2-7: pattern term (yz_pattern)
     int n[4];
 };
@@ -819,11 +819,11 @@ __device__ __forceinline__ int yz_sum(const YzLists& W, int v, int k, int da, in
     int c = 0;
     if (fl & 1) {
         const float w = yz_w(da), z = G(F_Z, da);
-        B0 += yv.x; B1 += yv.y; A0 += w; A1 += w * z; A3 += (w * z) * z; ++c;
+        B0 += yv.x; B1 += yv.x * z; A0 += w; A1 += w * z; A3 += (w * z) * z; ++c;
     }
     if (fl & 2) {
         const float w = yz_w(db), z = G(F_Z, db);
-        B0 += yv.z; B1 += yv.w; A0 += w; A1 += w * z; A3 += (w * z) * z; ++c;
+        B0 += yv.y; B1 += yv.y * z; A0 += w; A1 += w * z; A3 += (w * z) * z; ++c;
     }
     return c;
 }
@@ -1023,8 +1023,8 @@ struct Scr {
     __device__ Tk* tks() const { return reinterpret_cast<Tk*>(cfg().rp[R_TK]); }
     __device__ uint32_t* celloff() const { return reinterpret_cast<uint32_t*>(cfg().rp[R_CELL]); }
     __device__ uint16_t* wl() const { return reinterpret_cast<uint16_t*>(cfg().rp[R_LIST]); }
-    // per YZ list entry: (w y, w y z) of hit a, then of hit b, w the plane's
-    // weight and z its position; the y values themselves in wyy
+    // per YZ list entry: w y of hit a and of hit b (w the plane's weight), its
+    // flag byte (as wf holds it) in the bits of z, 0; the y values themselves in wyy
     __device__ float4* wy() const { return reinterpret_cast<float4*>(cfg().rp[R_YV]); }
     // (station 2: x = the entry's y, the mean of its accepted hits)
     __device__ float2* wyy() const { return reinterpret_cast<float2*>(cfg().rp[R_YY]); }
@@ -1595,21 +1595,31 @@ __device__ __forceinline__ Fit yz_fit_tab(int m, int pattern, float B0, float B1
     return f;
 }
 
-// B sums of an entry's present hits, a then b
-__device__ __forceinline__ void yz_bsum(unsigned fl, const float4& t, float& B0, float& B1) {
-    if (fl & 1) { B0 += t.x; B1 += t.y; }
-    if (fl & 2) { B0 += t.z; B1 += t.w; }
+// B sums of an entry's present hits, a then b: w y, and (w y) z with z the
+// plane's (za for hit a, zb for hit b), the products the fit sums
+__device__ __forceinline__ void yz_bsum(unsigned fl, const float4& t, float za, float zb, float& B0, float& B1) {
+    if (fl & 1) { B0 += t.x; B1 += t.x * za; }
+    if (fl & 2) { B0 += t.y; B1 += t.y * zb; }
+}
+
+// z of the planes of YZ view v (u2, v2, u3, v3) with station 3 st3: hit a, hit b
+__device__ __forceinline__ float2 yz_zab(int v, int st3) {
+    const int sid = v < 2 ? super_id(2, v + 1) : super_id(st3, v - 1);
+    return make_float2(G(F_Z, sid * 2), G(F_Z, sid * 2 - 1));
 }
 
 __device__ __forceinline__ bool yz_fit(const YzLists& W, int st3, const YzItem& it, Fit& f) {
-    const unsigned fu3 = gld(W.f[2] + it.iu3), fv3 = gld(W.f[3] + it.iv3);
-    const unsigned fu2 = gld(W.f[0] + it.iu2), fv2 = gld(W.f[1] + it.iv2);
+    const float4 tu3 = gld4(W.y[2] + it.iu3), tv3 = gld4(W.y[3] + it.iv3);
+    const float4 tu2 = gld4(W.y[0] + it.iu2), tv2 = gld4(W.y[1] + it.iv2);
+    const unsigned fu3 = __float_as_uint(tu3.z), fv3 = __float_as_uint(tv3.z);
+    const unsigned fu2 = __float_as_uint(tu2.z), fv2 = __float_as_uint(tv2.z);
     if (!fu3 || !fv3 || !fu2 || !fv2) return false;
     float B0 = 0.f, B1 = 0.f;
-    yz_bsum(fu3, gld4(W.y[2] + it.iu3), B0, B1);
-    yz_bsum(fv3, gld4(W.y[3] + it.iv3), B0, B1);
-    yz_bsum(fu2, gld4(W.y[0] + it.iu2), B0, B1);
-    yz_bsum(fv2, gld4(W.y[1] + it.iv2), B0, B1);
+    const float2 z0 = yz_zab(0, st3), z1 = yz_zab(1, st3), z2 = yz_zab(2, st3), z3 = yz_zab(3, st3);
+    yz_bsum(fu3, tu3, z2.x, z2.y, B0, B1);
+    yz_bsum(fv3, tv3, z3.x, z3.y, B0, B1);
+    yz_bsum(fu2, tu2, z0.x, z0.y, B0, B1);
+    yz_bsum(fv2, tv2, z1.x, z1.y, B0, B1);
     f = yz_fit_tab(st3 == 4 ? 1 : 0, yz_pattern_b(fu2, fv2, fu3, fv3), B0, B1);
     return true;
 }
@@ -1800,30 +1810,41 @@ __device__ __attribute__((noinline)) UnitFeatures unit_features_at(Scr S, int kb
     return f;
 }
 
+// plane z of the YZ views: [u2, v2, u3, v3][station 3 is D3m][hit a, b]
+__device__ __forceinline__ void yz_plane_z(float (&zv)[4][2][2]) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+#pragma unroll
+        for (int m = 0; m < 2; ++m) {
+            const float2 z = yz_zab(v, 3 + m);
+            zv[v][m][0] = z.x;
+            zv[v][m][1] = z.y;
+        }
+}
+
 // Pass 1 splits it in two: the loads of a candidate (YzFirst) are issued one
 // wave iteration ahead of its evaluation.
 struct YzFirst {
-    unsigned fl[4];
-    float4 t[4];
+    float4 t[4];         // the four list entries (flag bytes in .z)
     unsigned m;
     int ustart, index;   // first candidate of its unit, its own index (-1: no candidate)
 };
 __device__ __forceinline__ void yz_first_load(const Scr& S, int j, const int (&e)[4], YzFirst& c) {
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-        const int q = (j * 4 + v) * cfg().win_cap + e[v];
-        c.fl[v] = gld(S.wf() + q);
-        c.t[v] = gld4(S.wy() + q);
-    }
+    for (int v = 0; v < 4; ++v) c.t[v] = gld4(S.wy() + (j * 4 + v) * cfg().win_cap + e[v]);
 }
-__device__ __forceinline__ bool yz_first_eval(const YzFirst& c) {
-    if (c.index < 0 || !c.fl[0] || !c.fl[1] || !c.fl[2] || !c.fl[3]) return false;
+// zv: plane z of the views by station 3 (yz_plane_z)
+__device__ __forceinline__ bool yz_first_eval(const YzFirst& c, const float (&zv)[4][2][2]) {
+    const unsigned fl[4] = {__float_as_uint(c.t[0].z), __float_as_uint(c.t[1].z), __float_as_uint(c.t[2].z),
+                            __float_as_uint(c.t[3].z)};
+    if (c.index < 0 || !fl[0] || !fl[1] || !fl[2] || !fl[3]) return false;
+    const int m = (int)c.m;
     float B0 = 0.f, B1 = 0.f;
-    yz_bsum(c.fl[2], c.t[2], B0, B1);
-    yz_bsum(c.fl[3], c.t[3], B0, B1);
-    yz_bsum(c.fl[0], c.t[0], B0, B1);
-    yz_bsum(c.fl[1], c.t[1], B0, B1);
-    return yz_box1(yz_fit_tab((int)c.m, yz_pattern_b(c.fl[0], c.fl[1], c.fl[2], c.fl[3]), B0, B1));
+    yz_bsum(fl[2], c.t[2], m ? zv[2][1][0] : zv[2][0][0], m ? zv[2][1][1] : zv[2][0][1], B0, B1);
+    yz_bsum(fl[3], c.t[3], m ? zv[3][1][0] : zv[3][0][0], m ? zv[3][1][1] : zv[3][0][1], B0, B1);
+    yz_bsum(fl[0], c.t[0], zv[0][0][0], zv[0][0][1], B0, B1);
+    yz_bsum(fl[1], c.t[1], zv[1][0][0], zv[1][0][1], B0, B1);
+    return yz_box1(yz_fit_tab(m, yz_pattern_b(fl[0], fl[1], fl[2], fl[3]), B0, B1));
 }
 
 // Pass 1 streams in batches.  The survivors of the first box cut go into a
@@ -1976,6 +1997,8 @@ __device__ __attribute__((noinline)) int yz_pass1_kept(Scr S, int kb, int ntkl, 
     const int lane = lane_id();
     Tk* tks = S.tks();
     const unsigned d3m = (unsigned)ballot(lane < YZ_CHUNK && kb + lane < ntkl && tks[kb + lane].st3 == 4);
+    float zv[4][2][2];
+    yz_plane_z(zv);
     const uint32_t start = cursor;
     int np = 0;
     int lo0 = lo_unit;   // last kept unit starting at or before i0 (the same for every lane)
@@ -2030,7 +2053,7 @@ __device__ __attribute__((noinline)) int yz_pass1_kept(Scr S, int kb, int ntkl, 
         }
     };
     auto take = [&](uint32_t i0, const YzFirst& cur, uint32_t st_t, int ku) -> int {
-        const bool pass = yz_first_eval(cur);
+        const bool pass = yz_first_eval(cur, zv);
         const int iorig = cur.index, ustart = cur.ustart;
         const unsigned long long m = ballot(pass);
         const int c = __popcll(m);
@@ -2088,6 +2111,8 @@ __device__ __attribute__((noinline)) int yz_pass1_plain(Scr S, int kb, int ntkl,
 #pragma unroll
     for (int t = 0; t <= YZ_CHUNK; ++t) off[t] = __builtin_amdgcn_readfirstlane(Y.uoff[t]);
     const unsigned d3m = (unsigned)ballot(lane < YZ_CHUNK && kb + lane < ntkl && tks[kb + lane].st3 == 4);
+    float zv[4][2][2];
+    yz_plane_z(zv);
     const int start = cursor;
     int np = 0;
     // candidate i0 + lane: tracklet, list entries and unit, and its loads
@@ -2128,7 +2153,7 @@ __device__ __attribute__((noinline)) int yz_pass1_plain(Scr S, int kb, int ntkl,
     // or -1 when that unit is the batch's first); INT_MIN: go on
     auto take = [&](int i0, const YzFirst& cur) -> int {
         const int i = i0 + lane;
-        const bool pass = yz_first_eval(cur);
+        const bool pass = yz_first_eval(cur, zv);
         const int ustart = cur.ustart;
         const unsigned long long m = ballot(pass);
         const int c = __popcll(m);
@@ -2654,13 +2679,14 @@ STAGE void stage_yz(or_params P, Scr S, Prn R, int ntkl, int list_cap) {
                         const int o = (j * 4 + v) * cfg().win_cap + nacc;
                         lst[nacc] = lst[q];
                         // fit terms w y and (w y) z of each hit, as the fit sums them
-                        wy[o] = make_float4(yv.y * yv.x, (yv.y * yv.x) * za, yv.w * yv.z, (yv.w * yv.z) * zb);
+                        const uint8_t fb = (uint8_t)(fl | ((fl - 1u) * (v == 0 ? 3u : v == 1 ? 1u : v == 2 ? 27u : 9u) << 2));
+                        wy[o] = make_float4(yv.y * yv.x, yv.w * yv.z, __uint_as_float(fb), 0.f);
                         // station 3: the y of hits a and b; station 2: the entry's y
                         // (the mean of its accepted hits) for the pruner's features
                         wyy[o] = v >= 2 ? make_float2(yv.x, yv.z)
                                         : make_float2((fl & 1u) ? ((fl & 2u) ? 0.5f * (yv.x + yv.z) : yv.x) : yv.z, 0.f);
                         // the entry's term of the first-fit pattern index (yz_pattern)
-                        wf[o] = (uint8_t)(fl | ((fl - 1u) * (v == 0 ? 3u : v == 1 ? 1u : v == 2 ? 27u : 9u) << 2));
+                        wf[o] = fb;
                         ++nacc;
                     }
                 }
