@@ -1960,6 +1960,7 @@ __device__ __attribute__((noinline)) UnitScore yz_score_units(Scr S, Prn R, int 
             const int r = nk + rank_in(km);
             S.klist()[r] = ((uint32_t)j << 16) | (uint32_t)iuv3;
             S.kstart()[r] = run + (uint32_t)(inc - c);
+            if (res.dbase >= 0) R.dump[(res.dbase + u) * OR_UNIT_ROW + 3] = 0.5f;   // searched
         }
         nk += __popcll(km);
         run += (uint32_t)__shfl(inc, WAVE - 1, WAVE);

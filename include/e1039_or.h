@@ -421,8 +421,10 @@ or_status or_pruner_layers(or_session* s, const float* features, int64_t n, floa
 
 /* Training aid: during device-resident calls, one row of OR_UNIT_ROW floats
  * per scored unit is appended to rows (device memory, cap rows): event index,
- * tracklet_id, u3.v3 index, winner flag (1: the tracklet's YZ winner is in this
- * unit, 0: not, -1: unknown), then the 16 features.  A unit's station-3 y is
+ * tracklet_id, u3.v3 index (over the window lists' entries with an accepted
+ * hit, u3 fastest), unit flag (1: the tracklet's YZ winner is in this unit,
+ * 0.5: searched, not the winner, 0: pruned, -1: unknown, the chunk went unit
+ * by unit and every unit of it was searched), then the 16 features.  A unit's station-3 y is
  * the mean of the y of its accepted U3 and V3 hits (sign 0); the line through
  * it and the beam axis at z = 0 predicts later hits (distances in 10 cm):
  * 10 tx; y3/50; (yU3 - yV3)/10 and |yU3 - yV3|/10; (yU3a - yU3b)/10 and

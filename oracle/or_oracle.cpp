@@ -660,6 +660,17 @@ static int yzcheck_cross(const short* det, int n, int n3) {  /* i < n3 paired wi
     return -1;
 }
 
+/* Unit filter (test aid, no reference counterpart): the YZ search of a tracklet
+ * is restricted to the (u3, v3) units listed for it, as a pruned search leaves
+ * them; a tracklet with no listed unit searches all of them.  Keys, sorted:
+ * event << 36 | tracklet_id << 20 | unit, where unit = ku3 + kv3 * nku3 counts
+ * only the U3 and V3 window-list entries with an accepted hit (the enumeration
+ * a pruned search numbers its units by). */
+static const int64_t* g_unit_keys = nullptr;
+static int64_t g_unit_nkeys = 0;
+static int64_t unit_key(int64_t ev, int canon, int unit) { return (ev << 36) | ((int64_t)canon << 20) | unit; }
+static bool unit_listed(int64_t key) { return std::binary_search(g_unit_keys, g_unit_keys + g_unit_nkeys, key); }
+
 static bool run_yz_one(Ctx& C, Tkl& T) {
     const Geo& g = *C.g;
     float* R = T.rec;
@@ -725,8 +736,40 @@ static bool run_yz_one(Ctx& C, Tkl& T) {
     int best_orig[8];
     int win_i[4] = {-1, -1, -1, -1};
     const int P1a = 48, P1b = 47, P2a = 54, P2b = 53;
+    /* unit filter: index of each U3 / V3 entry among those with an accepted hit */
+    bool filtered = false;
+    std::vector<int> ku3, kv3;
+    int nku3 = 0;
+    if (g_unit_keys) {
+        const int64_t k0 = unit_key(C.ev, T.canon, 0);
+        const int64_t* it = std::lower_bound(g_unit_keys, g_unit_keys + g_unit_nkeys, k0);
+        filtered = it != g_unit_keys + g_unit_nkeys && (*it >> 20) == (k0 >> 20);
+        if (filtered) {
+            auto compact = [&](int v, int n, std::vector<int>& k) {
+                k.assign(n, -1);
+                int c = 0;
+                for (int i = 0; i < n; ++i) {
+                    bool acc = false;
+                    float y, ey;
+                    for (int ab = 0; ab < 2; ++ab) {
+                        const int idx = ab ? lists[v][i].b : lists[v][i].a;
+                        if (idx < 0) continue;
+                        const Plane& h = C.pl[vdet[v][ab]];
+                        acc |= y_uvhit(g, vdet[v][ab], (int)h.chan[idx], h.drift[idx], 0, x0, tx, y, ey);
+                    }
+                    if (acc) k[i] = c++;
+                }
+                return c;
+            };
+            nku3 = compact(2, nu3, ku3);
+            compact(3, nv3, kv3);
+        }
+    }
     for (int iuv3 = 0; iuv3 < nu3 * nv3; ++iuv3) {
         int iu3 = iuv3 % nu3, iv3 = iuv3 / nu3;
+        if (filtered && ku3[iu3] >= 0 && kv3[iv3] >= 0 &&
+            !unit_listed(unit_key(C.ev, T.canon, ku3[iu3] + kv3[iv3] * nku3)))
+            continue;
         nuv = 0;
         if (lists[2][iu3].a >= 0) add(vdet[2][0], lists[2][iu3].a);
         if (lists[2][iu3].b >= 0) add(vdet[2][1], lists[2][iu3].b);
@@ -1440,6 +1483,13 @@ void orc_global_cleaning(int n, float* stid, const float* chi2, const float* nhi
     }
     orc::run_global_cleaning(C);
     for (int i = 0; i < n; ++i) stid[i] = C.tk[i].rec[0];
+}
+
+/* YZ unit filter for the following orc_reconstruct calls (keys sorted, see
+ * g_unit_keys; NULL turns it off).  The keys must outlive those calls. */
+void orc_set_unit_filter(const int64_t* keys, int64_t n) {
+    orc::g_unit_keys = n > 0 ? keys : nullptr;
+    orc::g_unit_nkeys = n > 0 ? n : 0;
 }
 
 void orc_yz_state_check(int enable, long long* counts /* 3, may be NULL */) {

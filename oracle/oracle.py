@@ -59,6 +59,7 @@ def load():
     lib.orc_trace_fetch.restype = C.c_longlong
     lib.orc_pairs_binned.argtypes = [C.c_int, C.c_int, vp, vp, C.c_int, vp, vp, C.c_int, vp, vp]
     lib.orc_yz_state_check.argtypes = [C.c_int, vp]
+    lib.orc_set_unit_filter.argtypes = [vp, C.c_int64]
     f32 = C.c_float
     lib.orc_prop_segment.argtypes = [vp, C.c_int, f32, vp, C.c_int, f32, vp, C.c_int, f32, vp, C.c_int, f32,
                                      f32, f32, f32, f32, f32, f32]
@@ -94,8 +95,21 @@ class OracleGeometry:
             self.handle = None
 
 
-def reconstruct(geo: OracleGeometry, params, ev, records: bool = False, threads: int = 0, vertex: bool = False):
-    """Runs the oracle chain; returns the same Result layout as the product."""
+def unit_keys(events, tracklet_ids, units) -> np.ndarray:
+    """Sorted keys of searched YZ units for reconstruct(units=...): per unit its
+    event index, tracklet_id and u3.v3 index over the window-list entries with
+    an accepted hit (the index of the device's unit rows)."""
+    k = (np.asarray(events, np.int64) << 36) | (np.asarray(tracklet_ids, np.int64) << 20) | np.asarray(units, np.int64)
+    return np.unique(k)
+
+
+def reconstruct(geo: OracleGeometry, params, ev, records: bool = False, threads: int = 0, vertex: bool = False,
+                units=None):
+    """Runs the oracle chain; returns the same Result layout as the product.
+
+    units: optional sorted keys (unit_keys) restricting the YZ search of each
+    listed tracklet to its listed units, as a pruned search leaves them;
+    tracklets without a listed unit search all of theirs."""
     import sys
     sys.path.insert(0, os.path.join(os.path.dirname(HERE), "e1039-gpu_amd"))
     import orchain as oc
@@ -114,7 +128,15 @@ def reconstruct(geo: OracleGeometry, params, ev, records: bool = False, threads:
                      None if flag is None else flag.ctypes.data, arrs[4].ctypes.data, 0)
     res = oc.OrResult(status.ctypes.data, keep.ctypes.data, ntr.ctypes.data, tracks.ctypes.data, ntkl.ctypes.data,
                       None if rec is None else rec.ctypes.data, None if vtx is None else vtx.ctypes.data, None, 0)
-    load().orc_reconstruct(geo.handle, C.byref(params), C.byref(hits), C.byref(res), int(threads))
+    lib = load()
+    keys = None if units is None else np.ascontiguousarray(units, np.int64)
+    if keys is not None:
+        lib.orc_set_unit_filter(keys.ctypes.data, int(keys.size))
+    try:
+        lib.orc_reconstruct(geo.handle, C.byref(params), C.byref(hits), C.byref(res), int(threads))
+    finally:
+        if keys is not None:
+            lib.orc_set_unit_filter(None, 0)
     return oc.Result(status, keep, ntr, tracks, ntkl, rec, vtx)
 
 

@@ -7,6 +7,9 @@
   oracle, bit for bit: enumerating kept units is the plain chain when nothing
   is pruned.
 * The unit rows (or_session_set_unit_dump): one winner row per output track.
+* The pruned chain is the plain chain over the units it searched: those units
+  (from its unit rows) fed to the oracle's unit filter give its outputs bit
+  for bit.
 * The shipped pruner keeps >= 99.5 % of the plain chain's tracks identical.
 """
 import os
@@ -137,7 +140,7 @@ def test_keep_all_units_is_the_plain_chain(oc, orc, geo, ogeo, psess):
     n = int(count.item())
     assert 0 < n <= rows.shape[0]
     r = rows[:n].cpu().numpy()
-    assert set(np.unique(r[:, 3])) <= {-1.0, 0.0, 1.0}
+    assert set(np.unique(r[:, 3])) <= {-1.0, 0.5, 1.0}   # every unit searched
     assert (r[:, 0] >= 0).all() and (r[:, 0] < ev.n_events).all()
     # every output track's tracklet has exactly one winner unit
     win = r[r[:, 3] == 1.0]
@@ -164,6 +167,47 @@ def test_margin_keeping_everything_is_the_plain_chain(oc, orc, geo, ogeo):
         sess.set_pruner_margin()
     ref = orc.reconstruct(ogeo, sess.params, ev, records=True)
     same_outputs(got, ref)
+    sess.close()
+
+
+def test_pruned_chain_is_the_plain_chain_over_its_searched_units(oc, orc, geo, ogeo):
+    # the shipped pruner on mean-1,200 events; tracklets without unit rows
+    # (chunks beyond the unit list) and chunks that went unit by unit (flag -1)
+    # searched every unit, and the oracle's filter leaves them unrestricted
+    import torch
+
+    import pruner
+    if not os.path.exists(oc.DEFAULT_PRUNER):
+        pytest.skip("no shipped pruner weights")
+    flat, thr, meta = pruner.load()
+    sess = oc.Session(geo, oc.high_occupancy_params(), device=0)
+    ev = oc.generate(geo, 120, seed=0xF17, mult_mode=1, mult_mean=1200.0, mult_cap=4000)
+    rows = torch.empty((ev.n_events * 60000, oc.UNIT_ROW), dtype=torch.float32, device="cuda")
+    count = torch.zeros(1, dtype=torch.int64, device="cuda")
+    sess.set_pruner_margin(meta["margin"])
+    sess.set_pruner(flat, thr)
+    sess.set_unit_dump(rows, count)
+    try:
+        got = sess.reconstruct(ev, records=True)
+    finally:
+        sess.set_unit_dump(None)
+        sess.set_pruner(None)
+        sess.set_pruner_margin()
+    n = int(count.item())
+    assert 0 < n <= rows.shape[0]
+    r = rows[:n].cpu().numpy()
+    del rows
+    flag = r[:, 3]
+    assert set(np.unique(flag)) <= {-1.0, 0.0, 0.5, 1.0}
+    assert (flag == 0.0).mean() > 0.5, "the pruner kept most units"
+    s = flag >= 0.5
+    keys = orc.unit_keys(r[s, 0].astype(np.int64), r[s, 1].astype(np.int64), r[s, 2].astype(np.int64))
+    ref = orc.reconstruct(ogeo, sess.params, ev, records=True, units=keys)
+    assert ref.n_tracks.sum() > 1000
+    same_outputs(got, ref)
+    # the pruning changed some tracklet's YZ winner (the comparison is not the plain chain's)
+    plain = orc.reconstruct(ogeo, sess.params, ev, records=True)
+    assert not np.array_equal(plain.records.view(np.uint32), ref.records.view(np.uint32))
     sess.close()
 
 
