@@ -255,7 +255,7 @@ enum ScrRegion {
     R_CELL,   // uint32[1569]: XZ cell offsets
     R_LIST,   // uint16[64][win_cap]: window pair lists (YZ: tracklet x view; station 1: lane)
     R_YV,     // float4[64][win_cap]: per YZ list entry (w y of hit a, w y of hit b, flag byte, 0); station-1 records
-    R_YY,     // float2[64][win_cap]: y of YZ list entries' hits a, b (station 2: the entry's mean y, 0)
+    R_YY,     // float2[64][win_cap] (+ 16): y of YZ list entries' hits a, b (station 2: the entry's mean y, 0)
     R_YF,     // uint8[64][win_cap]: accepted-hit flags of YZ list entries | first-fit pattern term << 2
     R_XP,     // uint16[3][28][bin_cap]: binned X pairs
     R_SO,     // uint16[maxtkl]: tracklet index by slot order
@@ -1668,6 +1668,39 @@ __device__ __forceinline__ void yz_candidate(const Ev<MAXH, WPE>& E, const Scr& 
     yz_fill<6>(E, W, 1, it.iv2, H);
 }
 
+template <int ST3, int MAXH, int WPE>
+__device__ __forceinline__ float yz_chi2_st(const Ev<MAXH, WPE>& E, const Tk* Tp, int j, const YzSlots& H,
+                                            const Fit& f);
+
+// The representative candidate of a unit (see unit_features), evaluated as the
+// chain evaluates a candidate (yz_candidate, first box cut, left/right, the
+// sign-free tests, chi2) for a tracklet whose station 3 is ST3: the candidate's
+// planes are then compile-time constants, so their constants come through
+// scalar loads.  rep_pass: 0 fails the first box cut, 1 a sign-free test, 2 passes.
+template <int ST3, int MAXH, int WPE>
+__device__ __forceinline__ void yz_rep_eval(const Ev<MAXH, WPE>& E, const Scr& S, const Tk& T, const YzItem& it,
+                                            float& chi, float& rep_y0, int& rep_pass) {
+    YzLists W;
+    yz_lists_of<MAXH, WPE>(S, it.j, W);
+    YzSlots H;
+    Fit fr;
+    yz_fit(W, ST3, it, fr);   // the unit's entries and the nearest ones all hold an accepted hit
+    yz_dets(H, ST3);
+    H.pres = 0;
+    yz_fill<0>(E, W, 2, it.iu3, H);
+    yz_fill<2>(E, W, 3, it.iv3, H);
+    yz_fill<4>(E, W, 0, it.iu2, H);
+    yz_fill<6>(E, W, 1, it.iv2, H);
+    rep_y0 = fr.p0;
+    if (yz_box1(fr)) {
+        rep_pass = 1;
+        lr_slots_new<8>(T.x0, T.tx, fr.p0, fr.p1, T.ex0, T.etx, fr.e0, fr.e1, H.pres, H.det, H.pos, H.drift, H.sign,
+                        150.f);
+        if (yz_cuts_hx(E, yzw<MAXH>().hx[it.j], fr)) rep_pass = 2;
+        chi = yz_chi2_st<ST3>(E, &T, it.j, H, fr);
+    }
+}
+
 // Features of YZ unit (tracklet j, u3.v3 index iuv3) for the pruner.  The
 // window lists hold accepted entries only; a view's y is the mean of its
 // entry's accepted hits.  With y3 the mean y of the unit's station-3 hits at
@@ -1698,28 +1731,42 @@ __device__ __forceinline__ UnitFeatures unit_features(const Scr& S, const Tk& T,
     const int st3 = T.st3;
     const float z3 = 0.5f * (G(F_Z, super_id(st3, 1) * 2) + G(F_Z, super_id(st3, 2) * 2));
     const float k3 = y3 / z3;   // predicted y per cm of z
+    // per station-2 view the list entry nearest to the predicted y (the first in
+    // list order at the least distance, if below 100); both lists are read in
+    // blocks of 16 entries, one round of loads per block (the lists' region is
+    // padded, and entries past a list's end are not compared)
     float d2[2], yb2[2];
     int k2[2];
+    {
+        const float yp[2] = {k3 * G(F_Z, super_id(2, 1) * 2), k3 * G(F_Z, super_id(2, 2) * 2)};
+        const int n[2] = {nl.x, nl.y};
+        const float* src[2] = {reinterpret_cast<const float*>(S.wyy() + (j * 4) * win),
+                               reinterpret_cast<const float*>(S.wyy() + (j * 4 + 1) * win)};
 #pragma unroll
-    for (int v = 0; v < 2; ++v) {
-        const float yp = k3 * G(F_Z, super_id(2, v + 1) * 2);
-        float dmin = 100.f, ybest = yp;
-        int kb = 0;
-        const int n = v == 0 ? nl.x : nl.y;
-        const int q0 = (j * 4 + v) * win;
-        for (int k0 = 0; k0 < n; k0 += 4) {
-            float ye[4];
-#pragma unroll
-            for (int t = 0; t < 4; ++t) ye[t] = gld(reinterpret_cast<const float*>(S.wyy() + q0 + min(k0 + t, n - 1)));
-#pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                if (k0 + t >= n) break;
-                const float y = ye[t];
-                const float d = fabsf(y - yp);
-                if (d < dmin) { dmin = d; ybest = y; kb = k0 + t; }
-            }
+        for (int v = 0; v < 2; ++v) {
+            d2[v] = 100.f;
+            yb2[v] = yp[v];
+            k2[v] = 0;
         }
-        d2[v] = dmin; yb2[v] = ybest; k2[v] = kb;
+        const int nmax = max(n[0], n[1]);
+        for (int k0 = 0; k0 < nmax; k0 += 16) {
+            float ye[2][16];
+#pragma unroll
+            for (int v = 0; v < 2; ++v)
+#pragma unroll
+                for (int t = 0; t < 16; ++t) ye[v][t] = gld(src[v] + 2 * (k0 + t));   // .x of the entry's float2
+#pragma unroll
+            for (int v = 0; v < 2; ++v)
+#pragma unroll
+                for (int t = 0; t < 16; ++t) {
+                    const float d = fabsf(ye[v][t] - yp[v]);
+                    if (k0 + t < n[v] && d < d2[v]) {
+                        d2[v] = d;
+                        yb2[v] = ye[v][t];
+                        k2[v] = k0 + t;
+                    }
+                }
+        }
     }
     // representative candidate: the unit's station-3 entries with the nearest U2
     // and V2 entries, evaluated as the chain evaluates a candidate: first fit,
@@ -1730,17 +1777,8 @@ __device__ __forceinline__ UnitFeatures unit_features(const Scr& S, const Tk& T,
     if (nl.x > 0 && nl.y > 0) {
         YzItem it;
         it.j = j; it.local = 0; it.iu3 = iu3; it.iv3 = iv3; it.iu2 = k2[0]; it.iv2 = k2[1];
-        YzSlots H;
-        Fit fr;
-        yz_candidate(E, S, &T, it, H, fr);
-        rep_y0 = fr.p0;
-        if (yz_box1(fr)) {
-            rep_pass = 1;
-            lr_slots_new<8>(T.x0, T.tx, fr.p0, fr.p1, T.ex0, T.etx, fr.e0, fr.e1, H.pres, H.det, H.pos, H.drift,
-                            H.sign, 150.f);
-            if (yz_cuts_hx(E, yzw<MAXH>().hx[j], fr)) rep_pass = 2;
-            yz_chi2(E, &T, H, fr, chi, nullptr);
-        }
+        if (st3 == 3) yz_rep_eval<3>(E, S, T, it, chi, rep_y0, rep_pass);
+        else yz_rep_eval<4>(E, S, T, it, chi, rep_y0, rep_pass);
     }
     float dp[2];
     const Lds<MAXH>& L = g_lds<MAXH>;
@@ -4585,7 +4623,7 @@ static ScrLayout make_layout(const or_params& p, int list_cap) {
     put(R_CELL, 1569 * 4);
     put(R_LIST, 64 * win * 2);
     put(R_YV, std::max<int64_t>(64 * win * 16, 8 * 16 + win * (int64_t)sizeof(GX)));   // station 1: best + x records
-    put(R_YY, 64 * win * 8);
+    put(R_YY, (64 * win + 16) * 8);   // 16 entries of padding: the unit features read lists in blocks of 16
     put(R_YF, 64 * win);
     put(R_XP, 3 * 28 * bin * 2);
     put(R_SO, maxtkl * 2);
