@@ -1672,6 +1672,22 @@ template <int ST3, int MAXH, int WPE>
 __device__ __forceinline__ float yz_chi2_st(const Ev<MAXH, WPE>& E, const Tk* Tp, int j, const YzSlots& H,
                                             const Fit& f);
 
+// yz_candidate for a tracklet whose station 3 is ST3 (plane constants of the
+// candidate's slots known at compile time)
+template <int ST3, int MAXH, int WPE>
+__device__ __forceinline__ void yz_candidate_st(const Ev<MAXH, WPE>& E, const Scr& S, const YzItem& it, YzSlots& H,
+                                                Fit& f) {
+    YzLists W;
+    yz_lists_of<MAXH, WPE>(S, it.j, W);
+    yz_fit(W, ST3, it, f);
+    yz_dets(H, ST3);
+    H.pres = 0;
+    yz_fill<0>(E, W, 2, it.iu3, H);
+    yz_fill<2>(E, W, 3, it.iv3, H);
+    yz_fill<4>(E, W, 0, it.iu2, H);
+    yz_fill<6>(E, W, 1, it.iv2, H);
+}
+
 // The representative candidate of a unit (see unit_features), evaluated as the
 // chain evaluates a candidate (yz_candidate, first box cut, left/right, the
 // sign-free tests, chi2) for a tracklet whose station 3 is ST3: the candidate's
@@ -1680,17 +1696,9 @@ __device__ __forceinline__ float yz_chi2_st(const Ev<MAXH, WPE>& E, const Tk* Tp
 template <int ST3, int MAXH, int WPE>
 __device__ __forceinline__ void yz_rep_eval(const Ev<MAXH, WPE>& E, const Scr& S, const Tk& T, const YzItem& it,
                                             float& chi, float& rep_y0, int& rep_pass) {
-    YzLists W;
-    yz_lists_of<MAXH, WPE>(S, it.j, W);
     YzSlots H;
     Fit fr;
-    yz_fit(W, ST3, it, fr);   // the unit's entries and the nearest ones all hold an accepted hit
-    yz_dets(H, ST3);
-    H.pres = 0;
-    yz_fill<0>(E, W, 2, it.iu3, H);
-    yz_fill<2>(E, W, 3, it.iv3, H);
-    yz_fill<4>(E, W, 0, it.iu2, H);
-    yz_fill<6>(E, W, 1, it.iv2, H);
+    yz_candidate_st<ST3>(E, S, it, H, fr);   // the unit's entries and the nearest ones all hold an accepted hit
     rep_y0 = fr.p0;
     if (yz_box1(fr)) {
         rep_pass = 1;
@@ -2267,6 +2275,27 @@ __device__ __forceinline__ void yz_cache_get(const Ev<MAXH, WPE>& E, const uint4
     f.e1 = __uint_as_float(d.x);
 }
 
+// One survivor of pass 2 (below) whose tracklet's station 3 is ST3: its
+// candidate cache entry, the tests that need no sign (second box cut, prop-tube
+// Y, hodoscopes: they depend on the first fit and the tracklet only) and, unless
+// it is the first of its unit, the v3' / u2 pair decision.  Returns its state.
+template <int ST3, int MAXH, int WPE>
+__device__ __forceinline__ uchar4 yz_pass2_one(const Ev<MAXH, WPE>& E, const Scr& S, const Tk* Tp, const YzItem& it,
+                                               int k, bool first) {
+    YzSlots H;
+    Fit f;
+    yz_candidate_st<ST3>(E, S, it, H, f);
+    yz_cache_put(S.ycache() + 3 * k, H, f);
+    const bool go = yz_cuts_hx(E, yzw<MAXH>().hx[it.j], f);
+    uchar4 ps = make_uchar4((first ? 1 : 0) | (go ? 2 : 0) | (ST3 == 4 ? 4 : 0), 0, 0, 0);
+    if (!first && ((lr_slot_starts<8>(H.pres, H.det) >> 3) & 1u)) {
+        const int h = lr_pair_decide(Tp->x0, Tp->tx, f.p0, f.p1, Tp->ex0, Tp->etx, f.e0, f.e1, H.det[3], H.det[4],
+                                     H.pos[3], H.pos[4], H.drift[3], H.drift[4], 150.f);
+        if (h >= 0) ps.z = (uint8_t)(lr_si(h) & 3);
+    }
+    return ps;
+}
+
 // Pass 2 over the survivors k < np of a batch: per survivor, first-of-unit
 // flag, the candidate cache entry, the sign-free tests and the v3' / u2 pair
 // decision.  Returns the number of first survivors, listed in the pass-3 list
@@ -2290,20 +2319,8 @@ __device__ __attribute__((noinline)) int yz_pass2(Scr S, int kb, int np) {
                 first = pt.j != it.j || pt.iu3 != it.iu3 || pt.iv3 != it.iv3;
             }
             const Tk* Tp = &tks[kb + it.j];
-            YzSlots H;
-            Fit f;
-            yz_candidate(E, S, Tp, it, H, f);
-            yz_cache_put(S.ycache() + 3 * k, H, f);
-            // the tests that need no sign (second box cut, prop-tube Y,
-            // hodoscopes) depend on the first fit and the tracklet only
-            const bool go = yz_cuts_hx(E, Y.hx[it.j], f);
-            uchar4 ps = make_uchar4((first ? 1 : 0) | (go ? 2 : 0) | (Tp->st3 == 4 ? 4 : 0), 0, 0, 0);
-            if (!first && ((lr_slot_starts<8>(H.pres, H.det) >> 3) & 1u)) {
-                const int h = lr_pair_decide(Tp->x0, Tp->tx, f.p0, f.p1, Tp->ex0, Tp->etx, f.e0, f.e1, H.det[3],
-                                             H.det[4], H.pos[3], H.pos[4], H.drift[3], H.drift[4], 150.f);
-                if (h >= 0) ps.z = (uint8_t)(lr_si(h) & 3);
-            }
-            S.pst()[k] = ps;
+            // per lane by its tracklet's station 3 (a wave's survivors mostly share one tracklet)
+            S.pst()[k] = Tp->st3 == 3 ? yz_pass2_one<3>(E, S, Tp, it, k, first) : yz_pass2_one<4>(E, S, Tp, it, k, first);
         }
         const unsigned long long fm = ballot(first);
         if (first) S.qlist()[nf + rank_in(fm)] = (uint32_t)k;
@@ -2327,15 +2344,20 @@ __device__ __attribute__((noinline)) void yz_pass2_firsts(Scr S, int kb, int nf)
             const int k = (int)S.qlist()[q];
             const YzItem it = yz_decode<MAXH, WPE>((int)S.plist()[k]);
             const Tk* Tp = &tks[kb + it.j];
-            YzSlots H;
-            Fit f;
-            yz_cache_get(E, S.ycache() + 3 * k, Tp->st3, H, f);
-            lr_slots_new<8>(Tp->x0, Tp->tx, f.p0, f.p1, Tp->ex0, Tp->etx, f.e0, f.e1, H.pres, H.det, H.pos,
-                            H.drift, H.sign, 150.f);
-            uchar4 ps = S.pst()[k];
-            ps.y = yz_pack3(H);
-            ps.z = (uint8_t)(H.sign[3] & 3);
-            S.pst()[k] = ps;
+            // per lane by its tracklet's station 3, as in pass 2
+            auto lr = [&](auto st3c) {
+                YzSlots H;
+                Fit f;
+                yz_cache_get(E, S.ycache() + 3 * k, decltype(st3c)::value, H, f);
+                lr_slots_new<8>(Tp->x0, Tp->tx, f.p0, f.p1, Tp->ex0, Tp->etx, f.e0, f.e1, H.pres, H.det, H.pos,
+                                H.drift, H.sign, 150.f);
+                uchar4 ps = S.pst()[k];
+                ps.y = yz_pack3(H);
+                ps.z = (uint8_t)(H.sign[3] & 3);
+                S.pst()[k] = ps;
+            };
+            if (Tp->st3 == 3) lr(std::integral_constant<int, 3>{});
+            else lr(std::integral_constant<int, 4>{});
         }
     }
 }
