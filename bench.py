@@ -19,9 +19,9 @@ speedup and the track recall (plain-chain tracks the pruned chain outputs
 identically) go beside it.  With --other (default) the other configurations
 are timed as well, each on a batch of its own (config.other_workloads):
 Poisson(300) (plain method, reference caps), the 0.8 Poisson(200) + 0.2
-Poisson(1500) mixture (batched form: graph replay, autotuned), and
-Poisson(2000) with 1-4 tracks and the full chain (plain and pruned, final-fit
-errors against float64).
+Poisson(1500) mixture (batched form: graph replay, autotuned; plain and
+pruned), and Poisson(2000) with 1-4 tracks and the full chain (plain and
+pruned, final-fit errors against float64).
 
 Prints one JSON line with the throughput, the HBM roofline of the chain
 kernels (algorithmic bytes per event: 12 per input hit, 1 bit per hit of keep
@@ -333,13 +333,15 @@ def main():
             entry["plain_kernel_ms"] = o_kms
             entry["rejected_events"] = int(((o_plain.status & oc.EV_REJECT_MASK) != 0).sum())
             entry["tracks"] = int(o_plain.n_tracks.sum())
-            if oname == "c4" and flat is not None:
+            if oname in ("c2", "c4") and flat is not None:
+                # the learned-pruned chain on the same batch (for c2 also autotuned and replayed as a graph)
                 set_pruned(osess, True)
                 p_ms, _, o_pr = timed(osess, f"{oname} pruned", oev, odev, 2, 1, vertex=vertex)
                 set_pruned(osess, False)
                 orec, ofound, ototal = pruner.recall(o_plain, o_pr)
                 entry.update({"pruned_events_per_s": oev.n_events / (p_ms * 1e-3), "speedup_vs_plain": o_ms / p_ms,
                               "track_recall": orec, "pruner": "trained on the Poisson(1200) configuration"})
+            if oname == "c4" and flat is not None:
                 osub = oev.subset(0, min(oev.n_events, 1000))
                 ofr = osess.reconstruct(osub, fit=True)
                 entry["final_fit_vs_fp64"] = fitcheck.error_report(osub, ofr, geo)

@@ -20,7 +20,7 @@ sys.path.insert(0, HERE)
 STAGES = ["load+reduce", "X pairs", "XZ", "YZ", "cleaning", "global", "outputs"]
 SUBS = ["YZ window lists", "YZ pass 1 (first fit)", "YZ pass 2 (seeds)", "YZ pass 3 (tails)", "YZ winners",
         "YZ unit path", "st1 window lists", "st1 pass A (y fits)", "st1 pass B + write",
-        "YZ unit scoring (in pass 1)", "YZ candidates (count)", "YZ units scored (count)",
+        "YZ unit scoring (before pass 1)", "YZ candidates (count)", "YZ units scored (count)",
         "YZ units kept (count)", "YZ candidates of kept units (count)", "YZ first-cut survivors (count)",
         "YZ pass 3 sign-free tests (in pass 3)", "YZ pass 3 survivors of them (count)"]
 
