@@ -323,6 +323,20 @@ __device__ __forceinline__ int lane_id() { return threadIdx.x & (WAVE - 1); }
 // wave of the workgroup (wave-uniform, in a scalar register)
 __device__ __forceinline__ int wave_idx() { return __builtin_amdgcn_readfirstlane((int)threadIdx.x) >> 6; }
 __device__ __forceinline__ WaveCfg& cfg() { return g_cfgw[wave_idx()]; }
+// A value the whole wave holds alike, as the wave's scalar: arguments of the
+// non-inlined stage functions are taken as per-lane by the compiler, so their
+// uniform ones (event and chunk bounds, counts, cursors, shared pointers) are
+// marked at entry; loops and index arithmetic on them then run on the scalar unit.
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ uint32_t uni(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
+__device__ __forceinline__ int64_t uni(int64_t v) {
+    const uint32_t lo = uni((uint32_t)(uint64_t)v), hi = uni((uint32_t)((uint64_t)v >> 32));
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+template <typename T>
+__device__ __forceinline__ T* uni(T* p) {
+    return reinterpret_cast<T*>((uintptr_t)uni((int64_t)(uintptr_t)p));
+}
 __device__ __forceinline__ int rank_in(unsigned long long m) {
     return __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
 }
@@ -1070,6 +1084,8 @@ struct Prn {
 // valid (staging and reduction caps passed); station cuts set status bits.
 template <int MAXH, int WPE>
 STAGE bool stage_load_reduce(const Args& a, int64_t h0, int n) {
+    h0 = uni(h0);
+    n = uni(n);
     Lds<MAXH>& L = g_lds<MAXH>;
     const int lane = lane_id();
     const or_params& P = a.p;
@@ -1254,6 +1270,8 @@ STAGE bool stage_load_reduce(const Args& a, int64_t h0, int n) {
 
 template <int MAXH, int WPE>
 STAGE void stage_keep_bits(const Args& a, int64_t h0, int n) {
+    h0 = uni(h0);
+    n = uni(n);
     Lds<MAXH>& L = g_lds<MAXH>;
     const int lane = lane_id();
     const int64_t w0 = h0 >> 5, w1 = (h0 + n - 1) >> 5;
@@ -1845,6 +1863,8 @@ __device__ __forceinline__ int unit_tracklet(int u) {
 // features of unit u of the chunk starting at tracklet kb (zeros when u >= U)
 template <int MAXH, int WPE>
 __device__ __attribute__((noinline)) UnitFeatures unit_features_at(Scr S, int kb, int u, int U) {
+    kb = uni(kb);
+    U = uni(U);
     UnitFeatures f;
 #pragma unroll
     for (int q = 0; q < PRUNE_NF; ++q) f.v[q] = 0.f;
@@ -1915,6 +1935,13 @@ struct UnitScore {
 
 template <int MAXH, int WPE>
 __device__ __attribute__((noinline)) UnitScore yz_score_units(Scr S, Prn R, int kb, int ntkl) {
+    kb = uni(kb);
+    ntkl = uni(ntkl);
+    R.prune = uni(R.prune);
+    R.dump = uni(R.dump);
+    R.dump_cap = uni(R.dump_cap);
+    R.dump_count = uni(R.dump_count);
+    R.ev = uni(R.ev);
     Lds<MAXH>& L = g_lds<MAXH>;
     YzWave& Y = yzw<MAXH>();
     const int lane = lane_id();
@@ -2039,6 +2066,11 @@ __device__ __forceinline__ int lower_bound_u32(const uint32_t* p, int n, uint32_
 template <int MAXH, int WPE>
 __device__ __attribute__((noinline)) int yz_pass1_kept(Scr S, int kb, int ntkl, int cap, int nk, uint32_t run,
                                                        uint32_t& cursor, int& lo_unit) {
+    kb = uni(kb);
+    ntkl = uni(ntkl);
+    cap = uni(cap);
+    nk = uni(nk);
+    run = uni(run);
     Lds<MAXH>& L = g_lds<MAXH>;
     YzWave& Y = yzw<MAXH>();
     const int lane = lane_id();
@@ -2046,9 +2078,9 @@ __device__ __attribute__((noinline)) int yz_pass1_kept(Scr S, int kb, int ntkl, 
     const unsigned d3m = (unsigned)ballot(lane < YZ_CHUNK && kb + lane < ntkl && tks[kb + lane].st3 == 4);
     float zv[4][2][2];
     yz_plane_z(zv);
-    const uint32_t start = cursor;
+    const uint32_t start = uni(cursor);
     int np = 0;
-    int lo0 = lo_unit;   // last kept unit starting at or before i0 (the same for every lane)
+    int lo0 = uni(lo_unit);   // last kept unit starting at or before i0 (the same for every lane)
     // candidate i0 + lane of the kept enumeration: its kept unit (ku), that
     // unit's start in the kept enumeration (ks), and its loads; advances lo0
     auto load = [&](uint32_t i0, YzFirst& c, uint32_t& ks, int& ku) {
@@ -2150,6 +2182,10 @@ __device__ __attribute__((noinline)) int yz_pass1_kept(Scr S, int kb, int ntkl, 
 // together.  Returns as yz_pass1_kept.
 template <int MAXH, int WPE>
 __device__ __attribute__((noinline)) int yz_pass1_plain(Scr S, int kb, int ntkl, int cap, int I, int& cursor) {
+    kb = uni(kb);
+    ntkl = uni(ntkl);
+    cap = uni(cap);
+    I = uni(I);
     Lds<MAXH>& L = g_lds<MAXH>;
     YzWave& Y = yzw<MAXH>();
     const int lane = lane_id();
@@ -2160,7 +2196,7 @@ __device__ __attribute__((noinline)) int yz_pass1_plain(Scr S, int kb, int ntkl,
     const unsigned d3m = (unsigned)ballot(lane < YZ_CHUNK && kb + lane < ntkl && tks[kb + lane].st3 == 4);
     float zv[4][2][2];
     yz_plane_z(zv);
-    const int start = cursor;
+    const int start = uni(cursor);
     int np = 0;
     // candidate i0 + lane: tracklet, list entries and unit, and its loads
     auto fill = [&](int i, int j, YzFirst& c) {
@@ -2176,16 +2212,17 @@ __device__ __attribute__((noinline)) int yz_pass1_plain(Scr S, int kb, int ntkl,
         c.m = (d3m >> j) & 1;
         yz_first_load(S, j, e, c);
     };
+    // the tracklet of the wave's first candidate (i0 only grows, and i0 < I =
+    // uoff[YZ_CHUNK]); when every candidate of the wave lies in it, the
+    // tracklet index is wave-uniform
+    int jcur = 0;
     auto load = [&](int i0, YzFirst& c) {
         const int i = i0 + lane;
         c.index = -1;
         c.ustart = 0;
-        // the tracklet of the wave's first candidate; when every candidate of the
-        // wave lies in it, the tracklet index is wave-uniform
-        int j0 = 0;
-#pragma unroll
-        for (int t = 1; t < YZ_CHUNK; ++t) j0 += i0 >= off[t] ? 1 : 0;
-        const int end0 = __builtin_amdgcn_readfirstlane(Y.uoff[j0 + 1]);
+        int end0 = uni(Y.uoff[jcur + 1]);
+        while (i0 >= end0) end0 = uni(Y.uoff[++jcur + 1]);
+        const int j0 = jcur;
         if (i0 + WAVE <= end0 || I <= end0) {
             if (i < I) fill(i, j0, c);
         } else if (i < I) {
@@ -2302,6 +2339,8 @@ __device__ __forceinline__ uchar4 yz_pass2_one(const Ev<MAXH, WPE>& E, const Scr
 // (free until then) for yz_pass2_firsts.
 template <int MAXH, int WPE>
 __device__ __attribute__((noinline)) int yz_pass2(Scr S, int kb, int np) {
+    kb = uni(kb);
+    np = uni(np);
     Lds<MAXH>& L = g_lds<MAXH>;
     YzWave& Y = yzw<MAXH>();
     Ev<MAXH, WPE> E;
@@ -2333,6 +2372,8 @@ __device__ __attribute__((noinline)) int yz_pass2(Scr S, int kb, int np) {
 // left/right from zero signs; their station-3 signs seed the unit.
 template <int MAXH, int WPE>
 __device__ __attribute__((noinline)) void yz_pass2_firsts(Scr S, int kb, int nf) {
+    kb = uni(kb);
+    nf = uni(nf);
     Lds<MAXH>& L = g_lds<MAXH>;
     YzWave& Y = yzw<MAXH>();
     Ev<MAXH, WPE> E;
@@ -2367,6 +2408,7 @@ __device__ __attribute__((noinline)) void yz_pass2_firsts(Scr S, int kb, int nf)
 // tests are listed.  Returns their number.
 template <int MAXH, int WPE>
 __device__ __attribute__((noinline)) int2 yz_pass3_scan(Scr S, int np) {
+    np = uni(np);
     const int lane = lane_id();
     // pass 3: starting signs by scan (the unit's seed, and the last writer of
     // the v3' sign before this candidate); the candidates that passed the
@@ -2445,6 +2487,9 @@ __device__ __forceinline__ float yz_chi2_st(const Ev<MAXH, WPE>& E, const Tk* Tp
 // the D3p part of the list (nq from its front), then the D3m part (nq4 from its back)
 template <int MAXH, int WPE>
 __device__ __attribute__((noinline)) void yz_pass3(Scr S, int kb, int nq, int nq4) {
+    kb = uni(kb);
+    nq = uni(nq);
+    nq4 = uni(nq4);
     YzWave& Y = yzw<MAXH>();
     Ev<MAXH, WPE> E;
     const int lane = lane_id();
@@ -2485,6 +2530,14 @@ __device__ __attribute__((noinline)) void yz_pass3(Scr S, int kb, int nq, int nq
 // chunk then takes the unit-by-unit path.
 template <int MAXH, int WPE>
 __device__ __attribute__((noinline)) bool yz_chunk_par(Scr S, Prn R, int kb, int ntkl, int cap) {
+    kb = uni(kb);
+    ntkl = uni(ntkl);
+    cap = uni(cap);
+    R.prune = uni(R.prune);
+    R.dump = uni(R.dump);
+    R.dump_cap = uni(R.dump_cap);
+    R.dump_count = uni(R.dump_count);
+    R.ev = uni(R.ev);
     Lds<MAXH>& L = g_lds<MAXH>;
     YzWave& Y = yzw<MAXH>();
     Ev<MAXH, WPE> E;
@@ -2615,6 +2668,8 @@ __device__ __attribute__((noinline)) bool yz_chunk_par(Scr S, Prn R, int kb, int
 // Unit loop: one lane per (tracklet, u3.v3) unit, candidates in reference order.
 template <int MAXH, int WPE>
 __device__ __attribute__((noinline)) void yz_chunk_seq(Scr S, int kb, int ntkl) {
+    kb = uni(kb);
+    ntkl = uni(ntkl);
     Lds<MAXH>& L = g_lds<MAXH>;
     YzWave& Y = yzw<MAXH>();
     Ev<MAXH, WPE> E;
@@ -2683,6 +2738,8 @@ __device__ __attribute__((noinline)) void yz_chunk_seq(Scr S, int kb, int ntkl) 
 
 template <int MAXH, int WPE>
 STAGE void stage_yz(or_params P, Scr S, Prn R, int ntkl, int list_cap) {
+    ntkl = uni(ntkl);
+    list_cap = uni(list_cap);
     Lds<MAXH>& L = g_lds<MAXH>;
     YzWave& Y = yzw<MAXH>();
     Ev<MAXH, WPE> E;
@@ -2844,6 +2901,10 @@ __device__ __forceinline__ int clean_shared9(const uint32_t (&a)[9], const uint3
 
 template <int MAXH, int WPE>
 __device__ __attribute__((noinline)) void clean_many(Tk* tks, const uint16_t* slot_order, int ntkl, int nlive) {
+    tks = uni(tks);
+    slot_order = uni(slot_order);
+    ntkl = uni(ntkl);
+    nlive = uni(nlive);
     Lds<MAXH>& L = g_lds<MAXH>;
     const int lane = lane_id();
     uint4* ent = Scr().clean_hits();
@@ -2988,6 +3049,9 @@ __device__ __attribute__((noinline)) void clean_many(Tk* tks, const uint16_t* sl
 
 template <int MAXH, int WPE>
 STAGE void stage_clean(Tk* tks, const uint16_t* slot_order, int ntkl) {
+    tks = uni(tks);
+    slot_order = uni(slot_order);
+    ntkl = uni(ntkl);
     Lds<MAXH>& L = g_lds<MAXH>;
     const int lane = lane_id();
     // gather in slot order
@@ -3366,6 +3430,13 @@ constexpr int GLOBAL_LIST_CAP = 2048;   // pass-A survivors held before pass B r
 template <int MAXH, int WPE>
 __device__ __attribute__((noinline)) void global_track(Scr S, Tk* T, const uint16_t* lx, const uint16_t* lu,
                                                        const uint16_t* lv, int nx1, int nu1, int nv1) {
+    T = uni(T);
+    lx = uni(lx);
+    lu = uni(lu);
+    lv = uni(lv);
+    nx1 = uni(nx1);
+    nu1 = uni(nu1);
+    nv1 = uni(nv1);
     Ev<MAXH, WPE> E;
     const int lane = lane_id();
     constexpr int det6[6] = {4, 3, 2, 1, 6, 5};
@@ -3555,6 +3626,7 @@ constexpr int GLOBAL_BATCH = WAVE / 3;
 
 template <int MAXH, int WPE>
 STAGE void stage_global(or_params P, Scr S, int ntkl) {
+    ntkl = uni(ntkl);
     Lds<MAXH>& L = g_lds<MAXH>;
     YzWave& Y = yzw<MAXH>();
     Ev<MAXH, WPE> E;
@@ -3723,6 +3795,9 @@ __device__ bool prop_segment(int p1a, int p1b, int p2a, int p2b, float slope, fl
 
 template <int MAXH, int WPE>
 STAGE void stage_postglobal(Tk* tks, const uint16_t* slot_order, int ntkl) {
+    tks = uni(tks);
+    slot_order = uni(slot_order);
+    ntkl = uni(ntkl);
     Lds<MAXH>& L = g_lds<MAXH>;
     const int lane = lane_id();
     for (int k = lane; k < ntkl; k += WAVE) {
@@ -3773,6 +3848,8 @@ STAGE void stage_postglobal(Tk* tks, const uint16_t* slot_order, int ntkl) {
 
 template <int MAXH, int WPE>
 STAGE void stage_vertex(Tk* tks, int ntkl) {
+    tks = uni(tks);
+    ntkl = uni(ntkl);
     for (int k = lane_id(); k < ntkl; k += WAVE)
         if (tks[k].stid == 6) vertex_track(tks[k]);
     wsync();
@@ -4315,6 +4392,8 @@ __device__ __attribute__((noinline)) void final_fit(const Tk& T, float* out) {
 
 template <int MAXH, int WPE>
 STAGE void stage_out(const Args& a, Scr S, int64_t e, int ntkl) {
+    e = uni(e);
+    ntkl = uni(ntkl);
     Lds<MAXH>& L = g_lds<MAXH>;
     const int lane = lane_id();
     const or_params& P = a.p;
