@@ -1614,10 +1614,14 @@ __device__ __forceinline__ Fit yz_fit_tab(int m, int pattern, float B0, float B1
 }
 
 // B sums of an entry's present hits, a then b: w y, and (w y) z with z the
-// plane's (za for hit a, zb for hit b), the products the fit sums
-__device__ __forceinline__ void yz_bsum(unsigned fl, const float4& t, float za, float zb, float& B0, float& B1) {
-    if (fl & 1) { B0 += t.x; B1 += t.x * za; }
-    if (fl & 2) { B0 += t.y; B1 += t.y * zb; }
+// plane's (za for hit a, zb for hit b), the products the fit sums.  An entry
+// holds 0 for a hit that is absent or not accepted, and its terms are added
+// too: x + (+-0) = x for every x but -0, and the sums never are -0 (they start
+// at +0, and a round-to-nearest sum is -0 only when both terms are), so the
+// sums are those of the present hits alone, bit for bit.
+__device__ __forceinline__ void yz_bsum(const float4& t, float za, float zb, float& B0, float& B1) {
+    B0 += t.x; B1 += t.x * za;
+    B0 += t.y; B1 += t.y * zb;
 }
 
 // z of the planes of YZ view v (u2, v2, u3, v3) with station 3 st3: hit a, hit b
@@ -1634,10 +1638,10 @@ __device__ __forceinline__ bool yz_fit(const YzLists& W, int st3, const YzItem& 
     if (!fu3 || !fv3 || !fu2 || !fv2) return false;
     float B0 = 0.f, B1 = 0.f;
     const float2 z0 = yz_zab(0, st3), z1 = yz_zab(1, st3), z2 = yz_zab(2, st3), z3 = yz_zab(3, st3);
-    yz_bsum(fu3, tu3, z2.x, z2.y, B0, B1);
-    yz_bsum(fv3, tv3, z3.x, z3.y, B0, B1);
-    yz_bsum(fu2, tu2, z0.x, z0.y, B0, B1);
-    yz_bsum(fv2, tv2, z1.x, z1.y, B0, B1);
+    yz_bsum(tu3, z2.x, z2.y, B0, B1);
+    yz_bsum(tv3, z3.x, z3.y, B0, B1);
+    yz_bsum(tu2, z0.x, z0.y, B0, B1);
+    yz_bsum(tv2, z1.x, z1.y, B0, B1);
     f = yz_fit_tab(st3 == 4 ? 1 : 0, yz_pattern_b(fu2, fv2, fu3, fv3), B0, B1);
     return true;
 }
@@ -1906,10 +1910,10 @@ __device__ __forceinline__ bool yz_first_eval(const YzFirst& c, const float (&zv
     if (c.index < 0 || !fl[0] || !fl[1] || !fl[2] || !fl[3]) return false;
     const int m = (int)c.m;
     float B0 = 0.f, B1 = 0.f;
-    yz_bsum(fl[2], c.t[2], m ? zv[2][1][0] : zv[2][0][0], m ? zv[2][1][1] : zv[2][0][1], B0, B1);
-    yz_bsum(fl[3], c.t[3], m ? zv[3][1][0] : zv[3][0][0], m ? zv[3][1][1] : zv[3][0][1], B0, B1);
-    yz_bsum(fl[0], c.t[0], zv[0][0][0], zv[0][0][1], B0, B1);
-    yz_bsum(fl[1], c.t[1], zv[1][0][0], zv[1][0][1], B0, B1);
+    yz_bsum(c.t[2], m ? zv[2][1][0] : zv[2][0][0], m ? zv[2][1][1] : zv[2][0][1], B0, B1);
+    yz_bsum(c.t[3], m ? zv[3][1][0] : zv[3][0][0], m ? zv[3][1][1] : zv[3][0][1], B0, B1);
+    yz_bsum(c.t[0], zv[0][0][0], zv[0][0][1], B0, B1);
+    yz_bsum(c.t[1], zv[1][0][0], zv[1][0][1], B0, B1);
     return yz_box1(yz_fit_tab(m, yz_pattern_b(fl[0], fl[1], fl[2], fl[3]), B0, B1));
 }
 
@@ -2347,16 +2351,21 @@ __device__ __attribute__((noinline)) int yz_pass2(Scr S, int kb, int np) {
     const int lane = lane_id();
     Tk* tks = S.tks();
     int nf = 0;
+    int carry = -1;   // unit of the previous wave iteration's last survivor
     for (int k0 = 0; k0 < np; k0 += WAVE) {
         const int k = k0 + lane;
-        bool first = false;
+        YzItem it;
+        int unit = -1;   // (tracklet, u3, v3) of the survivor; list indices stay below 256
         if (k < np) {
-            const YzItem it = yz_decode<MAXH, WPE>((int)S.plist()[k]);
-            first = k == 0;
-            if (!first) {
-                const YzItem pt = yz_decode<MAXH, WPE>((int)S.plist()[k - 1]);
-                first = pt.j != it.j || pt.iu3 != it.iu3 || pt.iv3 != it.iv3;
-            }
+            it = yz_decode<MAXH, WPE>((int)S.plist()[k]);
+            unit = (it.j << 16) | (it.iu3 << 8) | it.iv3;
+        }
+        // first of its unit: survivor k - 1 (the lane before, or the previous
+        // iteration's last) lies in another unit
+        const int before = __shfl_up(unit, 1, WAVE);
+        const bool first = k < np && (k == 0 || unit != (lane == 0 ? carry : before));
+        carry = __shfl(unit, WAVE - 1, WAVE);
+        if (k < np) {
             const Tk* Tp = &tks[kb + it.j];
             // per lane by its tracklet's station 3 (a wave's survivors mostly share one tracklet)
             S.pst()[k] = Tp->st3 == 3 ? yz_pass2_one<3>(E, S, Tp, it, k, first) : yz_pass2_one<4>(E, S, Tp, it, k, first);
@@ -2798,7 +2807,10 @@ STAGE void stage_yz(or_params P, Scr S, Prn R, int ntkl, int list_cap) {
                         lst[nacc] = lst[q];
                         // fit terms w y and (w y) z of each hit, as the fit sums them
                         const uint8_t fb = (uint8_t)(fl | ((fl - 1u) * (v == 0 ? 3u : v == 1 ? 1u : v == 2 ? 27u : 9u) << 2));
-                        wy[o] = make_float4(yv.y * yv.x, yv.w * yv.z, __uint_as_float(fb), 0.f);
+                        // (0 for a hit that is absent or not accepted, so that the B
+                        // sums can add every term: see yz_bsum)
+                        wy[o] = make_float4((fl & 1u) ? yv.y * yv.x : 0.f, (fl & 2u) ? yv.w * yv.z : 0.f,
+                                            __uint_as_float(fb), 0.f);
                         // station 3: the y of hits a and b; station 2: the entry's y
                         // (the mean of its accepted hits) for the pruner's features
                         wyy[o] = v >= 2 ? make_float2(yv.x, yv.z)
