@@ -2085,33 +2085,70 @@ __device__ __attribute__((noinline)) int yz_pass1_kept(Scr S, int kb, int ntkl, 
     const uint32_t start = uni(cursor);
     int np = 0;
     int lo0 = uni(lo_unit);   // last kept unit starting at or before i0 (the same for every lane)
+    // The kept units' starts and codes as two windows of 64 in registers: lane t
+    // of (ast, akl) holds unit wlo + t, of (bst, bkl) unit wlo + 64 + t (start
+    // 0xFFFFFFFF past the last unit).  The second is loaded one window ahead, so
+    // the common iteration reads its two units by lane index, without a load.
+    int wlo = lo0;
+    auto wload = [&](int base, uint32_t& st, uint32_t& kl) {
+        const bool in = base + lane < nk;
+        st = in ? S.kstart()[base + lane] : 0xFFFFFFFFu;
+        kl = in ? S.klist()[base + lane] : 0u;
+    };
+    uint32_t ast, akl, bst, bkl;
+    wload(wlo, ast, akl);
+    wload(wlo + WAVE, bst, bkl);
+    // start and code of unit wlo + t, t < 128 wave-uniform
+    auto wst = [&](int t) -> uint32_t {
+        return t < WAVE ? (uint32_t)__builtin_amdgcn_readlane((int)ast, t)
+                        : (uint32_t)__builtin_amdgcn_readlane((int)bst, t - WAVE);
+    };
+    auto wkl = [&](int t) -> uint32_t {
+        return t < WAVE ? (uint32_t)__builtin_amdgcn_readlane((int)akl, t)
+                        : (uint32_t)__builtin_amdgcn_readlane((int)bkl, t - WAVE);
+    };
     // candidate i0 + lane of the kept enumeration: its kept unit (ku), that
     // unit's start in the kept enumeration (ks), and its loads; advances lo0
     auto load = [&](uint32_t i0, YzFirst& c, uint32_t& ks, int& ku) {
         const uint32_t i = i0 + (uint32_t)lane;
+        while (lo0 - wlo >= WAVE) {
+            ast = bst;
+            akl = bkl;
+            wlo += WAVE;
+            wload(wlo + WAVE, bst, bkl);
+        }
+        const int s0 = lo0 - wlo;
         // every kept unit has at least one candidate, so the wave's 64 candidates
-        // lie in units lo0 .. lo0 + 63: lane t holds the start and code of unit
-        // lo0 + t, and each lane finds the last start at or before its candidate
-        // by halving
-        const bool inwin = lo0 + lane < nk;
-        const uint32_t st = inwin ? S.kstart()[lo0 + lane] : 0xFFFFFFFFu;
-        const uint32_t kl = inwin ? S.klist()[lo0 + lane] : 0u;
-        // the starts are increasing over the lanes, so the units starting at or
-        // before i0 + 64 are a prefix of them
-        const int tn = __popcll(ballot(st <= i0 + WAVE)) - 1;
-        int tu = 0;
-        if ((uint32_t)__builtin_amdgcn_readlane((int)st, 2) > i0 + WAVE - 1) {
-            // the wave's candidates lie in units lo0 and lo0 + 1 (the common case)
-            tu = i >= (uint32_t)__builtin_amdgcn_readlane((int)st, 1) ? 1 : 0;
+        // lie in units lo0 .. lo0 + 63, whose starts increase
+        const uint32_t st1 = wst(s0 + 1), st2 = wst(s0 + 2);
+        int tu, tn;
+        uint32_t st_t, code;
+        if (st2 > i0 + WAVE - 1) {
+            // the wave's candidates lie in units lo0 and lo0 + 1 (the common case);
+            // the units starting at or before i0 + 64 are lo0, and lo0 + 1, + 2 as
+            // their starts tell
+            tu = i >= st1 ? 1 : 0;
+            st_t = tu ? st1 : wst(s0);
+            code = tu ? wkl(s0 + 1) : wkl(s0);
+            tn = (st1 <= i0 + WAVE ? 1 : 0) + (st2 <= i0 + WAVE ? 1 : 0);
         } else {
+            // lane t takes the start and code of unit lo0 + t, and each lane
+            // finds the last start at or before its candidate by halving
+            const int src = s0 + lane;
+            const uint32_t sa = __shfl(ast, src & (WAVE - 1), WAVE), sb = __shfl(bst, src & (WAVE - 1), WAVE);
+            const uint32_t ka = __shfl(akl, src & (WAVE - 1), WAVE), kbb = __shfl(bkl, src & (WAVE - 1), WAVE);
+            const uint32_t st = src < WAVE ? sa : sb, kl = src < WAVE ? ka : kbb;
+            // the units starting at or before i0 + 64 are a prefix of them
+            tn = __popcll(ballot(st <= i0 + WAVE)) - 1;
+            tu = 0;
 #pragma unroll
             for (int step = WAVE / 2; step > 0; step >>= 1) {
                 const uint32_t sv = __shfl(st, tu + step, WAVE);
                 if (sv <= i) tu += step;
             }
+            st_t = __shfl(st, tu, WAVE);
+            code = __shfl(kl, tu, WAVE);
         }
-        const uint32_t st_t = __shfl(st, tu, WAVE);
-        const uint32_t code = __shfl(kl, tu, WAVE);
         ks = st_t;
         ku = lo0 + tu;
         // the next window starts at the last unit starting at or before i0 + 64;
