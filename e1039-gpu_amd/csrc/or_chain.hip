@@ -242,7 +242,7 @@ struct Lds {
     uint32_t status;
     int ntkl;                 // tracklets after XZ (read by every wave of the event)
     int yz_next;              // next YZ chunk to hand out
-    int gl_next;              // next block of 64 tracklets of the station-1 stage
+    int gl_next;              // next block of tracklets of the station-1 stage
     int next_ev;              // work queue: the event the group takes next
 };
 
@@ -3672,6 +3672,10 @@ __device__ __attribute__((noinline)) void global_track(Scr S, Tk* T, const uint1
 // together, lane (tracklet, view); the tracklets are then extended one after
 // the other, each independent of the others.
 constexpr int GLOBAL_BATCH = WAVE / 3;
+// tracklets handed to a wave at a time: small blocks keep the event's waves
+// evenly busy (only the tracklets at stationID >= 5 of a block take work)
+constexpr int GLOBAL_BLOCK = 16;
+static_assert(GLOBAL_BLOCK <= GLOBAL_BATCH, "a block's window lists are built in one batch");
 
 template <int MAXH, int WPE>
 STAGE void stage_global(or_params P, Scr S, int ntkl) {
@@ -3683,13 +3687,13 @@ STAGE void stage_global(or_params P, Scr S, int ntkl) {
     Tk* tks = S.tks();
     int* nlf = &Y.nl[0][0];
     const int capw = cfg().win_cap;
-    // blocks of 64 tracklets, handed to the event's waves as they come free
+    // blocks of GLOBAL_BLOCK tracklets, handed to the event's waves as they come free
     for (;;) {
         int k0 = 0;
-        if (lane == 0) k0 = atomicAdd(&L.gl_next, WAVE);
+        if (lane == 0) k0 = atomicAdd(&L.gl_next, GLOBAL_BLOCK);
         k0 = __builtin_amdgcn_readfirstlane(__shfl(k0, 0, WAVE));
         if (k0 >= ntkl) break;
-        unsigned long long rest = ballot(k0 + lane < ntkl && tks[k0 + lane].stid >= 5);
+        unsigned long long rest = ballot(lane < GLOBAL_BLOCK && k0 + lane < ntkl && tks[k0 + lane].stid >= 5);
         while (rest) {
             OR_SUB_START
             {
@@ -4565,7 +4569,7 @@ constexpr int chain_waves_per_simd() {
 }
 
 // The chain: one event per workgroup of WPE waves, persistent over a work queue.
-// YZ (by chunks of tracklets) and station 1 (by blocks of 64 tracklets) run on
+// YZ (by chunks of tracklets) and station 1 (by blocks of tracklets) run on
 // all waves; the other stages on the first.
 template <int MAXH, int WPE>
 __global__ __launch_bounds__(WAVE * WPE, (chain_waves_per_simd<MAXH, WPE>())) void chain_kernel(Args a) {
