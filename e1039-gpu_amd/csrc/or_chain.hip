@@ -2315,8 +2315,10 @@ __device__ __attribute__((noinline)) int yz_pass1_plain(Scr S, int kb, int ntkl,
 }
 
 // Candidate cache of pass 2 -> pass 3: raw LDS index of each slot's hit
-// (16 bits each), the slot presence bits and the first fit.
-__device__ __forceinline__ void yz_cache_put(uint4* c, const YzSlots& H, const Fit& f) {
+// (16 bits each), the slot presence bits, the first fit, and the candidate's
+// tracklet in the chunk and index among the tracklet's candidates (so that the
+// later passes need not decode its index again).
+__device__ __forceinline__ void yz_cache_put(uint4* c, const YzSlots& H, const Fit& f, const YzItem& it) {
     uint32_t r[4];
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
@@ -2326,7 +2328,7 @@ __device__ __forceinline__ void yz_cache_put(uint4* c, const YzSlots& H, const F
     }
     c[0] = make_uint4(r[0], r[1], r[2], r[3]);
     c[1] = make_uint4(H.pres, __float_as_uint(f.p0), __float_as_uint(f.p1), __float_as_uint(f.e0));
-    c[2] = make_uint4(__float_as_uint(f.e1), 0u, 0u, 0u);
+    c[2] = make_uint4(__float_as_uint(f.e1), (uint32_t)it.j, (uint32_t)it.local, 0u);
 }
 
 // The candidate exactly as yz_candidate builds it, from the cache: slot
@@ -2363,7 +2365,7 @@ __device__ __forceinline__ uchar4 yz_pass2_one(const Ev<MAXH, WPE>& E, const Scr
     YzSlots H;
     Fit f;
     yz_candidate_st<ST3>(E, S, it, H, f);
-    yz_cache_put(S.ycache() + 3 * k, H, f);
+    yz_cache_put(S.ycache() + 3 * k, H, f, it);
     const bool go = yz_cuts_hx(E, yzw<MAXH>().hx[it.j], f);
     uchar4 ps = make_uchar4((first ? 1 : 0) | (go ? 2 : 0) | (ST3 == 4 ? 4 : 0), 0, 0, 0);
     if (!first && ((lr_slot_starts<8>(H.pres, H.det) >> 3) & 1u)) {
@@ -2429,8 +2431,8 @@ __device__ __attribute__((noinline)) void yz_pass2_firsts(Scr S, int kb, int nf)
         const int q = q0 + lane;
         if (q < nf) {
             const int k = (int)S.qlist()[q];
-            const YzItem it = yz_decode<MAXH, WPE>((int)S.plist()[k]);
-            const Tk* Tp = &tks[kb + it.j];
+            const int j = (int)gld(&S.ycache()[3 * k + 2].y);   // the survivor's tracklet (yz_cache_put)
+            const Tk* Tp = &tks[kb + j];
             // per lane by its tracklet's station 3, as in pass 2
             auto lr = [&](auto st3c) {
                 YzSlots H;
@@ -2543,21 +2545,23 @@ __device__ __attribute__((noinline)) void yz_pass3(Scr S, int kb, int nq, int nq
     const int cap = cfg().list_cap;
     auto eval = [&](auto st3c, int k) {
         constexpr int ST3 = decltype(st3c)::value;
-        const YzItem it = yz_decode<MAXH, WPE>((int)S.plist()[k]);
         const uchar4 ps = S.pst()[k];
-        const Tk* Tp = &tks[kb + it.j];
+        // the survivor's tracklet and index among the tracklet's candidates (yz_cache_put)
+        const uint4 c2 = gld4(S.ycache() + 3 * k + 2);
+        const int j = (int)c2.y;
+        const uint32_t local = c2.z;
+        const Tk* Tp = &tks[kb + j];
         YzSlots H;
         Fit f;
         yz_cache_get(E, S.ycache() + 3 * k, ST3, H, f);
         if (!(ps.x & 1u)) yz_unpack3(H, ps.w);
         lr_slots_new<8>(Tp->x0, Tp->tx, f.p0, f.p1, Tp->ex0, Tp->etx, f.e0, f.e1, H.pres, H.det, H.pos,
                         H.drift, H.sign, 150.f);
-        const float chi2 = yz_chi2_st<ST3>(E, Tp, it.j, H, f);
+        const float chi2 = yz_chi2_st<ST3>(E, Tp, j, H, f);
         // the reference starts its minimum at 10000.1 (reconstruction_kernels.cuh:1240)
         if (chi2 < 10000.1f) {
-            const unsigned long long key =
-                ((unsigned long long)__float_as_uint(chi2) << 32) | (unsigned long long)(uint32_t)it.local;
-            atomicMin(&Y.best[it.j], key);
+            const unsigned long long key = ((unsigned long long)__float_as_uint(chi2) << 32) | (unsigned long long)local;
+            atomicMin(&Y.best[j], key);
         }
     };
     for (int q0 = 0; q0 < nq; q0 += WAVE) {
