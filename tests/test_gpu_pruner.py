@@ -170,25 +170,39 @@ def test_margin_keeping_everything_is_the_plain_chain(oc, orc, geo, ogeo):
     sess.close()
 
 
-def test_pruned_chain_is_the_plain_chain_over_its_searched_units(oc, orc, geo, ogeo):
-    # the shipped pruner on mean-1,200 events; tracklets without unit rows
-    # (chunks beyond the unit list) and chunks that went unit by unit (flag -1)
-    # searched every unit, and the oracle's filter leaves them unrestricted
+PRUNED_WORKLOADS = {
+    # mean 1,200 hits (the learned-pruning configuration)
+    "mean1200": (120, dict(mult_mode=1, mult_mean=1200.0, mult_cap=4000), {}, 1000),
+    # the batched mixture: Poisson(200) events run in the one-wave 512-hit tile
+    "mixture": (300, dict(mult_mode=2, mult_mean=200.0, mult_mean2=1500.0, mult_frac2=0.2, mult_cap=4000), {}, 300),
+    # high intensity with the full chain (prop matching, global cleaning,
+    # vertexing; only stationID >= 7 tracks come out)
+    "mean2000_full": (60, dict(mult_mode=1, mult_mean=2000.0, mult_cap=6000, n_tracks_min=1, n_tracks_max=4),
+                      dict(max_tracklets_per_event=4096, full_chain=1), 60),
+}
+
+
+@pytest.mark.parametrize("workload", sorted(PRUNED_WORKLOADS))
+def test_pruned_chain_is_the_plain_chain_over_its_searched_units(oc, orc, geo, ogeo, workload):
+    # the shipped pruner; tracklets without unit rows (chunks beyond the unit
+    # list) and chunks that went unit by unit (flag -1) searched every unit, and
+    # the oracle's filter leaves them unrestricted
     import torch
 
     import pruner
     if not os.path.exists(oc.DEFAULT_PRUNER):
         pytest.skip("no shipped pruner weights")
+    n_events, gen, caps, min_tracks = PRUNED_WORKLOADS[workload]
     flat, thr, meta = pruner.load()
-    sess = oc.Session(geo, oc.high_occupancy_params(), device=0)
-    ev = oc.generate(geo, 120, seed=0xF17, mult_mode=1, mult_mean=1200.0, mult_cap=4000)
-    rows = torch.empty((ev.n_events * 60000, oc.UNIT_ROW), dtype=torch.float32, device="cuda")
+    sess = oc.Session(geo, oc.high_occupancy_params(**caps), device=0)
+    ev = oc.generate(geo, n_events, seed=0xF17, **gen)
+    rows = torch.empty((12_000_000, oc.UNIT_ROW), dtype=torch.float32, device="cuda")   # 960 MB
     count = torch.zeros(1, dtype=torch.int64, device="cuda")
     sess.set_pruner_margin(meta["margin"])
     sess.set_pruner(flat, thr)
     sess.set_unit_dump(rows, count)
     try:
-        got = sess.reconstruct(ev, records=True)
+        got = sess.reconstruct(ev, records=True, vertex=True)
     finally:
         sess.set_unit_dump(None)
         sess.set_pruner(None)
@@ -202,11 +216,13 @@ def test_pruned_chain_is_the_plain_chain_over_its_searched_units(oc, orc, geo, o
     assert (flag == 0.0).mean() > 0.5, "the pruner kept most units"
     s = flag >= 0.5
     keys = orc.unit_keys(r[s, 0].astype(np.int64), r[s, 1].astype(np.int64), r[s, 2].astype(np.int64))
-    ref = orc.reconstruct(ogeo, sess.params, ev, records=True, units=keys)
-    assert ref.n_tracks.sum() > 1000
+    ref = orc.reconstruct(ogeo, sess.params, ev, records=True, vertex=True, units=keys)
+    assert ref.n_tracks.sum() > min_tracks
     same_outputs(got, ref)
+    k = np.arange(ref.tracks.shape[1])[None, :] < ref.n_tracks[:, None]
+    assert np.array_equal(got.vertex[k].view(np.uint32), ref.vertex[k].view(np.uint32)), "vertex differs"
     # the pruning changed some tracklet's YZ winner (the comparison is not the plain chain's)
-    plain = orc.reconstruct(ogeo, sess.params, ev, records=True)
+    plain = orc.reconstruct(ogeo, sess.params, ev, records=True, vertex=True)
     assert not np.array_equal(plain.records.view(np.uint32), ref.records.view(np.uint32))
     sess.close()
 
