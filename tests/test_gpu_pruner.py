@@ -227,6 +227,43 @@ def test_pruned_chain_is_the_plain_chain_over_its_searched_units(oc, orc, geo, o
     sess.close()
 
 
+def test_pruned_graph_replay(oc, geo):
+    # the batched form with pruning: the captured launch sequence replayed
+    # several times, with eager calls in between, equals the eager pruned chain
+    import torch
+
+    import pruner
+    if not os.path.exists(oc.DEFAULT_PRUNER):
+        pytest.skip("no shipped pruner weights")
+    flat, thr, meta = pruner.load()
+    sess = oc.Session(geo, oc.high_occupancy_params(), device=0)
+    ev = oc.generate(geo, 400, seed=0xB7, mult_mode=2, mult_mean=200.0, mult_mean2=1500.0, mult_frac2=0.2,
+                     mult_cap=4000)
+    other = oc.generate(geo, 100, seed=0xB8, mult_mode=1, mult_mean=1200.0, mult_cap=4000)
+    sess.set_pruner_margin(meta["margin"])
+    sess.set_pruner(flat, thr)
+    try:
+        ref = sess.reconstruct(ev, records=True)
+        din = sess.upload(ev)
+        dout = sess.alloc_outputs(ev.n_events, ev.n_hits, tracklets=True, records=True)
+        for rep in range(3):
+            for k in ("status", "n_tracks", "tracks", "n_tracklets", "records", "keep_mask"):
+                dout[k].fill_(0x5A if k != "records" else 7)
+            sess.set_tuning(oc.TUNE_GRAPH, 1)
+            sess.run_device(din, dout)
+            torch.cuda.synchronize()
+            got = oc.Session.fetch(dout, ev.n_events, sess.params.max_tracks_out, sess.params.max_tracklets_per_event)
+            same_outputs(got, ref)
+            sess.set_tuning(oc.TUNE_GRAPH, 0)
+            sess.reconstruct(other)            # eager work between replays
+    finally:
+        sess.set_tuning(oc.TUNE_GRAPH, 0)
+        sess.set_pruner(None)
+        sess.set_pruner_margin()
+    assert ref.n_tracks.sum() > 500
+    sess.close()
+
+
 def test_shipped_pruner_recall(oc, geo):
     # the learned-pruning configuration: hits Poisson(1200), high-occupancy caps,
     # events of a seed the training never saw
