@@ -45,6 +45,7 @@ struct RtDev {
     RtHead h[ND];
     const double* x;
     const double4* cf;
+    int32_t nk;   // knots of all detectors
 };
 
 // Drift of one hit (online_reconstruction.cu:389-397): tdc < tmin -> cellwidth/2,
@@ -73,15 +74,25 @@ __device__ __forceinline__ float rt_drift(const RtHead& H, const double* x, cons
     return (float)(c.x + dt * (c.y + dt * (c.z + dt * c.w)));
 }
 
-constexpr int RT_BLOCK = 256, RT_UNROLL = 4;
+constexpr int RT_BLOCK = 256, RT_UNROLL = 8;
+// knots staged in LDS by rt_calibrate_kernel<true> (tables with more use global memory)
+constexpr int RT_LDS_KNOTS = 8192;
 
-// one hit per lane, RT_UNROLL hits per lane in flight; the per-detector heads in LDS
+// RT_UNROLL hits per lane in flight; the per-detector heads in LDS, and with
+// XLDS the knots too (the interval search then reads LDS, and one global load
+// per hit, its interval's cubic, remains)
+template <bool XLDS>
 __global__ __launch_bounds__(RT_BLOCK) void rt_calibrate_kernel(const RtDev* R, const int16_t* det, const float* tdc,
                                                                 const float* drift, int64_t n, float* out) {
     __shared__ RtHead H[ND];
+    extern __shared__ double rt_x[];
     for (int i = threadIdx.x; i < ND; i += RT_BLOCK) H[i] = R->h[i];
     const double* X = R->x;
     const double4* CF = R->cf;
+    if (XLDS) {
+        const int nk = R->nk;
+        for (int i = threadIdx.x; i < nk; i += RT_BLOCK) rt_x[i] = X[i];
+    }
     __syncthreads();
     for (int64_t base = (int64_t)blockIdx.x * RT_BLOCK * RT_UNROLL; base < n;
          base += (int64_t)gridDim.x * RT_BLOCK * RT_UNROLL) {
@@ -96,7 +107,8 @@ __global__ __launch_bounds__(RT_BLOCK) void rt_calibrate_kernel(const RtDev* R, 
 #pragma unroll
         for (int j = 0; j < RT_UNROLL; ++j) {
             const int64_t i = base + j * RT_BLOCK + threadIdx.x;
-            if (i < n) out[i] = (d[j] >= 1 && d[j] < ND) ? rt_drift(H[d[j]], X, CF, d[j], t[j], dr[j]) : dr[j];
+            if (i < n)
+                out[i] = (d[j] >= 1 && d[j] < ND) ? rt_drift(H[d[j]], XLDS ? rt_x : X, CF, d[j], t[j], dr[j]) : dr[j];
         }
     }
 }
@@ -4749,6 +4761,7 @@ struct or_session {
     int64_t dump_cap = 0;
     unsigned long long* dump_count = nullptr;
     RtDev* d_rt = nullptr;           // R-T calibration table (or_session_set_calibration)
+    int64_t rt_nk = 0;               // its knots
     size_t d_rt_bytes = 0;
     bool rt_on = false;
     struct GraphKey {
@@ -5352,6 +5365,7 @@ or_status or_session_set_calibration(or_session* s, const or_calibration* c) {
     double4* hc = reinterpret_cast<double4*>(img.data() + head + (xb + 31) / 32 * 32);
     h->x = reinterpret_cast<const double*>(base + head);
     h->cf = reinterpret_cast<const double4*>(base + head + (xb + 31) / 32 * 32);
+    h->nk = (int32_t)nk;
     int64_t o = 0;
     for (int d = 0; d < OR_N_DETECTORS; ++d) {
         const or_calibration::Det& D = c->det[d];
@@ -5368,6 +5382,7 @@ or_status or_session_set_calibration(or_session* s, const or_calibration* c) {
     }
     OR_HIP(hipStreamSynchronize(s->stream));
     OR_HIP(hipMemcpy(s->d_rt, img.data(), bytes, hipMemcpyHostToDevice));
+    s->rt_nk = nk;
     s->rt_on = true;
     return OR_OK;
 }
@@ -5381,11 +5396,20 @@ or_status or_rt_calibrate(or_session* s, const or_hits* in, float* drift_out) {
     int cus = 0;
     OR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s->device));
     const int64_t per_block = (int64_t)RT_BLOCK * RT_UNROLL;
-    const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(16LL * std::max(1, cus), (n + per_block - 1) / per_block));
+    // knots in LDS when they fit: a few blocks per CU, so that few copies of the table are staged
+    const bool xlds = s->rt_nk <= RT_LDS_KNOTS;
+    const size_t lds = xlds ? (size_t)std::max<int64_t>(s->rt_nk, 1) * sizeof(double) : 0;
+    const unsigned grid = (unsigned)std::max<int64_t>(
+        1, std::min<int64_t>((xlds ? 4LL : 16LL) * std::max(1, cus), (n + per_block - 1) / per_block));
+    auto launch = [&](const int16_t* d, const float* t, const float* r, float* o) {
+        if (xlds)
+            hipLaunchKernelGGL(rt_calibrate_kernel<true>, dim3(grid), dim3(RT_BLOCK), lds, s->stream, s->d_rt, d, t, r, n, o);
+        else
+            hipLaunchKernelGGL(rt_calibrate_kernel<false>, dim3(grid), dim3(RT_BLOCK), 0, s->stream, s->d_rt, d, t, r, n, o);
+        return hipGetLastError();
+    };
     if (in->on_device) {
-        hipLaunchKernelGGL(rt_calibrate_kernel, dim3(grid), dim3(RT_BLOCK), 0, s->stream, s->d_rt, in->det, in->tdc,
-                           in->drift, n, drift_out);
-        OR_HIP(hipGetLastError());
+        OR_HIP(launch(in->det, in->tdc, in->drift, drift_out));
         return OR_OK;
     }
     char* buf = nullptr;
@@ -5398,10 +5422,7 @@ or_status or_rt_calibrate(or_session* s, const or_hits* in, float* drift_out) {
     hipError_t e = hipMemcpyAsync(dd, in->det, (size_t)n * 2, hipMemcpyHostToDevice, s->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(dt, in->tdc, (size_t)n * 4, hipMemcpyHostToDevice, s->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(dr, in->drift, (size_t)n * 4, hipMemcpyHostToDevice, s->stream);
-    if (e == hipSuccess) {
-        hipLaunchKernelGGL(rt_calibrate_kernel, dim3(grid), dim3(RT_BLOCK), 0, s->stream, s->d_rt, dd, dt, dr, n, dout);
-        e = hipGetLastError();
-    }
+    if (e == hipSuccess) e = launch(dd, dt, dr, dout);
     if (e == hipSuccess) e = hipMemcpyAsync(drift_out, dout, (size_t)n * 4, hipMemcpyDeviceToHost, s->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
     const hipError_t ef = hipFree(buf);
