@@ -344,4 +344,30 @@ void orc_dimuons(int64_t n_events, int max_out, const int32_t* n_tracks, const u
     if (lost_out) *lost_out = lost;
 }
 
+// The swim of build_event for n pairs on their own (start state, then
+// closest_approach): a, b hold n rows of (x0, tx, y0, ty, invp, charge, z of the
+// track's most upstream hit) for the first and the second track; out gets n rows
+// of (v[3], p_pos[3], p_neg[3], z returned).  p_pos and p_neg start at 0 here.
+void orc_dimuon_swim(int n, const float* a, const float* b, float* out) {
+    std::vector<float> pa1(3 * (NS + 1)), ma1(3 * (NS + 1)), pa2(3 * (NS + 1)), ma2(3 * (NS + 1));
+    auto trk = [](const float* r, float* z) {
+        Trk T = {};
+        T.x0 = r[0]; T.tx = r[1]; T.y0 = r[2]; T.ty = r[3]; T.invp = r[4]; T.charge = r[5];
+        z[0] = r[6];
+        T.det_first = 0;
+        return T;
+    };
+    for (int i = 0; i < n; ++i) {
+        float za[1], zb[1];
+        const Trk A = trk(a + 7 * i, za), B = trk(b + 7 * i, zb);
+        start_state(A, za, pa1.data(), ma1.data());
+        start_state(B, zb, pa2.data(), ma2.data());
+        float* o = out + 10 * i;
+        float v[3], pp[3] = {0, 0, 0}, pn[3] = {0, 0, 0};
+        o[9] = closest_approach(v, pp, pn, A.charge, B.charge, A.ty, B.ty, pa1.data(), ma1.data(), pa2.data(),
+                                ma2.data());
+        for (int k = 0; k < 3; ++k) { o[k] = v[k]; o[3 + k] = pp[k]; o[6 + k] = pn[k]; }
+    }
+}
+
 }  // extern "C"

@@ -66,6 +66,7 @@ def load():
     lib.orc_prop_segment.restype = C.c_int
     lib.orc_global_cleaning.argtypes = [C.c_int, vp, vp, vp, vp, vp, vp]
     lib.orc_dimuons.argtypes = [C.c_int64, C.c_int, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
+    lib.orc_dimuon_swim.argtypes = [C.c_int, vp, vp, vp]
     lib.orc_rt_spline.argtypes = [C.c_int, vp, vp, vp, vp, vp]
     lib.orc_rt_eval.argtypes = [C.c_int, vp, vp, C.c_int64, vp, vp]
     lib.orc_rt_calibrate.argtypes = [vp, vp, vp, vp, vp, vp, C.c_int64, vp, vp, vp, vp]
@@ -220,6 +221,18 @@ def rt_calibrate(cal: OracleCalibration, cellwidth, det, tdc, drift):
 
 
 # ---- dimuons and display histograms (reconstruction_kernels.cuh:2820-3195) ----
+
+def dimuon_swim(a, b):
+    """The dimuon closest-approach swim for pairs of tracks.  a, b: (n, 7) rows of
+    (x0, tx, y0, ty, invp, charge, z of the most upstream hit), first and second
+    track.  Returns (n, 10): vertex x, y, z; p_pos; p_neg; the z returned."""
+    a = np.ascontiguousarray(a, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    assert a.shape == b.shape and a.ndim == 2 and a.shape[1] == 7
+    out = np.zeros((len(a), 10), np.float32)
+    load().orc_dimuon_swim(len(a), a.ctypes.data, b.ctypes.data, out.ctypes.data)
+    return out
+
 
 def dimuons(chain, ev, z_array, max_out: int, trigger=None, histograms: bool = True):
     """Dimuons of a full-chain result with vertex (chain: the Result of reconstruct(..., vertex=True))."""

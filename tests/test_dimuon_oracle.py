@@ -11,8 +11,8 @@ restating reconstruction_kernels.cuh:2820-3195 and reconstruction_helper.cuh:
   * histograms against a vectorised numpy fill of the same records and tracks
     (every bin's predicate evaluated in float32);
   * the not-a-bit-test trigger mapping and the "all" row.
-The swim arithmetic itself (float arrays, double constants) has no independent
-reference here beyond these properties."""
+The closest-approach swim on its own is pinned against a continuous-model
+integration in tests/test_dimuon_swim.py."""
 import numpy as np
 import pytest
 
