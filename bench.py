@@ -113,8 +113,8 @@ def main():
     ap.add_argument("--chain", choices=["both", "plain", "pruned"], default="both",
                     help="both: time plain and pruned; plain: plain only; pruned: plain once untimed (recall), pruned timed")
     ap.add_argument("--no-e2e", action="store_true", help="skip the host-buffer (PCIe) leg")
-    ap.add_argument("--waves-per-event", type=int, default=0, choices=[0, 1, 2, 4],
-                    help="waves per event of the tile classes above 512 hits (0: the library default)")
+    ap.add_argument("--waves-per-event", type=int, default=0, choices=[0, 1, 2, 4, 8],
+                    help="waves per event at most, each tile class up to its bound (0: the library default, 8)")
     ap.add_argument("--yz-batch", type=int, default=0, help="YZ survivor batch (0: the library default)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (reference for HBM byte counters) and exit")
@@ -451,6 +451,8 @@ def main():
             "events": ev.n_events,
             "hits": int(ev.n_hits),
             "waves_per_event": wpe,
+            "waves_per_event_note": "at most: the 1,024- and 2,048-hit tiles run min(value, 4), the 4,096- and "
+                                    "8,192-hit tiles min(value, 8)",
             "yz_batch": yz_batch,
             "kernel_ms": kernel_ms,
             "rejected_events": rejected,

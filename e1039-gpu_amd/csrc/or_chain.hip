@@ -190,8 +190,13 @@ constexpr int CLEAN_WORDS = 64;   // lane-parallel cleaning of up to 64 x 64 liv
 // Waves per event: a tile class runs with 1, 2 or 4 waves per workgroup (one
 // event per workgroup).  The YZ and station-1 stages hand their tracklet
 // chunks to all of them; the other stages run on the first wave.
-constexpr int WPE_MAX = 4;
-__host__ __device__ constexpr int lds_waves(int maxh) { return maxh <= 512 ? 1 : WPE_MAX; }
+constexpr int WPE_MAX = 8;
+// waves per event a tile class is built for at most: the 512-hit tile one, the
+// 1,024- and 2,048-hit tiles four, the 4,096- and 8,192-hit tiles eight (their
+// LDS admits two and one workgroups per CU, so eight waves keep four and two
+// per SIMD)
+__host__ __device__ constexpr int lds_waves(int maxh) { return maxh <= 512 ? 1 : (maxh <= 2048 ? 4 : WPE_MAX); }
+__host__ __device__ constexpr int lds_waves_log(int maxh) { return maxh <= 512 ? 0 : (maxh <= 2048 ? 2 : 3); }
 
 // YZ state of one wave's chunk: window list lengths, candidate or unit offsets
 // per tracklet, best unit score (pruning) and best candidate key per tracklet.
@@ -4740,8 +4745,8 @@ struct or_session {
     int maxh = 1024;
     int nclass = 2;
     int cus = 0;                  // compute units of the device
-    int occ[NCLASS][3] = {};      // resident workgroups per CU by class and waves per event (1, 2, 4)
-    int wpe_log = 2;              // waves per event 1 << wpe_log for the tile classes above 512 hits
+    int occ[NCLASS][4] = {};      // resident workgroups per CU by class and waves per event (1, 2, 4, 8)
+    int wpe_log = 3;              // waves per event at most 1 << wpe_log (each class up to its own bound)
     int32_t* d_lists = nullptr;   // NCLASS x list_cap event indices
     int32_t* d_counts = nullptr;  // NCLASS class counts, then NCLASS work queue heads
     int64_t list_cap = 0;
@@ -4857,16 +4862,16 @@ int occupancy_blocks() {
     return nb < 1 ? 1 : nb;
 }
 
-// resident workgroups per CU of a tile class at 1, 2, 4 waves per event (0: not built)
+// resident workgroups per CU of a tile class at 1, 2, 4, 8 waves per event (0: not built)
 template <int MAXH>
-void occupancy_of(int (&o)[3]) {
+void occupancy_of(int (&o)[4]) {
     o[0] = occupancy_blocks<MAXH, 1>();
+    o[1] = o[2] = o[3] = 0;
     if constexpr (lds_waves(MAXH) > 1) {
         o[1] = occupancy_blocks<MAXH, 2>();
         o[2] = occupancy_blocks<MAXH, 4>();
-    } else {
-        o[1] = o[2] = 0;
     }
+    if constexpr (lds_waves(MAXH) > 4) o[3] = occupancy_blocks<MAXH, 8>();
 }
 
 template <int MAXH, int WPE>
@@ -4884,7 +4889,10 @@ hipError_t launch_wpe(int k, const Args& a, int grid, hipStream_t s) {
         switch (k) {
             case 0: return launch_chain<MAXH, 1>(a, grid, s);
             case 1: return launch_chain<MAXH, 2>(a, grid, s);
-            default: return launch_chain<MAXH, 4>(a, grid, s);
+            case 2: return launch_chain<MAXH, 4>(a, grid, s);
+            default:
+                if constexpr (lds_waves(MAXH) > 4) return launch_chain<MAXH, 8>(a, grid, s);
+                else return launch_chain<MAXH, 4>(a, grid, s);
         }
     }
 }
@@ -4998,7 +5006,7 @@ or_status or_session_create(const or_geometry* g, const or_params* p, int device
     s->cus = prop.multiProcessorCount;
     s->grid = 0;
     for (int c = 0; c < s->nclass; ++c)
-        for (int k = 0; k < 3; ++k) s->grid = std::max(s->grid, s->cus * s->occ[c][k] << k);
+        for (int k = 0; k < 4; ++k) s->grid = std::max(s->grid, s->cus * s->occ[c][k] << k);
     s->lay = make_layout(prm, YZ_LIST_MIN);
     OR_HIP(hipMalloc(&s->d_counts, sizeof(int32_t) * 2 * NCLASS));
     OR_HIP(hipMalloc(&s->d_scratch, (size_t)s->lay.stride * s->grid));
@@ -5056,8 +5064,8 @@ or_status or_session_set_tuning(or_session* s, or_tuning key, int64_t value) {
             s->use_graph = value != 0;
             return OR_OK;
         case OR_TUNE_WAVES_PER_EVENT:
-            if (value != 1 && value != 2 && value != 4) return OR_ERR_INVALID_ARG;
-            s->wpe_log = value == 1 ? 0 : (value == 2 ? 1 : 2);
+            if (value != 1 && value != 2 && value != 4 && value != 8) return OR_ERR_INVALID_ARG;
+            s->wpe_log = value == 1 ? 0 : (value == 2 ? 1 : (value == 4 ? 2 : 3));
             return OR_OK;
     }
     return OR_ERR_INVALID_ARG;
@@ -5117,7 +5125,7 @@ static or_status issue_chain_with(or_session* s, const or_hits* in, const or_res
         a.ev_list = B.lists + c * B.list_stride;
         a.ev_count = B.counts + c;
         a.ev_head = B.counts + NCLASS + c;
-        int k = c == 0 ? 0 : s->wpe_log;
+        int k = std::min(s->wpe_log, lds_waves_log(class_maxh(c)));
         while (k > 0 && ((int64_t)1 << k) > B.waves_max) --k;   // a small fixed scratch: fewer waves per event
         const int64_t g = std::max<int64_t>(1, (int64_t)s->cus * s->occ[c][k] * s->grid_pct / 100);
         const int grid = (int)std::min<int64_t>(std::min<int64_t>(g, in->n_events), B.waves_max >> k);
@@ -5507,7 +5515,7 @@ or_status or_session_set_unit_dump(or_session* s, float* rows, int64_t cap, unsi
 
 or_status or_session_autotune(or_session* s, const or_hits* in, or_result* out, int reps, float* best_ms) {
     if (!s || !in || !out || !in->on_device || !out->on_device || reps < 1) return OR_ERR_INVALID_ARG;
-    const int wpe_logs[] = {0, 1, 2};
+    const int wpe_logs[] = {0, 1, 2, 3};
     const int list_caps[] = {s->lay.list_cap, s->lay.list_cap / 2, s->lay.list_cap / 4, s->lay.list_cap / 8};
     static const int grid_pcts[] = {75, 50};
     int best_w = s->wpe_log, best_cap = s->yz_list_cap, best_pct = s->grid_pct;

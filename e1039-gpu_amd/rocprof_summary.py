@@ -70,7 +70,7 @@ def main():
     ap.add_argument("--workload", default="c3")
     ap.add_argument("--events", type=int, required=True)
     ap.add_argument("--mode", choices=["plain", "pruned"], required=True)
-    ap.add_argument("--waves-per-event", type=int, default=4)
+    ap.add_argument("--waves-per-event", type=int, default=8, help="the session's setting (at most; 4 on the 2,048-hit tile)")
     ap.add_argument("--yz-batch", type=int, default=1024)
     ap.add_argument("--algorithmic-bytes", type=float, required=True, help="bytes per launch (bench.py's figure)")
     ap.add_argument("--commands", default="", help="file holding the commands that made the runs")
@@ -144,7 +144,8 @@ def main():
     cmds = open(a.commands).read().strip() if a.commands else ""
     kname = re.search(r"chain_kernel<[^>]*>", dom).group(0)
     lines = [f"# {kname}, {a.workload}, {a.mode} chain, {a.events} events",
-             "", f"Settings: {a.waves_per_event} waves per event, YZ survivor batch {a.yz_batch}; kernel sources "
+             "", f"Settings: waves per event at most {a.waves_per_event} ({min(a.waves_per_event, 4)} on the 1,024- and "
+             "2,048-hit tiles), YZ survivor batch {a.yz_batch}; kernel sources "
              f"sha256 {res['kernel_source_sha256_16']} ({', '.join(KERNEL_SOURCES)}).", "",
              "Commands (repository root on the GPU box as `$REPO`; one counter block per run):", "", "```", cmds,
              "```", "", "## Kernel trace", "",

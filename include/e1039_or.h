@@ -252,10 +252,12 @@ typedef enum {
     /* 1: capture the launch sequence of a call into a hipGraph and replay it
      * while the buffers, sizes and stream stay the same; 0 (default): launch */
     OR_TUNE_GRAPH = 3,
-    /* waves per event, 1, 2 or 4 (default 4), for the tile classes above 512
-     * hits (the 512-hit class always runs one): the waves of an event share its
-     * LDS tile and split its YZ and station-1 work by tracklets; the other
-     * stages run on one wave.  Results do not depend on it. */
+    /* waves per event at most, 1, 2, 4 or 8 (default 8): each tile class runs
+     * min(value, its bound) waves per event, the bound being 1 for the 512-hit
+     * class, 4 for the 1,024- and 2,048-hit classes and 8 for the 4,096- and
+     * 8,192-hit classes.  The waves of an event share its LDS tile and split its
+     * YZ and station-1 work by tracklets; the other stages run on one wave.
+     * Results do not depend on it. */
     OR_TUNE_WAVES_PER_EVENT = 4
 } or_tuning;
 or_status or_session_set_tuning(or_session* s, or_tuning key, int64_t value);

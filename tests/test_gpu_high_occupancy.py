@@ -97,17 +97,18 @@ def test_geometry_conflict(oc, geo, tmp_path):
     sess.close()
 
 
-@pytest.mark.parametrize("mean,cap,n", [(800.0, 1000, 30), (1200.0, 4000, 30), (2500.0, 6000, 8)])
+@pytest.mark.parametrize("mean,cap,n", [(800.0, 1000, 30), (1200.0, 4000, 30), (2500.0, 6000, 8), (4300.0, 8000, 2)])
 def test_waves_per_event(oc, orc, geo, ogeo, mean, cap, n):
-    # an event's YZ chunks split over 1, 2 or 4 waves (tile classes 1024, 2048,
-    # 4096 hits), with full and small survivor batches: the oracle's result at each
-    print(f"case: {n} events, hits Poisson({mean:g}) cap {cap}, waves per event 1/2/4", flush=True)
+    # an event's YZ chunks split over 1, 2, 4 or 8 waves (tile classes 1024,
+    # 2048 at most 4, 4096 and 8192 hits at most 8), with full and small
+    # survivor batches: the oracle's result at each
+    print(f"case: {n} events, hits Poisson({mean:g}) cap {cap}, waves per event 1/2/4/8", flush=True)
     ev = oc.generate(geo, n, seed=0x3E + int(mean), mult_mode=1, mult_mean=mean, mult_cap=cap)
     params = oc.high_occupancy_params(max_tracklets_per_event=4096)
     ref = orc.reconstruct(ogeo, params, ev, records=True)
     assert ref.n_tracklets.sum() > 100 * n
     sess = oc.Session(geo, params, device=0)
-    for wpe in (1, 2, 4):
+    for wpe in (1, 2, 4, 8):
         sess.set_tuning(oc.TUNE_WAVES_PER_EVENT, wpe)
         assert sess.get_tuning(oc.TUNE_WAVES_PER_EVENT) == wpe
         for batch in (4096, 150):

@@ -194,7 +194,7 @@ def test_graph_replay_and_tuning(oc, orc, geo, ogeo):
     torch.cuda.synchronize()
     assert sess.get_tuning(oc.TUNE_GRID_PERCENT) in (100, 75, 50)
     assert sess.get_tuning(oc.TUNE_YZ_LIST_CAP) in (4096, 2048, 1024, 512)
-    assert sess.get_tuning(oc.TUNE_WAVES_PER_EVENT) in (1, 2, 4)
+    assert sess.get_tuning(oc.TUNE_WAVES_PER_EVENT) in (1, 2, 4, 8)
     got = oc.Session.fetch(dout, ev.n_events, sess.params.max_tracks_out, sess.params.max_tracklets_per_event)
     assert_same(got, ref, ev, "after autotune")
     for pct in (10, 60):
