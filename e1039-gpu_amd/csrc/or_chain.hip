@@ -264,9 +264,9 @@ struct Lds {
 };
 
 // Per-wave scratch in HBM: one block per resident wave, regions at offsets the
-// host computes from the session's caps (ScrLayout); each wave keeps the
-// addresses of its regions in LDS (g_cfgw) so the stage functions take no
-// pointer arguments for them.
+// host computes from the session's caps (ScrLayout); the workgroup keeps the
+// addresses of its first wave's regions in LDS (g_cfg), from which each wave
+// finds its own, so the stage functions take no pointer arguments for them.
 enum ScrRegion {
     R_TK,     // Tk[maxtkl]: the event's tracklets
     R_CELL,   // uint32[1569]: XZ cell offsets
@@ -323,23 +323,31 @@ struct Args {
     unsigned long long* dump_count;
 };
 
-// region addresses and caps of each wave (written at kernel start)
-struct WaveCfg {
-    char* rp[R_COUNT];
-    int32_t maxtkl, bin_cap, win_cap, list_cap, unit_cap;
-};
-__shared__ WaveCfg g_cfgw[WPE_MAX];
-
 // regions of the event, shared by its waves; the others are per wave
 __host__ __device__ constexpr bool scr_shared(int r) {
     return r == R_TK || r == R_CELL || r == R_XP || r == R_SO || r == R_CL;
 }
 
+// region addresses of the workgroup's first wave and the caps (written at
+// kernel start, one record per workgroup whatever its waves): wave w's own
+// regions lie w scratch blocks (stride bytes each) further
+struct WaveCfg {
+    char* rp[R_COUNT];
+    int64_t stride;
+    int32_t maxtkl, bin_cap, win_cap, list_cap, unit_cap;
+};
+__shared__ WaveCfg g_cfg;
+
 __device__ __forceinline__ unsigned long long ballot(bool p) { return __ballot(p); }
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (WAVE - 1); }
 // wave of the workgroup (wave-uniform, in a scalar register)
 __device__ __forceinline__ int wave_idx() { return __builtin_amdgcn_readfirstlane((int)threadIdx.x) >> 6; }
-__device__ __forceinline__ WaveCfg& cfg() { return g_cfgw[wave_idx()]; }
+__device__ __forceinline__ const WaveCfg& cfg() { return g_cfg; }
+// address of the wave's scratch region R
+template <int R>
+__device__ __forceinline__ char* region() {
+    return scr_shared(R) ? g_cfg.rp[R] : g_cfg.rp[R] + (int64_t)wave_idx() * g_cfg.stride;
+}
 // A value the whole wave holds alike, as the wave's scalar: arguments of the
 // non-inlined stage functions are taken as per-lane by the compiler, so their
 // uniform ones (event and chunk bounds, counts, cursors, shared pointers) are
@@ -1049,41 +1057,41 @@ __device__ __attribute__((noinline)) bool yz_unit(const Ev<MAXH, WPE>& E, Tk* Tp
 // each stage, and stage-level branches depend only on wave-uniform values.
 #define STAGE __device__ __attribute__((noinline))
 
-// Accessors of the wave's scratch regions (addresses from g_cfgw).
+// Accessors of the wave's scratch regions (addresses from g_cfg, region<R>).
 struct Scr {
-    __device__ Tk* tks() const { return reinterpret_cast<Tk*>(cfg().rp[R_TK]); }
-    __device__ uint32_t* celloff() const { return reinterpret_cast<uint32_t*>(cfg().rp[R_CELL]); }
-    __device__ uint16_t* wl() const { return reinterpret_cast<uint16_t*>(cfg().rp[R_LIST]); }
+    __device__ Tk* tks() const { return reinterpret_cast<Tk*>(region<R_TK>()); }
+    __device__ uint32_t* celloff() const { return reinterpret_cast<uint32_t*>(region<R_CELL>()); }
+    __device__ uint16_t* wl() const { return reinterpret_cast<uint16_t*>(region<R_LIST>()); }
     // per YZ list entry: w y of hit a and of hit b (w the plane's weight), its
     // flag byte (as wf holds it) in the bits of z, 0; the y values themselves in wyy
-    __device__ float4* wy() const { return reinterpret_cast<float4*>(cfg().rp[R_YV]); }
+    __device__ float4* wy() const { return reinterpret_cast<float4*>(region<R_YV>()); }
     // (station 2: x = the entry's y, the mean of its accepted hits)
-    __device__ float2* wyy() const { return reinterpret_cast<float2*>(cfg().rp[R_YY]); }
-    __device__ uint8_t* wf() const { return reinterpret_cast<uint8_t*>(cfg().rp[R_YF]); }
+    __device__ float2* wyy() const { return reinterpret_cast<float2*>(region<R_YY>()); }
+    __device__ uint8_t* wf() const { return reinterpret_cast<uint8_t*>(region<R_YF>()); }
     // [3][28][bin_cap] X pairs (a | b << 8), 0xFF = none
-    __device__ uint16_t* xpairs() const { return reinterpret_cast<uint16_t*>(cfg().rp[R_XP]); }
+    __device__ uint16_t* xpairs() const { return reinterpret_cast<uint16_t*>(region<R_XP>()); }
     // tracklet index by slot order
-    __device__ uint16_t* slot_order() const { return reinterpret_cast<uint16_t*>(cfg().rp[R_SO]); }
+    __device__ uint16_t* slot_order() const { return reinterpret_cast<uint16_t*>(region<R_SO>()); }
     // YZ candidates passing the first box cut, in reference order
-    __device__ uint32_t* plist() const { return reinterpret_cast<uint32_t*>(cfg().rp[R_PL]); }
+    __device__ uint32_t* plist() const { return reinterpret_cast<uint32_t*>(region<R_PL>()); }
     // per candidate: x first-of-unit (bit 0), passed the sign-free tests (bit 1),
     // station 3 is D3m (bit 2); y station-3 signs after its left/right (first
     // of unit), z sign written to the last station-3 hit (0: none), w station-3
     // signs it starts from
-    __device__ uchar4* pst() const { return reinterpret_cast<uchar4*>(cfg().rp[R_PB]); }
+    __device__ uchar4* pst() const { return reinterpret_cast<uchar4*>(region<R_PB>()); }
     // kept YZ units of a chunk (j << 16 | u3.v3 index), in order, and their first
     // candidate in the kept enumeration
-    __device__ uint32_t* klist() const { return reinterpret_cast<uint32_t*>(cfg().rp[R_KL]); }
-    __device__ uint32_t* kstart() const { return reinterpret_cast<uint32_t*>(cfg().rp[R_KS]); }
+    __device__ uint32_t* klist() const { return reinterpret_cast<uint32_t*>(region<R_KL>()); }
+    __device__ uint32_t* kstart() const { return reinterpret_cast<uint32_t*>(region<R_KS>()); }
     // per box-cut survivor of a batch: its hits (raw LDS indices), which slots
     // hold one, and its first fit, as pass 2 built them (3 x 16 bytes)
-    __device__ uint4* ycache() const { return reinterpret_cast<uint4*>(cfg().rp[R_YC]); }
+    __device__ uint4* ycache() const { return reinterpret_cast<uint4*>(region<R_YC>()); }
     // survivors of a batch that pass the sign-free tests, in order
-    __device__ uint32_t* qlist() const { return reinterpret_cast<uint32_t*>(cfg().rp[R_QL]); }
+    __device__ uint32_t* qlist() const { return reinterpret_cast<uint32_t*>(region<R_QL>()); }
     // per tracklet of the chunk: the cache entry and sign state of its best candidate so far
-    __device__ uint4* brec() const { return reinterpret_cast<uint4*>(cfg().rp[R_BR]); }
+    __device__ uint4* brec() const { return reinterpret_cast<uint4*>(region<R_BR>()); }
     // cleaning: per live tracklet, the raw hit of each of detectors 13-30 (uint16, 0xFFFF none)
-    __device__ uint4* clean_hits() const { return reinterpret_cast<uint4*>(cfg().rp[R_CL]); }
+    __device__ uint4* clean_hits() const { return reinterpret_cast<uint4*>(region<R_CL>()); }
     __device__ int win() const { return cfg().win_cap; }
 };
 
@@ -4584,7 +4592,7 @@ __device__ unsigned long long g_stage_cycles[NCLASS_MAX][N_STAGE_TIMERS];
 template <int MAXH, int WPE>
 constexpr int chain_waves_per_simd() {
     if (MAXH <= 512) return OR_WAVES_PER_SIMD_512;
-    constexpr int lds = (int)sizeof(Lds<MAXH>) + (int)sizeof(WaveCfg) * WPE_MAX;
+    constexpr int lds = (int)sizeof(Lds<MAXH>) + (int)sizeof(WaveCfg);
     constexpr int w = (160 * 1024 / lds) * WPE / 4;
     return w < 1 ? 1 : (w > 4 ? 4 : w);
 }
@@ -4599,16 +4607,16 @@ __global__ __launch_bounds__(WAVE * WPE, (chain_waves_per_simd<MAXH, WPE>())) vo
     const int lane = lane_id();
     const int w = WPE == 1 ? 0 : wave_idx();
     {
-        // the event's regions from the first wave's block, the others from the wave's own
+        // the regions of the first wave's block; each wave's own lie its index blocks further (region<R>)
         const int64_t blk = (int64_t)blockIdx.x * WPE;
-        if (lane < R_COUNT)
-            g_cfgw[w].rp[lane] = a.scratch + (blk + (scr_shared(lane) ? 0 : w)) * a.lay.stride + a.lay.off[lane];
-        if (lane == 0) {
-            g_cfgw[w].maxtkl = a.lay.maxtkl;
-            g_cfgw[w].bin_cap = a.lay.bin_cap;
-            g_cfgw[w].win_cap = a.lay.win_cap;
-            g_cfgw[w].list_cap = a.lay.list_cap;
-            g_cfgw[w].unit_cap = a.lay.unit_cap;
+        if (w == 0 && lane < R_COUNT) g_cfg.rp[lane] = a.scratch + blk * a.lay.stride + a.lay.off[lane];
+        if (w == 0 && lane == 0) {
+            g_cfg.stride = a.lay.stride;
+            g_cfg.maxtkl = a.lay.maxtkl;
+            g_cfg.bin_cap = a.lay.bin_cap;
+            g_cfg.win_cap = a.lay.win_cap;
+            g_cfg.list_cap = a.lay.list_cap;
+            g_cfg.unit_cap = a.lay.unit_cap;
         }
     }
     __syncthreads();
